@@ -1,0 +1,133 @@
+/* kelp_hip.h -- C ABI of the MI355X particle time-stepping library.
+ *
+ * The reference (CliDyn/kelp) is Python + Numba; its step boundary is the
+ * integrator registry (src/kelp/numerics/integrators.py:6635-6660) called
+ * once per step by run_single_realization (src/kelp/tracking.py:2643-2655)
+ * as step_fn(state, vel, mesh, dt, time_dir, ...).  A ctypes binding of the
+ * entry points below replaces that call: the mesh and snapshots live in HBM
+ * (uploaded once), the particle state lives in caller-owned device buffers,
+ * and kelp_step advances every particle by one Euler/RK2/RK4 step in place.
+ *
+ * Conventions: every function returns KELP_OK (0) or a negative error code;
+ * kelp_last_error() describes the last failure of the calling thread.  All
+ * pointers named host_* are host memory; pointers in kelp_particles are
+ * device memory.  A context owns one HIP stream (default priority) and is
+ * not re-entrant: calls on one context must be serialised by the caller.
+ */
+#ifndef KELP_HIP_H
+#define KELP_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KELP_OK 0
+#define KELP_EINVAL (-1)  /* invalid argument / shape */
+#define KELP_EHIP (-2)    /* HIP runtime error */
+#define KELP_ENOMEM (-3)  /* device allocation failed */
+#define KELP_ESTATE (-4)  /* call out of order (no mesh, empty slot, ...) */
+
+#define KELP_DT_F64 0
+#define KELP_DT_F32 1
+
+typedef struct kelp_ctx kelp_ctx;
+
+/* Mesh contract of the reference's Mesh dataclass (src/kelp/core/mesh.py:41)
+ * plus its grid hash (build_grid_hash, core/mesh.py:844). */
+typedef struct {
+    int64_t n_nodes, n_elems, max_v, max_conn;
+    const double *host_nodes_lon, *host_nodes_lat; /* (N,) degrees */
+    const double *host_depth;                      /* (N,) m, positive down; may be NULL (2-D) */
+    const int32_t *host_elem_nodes;                /* (E, max_v), -1 padded; max_v 3 or 4 */
+    const int32_t *host_node_to_elems;             /* (N, max_conn), -1 padded */
+    const int32_t *host_node_to_elems_count;       /* (N,) */
+    const int32_t *host_gh_offsets;                /* (nx*ny + 1,) CSR over cells */
+    const int32_t *host_gh_node_ids;               /* (N,) */
+    double gh_cell, gh_lon_min, gh_lat_min;
+    int64_t gh_nx, gh_ny, gh_wrap, gh_far_rows;
+    double r_earth;
+    int64_t n_levels;                              /* K interfaces (0 for 2-D) */
+    const double *host_z_levels;                   /* (K,) m, positive down */
+} kelp_mesh_desc;
+
+/* Particle state, device memory, structure of arrays (the fields of the
+ * reference's ParticleState, core/particles.py:153).  Updated in place. */
+typedef struct {
+    int64_t n;
+    double *lon, *lat, *z;
+    int32_t *status;   /* ParticleStatus: 1 alive, 2 beached, 3 outside, 4 not released */
+    int32_t *nn;       /* nearest node */
+    int8_t *fr;        /* FailureReason (sticky; written on the step a particle dies) */
+    double *u, *v, *w; /* diagnostic velocity at the step start (w may be NULL in 2-D) */
+    const int64_t *pid; /* global particle id (RNG key); may be NULL without diffusion */
+} kelp_particles;
+
+/* One step (the arguments of step_fn at tracking.py:2643-2655). */
+typedef struct {
+    int32_t order;                 /* 1 Euler, 2 RK2 (midpoint), 4 RK4 */
+    int32_t slot_a, slot_b, slot_t; /* bracket pair (a, b) and stage target t; b/t may be -1 */
+    double alpha0;                 /* bracket fraction of a->b at the step start */
+    double alphas[4];              /* per-stage fraction towards t (c_k*dt/dt_to_next) */
+    double dt;                     /* positive seconds */
+    double time_dir;               /* +1 forward, -1 backward */
+    int32_t exact_nearest;         /* mesh.exact_nearest (default config: 1) */
+    int32_t reserved;
+} kelp_step_args;
+
+const char *kelp_last_error(void);
+int kelp_version(void);
+
+int kelp_ctx_create(int device, kelp_ctx **out);
+int kelp_ctx_destroy(kelp_ctx *ctx);
+int kelp_synchronize(kelp_ctx *ctx);
+/* The context's HIP stream (hipStream_t) as an opaque pointer. */
+void *kelp_ctx_stream(kelp_ctx *ctx);
+
+/* Upload the mesh (replaces any earlier mesh).  Builds the device layout
+ * and the node unit-vector table (core/mesh.py:713 _node_xyz_fill). */
+int kelp_mesh_upload(kelp_ctx *ctx, const kelp_mesh_desc *desc);
+/* Device copy of the node unit-vector table, node order, (N, 3) doubles. */
+int kelp_mesh_node_xyz(kelp_ctx *ctx, double *host_out);
+
+/* Field snapshot slot (the reference's VelocityField, core/io.py:180).
+ * u, v: n_planes x N node values (2-D: n_planes = 1; 3-D: K-1 layers);
+ * w: K x N interface values positive down or NULL.  dtype KELP_DT_*. */
+int kelp_field_upload(kelp_ctx *ctx, int slot, int dtype, int64_t n_planes,
+                      const void *host_u, const void *host_v,
+                      const void *host_w, int64_t n_w_planes);
+int kelp_field_clear(kelp_ctx *ctx, int slot);
+
+/* Advance every particle by one step, in place, on the context stream. */
+int kelp_step(kelp_ctx *ctx, const kelp_particles *p, const kelp_step_args *a);
+
+/* Batch queries (core/mesh.py:1896 find_nearest_nodes and :2289
+ * find_containing_elements) on device arrays. */
+int kelp_nearest(kelp_ctx *ctx, int64_t n, const double *lon, const double *lat,
+                 int exact, int32_t *out);
+int kelp_containing(kelp_ctx *ctx, int64_t n, const double *lon, const double *lat,
+                    const int32_t *nn, int32_t *out);
+
+/* Device memory helpers for callers without their own allocator. */
+int kelp_malloc(kelp_ctx *ctx, int64_t bytes, void **out);
+int kelp_free(kelp_ctx *ctx, void *ptr);
+int kelp_memcpy_h2d(kelp_ctx *ctx, void *dst, const void *src, int64_t bytes);
+int kelp_memcpy_d2h(kelp_ctx *ctx, void *dst, const void *src, int64_t bytes);
+
+/* Timing on the context stream: record event `slot` (0..15) now; elapsed
+ * milliseconds between two recorded slots (synchronises on `b`). */
+int kelp_event_record(kelp_ctx *ctx, int slot);
+int kelp_event_elapsed(kelp_ctx *ctx, int a, int b, float *ms);
+
+/* Diagnostics: evaluate the device math routines elementwise on device
+ * arrays (fn: 0 sin, 1 cos, 2 asin, 3 acos, 4 atan2(x, y), 5 log, 6 exp,
+ * 7 Python x % 360, 8 sqrt).  No reference counterpart; used to check the
+ * device compiler against a host build of the same routines. */
+int kelp_math_probe(kelp_ctx *ctx, int fn, int64_t n, const double *x, const double *y,
+                    double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

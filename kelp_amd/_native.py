@@ -1,0 +1,114 @@
+"""ctypes binding of libkelp_hip.so (the C ABI declared in include/kelp_hip.h).
+
+The library is built in-tree by :func:`kelp_amd.build.build`.  Loading fails
+loudly when it is missing: there is no CPU fallback for the step.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_double, c_float, c_int, c_int32, c_int64, c_void_p
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libkelp_hip.so")
+
+KELP_OK = 0
+KELP_DT_F64 = 0
+KELP_DT_F32 = 1
+
+
+class KelpError(RuntimeError):
+    """A C-ABI call returned an error code."""
+
+
+class MeshDesc(ctypes.Structure):
+    _fields_ = [
+        ("n_nodes", c_int64), ("n_elems", c_int64), ("max_v", c_int64), ("max_conn", c_int64),
+        ("host_nodes_lon", c_void_p), ("host_nodes_lat", c_void_p), ("host_depth", c_void_p),
+        ("host_elem_nodes", c_void_p), ("host_node_to_elems", c_void_p),
+        ("host_node_to_elems_count", c_void_p), ("host_gh_offsets", c_void_p),
+        ("host_gh_node_ids", c_void_p),
+        ("gh_cell", c_double), ("gh_lon_min", c_double), ("gh_lat_min", c_double),
+        ("gh_nx", c_int64), ("gh_ny", c_int64), ("gh_wrap", c_int64), ("gh_far_rows", c_int64),
+        ("r_earth", c_double),
+        ("n_levels", c_int64), ("host_z_levels", c_void_p),
+    ]
+
+
+class Particles(ctypes.Structure):
+    _fields_ = [
+        ("n", c_int64),
+        ("lon", c_void_p), ("lat", c_void_p), ("z", c_void_p),
+        ("status", c_void_p), ("nn", c_void_p), ("fr", c_void_p),
+        ("u", c_void_p), ("v", c_void_p), ("w", c_void_p),
+        ("pid", c_void_p),
+    ]
+
+
+class StepArgs(ctypes.Structure):
+    _fields_ = [
+        ("order", c_int32), ("slot_a", c_int32), ("slot_b", c_int32), ("slot_t", c_int32),
+        ("alpha0", c_double), ("alphas", c_double * 4),
+        ("dt", c_double), ("time_dir", c_double),
+        ("exact_nearest", c_int32), ("reserved", c_int32),
+    ]
+
+
+#: every symbol include/kelp_hip.h declares, with its (restype, argtypes)
+SIGNATURES = {
+    "kelp_last_error": (ctypes.c_char_p, []),
+    "kelp_version": (c_int, []),
+    "kelp_ctx_create": (c_int, [c_int, POINTER(c_void_p)]),
+    "kelp_ctx_destroy": (c_int, [c_void_p]),
+    "kelp_synchronize": (c_int, [c_void_p]),
+    "kelp_ctx_stream": (c_void_p, [c_void_p]),
+    "kelp_mesh_upload": (c_int, [c_void_p, POINTER(MeshDesc)]),
+    "kelp_mesh_node_xyz": (c_int, [c_void_p, c_void_p]),
+    "kelp_field_upload": (c_int, [c_void_p, c_int, c_int, c_int64, c_void_p, c_void_p,
+                                  c_void_p, c_int64]),
+    "kelp_field_clear": (c_int, [c_void_p, c_int]),
+    "kelp_step": (c_int, [c_void_p, POINTER(Particles), POINTER(StepArgs)]),
+    "kelp_nearest": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_int, c_void_p]),
+    "kelp_containing": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "kelp_malloc": (c_int, [c_void_p, c_int64, POINTER(c_void_p)]),
+    "kelp_free": (c_int, [c_void_p, c_void_p]),
+    "kelp_memcpy_h2d": (c_int, [c_void_p, c_void_p, c_void_p, c_int64]),
+    "kelp_memcpy_d2h": (c_int, [c_void_p, c_void_p, c_void_p, c_int64]),
+    "kelp_event_record": (c_int, [c_void_p, c_int]),
+    "kelp_event_elapsed": (c_int, [c_void_p, c_int, c_int, POINTER(c_float)]),
+    "kelp_math_probe": (c_int, [c_void_p, c_int, c_int64, c_void_p, c_void_p, c_void_p]),
+}
+
+_LIB = None
+
+
+def load(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load the HIP library (once).  Raises if it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise ImportError(
+            f"{path} is missing: build the HIP library first "
+            "(python -c 'import __graft_entry__ as g; g.build()' or python -m kelp_amd.build)")
+    # One HIP runtime per process: PyTorch-ROCm ships its own libamdhip64 /
+    # libhsa-runtime64 (sonames libamdhip64.so.7 / libhsa-runtime64.so.1).
+    # Loading torch first lets this library's NEEDED entries resolve to
+    # those already-loaded copies instead of a second runtime from
+    # /opt/rocm, which would fail to see the device once torch holds it.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = lib
+    return lib
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc != KELP_OK:
+        msg = load().kelp_last_error().decode(errors="replace")
+        raise KelpError(f"{what or 'kelp call'} failed ({rc}): {msg}")

@@ -1,0 +1,743 @@
+// kelp_device.cuh -- device-side building blocks of the particle step.
+//
+// One HIP thread owns one particle for a whole step (a wavefront = 64
+// particles).  Everything here is __device__ code over the HBM-resident
+// mesh layout built by the library at upload time:
+//
+//   lonlat[N]      double2 (lon, lat) per node: one 16-byte load per vertex
+//   gh_xyz[N]      double4 (x, y, z, node id bits) in grid-hash CSR order, so
+//                  a row segment of the hash is one contiguous stream
+//   gh_offsets     int32 CSR offsets over cells (cj * nx + ci)
+//   elems[E]       int4 vertex ids (-1 padded): one 16-byte load per element
+//   n2e_off/n2e    node -> element adjacency in CSR form (same order as the
+//                  reference's dense -1-padded table)
+//
+// The floating-point expressions follow the reference's Numba kernels
+// operation for operation (same operand order, no contraction, Python `%`
+// via km_pymod360, transcendental functions from kmath.h) so the results
+// equal the CPU restatement built with the same math bit for bit.  The
+// reference file:line each function follows is cited beside it.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kmath.h"
+
+namespace kelp {
+
+constexpr double kDeg2Rad = 0.017453292519943295;  // np.pi/180, core/geometry.py:31
+constexpr double kRad2Deg = 57.29577951308232;     // 180/np.pi, core/geometry.py:32
+constexpr double kSLM = 111120.0;                  // METERS_PER_DEG_LAT, core/geometry.py:28
+constexpr double kREarthStd = 6371000.0;           // R_EARTH_STANDARD, core/geometry.py:30
+constexpr double kBaryEps = -1e-6;                 // _BARY_EPS, core/mesh.py:33
+constexpr double kAreaTol = 1e-10;                 // _AREA_TOL, core/mesh.py:32
+constexpr int kShortRowCells = 32;                 // _GH_EXACT_SHORT_ROW_CELLS, core/mesh.py:1608
+
+constexpr int32_t kAlive = 1;
+constexpr int32_t kBeached = 2;
+constexpr int32_t kOutside = 3;
+constexpr int8_t kFrInterp = 1;
+constexpr int8_t kFrOOB = 2;
+
+struct DMesh {
+    const double2* __restrict__ lonlat;
+    const double4* __restrict__ gh_xyz;
+    const int32_t* __restrict__ gh_offsets;
+    const int4* __restrict__ elems;
+    const int32_t* __restrict__ n2e_off;
+    const int32_t* __restrict__ n2e;
+    const double* __restrict__ depth;     // node bottom depth (m, positive down)
+    const double* __restrict__ zlev;      // K interface depths (3-D)
+    double cell, lon_min, lat_min, r_earth;
+    int32_t nx, ny, wrap, far_rows;
+    int32_t n_nodes, n_elems, max_v, n_levels;
+};
+
+__device__ __forceinline__ double sqd(double x) { return x * x; }
+
+__device__ __forceinline__ int32_t node_id_of(double4 p)
+{
+    return (int32_t)__double_as_longlong(p.w);
+}
+
+// core/mesh.py:1193 _gh_fold_dlon
+__device__ inline double gh_fold_dlon(const DMesh& m, double lon_p)
+{
+    double dlon = km_pymod360(lon_p - m.lon_min);
+    if (!m.wrap) {
+        double grid_span = m.nx * m.cell;
+        if (dlon > grid_span && (dlon - grid_span) > (360.0 - dlon)) dlon -= 360.0;
+    }
+    return dlon;
+}
+
+// core/mesh.py:1211 _gh_cell_of
+__device__ inline void gh_cell_of(const DMesh& m, double lon_p, double lat_p, int& ci, int& cj)
+{
+    double dlon = gh_fold_dlon(m, lon_p);
+    ci = (int)(dlon / m.cell);
+    cj = (int)((lat_p - m.lat_min) / m.cell);
+    if (ci < 0) ci = 0;
+    if (ci >= m.nx) ci = m.nx - 1;
+    if (cj < 0) cj = 0;
+    if (cj >= m.ny) cj = m.ny - 1;
+}
+
+struct Query {
+    double x, y, z, cos_lat;
+};
+
+// core/mesh.py:1281-1286 (query unit vector)
+__device__ inline Query make_query(double lon_p, double lat_p)
+{
+    double lat_r = lat_p * kDeg2Rad;
+    double lon_r = lon_p * kDeg2Rad;
+    double sl, cl, so, co;
+    km_sincos(lat_r, &sl, &cl);
+    km_sincos(lon_r, &so, &co);
+    Query q;
+    q.cos_lat = cl;
+    q.x = cl * co;
+    q.y = cl * so;
+    q.z = sl;
+    return q;
+}
+
+// core/mesh.py:1570 _gh_scan_row_range: cells c_lo..c_hi of row cj2 are one
+// contiguous CSR slice; strict `<` keeps the first node found on a tie.
+__device__ inline void scan_range(const DMesh& m, int cj2, int c_lo, int c_hi, const Query& q,
+                                  int& best, double& best_d2)
+{
+    const int64_t row = (int64_t)cj2 * m.nx;
+    int32_t start = m.gh_offsets[row + c_lo];
+    int32_t end = m.gh_offsets[row + c_hi + 1];
+    for (int32_t k = start; k < end; ++k) {
+        double4 p = m.gh_xyz[k];
+        double d2 = sqd(p.x - q.x) + sqd(p.y - q.y) + sqd(p.z - q.z);
+        if (d2 < best_d2) {
+            best_d2 = d2;
+            best = node_id_of(p);
+        }
+    }
+}
+
+// core/mesh.py:1588 _gh_scan_row_window
+__device__ inline void scan_row_window(const DMesh& m, int cj2, int ci, int rx, const Query& q,
+                                       int& best, double& best_d2)
+{
+    const int nx = m.nx;
+    if (m.wrap && 2 * rx + 1 >= nx) {
+        scan_range(m, cj2, 0, nx - 1, q, best, best_d2);
+        return;
+    }
+    int c_lo = ci - rx, c_hi = ci + rx;
+    if (m.wrap) {
+        if (c_lo < 0) {
+            scan_range(m, cj2, c_lo + nx, nx - 1, q, best, best_d2);
+            c_lo = 0;
+        } else if (c_hi > nx - 1) {
+            scan_range(m, cj2, 0, c_hi - nx, q, best, best_d2);
+            c_hi = nx - 1;
+        }
+    } else {
+        if (c_lo < 0) c_lo = 0;
+        if (c_hi > nx - 1) c_hi = nx - 1;
+    }
+    scan_range(m, cj2, c_lo, c_hi, q, best, best_d2);
+}
+
+// core/mesh.py:1236 _grid_hash_nearest_3x3 (best_d2 returned for the exact
+// search, which re-derives the same value from the table)
+__device__ inline int nearest_3x3(const DMesh& m, int ci, int cj, const Query& q, double& best_d2)
+{
+    int best = -1;
+    best_d2 = 1e30;
+    const int nx = m.nx, ny = m.ny;
+    for (int dj = -1; dj < 2; ++dj) {
+        int cj2 = cj + dj;
+        if (cj2 < 0 || cj2 >= ny) continue;
+        // the three cells of a row are one CSR slice unless the window is
+        // clipped or wraps; scanning cell by cell keeps the reference order
+        // either way (cells ascend within the row)
+        int lo = ci - 1, hi = ci + 1;
+        if (lo >= 0 && hi < nx) {
+            scan_range(m, cj2, lo, hi, q, best, best_d2);
+            continue;
+        }
+        for (int di = -1; di < 2; ++di) {
+            int ci2 = ci + di;
+            if (ci2 < 0)
+                ci2 = m.wrap ? ci2 + nx : -1;
+            else if (ci2 >= nx)
+                ci2 = m.wrap ? ci2 - nx : -1;
+            if (ci2 < 0) continue;
+            scan_range(m, cj2, ci2, ci2, q, best, best_d2);
+        }
+    }
+    return best;
+}
+
+// core/mesh.py:1307 _cap_extents
+__device__ inline void cap_extents(double lon_p, double lat_p, double lon_n, double lat_n,
+                                   double& theta_deg, double& ext_lon)
+{
+    double lat_pr = lat_p * kDeg2Rad;
+    double lat_nr = lat_n * kDeg2Rad;
+    double sp, cp, sn, cn;
+    km_sincos(lat_pr, &sp, &cp);
+    km_sincos(lat_nr, &sn, &cn);
+    double cosd = (sp * sn + cp * cn * km_cos((lon_p - lon_n) * kDeg2Rad));
+    if (cosd > 1.0) cosd = 1.0;
+    if (cosd < -1.0) cosd = -1.0;
+    double theta = km_acos(cosd);
+    if (theta <= 0.0) {
+        theta_deg = 0.0;
+        ext_lon = 0.0;
+        return;
+    }
+    theta_deg = theta * kRad2Deg;
+    if (km_fabs(lat_p) + theta_deg >= 90.0) {
+        ext_lon = 1.0e30;
+        return;
+    }
+    double r = km_sin(theta) / cp;
+    if (r >= 1.0) {
+        ext_lon = 1.0e30;
+        return;
+    }
+    ext_lon = km_asin(r) * kRad2Deg;
+}
+
+__device__ __forceinline__ int pyimod(int a, int n)
+{
+    int r = a % n;
+    return r < 0 ? r + n : r;
+}
+
+// core/mesh.py:1476 _anchor_provable_rect
+__device__ inline int anchor_provable_rect(const DMesh& m, double lon_p, double lat_p, int node,
+                                           int rx, int ry)
+{
+    double2 ll = m.lonlat[node];
+    double theta_deg, ext_lon;
+    cap_extents(lon_p, lat_p, ll.x, ll.y, theta_deg, ext_lon);
+    if (theta_deg <= 0.0) return 0;
+    const double cell = m.cell, lon_min = m.lon_min, lat_min = m.lat_min;
+    const int nx = m.nx, ny = m.ny;
+    double dlon = gh_fold_dlon(m, lon_p);
+    double lon_q = lon_min + dlon;
+    int ci_raw = (int)floor(dlon / cell);
+    int cj_raw = (int)floor((lat_p - lat_min) / cell);
+    const double INF = 1.0e30;
+    int cj = cj_raw;
+    if (cj < 0) cj = 0;
+    if (cj > ny - 1) cj = ny - 1;
+    double cov_dn = (cj - ry >= 0) ? lat_p - (lat_min + (cj - ry) * cell) : INF;
+    double cov_up = (cj + ry <= ny - 1) ? (lat_min + (cj + ry + 1) * cell) - lat_p : INF;
+    int code = 0;
+    if (theta_deg > cov_dn || theta_deg > cov_up) code |= 1;
+    if (m.wrap) {
+        if (2 * rx + 1 >= nx) return code;
+        if (ext_lon >= INF) return code | 4;
+        double w_last = 360.0 - (nx - 1) * cell;
+        int ci = pyimod(ci_raw, nx);
+        double left = lon_min + (ci - rx) * cell;
+        if (ci - rx < 0) left += cell - w_last;
+        double right = lon_min + (ci + rx + 1) * cell;
+        if (ci + rx >= nx - 1) right -= cell - w_last;
+        if (ext_lon > lon_q - left || ext_lon > right - lon_q) code |= 2;
+        return code;
+    }
+    int ci = ci_raw;
+    if (ci < 0) ci = 0;
+    if (ci > nx - 1) ci = nx - 1;
+    double cov_l = (ci - rx >= 0) ? lon_q - (lon_min + (ci - rx) * cell) : INF;
+    double cov_r = (ci + rx <= nx - 1) ? (lon_min + (ci + rx + 1) * cell) - lon_q : INF;
+    if (cov_l >= INF && cov_r >= INF) return code;
+    if (ext_lon >= INF) return code | 4;
+    if (ext_lon > cov_l || ext_lon > cov_r) code |= 2;
+    double n_lo = lon_min + cell;
+    double n_hi = lon_min + (nx - 1) * cell;
+    for (int s = 0; s < 2; ++s) {
+        double shift = s == 0 ? -360.0 : 360.0;
+        double a = lon_q - ext_lon + shift;
+        double b = lon_q + ext_lon + shift;
+        if (b >= n_lo && a <= n_hi) return code | 4;
+    }
+    return code;
+}
+
+// core/mesh.py:1365 _grid_hash_nearest_widened (ring widening after an
+// empty 3x3 window)
+__device__ inline int widen_rings(const DMesh& m, int ci, int cj, const Query& q)
+{
+    int best = -1;
+    double best_d2 = 1e30;
+    const int nx = m.nx, ny = m.ny;
+    int max_ring = nx > ny ? nx : ny;
+    int found_ring = -1;
+    for (int r = 2; r <= max_ring; ++r) {
+        for (int dj = -r; dj < r + 1; ++dj) {
+            int cj2 = cj + dj;
+            if (cj2 < 0 || cj2 >= ny) continue;
+            int di_step = (dj == -r || dj == r) ? 1 : 2 * r;
+            for (int di = -r; di <= r; di += di_step) {
+                int ci2 = ci + di;
+                if (m.wrap)
+                    ci2 = pyimod(ci2, nx);
+                else if (ci2 < 0 || ci2 >= nx)
+                    continue;
+                scan_range(m, cj2, ci2, ci2, q, best, best_d2);
+            }
+        }
+        if (best >= 0) {
+            if (found_ring < 0)
+                found_ring = r;
+            else if (r > found_ring)
+                break;
+        }
+    }
+    return best;
+}
+
+__device__ inline int nearest_widened(const DMesh& m, double lon_p, double lat_p)
+{
+    int ci, cj;
+    gh_cell_of(m, lon_p, lat_p, ci, cj);
+    Query q = make_query(lon_p, lat_p);
+    double d2;
+    int best = nearest_3x3(m, ci, cj, q, d2);
+    if (best >= 0) return best;
+    return widen_rings(m, ci, cj, q);
+}
+
+// core/mesh.py:1612 _grid_hash_nearest_exact: the 3x3 window, accepted when
+// the certificate proves it; else the rectangle grown by the certificate's
+// deficits.  The common case (3x3 proven) returns after one window scan.
+__device__ __noinline__ int exact_widen(const DMesh& m, double lon_p, double lat_p, int best,
+                                        double best_d2, int code, int ci, int cj, const Query& q);
+
+__device__ inline int nearest_exact(const DMesh& m, double lon_p, double lat_p)
+{
+    const double cell = m.cell;
+    const double far = m.far_rows * cell;
+    if (lat_p < m.lat_min - far || lat_p > m.lat_min + m.ny * cell + far)
+        return nearest_widened(m, lon_p, lat_p);
+    int ci, cj;
+    gh_cell_of(m, lon_p, lat_p, ci, cj);
+    Query q = make_query(lon_p, lat_p);
+    double best_d2;
+    int best = nearest_3x3(m, ci, cj, q, best_d2);
+    int code = 0;
+    if (best >= 0) {
+        code = anchor_provable_rect(m, lon_p, lat_p, best, 1, 1);
+        if (code == 0) return best;
+        double2 ll = m.lonlat[best];
+        double theta_deg, ext;
+        cap_extents(lon_p, lat_p, ll.x, ll.y, theta_deg, ext);
+        if (theta_deg > far) return best;
+    }
+    return exact_widen(m, lon_p, lat_p, best, best_d2, code, ci, cj, q);
+}
+
+__device__ __noinline__ int exact_widen(const DMesh& m, double lon_p, double lat_p, int best,
+                                        double best_d2, int code, int ci, int cj, const Query& q)
+{
+    const double cell = m.cell;
+    const int nx = m.nx, ny = m.ny, wrap = m.wrap;
+    const double far = m.far_rows * cell;
+    if (best < 0) best_d2 = 1.0e30;
+    int rx = 1, ry = 1;
+    int short_row = 360.0 * q.cos_lat <= kShortRowCells * cell;
+    int rx_max;
+    if (wrap) {
+        rx_max = nx / 2;
+        if (rx_max < 1) rx_max = 1;
+    } else {
+        rx_max = ci > nx - 1 - ci ? ci : nx - 1 - ci;
+    }
+    int ry_max = cj > ny - 1 - cj ? cj : ny - 1 - cj;
+    double abs_lat = lat_p >= 0.0 ? lat_p : -lat_p;
+    double dlon_q = gh_fold_dlon(m, lon_p);
+    const double INF = 1.0e30;
+    for (;;) {
+        if (best >= 0 && code == 0) {
+            code = anchor_provable_rect(m, lon_p, lat_p, best, rx, ry);
+            if (code == 0) return best;
+        }
+        if (best < 0 || 2.0 * km_asin(0.5 * km_sqrt(best_d2)) * kRad2Deg > far) {
+            int cols_full = wrap ? (2 * rx + 1 >= nx) : (ci - rx <= 0 && ci + rx >= nx - 1);
+            double clear = (ry >= ry_max) ? INF : ry * cell;
+            if (!cols_full) {
+                double lat_edge = abs_lat + (ry + 1) * cell;
+                double clear_lon;
+                if (lat_edge >= 90.0) {
+                    clear_lon = 0.0;
+                } else {
+                    double dl = rx * cell;
+                    if (!wrap) {
+                        double x = dlon_q > 0.0 ? dlon_q : 0.0;
+                        double w = nx * cell;
+                        if (x > w) x = w;
+                        double near_edge = x < w - x ? x : w - x;
+                        double dl_cut = 360.0 - w + near_edge;
+                        if (dl_cut < dl) dl = dl_cut;
+                    }
+                    if (dl > 180.0) dl = 180.0;
+                    clear_lon = km_asin(km_sin(dl * kDeg2Rad) * km_cos(lat_edge * kDeg2Rad)) * kRad2Deg;
+                }
+                if (clear_lon < clear) clear = clear_lon;
+            }
+            if (clear >= far) {
+                if (best >= 0) return best;
+                return nearest_widened(m, lon_p, lat_p);
+            }
+        }
+        int grow_x, grow_y, all_x;
+        if (best >= 0) {
+            grow_x = (code & 2) != 0;
+            grow_y = (code & 1) != 0;
+            all_x = (code & 4) != 0;
+            code = 0;
+        } else if (short_row) {
+            grow_x = 0;
+            grow_y = 1;
+            all_x = 1;
+        } else {
+            grow_x = 1;
+            grow_y = 1;
+            all_x = 0;
+        }
+        if (rx >= rx_max) {
+            grow_x = 0;
+            all_x = 0;
+        }
+        if (ry >= ry_max) grow_y = 0;
+        if (!(grow_x || grow_y || all_x)) {
+            if (best >= 0) return best;
+            return nearest_widened(m, lon_p, lat_p);
+        }
+        int rx_new = all_x ? rx_max : (grow_x ? rx + 1 : rx);
+        int ry_new = grow_y ? ry + 1 : ry;
+        int j_lo = cj - ry > 0 ? cj - ry : 0;
+        int j_hi = cj + ry < ny - 1 ? cj + ry : ny - 1;
+        if (rx_new > rx) {
+            if (wrap && 2 * rx_new + 1 >= nx && rx_new - rx > 1) {
+                for (int cj2 = j_lo; cj2 < j_hi + 1; ++cj2) {
+                    int c_lo = pyimod(ci + rx + 1, nx);
+                    int c_hi = pyimod(ci - rx - 1, nx);
+                    if (c_lo <= c_hi) {
+                        scan_range(m, cj2, c_lo, c_hi, q, best, best_d2);
+                    } else {
+                        scan_range(m, cj2, c_lo, nx - 1, q, best, best_d2);
+                        scan_range(m, cj2, 0, c_hi, q, best, best_d2);
+                    }
+                }
+            } else if (!wrap && rx_new - rx > 1) {
+                for (int cj2 = j_lo; cj2 < j_hi + 1; ++cj2) {
+                    if (ci - rx - 1 >= 0) {
+                        int c_lo = ci - rx_new > 0 ? ci - rx_new : 0;
+                        scan_range(m, cj2, c_lo, ci - rx - 1, q, best, best_d2);
+                    }
+                    if (ci + rx + 1 <= nx - 1) {
+                        int c_hi = ci + rx_new < nx - 1 ? ci + rx_new : nx - 1;
+                        scan_range(m, cj2, ci + rx + 1, c_hi, q, best, best_d2);
+                    }
+                }
+            } else {
+                for (int r = rx + 1; r < rx_new + 1; ++r) {
+                    if (wrap) {
+                        int c_left = pyimod(ci - r, nx);
+                        int two = (2 * r + 1 <= nx);
+                        int c_right = pyimod(ci + r, nx);
+                        for (int cj2 = j_lo; cj2 < j_hi + 1; ++cj2) {
+                            scan_range(m, cj2, c_left, c_left, q, best, best_d2);
+                            if (two) scan_range(m, cj2, c_right, c_right, q, best, best_d2);
+                        }
+                    } else {
+                        for (int s = 0; s < 2; ++s) {
+                            int c2 = s == 0 ? ci - r : ci + r;
+                            if (c2 < 0 || c2 > nx - 1) continue;
+                            for (int cj2 = j_lo; cj2 < j_hi + 1; ++cj2)
+                                scan_range(m, cj2, c2, c2, q, best, best_d2);
+                        }
+                    }
+                }
+            }
+        }
+        for (int r = ry + 1; r < ry_new + 1; ++r) {
+            for (int s = 0; s < 2; ++s) {
+                int cj2 = s == 0 ? cj - r : cj + r;
+                if (cj2 < 0 || cj2 > ny - 1) continue;
+                scan_row_window(m, cj2, ci, rx_new, q, best, best_d2);
+            }
+        }
+        rx = rx_new;
+        ry = ry_new;
+    }
+}
+
+// integrators.py:1471 _fused_site_anchor
+__device__ inline int site_anchor(const DMesh& m, double lon_p, double lat_p, bool exact)
+{
+    return exact ? nearest_exact(m, lon_p, lat_p) : nearest_widened(m, lon_p, lat_p);
+}
+
+// ---------------------------------------------------------------------------
+// Containment
+// ---------------------------------------------------------------------------
+
+// Vertex ring of element e with the FESOM-C repeated-first-vertex slot
+// dropped (core/mesh.py:2221-2238).
+struct Ring {
+    int n;
+    int v[4];
+};
+
+__device__ __forceinline__ int elem_slot(int4 e, int k)
+{
+    return k == 0 ? e.x : (k == 1 ? e.y : (k == 2 ? e.z : e.w));
+}
+
+// core/mesh.py:2055 _barycentric_weights_tri
+__device__ inline bool bary_tri(double px, double py, double ax, double ay, double bx, double by,
+                                double cx, double cy, double& w1, double& w2, double& w3)
+{
+    double denom = (by - cy) * (ax - cx) + (cx - bx) * (ay - cy);
+    if (km_fabs(denom) < kAreaTol) {
+        w1 = w2 = w3 = -1.0;
+        return false;
+    }
+    w1 = ((by - cy) * (px - cx) + (cx - bx) * (py - cy)) / denom;
+    w2 = ((cy - ay) * (px - cx) + (ax - cx) * (py - cy)) / denom;
+    w3 = 1.0 - w1 - w2;
+    return true;
+}
+
+__device__ __forceinline__ bool in_tri(double px, double py, double ax, double ay, double bx, double by,
+                                       double cx, double cy)
+{
+    double w1, w2, w3;
+    bary_tri(px, py, ax, ay, bx, by, cx, cy, w1, w2, w3);
+    return w1 >= kBaryEps && w2 >= kBaryEps && w3 >= kBaryEps;
+}
+
+// core/mesh.py:2211 _element_contains (bbox prefilter on wrapped deltas,
+// then :2140 _polygon_contains_bool with the :2007 projection)
+__device__ inline bool element_contains(const DMesh& m, int c, double lon_p, double lat_p)
+{
+    int4 en = m.elems[c];
+    int valid[4];
+    int n = 0;
+    int mv = m.max_v;
+    for (int j = 0; j < mv; ++j) {
+        int nd = elem_slot(en, j);
+        bool keep = nd >= 0;
+        if (j == 3 && mv == 4 && en.w == en.x && en.x >= 0) keep = false;
+        if (keep) valid[n++] = nd;
+    }
+    if (n < 3) return false;
+    double lo[4], la[4];
+    for (int k = 0; k < n; ++k) {
+        double2 ll = m.lonlat[valid[k]];
+        lo[k] = ll.x;
+        la[k] = ll.y;
+    }
+    double lon_ref = lo[0];
+    double dmax = 0.0, dmin = 0.0;
+    for (int k = 0; k < n; ++k) {
+        double d = km_pymod360(lo[k] - lon_ref + 180.0) - 180.0;
+        if (k == 0 || d > dmax) dmax = d;
+        if (k == 0 || d < dmin) dmin = d;
+    }
+    double dlon_p = km_pymod360(lon_p - lon_ref + 180.0) - 180.0;
+    if (dlon_p > dmax || dlon_p < dmin) return false;
+    double lamax = la[0], lamin = la[0];
+    for (int k = 1; k < n; ++k) {
+        if (la[k] > lamax) lamax = la[k];
+        if (la[k] < lamin) lamin = la[k];
+    }
+    if (lat_p > lamax || lat_p < lamin) return false;
+    // projection (core/mesh.py:2007)
+    double lmin = lo[0], lmax = lo[0];
+    for (int k = 0; k < n; ++k) {
+        if (lo[k] < lmin) lmin = lo[k];
+        if (lo[k] > lmax) lmax = lo[k];
+    }
+    double lon0;
+    if (lmax - lmin > 180.0) {
+        double dsum = 0.0;
+        for (int k = 0; k < n; ++k) dsum += km_pymod360(lo[k] - lo[0] + 180.0) - 180.0;
+        lon0 = lo[0] + dsum / n;
+    } else {
+        double s = 0.0;
+        for (int k = 0; k < n; ++k) s += lo[k];
+        lon0 = s / n;
+    }
+    double sl = 0.0;
+    for (int k = 0; k < n; ++k) sl += la[k];
+    double lat0 = sl / n;
+    double cos_lat0 = km_cos(lat0 * kDeg2Rad);
+    double xn[4], yn[4];
+    for (int k = 0; k < n; ++k) {
+        double dlon = km_pymod360(lo[k] - lon0 + 180.0) - 180.0;
+        double dlat = la[k] - lat0;
+        xn[k] = dlon * kDeg2Rad * kREarthStd * cos_lat0;
+        yn[k] = dlat * kDeg2Rad * kREarthStd;
+    }
+    double dlp = km_pymod360(lon_p - lon0 + 180.0) - 180.0;
+    double px = dlp * kDeg2Rad * kREarthStd * cos_lat0;
+    double py = (lat_p - lat0) * kDeg2Rad * kREarthStd;
+    if (n == 3) return in_tri(px, py, xn[0], yn[0], xn[1], yn[1], xn[2], yn[2]);
+    for (int i = 1; i < n - 1; ++i)
+        if (in_tri(px, py, xn[0], yn[0], xn[i], yn[i], xn[i + 1], yn[i + 1])) return true;
+    return false;
+}
+
+__device__ __forceinline__ bool elem_touches(const DMesh& m, int c, int nn)
+{
+    int4 e = m.elems[c];
+    int mv = m.max_v;
+    if (e.x == nn) return true;
+    if (e.y == nn) return true;
+    if (e.z == nn) return true;
+    return mv == 4 && e.w == nn;
+}
+
+// The rare 2-ring pass of core/mesh.py:2440-2463, kept out of line so the
+// common 1-ring hit stays a short straight path.
+__device__ __noinline__ int containing_2ring(const DMesh& m, double lon_p, double lat_p, int nn)
+{
+    int k0 = m.n2e_off[nn], k1 = m.n2e_off[nn + 1];
+    for (int k = k0; k < k1; ++k) {
+        int e = m.n2e[k];
+        int4 en = m.elems[e];
+        for (int j = 0; j < m.max_v; ++j) {
+            int v = elem_slot(en, j);
+            if (v < 0 || v == nn) continue;
+            int q0 = m.n2e_off[v], q1 = m.n2e_off[v + 1];
+            for (int kk = q0; kk < q1; ++kk) {
+                int c = m.n2e[kk];
+                if (elem_touches(m, c, nn)) continue;
+                if (element_contains(m, c, lon_p, lat_p)) return c;
+            }
+        }
+    }
+    return -1;
+}
+
+// core/mesh.py:2393 _find_containing_element_local
+__device__ inline int find_containing(const DMesh& m, double lon_p, double lat_p, int nn)
+{
+    int k0 = m.n2e_off[nn], k1 = m.n2e_off[nn + 1];
+    if (k1 == k0) return -1;
+    for (int k = k0; k < k1; ++k) {
+        int c = m.n2e[k];
+        if (element_contains(m, c, lon_p, lat_p)) return c;
+    }
+    return containing_2ring(m, lon_p, lat_p, nn);
+}
+
+// ---------------------------------------------------------------------------
+// Vertex weights (interpolate.py:1376-1404 + core/mesh.py:2077/2120)
+// ---------------------------------------------------------------------------
+
+struct Weights {
+    int n;
+    int v[4];
+    double w[4];
+};
+
+__device__ inline bool vertex_weights(const DMesh& m, int e, double xp, double yp, Weights& W)
+{
+    int4 en = m.elems[e];
+    int nv = 0;
+    for (int k = 0; k < m.max_v; ++k) {
+        int nd = elem_slot(en, k);
+        if (nd < 0) continue;
+        if (nv > 0 && nd == W.v[0]) continue;
+        W.v[nv++] = nd;
+    }
+    W.n = nv;
+    if (nv < 3) return false;
+    const double R = m.r_earth;
+    double lo[4], la[4];
+    for (int k = 0; k < nv; ++k) {
+        double2 ll = m.lonlat[W.v[k]];
+        lo[k] = ll.x;
+        la[k] = ll.y;
+    }
+    double lon_ref = lo[0];
+    double dsum = 0.0, lat0 = 0.0;
+    for (int k = 0; k < nv; ++k) {
+        dsum += km_pymod360(lo[k] - lon_ref + 180.0) - 180.0;
+        lat0 += la[k];
+    }
+    double lon0 = lon_ref + dsum / nv;
+    lat0 /= nv;
+    double cosl = km_cos(lat0 * kDeg2Rad);
+    double vx[4], vy[4];
+    for (int k = 0; k < nv; ++k) {
+        double dlon = km_pymod360(lo[k] - lon0 + 180.0) - 180.0;
+        vx[k] = dlon * kDeg2Rad * R * cosl;
+        vy[k] = (la[k] - lat0) * kDeg2Rad * R;
+    }
+    double dlon_p = km_pymod360(xp - lon0 + 180.0) - 180.0;
+    double px = dlon_p * kDeg2Rad * R * cosl;
+    double py = (yp - lat0) * kDeg2Rad * R;
+    if (nv == 3) {
+        bary_tri(px, py, vx[0], vy[0], vx[1], vy[1], vx[2], vy[2], W.w[0], W.w[1], W.w[2]);
+        return W.w[0] != -1.0;
+    }
+    // product-form Wachspress (core/mesh.py:2077)
+    double A[4], C[4];
+    for (int k = 0; k < 4; ++k) {
+        int i1 = (k + 1) & 3;
+        int im = (k + 3) & 3;
+        A[k] = ((vx[k] - px) * (vy[i1] - py) - (vx[i1] - px) * (vy[k] - py));
+        C[k] = ((vx[k] - vx[im]) * (vy[i1] - vy[im]) - (vx[i1] - vx[im]) * (vy[k] - vy[im]));
+    }
+    double s = 0.0;
+    for (int k = 0; k < 4; ++k) {
+        double w = C[k] * A[(k + 1) & 3] * A[(k + 2) & 3];
+        W.w[k] = w;
+        s += w;
+    }
+    for (int k = 0; k < 4; ++k) W.w[k] = W.w[k] / s;
+    return W.w[0] != -1.0;
+}
+
+// Field storage types: float (the configs' float32 snapshots) or double.
+template <typename T>
+__device__ __forceinline__ double ld(const T* p, int64_t i)
+{
+    return (double)p[i];
+}
+
+// One stage's field view: bracket pair (a, b) with alpha0 and target t.
+template <typename T>
+struct StageFields {
+    const T* a;
+    const T* b;
+    const T* t;
+};
+
+// interpolate.py:1419-1437 per-vertex nested time blend
+template <typename T>
+__device__ __forceinline__ double vblend(const StageFields<T>& f, int64_t idx, double alpha0, double alpha)
+{
+    double beta0 = 1.0 - alpha0, beta = 1.0 - alpha;
+    if (alpha < 1e-12) {
+        if (alpha0 == 0.0) return ld(f.a, idx);
+        return beta0 * ld(f.a, idx) + alpha0 * ld(f.b, idx);
+    } else if (alpha > 1.0 - 1e-12) {
+        return ld(f.t, idx);
+    } else if (alpha0 == 0.0) {
+        return beta * ld(f.a, idx) + alpha * ld(f.t, idx);
+    }
+    double a0 = beta0 * ld(f.a, idx) + alpha0 * ld(f.b, idx);
+    return beta * a0 + alpha * ld(f.t, idx);
+}
+
+}  // namespace kelp

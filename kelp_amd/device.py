@@ -1,0 +1,213 @@
+"""Device context: HBM-resident mesh, snapshot slots and particle buffers.
+
+A thin Python owner of one ``kelp_ctx`` (include/kelp_hip.h).  Particle state
+lives in device tensors allocated through PyTorch-ROCm (plumbing only: every
+operation on it is a HIP kernel of libkelp_hip.so); the mesh and the field
+snapshots are uploaded once and stay in HBM.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from typing import Dict, Optional
+
+import numpy as np
+
+from . import _native
+from ._native import KELP_DT_F32, KELP_DT_F64, MeshDesc, Particles, StepArgs, check
+from .mesh import Mesh
+
+
+def _ptr(a) -> Optional[int]:
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data
+    return a.data_ptr()  # torch tensor
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("kelp_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+    return torch
+
+
+@dataclass
+class DeviceParticles:
+    """Structure-of-arrays particle state in device memory."""
+    lon: object
+    lat: object
+    z: object
+    status: object
+    nn: object
+    fr: object
+    u: object
+    v: object
+    w: object
+    pid: object
+
+    @property
+    def n(self) -> int:
+        return int(self.lon.shape[0])
+
+    def as_struct(self) -> Particles:
+        return Particles(self.n, _ptr(self.lon), _ptr(self.lat), _ptr(self.z), _ptr(self.status),
+                         _ptr(self.nn), _ptr(self.fr), _ptr(self.u), _ptr(self.v), _ptr(self.w),
+                         _ptr(self.pid))
+
+    def to_host(self) -> Dict[str, np.ndarray]:
+        return {k: getattr(self, k).cpu().numpy() for k in
+                ("lon", "lat", "z", "status", "nn", "fr", "u", "v", "w", "pid")}
+
+
+class DeviceContext:
+    """One GPU context: mesh + field slots + stream."""
+
+    def __init__(self, device: int = 0):
+        self.lib = _native.load()
+        self.torch = _torch()
+        h = ctypes.c_void_p()
+        check(self.lib.kelp_ctx_create(int(device), ctypes.byref(h)), "kelp_ctx_create")
+        self.h = h
+        self.device = int(device)
+        self.mesh: Optional[Mesh] = None
+        self._slot_keys: Dict[int, object] = {}
+
+    def close(self) -> None:
+        if self.h:
+            self.lib.kelp_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- mesh ---------------------------------------------------------------
+    def upload_mesh(self, mesh: Mesh) -> None:
+        if mesh.gh_offsets is None:
+            raise ValueError("mesh has no grid hash (call kelp_amd.mesh.build_grid_hash)")
+        keep = dict(
+            lon=np.ascontiguousarray(mesh.nodes_lon, np.float64),
+            lat=np.ascontiguousarray(mesh.nodes_lat, np.float64),
+            depth=np.ascontiguousarray(mesh.depth, np.float64),
+            en=np.ascontiguousarray(mesh.elem_nodes, np.int32),
+            n2e=np.ascontiguousarray(mesh.node_to_elems, np.int32),
+            cnt=np.ascontiguousarray(mesh.node_to_elems_count, np.int32),
+            off=np.ascontiguousarray(mesh.gh_offsets, np.int32),
+            ids=np.ascontiguousarray(mesh.gh_node_ids, np.int32),
+            z=np.ascontiguousarray(mesh.sigma, np.float64),
+        )
+        d = MeshDesc(
+            mesh.n_nodes, mesh.n_elements, mesh.max_v, keep["n2e"].shape[1],
+            keep["lon"].ctypes.data, keep["lat"].ctypes.data, keep["depth"].ctypes.data,
+            keep["en"].ctypes.data, keep["n2e"].ctypes.data, keep["cnt"].ctypes.data,
+            keep["off"].ctypes.data, keep["ids"].ctypes.data,
+            mesh.gh_cell_size, mesh.gh_lon_min, mesh.gh_lat_min,
+            mesh.gh_nx, mesh.gh_ny, int(mesh.gh_wrap_lon), mesh.gh_far_rows,
+            mesh.r_earth, int(keep["z"].size), keep["z"].ctypes.data if keep["z"].size else None)
+        check(self.lib.kelp_mesh_upload(self.h, ctypes.byref(d)), "kelp_mesh_upload")
+        self.mesh = mesh
+        self._slot_keys.clear()
+
+    def node_xyz(self) -> np.ndarray:
+        out = np.empty((self.mesh.n_nodes, 3), dtype=np.float64)
+        check(self.lib.kelp_mesh_node_xyz(self.h, out.ctypes.data), "kelp_mesh_node_xyz")
+        return out
+
+    # -- fields -------------------------------------------------------------
+    def upload_field(self, slot: int, u, v, w=None, key=None) -> None:
+        """Upload one snapshot into ``slot``; skipped when ``key`` matches the
+        slot's current content."""
+        if key is not None and self._slot_keys.get(slot) is key:
+            return
+        u = np.asarray(u)
+        dt = KELP_DT_F32 if u.dtype == np.float32 else KELP_DT_F64
+        npd = np.float32 if dt == KELP_DT_F32 else np.float64
+        u = np.ascontiguousarray(u, npd)
+        v = np.ascontiguousarray(v, npd)
+        n_planes = 1 if u.ndim == 1 else u.shape[0]
+        wp, nw = None, 0
+        if w is not None:
+            w = np.ascontiguousarray(w, npd)
+            wp, nw = w.ctypes.data, (1 if w.ndim == 1 else w.shape[0])
+        check(self.lib.kelp_field_upload(self.h, int(slot), dt, n_planes, u.ctypes.data,
+                                         v.ctypes.data, wp, nw), "kelp_field_upload")
+        self._slot_keys[slot] = key
+
+    # -- particles ----------------------------------------------------------
+    def particles(self, lon, lat, z=None, status=None, nn=None, fr=None, pid=None) -> DeviceParticles:
+        t = self.torch
+        dev = f"cuda:{self.device}"
+        P = len(lon)
+
+        def f64(a, fill=0.0):
+            if a is None:
+                return t.full((P,), fill, dtype=t.float64, device=dev)
+            return t.as_tensor(np.ascontiguousarray(a, np.float64), device=dev)
+
+        def i32(a, fill):
+            if a is None:
+                return t.full((P,), fill, dtype=t.int32, device=dev)
+            return t.as_tensor(np.ascontiguousarray(a, np.int32), device=dev)
+
+        fr_t = (t.zeros((P,), dtype=t.int8, device=dev) if fr is None
+                else t.as_tensor(np.ascontiguousarray(fr, np.int8), device=dev))
+        pid_t = (t.arange(P, dtype=t.int64, device=dev) if pid is None
+                 else t.as_tensor(np.ascontiguousarray(pid, np.int64), device=dev))
+        return DeviceParticles(lon=f64(lon), lat=f64(lat), z=f64(z), status=i32(status, 1),
+                               nn=i32(nn, -1), fr=fr_t, u=f64(None), v=f64(None), w=f64(None),
+                               pid=pid_t)
+
+    # -- operations ---------------------------------------------------------
+    def step(self, p: DeviceParticles, order: int, slot_a: int, dt: float, time_dir: float,
+             slot_b: int = -1, slot_t: int = -1, alpha0: float = 0.0, alphas=(0.0, 0.0, 0.0, 0.0),
+             exact_nearest: bool = True) -> None:
+        a = StepArgs()
+        a.order = int(order)
+        a.slot_a, a.slot_b, a.slot_t = int(slot_a), int(slot_b), int(slot_t)
+        a.alpha0 = float(alpha0)
+        for k in range(4):
+            a.alphas[k] = float(alphas[k]) if k < len(alphas) else 0.0
+        a.dt = float(dt)
+        a.time_dir = float(time_dir)
+        a.exact_nearest = 1 if exact_nearest else 0
+        s = p.as_struct()
+        check(self.lib.kelp_step(self.h, ctypes.byref(s), ctypes.byref(a)), "kelp_step")
+
+    def nearest(self, lon, lat, exact: bool = True) -> np.ndarray:
+        t = self.torch
+        dev = f"cuda:{self.device}"
+        lo = t.as_tensor(np.ascontiguousarray(lon, np.float64), device=dev)
+        la = t.as_tensor(np.ascontiguousarray(lat, np.float64), device=dev)
+        out = t.empty(lo.shape, dtype=t.int32, device=dev)
+        check(self.lib.kelp_nearest(self.h, lo.numel(), _ptr(lo), _ptr(la), int(exact), _ptr(out)),
+              "kelp_nearest")
+        self.synchronize()
+        return out.cpu().numpy()
+
+    def containing(self, lon, lat, nn) -> np.ndarray:
+        t = self.torch
+        dev = f"cuda:{self.device}"
+        lo = t.as_tensor(np.ascontiguousarray(lon, np.float64), device=dev)
+        la = t.as_tensor(np.ascontiguousarray(lat, np.float64), device=dev)
+        n = t.as_tensor(np.ascontiguousarray(nn, np.int32), device=dev)
+        out = t.empty(lo.shape, dtype=t.int32, device=dev)
+        check(self.lib.kelp_containing(self.h, lo.numel(), _ptr(lo), _ptr(la), _ptr(n), _ptr(out)),
+              "kelp_containing")
+        self.synchronize()
+        return out.cpu().numpy()
+
+    def synchronize(self) -> None:
+        check(self.lib.kelp_synchronize(self.h), "kelp_synchronize")
+
+    def event_record(self, slot: int) -> None:
+        check(self.lib.kelp_event_record(self.h, int(slot)), "kelp_event_record")
+
+    def event_elapsed_ms(self, a: int, b: int) -> float:
+        ms = ctypes.c_float()
+        check(self.lib.kelp_event_elapsed(self.h, int(a), int(b), ctypes.byref(ms)),
+              "kelp_event_elapsed")
+        return float(ms.value)

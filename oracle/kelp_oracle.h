@@ -1,0 +1,51 @@
+/* kelp_oracle.h -- CPU oracle for the KELP particle time-stepping path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  A plain-C, scalar restatement of the
+ * reference's Numba kernels, used by tests/, the smoke check and the
+ * benchmark's CPU-baseline leg as the checker.  The product (kelp_amd)
+ * never links, loads or calls it.
+ *
+ * Two builds of one source:
+ *   liboracle_glibc.so  transcendental functions from the host glibc libm,
+ *                       as the reference's Numba JIT lowers them;
+ *   liboracle_kmath.so  the same code with kelp_amd/csrc/kmath.h in place
+ *                       of libm: bit-for-bit the arithmetic of the device
+ *                       kernels, so device results can be compared exactly.
+ */
+#ifndef KELP_ORACLE_H
+#define KELP_ORACLE_H
+
+#include <stdint.h>
+
+/* field storage dtypes */
+#define OR_F64 0
+#define OR_F32 1
+
+typedef struct {
+    int32_t n_nodes, n_elems, max_v, max_conn;
+    const double *nodes_lon, *nodes_lat, *depth;
+    const int32_t *elem_nodes;   /* n_elems * max_v, -1 padded */
+    const int32_t *n2e;          /* n_nodes * max_conn, -1 padded */
+    const int32_t *n2e_count;    /* n_nodes */
+    const double *node_xyz;      /* n_nodes * 3 */
+    const int32_t *gh_offsets;   /* nx*ny + 1 */
+    const int32_t *gh_node_ids;  /* n_nodes */
+    double gh_cell, gh_lon_min, gh_lat_min;
+    int32_t gh_nx, gh_ny, gh_wrap, gh_far_rows;
+    double r_earth;
+    /* vertical (3-D): interfaces z_levels[K], positive down */
+    int32_t n_levels;
+    const double *z_levels;
+} or_mesh_t;
+
+/* Snapshot triple of one step: bracket pair (a, b) blended by alpha0 and
+ * the stage target t; each array holds n_layers planes of n_nodes values
+ * (2-D fields: one plane). */
+typedef struct {
+    const void *u_a, *v_a, *u_b, *v_b, *u_t, *v_t;
+    const void *w_a, *w_b, *w_t;   /* 3-D: (K, N) interface W, or NULL */
+    int32_t dtype;
+    double alpha0;
+} or_fields_t;
+
+#endif
