@@ -50,6 +50,7 @@ typedef struct {
     double r_earth;
     int64_t n_levels;                              /* K interfaces (0 for 2-D) */
     const double *host_z_levels;                   /* (K,) m, positive down */
+    const uint8_t *host_elem_coastal;              /* (E,) coastal flags (core/mesh.py:2765) or NULL */
 } kelp_mesh_desc;
 
 /* Particle state, device memory, structure of arrays (the fields of the
@@ -61,10 +62,13 @@ typedef struct {
     int32_t *nn;       /* nearest node */
     int8_t *fr;        /* FailureReason (sticky; written on the step a particle dies) */
     double *u, *v, *w; /* diagnostic velocity at the step start (w may be NULL in 2-D) */
-    const int64_t *pid; /* global particle id (RNG key); may be NULL without diffusion */
+    const int64_t *pid; /* global particle id = RNG key; needed with the kick or beaching */
+    int32_t *wet_streak; /* beaching wet streak (tracking loop state); needed with beaching */
 } kelp_particles;
 
-/* One step (the arguments of step_fn at tracking.py:2643-2655). */
+/* One step (the arguments of step_fn at tracking.py:2643-2655 and the
+ * beaching hook's per-run constants, tracking.py:199-359).  A field slot
+ * holding W makes the step 3-D (z-level, mesh n_levels >= 2). */
 typedef struct {
     int32_t order;                 /* 1 Euler, 2 RK2 (midpoint), 4 RK4 */
     int32_t slot_a, slot_b, slot_t; /* bracket pair (a, b) and stage target t; b/t may be -1 */
@@ -72,8 +76,20 @@ typedef struct {
     double alphas[4];              /* per-stage fraction towards t (c_k*dt/dt_to_next) */
     double dt;                     /* positive seconds */
     double time_dir;               /* +1 forward, -1 backward */
-    int32_t exact_nearest;         /* mesh.exact_nearest (default config: 1) */
-    int32_t reserved;
+    int32_t exact_nearest;         /* anchor: 1 exact (mesh.exact_nearest, the default);
+                                      0 guard off, fused kernels' widened window;
+                                      2 guard off, batch 3x3 window healed exactly */
+    int32_t w_method;              /* 3-D W: 0 linear at the anchor ("nearest"), 2 barycentric */
+    int32_t kick;                  /* split uniform-Kh random walk (integrators.py:673) */
+    int32_t beach;                 /* beaching hook */
+    double kh_x, kh_y;             /* m^2/s */
+    uint64_t seed, real_id, step_i; /* counter-RNG key: (seed, real_id, pid, step_i, comp) */
+    int32_t use_depth, use_coastal, use_stochastic, on_failure; /* beaching criteria */
+    int32_t resuspension;          /* 0 never, 1 auto, 2 stochastic, 3 auto_or_stochastic */
+    int32_t min_wet_steps;
+    double depth_min, zone_depth, t_beach, rewet_hysteresis, t_resuspend;
+    double beach_dt;               /* step length seen by the hook (abs() taken) */
+    uint64_t beach_seed;
 } kelp_step_args;
 
 const char *kelp_last_error(void);

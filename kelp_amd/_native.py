@@ -31,6 +31,7 @@ class MeshDesc(ctypes.Structure):
         ("gh_nx", c_int64), ("gh_ny", c_int64), ("gh_wrap", c_int64), ("gh_far_rows", c_int64),
         ("r_earth", c_double),
         ("n_levels", c_int64), ("host_z_levels", c_void_p),
+        ("host_elem_coastal", c_void_p),
     ]
 
 
@@ -40,7 +41,7 @@ class Particles(ctypes.Structure):
         ("lon", c_void_p), ("lat", c_void_p), ("z", c_void_p),
         ("status", c_void_p), ("nn", c_void_p), ("fr", c_void_p),
         ("u", c_void_p), ("v", c_void_p), ("w", c_void_p),
-        ("pid", c_void_p),
+        ("pid", c_void_p), ("wet_streak", c_void_p),
     ]
 
 
@@ -49,7 +50,15 @@ class StepArgs(ctypes.Structure):
         ("order", c_int32), ("slot_a", c_int32), ("slot_b", c_int32), ("slot_t", c_int32),
         ("alpha0", c_double), ("alphas", c_double * 4),
         ("dt", c_double), ("time_dir", c_double),
-        ("exact_nearest", c_int32), ("reserved", c_int32),
+        ("exact_nearest", c_int32), ("w_method", c_int32),
+        ("kick", c_int32), ("beach", c_int32),
+        ("kh_x", c_double), ("kh_y", c_double),
+        ("seed", ctypes.c_uint64), ("real_id", ctypes.c_uint64), ("step_i", ctypes.c_uint64),
+        ("use_depth", c_int32), ("use_coastal", c_int32), ("use_stochastic", c_int32),
+        ("on_failure", c_int32), ("resuspension", c_int32), ("min_wet_steps", c_int32),
+        ("depth_min", c_double), ("zone_depth", c_double), ("t_beach", c_double),
+        ("rewet_hysteresis", c_double), ("t_resuspend", c_double), ("beach_dt", c_double),
+        ("beach_seed", ctypes.c_uint64),
     ]
 
 

@@ -1,0 +1,44 @@
+"""Print the constants of kmath.h (200-bit mpmath): the three-part pi/2
+split, pi and pi/2 as hi+lo, ln2 split 42+53 bits, the atan(j/8) table and the
+series coefficients.  Run: python kelp_amd/csrc/gen_kmath_tables.py"""
+import mpmath as mp
+
+mp.mp.prec = 200
+
+
+def rnd(v, bits):
+    m, e = mp.frexp(v)
+    s = mp.mpf(2) ** bits
+    return mp.ldexp(mp.nint(m * s) / s, e)
+
+
+def h(v):
+    return float(v).hex()
+
+
+def main():
+    pio2 = mp.pi / 2
+    p1 = rnd(pio2, 33)
+    p2 = rnd(pio2 - p1, 33)
+    p3 = pio2 - p1 - p2
+    print("KM_PIO2_1", h(p1), "KM_PIO2_2", h(p2), "KM_PIO2_3", h(p3))
+    print("KM_2_OVER_PI", h(2 / mp.pi))
+    for name, v in (("PI", mp.pi), ("PIO2", pio2)):
+        hi = float(v)
+        print(f"KM_{name}_HI", h(hi), f"KM_{name}_LO", h(v - hi))
+    ln2 = mp.log(2)
+    hi = rnd(ln2, 42)
+    print("KM_LN2_HI", h(hi), "KM_LN2_LO", h(ln2 - hi), "KM_INV_LN2", h(1 / ln2))
+    print("KM_SQRT2", h(mp.sqrt(2)))
+    for j in range(2, 17):
+        a = mp.atan(mp.mpf(j) / 8)
+        print(f"atan({j}/8)", h(a), h(a - float(a)))
+    print("sin", [h((-1) ** k / mp.factorial(2 * k + 1)) for k in range(1, 9)])
+    print("cos", [h((-1) ** k / mp.factorial(2 * k)) for k in range(2, 10)])
+    print("atan", [h(mp.mpf((-1) ** k) / (2 * k + 1)) for k in range(1, 14)])
+    print("log", [h(mp.mpf(2) / (2 * k + 1)) for k in range(1, 12)])
+    print("exp", [h(1 / mp.factorial(k)) for k in range(2, 16)])
+
+
+if __name__ == "__main__":
+    main()

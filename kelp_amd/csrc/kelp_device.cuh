@@ -49,6 +49,7 @@ struct DMesh {
     const int32_t* __restrict__ n2e;
     const double* __restrict__ depth;     // node bottom depth (m, positive down)
     const double* __restrict__ zlev;      // K interface depths (3-D)
+    const uint8_t* __restrict__ coastal;  // (E,) coastal-element flags (beaching)
     double cell, lon_min, lat_min, r_earth;
     int32_t nx, ny, wrap, far_rows;
     int32_t n_nodes, n_elems, max_v, n_levels;
@@ -478,10 +479,23 @@ __device__ __noinline__ int exact_widen(const DMesh& m, double lon_p, double lat
     }
 }
 
-// integrators.py:1471 _fused_site_anchor
-__device__ inline int site_anchor(const DMesh& m, double lon_p, double lat_p, bool exact)
+// Anchor of one evaluation site.  mode 1: the provably exact search
+// (mesh.exact_nearest); 0: the fused kernels' guard-off widened window
+// (integrators.py:1471 _fused_site_anchor); 2: the batch guard-off anchor of
+// core/mesh.py:1896 find_nearest_nodes (3x3 window, an empty window healed
+// with the true nearest node).
+__device__ inline int site_anchor(const DMesh& m, double lon_p, double lat_p, int mode)
 {
-    return exact ? nearest_exact(m, lon_p, lat_p) : nearest_widened(m, lon_p, lat_p);
+    if (mode == 1) return nearest_exact(m, lon_p, lat_p);
+    if (mode == 2) {
+        int ci, cj;
+        gh_cell_of(m, lon_p, lat_p, ci, cj);
+        Query q = make_query(lon_p, lat_p);
+        double d2;
+        int b = nearest_3x3(m, ci, cj, q, d2);
+        return b >= 0 ? b : nearest_exact(m, lon_p, lat_p);
+    }
+    return nearest_widened(m, lon_p, lat_p);
 }
 
 // ---------------------------------------------------------------------------
@@ -738,6 +752,257 @@ __device__ __forceinline__ double vblend(const StageFields<T>& f, int64_t idx, d
     }
     double a0 = beta0 * ld(f.a, idx) + alpha0 * ld(f.b, idx);
     return beta * a0 + alpha * ld(f.t, idx);
+}
+
+// ---------------------------------------------------------------------------
+// Vertical (z-level)
+// ---------------------------------------------------------------------------
+
+// numerics/vertical.py:40 find_layer
+__device__ inline int find_layer(const DMesh& m, double zp, double bottom)
+{
+    const int K = m.n_levels;
+    const double* zl = m.zlev;
+    if (K < 2) return 0;
+    if (zp <= zl[0]) return 0;
+    int max_layer = K - 2;
+    for (int k = 0; k < K - 1; ++k) {
+        if (zl[k + 1] > bottom) {
+            max_layer = k - 1 > 0 ? k - 1 : 0;
+            break;
+        }
+    }
+    if (zp >= bottom || zp >= zl[max_layer + 1]) return max_layer;
+    for (int k = 0; k < max_layer + 1; ++k)
+        if (zp < zl[k + 1]) return k;
+    return max_layer;
+}
+
+// numerics/vertical.py:1043 _tblend_scalar
+__device__ __forceinline__ double tblend(double a, double b, double t, double alpha0, double alpha)
+{
+    if (alpha < 1e-12) {
+        if (alpha0 == 0.0) return a;
+        return (1.0 - alpha0) * a + alpha0 * b;
+    }
+    if (alpha > 1.0 - 1e-12) return t;
+    if (alpha0 == 0.0) return (1.0 - alpha) * a + alpha * t;
+    double a0 = (1.0 - alpha0) * a + alpha0 * b;
+    return (1.0 - alpha) * a0 + alpha * t;
+}
+
+template <typename T>
+__device__ __forceinline__ double wlev(const StageFields<T>& fw, int64_t idx, double alpha0, double alpha)
+{
+    return tblend(ld(fw.a, idx), ld(fw.b, idx), ld(fw.t, idx), alpha0, alpha);
+}
+
+// numerics/vertical.py:1065 _interp_w_linear_single_tinterp
+template <typename T>
+__device__ inline double w_linear(const DMesh& m, const StageFields<T>& fw, double alpha0, double alpha,
+                                  double zp, int node, double time_dir)
+{
+    const double* zl = m.zlev;
+    const int K = m.n_levels;
+    const int64_t N = m.n_nodes;
+    if (K < 2) return time_dir * wlev(fw, node, alpha0, alpha);
+    if (zp <= zl[0]) return time_dir * wlev(fw, node, alpha0, alpha);
+    if (zp >= zl[K - 1]) return time_dir * wlev(fw, (int64_t)(K - 1) * N + node, alpha0, alpha);
+    int k = K - 2;
+    for (int j = 0; j < K - 1; ++j) {
+        if (zp < zl[j + 1]) {
+            k = j;
+            break;
+        }
+    }
+    double dz = zl[k + 1] - zl[k];
+    if (dz < 1e-12) return time_dir * wlev(fw, (int64_t)k * N + node, alpha0, alpha);
+    double a = (zp - zl[k]) / dz;
+    double w_lo = wlev(fw, (int64_t)k * N + node, alpha0, alpha);
+    double w_hi = wlev(fw, (int64_t)(k + 1) * N + node, alpha0, alpha);
+    return time_dir * ((1.0 - a) * w_lo + a * w_hi);
+}
+
+// numerics/vertical.py:745 _interp_w_bary_single_tinterp: first fan
+// sub-triangle of a 1-ring element holding the point (projection around the
+// seam-safe vertex mean, core/mesh.py:1974), bottom-aware linear W per vertex;
+// fallback = linear W at the anchor, depth clamped to its bottom.
+template <typename T>
+__device__ __noinline__ double w_bary(const DMesh& m, const StageFields<T>& fw, double alpha0, double alpha,
+                                      double xp, double yp, double zp, int node, double time_dir)
+{
+    const int k0 = m.n2e_off[node], k1 = m.n2e_off[node + 1];
+    for (int k = k0; k < k1; ++k) {
+        int4 en = m.elems[m.n2e[k]];
+        int verts[4];
+        int nv = 0;
+        for (int v = 0; v < m.max_v; ++v) {
+            int nd = elem_slot(en, v);
+            if (nd < 0) continue;
+            if (nv > 0 && nd == verts[0]) continue;
+            verts[nv++] = nd;
+        }
+        if (nv < 3) continue;
+        double lo[4], la[4];
+        for (int v = 0; v < nv; ++v) {
+            double2 ll = m.lonlat[verts[v]];
+            lo[v] = ll.x;
+            la[v] = ll.y;
+        }
+        double first = lo[0], lmin = first, lmax = first, ssum = 0.0;
+        for (int v = 0; v < nv; ++v) {
+            ssum += lo[v];
+            if (lo[v] < lmin) lmin = lo[v];
+            if (lo[v] > lmax) lmax = lo[v];
+        }
+        double lon0;
+        if (lmax - lmin > 180.0) {
+            double d = 0.0;
+            for (int v = 0; v < nv; ++v) d += km_pymod360(lo[v] - first + 180.0) - 180.0;
+            lon0 = first + d / nv;
+        } else {
+            lon0 = ssum / nv;
+        }
+        double lat0 = 0.0;
+        for (int v = 0; v < nv; ++v) lat0 += la[v];
+        lat0 /= nv;
+        double cos_lat0 = km_cos(lat0 * kDeg2Rad);
+        const double R = 6371000.0;
+        double vx[4], vy[4];
+        for (int v = 0; v < nv; ++v) {
+            double dlon = km_pymod360(lo[v] - lon0 + 180.0) - 180.0;
+            double dlat = la[v] - lat0;
+            vx[v] = dlon * kDeg2Rad * R * cos_lat0;
+            vy[v] = dlat * kDeg2Rad * R;
+        }
+        double dlon_p = km_pymod360(xp - lon0 + 180.0) - 180.0;
+        double px = dlon_p * kDeg2Rad * R * cos_lat0;
+        double py = (yp - lat0) * kDeg2Rad * R;
+        for (int t = 0; t < nv - 2; ++t) {
+            int i1 = t + 1, i2 = t + 2;
+            double ax = vx[0], ay = vy[0], bx = vx[i1], by = vy[i1], cx = vx[i2], cy = vy[i2];
+            double denom = (by - cy) * (ax - cx) + (cx - bx) * (ay - cy);
+            if (km_fabs(denom) < 1e-14) continue;
+            double w1 = ((by - cy) * (px - cx) + (cx - bx) * (py - cy)) / denom;
+            double w2 = ((cy - ay) * (px - cx) + (ax - cx) * (py - cy)) / denom;
+            double w3 = 1.0 - w1 - w2;
+            const double eps = -1e-6;
+            if (w1 >= eps && w2 >= eps && w3 >= eps) {
+                if (nv == 4) {
+                    double A[4], C[4], wts[4];
+                    for (int q = 0; q < 4; ++q) {
+                        int q1 = (q + 1) & 3, qm = (q + 3) & 3;
+                        A[q] = ((vx[q] - px) * (vy[q1] - py) - (vx[q1] - px) * (vy[q] - py));
+                        C[q] = ((vx[q] - vx[qm]) * (vy[q1] - vy[qm]) - (vx[q1] - vx[qm]) * (vy[q] - vy[qm]));
+                    }
+                    double ssw = 0.0;
+                    for (int q = 0; q < 4; ++q) {
+                        double ww = C[q] * A[(q + 1) & 3] * A[(q + 2) & 3];
+                        wts[q] = ww;
+                        ssw += ww;
+                    }
+                    double acc = 0.0;
+                    for (int q = 0; q < 4; ++q) {
+                        int nd = verts[q];
+                        double dep = m.depth[nd];
+                        double zn = zp < dep ? zp : dep;
+                        acc += (wts[q] / ssw) * w_linear(m, fw, alpha0, alpha, zn, nd, time_dir);
+                    }
+                    return acc;
+                }
+                int n0 = verts[0], n1 = verts[i1], n2 = verts[i2];
+                double d0 = m.depth[n0], d1 = m.depth[n1], d2 = m.depth[n2];
+                double wv0 = w_linear(m, fw, alpha0, alpha, zp < d0 ? zp : d0, n0, time_dir);
+                double wv1 = w_linear(m, fw, alpha0, alpha, zp < d1 ? zp : d1, n1, time_dir);
+                double wv2 = w_linear(m, fw, alpha0, alpha, zp < d2 ? zp : d2, n2, time_dir);
+                return w1 * wv0 + w2 * wv1 + w3 * wv2;
+            }
+        }
+    }
+    double dep = m.depth[node];
+    return w_linear(m, fw, alpha0, alpha, zp < dep ? zp : dep, node, time_dir);
+}
+
+// ---------------------------------------------------------------------------
+// Counter RNG (numerics/diffusion.py:181-246, numerics/beaching.py:46)
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x)
+{
+    x = x + 0x9E3779B97F4A7C15ULL;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+    x = x ^ (x >> 31);
+    return x;
+}
+
+__device__ __forceinline__ uint64_t hash5(uint64_t k0, uint64_t k1, uint64_t k2, uint64_t k3, uint64_t k4)
+{
+    uint64_t h = k0;
+    h = splitmix64(h ^ k1);
+    h = splitmix64(h ^ k2);
+    h = splitmix64(h ^ k3);
+    h = splitmix64(h ^ k4);
+    return h;
+}
+
+__device__ inline void normal_pair(uint64_t seed, uint64_t real_id, uint64_t pid, uint64_t step,
+                                   uint64_t comp, double& g1, double& g2)
+{
+    const double INV_2_32 = 1.0 / 4294967296.0;
+    const double HALF_INV_2_32 = 0.5 * INV_2_32;
+    const double TWO_PI = 6.283185307179586;
+    uint64_t h = hash5(seed, real_id, pid, step, comp);
+    double high = (double)((h >> 32) & 0xFFFFFFFFULL);
+    double low = (double)(h & 0xFFFFFFFFULL);
+    double u1 = high * INV_2_32 + HALF_INV_2_32;
+    double u2 = low * INV_2_32 + HALF_INV_2_32;
+    double r = km_sqrt(-2.0 * km_log(u1));
+    double s, c;
+    km_sincos(TWO_PI * u2, &s, &c);
+    g1 = r * c;
+    g2 = r * s;
+}
+
+__device__ __forceinline__ double u01(uint64_t seed, uint64_t real_id, uint64_t pid, uint64_t step, uint64_t comp)
+{
+    uint64_t h = hash5(seed, real_id, pid, step, comp);
+    return (double)(h >> 11) * (1.0 / 9007199254740992.0);
+}
+
+constexpr double kRMetric = 111120.0 / 0.017453292519943295;  // diffusion.py:532
+
+// numerics/diffusion.py:539 sphere_displace
+__device__ inline void sphere_displace(double lon, double lat, double dx_m, double dy_m, double& lon_o,
+                                       double& lat_o)
+{
+    double dist = km_sqrt(dx_m * dx_m + dy_m * dy_m);
+    if (dist == 0.0) {
+        lon_o = lon;
+        lat_o = lat;
+        return;
+    }
+    double lam = lon * kDeg2Rad, phi = lat * kDeg2Rad;
+    double sl, cl, sp, cp;
+    km_sincos(lam, &sl, &cl);
+    km_sincos(phi, &sp, &cp);
+    double delta = dist / kRMetric;
+    double sd, c;
+    km_sincos(delta, &sd, &c);
+    double s = sd / dist;
+    double tx = -sl * dx_m - sp * cl * dy_m;
+    double ty = cl * dx_m - sp * sl * dy_m;
+    double tz = cp * dy_m;
+    double x2 = cp * cl * c + tx * s;
+    double y2 = cp * sl * c + ty * s;
+    double z2 = sp * c + tz * s;
+    if (z2 > 1.0)
+        z2 = 1.0;
+    else if (z2 < -1.0)
+        z2 = -1.0;
+    lat_o = km_asin(z2) / kDeg2Rad;
+    double lon2 = km_atan2(y2, x2) / kDeg2Rad;
+    lon_o = km_pymod360(lon2 + 180.0) - 180.0;
 }
 
 }  // namespace kelp

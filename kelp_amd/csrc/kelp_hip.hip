@@ -135,25 +135,42 @@ __global__ void gather_xyz_kernel(int64_t n, const double4* __restrict__ csr, do
 static unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
 // ---------------------------------------------------------------------------
-// step kernels
+// step kernel
 // ---------------------------------------------------------------------------
 
-struct StepParams {
+// One step's configuration (step_fn keywords, tracking.py:2643-2655, plus the
+// beaching hook's per-run constants, tracking.py:199).
+struct StepCfg {
     double alpha0;
     double alphas[4];
     double dt, time_dir;
-    int exact;
+    int exact, w_method, kick, beach;
+    double kh_x, kh_y;
+    uint64_t seed, real_id, step_i;
+    int use_depth, use_coastal, use_stochastic, on_failure;
+    int resuspension, min_wet_steps;
+    double depth_min, zone_depth, t_beach, rewet_hysteresis, t_resuspend, beach_dt;
+    uint64_t beach_seed;
 };
 
 struct PState {
     double* __restrict__ lon;
     double* __restrict__ lat;
+    double* __restrict__ z;
     int32_t* __restrict__ status;
     int32_t* __restrict__ nn;
     int8_t* __restrict__ fr;
     double* __restrict__ u;
     double* __restrict__ v;
+    double* __restrict__ w;
+    const int64_t* __restrict__ pid;
+    int32_t* __restrict__ ws;
     int64_t n;
+};
+
+template <typename T>
+struct Fields {
+    StageFields<T> u, v, w;
 };
 
 __device__ __forceinline__ double wrap_lon(double x) { return km_pymod360(x + 180.0) - 180.0; }
@@ -172,12 +189,21 @@ __device__ __forceinline__ double z0_of(double lat)
     return z0;
 }
 
-// interpolate.py:1341 _interp_barycentric_core_tinterp (2-D node layout):
-// containment around the anchor, vertex weights, blended vertex values.
+__device__ __forceinline__ double clamp_z(const DMesh& m, double z, int nn)
+{
+    if (z < m.zlev[0]) z = m.zlev[0];
+    double bottom = m.depth[nn];
+    if (z > bottom) z = bottom;
+    return z;
+}
+
+// interpolate.py:1341/:1444 _interp_barycentric_core_tinterp(_3d):
+// containment around the anchor, vertex weights, blended vertex values of
+// the layer plane starting at `off`.
 template <typename T>
-__device__ inline bool interp_uv_2d(const DMesh& m, const StageFields<T>& fu, const StageFields<T>& fv,
-                                    double alpha0, double alpha, double time_dir, double xp, double yp,
-                                    int nn, double& U, double& V)
+__device__ inline bool interp_uv(const DMesh& m, const Fields<T>& f, double alpha0, double alpha,
+                                 double time_dir, double xp, double yp, int nn, int64_t off, double& U,
+                                 double& V)
 {
     U = 0.0;
     V = 0.0;
@@ -187,9 +213,9 @@ __device__ inline bool interp_uv_2d(const DMesh& m, const StageFields<T>& fu, co
     if (!vertex_weights(m, e, xp, yp, W)) return false;
     double su = 0.0, sv = 0.0;
     for (int k = 0; k < W.n; ++k) {
-        int64_t idx = W.v[k];
-        double un = vblend(fu, idx, alpha0, alpha);
-        double vn = vblend(fv, idx, alpha0, alpha);
+        int64_t idx = off + W.v[k];
+        double un = vblend(f.u, idx, alpha0, alpha);
+        double vn = vblend(f.v, idx, alpha0, alpha);
         su += W.w[k] * un;
         sv += W.w[k] * vn;
     }
@@ -198,96 +224,247 @@ __device__ inline bool interp_uv_2d(const DMesh& m, const StageFields<T>& fu, co
     return true;
 }
 
-// One fused 2-D step per particle: integrators.py:2522
-// _fused_rk2_step_2d_bary_tinterp (ORDER 2), :2642 _fused_rk4_step_2d_bary_tinterp
-// (ORDER 4) and the barycentric Euler of :4997 _euler_step_unfused (ORDER 1).
-template <int ORDER, typename T>
-__global__ __launch_bounds__(256) void step2d_kernel(DMesh m, StageFields<T> fu, StageFields<T> fv,
-                                                     StepParams sp, PState s)
+// One evaluation site: anchor, layer at the anchor (vertical.py:40), u/v on
+// that layer, W by the configured method (integrators.py:1260 _compute_w).
+template <bool IS3D, typename T>
+__device__ inline bool eval_site(const DMesh& m, const Fields<T>& f, const StepCfg& c, double alpha,
+                                 double lon, double lat, double zp, int& nn, double& U, double& V, double& W)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= s.n) return;
-    if (s.status[i] != kAlive) {
-        s.u[i] = 0.0;
-        s.v[i] = 0.0;
-        return;
+    nn = site_anchor(m, lon, lat, c.exact);
+    int64_t off = 0;
+    if (IS3D) off = (int64_t)find_layer(m, zp, m.depth[nn]) * m.n_nodes;
+    bool ok = interp_uv(m, f, c.alpha0, alpha, c.time_dir, lon, lat, nn, off, U, V);
+    W = 0.0;
+    if (IS3D) {
+        if (c.w_method == 0)
+            W = w_linear(m, f.w, c.alpha0, alpha, zp, nn, c.time_dir);
+        else
+            W = w_bary(m, f.w, c.alpha0, alpha, lon, lat, zp, nn, c.time_dir);
     }
-    const double lon0 = s.lon[i], lat0 = s.lat[i];
-    const double dt = sp.dt, td = sp.time_dir;
-    const bool exact = sp.exact != 0;
-    int node_i = site_anchor(m, lon0, lat0, exact);
-    double U1, V1;
-    bool ok = interp_uv_2d(m, fu, fv, sp.alpha0, sp.alphas[0], td, lon0, lat0, node_i, U1, V1);
-    s.u[i] = U1;
-    s.v[i] = V1;
-    double lon_new = lon0, lat_new = lat0;
+    return ok;
+}
+
+// The deterministic step of one alive particle, in place.  Euler follows
+// integrators.py:4997 _euler_step_unfused, RK2 the fused barycentric kernels
+// :2522 (2-D) / :3648 (3-D), RK4 :2642 (2-D) and :5760 _rk4_step_unfused
+// (3-D).  Returns the failure code (0 = alive).
+template <int ORDER, bool IS3D, typename T>
+__device__ inline int det_step(const DMesh& m, const Fields<T>& f, const StepCfg& c, double& lon, double& lat,
+                               double& z, int& nn, double& Uo, double& Vo, double& Wo)
+{
+    const double dt = c.dt;
+    const double lon0 = lon, lat0 = lat, zz0 = z;
+    int node_i;
+    double U1, V1, W1;
+    bool ok = eval_site<IS3D>(m, f, c, c.alphas[0], lon0, lat0, zz0, node_i, U1, V1, W1);
+    Uo = U1;
+    Vo = V1;
+    Wo = W1;
+    double lon_new = lon0, lat_new = lat0, z_new = zz0;
     if (ok) {
         if (ORDER == 1) {
             lon_new = wrap_lon(lon0 + dt * U1 / z0_of(lat0));
             lat_new = clamp_lat(lat0 + dt * V1 / kSLM);
+            z_new = zz0 + W1 * dt;
         } else if (ORDER == 2) {
             double z0 = z0_of(lat0);
             double lon_mid = wrap_lon(lon0 + 0.5 * dt * U1 / z0);
             double lat_mid = clamp_lat(lat0 + 0.5 * dt * V1 / kSLM);
-            int nn_mid = site_anchor(m, lon_mid, lat_mid, exact);
-            double U2, V2;
-            ok = interp_uv_2d(m, fu, fv, sp.alpha0, sp.alphas[1], td, lon_mid, lat_mid, nn_mid, U2, V2);
+            double z_mid = zz0 + W1 * 0.5 * dt;
+            int nn_mid;
+            double U2, V2, W2;
+            ok = eval_site<IS3D>(m, f, c, c.alphas[1], lon_mid, lat_mid, z_mid, nn_mid, U2, V2, W2);
             if (ok) {
                 double z0m = z0_of(lat_mid);
                 lon_new = wrap_lon(lon0 + dt * U2 / z0m);
                 lat_new = clamp_lat(lat0 + dt * V2 / kSLM);
+                z_new = zz0 + W2 * dt;
             }
         } else {
             double dlon1 = dt * U1 / z0_of(lat0);
             double dlat1 = dt * V1 / kSLM;
+            double dz1 = dt * W1;
             double lon2 = wrap_lon(lon0 + 0.5 * dlon1);
             double lat2 = clamp_lat(lat0 + 0.5 * dlat1);
-            int nn2 = site_anchor(m, lon2, lat2, exact);
-            double U2, V2, U3, V3, U4, V4;
-            ok = interp_uv_2d(m, fu, fv, sp.alpha0, sp.alphas[1], td, lon2, lat2, nn2, U2, V2);
+            double z2 = zz0 + 0.5 * dz1;
+            int nn2, nn3, nn4;
+            double U2, V2, W2, U3, V3, W3, U4, V4, W4;
+            ok = eval_site<IS3D>(m, f, c, c.alphas[1], lon2, lat2, z2, nn2, U2, V2, W2);
             if (ok) {
                 double dlon2 = dt * U2 / z0_of(lat2);
                 double dlat2 = dt * V2 / kSLM;
+                double dz2 = dt * W2;
                 double lon3 = wrap_lon(lon0 + 0.5 * dlon2);
                 double lat3 = clamp_lat(lat0 + 0.5 * dlat2);
-                int nn3 = site_anchor(m, lon3, lat3, exact);
-                ok = interp_uv_2d(m, fu, fv, sp.alpha0, sp.alphas[2], td, lon3, lat3, nn3, U3, V3);
+                double z3 = zz0 + 0.5 * dz2;
+                ok = eval_site<IS3D>(m, f, c, c.alphas[2], lon3, lat3, z3, nn3, U3, V3, W3);
                 if (ok) {
                     double dlon3 = dt * U3 / z0_of(lat3);
                     double dlat3 = dt * V3 / kSLM;
+                    double dz3 = dt * W3;
                     double lon4 = wrap_lon(lon0 + dlon3);
                     double lat4 = clamp_lat(lat0 + dlat3);
-                    int nn4 = site_anchor(m, lon4, lat4, exact);
-                    ok = interp_uv_2d(m, fu, fv, sp.alpha0, sp.alphas[3], td, lon4, lat4, nn4, U4, V4);
+                    double z4 = zz0 + dz3;
+                    ok = eval_site<IS3D>(m, f, c, c.alphas[3], lon4, lat4, z4, nn4, U4, V4, W4);
                     if (ok) {
                         double dlon4 = dt * U4 / z0_of(lat4);
                         double dlat4 = dt * V4 / kSLM;
+                        double dz4 = dt * W4;
                         double dlon = (dlon1 + 2.0 * dlon2 + 2.0 * dlon3 + dlon4) / 6.0;
                         double dlat = (dlat1 + 2.0 * dlat2 + 2.0 * dlat3 + dlat4) / 6.0;
+                        double dz = (dz1 + 2.0 * dz2 + 2.0 * dz3 + dz4) / 6.0;
                         lon_new = wrap_lon(lon0 + dlon);
                         lat_new = clamp_lat(lat0 + dlat);
+                        z_new = zz0 + dz;
                     }
                 }
             }
         }
     }
     if (!ok) {
-        s.status[i] = kOutside;
-        s.fr[i] = kFrInterp;
-        s.nn[i] = node_i;
-        return;
+        nn = node_i;
+        return kFrInterp;
     }
-    int nn_new = site_anchor(m, lon_new, lat_new, exact);
+    int nn_new = site_anchor(m, lon_new, lat_new, c.exact);
     int e = find_containing(m, lon_new, lat_new, nn_new);
     if (e < 0) {
-        s.status[i] = kOutside;
-        s.fr[i] = kFrOOB;
-        s.nn[i] = node_i;
+        nn = node_i;
+        return kFrOOB;
+    }
+    lon = lon_new;
+    lat = lat_new;
+    if (IS3D) z = clamp_z(m, z_new, nn_new);
+    nn = nn_new;
+    return 0;
+}
+
+// integrators.py:673 _apply_split_diffusion_kick for one particle alive after
+// the deterministic step (uniform Kh, counter RNG keyed by the particle id).
+template <bool IS3D>
+__device__ __noinline__ void kick_one(const DMesh& m, const StepCfg& c, uint64_t pid, double pre_lon,
+                                      double pre_lat, double pre_z, int pre_nn, double& lon, double& lat,
+                                      double& z, int32_t& st, int& nn, int8_t& fr)
+{
+    double g1, g2;
+    normal_pair(c.seed, c.real_id, pid, c.step_i, 0, g1, g2);
+    double abs_dt = c.dt >= 0.0 ? c.dt : -c.dt;
+    double sqrt_abs_dt = km_sqrt(abs_dt);
+    double sigma_x = km_sqrt(2.0 * c.kh_x);
+    double sigma_y = km_sqrt(2.0 * c.kh_y);
+    double dx = sigma_x * sqrt_abs_dt * g1;
+    double dy = sigma_y * sqrt_abs_dt * g2;
+    if (c.kh_x != c.kh_y) {
+        double la = lat;
+        if (la < -89.9) la = -89.9;
+        if (la > 89.9) la = 89.9;
+        double corr = (c.kh_x - c.kh_y) * (km_sin(la * kDeg2Rad) / km_cos(la * kDeg2Rad)) / kRMetric *
+                      km_fabs(c.dt);
+        dy = dy + corr;
+    }
+    double lon_k, lat_k;
+    sphere_displace(lon, lat, dx, dy, lon_k, lat_k);
+    int nn_k = site_anchor(m, lon_k, lat_k, c.exact == 1 ? 1 : 2);  // batch find_nearest_nodes
+    int e = find_containing(m, lon_k, lat_k, nn_k);
+    if (e < 0) {
+        lon = pre_lon;
+        lat = pre_lat;
+        z = pre_z;
+        nn = pre_nn;
+        st = kOutside;
+        fr = kFrOOB;
         return;
     }
-    s.lon[i] = lon_new;
-    s.lat[i] = lat_new;
-    s.nn[i] = nn_new;
+    lon = lon_k;
+    lat = lat_k;
+    nn = nn_k;
+    if (IS3D) z = clamp_z(m, z, nn_k);
+}
+
+// tracking.py:199 _apply_beaching_hook + numerics/beaching.py:95
+// evaluate_beaching, one particle (prev = status at the step start).
+__device__ __noinline__ void beach_one(const DMesh& m, const StepCfg& c, uint64_t pid, int32_t prev, double lon,
+                                       double lat, int32_t& st, int& nn, int8_t& fr, int32_t& ws)
+{
+    const bool need_topo = c.use_coastal || c.use_stochastic;
+    const bool newly_failed = (st == kOutside) && (prev == kAlive);
+    if (newly_failed && nn < 0) nn = site_anchor(m, lon, lat, c.exact == 1 ? 1 : 2);
+    const int n = nn;
+    int containing = -1;
+    if (need_topo && (st == kAlive || st == kBeached || newly_failed) && n >= 0)
+        containing = find_containing(m, lon, lat, n);
+    const bool coastal = need_topo && containing >= 0 && m.coastal != nullptr && m.coastal[containing];
+    const double h_raw = n >= 0 ? m.depth[n] : __builtin_inf();
+    if (c.on_failure && newly_failed) {
+        if (h_raw < c.zone_depth || coastal) {
+            st = kBeached;
+            fr = 0;
+        }
+    }
+    const bool alive = st == kAlive, beached = st == kBeached;
+    bool to_beach = false;
+    if (c.use_depth) to_beach |= h_raw < c.depth_min;
+    if (c.use_coastal) to_beach |= coastal;
+    if (c.use_stochastic && c.t_beach > 0) {
+        bool in_zone = h_raw < c.zone_depth || coastal;
+        double p = 1.0 - km_exp(-km_fabs(c.beach_dt) / c.t_beach);
+        if (alive && in_zone && u01(c.beach_seed, c.real_id, pid, c.step_i, 10) < p) to_beach = true;
+    }
+    to_beach = to_beach && alive;
+    bool to_release = false;
+    int32_t w = ws;
+    if (c.resuspension != 0 && beached) {
+        bool rewet = h_raw > (c.depth_min + c.rewet_hysteresis) && n >= 0;
+        w = rewet ? w + 1 : 0;
+        if (c.resuspension == 1 || c.resuspension == 3) to_release |= w >= c.min_wet_steps;
+        if ((c.resuspension == 2 || c.resuspension == 3) && c.t_resuspend > 0) {
+            double p_r = 1.0 - km_exp(-km_fabs(c.beach_dt) / c.t_resuspend);
+            if (rewet && u01(c.beach_seed, c.real_id, pid, c.step_i, 11) < p_r) to_release = true;
+        }
+    }
+    if (to_release) w = 0;
+    if (!beached) w = 0;
+    ws = w;
+    if (to_beach) st = kBeached;
+    if (to_release) {
+        st = kAlive;
+        fr = 0;
+    }
+}
+
+// One full step per particle: deterministic step, split Kh kick, beaching.
+template <int ORDER, bool IS3D, typename T>
+__global__ __launch_bounds__(256) void step_kernel(DMesh m, Fields<T> f, StepCfg c, PState s)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.n) return;
+    const int32_t prev = s.status[i];
+    int32_t st = prev;
+    double lon = s.lon[i], lat = s.lat[i];
+    double z = IS3D ? s.z[i] : 0.0;
+    int nn = s.nn[i];
+    int8_t fr = s.fr[i];
+    double U = 0.0, V = 0.0, W = 0.0;
+    if (prev == kAlive) {
+        const double pre_lon = lon, pre_lat = lat, pre_z = z;
+        const int pre_nn = nn;
+        int code = det_step<ORDER, IS3D>(m, f, c, lon, lat, z, nn, U, V, W);
+        if (code) {
+            st = kOutside;
+            fr = (int8_t)code;
+        } else if (c.kick && !(c.kh_x == 0.0 && c.kh_y == 0.0)) {
+            kick_one<IS3D>(m, c, (uint64_t)s.pid[i], pre_lon, pre_lat, pre_z, pre_nn, lon, lat, z, st, nn, fr);
+        }
+    }
+    if (c.beach) beach_one(m, c, (uint64_t)s.pid[i], prev, lon, lat, st, nn, fr, s.ws[i]);
+    s.lon[i] = lon;
+    s.lat[i] = lat;
+    if (IS3D) s.z[i] = z;
+    s.status[i] = st;
+    s.nn[i] = nn;
+    s.fr[i] = fr;
+    s.u[i] = U;
+    s.v[i] = V;
+    if (IS3D && s.w) s.w[i] = W;
 }
 
 __global__ void nearest_kernel(DMesh m, int64_t n, const double* __restrict__ lon,
@@ -319,27 +496,46 @@ __global__ void containing_kernel(DMesh m, int64_t n, const double* __restrict__
 }
 
 template <typename T>
-static StageFields<T> stage_of(const FieldSlot& a, const FieldSlot* b, const FieldSlot* t, bool u)
+static StageFields<T> stage_of(const void* a, const void* b, const void* t)
 {
     StageFields<T> f;
-    f.a = static_cast<const T*>(u ? a.u : a.v);
-    f.b = b ? static_cast<const T*>(u ? b->u : b->v) : f.a;
-    f.t = t ? static_cast<const T*>(u ? t->u : t->v) : f.a;
+    f.a = static_cast<const T*>(a);
+    f.b = b ? static_cast<const T*>(b) : f.a;
+    f.t = t ? static_cast<const T*>(t) : f.a;
     return f;
 }
 
 template <typename T>
-static void launch_step2d(kelp_ctx* c, int order, const FieldSlot& a, const FieldSlot* b, const FieldSlot* t,
-                          const StepParams& sp, const PState& ps)
+static Fields<T> fields_of(const FieldSlot& a, const FieldSlot* b, const FieldSlot* t)
 {
-    StageFields<T> fu = stage_of<T>(a, b, t, true), fv = stage_of<T>(a, b, t, false);
+    Fields<T> f;
+    f.u = stage_of<T>(a.u, b ? b->u : nullptr, t ? t->u : nullptr);
+    f.v = stage_of<T>(a.v, b ? b->v : nullptr, t ? t->v : nullptr);
+    f.w = stage_of<T>(a.w, b ? b->w : nullptr, t ? t->w : nullptr);
+    return f;
+}
+
+template <bool IS3D, typename T>
+static void launch_step_t(kelp_ctx* c, int order, const Fields<T>& f, const StepCfg& cfg, const PState& ps)
+{
     unsigned g = grid_for(ps.n, 256);
     if (order == 1)
-        step2d_kernel<1, T><<<g, 256, 0, c->stream>>>(c->m, fu, fv, sp, ps);
+        step_kernel<1, IS3D, T><<<g, 256, 0, c->stream>>>(c->m, f, cfg, ps);
     else if (order == 2)
-        step2d_kernel<2, T><<<g, 256, 0, c->stream>>>(c->m, fu, fv, sp, ps);
+        step_kernel<2, IS3D, T><<<g, 256, 0, c->stream>>>(c->m, f, cfg, ps);
     else
-        step2d_kernel<4, T><<<g, 256, 0, c->stream>>>(c->m, fu, fv, sp, ps);
+        step_kernel<4, IS3D, T><<<g, 256, 0, c->stream>>>(c->m, f, cfg, ps);
+}
+
+template <typename T>
+static void launch_step(kelp_ctx* c, int order, bool is3d, const FieldSlot& a, const FieldSlot* b,
+                        const FieldSlot* t, const StepCfg& cfg, const PState& ps)
+{
+    Fields<T> f = fields_of<T>(a, b, t);
+    if (is3d)
+        launch_step_t<true, T>(c, order, f, cfg, ps);
+    else
+        launch_step_t<false, T>(c, order, f, cfg, ps);
 }
 
 static const FieldSlot* opt_slot(kelp_ctx* c, int k)
@@ -500,12 +696,14 @@ int kelp_mesh_upload(kelp_ctx* c, const kelp_mesh_desc* d)
     int32_t *d_off, *d_ids, *d_n2e_off, *d_n2e;
     int4* d_elems;
     double *d_depth, *d_zlev;
+    uint8_t* d_coastal = nullptr;
     int rc;
     if ((rc = dev_alloc(c, &d_lonlat, N)) || (rc = dev_alloc(c, &d_xyz, N)) ||
         (rc = dev_alloc(c, &d_off, n_cells + 1)) || (rc = dev_alloc(c, &d_ids, N)) ||
         (rc = dev_alloc(c, &d_elems, E)) || (rc = dev_alloc(c, &d_n2e_off, N + 1)) ||
         (rc = dev_alloc(c, &d_n2e, n2e.size())) || (rc = dev_alloc(c, &d_depth, N)) ||
-        (rc = dev_alloc(c, &d_zlev, d->n_levels > 0 ? d->n_levels : 1))) {
+        (rc = dev_alloc(c, &d_zlev, d->n_levels > 0 ? d->n_levels : 1)) ||
+        (d->host_elem_coastal && (rc = dev_alloc(c, &d_coastal, E)))) {
         free_mesh(c);
         return rc;
     }
@@ -521,6 +719,8 @@ int kelp_mesh_upload(kelp_ctx* c, const kelp_mesh_desc* d)
     HIP_TRY(hipMemcpy(d_depth, depth.data(), N * sizeof(double), hipMemcpyHostToDevice));
     if (d->n_levels > 0)
         HIP_TRY(hipMemcpy(d_zlev, d->host_z_levels, d->n_levels * sizeof(double), hipMemcpyHostToDevice));
+    if (d_coastal)
+        HIP_TRY(hipMemcpy(d_coastal, d->host_elem_coastal, E * sizeof(uint8_t), hipMemcpyHostToDevice));
     node_xyz_csr_kernel<<<grid_for(N, 256), 256, 0, c->stream>>>(N, d_lonlat, d_ids, d_xyz);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -533,6 +733,7 @@ int kelp_mesh_upload(kelp_ctx* c, const kelp_mesh_desc* d)
     m.n2e = d_n2e;
     m.depth = d_depth;
     m.zlev = d_zlev;
+    m.coastal = d_coastal;
     m.cell = d->gh_cell;
     m.lon_min = d->gh_lon_min;
     m.lat_min = d->gh_lat_min;
@@ -613,10 +814,17 @@ int kelp_step(kelp_ctx* c, const kelp_particles* p, const kelp_step_args* a)
         return fail(KELP_EINVAL, "kelp_step: order must be 1, 2 or 4 (got %d)", a->order);
     if (!(a->dt > 0.0)) return fail(KELP_EINVAL, "kelp_step: dt must be positive (direction goes in time_dir)");
     if (a->time_dir != 1.0 && a->time_dir != -1.0) return fail(KELP_EINVAL, "kelp_step: time_dir must be +1 or -1");
+    if (a->w_method != 0 && a->w_method != 2)
+        return fail(KELP_EINVAL, "kelp_step: w_method must be 0 (nearest) or 2 (barycentric)");
     if (p->n < 0) return fail(KELP_EINVAL, "kelp_step: negative particle count");
     if (p->n == 0) return KELP_OK;
     if (!p->lon || !p->lat || !p->status || !p->nn || !p->fr || !p->u || !p->v)
         return fail(KELP_EINVAL, "kelp_step: missing particle array");
+    if ((a->kick || a->beach) && !p->pid) return fail(KELP_EINVAL, "kelp_step: the kick and beaching need pid");
+    if (a->beach && !p->wet_streak) return fail(KELP_EINVAL, "kelp_step: beaching needs wet_streak");
+    if (a->kick && (!(a->kh_x >= 0.0) || !(a->kh_y >= 0.0))) return fail(KELP_EINVAL, "kelp_step: Kh must be >= 0");
+    if (a->beach && (a->use_coastal || a->use_stochastic) && !c->m.coastal)
+        return fail(KELP_ESTATE, "kelp_step: coastal/stochastic beaching needs the coastal-element flags");
     for (int k : {a->slot_a, a->slot_b, a->slot_t})
         if (k >= kMaxSlots || k < -1) return fail(KELP_EINVAL, "kelp_step: slot %d out of range", k);
     if (a->slot_a < 0 || c->slots[a->slot_a].dtype < 0) return fail(KELP_ESTATE, "kelp_step: slot_a is empty");
@@ -625,26 +833,59 @@ int kelp_step(kelp_ctx* c, const kelp_particles* p, const kelp_step_args* a)
     const FieldSlot* st = opt_slot(c, a->slot_t);
     for (const FieldSlot* s : {sb, st}) {
         if (!s) continue;
-        if (s->dtype != sa.dtype || s->n_planes != sa.n_planes)
+        if (s->dtype < 0) return fail(KELP_ESTATE, "kelp_step: slot_b/slot_t is empty");
+        if (s->dtype != sa.dtype || s->n_planes != sa.n_planes || s->n_w_planes != sa.n_w_planes)
             return fail(KELP_EINVAL, "kelp_step: snapshot slots differ in dtype or shape");
     }
     if (a->alpha0 != 0.0 && !sb) return fail(KELP_EINVAL, "kelp_step: alpha0 != 0 needs slot_b");
-    const int n_st = a->order;
-    for (int k = 0; k < n_st; ++k)
+    for (int k = 0; k < a->order; ++k)
         if (a->alphas[k] >= 1e-12 && !st) return fail(KELP_EINVAL, "kelp_step: stage alpha > 0 needs slot_t");
-    if (sa.n_planes != 1) return fail(KELP_EINVAL, "kelp_step: 3-D fields are not supported by this entry point");
+    const bool is3d = sa.w != nullptr;
+    const int K = c->m.n_levels;
+    if (is3d) {
+        if (K < 2) return fail(KELP_EINVAL, "kelp_step: a 3-D field needs a mesh with >= 2 levels");
+        if (sa.n_planes != K - 1 || sa.n_w_planes != K)
+            return fail(KELP_EINVAL, "kelp_step: 3-D field shapes (%lld, %lld) do not match K=%d",
+                        (long long)sa.n_planes, (long long)sa.n_w_planes, K);
+        if (!p->z) return fail(KELP_EINVAL, "kelp_step: 3-D step needs z");
+    } else if (sa.n_planes != 1) {
+        return fail(KELP_EINVAL, "kelp_step: a layered u/v field needs W (3-D)");
+    }
     HIP_TRY(hipSetDevice(c->device));
-    StepParams sp{};
-    sp.alpha0 = a->alpha0;
-    for (int k = 0; k < 4; ++k) sp.alphas[k] = a->alphas[k];
-    sp.dt = a->dt;
-    sp.time_dir = a->time_dir;
-    sp.exact = a->exact_nearest ? 1 : 0;
-    PState ps{p->lon, p->lat, p->status, p->nn, p->fr, p->u, p->v, p->n};
+    StepCfg cfg{};
+    cfg.alpha0 = a->alpha0;
+    for (int k = 0; k < 4; ++k) cfg.alphas[k] = a->alphas[k];
+    cfg.dt = a->dt;
+    cfg.time_dir = a->time_dir;
+    if (a->exact_nearest < 0 || a->exact_nearest > 2)
+        return fail(KELP_EINVAL, "kelp_step: exact_nearest must be 0, 1 or 2");
+    cfg.exact = a->exact_nearest;
+    cfg.w_method = a->w_method;
+    cfg.kick = a->kick ? 1 : 0;
+    cfg.beach = a->beach ? 1 : 0;
+    cfg.kh_x = a->kh_x;
+    cfg.kh_y = a->kh_y;
+    cfg.seed = a->seed;
+    cfg.real_id = a->real_id;
+    cfg.step_i = a->step_i;
+    cfg.use_depth = a->use_depth;
+    cfg.use_coastal = a->use_coastal;
+    cfg.use_stochastic = a->use_stochastic;
+    cfg.on_failure = a->on_failure;
+    cfg.resuspension = a->resuspension;
+    cfg.min_wet_steps = a->min_wet_steps;
+    cfg.depth_min = a->depth_min;
+    cfg.zone_depth = a->zone_depth;
+    cfg.t_beach = a->t_beach;
+    cfg.rewet_hysteresis = a->rewet_hysteresis;
+    cfg.t_resuspend = a->t_resuspend;
+    cfg.beach_dt = a->beach_dt;
+    cfg.beach_seed = a->beach_seed;
+    PState ps{p->lon, p->lat, p->z, p->status, p->nn, p->fr, p->u, p->v, p->w, p->pid, p->wet_streak, p->n};
     if (sa.dtype == KELP_DT_F32)
-        launch_step2d<float>(c, a->order, sa, sb, st, sp, ps);
+        launch_step<float>(c, a->order, is3d, sa, sb, st, cfg, ps);
     else
-        launch_step2d<double>(c, a->order, sa, sb, st, sp, ps);
+        launch_step<double>(c, a->order, is3d, sa, sb, st, cfg, ps);
     HIP_TRY(hipGetLastError());
     return KELP_OK;
 }

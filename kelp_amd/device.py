@@ -46,6 +46,7 @@ class DeviceParticles:
     v: object
     w: object
     pid: object
+    ws: object
 
     @property
     def n(self) -> int:
@@ -54,11 +55,54 @@ class DeviceParticles:
     def as_struct(self) -> Particles:
         return Particles(self.n, _ptr(self.lon), _ptr(self.lat), _ptr(self.z), _ptr(self.status),
                          _ptr(self.nn), _ptr(self.fr), _ptr(self.u), _ptr(self.v), _ptr(self.w),
-                         _ptr(self.pid))
+                         _ptr(self.pid), _ptr(self.ws))
 
     def to_host(self) -> Dict[str, np.ndarray]:
         return {k: getattr(self, k).cpu().numpy() for k in
-                ("lon", "lat", "z", "status", "nn", "fr", "u", "v", "w", "pid")}
+                ("lon", "lat", "z", "status", "nn", "fr", "u", "v", "w", "pid", "ws")}
+
+
+RESUSPENSION_CODES = {"never": 0, "auto": 1, "stochastic": 2, "auto_or_stochastic": 3}
+
+
+def make_step_args(order, slot_a, dt, time_dir, slot_b=-1, slot_t=-1, alpha0=0.0,
+                   alphas=(0.0, 0.0, 0.0, 0.0), exact_nearest=True, w_method=2, kick=None,
+                   beach=None, step_i=0, real_id=0) -> StepArgs:
+    a = StepArgs()
+    a.order = int(order)
+    a.slot_a, a.slot_b, a.slot_t = int(slot_a), int(slot_b), int(slot_t)
+    a.alpha0 = float(alpha0)
+    for k in range(4):
+        a.alphas[k] = float(alphas[k]) if k < len(alphas) else 0.0
+    a.dt = float(dt)
+    a.time_dir = float(time_dir)
+    a.exact_nearest = int(exact_nearest) if not isinstance(exact_nearest, bool) else int(exact_nearest)
+    a.w_method = int(w_method)
+    a.step_i = int(step_i)
+    a.real_id = int(real_id)
+    if kick:
+        a.kick = 1
+        a.kh_x = float(kick["kh_x"])
+        a.kh_y = float(kick["kh_y"])
+        a.seed = int(kick["seed"])
+    if beach:
+        a.beach = 1
+        crit = set(beach["criteria"])
+        a.use_depth = int("depth" in crit)
+        a.use_coastal = int("coastal_element" in crit)
+        a.use_stochastic = int("stochastic" in crit)
+        a.on_failure = int("on_failure" in crit)
+        a.resuspension = RESUSPENSION_CODES[beach.get("resuspension", "never")]
+        a.min_wet_steps = int(beach.get("min_wet_steps", 1))
+        a.depth_min = float(beach.get("depth_min", 0.3))
+        zone = beach.get("zone_depth")
+        a.zone_depth = float(2.0 * a.depth_min if zone is None else zone)
+        a.t_beach = float(beach.get("t_beach") or 0.0)
+        a.rewet_hysteresis = float(beach.get("rewet_hysteresis", 0.1))
+        a.t_resuspend = float(beach.get("t_resuspend") or 0.0)
+        a.beach_dt = float(beach.get("dt", dt))
+        a.beach_seed = int(beach.get("seed", a.seed))
+    return a
 
 
 class DeviceContext:
@@ -99,6 +143,8 @@ class DeviceContext:
             off=np.ascontiguousarray(mesh.gh_offsets, np.int32),
             ids=np.ascontiguousarray(mesh.gh_node_ids, np.int32),
             z=np.ascontiguousarray(mesh.sigma, np.float64),
+            coastal=(None if mesh.elem_is_coastal is None
+                     else np.ascontiguousarray(mesh.elem_is_coastal, np.uint8)),
         )
         d = MeshDesc(
             mesh.n_nodes, mesh.n_elements, mesh.max_v, keep["n2e"].shape[1],
@@ -107,7 +153,8 @@ class DeviceContext:
             keep["off"].ctypes.data, keep["ids"].ctypes.data,
             mesh.gh_cell_size, mesh.gh_lon_min, mesh.gh_lat_min,
             mesh.gh_nx, mesh.gh_ny, int(mesh.gh_wrap_lon), mesh.gh_far_rows,
-            mesh.r_earth, int(keep["z"].size), keep["z"].ctypes.data if keep["z"].size else None)
+            mesh.r_earth, int(keep["z"].size), keep["z"].ctypes.data if keep["z"].size else None,
+            None if keep["coastal"] is None else keep["coastal"].ctypes.data)
         check(self.lib.kelp_mesh_upload(self.h, ctypes.byref(d)), "kelp_mesh_upload")
         self.mesh = mesh
         self._slot_keys.clear()
@@ -138,7 +185,8 @@ class DeviceContext:
         self._slot_keys[slot] = key
 
     # -- particles ----------------------------------------------------------
-    def particles(self, lon, lat, z=None, status=None, nn=None, fr=None, pid=None) -> DeviceParticles:
+    def particles(self, lon, lat, z=None, status=None, nn=None, fr=None, pid=None,
+                  ws=None) -> DeviceParticles:
         t = self.torch
         dev = f"cuda:{self.device}"
         P = len(lon)
@@ -159,21 +207,17 @@ class DeviceContext:
                  else t.as_tensor(np.ascontiguousarray(pid, np.int64), device=dev))
         return DeviceParticles(lon=f64(lon), lat=f64(lat), z=f64(z), status=i32(status, 1),
                                nn=i32(nn, -1), fr=fr_t, u=f64(None), v=f64(None), w=f64(None),
-                               pid=pid_t)
+                               pid=pid_t, ws=i32(ws, 0))
 
     # -- operations ---------------------------------------------------------
     def step(self, p: DeviceParticles, order: int, slot_a: int, dt: float, time_dir: float,
              slot_b: int = -1, slot_t: int = -1, alpha0: float = 0.0, alphas=(0.0, 0.0, 0.0, 0.0),
-             exact_nearest: bool = True) -> None:
-        a = StepArgs()
-        a.order = int(order)
-        a.slot_a, a.slot_b, a.slot_t = int(slot_a), int(slot_b), int(slot_t)
-        a.alpha0 = float(alpha0)
-        for k in range(4):
-            a.alphas[k] = float(alphas[k]) if k < len(alphas) else 0.0
-        a.dt = float(dt)
-        a.time_dir = float(time_dir)
-        a.exact_nearest = 1 if exact_nearest else 0
+             exact_nearest: bool = True, w_method: int = 2, kick=None, beach=None,
+             step_i: int = 0, real_id: int = 0) -> None:
+        """One step in place.  ``kick``: dict(kh_x, kh_y, seed) for the split
+        uniform-Kh random walk; ``beach``: a :class:`BeachArgs`-like dict."""
+        a = make_step_args(order, slot_a, dt, time_dir, slot_b, slot_t, alpha0, alphas,
+                           exact_nearest, w_method, kick, beach, step_i, real_id)
         s = p.as_struct()
         check(self.lib.kelp_step(self.h, ctypes.byref(s), ctypes.byref(a)), "kelp_step")
 

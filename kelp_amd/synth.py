@@ -75,3 +75,186 @@ def double_gyre_workload(n_particles=10_000, seed=C1_SEED) -> Workload2D:
     u, v = double_gyre(mesh)
     lon, lat = uniform_seeds(n_particles, 0.5, 39.5, 30.5, 59.5, seed + 1)
     return Workload2D(mesh, u, v, lon, lat, order=2, dt=600.0, n_steps=1000, output_every=10)
+
+
+# ---------------------------------------------------------------------------
+# 3-D global workload: Fibonacci-sphere Delaunay mesh with a land mask
+# ---------------------------------------------------------------------------
+
+C2_SEED = 2024
+R_EARTH = 6_371_000.0
+SNAP_DT = 21600.0           # six-hourly snapshots
+
+
+def fibonacci_sphere(n):
+    """Golden-angle spiral: n nodes quasi-uniform on the sphere (degrees)."""
+    i = np.arange(n, dtype=np.float64) + 0.5
+    lat = np.degrees(np.arcsin(1.0 - 2.0 * i / n))
+    lon = (np.degrees(i * np.pi * (3.0 - np.sqrt(5.0))) % 360.0) - 180.0
+    return lon, lat
+
+
+def unit_xyz(lon, lat):
+    lr, la = np.radians(lon), np.radians(lat)
+    return np.stack([np.cos(la) * np.cos(lr), np.cos(la) * np.sin(lr), np.sin(la)], axis=1)
+
+
+def harmonic_field(lon, lat, lmax=8, seed=C2_SEED):
+    """Smooth random field: real spherical harmonics up to degree ``lmax``
+    with N(0, 1)/(l+1) coefficients, scaled to unit standard deviation over
+    the points."""
+    from scipy.special import sph_harm_y
+    rng = np.random.default_rng(seed)
+    theta = np.radians(90.0 - np.asarray(lat))       # polar angle
+    phi = np.radians(np.asarray(lon))                # azimuth
+    f = np.zeros(theta.shape)
+    for l in range(1, lmax + 1):
+        for m in range(0, l + 1):
+            y = sph_harm_y(l, m, theta, phi)
+            f += rng.normal() / (l + 1) * y.real
+            if m > 0:
+                f += rng.normal() / (l + 1) * y.imag
+    return (f - f.mean()) / f.std()
+
+
+def sphere_mesh(n_nodes, land_threshold=0.7, seed=C2_SEED, z_levels=None, name="fibonacci"):
+    """Spherical Delaunay triangulation (convex hull) of ``n_nodes`` Fibonacci
+    nodes; nodes where the harmonic field exceeds ``land_threshold`` are land
+    and every triangle touching one is dropped (orphan nodes removed).  The
+    bottom depth deepens away from the coast, 50 m at the land edge to 6000 m
+    in the open ocean."""
+    from scipy.spatial import ConvexHull
+    lon, lat = fibonacci_sphere(n_nodes)
+    tri = ConvexHull(unit_xyz(lon, lat)).simplices.astype(np.int64)
+    f = harmonic_field(lon, lat, seed=seed)
+    wet = f <= land_threshold
+    tri = tri[wet[tri].all(axis=1)]
+    used = np.zeros(n_nodes, dtype=bool)
+    used[tri.ravel()] = True
+    remap = np.full(n_nodes, -1, dtype=np.int64)
+    remap[used] = np.arange(int(used.sum()))
+    tri = remap[tri]
+    # counter-clockwise seen from outside
+    xyz = unit_xyz(lon[used], lat[used])
+    a, b, c = xyz[tri[:, 0]], xyz[tri[:, 1]], xyz[tri[:, 2]]
+    flip = np.einsum("ij,ij->i", np.cross(b - a, c - a), a) < 0
+    tri[flip, 1], tri[flip, 2] = tri[flip, 2].copy(), tri[flip, 1].copy()
+    fw = f[used]
+    depth = 50.0 + 5950.0 * np.clip((land_threshold - fw) / 2.0, 0.0, 1.0) ** 0.7
+    if z_levels is None:
+        z_levels = np.concatenate([[0.0], np.geomspace(5.0, 6000.0, 47)])
+    mesh = make_mesh(name, lon[used], lat[used], tri.astype(np.int32), depth=depth,
+                     z_levels=z_levels)
+    from .mesh import build_boundary_info
+    build_boundary_info(mesh)
+    return mesh
+
+
+def jet_rossby_snapshot(mesh, t, dtype=np.float32, amp=0.3, H=800.0):
+    """u, v on the K-1 layers (mid-layer depth), W on the K interfaces
+    (positive down), T and S on the layers, at time ``t`` seconds.
+
+    Horizontal flow = g(z) * (zonal jet + Rossby waves from a streamfunction +
+    a small divergent part grad(chi)), g(z) = exp(-z/H).  chi is a degree-2
+    harmonic (Laplacian -6 chi / R^2), so continuity gives
+    W(z) = 6 chi H (1 - exp(-z/H)) / R^2 exactly."""
+    lam = np.radians(mesh.nodes_lon)[None, :]
+    phi = np.radians(mesh.nodes_lat)[None, :]
+    zl = mesh.sigma
+    zm = 0.5 * (zl[:-1] + zl[1:])[:, None]
+    g = np.exp(-zm / H)
+    cphi, sphi = np.cos(phi), np.sin(phi)
+    # zonal jets
+    latd = mesh.nodes_lat[None, :]
+    u = amp * (np.exp(-((latd - 35.0) / 8.0) ** 2) - 0.7 * np.exp(-((latd + 45.0) / 9.0) ** 2))
+    v = np.zeros_like(u)
+    # Rossby modes psi = A cos^2(phi) sin(2 phi) cos(m (lam - c t))
+    for m, A, cdeg in ((3, 0.45 * amp * R_EARTH, -4.0), (5, 0.25 * amp * R_EARTH, -2.5)):
+        ph = m * (lam - np.radians(cdeg) * t / 86400.0)
+        dpsi_dphi = A * np.cos(ph) * (-2.0 * cphi * sphi * np.sin(2 * phi) + 2.0 * cphi ** 2 * np.cos(2 * phi))
+        dpsi_dlam = -A * m * cphi ** 2 * np.sin(2 * phi) * np.sin(ph)
+        u = u - dpsi_dphi / R_EARTH
+        v = v + dpsi_dlam / (R_EARTH * np.maximum(cphi, 1e-6))
+    # divergent part chi = B cos^2(phi) cos(2 lam - w t)
+    B = 0.05 * amp * R_EARTH
+    ph2 = 2.0 * lam - 2.0 * np.pi * t / (5 * 86400.0)
+    chi = B * cphi ** 2 * np.cos(ph2)
+    u = u + (-2.0 * B * cphi * np.sin(ph2)) / R_EARTH          # (1/(R cos)) dchi/dlam
+    v = v + (-2.0 * B * cphi * sphi * np.cos(ph2)) / R_EARTH   # (1/R) dchi/dphi
+    U = (g * u).astype(dtype)
+    V = (g * v).astype(dtype)
+    W = (6.0 * chi * H * (1.0 - np.exp(-zl[:, None] / H)) / R_EARTH ** 2).astype(dtype)
+    T = (2.0 + 26.0 * np.exp(-zm / 700.0) * (0.6 + 0.4 * cphi)).astype(dtype)
+    S = (34.0 + 1.5 * (0.5 + 0.5 * sphi * np.exp(-zm / 1500.0))).astype(dtype)
+    return U, V, W, T, S
+
+
+def wet_area_seeds(mesh, n, zmax=1000.0, seed=C2_SEED + 1):
+    """``n`` seeds uniform over the wet area (area-weighted triangles,
+    uniform barycentric sampling, projected to the sphere) with depths
+    uniform in [0, zmax]."""
+    rng = np.random.default_rng(seed)
+    xyz = unit_xyz(mesh.nodes_lon, mesh.nodes_lat)
+    en = mesh.elem_nodes
+    a, b, c = xyz[en[:, 0]], xyz[en[:, 1]], xyz[en[:, 2]]
+    area = 0.5 * np.linalg.norm(np.cross(b - a, c - a), axis=1)
+    e = rng.choice(en.shape[0], size=n, p=area / area.sum())
+    r1, r2 = rng.random(n), rng.random(n)
+    s = np.sqrt(r1)
+    p = (1 - s)[:, None] * a[e] + (s * (1 - r2))[:, None] * b[e] + (s * r2)[:, None] * c[e]
+    p /= np.linalg.norm(p, axis=1, keepdims=True)
+    lon = np.degrees(np.arctan2(p[:, 1], p[:, 0]))
+    lat = np.degrees(np.arcsin(np.clip(p[:, 2], -1, 1)))
+    z = rng.uniform(0.0, zmax, n)
+    return lon, lat, z
+
+
+@dataclass
+class Workload3D:
+    mesh: Mesh
+    snaps: list                 # [(U, V, W, T, S)] per snapshot
+    snap_dt: float
+    lon: np.ndarray
+    lat: np.ndarray
+    z: np.ndarray
+    order: int
+    dt: float
+    n_steps: int
+    kh: float
+    output_every: int
+
+
+def global_3d_workload(n_nodes=1_000_000, n_particles=1_000_000, n_snaps=8,
+                       seed=C2_SEED) -> Workload3D:
+    """The 3-D global workload: 1M-node Fibonacci Delaunay mesh, 47 layers,
+    8 six-hourly float32 snapshots of u, v, w, T, S; RK4, dt 1800 s, 336
+    steps, time interpolation, Kh = 10 m^2/s random walk, beaching."""
+    mesh = sphere_mesh(n_nodes, seed=seed)
+    snaps = [jet_rossby_snapshot(mesh, s * SNAP_DT) for s in range(n_snaps)]
+    lon, lat, z = wet_area_seeds(mesh, n_particles, seed=seed + 1)
+    return Workload3D(mesh, snaps, SNAP_DT, lon, lat, z, order=4, dt=1800.0, n_steps=336,
+                      kh=10.0, output_every=12)
+
+
+def snapshot_bracket(t, snap_dt, n_snaps, direction=1.0):
+    """Bracket (idx_a, idx_b, alpha0, dt_to_next) of time ``t`` over cycled
+    snapshots (``time.on_exhausted: cycle``), as the payload of
+    tracking._build_vel_payload: alpha0 snaps to 0/1 within 1e-12 and the
+    stage target is b forward / a backward."""
+    k = int(np.floor(t / snap_dt))
+    ta = k * snap_dt
+    tb = ta + snap_dt
+    if direction < 0 and t == ta:
+        k -= 1
+        ta, tb = ta - snap_dt, ta
+    ia, ib = k % n_snaps, (k + 1) % n_snaps
+    alpha = (t - ta) / (tb - ta)
+    if direction > 0:
+        dt_to_next, it = tb - t, ib
+    else:
+        dt_to_next, it = t - ta, ia
+    if abs(alpha) < 1e-12:
+        return ia, -1, 0.0, it, dt_to_next
+    if abs(alpha - 1.0) < 1e-12:
+        return ib, -1, 0.0, it, dt_to_next
+    return ia, ib, alpha, it, dt_to_next

@@ -470,11 +470,21 @@ int or_nearest_exact(const or_mesh_t *m, double lon_p, double lat_p)
     }
 }
 
-/* integrators.py:1471 _fused_site_anchor */
+/* Anchor of one evaluation site.  exact = 1: the provably exact search
+ * (mesh.exact_nearest, every path); 0: the fused single-particle kernels'
+ * guard-off anchor, integrators.py:1471 _fused_site_anchor (widened window);
+ * 2: the batch guard-off anchor of core/mesh.py:1896 find_nearest_nodes
+ * (3x3 window, an empty window healed with the true nearest node -- the
+ * reference asks a KD-tree), used by the unfused paths, the kick and the
+ * fused 3-D kernels' rerun. */
 int or_site_anchor(const or_mesh_t *m, double lon_p, double lat_p, int exact)
 {
-    if (!exact) return or_nearest_widened(m, lon_p, lat_p);
-    return or_nearest_exact(m, lon_p, lat_p);
+    if (exact == 1) return or_nearest_exact(m, lon_p, lat_p);
+    if (exact == 2) {
+        int b = or_nearest_3x3(m, lon_p, lat_p);
+        return b >= 0 ? b : or_nearest_exact(m, lon_p, lat_p);
+    }
+    return or_nearest_widened(m, lon_p, lat_p);
 }
 
 /* ------------------------------------------------------------------------
@@ -762,7 +772,253 @@ int or_interp_bary_tinterp(const or_mesh_t *m, const or_fields_t *f, double xp, 
 }
 
 /* ------------------------------------------------------------------------
- * Fused 2-D steps (node-layout barycentric)
+ * Vertical (z-level)
+ * ---------------------------------------------------------------------- */
+
+/* numerics/vertical.py:40 find_layer */
+static int find_layer(const double *zl, int K, double zp, double bottom)
+{
+    if (K < 2) return 0;
+    if (zp <= zl[0]) return 0;
+    int max_layer = K - 2;
+    for (int k = 0; k < K - 1; k++) {
+        if (zl[k + 1] > bottom) {
+            max_layer = k - 1 > 0 ? k - 1 : 0;
+            break;
+        }
+    }
+    if (zp >= bottom || zp >= zl[max_layer + 1]) return max_layer;
+    for (int k = 0; k < max_layer + 1; k++)
+        if (zp < zl[k + 1]) return k;
+    return max_layer;
+}
+
+/* numerics/vertical.py:1043 _tblend_scalar */
+static inline double tblend(double a, double b, double t, double alpha0, double alpha)
+{
+    if (alpha < 1e-12) {
+        if (alpha0 == 0.0) return a;
+        return (1.0 - alpha0) * a + alpha0 * b;
+    }
+    if (alpha > 1.0 - 1e-12) return t;
+    if (alpha0 == 0.0) return (1.0 - alpha) * a + alpha * t;
+    double a0 = (1.0 - alpha0) * a + alpha0 * b;
+    return (1.0 - alpha) * a0 + alpha * t;
+}
+
+static inline double wlev(const or_fields_t *f, int64_t idx, double alpha)
+{
+    return tblend(fld(f->w_a, f->dtype, idx), fld(f->w_b, f->dtype, idx), fld(f->w_t, f->dtype, idx),
+                  f->alpha0, alpha);
+}
+
+/* numerics/vertical.py:1065 _interp_w_linear_single_tinterp (reduces to
+ * :223 _interp_w_linear_single for a plain field) */
+static double w_linear(const or_mesh_t *m, const or_fields_t *f, double alpha, double zp, int node,
+                       double time_dir)
+{
+    const double *zl = m->z_levels;
+    int K = m->n_levels;
+    int64_t N = m->n_nodes;
+    if (K < 2) return time_dir * wlev(f, node, alpha);
+    if (zp <= zl[0]) return time_dir * wlev(f, node, alpha);
+    if (zp >= zl[K - 1]) return time_dir * wlev(f, (int64_t)(K - 1) * N + node, alpha);
+    int k = K - 2;
+    for (int j = 0; j < K - 1; j++) {
+        if (zp < zl[j + 1]) {
+            k = j;
+            break;
+        }
+    }
+    double dz = zl[k + 1] - zl[k];
+    if (dz < 1e-12) return time_dir * wlev(f, (int64_t)k * N + node, alpha);
+    double a = (zp - zl[k]) / dz;
+    double w_lo = wlev(f, (int64_t)k * N + node, alpha);
+    double w_hi = wlev(f, (int64_t)(k + 1) * N + node, alpha);
+    return time_dir * ((1.0 - a) * w_lo + a * w_hi);
+}
+
+/* core/mesh.py:1974 _seam_safe_lon0 */
+static double seam_safe_lon0(const or_mesh_t *m, const int *verts, int n)
+{
+    double first = m->nodes_lon[verts[0]];
+    double lo = first, hi = first, s = 0.0;
+    for (int v = 0; v < n; v++) {
+        double x = m->nodes_lon[verts[v]];
+        s += x;
+        if (x < lo) lo = x;
+        if (x > hi) hi = x;
+    }
+    if (hi - lo > 180.0) {
+        double d = 0.0;
+        for (int v = 0; v < n; v++) d += PYMOD360(m->nodes_lon[verts[v]] - first + 180.0) - 180.0;
+        return first + d / n;
+    }
+    return s / n;
+}
+
+/* numerics/vertical.py:745 _interp_w_bary_single_tinterp (== :604
+ * _interp_w_barycentric_single on a plain field): the first fan
+ * sub-triangle of a 1-ring element that holds the point; bottom-aware
+ * linear W per vertex; fallback = linear at the anchor. */
+static double w_bary(const or_mesh_t *m, const or_fields_t *f, double alpha, double xp, double yp,
+                     double zp, int node, double time_dir)
+{
+    const double DEG2R = 0.017453292519943295;
+    int n_el = m->n2e_count[node];
+    if (n_el == 0) {
+        double zc = zp < m->depth[node] ? zp : m->depth[node];
+        return w_linear(m, f, alpha, zc, node, time_dir);
+    }
+    const int32_t *row = m->n2e + (int64_t)node * m->max_conn;
+    for (int k = 0; k < n_el; k++) {
+        int e = row[k];
+        if (e < 0) continue;
+        int verts[4], nv = 0;
+        for (int v = 0; v < m->max_v; v++) {
+            int nd = m->elem_nodes[(int64_t)e * m->max_v + v];
+            if (nd < 0) continue;
+            if (nv > 0 && nd == verts[0]) continue;
+            verts[nv++] = nd;
+        }
+        if (nv < 3) continue;
+        double lon0 = seam_safe_lon0(m, verts, nv);
+        double lat0 = 0.0;
+        for (int v = 0; v < nv; v++) lat0 += m->nodes_lat[verts[v]];
+        lat0 /= nv;
+        double cos_lat0 = M_COS(lat0 * DEG2R);
+        const double R = 6371000.0;
+        double vx[4], vy[4];
+        for (int v = 0; v < nv; v++) {
+            double dlon = PYMOD360(m->nodes_lon[verts[v]] - lon0 + 180.0) - 180.0;
+            double dlat = m->nodes_lat[verts[v]] - lat0;
+            vx[v] = dlon * DEG2R * R * cos_lat0;
+            vy[v] = dlat * DEG2R * R;
+        }
+        double dlon_p = PYMOD360(xp - lon0 + 180.0) - 180.0;
+        double dlat_p = yp - lat0;
+        double px = dlon_p * DEG2R * R * cos_lat0;
+        double py = dlat_p * DEG2R * R;
+        for (int t = 0; t < nv - 2; t++) {
+            int i1 = t + 1, i2 = t + 2;
+            double ax = vx[0], ay = vy[0], bx = vx[i1], by = vy[i1], cx = vx[i2], cy = vy[i2];
+            double denom = (by - cy) * (ax - cx) + (cx - bx) * (ay - cy);
+            if (fabs(denom) < 1e-14) continue;
+            double w1 = ((by - cy) * (px - cx) + (cx - bx) * (py - cy)) / denom;
+            double w2 = ((cy - ay) * (px - cx) + (ax - cx) * (py - cy)) / denom;
+            double w3 = 1.0 - w1 - w2;
+            const double eps = -1e-6;
+            if (w1 >= eps && w2 >= eps && w3 >= eps) {
+                if (nv == 4) {
+                    double wts[4];
+                    wachspress_quad(px, py, vx, vy, wts);
+                    double acc = 0.0;
+                    for (int vi = 0; vi < 4; vi++) {
+                        int nd = verts[vi];
+                        double zn = zp < m->depth[nd] ? zp : m->depth[nd];
+                        acc += wts[vi] * w_linear(m, f, alpha, zn, nd, time_dir);
+                    }
+                    return acc;
+                }
+                int n0 = verts[0], n1 = verts[i1], n2 = verts[i2];
+                double z0n = zp < m->depth[n0] ? zp : m->depth[n0];
+                double z1n = zp < m->depth[n1] ? zp : m->depth[n1];
+                double z2n = zp < m->depth[n2] ? zp : m->depth[n2];
+                double wv0 = w_linear(m, f, alpha, z0n, n0, time_dir);
+                double wv1 = w_linear(m, f, alpha, z1n, n1, time_dir);
+                double wv2 = w_linear(m, f, alpha, z2n, n2, time_dir);
+                return w1 * wv0 + w2 * wv1 + w3 * wv2;
+            }
+        }
+    }
+    double zc = zp < m->depth[node] ? zp : m->depth[node];
+    return w_linear(m, f, alpha, zc, node, time_dir);
+}
+
+/* ------------------------------------------------------------------------
+ * Counter RNG and kicks (numerics/diffusion.py)
+ * ---------------------------------------------------------------------- */
+
+/* diffusion.py:181 _splitmix64 */
+static inline uint64_t splitmix64(uint64_t x)
+{
+    x = x + 0x9E3779B97F4A7C15ULL;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+    x = x ^ (x >> 31);
+    return x;
+}
+
+/* diffusion.py:194 _philox_uint64 (a keyed SplitMix64 chain) */
+uint64_t or_hash5(uint64_t k0, uint64_t k1, uint64_t k2, uint64_t k3, uint64_t k4)
+{
+    uint64_t h = k0;
+    h = splitmix64(h ^ k1);
+    h = splitmix64(h ^ k2);
+    h = splitmix64(h ^ k3);
+    h = splitmix64(h ^ k4);
+    return h;
+}
+
+/* diffusion.py:219 + :234 _uint64_to_two_uniforms / _normal_pair */
+void or_normal_pair(uint64_t seed, uint64_t real_id, uint64_t pid, uint64_t step, uint64_t comp,
+                    double *g1, double *g2)
+{
+    const double INV_2_32 = 1.0 / 4294967296.0;
+    const double HALF_INV_2_32 = 0.5 * INV_2_32;
+    const double TWO_PI = 6.283185307179586; /* 2.0 * math.pi */
+    uint64_t h = or_hash5(seed, real_id, pid, step, comp);
+    double high = (double)((h >> 32) & 0xFFFFFFFFULL);
+    double low = (double)(h & 0xFFFFFFFFULL);
+    double u1 = high * INV_2_32 + HALF_INV_2_32;
+    double u2 = low * INV_2_32 + HALF_INV_2_32;
+    double r = sqrt(-2.0 * M_LOG(u1));
+    *g1 = r * M_COS(TWO_PI * u2);
+    *g2 = r * M_SIN(TWO_PI * u2);
+}
+
+/* beaching.py:46 _u01 */
+double or_u01(uint64_t seed, uint64_t real_id, uint64_t pid, uint64_t step, uint64_t comp)
+{
+    uint64_t h = or_hash5(seed, real_id, pid, step, comp);
+    return (double)(h >> 11) * (1.0 / 9007199254740992.0);
+}
+
+#define R_METRIC (111120.0 / 0.017453292519943295) /* diffusion.py:532 */
+
+/* diffusion.py:539 sphere_displace */
+static void sphere_displace(double lon, double lat, double dx_m, double dy_m, double *lon_o, double *lat_o)
+{
+    double dist = sqrt(dx_m * dx_m + dy_m * dy_m);
+    if (dist == 0.0) {
+        *lon_o = lon;
+        *lat_o = lat;
+        return;
+    }
+    double lam = lon * DEG2RAD, phi = lat * DEG2RAD;
+    double cl = M_COS(lam), sl = M_SIN(lam), cp = M_COS(phi), sp = M_SIN(phi);
+    double delta = dist / R_METRIC;
+    double c = M_COS(delta);
+    double s = M_SIN(delta) / dist;
+    double tx = -sl * dx_m - sp * cl * dy_m;
+    double ty = cl * dx_m - sp * sl * dy_m;
+    double tz = cp * dy_m;
+    double x2 = cp * cl * c + tx * s;
+    double y2 = cp * sl * c + ty * s;
+    double z2 = sp * c + tz * s;
+    if (z2 > 1.0)
+        z2 = 1.0;
+    else if (z2 < -1.0)
+        z2 = -1.0;
+    double lat2 = M_ASIN(z2) / DEG2RAD;
+    double lon2 = M_ATAN2(y2, x2) / DEG2RAD;
+    lon2 = PYMOD360(lon2 + 180.0) - 180.0;
+    *lon_o = lon2;
+    *lat_o = lat2;
+}
+
+/* ------------------------------------------------------------------------
+ * The step
  * ---------------------------------------------------------------------- */
 
 static inline double wrap_lon(double x) { return PYMOD360(x + 180.0) - 180.0; }
@@ -781,125 +1037,271 @@ static inline double z0_of(double lat)
     return z0;
 }
 
-/* One particle of integrators.py:2522 _fused_rk2_step_2d_bary_tinterp,
- * :2642 _fused_rk4_step_2d_bary_tinterp and the Euler restatement of
- * _euler_step_unfused (:4997) for barycentric node fields (stage alpha 0).
- * order: 1 Euler, 2 RK2, 4 RK4.  alphas: per-stage blend fractions. */
-static void step2d_one(const or_mesh_t *m, const or_fields_t *f, int order, const double *alphas,
-                       double dt, double time_dir, int exact,
-                       double lon0, double lat0, int32_t nn_prev,
-                       double *lon_o, double *lat_o, int32_t *st_o, int32_t *nn_o,
-                       double *U_o, double *V_o, int8_t *fr_o)
+/* u, v (and W in 3-D) at one evaluation site: the anchor, the particle's
+ * layer at that anchor (vertical.py:40), the layer-sliced barycentric u/v
+ * (interpolate.py:1444) and W by the configured method (integrators.py:1260
+ * _compute_w).  Returns ok of the u/v interpolation. */
+static int eval_site(const or_mesh_t *m, const or_fields_t *f, const or_step_args_t *a, int stage,
+                     double lon, double lat, double zp, int *nn_o, double *U, double *V, double *W,
+                     int want_w)
 {
-    int node_i = or_site_anchor(m, lon0, lat0, exact);
-    (void)nn_prev;
-    double U1, V1;
-    int ok1 = or_interp_bary_tinterp(m, f, lon0, lat0, node_i, time_dir, -1, alphas[0], &U1, &V1, NULL);
-    *U_o = U1;
-    *V_o = V1;
-    if (!ok1) {
-        *lon_o = lon0;
-        *lat_o = lat0;
-        *st_o = ST_OUTSIDE;
-        *fr_o = FR_INTERP;
-        *nn_o = node_i;
-        return;
+    int nn = or_site_anchor(m, lon, lat, a->exact);
+    *nn_o = nn;
+    int layer = -1;
+    if (a->is3d) layer = find_layer(m->z_levels, m->n_levels, zp, m->depth[nn]);
+    int ok = or_interp_bary_tinterp(m, f, lon, lat, nn, a->time_dir, layer, a->alphas[stage], U, V, NULL);
+    *W = 0.0;
+    if (a->is3d && want_w) {
+        if (a->w_method == 0)
+            *W = w_linear(m, f, a->alphas[stage], zp, nn, a->time_dir);
+        else
+            *W = w_bary(m, f, a->alphas[stage], lon, lat, zp, nn, a->time_dir);
     }
-    double lon_new, lat_new;
-    if (order == 1) {
-        lon_new = lon0 + dt * U1 / z0_of(lat0);
-        lat_new = lat0 + dt * V1 / SLM;
-        lon_new = wrap_lon(lon_new);
-        lat_new = clamp_lat(lat_new);
-    } else if (order == 2) {
-        double z0 = z0_of(lat0);
-        double lon_mid = lon0 + 0.5 * dt * U1 / z0;
-        double lat_mid = lat0 + 0.5 * dt * V1 / SLM;
-        lon_mid = wrap_lon(lon_mid);
-        lat_mid = clamp_lat(lat_mid);
-        int nn_mid = or_site_anchor(m, lon_mid, lat_mid, exact);
-        double U2, V2;
-        int ok2 = or_interp_bary_tinterp(m, f, lon_mid, lat_mid, nn_mid, time_dir, -1, alphas[1], &U2, &V2, NULL);
-        if (!ok2) {
-            *lon_o = lon0;
-            *lat_o = lat0;
-            *st_o = ST_OUTSIDE;
-            *fr_o = FR_INTERP;
-            *nn_o = node_i;
-            return;
-        }
-        double z0m = z0_of(lat_mid);
-        lon_new = lon0 + dt * U2 / z0m;
-        lat_new = lat0 + dt * V2 / SLM;
-        lon_new = wrap_lon(lon_new);
-        lat_new = clamp_lat(lat_new);
-    } else {
-        double dlon1 = dt * U1 / z0_of(lat0);
-        double dlat1 = dt * V1 / SLM;
-        double lon2 = wrap_lon(lon0 + 0.5 * dlon1);
-        double lat2 = clamp_lat(lat0 + 0.5 * dlat1);
-        int nn2 = or_site_anchor(m, lon2, lat2, exact);
-        double U2, V2;
-        if (!or_interp_bary_tinterp(m, f, lon2, lat2, nn2, time_dir, -1, alphas[1], &U2, &V2, NULL))
-            goto interp_fail;
-        double dlon2 = dt * U2 / z0_of(lat2);
-        double dlat2 = dt * V2 / SLM;
-        double lon3 = wrap_lon(lon0 + 0.5 * dlon2);
-        double lat3 = clamp_lat(lat0 + 0.5 * dlat2);
-        int nn3 = or_site_anchor(m, lon3, lat3, exact);
-        double U3, V3;
-        if (!or_interp_bary_tinterp(m, f, lon3, lat3, nn3, time_dir, -1, alphas[2], &U3, &V3, NULL))
-            goto interp_fail;
-        double dlon3 = dt * U3 / z0_of(lat3);
-        double dlat3 = dt * V3 / SLM;
-        double lon4 = wrap_lon(lon0 + dlon3);
-        double lat4 = clamp_lat(lat0 + dlat3);
-        int nn4 = or_site_anchor(m, lon4, lat4, exact);
-        double U4, V4;
-        if (!or_interp_bary_tinterp(m, f, lon4, lat4, nn4, time_dir, -1, alphas[3], &U4, &V4, NULL))
-            goto interp_fail;
-        double dlon4 = dt * U4 / z0_of(lat4);
-        double dlat4 = dt * V4 / SLM;
-        double dlon = (dlon1 + 2.0 * dlon2 + 2.0 * dlon3 + dlon4) / 6.0;
-        double dlat = (dlat1 + 2.0 * dlat2 + 2.0 * dlat3 + dlat4) / 6.0;
-        lon_new = wrap_lon(lon0 + dlon);
-        lat_new = clamp_lat(lat0 + dlat);
-    }
-    {
-        int nn_new = or_site_anchor(m, lon_new, lat_new, exact);
-        int e = or_find_containing_local(m, lon_new, lat_new, nn_new);
-        if (e < 0) {
-            *lon_o = lon0;
-            *lat_o = lat0;
-            *st_o = ST_OUTSIDE;
-            *fr_o = FR_OOB;
-            *nn_o = node_i;
-        } else {
-            *lon_o = lon_new;
-            *lat_o = lat_new;
-            *st_o = ST_ALIVE;
-            *nn_o = nn_new;
-        }
-    }
-    return;
-interp_fail:
-    *lon_o = lon0;
-    *lat_o = lat0;
-    *st_o = ST_OUTSIDE;
-    *fr_o = FR_INTERP;
-    *nn_o = node_i;
+    return ok;
 }
 
-/* One fused 2-D step over all particles.  Dead particles (status != ALIVE)
- * keep position/status/nn; their U, V are 0 and fr 0, as the runner's
- * output allocation (integrators.py:3393-3400).  fr_out holds this step's
- * kernel failure codes only (the caller merges them stickily). */
+static inline double clamp_z(const or_mesh_t *m, double z, int nn)
+{
+    if (z < m->z_levels[0]) z = m->z_levels[0];
+    double bottom = m->depth[nn];
+    if (z > bottom) z = bottom;
+    return z;
+}
+
+/* The deterministic step of one alive particle.  Euler follows
+ * integrators.py:4997 _euler_step_unfused, RK2 :2522/:3648 (the fused
+ * 2-D/3-D barycentric kernels, bit-identical to :5380), RK4 :2642 (2-D) and
+ * :5760 _rk4_step_unfused (3-D).  In place; returns 0 alive, else the
+ * failure code. */
+static int det_step(const or_mesh_t *m, const or_fields_t *f, const or_step_args_t *a,
+                    double *lon, double *lat, double *z, int32_t *st, int32_t *nn, int8_t *fr,
+                    double *Uo, double *Vo, double *Wo)
+{
+    const double dt = a->dt;
+    const double lon0 = *lon, lat0 = *lat, zz0 = *z;
+    int node_i;
+    double U1, V1, W1;
+    int ok = eval_site(m, f, a, 0, lon0, lat0, zz0, &node_i, &U1, &V1, &W1, 1);
+    *Uo = U1;
+    *Vo = V1;
+    *Wo = W1;
+    double lon_new = lon0, lat_new = lat0, z_new = zz0;
+    if (ok) {
+        if (a->order == 1) {
+            lon_new = wrap_lon(lon0 + dt * U1 / z0_of(lat0));
+            lat_new = clamp_lat(lat0 + dt * V1 / SLM);
+            z_new = zz0 + W1 * dt;
+        } else if (a->order == 2) {
+            double z0 = z0_of(lat0);
+            double lon_mid = wrap_lon(lon0 + 0.5 * dt * U1 / z0);
+            double lat_mid = clamp_lat(lat0 + 0.5 * dt * V1 / SLM);
+            double z_mid = zz0 + W1 * 0.5 * dt;
+            int nn_mid;
+            double U2, V2, W2;
+            ok = eval_site(m, f, a, 1, lon_mid, lat_mid, z_mid, &nn_mid, &U2, &V2, &W2, 1);
+            if (ok) {
+                double z0m = z0_of(lat_mid);
+                lon_new = wrap_lon(lon0 + dt * U2 / z0m);
+                lat_new = clamp_lat(lat0 + dt * V2 / SLM);
+                z_new = zz0 + W2 * dt;
+            }
+        } else {
+            double dlon1 = dt * U1 / z0_of(lat0);
+            double dlat1 = dt * V1 / SLM;
+            double dz1 = dt * W1;
+            double lon2 = wrap_lon(lon0 + 0.5 * dlon1);
+            double lat2 = clamp_lat(lat0 + 0.5 * dlat1);
+            double z2 = zz0 + 0.5 * dz1;
+            int nn2, nn3, nn4;
+            double U2, V2, W2, U3, V3, W3, U4, V4, W4;
+            ok = eval_site(m, f, a, 1, lon2, lat2, z2, &nn2, &U2, &V2, &W2, 1);
+            if (ok) {
+                double dlon2 = dt * U2 / z0_of(lat2);
+                double dlat2 = dt * V2 / SLM;
+                double dz2 = dt * W2;
+                double lon3 = wrap_lon(lon0 + 0.5 * dlon2);
+                double lat3 = clamp_lat(lat0 + 0.5 * dlat2);
+                double z3 = zz0 + 0.5 * dz2;
+                ok = eval_site(m, f, a, 2, lon3, lat3, z3, &nn3, &U3, &V3, &W3, 1);
+                if (ok) {
+                    double dlon3 = dt * U3 / z0_of(lat3);
+                    double dlat3 = dt * V3 / SLM;
+                    double dz3 = dt * W3;
+                    double lon4 = wrap_lon(lon0 + dlon3);
+                    double lat4 = clamp_lat(lat0 + dlat3);
+                    double z4 = zz0 + dz3;
+                    ok = eval_site(m, f, a, 3, lon4, lat4, z4, &nn4, &U4, &V4, &W4, 1);
+                    if (ok) {
+                        double dlon4 = dt * U4 / z0_of(lat4);
+                        double dlat4 = dt * V4 / SLM;
+                        double dz4 = dt * W4;
+                        double dlon = (dlon1 + 2.0 * dlon2 + 2.0 * dlon3 + dlon4) / 6.0;
+                        double dlat = (dlat1 + 2.0 * dlat2 + 2.0 * dlat3 + dlat4) / 6.0;
+                        double dz = (dz1 + 2.0 * dz2 + 2.0 * dz3 + dz4) / 6.0;
+                        lon_new = wrap_lon(lon0 + dlon);
+                        lat_new = clamp_lat(lat0 + dlat);
+                        z_new = zz0 + dz;
+                    }
+                }
+            }
+        }
+    }
+    if (!ok) {
+        *st = ST_OUTSIDE;
+        *fr = FR_INTERP;
+        *nn = node_i;
+        return FR_INTERP;
+    }
+    int nn_new = or_site_anchor(m, lon_new, lat_new, a->exact);
+    int e = or_find_containing_local(m, lon_new, lat_new, nn_new);
+    if (e < 0) {
+        *st = ST_OUTSIDE;
+        *fr = FR_OOB;
+        *nn = node_i;
+        return FR_OOB;
+    }
+    *lon = lon_new;
+    *lat = lat_new;
+    if (a->is3d) *z = clamp_z(m, z_new, nn_new);
+    *nn = nn_new;
+    return 0;
+}
+
+/* integrators.py:673 _apply_split_diffusion_kick for one particle alive
+ * after the deterministic step; pre_* is the start-of-step state. */
+static void kick_one(const or_mesh_t *m, const or_step_args_t *a, uint64_t pid,
+                     double pre_lon, double pre_lat, double pre_z, int32_t pre_nn,
+                     double *lon, double *lat, double *z, int32_t *st, int32_t *nn, int8_t *fr)
+{
+    double g1, g2;
+    or_normal_pair(a->seed, a->real_id, pid, a->step_i, 0, &g1, &g2);
+    double abs_dt = a->dt >= 0.0 ? a->dt : -a->dt;
+    double sqrt_abs_dt = sqrt(abs_dt);
+    double sigma_x = sqrt(2.0 * a->kh_x);
+    double sigma_y = sqrt(2.0 * a->kh_y);
+    double dx = sigma_x * sqrt_abs_dt * g1;
+    double dy = sigma_y * sqrt_abs_dt * g2;
+    if (a->kh_x != a->kh_y) {
+        double la = *lat;
+        if (la < -89.9) la = -89.9;
+        if (la > 89.9) la = 89.9;
+        double corr = (a->kh_x - a->kh_y) * (M_SIN(la * DEG2RAD) / M_COS(la * DEG2RAD)) / R_METRIC * fabs(a->dt);
+        dy = dy + corr;
+    }
+    double lon_k, lat_k;
+    sphere_displace(*lon, *lat, dx, dy, &lon_k, &lat_k);
+    int nn_k = or_site_anchor(m, lon_k, lat_k, a->exact == 1 ? 1 : 2); /* find_nearest_nodes */
+    int e = or_find_containing_local(m, lon_k, lat_k, nn_k);
+    if (e < 0) {
+        *lon = pre_lon;
+        *lat = pre_lat;
+        *z = pre_z;
+        *nn = pre_nn;
+        *st = ST_OUTSIDE;
+        *fr = FR_OOB;
+        return;
+    }
+    *lon = lon_k;
+    *lat = lat_k;
+    *nn = nn_k;
+    if (a->is3d) *z = clamp_z(m, *z, nn_k);
+}
+
+/* tracking.py:199 _apply_beaching_hook + beaching.py:95 evaluate_beaching
+ * for one particle (prev = status at the step start). */
+static void beach_one(const or_mesh_t *m, const or_step_args_t *a, uint64_t pid, int32_t prev,
+                      double lon, double lat, int32_t *st, int32_t *nn, int8_t *fr, int32_t *ws)
+{
+    const int ALIVE = 1, BEACHED = 2, OUTSIDE = 3;
+    int need_topo = a->use_coastal || a->use_stochastic;
+    int newly_failed = (*st == OUTSIDE) && (prev == ALIVE);
+    if (newly_failed && *nn < 0) *nn = or_site_anchor(m, lon, lat, a->exact == 1 ? 1 : 2);
+    int n = *nn;
+    int containing = -1;
+    if (need_topo && (*st == ALIVE || *st == BEACHED || newly_failed) && n >= 0)
+        containing = or_find_containing_local(m, lon, lat, n);
+    int coastal = need_topo && containing >= 0 && m->elem_coastal && m->elem_coastal[containing];
+    double h_raw = n >= 0 ? m->depth[n] : INFINITY;
+    if (a->on_failure && newly_failed) {
+        int context = h_raw < a->zone_depth || coastal;
+        if (context) {
+            *st = BEACHED;
+            *fr = 0;
+        }
+    }
+    int alive = *st == ALIVE, beached = *st == BEACHED;
+    int to_beach = 0;
+    if (a->use_depth) to_beach |= h_raw < a->depth_min;
+    if (a->use_coastal) to_beach |= coastal;
+    if (a->use_stochastic && a->t_beach > 0) {
+        int in_zone = h_raw < a->zone_depth || coastal;
+        double p = 1.0 - M_EXP(-fabs(a->beach_dt) / a->t_beach);
+        if (alive && in_zone && or_u01(a->beach_seed, a->real_id, pid, a->step_i, 10) < p) to_beach = 1;
+    }
+    to_beach &= alive;
+    int to_release = 0;
+    int w = *ws;
+    if (a->resuspension != 0 && beached) {
+        int rewet = h_raw > (a->depth_min + a->rewet_hysteresis) && n >= 0;
+        if (rewet)
+            w += 1;
+        else
+            w = 0;
+        if (a->resuspension == 1 || a->resuspension == 3) to_release |= w >= a->min_wet_steps;
+        if ((a->resuspension == 2 || a->resuspension == 3) && a->t_resuspend > 0) {
+            double p_r = 1.0 - M_EXP(-fabs(a->beach_dt) / a->t_resuspend);
+            if (rewet && or_u01(a->beach_seed, a->real_id, pid, a->step_i, 11) < p_r) to_release = 1;
+        }
+    }
+    if (to_release) w = 0;
+    if (!beached) w = 0;
+    *ws = w;
+    if (to_beach) *st = BEACHED;
+    if (to_release) {
+        *st = ALIVE;
+        *fr = 0;
+    }
+}
+
+/* One full step over all particles, in place: deterministic step (dead
+ * particles keep everything and get u = v = w = 0), split Kh kick, beaching
+ * hook.  fr is the sticky failure reason; pid the RNG key; ws the
+ * beaching wet streak. */
+void or_step(const or_mesh_t *m, const or_fields_t *f, const or_step_args_t *a, int64_t P,
+             double *lon, double *lat, double *z, int32_t *status, int32_t *nn, int8_t *fr,
+             double *U, double *V, double *W, const int64_t *pid, int32_t *ws)
+{
+    for (int64_t i = 0; i < P; i++) {
+        int32_t prev = status[i];
+        double pre_lon = lon[i], pre_lat = lat[i], pre_z = z[i];
+        int32_t pre_nn = nn[i];
+        U[i] = 0.0;
+        V[i] = 0.0;
+        W[i] = 0.0;
+        if (prev == ST_ALIVE) {
+            det_step(m, f, a, &lon[i], &lat[i], &z[i], &status[i], &nn[i], &fr[i], &U[i], &V[i], &W[i]);
+            if (a->kick && status[i] == ST_ALIVE && !(a->kh_x == 0.0 && a->kh_y == 0.0))
+                kick_one(m, a, (uint64_t)pid[i], pre_lon, pre_lat, pre_z, pre_nn,
+                         &lon[i], &lat[i], &z[i], &status[i], &nn[i], &fr[i]);
+        }
+        if (a->beach) beach_one(m, a, (uint64_t)pid[i], prev, lon[i], lat[i], &status[i], &nn[i], &fr[i], &ws[i]);
+    }
+}
+
+/* 2-D convenience form used by the first tests (no kick, no beaching). */
 void or_step2d(const or_mesh_t *m, const or_fields_t *f, int order, const double *alphas,
                double dt, double time_dir, int exact, int64_t P,
                const double *lon, const double *lat, const int32_t *status, const int32_t *nn,
                double *lon_out, double *lat_out, int32_t *status_out, int32_t *nn_out,
                double *U_out, double *V_out, int8_t *fr_out)
 {
+    or_step_args_t a;
+    memset(&a, 0, sizeof a);
+    a.order = order;
+    a.exact = exact;
+    a.dt = dt;
+    a.time_dir = time_dir;
+    for (int k = 0; k < 4; k++) a.alphas[k] = alphas[k];
     for (int64_t i = 0; i < P; i++) {
         lon_out[i] = lon[i];
         lat_out[i] = lat[i];
@@ -909,8 +1311,9 @@ void or_step2d(const or_mesh_t *m, const or_fields_t *f, int order, const double
         V_out[i] = 0.0;
         fr_out[i] = 0;
         if (status[i] != ST_ALIVE) continue;
-        step2d_one(m, f, order, alphas, dt, time_dir, exact, lon[i], lat[i], nn[i],
-                   &lon_out[i], &lat_out[i], &status_out[i], &nn_out[i], &U_out[i], &V_out[i], &fr_out[i]);
+        double zz = 0.0, ww;
+        det_step(m, f, &a, &lon_out[i], &lat_out[i], &zz, &status_out[i], &nn_out[i], &fr_out[i],
+                 &U_out[i], &V_out[i], &ww);
     }
 }
 

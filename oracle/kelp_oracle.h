@@ -36,6 +36,7 @@ typedef struct {
     /* vertical (3-D): interfaces z_levels[K], positive down */
     int32_t n_levels;
     const double *z_levels;
+    const uint8_t *elem_coastal;  /* (E,) coastal-element flags or NULL */
 } or_mesh_t;
 
 /* Snapshot triple of one step: bracket pair (a, b) blended by alpha0 and
@@ -47,5 +48,27 @@ typedef struct {
     int32_t dtype;
     double alpha0;
 } or_fields_t;
+
+/* One step's configuration (the keyword arguments of step_fn,
+ * tracking.py:2643-2655, plus the beaching hook's per-run constants). */
+typedef struct {
+    int32_t order;     /* 1 Euler, 2 RK2, 4 RK4 */
+    int32_t is3d;
+    int32_t w_method;  /* 0 nearest (linear at the anchor), 2 barycentric */
+    int32_t exact;     /* mesh.exact_nearest */
+    double dt, time_dir;
+    double alphas[4];  /* stage fractions towards the target snapshot */
+    /* split uniform-Kh kick (integrators.py:673) */
+    int32_t kick;
+    int32_t beach;
+    double kh_x, kh_y;
+    uint64_t seed, real_id, step_i;
+    /* beaching (numerics/beaching.py:95, tracking.py:199) */
+    int32_t use_depth, use_coastal, use_stochastic, on_failure;
+    int32_t resuspension, min_wet_steps;
+    double depth_min, zone_depth, t_beach, rewet_hysteresis, t_resuspend;
+    uint64_t beach_seed;
+    double beach_dt;
+} or_step_args_t;
 
 #endif

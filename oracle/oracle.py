@@ -32,8 +32,56 @@ class OrMesh(ctypes.Structure):
         ("gh_cell", c_double), ("gh_lon_min", c_double), ("gh_lat_min", c_double),
         ("gh_nx", c_int32), ("gh_ny", c_int32), ("gh_wrap", c_int32), ("gh_far_rows", c_int32),
         ("r_earth", c_double),
-        ("n_levels", c_int32), ("z_levels", c_void_p),
+        ("n_levels", c_int32), ("z_levels", c_void_p), ("elem_coastal", c_void_p),
     ]
+
+
+class OrStepArgs(ctypes.Structure):
+    _fields_ = [
+        ("order", c_int32), ("is3d", c_int32), ("w_method", c_int32), ("exact", c_int32),
+        ("dt", c_double), ("time_dir", c_double), ("alphas", c_double * 4),
+        ("kick", c_int32), ("beach", c_int32), ("kh_x", c_double), ("kh_y", c_double),
+        ("seed", ctypes.c_uint64), ("real_id", ctypes.c_uint64), ("step_i", ctypes.c_uint64),
+        ("use_depth", c_int32), ("use_coastal", c_int32), ("use_stochastic", c_int32),
+        ("on_failure", c_int32), ("resuspension", c_int32), ("min_wet_steps", c_int32),
+        ("depth_min", c_double), ("zone_depth", c_double), ("t_beach", c_double),
+        ("rewet_hysteresis", c_double), ("t_resuspend", c_double),
+        ("beach_seed", ctypes.c_uint64), ("beach_dt", c_double),
+    ]
+
+
+RESUSPENSION_CODES = {"never": 0, "auto": 1, "stochastic": 2, "auto_or_stochastic": 3}
+
+
+def step_args(order, dt, time_dir, is3d=False, w_method=2, exact=True, alphas=(0, 0, 0, 0),
+              kick=None, beach=None, step_i=0, real_id=0) -> OrStepArgs:
+    a = OrStepArgs()
+    a.order, a.is3d, a.w_method, a.exact = int(order), int(is3d), int(w_method), int(exact)
+    a.dt, a.time_dir = float(dt), float(time_dir)
+    for k in range(4):
+        a.alphas[k] = float(alphas[k]) if k < len(alphas) else 0.0
+    a.step_i, a.real_id = int(step_i), int(real_id)
+    if kick:
+        a.kick = 1
+        a.kh_x, a.kh_y, a.seed = float(kick["kh_x"]), float(kick["kh_y"]), int(kick["seed"])
+    if beach:
+        a.beach = 1
+        crit = set(beach["criteria"])
+        a.use_depth = int("depth" in crit)
+        a.use_coastal = int("coastal_element" in crit)
+        a.use_stochastic = int("stochastic" in crit)
+        a.on_failure = int("on_failure" in crit)
+        a.resuspension = RESUSPENSION_CODES[beach.get("resuspension", "never")]
+        a.min_wet_steps = int(beach.get("min_wet_steps", 1))
+        a.depth_min = float(beach.get("depth_min", 0.3))
+        zone = beach.get("zone_depth")
+        a.zone_depth = float(2.0 * a.depth_min if zone is None else zone)
+        a.t_beach = float(beach.get("t_beach") or 0.0)
+        a.rewet_hysteresis = float(beach.get("rewet_hysteresis", 0.1))
+        a.t_resuspend = float(beach.get("t_resuspend") or 0.0)
+        a.beach_dt = float(beach.get("dt", dt))
+        a.beach_seed = int(beach.get("seed", a.seed))
+    return a
 
 
 class OrFields(ctypes.Structure):
@@ -59,6 +107,15 @@ def lib(flavour: str = "glibc") -> ctypes.CDLL:
     L.or_step2d.argtypes = [POINTER(OrMesh), POINTER(OrFields), c_int, vp, c_double, c_double,
                             c_int, c_int64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
     L.or_step2d.restype = None
+    L.or_step.argtypes = [POINTER(OrMesh), POINTER(OrFields), POINTER(OrStepArgs), c_int64,
+                          vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.or_step.restype = None
+    L.or_hash5.argtypes = [ctypes.c_uint64] * 5
+    L.or_hash5.restype = ctypes.c_uint64
+    L.or_u01.argtypes = [ctypes.c_uint64] * 5
+    L.or_u01.restype = c_double
+    L.or_normal_pair.argtypes = [ctypes.c_uint64] * 5 + [POINTER(c_double), POINTER(c_double)]
+    L.or_normal_pair.restype = None
     L.or_nearest_batch.argtypes = [POINTER(OrMesh), c_int64, vp, vp, c_int, vp]
     L.or_contain_batch.argtypes = [POINTER(OrMesh), c_int64, vp, vp, vp, vp]
     L.or_node_xyz.argtypes = [c_int64, vp, vp, vp]
@@ -89,7 +146,9 @@ class OracleMesh:
             depth=_c(mesh.depth, np.float64), en=_c(mesh.elem_nodes, np.int32),
             n2e=_c(mesh.node_to_elems, np.int32), cnt=_c(mesh.node_to_elems_count, np.int32),
             off=_c(mesh.gh_offsets, np.int32), ids=_c(mesh.gh_node_ids, np.int32),
-            z=_c(mesh.sigma if mesh.sigma.size else np.zeros(1), np.float64))
+            z=_c(mesh.sigma if mesh.sigma.size else np.zeros(1), np.float64),
+            coastal=(None if getattr(mesh, "elem_is_coastal", None) is None
+                     else _c(mesh.elem_is_coastal, np.uint8)))
         xyz = np.empty((mesh.n_nodes, 3), dtype=np.float64)
         self.L.or_node_xyz(mesh.n_nodes, k["lon"].ctypes.data, k["lat"].ctypes.data, xyz.ctypes.data)
         k["xyz"] = xyz
@@ -101,7 +160,8 @@ class OracleMesh:
             xyz.ctypes.data, k["off"].ctypes.data, k["ids"].ctypes.data,
             mesh.gh_cell_size, mesh.gh_lon_min, mesh.gh_lat_min,
             mesh.gh_nx, mesh.gh_ny, int(mesh.gh_wrap_lon), mesh.gh_far_rows,
-            mesh.r_earth, int(mesh.sigma.size), k["z"].ctypes.data)
+            mesh.r_earth, int(mesh.sigma.size), k["z"].ctypes.data,
+            None if k["coastal"] is None else k["coastal"].ctypes.data)
 
     def nearest(self, lon, lat, exact=True) -> np.ndarray:
         lon = _c(lon, np.float64)
@@ -152,6 +212,39 @@ class OracleMesh:
                          out["lon"].ctypes.data, out["lat"].ctypes.data, out["status"].ctypes.data,
                          out["nn"].ctypes.data, out["u"].ctypes.data, out["v"].ctypes.data,
                          out["fr"].ctypes.data)
+        return out
+
+
+    def step(self, state: dict, fields: dict, args: OrStepArgs) -> dict:
+        """One full step (deterministic + kick + beaching) on copies of the
+        ``state`` arrays (lon, lat, z, status, nn, fr, pid, ws).  ``fields``:
+        u_a, v_a[, w_a] and optionally the b/t snapshots and alpha0."""
+        ua = np.asarray(fields["u_a"])
+        dt_code = 1 if ua.dtype == np.float32 else 0
+        npd = np.float32 if dt_code else np.float64
+        keep = {}
+        for nm in ("u", "v", "w"):
+            a = fields.get(nm + "_a")
+            if a is None:
+                continue
+            keep[nm + "_a"] = _c(a, npd)
+            for sfx in ("b", "t"):
+                x = fields.get(f"{nm}_{sfx}")
+                keep[f"{nm}_{sfx}"] = keep[nm + "_a"] if x is None else _c(x, npd)
+        ptr = lambda k: keep[k].ctypes.data if k in keep else None
+        f = OrFields(ptr("u_a"), ptr("v_a"), ptr("u_b"), ptr("v_b"), ptr("u_t"), ptr("v_t"),
+                     ptr("w_a"), ptr("w_b"), ptr("w_t"), dt_code, float(fields.get("alpha0", 0.0)))
+        P = len(state["lon"])
+        out = dict(lon=np.array(state["lon"], np.float64), lat=np.array(state["lat"], np.float64),
+                   z=np.array(state.get("z", np.zeros(P)), np.float64),
+                   status=np.array(state["status"], np.int32), nn=np.array(state["nn"], np.int32),
+                   fr=np.array(state.get("fr", np.zeros(P)), np.int8),
+                   u=np.zeros(P), v=np.zeros(P), w=np.zeros(P),
+                   pid=np.array(state.get("pid", np.arange(P)), np.int64),
+                   ws=np.array(state.get("ws", np.zeros(P)), np.int32))
+        self.L.or_step(ctypes.byref(self.s), ctypes.byref(f), ctypes.byref(args), P,
+                       *[out[k].ctypes.data for k in ("lon", "lat", "z", "status", "nn", "fr",
+                                                      "u", "v", "w", "pid", "ws")])
         return out
 
 

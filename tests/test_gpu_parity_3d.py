@@ -1,0 +1,112 @@
+"""Device 3-D step (z-level layers, W, time blend), split Kh random walk and
+beaching hook against the oracle (GPU).
+
+Bar: bit-identical to the oracle built with the device's math (kmath) in
+every output (positions, depth, status, nearest node, failure code,
+velocities, wet streak); against the glibc oracle the integer outputs match
+and positions agree within 1e-9 deg / 1e-6 m.
+"""
+import numpy as np
+import pytest
+
+from kelp_amd import synth
+from oracle.oracle import OracleMesh, step_args
+
+pytestmark = pytest.mark.gpu
+
+DT = 1800.0
+BEACH = dict(criteria=["depth", "on_failure", "stochastic"], depth_min=60.0, t_beach=86400.0,
+             resuspension="auto_or_stochastic", t_resuspend=43200.0, min_wet_steps=2)
+
+
+@pytest.fixture(scope="module")
+def world():
+    mesh = synth.sphere_mesh(20000)
+    snaps = [synth.jet_rossby_snapshot(mesh, s * synth.SNAP_DT) for s in range(3)]
+    lon, lat, z = synth.wet_area_seeds(mesh, 6000)
+    pid = 1000 + np.random.default_rng(5).permutation(lon.size)
+    return mesh, snaps, lon, lat, z, pid
+
+
+def _fields(snaps, t, direction, dtype):
+    ia, ib, a0, it, dtn = synth.snapshot_bracket(t, synth.SNAP_DT, len(snaps), direction)
+    cast = lambda x: x.astype(dtype)
+    A, T = snaps[ia], snaps[it]
+    f = dict(u_a=cast(A[0]), v_a=cast(A[1]), w_a=cast(A[2]), u_t=cast(T[0]), v_t=cast(T[1]),
+             w_t=cast(T[2]), alpha0=a0)
+    if ib >= 0:
+        B = snaps[ib]
+        f.update(u_b=cast(B[0]), v_b=cast(B[1]), w_b=cast(B[2]))
+    return f, (ia, ib, it, a0, dtn)
+
+
+def _run(gpu_ctx, world, order, w_method, direction, n_steps, kick, beach, dtype=np.float32,
+         flavour="kmath", steady=False):
+    mesh, snaps, lon, lat, z, pid = world
+    gpu_ctx.upload_mesh(mesh)
+    for s, sn in enumerate(snaps):
+        gpu_ctx.upload_field(s, sn[0].astype(dtype), sn[1].astype(dtype), sn[2].astype(dtype))
+    p = gpu_ctx.particles(lon, lat, z, pid=pid)
+    om = OracleMesh(mesh, flavour)
+    st = dict(lon=lon, lat=lat, z=z, status=np.ones(lon.size, np.int32),
+              nn=np.full(lon.size, -1, np.int32), pid=pid)
+    t0 = 0.0 if direction > 0 else 2 * synth.SNAP_DT
+    for k in range(n_steps):
+        t = t0 + direction * k * DT
+        if steady:
+            f = dict(u_a=snaps[0][0].astype(dtype), v_a=snaps[0][1].astype(dtype),
+                     w_a=snaps[0][2].astype(dtype))
+            ia, ib, it, a0, al = 0, -1, -1, 0.0, (0, 0, 0, 0)
+        else:
+            f, (ia, ib, it, a0, dtn) = _fields(snaps, t, direction, dtype)
+            c = (0.0, 0.5, 0.5, 1.0) if order == 4 else (0.0, 0.5)
+            al = [ck * DT / dtn for ck in c]
+        args = step_args(order, DT, direction, is3d=True, w_method=w_method, alphas=al,
+                         kick=kick, beach=beach, step_i=k)
+        st = om.step(st, f, args)
+        gpu_ctx.step(p, order, ia, DT, direction, slot_b=ib, slot_t=it, alpha0=a0, alphas=al,
+                     w_method=w_method, kick=kick, beach=beach, step_i=k)
+    gpu_ctx.synchronize()
+    return p.to_host(), st
+
+
+def _assert_bitwise(h, st):
+    for k in ("lon", "lat", "z", "u", "v", "w"):
+        assert np.array_equal(h[k].view(np.int64), st[k].view(np.int64)), k
+    for k in ("status", "nn", "fr", "ws"):
+        assert np.array_equal(h[k], st[k]), k
+
+
+@pytest.mark.parametrize("order,w_method", [(4, 2), (2, 2), (2, 0), (1, 2)])
+def test_3d_tinterp_bitwise(gpu_ctx, world, order, w_method):
+    h, st = _run(gpu_ctx, world, order, w_method, 1.0, 16, None, None)
+    _assert_bitwise(h, st)
+
+
+def test_3d_rk4_kick_beach_bitwise(gpu_ctx, world):
+    kick = dict(kh_x=10.0, kh_y=10.0, seed=2026)
+    h, st = _run(gpu_ctx, world, 4, 2, 1.0, 24, kick, BEACH)
+    assert (h["status"] == 2).sum() > 0
+    _assert_bitwise(h, st)
+
+
+def test_3d_rk2_backward_anisotropic_kick(gpu_ctx, world):
+    kick = dict(kh_x=50.0, kh_y=20.0, seed=7)
+    h, st = _run(gpu_ctx, world, 2, 0, -1.0, 16, kick, None)
+    _assert_bitwise(h, st)
+
+
+def test_3d_steady_f64_coastal(gpu_ctx, world):
+    beach = dict(criteria=["coastal_element", "on_failure"], resuspension="auto", min_wet_steps=1)
+    h, st = _run(gpu_ctx, world, 4, 2, 1.0, 12, None, beach, dtype=np.float64, steady=True)
+    _assert_bitwise(h, st)
+
+
+def test_3d_vs_glibc_oracle(gpu_ctx, world):
+    kick = dict(kh_x=10.0, kh_y=10.0, seed=2026)
+    h, st = _run(gpu_ctx, world, 4, 2, 1.0, 24, kick, BEACH, flavour="glibc")
+    for k in ("status", "nn", "fr", "ws"):
+        assert np.array_equal(h[k], st[k]), k
+    assert np.max(np.abs(h["lon"] - st["lon"])) < 1e-9
+    assert np.max(np.abs(h["lat"] - st["lat"])) < 1e-9
+    assert np.max(np.abs(h["z"] - st["z"])) < 1e-6
