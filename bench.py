@@ -1,11 +1,20 @@
 """Benchmark of the particle time-stepping hot path on one MI355X.
 
 ``python bench.py [--steps K] [--warmup W] [--workload NAME]`` prints one JSON
-line: whole-job particle-steps/s with the mesh, snapshots and particle state
-already resident in HBM, the roofline of the step kernel (algorithmic bytes
-per launch over its measured average duration, HIP events on the stream the
-kernel runs on) and a CPU baseline (the oracle, one host core, on a bounded
-sample of the same workload).
+line: whole-job particle-steps/s (particles alive at each step's start,
+counted on the device, over the wall time of the K timed steps) with the
+mesh, the snapshots and the particle state already resident in HBM; the
+roofline of the step kernel (algorithmic bytes per launch over its average
+duration, HIP events on the stream the kernel runs on); and a CPU baseline
+(the oracle, one host core, on a bounded sample of the same workload).
+
+Workloads:
+  global_3d    (default) 1M-node Fibonacci-sphere Delaunay mesh with a land
+               mask, 47 z-layers, 8 six-hourly float32 snapshots, 1M particles,
+               RK4, dt 1800 s, linear time blend, Kh = 10 m^2/s random walk,
+               beaching (on_failure + stochastic coastal).
+  double_gyre  2-D jittered 0.25-degree lattice, steady double gyre, 10,000
+               particles, RK2, dt 600 s.
 """
 from __future__ import annotations
 
@@ -23,99 +32,188 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 
-def workload_double_gyre(args):
-    """2-D double gyre on the jittered 0.25-degree lattice, 10,000 particles, RK2,
-    dt 600 s.  Algorithmic bytes per particle-step: 377 (state 65 B + 3 sites x
-    88 B of anchor row + element connectivity/coordinates + 2 stages x 24 B of
-    f32 u,v at 3 vertices)."""
-    from kelp_amd import synth
-    w = synth.double_gyre_workload(n_particles=args.particles or 10_000)
-    return dict(name="double_gyre_2d_rk2", w=w, b_alg=377.0,
-                config={"workload": "double_gyre_2d_rk2", "mesh_nodes": w.mesh.n_nodes,
-                        "mesh_elements": w.mesh.n_elements, "particles": int(w.lon.size),
-                        "integrator": "rk2", "dt_s": w.dt, "field_dtype": "f32",
-                        "steps_per_run": w.n_steps})
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-WORKLOADS = {"double_gyre": workload_double_gyre}
+class Global3D:
+    """RK4 3-D with time blend, random walk and beaching.
+
+    Algorithmic bytes per particle-step (float32 fields, 2 snapshots blended,
+    triangles, barycentric W): state 89 B + 5 evaluation sites x 88 B (anchor
+    row 28 B + 3 vertices x 20 B of ids/coordinates) + 4 stages x 104 B (u, v
+    at 3 vertices x 2 snapshots x 4 B = 48 B, W at 3 vertices x 2 interfaces x
+    2 snapshots x 4 B = 48 B, 8 B node depth) + the kick's anchor and element
+    96 B + beaching 8 B = 1049 B.
+    """
+    name = "global_3d_rk4_kh_beach"
+    b_alg = 1049.0
+
+    def __init__(self, args):
+        from kelp_amd import synth
+        self.synth = synth
+        t = time.time()
+        self.w = synth.global_3d_workload(n_nodes=args.nodes or 1_000_000,
+                                          n_particles=args.particles or 1_000_000)
+        log(f"workload built in {time.time() - t:.1f} s: {self.w.mesh.n_nodes} nodes, "
+            f"{self.w.mesh.n_elements} elements, {self.w.lon.size} particles")
+        self.kick = dict(kh_x=self.w.kh, kh_y=self.w.kh, seed=20240501)
+        self.beach = dict(criteria=["on_failure", "stochastic"], t_beach=86400.0,
+                          depth_min=0.3, seed=20240501)
+        self.order = 4
+        self.config = {"workload": self.name, "mesh_nodes": self.w.mesh.n_nodes,
+                       "mesh_elements": self.w.mesh.n_elements, "z_levels": int(self.w.mesh.sigma.size),
+                       "particles": int(self.w.lon.size), "integrator": "rk4", "dt_s": self.w.dt,
+                       "snapshots": len(self.w.snaps), "snapshot_dt_s": self.w.snap_dt,
+                       "field_dtype": "f32", "kh_m2_s": self.w.kh, "w_interp": "barycentric",
+                       "beaching": self.beach["criteria"], "steps_per_run": self.w.n_steps}
+
+    def upload(self, ctx):
+        ctx.upload_mesh(self.w.mesh)
+        for s, (U, V, W, _T, _S) in enumerate(self.w.snaps):
+            ctx.upload_field(s, U, V, W)
+        return ctx.particles(self.w.lon, self.w.lat, self.w.z)
+
+    def step(self, ctx, p, k):
+        w = self.w
+        t = k * w.dt
+        ia, ib, a0, it, dtn = self.synth.snapshot_bracket(t, w.snap_dt, len(w.snaps))
+        al = [c * w.dt / dtn for c in (0.0, 0.5, 0.5, 1.0)]
+        ctx.step(p, 4, ia, w.dt, 1.0, slot_b=ib, slot_t=it, alpha0=a0, alphas=al,
+                 exact_nearest=1, w_method=2, kick=self.kick, beach=self.beach, step_i=k)
+
+    def cpu_baseline(self, budget_s):
+        from oracle.oracle import OracleMesh, step_args
+        w = self.w
+        om = OracleMesh(w.mesh, "glibc")
+        n = min(w.lon.size, 20_000)
+        st = dict(lon=w.lon[:n], lat=w.lat[:n], z=w.z[:n], status=np.ones(n, np.int32),
+                  nn=np.full(n, -1, np.int32))
+        steps, alive_steps = 0, 0
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < budget_s and steps < w.n_steps:
+            t = steps * w.dt
+            ia, ib, a0, it, dtn = self.synth.snapshot_bracket(t, w.snap_dt, len(w.snaps))
+            A, T = w.snaps[ia], w.snaps[it]
+            f = dict(u_a=A[0], v_a=A[1], w_a=A[2], u_t=T[0], v_t=T[1], w_t=T[2], alpha0=a0)
+            if ib >= 0:
+                B = w.snaps[ib]
+                f.update(u_b=B[0], v_b=B[1], w_b=B[2])
+            al = [c * w.dt / dtn for c in (0.0, 0.5, 0.5, 1.0)]
+            alive_steps += int(np.sum(st["status"] == 1))
+            st = om.step(st, f, step_args(4, w.dt, 1.0, is3d=True, w_method=2, exact=1, alphas=al,
+                                          kick=self.kick, beach=self.beach, step_i=steps))
+            steps += 1
+        el = time.perf_counter() - t0
+        return {"value": alive_steps / el, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+                "sample": f"{n} particles x {steps} steps of the same workload, C oracle "
+                          f"(glibc libm), 1 thread, {el:.1f} s"}
 
 
-def cpu_baseline(w, order, budget_s):
-    """The oracle (glibc libm, one core) on the first particles of the same
-    workload for as many steps as fit in ``budget_s``."""
-    from oracle.oracle import OracleMesh
-    om = OracleMesh(w.mesh, "glibc")
-    n = min(w.lon.size, 10_000)
-    st = dict(lon=w.lon[:n].copy(), lat=w.lat[:n].copy(), status=np.ones(n, np.int32),
-              nn=np.full(n, -1, np.int32))
-    steps = 0
-    t0 = time.perf_counter()
-    while time.perf_counter() - t0 < budget_s:
-        st = om.step2d(st, order, w.u, w.v, w.dt, 1.0)
-        steps += 1
-    el = time.perf_counter() - t0
-    return {"value": n * steps / el, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": f"{n} particles x {steps} steps, C oracle (glibc libm), 1 thread, {el:.1f} s"}
+class DoubleGyre:
+    """2-D RK2, steady f32 double gyre.  Bytes per particle-step: state 65 B +
+    3 sites x 88 B + 2 stages x 24 B (u, v at 3 vertices, f32) = 377 B."""
+    name = "double_gyre_2d_rk2"
+    b_alg = 377.0
+
+    def __init__(self, args):
+        from kelp_amd import synth
+        self.w = synth.double_gyre_workload(n_particles=args.particles or 10_000)
+        self.order = 2
+        self.config = {"workload": self.name, "mesh_nodes": self.w.mesh.n_nodes,
+                       "mesh_elements": self.w.mesh.n_elements, "particles": int(self.w.lon.size),
+                       "integrator": "rk2", "dt_s": self.w.dt, "field_dtype": "f32",
+                       "steps_per_run": self.w.n_steps}
+
+    def upload(self, ctx):
+        ctx.upload_mesh(self.w.mesh)
+        ctx.upload_field(0, self.w.u, self.w.v)
+        return ctx.particles(self.w.lon, self.w.lat)
+
+    def step(self, ctx, p, k):
+        ctx.step(p, 2, 0, self.w.dt, 1.0, exact_nearest=1)
+
+    def cpu_baseline(self, budget_s):
+        from oracle.oracle import OracleMesh
+        w = self.w
+        om = OracleMesh(w.mesh, "glibc")
+        n = min(w.lon.size, 10_000)
+        st = dict(lon=w.lon[:n].copy(), lat=w.lat[:n].copy(), status=np.ones(n, np.int32),
+                  nn=np.full(n, -1, np.int32))
+        steps, alive_steps = 0, 0
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < budget_s:
+            alive_steps += int(np.sum(st["status"] == 1))
+            st = om.step2d(st, 2, w.u, w.v, w.dt, 1.0)
+            steps += 1
+        el = time.perf_counter() - t0
+        return {"value": alive_steps / el, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+                "sample": f"{n} particles x {steps} steps, C oracle (glibc libm), 1 thread, {el:.1f} s"}
+
+
+WORKLOADS = {"global_3d": Global3D, "double_gyre": DoubleGyre}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="double_gyre", choices=sorted(WORKLOADS))
+    ap.add_argument("--steps", type=int, default=120)
+    ap.add_argument("--warmup", type=int, default=12)
+    ap.add_argument("--workload", default="global_3d", choices=sorted(WORKLOADS))
     ap.add_argument("--particles", type=int, default=0)
-    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--nodes", type=int, default=0)
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
-                          "n_gpus": args.gpus}))
+                          "n_gpus": args.gpus}), flush=True)
         sys.exit(2)
 
     from kelp_amd.device import DeviceContext
-    spec = WORKLOADS[args.workload](args)
-    w = spec["w"]
+    wl = WORKLOADS[args.workload](args)
     ctx = DeviceContext(0)
-    ctx.upload_mesh(w.mesh)
-    ctx.upload_field(0, w.u, w.v)
-    p = ctx.particles(w.lon, w.lat)
-    order = w.order
-    for _ in range(args.warmup):
-        ctx.step(p, order, 0, w.dt, 1.0)
-    ctx.synchronize()
     torch = ctx.torch
+    p = wl.upload(ctx)
+    p.alive_count = torch.zeros(1, dtype=torch.int64, device="cuda:0")
+    log("uploaded; warming up")
+    for k in range(args.warmup):
+        wl.step(ctx, p, k)
+    ctx.synchronize()
     torch.cuda.synchronize()
-    alive_steps = 0
+    p.alive_count.zero_()
+    torch.cuda.synchronize()
+    log(f"timing {args.steps} steps")
     t0 = time.perf_counter()
     ctx.event_record(0)
-    for _ in range(args.steps):
-        ctx.step(p, order, 0, w.dt, 1.0)
+    for k in range(args.warmup, args.warmup + args.steps):
+        wl.step(ctx, p, k)
     ctx.event_record(1)
     ctx.synchronize()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     kern_ms = ctx.event_elapsed_ms(0, 1)
-    alive = int((p.status == 1).sum().item())
-    n = int(w.lon.size)
-    # every timed step advanced the particles alive at its start; the count
-    # alive at the end is a lower bound, the start count an upper bound
-    alive_steps = n * args.steps
+    alive_steps = int(p.alive_count.item())
+    alive_end = int((p.status == 1).sum().item())
     pss = alive_steps / wall
     avg_launch_s = kern_ms / 1e3 / args.steps
-    achieved = spec["b_alg"] * n / avg_launch_s / 1e9
+    per_launch_units = alive_steps / args.steps
+    achieved = wl.b_alg * per_launch_units / avg_launch_s / 1e9
     line = {
         "metric": "particle-steps/s", "value": pss, "unit": "particle-steps/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": dict(spec["config"], alive_end=alive),
+        "config": dict(wl.config, alive_end=alive_end),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel_avg_us": avg_launch_s * 1e6, "b_alg_per_particle_step": spec["b_alg"]},
+                     "kernel": "step_kernel", "kernel_avg_us": avg_launch_s * 1e6,
+                     "b_alg_per_particle_step": wl.b_alg,
+                     "particle_steps_per_launch": per_launch_units},
     }
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(w, order, args.cpu_budget)
+        log("CPU baseline")
+        line["cpu_baseline"] = wl.cpu_baseline(args.cpu_budget)
     print(json.dumps(line), flush=True)
     ctx.close()
 

@@ -64,6 +64,7 @@ typedef struct {
     double *u, *v, *w; /* diagnostic velocity at the step start (w may be NULL in 2-D) */
     const int64_t *pid; /* global particle id = RNG key; needed with the kick or beaching */
     int32_t *wet_streak; /* beaching wet streak (tracking loop state); needed with beaching */
+    uint64_t *alive_count; /* optional: += particles alive at the step start */
 } kelp_particles;
 
 /* One step (the arguments of step_fn at tracking.py:2643-2655 and the

@@ -41,7 +41,7 @@ class Particles(ctypes.Structure):
         ("lon", c_void_p), ("lat", c_void_p), ("z", c_void_p),
         ("status", c_void_p), ("nn", c_void_p), ("fr", c_void_p),
         ("u", c_void_p), ("v", c_void_p), ("w", c_void_p),
-        ("pid", c_void_p), ("wet_streak", c_void_p),
+        ("pid", c_void_p), ("wet_streak", c_void_p), ("alive_count", c_void_p),
     ]
 
 

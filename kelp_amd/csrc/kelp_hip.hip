@@ -165,6 +165,7 @@ struct PState {
     double* __restrict__ w;
     const int64_t* __restrict__ pid;
     int32_t* __restrict__ ws;
+    unsigned long long* __restrict__ alive_count;
     int64_t n;
 };
 
@@ -438,6 +439,13 @@ __global__ __launch_bounds__(256) void step_kernel(DMesh m, Fields<T> f, StepCfg
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= s.n) return;
     const int32_t prev = s.status[i];
+    if (s.alive_count) {
+        // particles alive at the step start: one atomic per wavefront
+        unsigned long long alive = __ballot(prev == kAlive);
+        unsigned long long active = __ballot(1);
+        if ((int)(threadIdx.x & 63) == __ffsll((long long)active) - 1 && alive)
+            atomicAdd(s.alive_count, (unsigned long long)__popcll(alive));
+    }
     int32_t st = prev;
     double lon = s.lon[i], lat = s.lat[i];
     double z = IS3D ? s.z[i] : 0.0;
@@ -881,7 +889,8 @@ int kelp_step(kelp_ctx* c, const kelp_particles* p, const kelp_step_args* a)
     cfg.t_resuspend = a->t_resuspend;
     cfg.beach_dt = a->beach_dt;
     cfg.beach_seed = a->beach_seed;
-    PState ps{p->lon, p->lat, p->z, p->status, p->nn, p->fr, p->u, p->v, p->w, p->pid, p->wet_streak, p->n};
+    PState ps{p->lon, p->lat, p->z, p->status, p->nn, p->fr, p->u, p->v, p->w, p->pid, p->wet_streak,
+              reinterpret_cast<unsigned long long*>(p->alive_count), p->n};
     if (sa.dtype == KELP_DT_F32)
         launch_step<float>(c, a->order, is3d, sa, sb, st, cfg, ps);
     else

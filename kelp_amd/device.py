@@ -47,6 +47,7 @@ class DeviceParticles:
     w: object
     pid: object
     ws: object
+    alive_count: object = None
 
     @property
     def n(self) -> int:
@@ -55,7 +56,7 @@ class DeviceParticles:
     def as_struct(self) -> Particles:
         return Particles(self.n, _ptr(self.lon), _ptr(self.lat), _ptr(self.z), _ptr(self.status),
                          _ptr(self.nn), _ptr(self.fr), _ptr(self.u), _ptr(self.v), _ptr(self.w),
-                         _ptr(self.pid), _ptr(self.ws))
+                         _ptr(self.pid), _ptr(self.ws), _ptr(self.alive_count))
 
     def to_host(self) -> Dict[str, np.ndarray]:
         return {k: getattr(self, k).cpu().numpy() for k in
