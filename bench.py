@@ -164,6 +164,10 @@ def main():
     ap.add_argument("--nodes", type=int, default=0)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--resort-every", type=int, default=4,
+                    help="re-sort particles by grid-hash cell every N steps (0: never)")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="first copy a buffer of this size 3x (calibrates profiler byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
@@ -174,11 +178,26 @@ def main():
     wl = WORKLOADS[args.workload](args)
     ctx = DeviceContext(0)
     torch = ctx.torch
+    if args.copy_probe_mib > 0:
+        n = args.copy_probe_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy probe: {args.copy_probe_mib} MiB read + {args.copy_probe_mib} MiB written per copy")
+        del src, dst
     p = wl.upload(ctx)
     p.alive_count = torch.zeros(1, dtype=torch.int64, device="cuda:0")
     log("uploaded; warming up")
-    for k in range(args.warmup):
+
+    def run_step(k):
+        if args.resort_every > 0 and k % args.resort_every == 0:
+            ctx.sort(p)
         wl.step(ctx, p, k)
+
+    for k in range(args.warmup):
+        run_step(k)
     ctx.synchronize()
     torch.cuda.synchronize()
     p.alive_count.zero_()
@@ -187,7 +206,7 @@ def main():
     t0 = time.perf_counter()
     ctx.event_record(0)
     for k in range(args.warmup, args.warmup + args.steps):
-        wl.step(ctx, p, k)
+        run_step(k)
     ctx.event_record(1)
     ctx.synchronize()
     torch.cuda.synchronize()
@@ -204,7 +223,7 @@ def main():
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": dict(wl.config, alive_end=alive_end),
+        "config": dict(wl.config, alive_end=alive_end, resort_every=args.resort_every),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "step_kernel", "kernel_avg_us": avg_launch_s * 1e6,

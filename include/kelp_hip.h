@@ -62,7 +62,7 @@ typedef struct {
     int32_t *nn;       /* nearest node */
     int8_t *fr;        /* FailureReason (sticky; written on the step a particle dies) */
     double *u, *v, *w; /* diagnostic velocity at the step start (w may be NULL in 2-D) */
-    const int64_t *pid; /* global particle id = RNG key; needed with the kick or beaching */
+    int64_t *pid;      /* global particle id = RNG key; needed with the kick, beaching, sorting */
     int32_t *wet_streak; /* beaching wet streak (tracking loop state); needed with beaching */
     uint64_t *alive_count; /* optional: += particles alive at the step start */
 } kelp_particles;
@@ -118,6 +118,14 @@ int kelp_field_clear(kelp_ctx *ctx, int slot);
 
 /* Advance every particle by one step, in place, on the context stream. */
 int kelp_step(kelp_ctx *ctx, const kelp_particles *p, const kelp_step_args *a);
+
+/* Reorder every particle array in place by the grid-hash cell of each
+ * particle (alive first, then beached, then the rest) so that the step's
+ * mesh and field gathers of neighbouring lanes share cache lines.  Results
+ * do not depend on the order (draws are keyed by pid); restore the release
+ * order from pid.  No reference counterpart: the reference's threads work
+ * on release order (numerics/integrators.py prange loops). */
+int kelp_sort_particles(kelp_ctx *ctx, const kelp_particles *p);
 
 /* Batch queries (core/mesh.py:1896 find_nearest_nodes and :2289
  * find_containing_elements) on device arrays. */

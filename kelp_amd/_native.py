@@ -76,6 +76,7 @@ SIGNATURES = {
                                   c_void_p, c_int64]),
     "kelp_field_clear": (c_int, [c_void_p, c_int]),
     "kelp_step": (c_int, [c_void_p, POINTER(Particles), POINTER(StepArgs)]),
+    "kelp_sort_particles": (c_int, [c_void_p, POINTER(Particles)]),
     "kelp_nearest": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_int, c_void_p]),
     "kelp_containing": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p]),
     "kelp_malloc": (c_int, [c_void_p, c_int64, POINTER(c_void_p)]),

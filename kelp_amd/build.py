@@ -13,7 +13,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SRC = os.path.join(HERE, "csrc", "kelp_hip.hip")
+SRCS = [os.path.join(HERE, "csrc", f) for f in ("kelp_hip.hip", "kelp_sort.hip")]
 OUT = os.path.join(HERE, "libkelp_hip.so")
 
 HIPCC_FLAGS = [
@@ -33,7 +33,7 @@ def hipcc() -> str:
 
 def build_hip(verbose: bool = False) -> str:
     tmp = OUT + ".tmp"
-    cmd = [hipcc(), *HIPCC_FLAGS, "-o", tmp, SRC]
+    cmd = [hipcc(), *HIPCC_FLAGS, "-o", tmp, *SRCS]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed ({r.returncode}):\n{' '.join(cmd)}\n{r.stderr[-4000:]}")

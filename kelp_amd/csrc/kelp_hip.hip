@@ -14,8 +14,7 @@
 #include <new>
 #include <vector>
 
-#include "../../include/kelp_hip.h"
-#include "kelp_device.cuh"
+#include "kelp_internal.h"
 
 using namespace kelp;
 
@@ -27,7 +26,7 @@ using namespace kelp;
 
 static thread_local char g_err[512] = "";
 
-static int fail(int code, const char* fmt, ...)
+int kelp_fail(int code, const char* fmt, ...)
 {
     va_list ap;
     va_start(ap, fmt);
@@ -36,39 +35,28 @@ static int fail(int code, const char* fmt, ...)
     return code;
 }
 
-#define HIP_TRY(expr)                                                                      \
-    do {                                                                                   \
-        hipError_t e_ = (expr);                                                            \
-        if (e_ != hipSuccess)                                                              \
-            return fail(e_ == hipErrorOutOfMemory ? KELP_ENOMEM : KELP_EHIP, "%s: %s (%s:%d)", \
-                        #expr, hipGetErrorString(e_), __FILE__, __LINE__);                 \
-    } while (0)
+#define fail kelp_fail
+
+int kelp_reserve(DevBuf& b, size_t bytes)
+{
+    if (b.bytes >= bytes && b.p) return KELP_OK;
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+    HIP_TRY(hipMalloc(&b.p, bytes > 0 ? bytes : 16));
+    b.bytes = bytes;
+    return KELP_OK;
+}
+
+void kelp_release(DevBuf& b)
+{
+    if (b.p) (void)hipFree(b.p);
+    b = DevBuf{};
+}
 
 // ---------------------------------------------------------------------------
 // context
 // ---------------------------------------------------------------------------
-
-constexpr int kMaxSlots = 16;
-constexpr int kMaxEvents = 16;
-
-struct FieldSlot {
-    int dtype = -1;  // -1: empty
-    int64_t n_planes = 0, n_w_planes = 0;
-    void* u = nullptr;
-    void* v = nullptr;
-    void* w = nullptr;
-};
-
-struct kelp_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t events[kMaxEvents] = {};
-    bool has_mesh = false;
-    DMesh m{};
-    // owned device buffers of the mesh
-    std::vector<void*> mesh_bufs;
-    FieldSlot slots[kMaxSlots];
-};
 
 static void free_mesh(kelp_ctx* c)
 {
@@ -620,6 +608,8 @@ int kelp_ctx_destroy(kelp_ctx* c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_mesh(c);
     for (auto& s : c->slots) free_slot(s);
+    for (DevBuf* b : {&c->sort_tmp, &c->sort_keys, &c->sort_keys2, &c->sort_idx, &c->sort_perm, &c->sort_stage})
+        kelp_release(*b);
     for (int k = 0; k < kMaxEvents; ++k)
         if (c->events[k]) (void)hipEventDestroy(c->events[k]);
     if (c->stream) (void)hipStreamDestroy(c->stream);

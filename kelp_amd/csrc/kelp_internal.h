@@ -1,0 +1,54 @@
+// kelp_internal.h -- context and error plumbing shared by the library's
+// translation units (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <vector>
+
+#include "../../include/kelp_hip.h"
+#include "kelp_device.cuh"
+
+int kelp_fail(int code, const char* fmt, ...);
+
+#define HIP_TRY(expr)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return kelp_fail(e_ == hipErrorOutOfMemory ? KELP_ENOMEM : KELP_EHIP, "%s: %s (%s:%d)", \
+                             #expr, hipGetErrorString(e_), __FILE__, __LINE__);            \
+    } while (0)
+
+constexpr int kMaxSlots = 16;
+constexpr int kMaxEvents = 16;
+
+struct FieldSlot {
+    int dtype = -1;  // -1: empty
+    int64_t n_planes = 0, n_w_planes = 0;
+    void* u = nullptr;
+    void* v = nullptr;
+    void* w = nullptr;
+};
+
+// growable device buffer owned by a context
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+struct kelp_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t events[kMaxEvents] = {};
+    bool has_mesh = false;
+    kelp::DMesh m{};
+    std::vector<void*> mesh_bufs;  // owned device buffers of the mesh
+    FieldSlot slots[kMaxSlots];
+    DevBuf sort_tmp, sort_keys, sort_keys2, sort_idx, sort_perm, sort_stage;
+};
+
+// (re)allocate `b` to at least `bytes` (contents not preserved)
+int kelp_reserve(DevBuf& b, size_t bytes);
+void kelp_release(DevBuf& b);

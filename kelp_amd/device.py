@@ -222,6 +222,11 @@ class DeviceContext:
         s = p.as_struct()
         check(self.lib.kelp_step(self.h, ctypes.byref(s), ctypes.byref(a)), "kelp_step")
 
+    def sort(self, p: DeviceParticles) -> None:
+        """Reorder the particle arrays by grid-hash cell (see kelp_sort_particles)."""
+        s = p.as_struct()
+        check(self.lib.kelp_sort_particles(self.h, ctypes.byref(s)), "kelp_sort_particles")
+
     def nearest(self, lon, lat, exact: bool = True) -> np.ndarray:
         t = self.torch
         dev = f"cuda:{self.device}"

