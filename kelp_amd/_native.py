@@ -91,11 +91,13 @@ SIGNATURES = {
 _LIB = None
 
 
-def load(path: str = LIB_PATH) -> ctypes.CDLL:
-    """Load the HIP library (once).  Raises if it has not been built."""
+def load(path: str = "") -> ctypes.CDLL:
+    """Load the HIP library (once).  Raises if it has not been built.
+    ``KELP_HIP_LIB`` overrides the in-tree path (alternative builds)."""
     global _LIB
     if _LIB is not None:
         return _LIB
+    path = path or os.environ.get("KELP_HIP_LIB") or LIB_PATH
     if not os.path.exists(path):
         raise ImportError(
             f"{path} is missing: build the HIP library first "

@@ -500,18 +500,61 @@ __device__ inline int site_anchor(const DMesh& m, double lon_p, double lat_p, in
 
 // ---------------------------------------------------------------------------
 // Containment
+//
+// MAXV is the element width of the mesh (3: triangles, 4: quads and mixed).
+// Vertex lists are compacted with predicated, fully unrolled loops so the
+// per-lane arrays stay in registers.
 // ---------------------------------------------------------------------------
-
-// Vertex ring of element e with the FESOM-C repeated-first-vertex slot
-// dropped (core/mesh.py:2221-2238).
-struct Ring {
-    int n;
-    int v[4];
-};
 
 __device__ __forceinline__ int elem_slot(int4 e, int k)
 {
     return k == 0 ? e.x : (k == 1 ? e.y : (k == 2 ? e.z : e.w));
+}
+
+// set v[q] = x for the q == idx (idx is a runtime value < 4)
+__device__ __forceinline__ void put4(int (&v)[4], int idx, int x)
+{
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        if (q == idx) v[q] = x;
+}
+
+// Valid vertices of element c for the containment test: -1 slots and the
+// FESOM-C repeated-first-vertex slot 3 dropped (core/mesh.py:2221-2238).
+template <int MAXV>
+__device__ __forceinline__ int contain_ring(const DMesh& m, int c, int (&v)[4])
+{
+    int4 en = m.elems[c];
+    int n = 0;
+#pragma unroll
+    for (int j = 0; j < MAXV; ++j) {
+        int nd = elem_slot(en, j);
+        bool keep = nd >= 0;
+        if (j == 3 && en.w == en.x && en.x >= 0) keep = false;
+        if (keep) {
+            put4(v, n, nd);
+            ++n;
+        }
+    }
+    return n;
+}
+
+// Compacted vertex ring for interpolation: -1 slots and any repeat of the
+// first vertex dropped (interpolate.py:1376-1386, vertical.py:634-644).
+template <int MAXV>
+__device__ __forceinline__ int interp_ring(int4 en, int (&v)[4])
+{
+    int n = 0;
+#pragma unroll
+    for (int k = 0; k < MAXV; ++k) {
+        int nd = elem_slot(en, k);
+        bool keep = nd >= 0 && !(n > 0 && nd == v[0]);
+        if (keep) {
+            put4(v, n, nd);
+            ++n;
+        }
+    }
+    return n;
 }
 
 // core/mesh.py:2055 _barycentric_weights_tri
@@ -539,102 +582,117 @@ __device__ __forceinline__ bool in_tri(double px, double py, double ax, double a
 
 // core/mesh.py:2211 _element_contains (bbox prefilter on wrapped deltas,
 // then :2140 _polygon_contains_bool with the :2007 projection)
+template <int MAXV>
 __device__ inline bool element_contains(const DMesh& m, int c, double lon_p, double lat_p)
 {
-    int4 en = m.elems[c];
-    int valid[4];
-    int n = 0;
-    int mv = m.max_v;
-    for (int j = 0; j < mv; ++j) {
-        int nd = elem_slot(en, j);
-        bool keep = nd >= 0;
-        if (j == 3 && mv == 4 && en.w == en.x && en.x >= 0) keep = false;
-        if (keep) valid[n++] = nd;
-    }
+    int v[4] = {0, 0, 0, 0};
+    const int n = contain_ring<MAXV>(m, c, v);
     if (n < 3) return false;
-    double lo[4], la[4];
-    for (int k = 0; k < n; ++k) {
-        double2 ll = m.lonlat[valid[k]];
-        lo[k] = ll.x;
-        la[k] = ll.y;
+    double lo[4] = {0, 0, 0, 0}, la[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < MAXV; ++k) {
+        if (k < n) {
+            double2 ll = m.lonlat[v[k]];
+            lo[k] = ll.x;
+            la[k] = ll.y;
+        }
     }
-    double lon_ref = lo[0];
+    const double lon_ref = lo[0];
     double dmax = 0.0, dmin = 0.0;
-    for (int k = 0; k < n; ++k) {
-        double d = km_pymod360(lo[k] - lon_ref + 180.0) - 180.0;
-        if (k == 0 || d > dmax) dmax = d;
-        if (k == 0 || d < dmin) dmin = d;
+#pragma unroll
+    for (int k = 0; k < MAXV; ++k) {
+        if (k < n) {
+            double d = km_pymod360(lo[k] - lon_ref + 180.0) - 180.0;
+            if (k == 0 || d > dmax) dmax = d;
+            if (k == 0 || d < dmin) dmin = d;
+        }
     }
     double dlon_p = km_pymod360(lon_p - lon_ref + 180.0) - 180.0;
     if (dlon_p > dmax || dlon_p < dmin) return false;
     double lamax = la[0], lamin = la[0];
-    for (int k = 1; k < n; ++k) {
-        if (la[k] > lamax) lamax = la[k];
-        if (la[k] < lamin) lamin = la[k];
+#pragma unroll
+    for (int k = 1; k < MAXV; ++k) {
+        if (k < n) {
+            if (la[k] > lamax) lamax = la[k];
+            if (la[k] < lamin) lamin = la[k];
+        }
     }
     if (lat_p > lamax || lat_p < lamin) return false;
     // projection (core/mesh.py:2007)
     double lmin = lo[0], lmax = lo[0];
-    for (int k = 0; k < n; ++k) {
-        if (lo[k] < lmin) lmin = lo[k];
-        if (lo[k] > lmax) lmax = lo[k];
+#pragma unroll
+    for (int k = 0; k < MAXV; ++k) {
+        if (k < n) {
+            if (lo[k] < lmin) lmin = lo[k];
+            if (lo[k] > lmax) lmax = lo[k];
+        }
     }
+    const double nd = (double)n;
     double lon0;
     if (lmax - lmin > 180.0) {
         double dsum = 0.0;
-        for (int k = 0; k < n; ++k) dsum += km_pymod360(lo[k] - lo[0] + 180.0) - 180.0;
-        lon0 = lo[0] + dsum / n;
+#pragma unroll
+        for (int k = 0; k < MAXV; ++k)
+            if (k < n) dsum += km_pymod360(lo[k] - lo[0] + 180.0) - 180.0;
+        lon0 = lo[0] + dsum / nd;
     } else {
-        double s = 0.0;
-        for (int k = 0; k < n; ++k) s += lo[k];
-        lon0 = s / n;
+        double ssum = 0.0;
+#pragma unroll
+        for (int k = 0; k < MAXV; ++k)
+            if (k < n) ssum += lo[k];
+        lon0 = ssum / nd;
     }
     double sl = 0.0;
-    for (int k = 0; k < n; ++k) sl += la[k];
-    double lat0 = sl / n;
-    double cos_lat0 = km_cos(lat0 * kDeg2Rad);
-    double xn[4], yn[4];
-    for (int k = 0; k < n; ++k) {
-        double dlon = km_pymod360(lo[k] - lon0 + 180.0) - 180.0;
-        double dlat = la[k] - lat0;
-        xn[k] = dlon * kDeg2Rad * kREarthStd * cos_lat0;
-        yn[k] = dlat * kDeg2Rad * kREarthStd;
+#pragma unroll
+    for (int k = 0; k < MAXV; ++k)
+        if (k < n) sl += la[k];
+    const double lat0 = sl / nd;
+    const double cos_lat0 = km_cos(lat0 * kDeg2Rad);
+    double xn[4] = {0, 0, 0, 0}, yn[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < MAXV; ++k) {
+        if (k < n) {
+            double dlon = km_pymod360(lo[k] - lon0 + 180.0) - 180.0;
+            double dlat = la[k] - lat0;
+            xn[k] = dlon * kDeg2Rad * kREarthStd * cos_lat0;
+            yn[k] = dlat * kDeg2Rad * kREarthStd;
+        }
     }
     double dlp = km_pymod360(lon_p - lon0 + 180.0) - 180.0;
     double px = dlp * kDeg2Rad * kREarthStd * cos_lat0;
     double py = (lat_p - lat0) * kDeg2Rad * kREarthStd;
-    if (n == 3) return in_tri(px, py, xn[0], yn[0], xn[1], yn[1], xn[2], yn[2]);
-    for (int i = 1; i < n - 1; ++i)
-        if (in_tri(px, py, xn[0], yn[0], xn[i], yn[i], xn[i + 1], yn[i + 1])) return true;
-    return false;
+    if (n == 3 || MAXV == 3) return in_tri(px, py, xn[0], yn[0], xn[1], yn[1], xn[2], yn[2]);
+    // quad: fan (0, 1, 2), (0, 2, 3)
+    if (in_tri(px, py, xn[0], yn[0], xn[1], yn[1], xn[2], yn[2])) return true;
+    return in_tri(px, py, xn[0], yn[0], xn[2], yn[2], xn[3], yn[3]);
 }
 
+template <int MAXV>
 __device__ __forceinline__ bool elem_touches(const DMesh& m, int c, int nn)
 {
     int4 e = m.elems[c];
-    int mv = m.max_v;
-    if (e.x == nn) return true;
-    if (e.y == nn) return true;
-    if (e.z == nn) return true;
-    return mv == 4 && e.w == nn;
+    if (e.x == nn || e.y == nn || e.z == nn) return true;
+    return MAXV == 4 && e.w == nn;
 }
 
 // The rare 2-ring pass of core/mesh.py:2440-2463, kept out of line so the
 // common 1-ring hit stays a short straight path.
+template <int MAXV>
 __device__ __noinline__ int containing_2ring(const DMesh& m, double lon_p, double lat_p, int nn)
 {
     int k0 = m.n2e_off[nn], k1 = m.n2e_off[nn + 1];
     for (int k = k0; k < k1; ++k) {
         int e = m.n2e[k];
         int4 en = m.elems[e];
-        for (int j = 0; j < m.max_v; ++j) {
+#pragma unroll
+        for (int j = 0; j < MAXV; ++j) {
             int v = elem_slot(en, j);
             if (v < 0 || v == nn) continue;
             int q0 = m.n2e_off[v], q1 = m.n2e_off[v + 1];
             for (int kk = q0; kk < q1; ++kk) {
                 int c = m.n2e[kk];
-                if (elem_touches(m, c, nn)) continue;
-                if (element_contains(m, c, lon_p, lat_p)) return c;
+                if (elem_touches<MAXV>(m, c, nn)) continue;
+                if (element_contains<MAXV>(m, c, lon_p, lat_p)) return c;
             }
         }
     }
@@ -642,15 +700,16 @@ __device__ __noinline__ int containing_2ring(const DMesh& m, double lon_p, doubl
 }
 
 // core/mesh.py:2393 _find_containing_element_local
+template <int MAXV>
 __device__ inline int find_containing(const DMesh& m, double lon_p, double lat_p, int nn)
 {
     int k0 = m.n2e_off[nn], k1 = m.n2e_off[nn + 1];
     if (k1 == k0) return -1;
     for (int k = k0; k < k1; ++k) {
         int c = m.n2e[k];
-        if (element_contains(m, c, lon_p, lat_p)) return c;
+        if (element_contains<MAXV>(m, c, lon_p, lat_p)) return c;
     }
-    return containing_2ring(m, lon_p, lat_p, nn);
+    return containing_2ring<MAXV>(m, lon_p, lat_p, nn);
 }
 
 // ---------------------------------------------------------------------------
@@ -663,62 +722,75 @@ struct Weights {
     double w[4];
 };
 
-__device__ inline bool vertex_weights(const DMesh& m, int e, double xp, double yp, Weights& W)
+// product-form Wachspress weights of a convex quad (core/mesh.py:2077)
+__device__ __forceinline__ void wachspress4(double px, double py, const double (&vx)[4], const double (&vy)[4],
+                                            double (&w)[4])
 {
-    int4 en = m.elems[e];
-    int nv = 0;
-    for (int k = 0; k < m.max_v; ++k) {
-        int nd = elem_slot(en, k);
-        if (nd < 0) continue;
-        if (nv > 0 && nd == W.v[0]) continue;
-        W.v[nv++] = nd;
-    }
-    W.n = nv;
-    if (nv < 3) return false;
-    const double R = m.r_earth;
-    double lo[4], la[4];
-    for (int k = 0; k < nv; ++k) {
-        double2 ll = m.lonlat[W.v[k]];
-        lo[k] = ll.x;
-        la[k] = ll.y;
-    }
-    double lon_ref = lo[0];
-    double dsum = 0.0, lat0 = 0.0;
-    for (int k = 0; k < nv; ++k) {
-        dsum += km_pymod360(lo[k] - lon_ref + 180.0) - 180.0;
-        lat0 += la[k];
-    }
-    double lon0 = lon_ref + dsum / nv;
-    lat0 /= nv;
-    double cosl = km_cos(lat0 * kDeg2Rad);
-    double vx[4], vy[4];
-    for (int k = 0; k < nv; ++k) {
-        double dlon = km_pymod360(lo[k] - lon0 + 180.0) - 180.0;
-        vx[k] = dlon * kDeg2Rad * R * cosl;
-        vy[k] = (la[k] - lat0) * kDeg2Rad * R;
-    }
-    double dlon_p = km_pymod360(xp - lon0 + 180.0) - 180.0;
-    double px = dlon_p * kDeg2Rad * R * cosl;
-    double py = (yp - lat0) * kDeg2Rad * R;
-    if (nv == 3) {
-        bary_tri(px, py, vx[0], vy[0], vx[1], vy[1], vx[2], vy[2], W.w[0], W.w[1], W.w[2]);
-        return W.w[0] != -1.0;
-    }
-    // product-form Wachspress (core/mesh.py:2077)
     double A[4], C[4];
+#pragma unroll
     for (int k = 0; k < 4; ++k) {
-        int i1 = (k + 1) & 3;
-        int im = (k + 3) & 3;
+        const int i1 = (k + 1) & 3, im = (k + 3) & 3;
         A[k] = ((vx[k] - px) * (vy[i1] - py) - (vx[i1] - px) * (vy[k] - py));
         C[k] = ((vx[k] - vx[im]) * (vy[i1] - vy[im]) - (vx[i1] - vx[im]) * (vy[k] - vy[im]));
     }
     double s = 0.0;
+#pragma unroll
     for (int k = 0; k < 4; ++k) {
-        double w = C[k] * A[(k + 1) & 3] * A[(k + 2) & 3];
-        W.w[k] = w;
-        s += w;
+        w[k] = C[k] * A[(k + 1) & 3] * A[(k + 2) & 3];
+        s += w[k];
     }
-    for (int k = 0; k < 4; ++k) W.w[k] = W.w[k] / s;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) w[k] = w[k] / s;
+}
+
+template <int MAXV>
+__device__ inline bool vertex_weights(const DMesh& m, int e, double xp, double yp, Weights& W)
+{
+    W.v[0] = W.v[1] = W.v[2] = W.v[3] = 0;
+    W.w[0] = W.w[1] = W.w[2] = W.w[3] = 0.0;
+    const int nv = interp_ring<MAXV>(m.elems[e], W.v);
+    W.n = nv;
+    if (nv < 3) return false;
+    const double R = m.r_earth;
+    double lo[4] = {0, 0, 0, 0}, la[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < MAXV; ++k) {
+        if (k < nv) {
+            double2 ll = m.lonlat[W.v[k]];
+            lo[k] = ll.x;
+            la[k] = ll.y;
+        }
+    }
+    const double lon_ref = lo[0];
+    double dsum = 0.0, lat0 = 0.0;
+#pragma unroll
+    for (int k = 0; k < MAXV; ++k) {
+        if (k < nv) {
+            dsum += km_pymod360(lo[k] - lon_ref + 180.0) - 180.0;
+            lat0 += la[k];
+        }
+    }
+    const double nd = (double)nv;
+    const double lon0 = lon_ref + dsum / nd;
+    lat0 /= nd;
+    const double cosl = km_cos(lat0 * kDeg2Rad);
+    double vx[4] = {0, 0, 0, 0}, vy[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < MAXV; ++k) {
+        if (k < nv) {
+            double dlon = km_pymod360(lo[k] - lon0 + 180.0) - 180.0;
+            vx[k] = dlon * kDeg2Rad * R * cosl;
+            vy[k] = (la[k] - lat0) * kDeg2Rad * R;
+        }
+    }
+    double dlon_p = km_pymod360(xp - lon0 + 180.0) - 180.0;
+    double px = dlon_p * kDeg2Rad * R * cosl;
+    double py = (yp - lat0) * kDeg2Rad * R;
+    if (nv == 3 || MAXV == 3) {
+        bary_tri(px, py, vx[0], vy[0], vx[1], vy[1], vx[2], vy[2], W.w[0], W.w[1], W.w[2]);
+        return W.w[0] != -1.0;
+    }
+    wachspress4(px, py, vx, vy, W.w);
     return W.w[0] != -1.0;
 }
 
@@ -758,6 +830,23 @@ __device__ __forceinline__ double vblend(const StageFields<T>& f, int64_t idx, d
 // Vertical (z-level)
 // ---------------------------------------------------------------------------
 
+// First j in [0, K-1) with zl[j+1] > x, K-1 when there is none.  The levels
+// are non-decreasing (checked at upload), so the predicate is monotone and a
+// binary search returns the index of the reference's linear scans.
+__device__ __forceinline__ int first_above(const DMesh& m, double x)
+{
+    const double* zl = m.zlev;
+    int lo = 0, hi = m.n_levels - 1;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (zl[mid + 1] > x)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    return lo;
+}
+
 // numerics/vertical.py:40 find_layer
 __device__ inline int find_layer(const DMesh& m, double zp, double bottom)
 {
@@ -765,17 +854,11 @@ __device__ inline int find_layer(const DMesh& m, double zp, double bottom)
     const double* zl = m.zlev;
     if (K < 2) return 0;
     if (zp <= zl[0]) return 0;
-    int max_layer = K - 2;
-    for (int k = 0; k < K - 1; ++k) {
-        if (zl[k + 1] > bottom) {
-            max_layer = k - 1 > 0 ? k - 1 : 0;
-            break;
-        }
-    }
+    int k = first_above(m, bottom);
+    int max_layer = k < K - 1 ? (k - 1 > 0 ? k - 1 : 0) : K - 2;
     if (zp >= bottom || zp >= zl[max_layer + 1]) return max_layer;
-    for (int k = 0; k < max_layer + 1; ++k)
-        if (zp < zl[k + 1]) return k;
-    return max_layer;
+    const int r = first_above(m, zp);  // > max_layer only for a NaN depth
+    return r <= max_layer ? r : max_layer;
 }
 
 // numerics/vertical.py:1043 _tblend_scalar
@@ -808,13 +891,8 @@ __device__ inline double w_linear(const DMesh& m, const StageFields<T>& fw, doub
     if (K < 2) return time_dir * wlev(fw, node, alpha0, alpha);
     if (zp <= zl[0]) return time_dir * wlev(fw, node, alpha0, alpha);
     if (zp >= zl[K - 1]) return time_dir * wlev(fw, (int64_t)(K - 1) * N + node, alpha0, alpha);
-    int k = K - 2;
-    for (int j = 0; j < K - 1; ++j) {
-        if (zp < zl[j + 1]) {
-            k = j;
-            break;
-        }
-    }
+    int k = first_above(m, zp);  // < K - 1 here unless zp is NaN
+    if (k > K - 2) k = K - 2;
     double dz = zl[k + 1] - zl[k];
     if (dz < 1e-12) return time_dir * wlev(fw, (int64_t)k * N + node, alpha0, alpha);
     double a = (zp - zl[k]) / dz;
@@ -827,99 +905,101 @@ __device__ inline double w_linear(const DMesh& m, const StageFields<T>& fw, doub
 // sub-triangle of a 1-ring element holding the point (projection around the
 // seam-safe vertex mean, core/mesh.py:1974), bottom-aware linear W per vertex;
 // fallback = linear W at the anchor, depth clamped to its bottom.
-template <typename T>
-__device__ __noinline__ double w_bary(const DMesh& m, const StageFields<T>& fw, double alpha0, double alpha,
-                                      double xp, double yp, double zp, int node, double time_dir)
+template <int MAXV, typename T>
+__device__ inline double w_bary(const DMesh& m, const StageFields<T>& fw, double alpha0, double alpha, double xp,
+                                double yp, double zp, int node, double time_dir)
 {
     const int k0 = m.n2e_off[node], k1 = m.n2e_off[node + 1];
     for (int k = k0; k < k1; ++k) {
-        int4 en = m.elems[m.n2e[k]];
-        int verts[4];
-        int nv = 0;
-        for (int v = 0; v < m.max_v; ++v) {
-            int nd = elem_slot(en, v);
-            if (nd < 0) continue;
-            if (nv > 0 && nd == verts[0]) continue;
-            verts[nv++] = nd;
-        }
+        int verts[4] = {0, 0, 0, 0};
+        const int nv = interp_ring<MAXV>(m.elems[m.n2e[k]], verts);
         if (nv < 3) continue;
-        double lo[4], la[4];
-        for (int v = 0; v < nv; ++v) {
-            double2 ll = m.lonlat[verts[v]];
-            lo[v] = ll.x;
-            la[v] = ll.y;
+        double lo[4] = {0, 0, 0, 0}, la[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int v = 0; v < MAXV; ++v) {
+            if (v < nv) {
+                double2 ll = m.lonlat[verts[v]];
+                lo[v] = ll.x;
+                la[v] = ll.y;
+            }
         }
-        double first = lo[0], lmin = first, lmax = first, ssum = 0.0;
-        for (int v = 0; v < nv; ++v) {
-            ssum += lo[v];
-            if (lo[v] < lmin) lmin = lo[v];
-            if (lo[v] > lmax) lmax = lo[v];
+        const double first = lo[0];
+        double lmin = first, lmax = first, ssum = 0.0;
+#pragma unroll
+        for (int v = 0; v < MAXV; ++v) {
+            if (v < nv) {
+                ssum += lo[v];
+                if (lo[v] < lmin) lmin = lo[v];
+                if (lo[v] > lmax) lmax = lo[v];
+            }
         }
+        const double nd = (double)nv;
         double lon0;
         if (lmax - lmin > 180.0) {
             double d = 0.0;
-            for (int v = 0; v < nv; ++v) d += km_pymod360(lo[v] - first + 180.0) - 180.0;
-            lon0 = first + d / nv;
+#pragma unroll
+            for (int v = 0; v < MAXV; ++v)
+                if (v < nv) d += km_pymod360(lo[v] - first + 180.0) - 180.0;
+            lon0 = first + d / nd;
         } else {
-            lon0 = ssum / nv;
+            lon0 = ssum / nd;
         }
         double lat0 = 0.0;
-        for (int v = 0; v < nv; ++v) lat0 += la[v];
-        lat0 /= nv;
-        double cos_lat0 = km_cos(lat0 * kDeg2Rad);
+#pragma unroll
+        for (int v = 0; v < MAXV; ++v)
+            if (v < nv) lat0 += la[v];
+        lat0 /= nd;
+        const double cos_lat0 = km_cos(lat0 * kDeg2Rad);
         const double R = 6371000.0;
-        double vx[4], vy[4];
-        for (int v = 0; v < nv; ++v) {
-            double dlon = km_pymod360(lo[v] - lon0 + 180.0) - 180.0;
-            double dlat = la[v] - lat0;
-            vx[v] = dlon * kDeg2Rad * R * cos_lat0;
-            vy[v] = dlat * kDeg2Rad * R;
+        double vx[4] = {0, 0, 0, 0}, vy[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int v = 0; v < MAXV; ++v) {
+            if (v < nv) {
+                double dlon = km_pymod360(lo[v] - lon0 + 180.0) - 180.0;
+                double dlat = la[v] - lat0;
+                vx[v] = dlon * kDeg2Rad * R * cos_lat0;
+                vy[v] = dlat * kDeg2Rad * R;
+            }
         }
-        double dlon_p = km_pymod360(xp - lon0 + 180.0) - 180.0;
-        double px = dlon_p * kDeg2Rad * R * cos_lat0;
-        double py = (yp - lat0) * kDeg2Rad * R;
-        for (int t = 0; t < nv - 2; ++t) {
-            int i1 = t + 1, i2 = t + 2;
-            double ax = vx[0], ay = vy[0], bx = vx[i1], by = vy[i1], cx = vx[i2], cy = vy[i2];
-            double denom = (by - cy) * (ax - cx) + (cx - bx) * (ay - cy);
-            if (km_fabs(denom) < 1e-14) continue;
-            double w1 = ((by - cy) * (px - cx) + (cx - bx) * (py - cy)) / denom;
-            double w2 = ((cy - ay) * (px - cx) + (ax - cx) * (py - cy)) / denom;
-            double w3 = 1.0 - w1 - w2;
-            const double eps = -1e-6;
-            if (w1 >= eps && w2 >= eps && w3 >= eps) {
-                if (nv == 4) {
-                    double A[4], C[4], wts[4];
-                    for (int q = 0; q < 4; ++q) {
-                        int q1 = (q + 1) & 3, qm = (q + 3) & 3;
-                        A[q] = ((vx[q] - px) * (vy[q1] - py) - (vx[q1] - px) * (vy[q] - py));
-                        C[q] = ((vx[q] - vx[qm]) * (vy[q1] - vy[qm]) - (vx[q1] - vx[qm]) * (vy[q] - vy[qm]));
+        const double dlon_p = km_pymod360(xp - lon0 + 180.0) - 180.0;
+        const double px = dlon_p * kDeg2Rad * R * cos_lat0;
+        const double py = (yp - lat0) * kDeg2Rad * R;
+#pragma unroll
+        for (int t = 0; t < MAXV - 2; ++t) {
+            if (t < nv - 2) {
+                const int i1 = t + 1, i2 = t + 2;
+                const double ax = vx[0], ay = vy[0], bx = vx[i1], by = vy[i1], cx = vx[i2], cy = vy[i2];
+                const double denom = (by - cy) * (ax - cx) + (cx - bx) * (ay - cy);
+                if (!(km_fabs(denom) < 1e-14)) {
+                    const double w1 = ((by - cy) * (px - cx) + (cx - bx) * (py - cy)) / denom;
+                    const double w2 = ((cy - ay) * (px - cx) + (ax - cx) * (py - cy)) / denom;
+                    const double w3 = 1.0 - w1 - w2;
+                    const double eps = -1e-6;
+                    if (w1 >= eps && w2 >= eps && w3 >= eps) {
+                        if (MAXV == 4 && nv == 4) {
+                            double wts[4];
+                            wachspress4(px, py, vx, vy, wts);
+                            double acc = 0.0;
+#pragma unroll
+                            for (int q = 0; q < 4; ++q) {
+                                const int nq = verts[q];
+                                const double dep = m.depth[nq];
+                                acc += wts[q] * w_linear(m, fw, alpha0, alpha, zp < dep ? zp : dep, nq, time_dir);
+                            }
+                            return acc;
+                        }
+                        const int n0 = verts[0], n1 = verts[i1], n2 = verts[i2];
+                        const double d0 = m.depth[n0], d1 = m.depth[n1], d2 = m.depth[n2];
+                        const double wv0 = w_linear(m, fw, alpha0, alpha, zp < d0 ? zp : d0, n0, time_dir);
+                        const double wv1 = w_linear(m, fw, alpha0, alpha, zp < d1 ? zp : d1, n1, time_dir);
+                        const double wv2 = w_linear(m, fw, alpha0, alpha, zp < d2 ? zp : d2, n2, time_dir);
+                        return w1 * wv0 + w2 * wv1 + w3 * wv2;
                     }
-                    double ssw = 0.0;
-                    for (int q = 0; q < 4; ++q) {
-                        double ww = C[q] * A[(q + 1) & 3] * A[(q + 2) & 3];
-                        wts[q] = ww;
-                        ssw += ww;
-                    }
-                    double acc = 0.0;
-                    for (int q = 0; q < 4; ++q) {
-                        int nd = verts[q];
-                        double dep = m.depth[nd];
-                        double zn = zp < dep ? zp : dep;
-                        acc += (wts[q] / ssw) * w_linear(m, fw, alpha0, alpha, zn, nd, time_dir);
-                    }
-                    return acc;
                 }
-                int n0 = verts[0], n1 = verts[i1], n2 = verts[i2];
-                double d0 = m.depth[n0], d1 = m.depth[n1], d2 = m.depth[n2];
-                double wv0 = w_linear(m, fw, alpha0, alpha, zp < d0 ? zp : d0, n0, time_dir);
-                double wv1 = w_linear(m, fw, alpha0, alpha, zp < d1 ? zp : d1, n1, time_dir);
-                double wv2 = w_linear(m, fw, alpha0, alpha, zp < d2 ? zp : d2, n2, time_dir);
-                return w1 * wv0 + w2 * wv1 + w3 * wv2;
             }
         }
     }
-    double dep = m.depth[node];
+    const double dep = m.depth[node];
     return w_linear(m, fw, alpha0, alpha, zp < dep ? zp : dep, node, time_dir);
 }
 

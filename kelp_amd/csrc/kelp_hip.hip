@@ -189,24 +189,27 @@ __device__ __forceinline__ double clamp_z(const DMesh& m, double z, int nn)
 // interpolate.py:1341/:1444 _interp_barycentric_core_tinterp(_3d):
 // containment around the anchor, vertex weights, blended vertex values of
 // the layer plane starting at `off`.
-template <typename T>
+template <int MAXV, typename T>
 __device__ inline bool interp_uv(const DMesh& m, const Fields<T>& f, double alpha0, double alpha,
                                  double time_dir, double xp, double yp, int nn, int64_t off, double& U,
                                  double& V)
 {
     U = 0.0;
     V = 0.0;
-    int e = find_containing(m, xp, yp, nn);
+    int e = find_containing<MAXV>(m, xp, yp, nn);
     if (e < 0) return false;
     Weights W;
-    if (!vertex_weights(m, e, xp, yp, W)) return false;
+    if (!vertex_weights<MAXV>(m, e, xp, yp, W)) return false;
     double su = 0.0, sv = 0.0;
-    for (int k = 0; k < W.n; ++k) {
-        int64_t idx = off + W.v[k];
-        double un = vblend(f.u, idx, alpha0, alpha);
-        double vn = vblend(f.v, idx, alpha0, alpha);
-        su += W.w[k] * un;
-        sv += W.w[k] * vn;
+#pragma unroll
+    for (int k = 0; k < MAXV; ++k) {
+        if (k < W.n) {
+            int64_t idx = off + W.v[k];
+            double un = vblend(f.u, idx, alpha0, alpha);
+            double vn = vblend(f.v, idx, alpha0, alpha);
+            su += W.w[k] * un;
+            sv += W.w[k] * vn;
+        }
     }
     U = time_dir * su;
     V = time_dir * sv;
@@ -215,20 +218,20 @@ __device__ inline bool interp_uv(const DMesh& m, const Fields<T>& f, double alph
 
 // One evaluation site: anchor, layer at the anchor (vertical.py:40), u/v on
 // that layer, W by the configured method (integrators.py:1260 _compute_w).
-template <bool IS3D, typename T>
+template <int MAXV, bool IS3D, typename T>
 __device__ inline bool eval_site(const DMesh& m, const Fields<T>& f, const StepCfg& c, double alpha,
                                  double lon, double lat, double zp, int& nn, double& U, double& V, double& W)
 {
     nn = site_anchor(m, lon, lat, c.exact);
     int64_t off = 0;
     if (IS3D) off = (int64_t)find_layer(m, zp, m.depth[nn]) * m.n_nodes;
-    bool ok = interp_uv(m, f, c.alpha0, alpha, c.time_dir, lon, lat, nn, off, U, V);
+    bool ok = interp_uv<MAXV>(m, f, c.alpha0, alpha, c.time_dir, lon, lat, nn, off, U, V);
     W = 0.0;
     if (IS3D) {
         if (c.w_method == 0)
             W = w_linear(m, f.w, c.alpha0, alpha, zp, nn, c.time_dir);
         else
-            W = w_bary(m, f.w, c.alpha0, alpha, lon, lat, zp, nn, c.time_dir);
+            W = w_bary<MAXV>(m, f.w, c.alpha0, alpha, lon, lat, zp, nn, c.time_dir);
     }
     return ok;
 }
@@ -237,7 +240,7 @@ __device__ inline bool eval_site(const DMesh& m, const Fields<T>& f, const StepC
 // integrators.py:4997 _euler_step_unfused, RK2 the fused barycentric kernels
 // :2522 (2-D) / :3648 (3-D), RK4 :2642 (2-D) and :5760 _rk4_step_unfused
 // (3-D).  Returns the failure code (0 = alive).
-template <int ORDER, bool IS3D, typename T>
+template <int ORDER, int MAXV, bool IS3D, typename T>
 __device__ inline int det_step(const DMesh& m, const Fields<T>& f, const StepCfg& c, double& lon, double& lat,
                                double& z, int& nn, double& Uo, double& Vo, double& Wo)
 {
@@ -245,7 +248,7 @@ __device__ inline int det_step(const DMesh& m, const Fields<T>& f, const StepCfg
     const double lon0 = lon, lat0 = lat, zz0 = z;
     int node_i;
     double U1, V1, W1;
-    bool ok = eval_site<IS3D>(m, f, c, c.alphas[0], lon0, lat0, zz0, node_i, U1, V1, W1);
+    bool ok = eval_site<MAXV, IS3D>(m, f, c, c.alphas[0], lon0, lat0, zz0, node_i, U1, V1, W1);
     Uo = U1;
     Vo = V1;
     Wo = W1;
@@ -262,7 +265,7 @@ __device__ inline int det_step(const DMesh& m, const Fields<T>& f, const StepCfg
             double z_mid = zz0 + W1 * 0.5 * dt;
             int nn_mid;
             double U2, V2, W2;
-            ok = eval_site<IS3D>(m, f, c, c.alphas[1], lon_mid, lat_mid, z_mid, nn_mid, U2, V2, W2);
+            ok = eval_site<MAXV, IS3D>(m, f, c, c.alphas[1], lon_mid, lat_mid, z_mid, nn_mid, U2, V2, W2);
             if (ok) {
                 double z0m = z0_of(lat_mid);
                 lon_new = wrap_lon(lon0 + dt * U2 / z0m);
@@ -278,7 +281,7 @@ __device__ inline int det_step(const DMesh& m, const Fields<T>& f, const StepCfg
             double z2 = zz0 + 0.5 * dz1;
             int nn2, nn3, nn4;
             double U2, V2, W2, U3, V3, W3, U4, V4, W4;
-            ok = eval_site<IS3D>(m, f, c, c.alphas[1], lon2, lat2, z2, nn2, U2, V2, W2);
+            ok = eval_site<MAXV, IS3D>(m, f, c, c.alphas[1], lon2, lat2, z2, nn2, U2, V2, W2);
             if (ok) {
                 double dlon2 = dt * U2 / z0_of(lat2);
                 double dlat2 = dt * V2 / kSLM;
@@ -286,7 +289,7 @@ __device__ inline int det_step(const DMesh& m, const Fields<T>& f, const StepCfg
                 double lon3 = wrap_lon(lon0 + 0.5 * dlon2);
                 double lat3 = clamp_lat(lat0 + 0.5 * dlat2);
                 double z3 = zz0 + 0.5 * dz2;
-                ok = eval_site<IS3D>(m, f, c, c.alphas[2], lon3, lat3, z3, nn3, U3, V3, W3);
+                ok = eval_site<MAXV, IS3D>(m, f, c, c.alphas[2], lon3, lat3, z3, nn3, U3, V3, W3);
                 if (ok) {
                     double dlon3 = dt * U3 / z0_of(lat3);
                     double dlat3 = dt * V3 / kSLM;
@@ -294,7 +297,7 @@ __device__ inline int det_step(const DMesh& m, const Fields<T>& f, const StepCfg
                     double lon4 = wrap_lon(lon0 + dlon3);
                     double lat4 = clamp_lat(lat0 + dlat3);
                     double z4 = zz0 + dz3;
-                    ok = eval_site<IS3D>(m, f, c, c.alphas[3], lon4, lat4, z4, nn4, U4, V4, W4);
+                    ok = eval_site<MAXV, IS3D>(m, f, c, c.alphas[3], lon4, lat4, z4, nn4, U4, V4, W4);
                     if (ok) {
                         double dlon4 = dt * U4 / z0_of(lat4);
                         double dlat4 = dt * V4 / kSLM;
@@ -315,7 +318,7 @@ __device__ inline int det_step(const DMesh& m, const Fields<T>& f, const StepCfg
         return kFrInterp;
     }
     int nn_new = site_anchor(m, lon_new, lat_new, c.exact);
-    int e = find_containing(m, lon_new, lat_new, nn_new);
+    int e = find_containing<MAXV>(m, lon_new, lat_new, nn_new);
     if (e < 0) {
         nn = node_i;
         return kFrOOB;
@@ -329,8 +332,8 @@ __device__ inline int det_step(const DMesh& m, const Fields<T>& f, const StepCfg
 
 // integrators.py:673 _apply_split_diffusion_kick for one particle alive after
 // the deterministic step (uniform Kh, counter RNG keyed by the particle id).
-template <bool IS3D>
-__device__ __noinline__ void kick_one(const DMesh& m, const StepCfg& c, uint64_t pid, double pre_lon,
+template <int MAXV, bool IS3D>
+__device__ inline void kick_one(const DMesh& m, const StepCfg& c, uint64_t pid, double pre_lon,
                                       double pre_lat, double pre_z, int pre_nn, double& lon, double& lat,
                                       double& z, int32_t& st, int& nn, int8_t& fr)
 {
@@ -353,7 +356,7 @@ __device__ __noinline__ void kick_one(const DMesh& m, const StepCfg& c, uint64_t
     double lon_k, lat_k;
     sphere_displace(lon, lat, dx, dy, lon_k, lat_k);
     int nn_k = site_anchor(m, lon_k, lat_k, c.exact == 1 ? 1 : 2);  // batch find_nearest_nodes
-    int e = find_containing(m, lon_k, lat_k, nn_k);
+    int e = find_containing<MAXV>(m, lon_k, lat_k, nn_k);
     if (e < 0) {
         lon = pre_lon;
         lat = pre_lat;
@@ -371,7 +374,8 @@ __device__ __noinline__ void kick_one(const DMesh& m, const StepCfg& c, uint64_t
 
 // tracking.py:199 _apply_beaching_hook + numerics/beaching.py:95
 // evaluate_beaching, one particle (prev = status at the step start).
-__device__ __noinline__ void beach_one(const DMesh& m, const StepCfg& c, uint64_t pid, int32_t prev, double lon,
+template <int MAXV>
+__device__ inline void beach_one(const DMesh& m, const StepCfg& c, uint64_t pid, int32_t prev, double lon,
                                        double lat, int32_t& st, int& nn, int8_t& fr, int32_t& ws)
 {
     const bool need_topo = c.use_coastal || c.use_stochastic;
@@ -380,7 +384,7 @@ __device__ __noinline__ void beach_one(const DMesh& m, const StepCfg& c, uint64_
     const int n = nn;
     int containing = -1;
     if (need_topo && (st == kAlive || st == kBeached || newly_failed) && n >= 0)
-        containing = find_containing(m, lon, lat, n);
+        containing = find_containing<MAXV>(m, lon, lat, n);
     const bool coastal = need_topo && containing >= 0 && m.coastal != nullptr && m.coastal[containing];
     const double h_raw = n >= 0 ? m.depth[n] : __builtin_inf();
     if (c.on_failure && newly_failed) {
@@ -421,8 +425,18 @@ __device__ __noinline__ void beach_one(const DMesh& m, const StepCfg& c, uint64_
 }
 
 // One full step per particle: deterministic step, split Kh kick, beaching.
-template <int ORDER, bool IS3D, typename T>
-__global__ __launch_bounds__(256) void step_kernel(DMesh m, Fields<T> f, StepCfg c, PState s)
+// Waves per SIMD the compiler budgets registers for (0: its own choice).
+#ifndef KELP_STEP_WAVES
+#define KELP_STEP_WAVES 3
+#endif
+#if KELP_STEP_WAVES > 0
+#define KELP_STEP_ATTR __attribute__((amdgpu_waves_per_eu(KELP_STEP_WAVES)))
+#else
+#define KELP_STEP_ATTR
+#endif
+
+template <int ORDER, int MAXV, bool IS3D, typename T>
+__global__ __launch_bounds__(256) KELP_STEP_ATTR void step_kernel(DMesh m, Fields<T> f, StepCfg c, PState s)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= s.n) return;
@@ -443,15 +457,15 @@ __global__ __launch_bounds__(256) void step_kernel(DMesh m, Fields<T> f, StepCfg
     if (prev == kAlive) {
         const double pre_lon = lon, pre_lat = lat, pre_z = z;
         const int pre_nn = nn;
-        int code = det_step<ORDER, IS3D>(m, f, c, lon, lat, z, nn, U, V, W);
+        int code = det_step<ORDER, MAXV, IS3D>(m, f, c, lon, lat, z, nn, U, V, W);
         if (code) {
             st = kOutside;
             fr = (int8_t)code;
         } else if (c.kick && !(c.kh_x == 0.0 && c.kh_y == 0.0)) {
-            kick_one<IS3D>(m, c, (uint64_t)s.pid[i], pre_lon, pre_lat, pre_z, pre_nn, lon, lat, z, st, nn, fr);
+            kick_one<MAXV, IS3D>(m, c, (uint64_t)s.pid[i], pre_lon, pre_lat, pre_z, pre_nn, lon, lat, z, st, nn, fr);
         }
     }
-    if (c.beach) beach_one(m, c, (uint64_t)s.pid[i], prev, lon, lat, st, nn, fr, s.ws[i]);
+    if (c.beach) beach_one<MAXV>(m, c, (uint64_t)s.pid[i], prev, lon, lat, st, nn, fr, s.ws[i]);
     s.lon[i] = lon;
     s.lat[i] = lat;
     if (IS3D) s.z[i] = z;
@@ -488,7 +502,7 @@ __global__ void containing_kernel(DMesh m, int64_t n, const double* __restrict__
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    out[i] = find_containing(m, lon[i], lat[i], nn[i]);
+    out[i] = m.max_v == 3 ? find_containing<3>(m, lon[i], lat[i], nn[i]) : find_containing<4>(m, lon[i], lat[i], nn[i]);
 }
 
 template <typename T>
@@ -511,16 +525,25 @@ static Fields<T> fields_of(const FieldSlot& a, const FieldSlot* b, const FieldSl
     return f;
 }
 
-template <bool IS3D, typename T>
-static void launch_step_t(kelp_ctx* c, int order, const Fields<T>& f, const StepCfg& cfg, const PState& ps)
+template <int MAXV, bool IS3D, typename T>
+static void launch_step_v(kelp_ctx* c, int order, const Fields<T>& f, const StepCfg& cfg, const PState& ps)
 {
     unsigned g = grid_for(ps.n, 256);
     if (order == 1)
-        step_kernel<1, IS3D, T><<<g, 256, 0, c->stream>>>(c->m, f, cfg, ps);
+        step_kernel<1, MAXV, IS3D, T><<<g, 256, 0, c->stream>>>(c->m, f, cfg, ps);
     else if (order == 2)
-        step_kernel<2, IS3D, T><<<g, 256, 0, c->stream>>>(c->m, f, cfg, ps);
+        step_kernel<2, MAXV, IS3D, T><<<g, 256, 0, c->stream>>>(c->m, f, cfg, ps);
     else
-        step_kernel<4, IS3D, T><<<g, 256, 0, c->stream>>>(c->m, f, cfg, ps);
+        step_kernel<4, MAXV, IS3D, T><<<g, 256, 0, c->stream>>>(c->m, f, cfg, ps);
+}
+
+template <bool IS3D, typename T>
+static void launch_step_t(kelp_ctx* c, int order, const Fields<T>& f, const StepCfg& cfg, const PState& ps)
+{
+    if (c->m.max_v == 3)
+        launch_step_v<3, IS3D, T>(c, order, f, cfg, ps);
+    else
+        launch_step_v<4, IS3D, T>(c, order, f, cfg, ps);
 }
 
 template <typename T>
@@ -645,6 +668,9 @@ int kelp_mesh_upload(kelp_ctx* c, const kelp_mesh_desc* d)
         return fail(KELP_EINVAL, "kelp_mesh_upload: grid hash offsets do not span the nodes");
     if (d->n_levels < 0 || (d->n_levels > 0 && !d->host_z_levels))
         return fail(KELP_EINVAL, "kelp_mesh_upload: bad vertical levels");
+    for (int64_t k = 1; k < d->n_levels; ++k)
+        if (!(d->host_z_levels[k] >= d->host_z_levels[k - 1]))
+            return fail(KELP_EINVAL, "kelp_mesh_upload: z levels must be non-decreasing");
     HIP_TRY(hipSetDevice(c->device));
     free_mesh(c);
 
