@@ -59,7 +59,7 @@ typedef struct {
     int64_t n;
     double *lon, *lat, *z;
     int32_t *status;   /* ParticleStatus: 1 alive, 2 beached, 3 outside, 4 not released */
-    int32_t *nn;       /* nearest node */
+    int32_t *nn;       /* nearest node, internal numbering (see kelp_nodes_to_mesh_ids) */
     int8_t *fr;        /* FailureReason (sticky; written on the step a particle dies) */
     double *u, *v, *w; /* diagnostic velocity at the step start (w may be NULL in 2-D) */
     int64_t *pid;      /* global particle id = RNG key; needed with the kick, beaching, sorting */
@@ -107,10 +107,17 @@ void *kelp_ctx_stream(kelp_ctx *ctx);
 int kelp_mesh_upload(kelp_ctx *ctx, const kelp_mesh_desc *desc);
 /* Device copy of the node unit-vector table, node order, (N, 3) doubles. */
 int kelp_mesh_node_xyz(kelp_ctx *ctx, double *host_out);
+/* The library numbers nodes internally (grid-hash scan order) for locality;
+ * kelp_particles.nn holds internal ids.  Translate a device array of node ids
+ * in place between the mesh's numbering and the internal one (-1 kept).  No
+ * reference counterpart (the reference works in mesh numbering throughout). */
+int kelp_nodes_to_mesh_ids(kelp_ctx *ctx, int64_t n, int32_t *ids);
+int kelp_nodes_from_mesh_ids(kelp_ctx *ctx, int64_t n, int32_t *ids);
 
 /* Field snapshot slot (the reference's VelocityField, core/io.py:180).
- * u, v: n_planes x N node values (2-D: n_planes = 1; 3-D: K-1 layers);
- * w: K x N interface values positive down or NULL.  dtype KELP_DT_*. */
+ * u, v: n_planes x N node values in mesh node order (2-D: n_planes = 1; 3-D:
+ * K-1 layers); w: K x N interface values positive down, or NULL (2-D).
+ * dtype KELP_DT_*.  Stored packed per (layer, node). */
 int kelp_field_upload(kelp_ctx *ctx, int slot, int dtype, int64_t n_planes,
                       const void *host_u, const void *host_v,
                       const void *host_w, int64_t n_w_planes);

@@ -72,6 +72,8 @@ SIGNATURES = {
     "kelp_ctx_stream": (c_void_p, [c_void_p]),
     "kelp_mesh_upload": (c_int, [c_void_p, POINTER(MeshDesc)]),
     "kelp_mesh_node_xyz": (c_int, [c_void_p, c_void_p]),
+    "kelp_nodes_to_mesh_ids": (c_int, [c_void_p, c_int64, c_void_p]),
+    "kelp_nodes_from_mesh_ids": (c_int, [c_void_p, c_int64, c_void_p]),
     "kelp_field_upload": (c_int, [c_void_p, c_int, c_int, c_int64, c_void_p, c_void_p,
                                   c_void_p, c_int64]),
     "kelp_field_clear": (c_int, [c_void_p, c_int]),

@@ -794,36 +794,105 @@ __device__ inline bool vertex_weights(const DMesh& m, int e, double xp, double y
     return W.w[0] != -1.0;
 }
 
-// Field storage types: float (the configs' float32 snapshots) or double.
-template <typename T>
-__device__ __forceinline__ double ld(const T* p, int64_t i)
-{
-    return (double)p[i];
-}
+// ---------------------------------------------------------------------------
+// Snapshot records
+//
+// A snapshot is stored as one record per (layer, node), node in the mesh's
+// internal order: 3-D (u, v, w_top, w_bottom) of layer L -- the layer's u/v and
+// the W of its two bounding interfaces L and L+1 -- as float4/double4; 2-D
+// (u, v) as float2/double2.  One vertex's stage values are one 16/32-byte load
+// instead of four scattered ones.  Values are the uploaded floats unchanged.
+// ---------------------------------------------------------------------------
 
-// One stage's field view: bracket pair (a, b) with alpha0 and target t.
-template <typename T>
-struct StageFields {
-    const T* a;
-    const T* b;
-    const T* t;
+template <typename T, bool IS3D>
+struct RecOf;
+template <>
+struct RecOf<float, true> {
+    using type = float4;
+};
+template <>
+struct RecOf<double, true> {
+    using type = double4;
+};
+template <>
+struct RecOf<float, false> {
+    using type = float2;
+};
+template <>
+struct RecOf<double, false> {
+    using type = double2;
 };
 
-// interpolate.py:1419-1437 per-vertex nested time blend
-template <typename T>
-__device__ __forceinline__ double vblend(const StageFields<T>& f, int64_t idx, double alpha0, double alpha)
+// bracket pair (a, b) blended by alpha0 and the stage target t
+template <typename T, bool IS3D>
+struct Snap {
+    using R = typename RecOf<T, IS3D>::type;
+    const R* __restrict__ a;
+    const R* __restrict__ b;
+    const R* __restrict__ t;
+};
+
+// numerics/vertical.py:1043 _tblend_scalar == interpolate.py:1419-1437: the
+// nested two-level time blend of one value
+__device__ __forceinline__ double tblend(double a, double b, double t, double alpha0, double alpha)
 {
-    double beta0 = 1.0 - alpha0, beta = 1.0 - alpha;
     if (alpha < 1e-12) {
-        if (alpha0 == 0.0) return ld(f.a, idx);
-        return beta0 * ld(f.a, idx) + alpha0 * ld(f.b, idx);
-    } else if (alpha > 1.0 - 1e-12) {
-        return ld(f.t, idx);
-    } else if (alpha0 == 0.0) {
-        return beta * ld(f.a, idx) + alpha * ld(f.t, idx);
+        if (alpha0 == 0.0) return a;
+        return (1.0 - alpha0) * a + alpha0 * b;
     }
-    double a0 = beta0 * ld(f.a, idx) + alpha0 * ld(f.b, idx);
-    return beta * a0 + alpha * ld(f.t, idx);
+    if (alpha > 1.0 - 1e-12) return t;
+    if (alpha0 == 0.0) return (1.0 - alpha) * a + alpha * t;
+    double a0 = (1.0 - alpha0) * a + alpha0 * b;
+    return (1.0 - alpha) * a0 + alpha * t;
+}
+
+// blended (u, v) of record idx; only the snapshots the blend uses are read
+template <typename T, bool IS3D>
+__device__ __forceinline__ void uv_blend(const Snap<T, IS3D>& f, int64_t idx, double alpha0, double alpha,
+                                         double& u, double& v)
+{
+    if (alpha < 1e-12) {
+        auto ra = f.a[idx];
+        if (alpha0 == 0.0) {
+            u = (double)ra.x;
+            v = (double)ra.y;
+            return;
+        }
+        auto rb = f.b[idx];
+        u = (1.0 - alpha0) * (double)ra.x + alpha0 * (double)rb.x;
+        v = (1.0 - alpha0) * (double)ra.y + alpha0 * (double)rb.y;
+        return;
+    }
+    if (alpha > 1.0 - 1e-12) {
+        auto rt = f.t[idx];
+        u = (double)rt.x;
+        v = (double)rt.y;
+        return;
+    }
+    auto ra = f.a[idx];
+    auto rt = f.t[idx];
+    if (alpha0 == 0.0) {
+        u = (1.0 - alpha) * (double)ra.x + alpha * (double)rt.x;
+        v = (1.0 - alpha) * (double)ra.y + alpha * (double)rt.y;
+        return;
+    }
+    auto rb = f.b[idx];
+    double ua0 = (1.0 - alpha0) * (double)ra.x + alpha0 * (double)rb.x;
+    double va0 = (1.0 - alpha0) * (double)ra.y + alpha0 * (double)rb.y;
+    u = (1.0 - alpha) * ua0 + alpha * (double)rt.x;
+    v = (1.0 - alpha) * va0 + alpha * (double)rt.y;
+}
+
+// blended W of the two interfaces bounding record idx (w_top, w_bottom)
+template <typename T>
+__device__ __forceinline__ void w_pair_blend(const Snap<T, true>& f, int64_t idx, double alpha0, double alpha,
+                                             double& w_top, double& w_bot)
+{
+    auto ra = f.a[idx];
+    auto rb = f.b[idx];
+    auto rt = f.t[idx];
+    w_top = tblend((double)ra.z, (double)rb.z, (double)rt.z, alpha0, alpha);
+    w_bot = tblend((double)ra.w, (double)rb.w, (double)rt.w, alpha0, alpha);
 }
 
 // ---------------------------------------------------------------------------
@@ -861,44 +930,33 @@ __device__ inline int find_layer(const DMesh& m, double zp, double bottom)
     return r <= max_layer ? r : max_layer;
 }
 
-// numerics/vertical.py:1043 _tblend_scalar
-__device__ __forceinline__ double tblend(double a, double b, double t, double alpha0, double alpha)
-{
-    if (alpha < 1e-12) {
-        if (alpha0 == 0.0) return a;
-        return (1.0 - alpha0) * a + alpha0 * b;
-    }
-    if (alpha > 1.0 - 1e-12) return t;
-    if (alpha0 == 0.0) return (1.0 - alpha) * a + alpha * t;
-    double a0 = (1.0 - alpha0) * a + alpha0 * b;
-    return (1.0 - alpha) * a0 + alpha * t;
-}
-
+// numerics/vertical.py:1065 _interp_w_linear_single_tinterp: W linear in
+// depth between the interfaces bracketing zp at one node (records hold the
+// interface pair of each layer: level k < K-1 is record k's top, level K-1 the
+// last record's bottom)
 template <typename T>
-__device__ __forceinline__ double wlev(const StageFields<T>& fw, int64_t idx, double alpha0, double alpha)
-{
-    return tblend(ld(fw.a, idx), ld(fw.b, idx), ld(fw.t, idx), alpha0, alpha);
-}
-
-// numerics/vertical.py:1065 _interp_w_linear_single_tinterp
-template <typename T>
-__device__ inline double w_linear(const DMesh& m, const StageFields<T>& fw, double alpha0, double alpha,
+__device__ inline double w_linear(const DMesh& m, const Snap<T, true>& fw, double alpha0, double alpha,
                                   double zp, int node, double time_dir)
 {
     const double* zl = m.zlev;
     const int K = m.n_levels;
     const int64_t N = m.n_nodes;
-    if (K < 2) return time_dir * wlev(fw, node, alpha0, alpha);
-    if (zp <= zl[0]) return time_dir * wlev(fw, node, alpha0, alpha);
-    if (zp >= zl[K - 1]) return time_dir * wlev(fw, (int64_t)(K - 1) * N + node, alpha0, alpha);
+    double wt, wb;
+    if (zp <= zl[0]) {
+        w_pair_blend(fw, node, alpha0, alpha, wt, wb);
+        return time_dir * wt;
+    }
+    if (zp >= zl[K - 1]) {
+        w_pair_blend(fw, (int64_t)(K - 2) * N + node, alpha0, alpha, wt, wb);
+        return time_dir * wb;
+    }
     int k = first_above(m, zp);  // < K - 1 here unless zp is NaN
     if (k > K - 2) k = K - 2;
+    w_pair_blend(fw, (int64_t)k * N + node, alpha0, alpha, wt, wb);
     double dz = zl[k + 1] - zl[k];
-    if (dz < 1e-12) return time_dir * wlev(fw, (int64_t)k * N + node, alpha0, alpha);
+    if (dz < 1e-12) return time_dir * wt;
     double a = (zp - zl[k]) / dz;
-    double w_lo = wlev(fw, (int64_t)k * N + node, alpha0, alpha);
-    double w_hi = wlev(fw, (int64_t)(k + 1) * N + node, alpha0, alpha);
-    return time_dir * ((1.0 - a) * w_lo + a * w_hi);
+    return time_dir * ((1.0 - a) * wt + a * wb);
 }
 
 // numerics/vertical.py:745 _interp_w_bary_single_tinterp: first fan
@@ -906,7 +964,7 @@ __device__ inline double w_linear(const DMesh& m, const StageFields<T>& fw, doub
 // seam-safe vertex mean, core/mesh.py:1974), bottom-aware linear W per vertex;
 // fallback = linear W at the anchor, depth clamped to its bottom.
 template <int MAXV, typename T>
-__device__ inline double w_bary(const DMesh& m, const StageFields<T>& fw, double alpha0, double alpha, double xp,
+__device__ inline double w_bary(const DMesh& m, const Snap<T, true>& fw, double alpha0, double alpha, double xp,
                                 double yp, double zp, int node, double time_dir)
 {
     const int k0 = m.n2e_off[node], k1 = m.n2e_off[node + 1];

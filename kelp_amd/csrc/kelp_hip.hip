@@ -14,6 +14,8 @@
 #include <new>
 #include <vector>
 
+#include <algorithm>
+
 #include "kelp_internal.h"
 
 using namespace kelp;
@@ -64,13 +66,12 @@ static void free_mesh(kelp_ctx* c)
     c->mesh_bufs.clear();
     c->has_mesh = false;
     c->m = DMesh{};
+    c->node_old_of_new = c->node_new_of_old = c->elem_old_of_new = c->elem_new_of_old = nullptr;
 }
 
 static void free_slot(FieldSlot& s)
 {
-    if (s.u) (void)hipFree(s.u);
-    if (s.v) (void)hipFree(s.v);
-    if (s.w) (void)hipFree(s.w);
+    if (s.rec) (void)hipFree(s.rec);
     s = FieldSlot{};
 }
 
@@ -90,34 +91,74 @@ static int dev_alloc(kelp_ctx* c, T** out, size_t count)
 // mesh build kernels
 // ---------------------------------------------------------------------------
 
-// core/mesh.py:713 _node_xyz_fill, written straight into the CSR order of
-// the hash (slot k holds node gh_node_ids[k]).
-__global__ void node_xyz_csr_kernel(int64_t n, const double2* __restrict__ lonlat,
-                                    const int32_t* __restrict__ ids, double4* __restrict__ out)
+// core/mesh.py:713 _node_xyz_fill.  Internal node k is the node in slot k of
+// the hash's CSR order, so the table is already in scan order.
+__global__ void node_xyz_kernel(int64_t n, const double2* __restrict__ lonlat, double4* __restrict__ out)
 {
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    int32_t node = ids[k];
-    double2 ll = lonlat[node];
+    double2 ll = lonlat[k];
     double lat_r = ll.y * kDeg2Rad, lon_r = ll.x * kDeg2Rad;
     double c = km_cos(lat_r);
     double4 r;
     r.x = c * km_cos(lon_r);
     r.y = c * km_sin(lon_r);
     r.z = km_sin(lat_r);
-    r.w = __longlong_as_double((long long)node);
+    r.w = __longlong_as_double((long long)k);
     out[k] = r;
 }
 
-__global__ void gather_xyz_kernel(int64_t n, const double4* __restrict__ csr, double* __restrict__ out)
+// node unit vectors back in mesh (original) node order, (N, 3)
+__global__ void gather_xyz_kernel(int64_t n, const double4* __restrict__ xyz, const int32_t* __restrict__ old_of_new,
+                                  double* __restrict__ out)
 {
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    double4 p = csr[k];
-    int32_t node = node_id_of(p);
-    out[3 * (int64_t)node + 0] = p.x;
-    out[3 * (int64_t)node + 1] = p.y;
-    out[3 * (int64_t)node + 2] = p.z;
+    double4 p = xyz[k];
+    int64_t node = old_of_new[k];
+    out[3 * node + 0] = p.x;
+    out[3 * node + 1] = p.y;
+    out[3 * node + 2] = p.z;
+}
+
+// pack a snapshot's planes (mesh node order) into records (internal order)
+template <typename T>
+__global__ void pack3d_kernel(int64_t N, int64_t n_layers, const int32_t* __restrict__ old_of_new,
+                              const T* __restrict__ u, const T* __restrict__ v, const T* __restrict__ w,
+                              typename RecOf<T, true>::type* __restrict__ rec)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N * n_layers) return;
+    int64_t L = i / N, k = i - L * N;
+    int64_t o = old_of_new[k];
+    typename RecOf<T, true>::type r;
+    r.x = u[L * N + o];
+    r.y = v[L * N + o];
+    r.z = w[L * N + o];
+    r.w = w[(L + 1) * N + o];
+    rec[i] = r;
+}
+
+template <typename T>
+__global__ void pack2d_kernel(int64_t N, const int32_t* __restrict__ old_of_new, const T* __restrict__ u,
+                              const T* __restrict__ v, typename RecOf<T, false>::type* __restrict__ rec)
+{
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    int64_t o = old_of_new[k];
+    typename RecOf<T, false>::type r;
+    r.x = u[o];
+    r.y = v[o];
+    rec[k] = r;
+}
+
+// in-place id translation through a table (-1 stays -1)
+__global__ void map_ids_kernel(int64_t n, const int32_t* __restrict__ table, int32_t* __restrict__ ids)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int32_t x = ids[i];
+    if (x >= 0) ids[i] = table[x];
 }
 
 static unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
@@ -157,11 +198,6 @@ struct PState {
     int64_t n;
 };
 
-template <typename T>
-struct Fields {
-    StageFields<T> u, v, w;
-};
-
 __device__ __forceinline__ double wrap_lon(double x) { return km_pymod360(x + 180.0) - 180.0; }
 
 __device__ __forceinline__ double clamp_lat(double y)
@@ -189,8 +225,8 @@ __device__ __forceinline__ double clamp_z(const DMesh& m, double z, int nn)
 // interpolate.py:1341/:1444 _interp_barycentric_core_tinterp(_3d):
 // containment around the anchor, vertex weights, blended vertex values of
 // the layer plane starting at `off`.
-template <int MAXV, typename T>
-__device__ inline bool interp_uv(const DMesh& m, const Fields<T>& f, double alpha0, double alpha,
+template <int MAXV, typename T, bool IS3D>
+__device__ inline bool interp_uv(const DMesh& m, const Snap<T, IS3D>& f, double alpha0, double alpha,
                                  double time_dir, double xp, double yp, int nn, int64_t off, double& U,
                                  double& V)
 {
@@ -204,9 +240,8 @@ __device__ inline bool interp_uv(const DMesh& m, const Fields<T>& f, double alph
 #pragma unroll
     for (int k = 0; k < MAXV; ++k) {
         if (k < W.n) {
-            int64_t idx = off + W.v[k];
-            double un = vblend(f.u, idx, alpha0, alpha);
-            double vn = vblend(f.v, idx, alpha0, alpha);
+            double un, vn;
+            uv_blend(f, off + W.v[k], alpha0, alpha, un, vn);
             su += W.w[k] * un;
             sv += W.w[k] * vn;
         }
@@ -219,7 +254,7 @@ __device__ inline bool interp_uv(const DMesh& m, const Fields<T>& f, double alph
 // One evaluation site: anchor, layer at the anchor (vertical.py:40), u/v on
 // that layer, W by the configured method (integrators.py:1260 _compute_w).
 template <int MAXV, bool IS3D, typename T>
-__device__ inline bool eval_site(const DMesh& m, const Fields<T>& f, const StepCfg& c, double alpha,
+__device__ inline bool eval_site(const DMesh& m, const Snap<T, IS3D>& f, const StepCfg& c, double alpha,
                                  double lon, double lat, double zp, int& nn, double& U, double& V, double& W)
 {
     nn = site_anchor(m, lon, lat, c.exact);
@@ -227,11 +262,11 @@ __device__ inline bool eval_site(const DMesh& m, const Fields<T>& f, const StepC
     if (IS3D) off = (int64_t)find_layer(m, zp, m.depth[nn]) * m.n_nodes;
     bool ok = interp_uv<MAXV>(m, f, c.alpha0, alpha, c.time_dir, lon, lat, nn, off, U, V);
     W = 0.0;
-    if (IS3D) {
+    if constexpr (IS3D) {
         if (c.w_method == 0)
-            W = w_linear(m, f.w, c.alpha0, alpha, zp, nn, c.time_dir);
+            W = w_linear(m, f, c.alpha0, alpha, zp, nn, c.time_dir);
         else
-            W = w_bary<MAXV>(m, f.w, c.alpha0, alpha, lon, lat, zp, nn, c.time_dir);
+            W = w_bary<MAXV>(m, f, c.alpha0, alpha, lon, lat, zp, nn, c.time_dir);
     }
     return ok;
 }
@@ -241,7 +276,7 @@ __device__ inline bool eval_site(const DMesh& m, const Fields<T>& f, const StepC
 // :2522 (2-D) / :3648 (3-D), RK4 :2642 (2-D) and :5760 _rk4_step_unfused
 // (3-D).  Returns the failure code (0 = alive).
 template <int ORDER, int MAXV, bool IS3D, typename T>
-__device__ inline int det_step(const DMesh& m, const Fields<T>& f, const StepCfg& c, double& lon, double& lat,
+__device__ inline int det_step(const DMesh& m, const Snap<T, IS3D>& f, const StepCfg& c, double& lon, double& lat,
                                double& z, int& nn, double& Uo, double& Vo, double& Wo)
 {
     const double dt = c.dt;
@@ -436,7 +471,7 @@ __device__ inline void beach_one(const DMesh& m, const StepCfg& c, uint64_t pid,
 #endif
 
 template <int ORDER, int MAXV, bool IS3D, typename T>
-__global__ __launch_bounds__(256) KELP_STEP_ATTR void step_kernel(DMesh m, Fields<T> f, StepCfg c, PState s)
+__global__ __launch_bounds__(256) KELP_STEP_ATTR void step_kernel(DMesh m, Snap<T, IS3D> f, StepCfg c, PState s)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= s.n) return;
@@ -477,92 +512,36 @@ __global__ __launch_bounds__(256) KELP_STEP_ATTR void step_kernel(DMesh m, Field
     if (IS3D && s.w) s.w[i] = W;
 }
 
+// batch anchors (core/mesh.py:1896 find_nearest_nodes), mesh node ids out
 __global__ void nearest_kernel(DMesh m, int64_t n, const double* __restrict__ lon,
-                               const double* __restrict__ lat, int exact, int32_t* __restrict__ out)
+                               const double* __restrict__ lat, int exact, const int32_t* __restrict__ old_of_new,
+                               int32_t* __restrict__ out)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    if (exact) {
-        out[i] = nearest_exact(m, lon[i], lat[i]);
-    } else {
-        // guard off: the plain 3x3 window; an empty window is healed by the
-        // exact search (the reference heals it with a KD-tree query).
-        int ci, cj;
-        gh_cell_of(m, lon[i], lat[i], ci, cj);
-        Query q = make_query(lon[i], lat[i]);
-        double d2;
-        int b = nearest_3x3(m, ci, cj, q, d2);
-        out[i] = b >= 0 ? b : nearest_exact(m, lon[i], lat[i]);
-    }
+    // guard off: the plain 3x3 window, an empty window healed exactly (the
+    // reference heals it with a KD-tree query)
+    int b = site_anchor(m, lon[i], lat[i], exact ? 1 : 2);
+    out[i] = b >= 0 ? old_of_new[b] : -1;
 }
 
+// batch containment (core/mesh.py:2267); mesh node ids in, mesh element ids out
+template <int MAXV>
 __global__ void containing_kernel(DMesh m, int64_t n, const double* __restrict__ lon,
                                   const double* __restrict__ lat, const int32_t* __restrict__ nn,
-                                  int32_t* __restrict__ out)
+                                  const int32_t* __restrict__ node_new_of_old,
+                                  const int32_t* __restrict__ elem_old_of_new, int32_t* __restrict__ out)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    out[i] = m.max_v == 3 ? find_containing<3>(m, lon[i], lat[i], nn[i]) : find_containing<4>(m, lon[i], lat[i], nn[i]);
+    int nd = nn[i];
+    if (nd < 0 || nd >= m.n_nodes) {
+        out[i] = -1;
+        return;
+    }
+    int e = find_containing<MAXV>(m, lon[i], lat[i], node_new_of_old[nd]);
+    out[i] = e >= 0 ? elem_old_of_new[e] : -1;
 }
-
-template <typename T>
-static StageFields<T> stage_of(const void* a, const void* b, const void* t)
-{
-    StageFields<T> f;
-    f.a = static_cast<const T*>(a);
-    f.b = b ? static_cast<const T*>(b) : f.a;
-    f.t = t ? static_cast<const T*>(t) : f.a;
-    return f;
-}
-
-template <typename T>
-static Fields<T> fields_of(const FieldSlot& a, const FieldSlot* b, const FieldSlot* t)
-{
-    Fields<T> f;
-    f.u = stage_of<T>(a.u, b ? b->u : nullptr, t ? t->u : nullptr);
-    f.v = stage_of<T>(a.v, b ? b->v : nullptr, t ? t->v : nullptr);
-    f.w = stage_of<T>(a.w, b ? b->w : nullptr, t ? t->w : nullptr);
-    return f;
-}
-
-template <int MAXV, bool IS3D, typename T>
-static void launch_step_v(kelp_ctx* c, int order, const Fields<T>& f, const StepCfg& cfg, const PState& ps)
-{
-    unsigned g = grid_for(ps.n, 256);
-    if (order == 1)
-        step_kernel<1, MAXV, IS3D, T><<<g, 256, 0, c->stream>>>(c->m, f, cfg, ps);
-    else if (order == 2)
-        step_kernel<2, MAXV, IS3D, T><<<g, 256, 0, c->stream>>>(c->m, f, cfg, ps);
-    else
-        step_kernel<4, MAXV, IS3D, T><<<g, 256, 0, c->stream>>>(c->m, f, cfg, ps);
-}
-
-template <bool IS3D, typename T>
-static void launch_step_t(kelp_ctx* c, int order, const Fields<T>& f, const StepCfg& cfg, const PState& ps)
-{
-    if (c->m.max_v == 3)
-        launch_step_v<3, IS3D, T>(c, order, f, cfg, ps);
-    else
-        launch_step_v<4, IS3D, T>(c, order, f, cfg, ps);
-}
-
-template <typename T>
-static void launch_step(kelp_ctx* c, int order, bool is3d, const FieldSlot& a, const FieldSlot* b,
-                        const FieldSlot* t, const StepCfg& cfg, const PState& ps)
-{
-    Fields<T> f = fields_of<T>(a, b, t);
-    if (is3d)
-        launch_step_t<true, T>(c, order, f, cfg, ps);
-    else
-        launch_step_t<false, T>(c, order, f, cfg, ps);
-}
-
-static const FieldSlot* opt_slot(kelp_ctx* c, int k)
-{
-    if (k < 0) return nullptr;
-    return &c->slots[k];
-}
-
 
 // Elementwise evaluation of the device math routines (diagnostics: lets a
 // test compare them with a host build of the same kmath.h).
@@ -584,6 +563,56 @@ __global__ void math_probe_kernel(int fn, int64_t n, const double* __restrict__ 
     default: r = km_sqrt(a); break;
     }
     out[i] = r;
+}
+
+template <typename T, bool IS3D>
+static Snap<T, IS3D> snap_of(const FieldSlot& a, const FieldSlot* b, const FieldSlot* t)
+{
+    using R = typename RecOf<T, IS3D>::type;
+    Snap<T, IS3D> f;
+    f.a = static_cast<const R*>(a.rec);
+    f.b = b ? static_cast<const R*>(b->rec) : f.a;
+    f.t = t ? static_cast<const R*>(t->rec) : f.a;
+    return f;
+}
+
+template <int MAXV, bool IS3D, typename T>
+static void launch_step_v(kelp_ctx* c, int order, const Snap<T, IS3D>& f, const StepCfg& cfg, const PState& ps)
+{
+    unsigned g = grid_for(ps.n, 256);
+    if (order == 1)
+        step_kernel<1, MAXV, IS3D, T><<<g, 256, 0, c->stream>>>(c->m, f, cfg, ps);
+    else if (order == 2)
+        step_kernel<2, MAXV, IS3D, T><<<g, 256, 0, c->stream>>>(c->m, f, cfg, ps);
+    else
+        step_kernel<4, MAXV, IS3D, T><<<g, 256, 0, c->stream>>>(c->m, f, cfg, ps);
+}
+
+template <bool IS3D, typename T>
+static void launch_step_t(kelp_ctx* c, int order, const FieldSlot& a, const FieldSlot* b, const FieldSlot* t,
+                          const StepCfg& cfg, const PState& ps)
+{
+    Snap<T, IS3D> f = snap_of<T, IS3D>(a, b, t);
+    if (c->m.max_v == 3)
+        launch_step_v<3, IS3D, T>(c, order, f, cfg, ps);
+    else
+        launch_step_v<4, IS3D, T>(c, order, f, cfg, ps);
+}
+
+template <typename T>
+static void launch_step(kelp_ctx* c, int order, bool is3d, const FieldSlot& a, const FieldSlot* b,
+                        const FieldSlot* t, const StepCfg& cfg, const PState& ps)
+{
+    if (is3d)
+        launch_step_t<true, T>(c, order, a, b, t, cfg, ps);
+    else
+        launch_step_t<false, T>(c, order, a, b, t, cfg, ps);
+}
+
+static const FieldSlot* opt_slot(kelp_ctx* c, int k)
+{
+    if (k < 0) return nullptr;
+    return &c->slots[k];
 }
 
 // ---------------------------------------------------------------------------
@@ -674,78 +703,112 @@ int kelp_mesh_upload(kelp_ctx* c, const kelp_mesh_desc* d)
     HIP_TRY(hipSetDevice(c->device));
     free_mesh(c);
 
-    // host staging in the device layout
-    std::vector<double2> lonlat(N);
-    for (int64_t n = 0; n < N; ++n) {
-        lonlat[n].x = d->host_nodes_lon[n];
-        lonlat[n].y = d->host_nodes_lat[n];
+    // Internal numbering.  Node k is the node in slot k of the hash's CSR
+    // order (cells row-major, ascending mesh id inside a cell), so a cell's
+    // nodes, a window's rows and the mesh neighbours of a region sit close in
+    // memory.  Elements follow their lowest internal vertex.  Every order the
+    // reference scans in is kept: the CSR slot order (nearest-node ties),
+    // each node's element list in ascending mesh element id (first containing
+    // element), each element's vertex slots (rings, fans).
+    std::vector<int32_t> old_of_new(N), new_of_old(N, -1);
+    for (int64_t k = 0; k < N; ++k) {
+        int32_t o = d->host_gh_node_ids[k];
+        if (o < 0 || o >= N || new_of_old[o] >= 0)
+            return fail(KELP_EINVAL, "kelp_mesh_upload: grid hash node ids are not a permutation");
+        old_of_new[k] = o;
+        new_of_old[o] = (int32_t)k;
     }
-    std::vector<int4> elems(E);
+    std::vector<int64_t> ekey(E);
     for (int64_t e = 0; e < E; ++e) {
-        int v[4] = {-1, -1, -1, -1};
+        int64_t lo = INT64_MAX;
         for (int k = 0; k < d->max_v; ++k) {
             int x = d->host_elem_nodes[e * d->max_v + k];
-            if (x < -1 || x >= N) return fail(KELP_EINVAL, "kelp_mesh_upload: elem %lld vertex %d out of range",
-                                              (long long)e, x);
-            v[k] = x;
+            if (x < -1 || x >= N)
+                return fail(KELP_EINVAL, "kelp_mesh_upload: elem %lld vertex %d out of range", (long long)e, x);
+            if (x >= 0 && new_of_old[x] < lo) lo = new_of_old[x];
         }
-        elems[e] = make_int4(v[0], v[1], v[2], v[3]);
+        ekey[e] = lo;
+    }
+    std::vector<int32_t> e_old_of_new(E), e_new_of_old(E);
+    for (int64_t e = 0; e < E; ++e) e_old_of_new[e] = (int32_t)e;
+    std::stable_sort(e_old_of_new.begin(), e_old_of_new.end(),
+                     [&](int32_t a, int32_t b) { return ekey[a] < ekey[b]; });
+    for (int64_t e = 0; e < E; ++e) e_new_of_old[e_old_of_new[e]] = (int32_t)e;
+
+    std::vector<double2> lonlat(N);
+    std::vector<double> depth(N, 0.0);
+    for (int64_t k = 0; k < N; ++k) {
+        int64_t o = old_of_new[k];
+        lonlat[k].x = d->host_nodes_lon[o];
+        lonlat[k].y = d->host_nodes_lat[o];
+        if (d->host_depth) depth[k] = d->host_depth[o];
+    }
+    std::vector<int4> elems(E);
+    std::vector<uint8_t> coastal(d->host_elem_coastal ? E : 0);
+    for (int64_t en = 0; en < E; ++en) {
+        int64_t eo = e_old_of_new[en];
+        int v[4] = {-1, -1, -1, -1};
+        for (int k = 0; k < d->max_v; ++k) {
+            int x = d->host_elem_nodes[eo * d->max_v + k];
+            v[k] = x >= 0 ? new_of_old[x] : -1;
+        }
+        elems[en] = make_int4(v[0], v[1], v[2], v[3]);
+        if (d->host_elem_coastal) coastal[en] = d->host_elem_coastal[eo] ? 1 : 0;
     }
     std::vector<int32_t> n2e_off(N + 1);
     n2e_off[0] = 0;
-    for (int64_t n = 0; n < N; ++n) {
-        int32_t cnt = d->host_node_to_elems_count[n];
+    for (int64_t k = 0; k < N; ++k) {
+        int64_t o = old_of_new[k];
+        int32_t cnt = d->host_node_to_elems_count[o];
         if (cnt < 0 || cnt > d->max_conn) return fail(KELP_EINVAL, "kelp_mesh_upload: bad node_to_elems_count");
         int32_t valid = 0;
-        for (int32_t k = 0; k < cnt; ++k)
-            if (d->host_node_to_elems[n * d->max_conn + k] >= 0) ++valid;
-        n2e_off[n + 1] = n2e_off[n] + valid;
+        for (int32_t j = 0; j < cnt; ++j)
+            if (d->host_node_to_elems[o * d->max_conn + j] >= 0) ++valid;
+        n2e_off[k + 1] = n2e_off[k] + valid;
     }
     std::vector<int32_t> n2e(n2e_off[N] > 0 ? n2e_off[N] : 1);
-    for (int64_t n = 0; n < N; ++n) {
-        int32_t w = n2e_off[n];
-        for (int32_t k = 0; k < d->host_node_to_elems_count[n]; ++k) {
-            int32_t e = d->host_node_to_elems[n * d->max_conn + k];
+    for (int64_t k = 0; k < N; ++k) {
+        int64_t o = old_of_new[k];
+        int32_t w = n2e_off[k];
+        for (int32_t j = 0; j < d->host_node_to_elems_count[o]; ++j) {
+            int32_t e = d->host_node_to_elems[o * d->max_conn + j];
             if (e >= E) return fail(KELP_EINVAL, "kelp_mesh_upload: node_to_elems entry out of range");
-            if (e >= 0) n2e[w++] = e;
+            if (e >= 0) n2e[w++] = e_new_of_old[e];
         }
     }
-    for (int64_t k = 0; k < N; ++k)
-        if (d->host_gh_node_ids[k] < 0 || d->host_gh_node_ids[k] >= N)
-            return fail(KELP_EINVAL, "kelp_mesh_upload: grid hash node id out of range");
 
     DMesh m{};
     double2* d_lonlat;
     double4* d_xyz;
-    int32_t *d_off, *d_ids, *d_n2e_off, *d_n2e;
+    int32_t *d_off, *d_n2e_off, *d_n2e, *d_nodeo, *d_noden, *d_elemo, *d_elemn;
     int4* d_elems;
     double *d_depth, *d_zlev;
     uint8_t* d_coastal = nullptr;
     int rc;
     if ((rc = dev_alloc(c, &d_lonlat, N)) || (rc = dev_alloc(c, &d_xyz, N)) ||
-        (rc = dev_alloc(c, &d_off, n_cells + 1)) || (rc = dev_alloc(c, &d_ids, N)) ||
-        (rc = dev_alloc(c, &d_elems, E)) || (rc = dev_alloc(c, &d_n2e_off, N + 1)) ||
-        (rc = dev_alloc(c, &d_n2e, n2e.size())) || (rc = dev_alloc(c, &d_depth, N)) ||
-        (rc = dev_alloc(c, &d_zlev, d->n_levels > 0 ? d->n_levels : 1)) ||
+        (rc = dev_alloc(c, &d_off, n_cells + 1)) || (rc = dev_alloc(c, &d_elems, E)) ||
+        (rc = dev_alloc(c, &d_n2e_off, N + 1)) || (rc = dev_alloc(c, &d_n2e, n2e.size())) ||
+        (rc = dev_alloc(c, &d_depth, N)) || (rc = dev_alloc(c, &d_zlev, d->n_levels > 0 ? d->n_levels : 1)) ||
+        (rc = dev_alloc(c, &d_nodeo, N)) || (rc = dev_alloc(c, &d_noden, N)) ||
+        (rc = dev_alloc(c, &d_elemo, E)) || (rc = dev_alloc(c, &d_elemn, E)) ||
         (d->host_elem_coastal && (rc = dev_alloc(c, &d_coastal, E)))) {
         free_mesh(c);
         return rc;
     }
-    std::vector<double> depth(N, 0.0);
-    if (d->host_depth)
-        for (int64_t n = 0; n < N; ++n) depth[n] = d->host_depth[n];
     HIP_TRY(hipMemcpy(d_lonlat, lonlat.data(), N * sizeof(double2), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(d_off, d->host_gh_offsets, (n_cells + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d_ids, d->host_gh_node_ids, N * sizeof(int32_t), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(d_elems, elems.data(), E * sizeof(int4), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(d_n2e_off, n2e_off.data(), (N + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(d_n2e, n2e.data(), n2e.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(d_depth, depth.data(), N * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d_nodeo, old_of_new.data(), N * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d_noden, new_of_old.data(), N * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d_elemo, e_old_of_new.data(), E * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d_elemn, e_new_of_old.data(), E * sizeof(int32_t), hipMemcpyHostToDevice));
     if (d->n_levels > 0)
         HIP_TRY(hipMemcpy(d_zlev, d->host_z_levels, d->n_levels * sizeof(double), hipMemcpyHostToDevice));
-    if (d_coastal)
-        HIP_TRY(hipMemcpy(d_coastal, d->host_elem_coastal, E * sizeof(uint8_t), hipMemcpyHostToDevice));
-    node_xyz_csr_kernel<<<grid_for(N, 256), 256, 0, c->stream>>>(N, d_lonlat, d_ids, d_xyz);
+    if (d_coastal) HIP_TRY(hipMemcpy(d_coastal, coastal.data(), E * sizeof(uint8_t), hipMemcpyHostToDevice));
+    node_xyz_kernel<<<grid_for(N, 256), 256, 0, c->stream>>>(N, d_lonlat, d_xyz);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(c->stream));
 
@@ -771,6 +834,10 @@ int kelp_mesh_upload(kelp_ctx* c, const kelp_mesh_desc* d)
     m.max_v = (int32_t)d->max_v;
     m.n_levels = (int32_t)d->n_levels;
     c->m = m;
+    c->node_old_of_new = d_nodeo;
+    c->node_new_of_old = d_noden;
+    c->elem_old_of_new = d_elemo;
+    c->elem_new_of_old = d_elemn;
     c->has_mesh = true;
     return KELP_OK;
 }
@@ -782,12 +849,32 @@ int kelp_mesh_node_xyz(kelp_ctx* c, double* host_out)
     const int64_t N = c->m.n_nodes;
     double* tmp = nullptr;
     HIP_TRY(hipMalloc(&tmp, N * 3 * sizeof(double)));
-    gather_xyz_kernel<<<grid_for(N, 256), 256, 0, c->stream>>>(N, c->m.gh_xyz, tmp);
+    gather_xyz_kernel<<<grid_for(N, 256), 256, 0, c->stream>>>(N, c->m.gh_xyz, c->node_old_of_new, tmp);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(host_out, tmp, N * 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     (void)hipFree(tmp);
     if (e != hipSuccess) return fail(KELP_EHIP, "kelp_mesh_node_xyz: %s", hipGetErrorString(e));
+    return KELP_OK;
+}
+
+int kelp_nodes_to_mesh_ids(kelp_ctx* c, int64_t n, int32_t* ids)
+{
+    if (!c || n < 0 || (n > 0 && !ids)) return fail(KELP_EINVAL, "kelp_nodes_to_mesh_ids: bad argument");
+    if (!c->has_mesh) return fail(KELP_ESTATE, "kelp_nodes_to_mesh_ids: no mesh uploaded");
+    if (n == 0) return KELP_OK;
+    map_ids_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(n, c->node_old_of_new, ids);
+    HIP_TRY(hipGetLastError());
+    return KELP_OK;
+}
+
+int kelp_nodes_from_mesh_ids(kelp_ctx* c, int64_t n, int32_t* ids)
+{
+    if (!c || n < 0 || (n > 0 && !ids)) return fail(KELP_EINVAL, "kelp_nodes_from_mesh_ids: bad argument");
+    if (!c->has_mesh) return fail(KELP_ESTATE, "kelp_nodes_from_mesh_ids: no mesh uploaded");
+    if (n == 0) return KELP_OK;
+    map_ids_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(n, c->node_new_of_old, ids);
+    HIP_TRY(hipGetLastError());
     return KELP_OK;
 }
 
@@ -799,25 +886,55 @@ int kelp_field_upload(kelp_ctx* c, int slot, int dtype, int64_t n_planes, const 
     if (slot < 0 || slot >= kMaxSlots) return fail(KELP_EINVAL, "kelp_field_upload: slot %d not in [0, %d)", slot, kMaxSlots);
     if (dtype != KELP_DT_F64 && dtype != KELP_DT_F32) return fail(KELP_EINVAL, "kelp_field_upload: bad dtype %d", dtype);
     if (n_planes < 1 || !hu || !hv) return fail(KELP_EINVAL, "kelp_field_upload: u/v missing");
-    if (hw && n_w_planes < 1) return fail(KELP_EINVAL, "kelp_field_upload: w planes");
+    const bool is3d = hw != nullptr;
+    if (is3d && n_w_planes != n_planes + 1)
+        return fail(KELP_EINVAL, "kelp_field_upload: W needs n_planes + 1 interface planes");
+    if (!is3d && n_planes != 1) return fail(KELP_EINVAL, "kelp_field_upload: layered u/v need W (3-D)");
     HIP_TRY(hipSetDevice(c->device));
     FieldSlot& s = c->slots[slot];
     free_slot(s);
+    const int64_t N = c->m.n_nodes;
     const size_t es = dtype == KELP_DT_F64 ? 8 : 4;
-    const size_t uv_bytes = (size_t)n_planes * c->m.n_nodes * es;
-    HIP_TRY(hipMalloc(&s.u, uv_bytes));
-    HIP_TRY(hipMalloc(&s.v, uv_bytes));
-    HIP_TRY(hipMemcpyAsync(s.u, hu, uv_bytes, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipMemcpyAsync(s.v, hv, uv_bytes, hipMemcpyHostToDevice, c->stream));
-    if (hw) {
-        const size_t w_bytes = (size_t)n_w_planes * c->m.n_nodes * es;
-        HIP_TRY(hipMalloc(&s.w, w_bytes));
-        HIP_TRY(hipMemcpyAsync(s.w, hw, w_bytes, hipMemcpyHostToDevice, c->stream));
+    const size_t uv_bytes = (size_t)n_planes * N * es;
+    const size_t w_bytes = is3d ? (size_t)n_w_planes * N * es : 0;
+    const size_t rec_bytes = (size_t)n_planes * N * es * (is3d ? 4 : 2);
+    void *tu = nullptr, *tv = nullptr, *tw = nullptr;
+    HIP_TRY(hipMalloc(&s.rec, rec_bytes));
+    hipError_t e = hipMalloc(&tu, uv_bytes);
+    if (e == hipSuccess) e = hipMalloc(&tv, uv_bytes);
+    if (e == hipSuccess && is3d) e = hipMalloc(&tw, w_bytes);
+    if (e == hipSuccess) e = hipMemcpyAsync(tu, hu, uv_bytes, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(tv, hv, uv_bytes, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess && is3d) e = hipMemcpyAsync(tw, hw, w_bytes, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        const int64_t n = is3d ? N * n_planes : N;
+        const unsigned g = grid_for(n, 256);
+        if (is3d && dtype == KELP_DT_F32)
+            pack3d_kernel<float><<<g, 256, 0, c->stream>>>(N, n_planes, c->node_old_of_new, (const float*)tu,
+                                                           (const float*)tv, (const float*)tw, (float4*)s.rec);
+        else if (is3d)
+            pack3d_kernel<double><<<g, 256, 0, c->stream>>>(N, n_planes, c->node_old_of_new, (const double*)tu,
+                                                            (const double*)tv, (const double*)tw, (double4*)s.rec);
+        else if (dtype == KELP_DT_F32)
+            pack2d_kernel<float><<<g, 256, 0, c->stream>>>(N, c->node_old_of_new, (const float*)tu,
+                                                           (const float*)tv, (float2*)s.rec);
+        else
+            pack2d_kernel<double><<<g, 256, 0, c->stream>>>(N, c->node_old_of_new, (const double*)tu,
+                                                            (const double*)tv, (double2*)s.rec);
+        e = hipGetLastError();
     }
-    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (tu) (void)hipFree(tu);
+    if (tv) (void)hipFree(tv);
+    if (tw) (void)hipFree(tw);
+    if (e != hipSuccess) {
+        free_slot(s);
+        return fail(e == hipErrorOutOfMemory ? KELP_ENOMEM : KELP_EHIP, "kelp_field_upload: %s", hipGetErrorString(e));
+    }
     s.dtype = dtype;
     s.n_planes = n_planes;
-    s.n_w_planes = hw ? n_w_planes : 0;
+    s.n_w_planes = is3d ? n_w_planes : 0;
+    s.is3d = is3d;
     return KELP_OK;
 }
 
@@ -864,7 +981,7 @@ int kelp_step(kelp_ctx* c, const kelp_particles* p, const kelp_step_args* a)
     if (a->alpha0 != 0.0 && !sb) return fail(KELP_EINVAL, "kelp_step: alpha0 != 0 needs slot_b");
     for (int k = 0; k < a->order; ++k)
         if (a->alphas[k] >= 1e-12 && !st) return fail(KELP_EINVAL, "kelp_step: stage alpha > 0 needs slot_t");
-    const bool is3d = sa.w != nullptr;
+    const bool is3d = sa.is3d;
     const int K = c->m.n_levels;
     if (is3d) {
         if (K < 2) return fail(KELP_EINVAL, "kelp_step: a 3-D field needs a mesh with >= 2 levels");
@@ -921,7 +1038,7 @@ int kelp_nearest(kelp_ctx* c, int64_t n, const double* lon, const double* lat, i
     if (!c->has_mesh) return fail(KELP_ESTATE, "kelp_nearest: no mesh uploaded");
     if (n <= 0) return KELP_OK;
     HIP_TRY(hipSetDevice(c->device));
-    nearest_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(c->m, n, lon, lat, exact, out);
+    nearest_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(c->m, n, lon, lat, exact, c->node_old_of_new, out);
     HIP_TRY(hipGetLastError());
     return KELP_OK;
 }
@@ -933,7 +1050,12 @@ int kelp_containing(kelp_ctx* c, int64_t n, const double* lon, const double* lat
     if (!c->has_mesh) return fail(KELP_ESTATE, "kelp_containing: no mesh uploaded");
     if (n <= 0) return KELP_OK;
     HIP_TRY(hipSetDevice(c->device));
-    containing_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(c->m, n, lon, lat, nn, out);
+    if (c->m.max_v == 3)
+        containing_kernel<3><<<grid_for(n, 256), 256, 0, c->stream>>>(c->m, n, lon, lat, nn, c->node_new_of_old,
+                                                                       c->elem_old_of_new, out);
+    else
+        containing_kernel<4><<<grid_for(n, 256), 256, 0, c->stream>>>(c->m, n, lon, lat, nn, c->node_new_of_old,
+                                                                       c->elem_old_of_new, out);
     HIP_TRY(hipGetLastError());
     return KELP_OK;
 }

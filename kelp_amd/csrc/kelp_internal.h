@@ -24,12 +24,13 @@ int kelp_fail(int code, const char* fmt, ...);
 constexpr int kMaxSlots = 16;
 constexpr int kMaxEvents = 16;
 
+// One snapshot in HBM as packed records (see kelp_device.cuh, Snap): 3-D
+// (u, v, w_top, w_bottom) per (layer, node), 2-D (u, v) per node.
 struct FieldSlot {
     int dtype = -1;  // -1: empty
     int64_t n_planes = 0, n_w_planes = 0;
-    void* u = nullptr;
-    void* v = nullptr;
-    void* w = nullptr;
+    bool is3d = false;
+    void* rec = nullptr;
 };
 
 // growable device buffer owned by a context
@@ -45,6 +46,11 @@ struct kelp_ctx {
     bool has_mesh = false;
     kelp::DMesh m{};
     std::vector<void*> mesh_bufs;  // owned device buffers of the mesh
+    // internal renumbering (node id = grid-hash CSR slot); device arrays
+    int32_t* node_old_of_new = nullptr;
+    int32_t* node_new_of_old = nullptr;
+    int32_t* elem_old_of_new = nullptr;
+    int32_t* elem_new_of_old = nullptr;
     FieldSlot slots[kMaxSlots];
     DevBuf sort_tmp, sort_keys, sort_keys2, sort_idx, sort_perm, sort_stage;
 };

@@ -48,6 +48,7 @@ class DeviceParticles:
     pid: object
     ws: object
     alive_count: object = None
+    ctx: object = None      # owning DeviceContext (node id translation)
 
     @property
     def n(self) -> int:
@@ -59,8 +60,16 @@ class DeviceParticles:
                          _ptr(self.pid), _ptr(self.ws), _ptr(self.alive_count))
 
     def to_host(self) -> Dict[str, np.ndarray]:
-        return {k: getattr(self, k).cpu().numpy() for k in
-                ("lon", "lat", "z", "status", "nn", "fr", "u", "v", "w", "pid", "ws")}
+        """Host copies of every array; ``nn`` in the mesh's node numbering."""
+        out = {k: getattr(self, k).cpu().numpy() for k in
+               ("lon", "lat", "z", "status", "fr", "u", "v", "w", "pid", "ws")}
+        nn = self.nn.clone()
+        if self.ctx is not None:
+            check(self.ctx.lib.kelp_nodes_to_mesh_ids(self.ctx.h, nn.numel(), nn.data_ptr()),
+                  "kelp_nodes_to_mesh_ids")
+            self.ctx.synchronize()
+        out["nn"] = nn.cpu().numpy()
+        return out
 
 
 RESUSPENSION_CODES = {"never": 0, "auto": 1, "stochastic": 2, "auto_or_stochastic": 3}
@@ -206,9 +215,14 @@ class DeviceContext:
                 else t.as_tensor(np.ascontiguousarray(fr, np.int8), device=dev))
         pid_t = (t.arange(P, dtype=t.int64, device=dev) if pid is None
                  else t.as_tensor(np.ascontiguousarray(pid, np.int64), device=dev))
+        nn_t = i32(nn, -1)
+        if nn is not None:
+            check(self.lib.kelp_nodes_from_mesh_ids(self.h, nn_t.numel(), nn_t.data_ptr()),
+                  "kelp_nodes_from_mesh_ids")
+            self.synchronize()
         return DeviceParticles(lon=f64(lon), lat=f64(lat), z=f64(z), status=i32(status, 1),
-                               nn=i32(nn, -1), fr=fr_t, u=f64(None), v=f64(None), w=f64(None),
-                               pid=pid_t, ws=i32(ws, 0))
+                               nn=nn_t, fr=fr_t, u=f64(None), v=f64(None), w=f64(None),
+                               pid=pid_t, ws=i32(ws, 0), ctx=self)
 
     # -- operations ---------------------------------------------------------
     def step(self, p: DeviceParticles, order: int, slot_a: int, dt: float, time_dir: float,
