@@ -60,7 +60,11 @@ class DeviceParticles:
                          _ptr(self.pid), _ptr(self.ws), _ptr(self.alive_count))
 
     def to_host(self) -> Dict[str, np.ndarray]:
-        """Host copies of every array; ``nn`` in the mesh's node numbering."""
+        """Host copies of every array; ``nn`` in the mesh's node numbering.
+        Waits for the owning context's stream first (the copies run on
+        torch's stream, which does not order against it)."""
+        if self.ctx is not None:
+            self.ctx.synchronize()
         out = {k: getattr(self, k).cpu().numpy() for k in
                ("lon", "lat", "z", "status", "fr", "u", "v", "w", "pid", "ws")}
         nn = self.nn.clone()

@@ -77,6 +77,23 @@ class Mesh:
     def is_3d(self) -> bool:
         return self.sigma.size > 0
 
+    @property
+    def z_levels(self) -> np.ndarray:
+        return self.sigma
+
+
+def set_exact_nearest(mesh: Mesh, flag: bool) -> None:
+    """The exact nearest-node anchor guard (core/mesh.py:321): on, every
+    anchor is the provably nearest node; off, anchors come from the 3x3
+    hash window (misses healed).  Stored in ``mesh.metadata``."""
+    mesh.metadata["exact_nearest"] = bool(flag)
+
+
+def exact_nearest_on(mesh: Mesh) -> bool:
+    """numerics/integrators.py:3283 _exact_nearest_on."""
+    md = getattr(mesh, "metadata", None)
+    return bool(md and md.get("exact_nearest", False))
+
 
 def build_node_to_elems(n_nodes: int, elem_nodes: np.ndarray):
     """Node -> element adjacency ``(N, max_conn)`` (-1 padded) and counts.
