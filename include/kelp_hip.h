@@ -91,6 +91,9 @@ typedef struct {
     double depth_min, zone_depth, t_beach, rewet_hysteresis, t_resuspend;
     double beach_dt;               /* step length seen by the hook (abs() taken) */
     uint64_t beach_seed;
+    int32_t az;                    /* vertical Az kick after the horizontal one
+                                      (integrators.py:842), constant source, 3-D */
+    double az_const;               /* m^2/s; 0 = no kick */
 } kelp_step_args;
 
 const char *kelp_last_error(void);

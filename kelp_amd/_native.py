@@ -59,6 +59,7 @@ class StepArgs(ctypes.Structure):
         ("depth_min", c_double), ("zone_depth", c_double), ("t_beach", c_double),
         ("rewet_hysteresis", c_double), ("t_resuspend", c_double), ("beach_dt", c_double),
         ("beach_seed", ctypes.c_uint64),
+        ("az", c_int32), ("az_const", c_double),
     ]
 
 

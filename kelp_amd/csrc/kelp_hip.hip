@@ -180,6 +180,8 @@ struct StepCfg {
     int resuspension, min_wet_steps;
     double depth_min, zone_depth, t_beach, rewet_hysteresis, t_resuspend, beach_dt;
     uint64_t beach_seed;
+    int az;
+    double az_const;
 };
 
 struct PState {
@@ -407,6 +409,21 @@ __device__ inline void kick_one(const DMesh& m, const StepCfg& c, uint64_t pid, 
     if (IS3D) z = clamp_z(m, z, nn_k);
 }
 
+// integrators.py:842 _apply_split_az_kick, constant source on a z-level
+// mesh (numerics/diffusion.py:427 _loop_az_kick_zlevel, gradient 0, and
+// :302 diffusion_az_step_meters): one particle alive after the horizontal
+// kick, folded back into [z_levels[0], depth[nn]].
+__device__ inline void az_one(const DMesh& m, const StepCfg& c, uint64_t pid, int node, double& z)
+{
+    double xi, unused;
+    normal_pair(c.seed, c.real_id, pid, c.step_i, 1, xi, unused);
+    const double sdt = c.time_dir * c.dt;
+    const double abs_dt = sdt >= 0.0 ? sdt : -sdt;
+    const double g = 0.0;
+    const double dz = g * abs_dt + km_sqrt(2.0 * c.az_const * abs_dt) * xi;
+    z = km_reflect_depth(z + dz, m.zlev[0], m.depth[node]);
+}
+
 // tracking.py:199 _apply_beaching_hook + numerics/beaching.py:95
 // evaluate_beaching, one particle (prev = status at the step start).
 template <int MAXV>
@@ -499,6 +516,7 @@ __global__ __launch_bounds__(256) KELP_STEP_ATTR void step_kernel(DMesh m, Snap<
         } else if (c.kick && !(c.kh_x == 0.0 && c.kh_y == 0.0)) {
             kick_one<MAXV, IS3D>(m, c, (uint64_t)s.pid[i], pre_lon, pre_lat, pre_z, pre_nn, lon, lat, z, st, nn, fr);
         }
+        if (IS3D && c.az && st == kAlive) az_one(m, c, (uint64_t)s.pid[i], nn, z);
     }
     if (c.beach) beach_one<MAXV>(m, c, (uint64_t)s.pid[i], prev, lon, lat, st, nn, fr, s.ws[i]);
     s.lon[i] = lon;
@@ -560,6 +578,8 @@ __global__ void math_probe_kernel(int fn, int64_t n, const double* __restrict__ 
     case 5: r = km_log(a); break;
     case 6: r = km_exp(a); break;
     case 7: r = km_pymod360(a); break;
+    case 9: r = km_pymod_pos(a, y[i]); break;
+    case 10: r = km_reflect_depth(a, 0.0, y[i]); break;
     default: r = km_sqrt(a); break;
     }
     out[i] = r;
@@ -961,7 +981,10 @@ int kelp_step(kelp_ctx* c, const kelp_particles* p, const kelp_step_args* a)
     if (p->n == 0) return KELP_OK;
     if (!p->lon || !p->lat || !p->status || !p->nn || !p->fr || !p->u || !p->v)
         return fail(KELP_EINVAL, "kelp_step: missing particle array");
-    if ((a->kick || a->beach) && !p->pid) return fail(KELP_EINVAL, "kelp_step: the kick and beaching need pid");
+    if ((a->kick || a->beach || a->az) && !p->pid)
+        return fail(KELP_EINVAL, "kelp_step: the kicks and beaching need pid");
+    if (a->az && !(a->az_const >= 0.0 && a->az_const < __builtin_inf()))
+        return fail(KELP_EINVAL, "kelp_step: az_const must be finite and >= 0");
     if (a->beach && !p->wet_streak) return fail(KELP_EINVAL, "kelp_step: beaching needs wet_streak");
     if (a->kick && (!(a->kh_x >= 0.0) || !(a->kh_y >= 0.0))) return fail(KELP_EINVAL, "kelp_step: Kh must be >= 0");
     if (a->beach && (a->use_coastal || a->use_stochastic) && !c->m.coastal)
@@ -989,6 +1012,8 @@ int kelp_step(kelp_ctx* c, const kelp_particles* p, const kelp_step_args* a)
             return fail(KELP_EINVAL, "kelp_step: 3-D field shapes (%lld, %lld) do not match K=%d",
                         (long long)sa.n_planes, (long long)sa.n_w_planes, K);
         if (!p->z) return fail(KELP_EINVAL, "kelp_step: 3-D step needs z");
+    } else if (a->az && a->az_const > 0.0) {
+        return fail(KELP_EINVAL, "kelp_step: the Az kick needs a 3-D run (integrators.py:893)");
     } else if (sa.n_planes != 1) {
         return fail(KELP_EINVAL, "kelp_step: a layered u/v field needs W (3-D)");
     }
@@ -1022,6 +1047,8 @@ int kelp_step(kelp_ctx* c, const kelp_particles* p, const kelp_step_args* a)
     cfg.t_resuspend = a->t_resuspend;
     cfg.beach_dt = a->beach_dt;
     cfg.beach_seed = a->beach_seed;
+    cfg.az = a->az && a->az_const > 0.0;
+    cfg.az_const = a->az_const;
     PState ps{p->lon, p->lat, p->z, p->status, p->nn, p->fr, p->u, p->v, p->w, p->pid, p->wet_streak,
               reinterpret_cast<unsigned long long*>(p->alive_count), p->n};
     if (sa.dtype == KELP_DT_F32)

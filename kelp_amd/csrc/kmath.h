@@ -80,6 +80,32 @@ KM_FN double km_rint_small(double v)
     return t - KM_RND_BIG;
 }
 
+/* Python float `%` for a positive divisor d (CPython float_rem): the exact
+ * remainder fmod(x, d) (exactly representable, so every correct fmod returns
+ * the same bits), then a rounded +d when it is negative, +0.0 when zero. */
+KM_FN double km_pymod_pos(double x, double d)
+{
+    double m = __builtin_fmod(x, d);
+    if (m != 0.0) {
+        if (m < 0.0) m += d;
+    } else {
+        m = 0.0;
+    }
+    return m;
+}
+
+/* numerics/diffusion.py:334 reflect_depth: specular fold of z into
+ * [top, bot]; a degenerate column collapses to top. */
+KM_FN double km_reflect_depth(double z, double top, double bot)
+{
+    double L = bot - top;
+    if (L <= 0.0) return top;
+    double y = km_pymod_pos(z - top, 2.0 * L);
+    if (y < 0.0) y += 2.0 * L;
+    if (y > L) y = 2.0 * L - y;
+    return top + y;
+}
+
 /* Exact x mod 360 with the sign of the divisor -- the remainder Python's
  * float `%` (and numpy's remainder) return: fmod, then +360 when the
  * remainder is negative, +0.0 when it is zero.  Exact for |x| < 2^50. */

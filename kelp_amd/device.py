@@ -99,6 +99,8 @@ def make_step_args(order, slot_a, dt, time_dir, slot_b=-1, slot_t=-1, alpha0=0.0
         a.kh_x = float(kick["kh_x"])
         a.kh_y = float(kick["kh_y"])
         a.seed = int(kick["seed"])
+        if kick.get("az_const"):
+            a.az, a.az_const = 1, float(kick["az_const"])
     if beach:
         a.beach = 1
         crit = set(beach["criteria"])
@@ -233,8 +235,9 @@ class DeviceContext:
              slot_b: int = -1, slot_t: int = -1, alpha0: float = 0.0, alphas=(0.0, 0.0, 0.0, 0.0),
              exact_nearest: bool = True, w_method: int = 2, kick=None, beach=None,
              step_i: int = 0, real_id: int = 0) -> None:
-        """One step in place.  ``kick``: dict(kh_x, kh_y, seed) for the split
-        uniform-Kh random walk; ``beach``: a :class:`BeachArgs`-like dict."""
+        """One step in place.  ``kick``: dict(kh_x, kh_y, seed[, az_const])
+        for the split uniform-Kh random walk (and the vertical Az kick);
+        ``beach``: dict of the beaching options (criteria, depth_min, ...)."""
         a = make_step_args(order, slot_a, dt, time_dir, slot_b, slot_t, alpha0, alphas,
                            exact_nearest, w_method, kick, beach, step_i, real_id)
         s = p.as_struct()

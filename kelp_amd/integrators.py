@@ -21,7 +21,7 @@ anchor mode 2 of the kernel.
 
 Combinations the device path does not implement raise
 ``NotImplementedError`` (element-layout u/v, W leastsq/idw, sigma meshes,
-EM/M1 diffusion, the Az kick); there is no CPU fallback.
+EM/M1 diffusion, an Az kick from an Az field); there is no CPU fallback.
 """
 from __future__ import annotations
 
@@ -145,7 +145,12 @@ def _check_supported(vel, mesh, uv_interp, w_interp, diffusion, is3d):
         if diffusion.composition not in ("split", "split_rk"):
             raise NotImplementedError(f"device path: composition {diffusion.composition!r}")
         if diffusion.az_enabled:
-            raise NotImplementedError("device path: the Az kick is not implemented")
+            if diffusion.az_requires_field or getattr(base, "az", None) is not None:
+                raise NotImplementedError("device path: the Az kick from an Az field is not "
+                                          "implemented (constant az_const only)")
+            if not is3d and diffusion.az_const > 0.0:
+                # integrators.py:893
+                raise ValueError("diffusion.az_enabled requires a 3-D run (Fdrift3D)")
 
 
 def _step(order: int, state: ParticleState, vel, mesh: Mesh, dt: float, time_dir: float,
@@ -194,6 +199,8 @@ def _step(order: int, state: ParticleState, vel, mesh: Mesh, dt: float, time_dir
     step_i, real_id = 0, 0
     if diffusion is not None:
         kick = dict(kh_x=diffusion.kh_x, kh_y=diffusion.kh_y, seed=diffusion.seed)
+        if diffusion.az_enabled and diffusion.az_const > 0.0:
+            kick["az_const"] = float(diffusion.az_const)
         step_i, real_id = int(diffusion.step_i), int(diffusion.real_id)
     exact = 1 if exact_nearest_on(mesh) else 2
     ctx.step(p, order, slot_a, dt, time_dir, slot_b=slot_b, slot_t=slot_t, alpha0=alpha0,

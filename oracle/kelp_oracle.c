@@ -1207,6 +1207,51 @@ static void kick_one(const or_mesh_t *m, const or_step_args_t *a, uint64_t pid,
     if (a->is3d) *z = clamp_z(m, *z, nn_k);
 }
 
+#ifdef ORACLE_KMATH
+#define REFLECT_DEPTH km_reflect_depth
+#define PYMOD_POS km_pymod_pos
+#else
+/* Python float `%` for a positive divisor (CPython float_rem): fmod is
+ * exact; the remainder takes the divisor's sign and a zero is +0.0. */
+static double PYMOD_POS(double x, double d)
+{
+    double m = fmod(x, d);
+    if (m != 0.0) {
+        if (m < 0.0) m += d;
+    } else {
+        m = 0.0;
+    }
+    return m;
+}
+
+/* numerics/diffusion.py:334 reflect_depth: specular fold into [top, bot]. */
+static double REFLECT_DEPTH(double z, double top, double bot)
+{
+    double L = bot - top;
+    if (L <= 0.0) return top;
+    double y = PYMOD_POS(z - top, 2.0 * L);
+    if (y < 0.0) y += 2.0 * L;
+    if (y > L) y = 2.0 * L - y;
+    return top + y;
+}
+#endif
+
+/* integrators.py:842 _apply_split_az_kick with a constant source, z-level
+ * mesh: numerics/diffusion.py:427 _loop_az_kick_zlevel (gradient 0) and
+ * :302 diffusion_az_step_meters, one particle alive after the horizontal
+ * kick.  The single normal is the first Box-Muller half of component 1. */
+static void az_one(const or_mesh_t *m, const or_step_args_t *a, uint64_t pid, int32_t node, double *z)
+{
+    double az = a->az_const < 0.0 ? 0.0 : a->az_const;
+    double g = 0.0;
+    double xi, unused;
+    or_normal_pair(a->seed, a->real_id, pid, a->step_i, 1, &xi, &unused);
+    double sdt = a->time_dir * a->dt;
+    double abs_dt = sdt >= 0.0 ? sdt : -sdt;
+    double dz = g * abs_dt + sqrt(2.0 * az * abs_dt) * xi;
+    *z = REFLECT_DEPTH(*z + dz, m->z_levels[0], m->depth[node]);
+}
+
 /* tracking.py:199 _apply_beaching_hook + beaching.py:95 evaluate_beaching
  * for one particle (prev = status at the step start). */
 static void beach_one(const or_mesh_t *m, const or_step_args_t *a, uint64_t pid, int32_t prev,
@@ -1283,6 +1328,8 @@ void or_step(const or_mesh_t *m, const or_fields_t *f, const or_step_args_t *a, 
             if (a->kick && status[i] == ST_ALIVE && !(a->kh_x == 0.0 && a->kh_y == 0.0))
                 kick_one(m, a, (uint64_t)pid[i], pre_lon, pre_lat, pre_z, pre_nn,
                          &lon[i], &lat[i], &z[i], &status[i], &nn[i], &fr[i]);
+            if (a->az && a->is3d && a->az_const > 0.0 && status[i] == ST_ALIVE)
+                az_one(m, a, (uint64_t)pid[i], nn[i], &z[i]);
         }
         if (a->beach) beach_one(m, a, (uint64_t)pid[i], prev, lon[i], lat[i], &status[i], &nn[i], &fr[i], &ws[i]);
     }
@@ -1358,6 +1405,8 @@ void or_math_probe(int fn, int64_t n, const double *x, const double *y, double *
         case 5: out[i] = M_LOG(x[i]); break;
         case 6: out[i] = M_EXP(x[i]); break;
         case 7: out[i] = PYMOD360(x[i]); break;
+        case 9: out[i] = PYMOD_POS(x[i], y[i]); break;
+        case 10: out[i] = REFLECT_DEPTH(x[i], 0.0, y[i]); break;
         default: out[i] = sqrt(x[i]); break;
         }
     }

@@ -69,6 +69,9 @@ typedef struct {
     double depth_min, zone_depth, t_beach, rewet_hysteresis, t_resuspend;
     uint64_t beach_seed;
     double beach_dt;
+    /* vertical Az kick, constant source (integrators.py:842) */
+    int32_t az;
+    double az_const;
 } or_step_args_t;
 
 #endif

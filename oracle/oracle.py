@@ -47,6 +47,7 @@ class OrStepArgs(ctypes.Structure):
         ("depth_min", c_double), ("zone_depth", c_double), ("t_beach", c_double),
         ("rewet_hysteresis", c_double), ("t_resuspend", c_double),
         ("beach_seed", ctypes.c_uint64), ("beach_dt", c_double),
+        ("az", c_int32), ("az_const", c_double),
     ]
 
 
@@ -64,6 +65,8 @@ def step_args(order, dt, time_dir, is3d=False, w_method=2, exact=True, alphas=(0
     if kick:
         a.kick = 1
         a.kh_x, a.kh_y, a.seed = float(kick["kh_x"]), float(kick["kh_y"]), int(kick["seed"])
+        if kick.get("az_const"):
+            a.az, a.az_const = 1, float(kick["az_const"])
     if beach:
         a.beach = 1
         crit = set(beach["criteria"])
@@ -126,6 +129,10 @@ def lib(flavour: str = "glibc") -> ctypes.CDLL:
     L.or_find_containing_local.restype = c_int
     L.or_element_contains.argtypes = [POINTER(OrMesh), c_int, c_double, c_double]
     L.or_element_contains.restype = c_int
+    L.or_interp_bary_tinterp.argtypes = [POINTER(OrMesh), POINTER(OrFields), c_double, c_double,
+                                         c_int, c_double, c_int, c_double,
+                                         POINTER(c_double), POINTER(c_double), POINTER(c_int)]
+    L.or_interp_bary_tinterp.restype = c_int
     _LIBS[flavour] = L
     return L
 
@@ -179,6 +186,25 @@ class OracleMesh:
         self.L.or_contain_batch(ctypes.byref(self.s), lon.size, lon.ctypes.data, lat.ctypes.data,
                                 nn.ctypes.data, out.ctypes.data)
         return out
+
+    def interp2d(self, lon, lat, nn, u, v):
+        """Barycentric/Wachspress interpolation of node values (u, v) at
+        each point around its anchor ``nn`` (interpolate.py:1341, steady,
+        time_dir +1).  Returns (U, V, ok, elem)."""
+        u = _c(u, np.float64)
+        v = _c(v, np.float64)
+        f = OrFields(u.ctypes.data, v.ctypes.data, u.ctypes.data, v.ctypes.data,
+                     u.ctypes.data, v.ctypes.data, None, None, None, 0, 0.0)
+        n = len(lon)
+        U, V = np.empty(n), np.empty(n)
+        ok, el = np.empty(n, bool), np.empty(n, np.int32)
+        for i in range(n):
+            a, b, e = c_double(), c_double(), c_int()
+            ok[i] = bool(self.L.or_interp_bary_tinterp(
+                ctypes.byref(self.s), ctypes.byref(f), float(lon[i]), float(lat[i]), int(nn[i]),
+                1.0, -1, 0.0, ctypes.byref(a), ctypes.byref(b), ctypes.byref(e)))
+            U[i], V[i], el[i] = a.value, b.value, e.value
+        return U, V, ok, el
 
     def element_contains(self, e, lon, lat) -> bool:
         return bool(self.L.or_element_contains(ctypes.byref(self.s), int(e), float(lon), float(lat)))

@@ -27,6 +27,8 @@ CASES = {
     "fesom2_node_rk2_fused_lerp": (2, (1, 2), None),
     "fesom2_node_rk4_unfused": (4, (21, 22), None),
     "split_uniform_diffusion_rk2": (2, (31, 32), dict(kh_x=50.0, kh_y=20.0, seed=2026)),
+    # steady single snapshot, vertical Az kick only (tests/_gen_golden_3d.py:305-318)
+    "az_const_euler": (1, (41,), dict(kh_x=0.0, kh_y=0.0, seed=99, az_const=5e-3)),
 }
 
 
@@ -80,13 +82,22 @@ def run_case(name, direction, stepper):
     {key: (step, particle) array}."""
     order, snap_seeds, kick = CASES[name]
     mesh = dual_mesh()
-    a, b = (field(mesh, s) for s in snap_seeds)
+    snaps = [field(mesh, s) for s in snap_seeds]
+    steady = len(snaps) == 1
+    a, b = snaps[0], snaps[-1]
     lon, lat, z = seeds()
     st = dict(lon=lon, lat=lat, z=z, status=np.ones(lon.size, np.int32),
               nn=np.full(lon.size, -1, np.int32), fr=np.zeros(lon.size, np.int8))
     rows = {k: [] for k in ("lon", "lat", "z", "status", "failure_reason", "u", "v", "w")}
-    c = (0.0, 0.5, 0.5, 1.0) if order == 4 else (0.0, 0.5)
+    c = {1: (0.0,), 2: (0.0, 0.5), 4: (0.0, 0.5, 0.5, 1.0)}[order]
     for i in range(NODE_STEPS):
+        if steady:
+            fields = dict(u_a=a[0], v_a=a[1], w_a=a[2], u_t=a[0], v_t=a[1], w_t=a[2], alpha0=0.0)
+            st = stepper(mesh, st, fields, order, [0.0] * len(c), i, kick)
+            for k, src in (("lon", "lon"), ("lat", "lat"), ("z", "z"), ("status", "status"),
+                           ("failure_reason", "fr"), ("u", "u"), ("v", "v"), ("w", "w")):
+                rows[k].append(np.array(st[src], copy=True))
+            continue
         kind, alpha0, tgt_b, dtn = payload(i, direction)
         base = {"a": a, "b": b, "lerp": a}[kind]
         tgt = b if tgt_b else a

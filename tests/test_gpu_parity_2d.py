@@ -37,14 +37,19 @@ def test_device_math_matches_host_build(gpu_ctx):
              8: np.exp(rng.uniform(-50, 50, n))}
     y = rng.uniform(-3, 3, n)
     cases[4] = rng.uniform(-3, 3, n)
+    ys = {9: np.exp(rng.uniform(-3, 8, n)), 10: rng.uniform(-5, 5000, n)}
+    # Python `%` by a positive divisor, quotients up to 2^60; the reflect fold
+    cases[9] = rng.choice([-1.0, 1.0], n) * np.exp(rng.uniform(-5, 45, n))
+    cases[10] = rng.uniform(-3, 4, n) * ys[10]
     for fn, x in cases.items():
+        yv = ys.get(fn, y)
         xd = t.as_tensor(x, device="cuda:0")
-        yd = t.as_tensor(y, device="cuda:0")
+        yd = t.as_tensor(yv, device="cuda:0")
         od = t.empty_like(xd)
         rc = gpu_ctx.lib.kelp_math_probe(gpu_ctx.h, fn, n, xd.data_ptr(), yd.data_ptr(), od.data_ptr())
         assert rc == 0
         gpu_ctx.synchronize()
-        host = math_probe(fn, x, y if fn == 4 else None, flavour="kmath")
+        host = math_probe(fn, x, yv if fn in (4, 9, 10) else None, flavour="kmath")
         dev = od.cpu().numpy()
         assert np.array_equal(dev.view(np.int64), host.view(np.int64)), f"fn {fn}"
 

@@ -140,3 +140,12 @@ def test_sort_is_result_neutral(gpu_ctx, world):
         assert np.array_equal(a[k].view(np.int64), b[k].view(np.int64)), k
     for k in ("status", "nn", "fr", "ws", "pid"):
         assert np.array_equal(a[k], b[k]), k
+
+
+@pytest.mark.parametrize("order", [1, 4])
+def test_3d_az_kick_bitwise(gpu_ctx, world, order):
+    """Horizontal Kh walk followed by the vertical Az kick (reflective
+    column ends), forward and with a time blend."""
+    kick = dict(kh_x=10.0, kh_y=10.0, seed=31, az_const=5e-3)
+    h, st = _run(gpu_ctx, world, order, 2, 1.0, 12, kick, None)
+    _assert_bitwise(h, st)

@@ -50,7 +50,10 @@ def test_refused_combinations_raise_before_the_device(small):
     with pytest.raises(NotImplementedError, match="scheme"):
         I.euler_step(st, vel, mesh, 600.0, 1.0, uv_interp="barycentric", diffusion=d)
     d = DiffusionStepArgs("uniform", 1.0, 1.0, 1, 0, 0, az_enabled=True, az_const=1e-4)
-    with pytest.raises(NotImplementedError, match="Az"):
+    with pytest.raises(ValueError, match="3-D"):
+        I.euler_step(st, vel, mesh, 600.0, 1.0, uv_interp="barycentric", diffusion=d)
+    d = DiffusionStepArgs("uniform", 1.0, 1.0, 1, 0, 0, az_enabled=True, az_requires_field=True)
+    with pytest.raises(NotImplementedError, match="Az field"):
         I.euler_step(st, vel, mesh, 600.0, 1.0, uv_interp="barycentric", diffusion=d)
     with pytest.raises(ValueError, match="together"):
         I.rk4_step(st, vel, mesh, 600.0, 1.0, uv_interp="barycentric", vel_next=vel)

@@ -41,6 +41,7 @@ def _oracle_stepper(flavour):
     ("fesom2_node_rk2_fused_lerp", 1.0), ("fesom2_node_rk2_fused_lerp", -1.0),
     ("fesom2_node_rk4_unfused", 1.0),
     ("split_uniform_diffusion_rk2", 1.0), ("split_uniform_diffusion_rk2", -1.0),
+    ("az_const_euler", 1.0),
 ])
 @pytest.mark.parametrize("flavour", ["glibc", "kmath"])
 def test_oracle_matches_reference_golden(name, direction, flavour):
