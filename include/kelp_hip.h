@@ -51,6 +51,8 @@ typedef struct {
     int64_t n_levels;                              /* K interfaces (0 for 2-D) */
     const double *host_z_levels;                   /* (K,) m, positive down */
     const uint8_t *host_elem_coastal;              /* (E,) coastal flags (core/mesh.py:2765) or NULL */
+    const double *host_elem_center_lon;            /* (E,) element centres (Mesh.elem_center_lon/lat); */
+    const double *host_elem_center_lat;            /*   needed only for element-centre u/v */
 } kelp_mesh_desc;
 
 /* Particle state, device memory, structure of arrays (the fields of the
@@ -80,7 +82,8 @@ typedef struct {
     int32_t exact_nearest;         /* anchor: 1 exact (mesh.exact_nearest, the default);
                                       0 guard off, fused kernels' widened window;
                                       2 guard off, batch 3x3 window healed exactly */
-    int32_t w_method;              /* 3-D W: 0 linear at the anchor ("nearest"), 2 barycentric */
+    int32_t w_method;              /* 3-D W: 0 linear at the anchor ("nearest"), 1 leastsq,
+                                      2 barycentric */
     int32_t kick;                  /* split uniform-Kh random walk (integrators.py:673) */
     int32_t beach;                 /* beaching hook */
     double kh_x, kh_y;             /* m^2/s */
@@ -124,6 +127,13 @@ int kelp_nodes_from_mesh_ids(kelp_ctx *ctx, int64_t n, int32_t *ids);
 int kelp_field_upload(kelp_ctx *ctx, int slot, int dtype, int64_t n_planes,
                       const void *host_u, const void *host_v,
                       const void *host_w, int64_t n_w_planes);
+/* The same with u, v at the element centres (the reference's default
+ * uv_layout "element", VelocityField.u of shape (E,) / (K-1, E)): the step
+ * interpolates them by least squares over the anchor's element ring
+ * (numerics/interpolate.py:52).  W stays at the nodes. */
+int kelp_field_upload_elem(kelp_ctx *ctx, int slot, int dtype, int64_t n_planes,
+                           const void *host_u, const void *host_v,
+                           const void *host_w, int64_t n_w_planes);
 int kelp_field_clear(kelp_ctx *ctx, int slot);
 
 /* Advance every particle by one step, in place, on the context stream. */

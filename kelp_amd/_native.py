@@ -32,6 +32,7 @@ class MeshDesc(ctypes.Structure):
         ("r_earth", c_double),
         ("n_levels", c_int64), ("host_z_levels", c_void_p),
         ("host_elem_coastal", c_void_p),
+        ("host_elem_center_lon", c_void_p), ("host_elem_center_lat", c_void_p),
     ]
 
 
@@ -77,6 +78,8 @@ SIGNATURES = {
     "kelp_nodes_from_mesh_ids": (c_int, [c_void_p, c_int64, c_void_p]),
     "kelp_field_upload": (c_int, [c_void_p, c_int, c_int, c_int64, c_void_p, c_void_p,
                                   c_void_p, c_int64]),
+    "kelp_field_upload_elem": (c_int, [c_void_p, c_int, c_int, c_int64, c_void_p, c_void_p,
+                                       c_void_p, c_int64]),
     "kelp_field_clear": (c_int, [c_void_p, c_int]),
     "kelp_step": (c_int, [c_void_p, POINTER(Particles), POINTER(StepArgs)]),
     "kelp_sort_particles": (c_int, [c_void_p, POINTER(Particles)]),

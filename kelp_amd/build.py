@@ -1,7 +1,10 @@
 """Build the HIP library (and the CPU oracle used by the tests).
 
-``python -m kelp_amd.build`` compiles kelp_amd/csrc/kelp_hip.hip for gfx950
-into kelp_amd/libkelp_hip.so with hipcc.  Contraction is disabled
+``python -m kelp_amd.build`` compiles the sources under kelp_amd/csrc for
+gfx950 into kelp_amd/libkelp_hip.so with hipcc.  The step kernel is
+instantiated per (3-D, field dtype, u/v layout) in its own object file
+(kelp_step_variant.hip compiled with -DKV_*), so the objects compile in
+parallel and link into one library.  Contraction is disabled
 (-ffp-contract=off) so every product and sum rounds like the reference's
 Numba kernels; packed-FP32 instructions are disabled as a precaution.
 """
@@ -10,17 +13,26 @@ from __future__ import annotations
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SRCS = [os.path.join(HERE, "csrc", f) for f in ("kelp_hip.hip", "kelp_sort.hip")]
+CSRC = os.path.join(HERE, "csrc")
+OBJ_DIR = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "libkelp_hip.so")
 
 HIPCC_FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     "-ffp-contract=off", "-fno-fast-math",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function",
+]
+
+#: (object name, source, extra defines)
+UNITS = [("kelp_hip", "kelp_hip.hip", []), ("kelp_sort", "kelp_sort.hip", [])] + [
+    (f"kelp_step_{'3d' if d3 else '2d'}_{'f32' if f32 else 'f64'}_{'elem' if el else 'node'}",
+     "kelp_step_variant.hip", [f"-DKV_IS3D={d3}", f"-DKV_F32={f32}", f"-DKV_ELEM={el}"])
+    for d3 in (0, 1) for f32 in (0, 1) for el in (0, 1)
 ]
 
 
@@ -31,14 +43,29 @@ def hipcc() -> str:
     return "hipcc"
 
 
-def build_hip(verbose: bool = False) -> str:
-    tmp = OUT + ".tmp"
-    cmd = [hipcc(), *HIPCC_FLAGS, "-o", tmp, *SRCS]
+def _run(cmd):
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed ({r.returncode}):\n{' '.join(cmd)}\n{r.stderr[-4000:]}")
-    if verbose and r.stderr:
-        print(r.stderr[-2000:], file=sys.stderr)
+    return r.stderr
+
+
+def build_hip(verbose: bool = False, jobs: int = 0) -> str:
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    jobs = jobs or min(len(UNITS), max(1, os.cpu_count() or 1))
+    cmds, objs = [], []
+    for name, src, defs in UNITS:
+        obj = os.path.join(OBJ_DIR, name + ".o")
+        objs.append(obj)
+        cmds.append([hipcc(), *HIPCC_FLAGS, *defs, "-c", "-o", obj, os.path.join(CSRC, src)])
+    with ThreadPoolExecutor(jobs) as ex:
+        logs = list(ex.map(_run, cmds))
+    tmp = OUT + ".tmp"
+    logs.append(_run([hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]))
+    if verbose:
+        for lg in logs:
+            if lg:
+                print(lg[-2000:], file=sys.stderr)
     os.replace(tmp, OUT)
     return OUT
 

@@ -50,6 +50,7 @@ struct DMesh {
     const double* __restrict__ depth;     // node bottom depth (m, positive down)
     const double* __restrict__ zlev;      // K interface depths (3-D)
     const uint8_t* __restrict__ coastal;  // (E,) coastal-element flags (beaching)
+    const double2* __restrict__ ecent;    // (E,) element centres (lon, lat); element-layout u/v
     double cell, lon_min, lat_min, r_earth;
     int32_t nx, ny, wrap, far_rows;
     int32_t n_nodes, n_elems, max_v, n_levels;
@@ -895,6 +896,65 @@ __device__ __forceinline__ void w_pair_blend(const Snap<T, true>& f, int64_t idx
     w_bot = tblend((double)ra.w, (double)rb.w, (double)rt.w, alpha0, alpha);
 }
 
+// interpolate.py:207 _interp_leastsq_core_tinterp (== :52 for a plain field):
+// a plane through the blended element-centre values of the anchor's 1-ring
+// in particle-centred metric coordinates (antimeridian gate per centre),
+// Sarrus determinant with the scale-relative singularity guard, Cramer's rule
+// for the constant term.  `off` is the layer plane's first element record.
+template <typename T>
+__device__ inline bool interp_lsq(const DMesh& m, const Snap<T, false>& fe, double alpha0, double alpha,
+                                  double time_dir, double xp, double yp, int nn, int64_t off, double& U,
+                                  double& V)
+{
+    U = 0.0;
+    V = 0.0;
+    const int k0 = m.n2e_off[nn], k1 = m.n2e_off[nn + 1];
+    if (k1 - k0 < 1) return false;
+    const double z0 = km_cos(yp * kDeg2Rad) * kSLM;
+    const double xp_m = xp * z0;
+    const double yp_m = yp * kSLM;
+    double a11 = 0.0, a12 = 0.0, a13 = 0.0, a22 = 0.0, a23 = 0.0, a33 = 0.0;
+    double fUn = 0.0, f1Un = 0.0, f2Un = 0.0, fVn = 0.0, f1Vn = 0.0, f2Vn = 0.0;
+    for (int k = k0; k < k1; ++k) {
+        const int e = m.n2e[k];
+        const double2 cc = m.ecent[e];
+        double ex = cc.x * z0 - xp_m;
+        const double dlon_raw = cc.x - xp;
+        if (dlon_raw > 180.0)
+            ex -= 360.0 * z0;
+        else if (dlon_raw < -180.0)
+            ex += 360.0 * z0;
+        const double ey = cc.y * kSLM - yp_m;
+        double u, v;
+        uv_blend(fe, off + e, alpha0, alpha, u, v);
+        const double uu = time_dir * u;
+        const double vv = time_dir * v;
+        a11 = a11 + 1.0;
+        a12 = a12 + ex;
+        a13 = a13 + ey;
+        a22 = a22 + ex * ex;
+        a23 = a23 + ex * ey;
+        a33 = a33 + ey * ey;
+        fUn = fUn + uu;
+        f1Un = f1Un + uu * ex;
+        f2Un = f2Un + uu * ey;
+        fVn = fVn + vv;
+        f1Vn = f1Vn + vv * ex;
+        f2Vn = f2Vn + vv * ey;
+    }
+    const double a21 = a12, a31 = a13, a32 = a23;
+    const double det = (a11 * a22 * a33 + a12 * a23 * a31 + a13 * a21 * a32 - a13 * a22 * a31 -
+                        a12 * a21 * a33 - a11 * a23 * a32);
+    if (km_fabs(det) < 1e-30 || km_fabs(det) < 1e-12 * km_fabs(a11 * a22 * a33)) return false;
+    const double inv = 1.0 / km_fabs(det);
+    const double b11 = a22 * a33 - a23 * a32;
+    const double b12 = a13 * a32 - a12 * a33;
+    const double b13 = a12 * a23 - a13 * a22;
+    U = (b11 * fUn + b12 * f1Un + b13 * f2Un) * inv;
+    V = (b11 * fVn + b12 * f1Vn + b13 * f2Vn) * inv;
+    return true;
+}
+
 // ---------------------------------------------------------------------------
 // Vertical (z-level)
 // ---------------------------------------------------------------------------
@@ -1059,6 +1119,91 @@ __device__ inline double w_bary(const DMesh& m, const Snap<T, true>& fw, double 
     }
     const double dep = m.depth[node];
     return w_linear(m, fw, alpha0, alpha, zp < dep ? zp : dep, node, time_dir);
+}
+
+// Did vertex slot (k, v) of the anchor's element list hold node `nd` at an
+// earlier position of the scan (an earlier element, or an earlier vertex of
+// the same element)?  The 1-ring is re-read from the mesh (cached lines)
+// instead of kept in a per-lane list.
+template <int MAXV>
+__device__ __forceinline__ bool seen_before(const DMesh& m, int k0, int k, int v, int nd)
+{
+    for (int kk = k0; kk < k; ++kk) {
+        const int e2 = m.n2e[kk];
+        if (e2 < 0) continue;
+        const int4 q = m.elems[e2];
+        if (q.x == nd || q.y == nd || q.z == nd || (MAXV == 4 && q.w == nd)) return true;
+    }
+    const int4 q = m.elems[m.n2e[k]];
+    if (v > 0 && q.x == nd) return true;
+    if (v > 1 && q.y == nd) return true;
+    if (v > 2 && q.z == nd) return true;
+    return false;
+}
+
+// numerics/vertical.py:493 _interp_w_leastsq_single on the stage field
+// (interface values blended in time): the unique 1-ring nodes in the order
+// of vertical.py:446 _collect_neighbour_nodes (centre first, then each
+// element's vertices, at most 64), W linear in depth at each (clamped to that
+// node's bottom), a plane fit evaluated at the particle; fewer than 3 nodes
+// or a singular fit fall back to linear W at the anchor.
+template <int MAXV, typename T>
+__device__ inline double w_lsq(const DMesh& m, const Snap<T, true>& fw, double alpha0, double alpha, double xp,
+                               double yp, double zp, int node, double time_dir)
+{
+    const int k0 = m.n2e_off[node], k1 = m.n2e_off[node + 1];
+    double z0 = km_cos(yp * kDeg2Rad) * kSLM;
+    if (z0 < 1.0) z0 = 1.0;
+    double a11 = 0.0, a12 = 0.0, a13 = 0.0, a22 = 0.0, a23 = 0.0, a33 = 0.0;
+    double fw_ = 0.0, f1w = 0.0, f2w = 0.0;
+    int count = 0;
+    // k = k0 - 1 stands for the centre node
+    for (int k = k0 - 1; k < k1; ++k) {
+        int4 q = make_int4(node, -1, -1, -1);
+        int nv = 1;
+        if (k >= k0) {
+            const int e = m.n2e[k];
+            if (e < 0) continue;
+            q = m.elems[e];
+            nv = MAXV;
+        }
+#pragma unroll
+        for (int v = 0; v < MAXV; ++v) {
+            if (v >= nv) break;
+            const int nd = v == 0 ? q.x : (v == 1 ? q.y : (v == 2 ? q.z : q.w));
+            if (nd < 0) continue;
+            if (k >= k0 && (nd == node || seen_before<MAXV>(m, k0, k, v, nd))) continue;
+            if (count >= 64) continue;
+            ++count;
+            const double bottom = m.depth[nd];
+            const double w_val = w_linear(m, fw, alpha0, alpha, zp > bottom ? bottom : zp, nd, time_dir);
+            const double2 ll = m.lonlat[nd];
+            const double dlon = km_pymod360(ll.x - xp + 180.0) - 180.0;
+            const double ex = dlon * z0;
+            const double ey = (ll.y - yp) * kSLM;
+            a11 += 1.0;
+            a12 += ex;
+            a13 += ey;
+            a22 += ex * ex;
+            a23 += ex * ey;
+            a33 += ey * ey;
+            fw_ += w_val;
+            f1w += w_val * ex;
+            f2w += w_val * ey;
+        }
+    }
+    const double a21 = a12, a31 = a13, a32 = a23;
+    const double det = (a11 * a22 * a33 + a12 * a23 * a31 + a13 * a21 * a32 - a13 * a22 * a31 -
+                        a12 * a21 * a33 - a11 * a23 * a32);
+    if (count < 3 || km_fabs(det) < 1e-30 || km_fabs(det) < 1e-12 * km_fabs(a11 * a22 * a33)) {
+        const double dep = m.depth[node];
+        return w_linear(m, fw, alpha0, alpha, zp < dep ? zp : dep, node, time_dir);
+    }
+    const double inv = 1.0 / km_fabs(det);
+    const double b11 = a22 * a33 - a23 * a32;
+    const double b12 = a13 * a32 - a12 * a33;
+    const double b13 = a12 * a23 - a13 * a22;
+    return (b11 * fw_ + b12 * f1w + b13 * f2w) * inv;
 }
 
 // ---------------------------------------------------------------------------

@@ -17,6 +17,7 @@
 #include <algorithm>
 
 #include "kelp_internal.h"
+#include "kelp_step.cuh"
 
 using namespace kelp;
 
@@ -72,6 +73,7 @@ static void free_mesh(kelp_ctx* c)
 static void free_slot(FieldSlot& s)
 {
     if (s.rec) (void)hipFree(s.rec);
+    if (s.erec) (void)hipFree(s.erec);
     s = FieldSlot{};
 }
 
@@ -132,8 +134,8 @@ __global__ void pack3d_kernel(int64_t N, int64_t n_layers, const int32_t* __rest
     int64_t L = i / N, k = i - L * N;
     int64_t o = old_of_new[k];
     typename RecOf<T, true>::type r;
-    r.x = u[L * N + o];
-    r.y = v[L * N + o];
+    r.x = u ? u[L * N + o] : (T)0;
+    r.y = v ? v[L * N + o] : (T)0;
     r.z = w[L * N + o];
     r.w = w[(L + 1) * N + o];
     rec[i] = r;
@@ -152,6 +154,23 @@ __global__ void pack2d_kernel(int64_t N, const int32_t* __restrict__ old_of_new,
     rec[k] = r;
 }
 
+// element-centre u/v planes (mesh element order) into (u, v) records per
+// (layer, element) in internal element order
+template <typename T>
+__global__ void pack_elem_kernel(int64_t E, int64_t n_layers, const int32_t* __restrict__ old_of_new,
+                                 const T* __restrict__ u, const T* __restrict__ v,
+                                 typename RecOf<T, false>::type* __restrict__ rec)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= E * n_layers) return;
+    int64_t L = i / E, k = i - L * E;
+    int64_t o = old_of_new[k];
+    typename RecOf<T, false>::type r;
+    r.x = u[L * E + o];
+    r.y = v[L * E + o];
+    rec[i] = r;
+}
+
 // in-place id translation through a table (-1 stays -1)
 __global__ void map_ids_kernel(int64_t n, const int32_t* __restrict__ table, int32_t* __restrict__ ids)
 {
@@ -162,373 +181,6 @@ __global__ void map_ids_kernel(int64_t n, const int32_t* __restrict__ table, int
 }
 
 static unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
-
-// ---------------------------------------------------------------------------
-// step kernel
-// ---------------------------------------------------------------------------
-
-// One step's configuration (step_fn keywords, tracking.py:2643-2655, plus the
-// beaching hook's per-run constants, tracking.py:199).
-struct StepCfg {
-    double alpha0;
-    double alphas[4];
-    double dt, time_dir;
-    int exact, w_method, kick, beach;
-    double kh_x, kh_y;
-    uint64_t seed, real_id, step_i;
-    int use_depth, use_coastal, use_stochastic, on_failure;
-    int resuspension, min_wet_steps;
-    double depth_min, zone_depth, t_beach, rewet_hysteresis, t_resuspend, beach_dt;
-    uint64_t beach_seed;
-    int az;
-    double az_const;
-};
-
-struct PState {
-    double* __restrict__ lon;
-    double* __restrict__ lat;
-    double* __restrict__ z;
-    int32_t* __restrict__ status;
-    int32_t* __restrict__ nn;
-    int8_t* __restrict__ fr;
-    double* __restrict__ u;
-    double* __restrict__ v;
-    double* __restrict__ w;
-    const int64_t* __restrict__ pid;
-    int32_t* __restrict__ ws;
-    unsigned long long* __restrict__ alive_count;
-    int64_t n;
-};
-
-__device__ __forceinline__ double wrap_lon(double x) { return km_pymod360(x + 180.0) - 180.0; }
-
-__device__ __forceinline__ double clamp_lat(double y)
-{
-    if (y > 90.0) return 90.0;
-    if (y < -90.0) return -90.0;
-    return y;
-}
-
-__device__ __forceinline__ double z0_of(double lat)
-{
-    double z0 = km_cos(lat * kDeg2Rad) * kSLM;
-    if (z0 < 1.0) z0 = 1.0;
-    return z0;
-}
-
-__device__ __forceinline__ double clamp_z(const DMesh& m, double z, int nn)
-{
-    if (z < m.zlev[0]) z = m.zlev[0];
-    double bottom = m.depth[nn];
-    if (z > bottom) z = bottom;
-    return z;
-}
-
-// interpolate.py:1341/:1444 _interp_barycentric_core_tinterp(_3d):
-// containment around the anchor, vertex weights, blended vertex values of
-// the layer plane starting at `off`.
-template <int MAXV, typename T, bool IS3D>
-__device__ inline bool interp_uv(const DMesh& m, const Snap<T, IS3D>& f, double alpha0, double alpha,
-                                 double time_dir, double xp, double yp, int nn, int64_t off, double& U,
-                                 double& V)
-{
-    U = 0.0;
-    V = 0.0;
-    int e = find_containing<MAXV>(m, xp, yp, nn);
-    if (e < 0) return false;
-    Weights W;
-    if (!vertex_weights<MAXV>(m, e, xp, yp, W)) return false;
-    double su = 0.0, sv = 0.0;
-#pragma unroll
-    for (int k = 0; k < MAXV; ++k) {
-        if (k < W.n) {
-            double un, vn;
-            uv_blend(f, off + W.v[k], alpha0, alpha, un, vn);
-            su += W.w[k] * un;
-            sv += W.w[k] * vn;
-        }
-    }
-    U = time_dir * su;
-    V = time_dir * sv;
-    return true;
-}
-
-// One evaluation site: anchor, layer at the anchor (vertical.py:40), u/v on
-// that layer, W by the configured method (integrators.py:1260 _compute_w).
-template <int MAXV, bool IS3D, typename T>
-__device__ inline bool eval_site(const DMesh& m, const Snap<T, IS3D>& f, const StepCfg& c, double alpha,
-                                 double lon, double lat, double zp, int& nn, double& U, double& V, double& W)
-{
-    nn = site_anchor(m, lon, lat, c.exact);
-    int64_t off = 0;
-    if (IS3D) off = (int64_t)find_layer(m, zp, m.depth[nn]) * m.n_nodes;
-    bool ok = interp_uv<MAXV>(m, f, c.alpha0, alpha, c.time_dir, lon, lat, nn, off, U, V);
-    W = 0.0;
-    if constexpr (IS3D) {
-        if (c.w_method == 0)
-            W = w_linear(m, f, c.alpha0, alpha, zp, nn, c.time_dir);
-        else
-            W = w_bary<MAXV>(m, f, c.alpha0, alpha, lon, lat, zp, nn, c.time_dir);
-    }
-    return ok;
-}
-
-// The deterministic step of one alive particle, in place.  Euler follows
-// integrators.py:4997 _euler_step_unfused, RK2 the fused barycentric kernels
-// :2522 (2-D) / :3648 (3-D), RK4 :2642 (2-D) and :5760 _rk4_step_unfused
-// (3-D).  Returns the failure code (0 = alive).
-template <int ORDER, int MAXV, bool IS3D, typename T>
-__device__ inline int det_step(const DMesh& m, const Snap<T, IS3D>& f, const StepCfg& c, double& lon, double& lat,
-                               double& z, int& nn, double& Uo, double& Vo, double& Wo)
-{
-    const double dt = c.dt;
-    const double lon0 = lon, lat0 = lat, zz0 = z;
-    int node_i;
-    double U1, V1, W1;
-    bool ok = eval_site<MAXV, IS3D>(m, f, c, c.alphas[0], lon0, lat0, zz0, node_i, U1, V1, W1);
-    Uo = U1;
-    Vo = V1;
-    Wo = W1;
-    double lon_new = lon0, lat_new = lat0, z_new = zz0;
-    if (ok) {
-        if (ORDER == 1) {
-            lon_new = wrap_lon(lon0 + dt * U1 / z0_of(lat0));
-            lat_new = clamp_lat(lat0 + dt * V1 / kSLM);
-            z_new = zz0 + W1 * dt;
-        } else if (ORDER == 2) {
-            double z0 = z0_of(lat0);
-            double lon_mid = wrap_lon(lon0 + 0.5 * dt * U1 / z0);
-            double lat_mid = clamp_lat(lat0 + 0.5 * dt * V1 / kSLM);
-            double z_mid = zz0 + W1 * 0.5 * dt;
-            int nn_mid;
-            double U2, V2, W2;
-            ok = eval_site<MAXV, IS3D>(m, f, c, c.alphas[1], lon_mid, lat_mid, z_mid, nn_mid, U2, V2, W2);
-            if (ok) {
-                double z0m = z0_of(lat_mid);
-                lon_new = wrap_lon(lon0 + dt * U2 / z0m);
-                lat_new = clamp_lat(lat0 + dt * V2 / kSLM);
-                z_new = zz0 + W2 * dt;
-            }
-        } else {
-            double dlon1 = dt * U1 / z0_of(lat0);
-            double dlat1 = dt * V1 / kSLM;
-            double dz1 = dt * W1;
-            double lon2 = wrap_lon(lon0 + 0.5 * dlon1);
-            double lat2 = clamp_lat(lat0 + 0.5 * dlat1);
-            double z2 = zz0 + 0.5 * dz1;
-            int nn2, nn3, nn4;
-            double U2, V2, W2, U3, V3, W3, U4, V4, W4;
-            ok = eval_site<MAXV, IS3D>(m, f, c, c.alphas[1], lon2, lat2, z2, nn2, U2, V2, W2);
-            if (ok) {
-                double dlon2 = dt * U2 / z0_of(lat2);
-                double dlat2 = dt * V2 / kSLM;
-                double dz2 = dt * W2;
-                double lon3 = wrap_lon(lon0 + 0.5 * dlon2);
-                double lat3 = clamp_lat(lat0 + 0.5 * dlat2);
-                double z3 = zz0 + 0.5 * dz2;
-                ok = eval_site<MAXV, IS3D>(m, f, c, c.alphas[2], lon3, lat3, z3, nn3, U3, V3, W3);
-                if (ok) {
-                    double dlon3 = dt * U3 / z0_of(lat3);
-                    double dlat3 = dt * V3 / kSLM;
-                    double dz3 = dt * W3;
-                    double lon4 = wrap_lon(lon0 + dlon3);
-                    double lat4 = clamp_lat(lat0 + dlat3);
-                    double z4 = zz0 + dz3;
-                    ok = eval_site<MAXV, IS3D>(m, f, c, c.alphas[3], lon4, lat4, z4, nn4, U4, V4, W4);
-                    if (ok) {
-                        double dlon4 = dt * U4 / z0_of(lat4);
-                        double dlat4 = dt * V4 / kSLM;
-                        double dz4 = dt * W4;
-                        double dlon = (dlon1 + 2.0 * dlon2 + 2.0 * dlon3 + dlon4) / 6.0;
-                        double dlat = (dlat1 + 2.0 * dlat2 + 2.0 * dlat3 + dlat4) / 6.0;
-                        double dz = (dz1 + 2.0 * dz2 + 2.0 * dz3 + dz4) / 6.0;
-                        lon_new = wrap_lon(lon0 + dlon);
-                        lat_new = clamp_lat(lat0 + dlat);
-                        z_new = zz0 + dz;
-                    }
-                }
-            }
-        }
-    }
-    if (!ok) {
-        nn = node_i;
-        return kFrInterp;
-    }
-    int nn_new = site_anchor(m, lon_new, lat_new, c.exact);
-    int e = find_containing<MAXV>(m, lon_new, lat_new, nn_new);
-    if (e < 0) {
-        nn = node_i;
-        return kFrOOB;
-    }
-    lon = lon_new;
-    lat = lat_new;
-    if (IS3D) z = clamp_z(m, z_new, nn_new);
-    nn = nn_new;
-    return 0;
-}
-
-// integrators.py:673 _apply_split_diffusion_kick for one particle alive after
-// the deterministic step (uniform Kh, counter RNG keyed by the particle id).
-template <int MAXV, bool IS3D>
-__device__ inline void kick_one(const DMesh& m, const StepCfg& c, uint64_t pid, double pre_lon,
-                                      double pre_lat, double pre_z, int pre_nn, double& lon, double& lat,
-                                      double& z, int32_t& st, int& nn, int8_t& fr)
-{
-    double g1, g2;
-    normal_pair(c.seed, c.real_id, pid, c.step_i, 0, g1, g2);
-    double abs_dt = c.dt >= 0.0 ? c.dt : -c.dt;
-    double sqrt_abs_dt = km_sqrt(abs_dt);
-    double sigma_x = km_sqrt(2.0 * c.kh_x);
-    double sigma_y = km_sqrt(2.0 * c.kh_y);
-    double dx = sigma_x * sqrt_abs_dt * g1;
-    double dy = sigma_y * sqrt_abs_dt * g2;
-    if (c.kh_x != c.kh_y) {
-        double la = lat;
-        if (la < -89.9) la = -89.9;
-        if (la > 89.9) la = 89.9;
-        double corr = (c.kh_x - c.kh_y) * (km_sin(la * kDeg2Rad) / km_cos(la * kDeg2Rad)) / kRMetric *
-                      km_fabs(c.dt);
-        dy = dy + corr;
-    }
-    double lon_k, lat_k;
-    sphere_displace(lon, lat, dx, dy, lon_k, lat_k);
-    int nn_k = site_anchor(m, lon_k, lat_k, c.exact == 1 ? 1 : 2);  // batch find_nearest_nodes
-    int e = find_containing<MAXV>(m, lon_k, lat_k, nn_k);
-    if (e < 0) {
-        lon = pre_lon;
-        lat = pre_lat;
-        z = pre_z;
-        nn = pre_nn;
-        st = kOutside;
-        fr = kFrOOB;
-        return;
-    }
-    lon = lon_k;
-    lat = lat_k;
-    nn = nn_k;
-    if (IS3D) z = clamp_z(m, z, nn_k);
-}
-
-// integrators.py:842 _apply_split_az_kick, constant source on a z-level
-// mesh (numerics/diffusion.py:427 _loop_az_kick_zlevel, gradient 0, and
-// :302 diffusion_az_step_meters): one particle alive after the horizontal
-// kick, folded back into [z_levels[0], depth[nn]].
-__device__ inline void az_one(const DMesh& m, const StepCfg& c, uint64_t pid, int node, double& z)
-{
-    double xi, unused;
-    normal_pair(c.seed, c.real_id, pid, c.step_i, 1, xi, unused);
-    const double sdt = c.time_dir * c.dt;
-    const double abs_dt = sdt >= 0.0 ? sdt : -sdt;
-    const double g = 0.0;
-    const double dz = g * abs_dt + km_sqrt(2.0 * c.az_const * abs_dt) * xi;
-    z = km_reflect_depth(z + dz, m.zlev[0], m.depth[node]);
-}
-
-// tracking.py:199 _apply_beaching_hook + numerics/beaching.py:95
-// evaluate_beaching, one particle (prev = status at the step start).
-template <int MAXV>
-__device__ inline void beach_one(const DMesh& m, const StepCfg& c, uint64_t pid, int32_t prev, double lon,
-                                       double lat, int32_t& st, int& nn, int8_t& fr, int32_t& ws)
-{
-    const bool need_topo = c.use_coastal || c.use_stochastic;
-    const bool newly_failed = (st == kOutside) && (prev == kAlive);
-    if (newly_failed && nn < 0) nn = site_anchor(m, lon, lat, c.exact == 1 ? 1 : 2);
-    const int n = nn;
-    int containing = -1;
-    if (need_topo && (st == kAlive || st == kBeached || newly_failed) && n >= 0)
-        containing = find_containing<MAXV>(m, lon, lat, n);
-    const bool coastal = need_topo && containing >= 0 && m.coastal != nullptr && m.coastal[containing];
-    const double h_raw = n >= 0 ? m.depth[n] : __builtin_inf();
-    if (c.on_failure && newly_failed) {
-        if (h_raw < c.zone_depth || coastal) {
-            st = kBeached;
-            fr = 0;
-        }
-    }
-    const bool alive = st == kAlive, beached = st == kBeached;
-    bool to_beach = false;
-    if (c.use_depth) to_beach |= h_raw < c.depth_min;
-    if (c.use_coastal) to_beach |= coastal;
-    if (c.use_stochastic && c.t_beach > 0) {
-        bool in_zone = h_raw < c.zone_depth || coastal;
-        double p = 1.0 - km_exp(-km_fabs(c.beach_dt) / c.t_beach);
-        if (alive && in_zone && u01(c.beach_seed, c.real_id, pid, c.step_i, 10) < p) to_beach = true;
-    }
-    to_beach = to_beach && alive;
-    bool to_release = false;
-    int32_t w = ws;
-    if (c.resuspension != 0 && beached) {
-        bool rewet = h_raw > (c.depth_min + c.rewet_hysteresis) && n >= 0;
-        w = rewet ? w + 1 : 0;
-        if (c.resuspension == 1 || c.resuspension == 3) to_release |= w >= c.min_wet_steps;
-        if ((c.resuspension == 2 || c.resuspension == 3) && c.t_resuspend > 0) {
-            double p_r = 1.0 - km_exp(-km_fabs(c.beach_dt) / c.t_resuspend);
-            if (rewet && u01(c.beach_seed, c.real_id, pid, c.step_i, 11) < p_r) to_release = true;
-        }
-    }
-    if (to_release) w = 0;
-    if (!beached) w = 0;
-    ws = w;
-    if (to_beach) st = kBeached;
-    if (to_release) {
-        st = kAlive;
-        fr = 0;
-    }
-}
-
-// One full step per particle: deterministic step, split Kh kick, beaching.
-// Waves per SIMD the compiler budgets registers for (0: its own choice).
-#ifndef KELP_STEP_WAVES
-#define KELP_STEP_WAVES 3
-#endif
-#if KELP_STEP_WAVES > 0
-#define KELP_STEP_ATTR __attribute__((amdgpu_waves_per_eu(KELP_STEP_WAVES)))
-#else
-#define KELP_STEP_ATTR
-#endif
-
-template <int ORDER, int MAXV, bool IS3D, typename T>
-__global__ __launch_bounds__(256) KELP_STEP_ATTR void step_kernel(DMesh m, Snap<T, IS3D> f, StepCfg c, PState s)
-{
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= s.n) return;
-    const int32_t prev = s.status[i];
-    if (s.alive_count) {
-        // particles alive at the step start: one atomic per wavefront
-        unsigned long long alive = __ballot(prev == kAlive);
-        unsigned long long active = __ballot(1);
-        if ((int)(threadIdx.x & 63) == __ffsll((long long)active) - 1 && alive)
-            atomicAdd(s.alive_count, (unsigned long long)__popcll(alive));
-    }
-    int32_t st = prev;
-    double lon = s.lon[i], lat = s.lat[i];
-    double z = IS3D ? s.z[i] : 0.0;
-    int nn = s.nn[i];
-    int8_t fr = s.fr[i];
-    double U = 0.0, V = 0.0, W = 0.0;
-    if (prev == kAlive) {
-        const double pre_lon = lon, pre_lat = lat, pre_z = z;
-        const int pre_nn = nn;
-        int code = det_step<ORDER, MAXV, IS3D>(m, f, c, lon, lat, z, nn, U, V, W);
-        if (code) {
-            st = kOutside;
-            fr = (int8_t)code;
-        } else if (c.kick && !(c.kh_x == 0.0 && c.kh_y == 0.0)) {
-            kick_one<MAXV, IS3D>(m, c, (uint64_t)s.pid[i], pre_lon, pre_lat, pre_z, pre_nn, lon, lat, z, st, nn, fr);
-        }
-        if (IS3D && c.az && st == kAlive) az_one(m, c, (uint64_t)s.pid[i], nn, z);
-    }
-    if (c.beach) beach_one<MAXV>(m, c, (uint64_t)s.pid[i], prev, lon, lat, st, nn, fr, s.ws[i]);
-    s.lon[i] = lon;
-    s.lat[i] = lat;
-    if (IS3D) s.z[i] = z;
-    s.status[i] = st;
-    s.nn[i] = nn;
-    s.fr[i] = fr;
-    s.u[i] = U;
-    s.v[i] = V;
-    if (IS3D && s.w) s.w[i] = W;
-}
 
 // batch anchors (core/mesh.py:1896 find_nearest_nodes), mesh node ids out
 __global__ void nearest_kernel(DMesh m, int64_t n, const double* __restrict__ lon,
@@ -586,47 +238,41 @@ __global__ void math_probe_kernel(int fn, int64_t n, const double* __restrict__ 
 }
 
 template <typename T, bool IS3D>
-static Snap<T, IS3D> snap_of(const FieldSlot& a, const FieldSlot* b, const FieldSlot* t)
+static Snap<T, IS3D> snap_of(const void* a, const void* b, const void* t)
 {
     using R = typename RecOf<T, IS3D>::type;
     Snap<T, IS3D> f;
-    f.a = static_cast<const R*>(a.rec);
-    f.b = b ? static_cast<const R*>(b->rec) : f.a;
-    f.t = t ? static_cast<const R*>(t->rec) : f.a;
+    f.a = static_cast<const R*>(a);
+    f.b = b ? static_cast<const R*>(b) : f.a;
+    f.t = t ? static_cast<const R*>(t) : f.a;
     return f;
 }
 
-template <int MAXV, bool IS3D, typename T>
-static void launch_step_v(kelp_ctx* c, int order, const Snap<T, IS3D>& f, const StepCfg& cfg, const PState& ps)
-{
-    unsigned g = grid_for(ps.n, 256);
-    if (order == 1)
-        step_kernel<1, MAXV, IS3D, T><<<g, 256, 0, c->stream>>>(c->m, f, cfg, ps);
-    else if (order == 2)
-        step_kernel<2, MAXV, IS3D, T><<<g, 256, 0, c->stream>>>(c->m, f, cfg, ps);
-    else
-        step_kernel<4, MAXV, IS3D, T><<<g, 256, 0, c->stream>>>(c->m, f, cfg, ps);
-}
-
-template <bool IS3D, typename T>
+template <bool IS3D, typename T, bool ELEM>
 static void launch_step_t(kelp_ctx* c, int order, const FieldSlot& a, const FieldSlot* b, const FieldSlot* t,
                           const StepCfg& cfg, const PState& ps)
 {
-    Snap<T, IS3D> f = snap_of<T, IS3D>(a, b, t);
+    Snap<T, IS3D> f = snap_of<T, IS3D>(a.rec, b ? b->rec : nullptr, t ? t->rec : nullptr);
+    Snap<T, false> fe = snap_of<T, false>(a.erec, b ? b->erec : nullptr, t ? t->erec : nullptr);
     if (c->m.max_v == 3)
-        launch_step_v<3, IS3D, T>(c, order, f, cfg, ps);
+        launch_step_v<3, IS3D, T, ELEM>(c->stream, order, c->m, f, fe, cfg, ps);
     else
-        launch_step_v<4, IS3D, T>(c, order, f, cfg, ps);
+        launch_step_v<4, IS3D, T, ELEM>(c->stream, order, c->m, f, fe, cfg, ps);
 }
 
 template <typename T>
 static void launch_step(kelp_ctx* c, int order, bool is3d, const FieldSlot& a, const FieldSlot* b,
                         const FieldSlot* t, const StepCfg& cfg, const PState& ps)
 {
-    if (is3d)
-        launch_step_t<true, T>(c, order, a, b, t, cfg, ps);
+    const bool elem = a.layout == 1;
+    if (is3d && elem)
+        launch_step_t<true, T, true>(c, order, a, b, t, cfg, ps);
+    else if (is3d)
+        launch_step_t<true, T, false>(c, order, a, b, t, cfg, ps);
+    else if (elem)
+        launch_step_t<false, T, true>(c, order, a, b, t, cfg, ps);
     else
-        launch_step_t<false, T>(c, order, a, b, t, cfg, ps);
+        launch_step_t<false, T, false>(c, order, a, b, t, cfg, ps);
 }
 
 static const FieldSlot* opt_slot(kelp_ctx* c, int k)
@@ -765,6 +411,8 @@ int kelp_mesh_upload(kelp_ctx* c, const kelp_mesh_desc* d)
     }
     std::vector<int4> elems(E);
     std::vector<uint8_t> coastal(d->host_elem_coastal ? E : 0);
+    const bool has_centres = d->host_elem_center_lon && d->host_elem_center_lat;
+    std::vector<double2> ecent(has_centres ? E : 0);
     for (int64_t en = 0; en < E; ++en) {
         int64_t eo = e_old_of_new[en];
         int v[4] = {-1, -1, -1, -1};
@@ -774,6 +422,7 @@ int kelp_mesh_upload(kelp_ctx* c, const kelp_mesh_desc* d)
         }
         elems[en] = make_int4(v[0], v[1], v[2], v[3]);
         if (d->host_elem_coastal) coastal[en] = d->host_elem_coastal[eo] ? 1 : 0;
+        if (has_centres) ecent[en] = make_double2(d->host_elem_center_lon[eo], d->host_elem_center_lat[eo]);
     }
     std::vector<int32_t> n2e_off(N + 1);
     n2e_off[0] = 0;
@@ -804,6 +453,7 @@ int kelp_mesh_upload(kelp_ctx* c, const kelp_mesh_desc* d)
     int4* d_elems;
     double *d_depth, *d_zlev;
     uint8_t* d_coastal = nullptr;
+    double2* d_ecent = nullptr;
     int rc;
     if ((rc = dev_alloc(c, &d_lonlat, N)) || (rc = dev_alloc(c, &d_xyz, N)) ||
         (rc = dev_alloc(c, &d_off, n_cells + 1)) || (rc = dev_alloc(c, &d_elems, E)) ||
@@ -811,7 +461,8 @@ int kelp_mesh_upload(kelp_ctx* c, const kelp_mesh_desc* d)
         (rc = dev_alloc(c, &d_depth, N)) || (rc = dev_alloc(c, &d_zlev, d->n_levels > 0 ? d->n_levels : 1)) ||
         (rc = dev_alloc(c, &d_nodeo, N)) || (rc = dev_alloc(c, &d_noden, N)) ||
         (rc = dev_alloc(c, &d_elemo, E)) || (rc = dev_alloc(c, &d_elemn, E)) ||
-        (d->host_elem_coastal && (rc = dev_alloc(c, &d_coastal, E)))) {
+        (d->host_elem_coastal && (rc = dev_alloc(c, &d_coastal, E))) ||
+        (has_centres && (rc = dev_alloc(c, &d_ecent, E)))) {
         free_mesh(c);
         return rc;
     }
@@ -828,6 +479,7 @@ int kelp_mesh_upload(kelp_ctx* c, const kelp_mesh_desc* d)
     if (d->n_levels > 0)
         HIP_TRY(hipMemcpy(d_zlev, d->host_z_levels, d->n_levels * sizeof(double), hipMemcpyHostToDevice));
     if (d_coastal) HIP_TRY(hipMemcpy(d_coastal, coastal.data(), E * sizeof(uint8_t), hipMemcpyHostToDevice));
+    if (d_ecent) HIP_TRY(hipMemcpy(d_ecent, ecent.data(), E * sizeof(double2), hipMemcpyHostToDevice));
     node_xyz_kernel<<<grid_for(N, 256), 256, 0, c->stream>>>(N, d_lonlat, d_xyz);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -841,6 +493,7 @@ int kelp_mesh_upload(kelp_ctx* c, const kelp_mesh_desc* d)
     m.depth = d_depth;
     m.zlev = d_zlev;
     m.coastal = d_coastal;
+    m.ecent = d_ecent;
     m.cell = d->gh_cell;
     m.lon_min = d->gh_lon_min;
     m.lat_min = d->gh_lat_min;
@@ -898,49 +551,73 @@ int kelp_nodes_from_mesh_ids(kelp_ctx* c, int64_t n, int32_t* ids)
     return KELP_OK;
 }
 
-int kelp_field_upload(kelp_ctx* c, int slot, int dtype, int64_t n_planes, const void* hu, const void* hv,
-                      const void* hw, int64_t n_w_planes)
+// Upload one snapshot: u/v planes at the nodes (layout 0) or at the element
+// centres (layout 1), W planes at the nodes; packed on the device.
+static int field_upload(kelp_ctx* c, const char* fn, int slot, int dtype, int layout, int64_t n_planes,
+                        const void* hu, const void* hv, const void* hw, int64_t n_w_planes)
 {
-    if (!c) return fail(KELP_EINVAL, "kelp_field_upload: NULL context");
-    if (!c->has_mesh) return fail(KELP_ESTATE, "kelp_field_upload: upload the mesh first");
-    if (slot < 0 || slot >= kMaxSlots) return fail(KELP_EINVAL, "kelp_field_upload: slot %d not in [0, %d)", slot, kMaxSlots);
-    if (dtype != KELP_DT_F64 && dtype != KELP_DT_F32) return fail(KELP_EINVAL, "kelp_field_upload: bad dtype %d", dtype);
-    if (n_planes < 1 || !hu || !hv) return fail(KELP_EINVAL, "kelp_field_upload: u/v missing");
+    if (!c) return fail(KELP_EINVAL, "%s: NULL context", fn);
+    if (!c->has_mesh) return fail(KELP_ESTATE, "%s: upload the mesh first", fn);
+    if (slot < 0 || slot >= kMaxSlots) return fail(KELP_EINVAL, "%s: slot %d not in [0, %d)", fn, slot, kMaxSlots);
+    if (dtype != KELP_DT_F64 && dtype != KELP_DT_F32) return fail(KELP_EINVAL, "%s: bad dtype %d", fn, dtype);
+    if (n_planes < 1 || !hu || !hv) return fail(KELP_EINVAL, "%s: u/v missing", fn);
     const bool is3d = hw != nullptr;
-    if (is3d && n_w_planes != n_planes + 1)
-        return fail(KELP_EINVAL, "kelp_field_upload: W needs n_planes + 1 interface planes");
-    if (!is3d && n_planes != 1) return fail(KELP_EINVAL, "kelp_field_upload: layered u/v need W (3-D)");
+    if (is3d && n_w_planes != n_planes + 1) return fail(KELP_EINVAL, "%s: W needs n_planes + 1 interface planes", fn);
+    if (!is3d && n_planes != 1) return fail(KELP_EINVAL, "%s: layered u/v need W (3-D)", fn);
+    if (layout == 1 && !c->m.ecent)
+        return fail(KELP_ESTATE, "%s: element-centre u/v need the mesh's element centres", fn);
     HIP_TRY(hipSetDevice(c->device));
     FieldSlot& s = c->slots[slot];
     free_slot(s);
-    const int64_t N = c->m.n_nodes;
+    const int64_t N = c->m.n_nodes, E = c->m.n_elems;
+    const int64_t M = layout == 1 ? E : N;  // u/v values per plane
     const size_t es = dtype == KELP_DT_F64 ? 8 : 4;
-    const size_t uv_bytes = (size_t)n_planes * N * es;
+    const size_t uv_bytes = (size_t)n_planes * M * es;
     const size_t w_bytes = is3d ? (size_t)n_w_planes * N * es : 0;
-    const size_t rec_bytes = (size_t)n_planes * N * es * (is3d ? 4 : 2);
     void *tu = nullptr, *tv = nullptr, *tw = nullptr;
-    HIP_TRY(hipMalloc(&s.rec, rec_bytes));
-    hipError_t e = hipMalloc(&tu, uv_bytes);
+    hipError_t e = hipSuccess;
+    if (is3d || layout == 0) e = hipMalloc(&s.rec, (size_t)n_planes * N * es * (is3d ? 4 : 2));
+    if (e == hipSuccess && layout == 1) e = hipMalloc(&s.erec, (size_t)n_planes * E * es * 2);
+    if (e == hipSuccess) e = hipMalloc(&tu, uv_bytes);
     if (e == hipSuccess) e = hipMalloc(&tv, uv_bytes);
     if (e == hipSuccess && is3d) e = hipMalloc(&tw, w_bytes);
     if (e == hipSuccess) e = hipMemcpyAsync(tu, hu, uv_bytes, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(tv, hv, uv_bytes, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess && is3d) e = hipMemcpyAsync(tw, hw, w_bytes, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) {
-        const int64_t n = is3d ? N * n_planes : N;
-        const unsigned g = grid_for(n, 256);
-        if (is3d && dtype == KELP_DT_F32)
-            pack3d_kernel<float><<<g, 256, 0, c->stream>>>(N, n_planes, c->node_old_of_new, (const float*)tu,
-                                                           (const float*)tv, (const float*)tw, (float4*)s.rec);
-        else if (is3d)
-            pack3d_kernel<double><<<g, 256, 0, c->stream>>>(N, n_planes, c->node_old_of_new, (const double*)tu,
-                                                            (const double*)tv, (const double*)tw, (double4*)s.rec);
-        else if (dtype == KELP_DT_F32)
-            pack2d_kernel<float><<<g, 256, 0, c->stream>>>(N, c->node_old_of_new, (const float*)tu,
-                                                           (const float*)tv, (float2*)s.rec);
-        else
-            pack2d_kernel<double><<<g, 256, 0, c->stream>>>(N, c->node_old_of_new, (const double*)tu,
-                                                            (const double*)tv, (double2*)s.rec);
+        const bool f32 = dtype == KELP_DT_F32;
+        // node records: u/v (node layout) and the W pair (3-D)
+        const void* nu = layout == 0 ? tu : nullptr;
+        const void* nv = layout == 0 ? tv : nullptr;
+        if (is3d) {
+            const unsigned g = grid_for(N * n_planes, 256);
+            if (f32)
+                pack3d_kernel<float><<<g, 256, 0, c->stream>>>(N, n_planes, c->node_old_of_new, (const float*)nu,
+                                                               (const float*)nv, (const float*)tw, (float4*)s.rec);
+            else
+                pack3d_kernel<double><<<g, 256, 0, c->stream>>>(N, n_planes, c->node_old_of_new,
+                                                                (const double*)nu, (const double*)nv,
+                                                                (const double*)tw, (double4*)s.rec);
+        } else if (layout == 0) {
+            const unsigned g = grid_for(N, 256);
+            if (f32)
+                pack2d_kernel<float><<<g, 256, 0, c->stream>>>(N, c->node_old_of_new, (const float*)tu,
+                                                               (const float*)tv, (float2*)s.rec);
+            else
+                pack2d_kernel<double><<<g, 256, 0, c->stream>>>(N, c->node_old_of_new, (const double*)tu,
+                                                                (const double*)tv, (double2*)s.rec);
+        }
+        if (layout == 1) {
+            const unsigned g = grid_for(E * n_planes, 256);
+            if (f32)
+                pack_elem_kernel<float><<<g, 256, 0, c->stream>>>(E, n_planes, c->elem_old_of_new,
+                                                                  (const float*)tu, (const float*)tv,
+                                                                  (float2*)s.erec);
+            else
+                pack_elem_kernel<double><<<g, 256, 0, c->stream>>>(E, n_planes, c->elem_old_of_new,
+                                                                   (const double*)tu, (const double*)tv,
+                                                                   (double2*)s.erec);
+        }
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -949,13 +626,26 @@ int kelp_field_upload(kelp_ctx* c, int slot, int dtype, int64_t n_planes, const 
     if (tw) (void)hipFree(tw);
     if (e != hipSuccess) {
         free_slot(s);
-        return fail(e == hipErrorOutOfMemory ? KELP_ENOMEM : KELP_EHIP, "kelp_field_upload: %s", hipGetErrorString(e));
+        return fail(e == hipErrorOutOfMemory ? KELP_ENOMEM : KELP_EHIP, "%s: %s", fn, hipGetErrorString(e));
     }
     s.dtype = dtype;
+    s.layout = layout;
     s.n_planes = n_planes;
     s.n_w_planes = is3d ? n_w_planes : 0;
     s.is3d = is3d;
     return KELP_OK;
+}
+
+int kelp_field_upload(kelp_ctx* c, int slot, int dtype, int64_t n_planes, const void* hu, const void* hv,
+                      const void* hw, int64_t n_w_planes)
+{
+    return field_upload(c, "kelp_field_upload", slot, dtype, 0, n_planes, hu, hv, hw, n_w_planes);
+}
+
+int kelp_field_upload_elem(kelp_ctx* c, int slot, int dtype, int64_t n_planes, const void* hu, const void* hv,
+                           const void* hw, int64_t n_w_planes)
+{
+    return field_upload(c, "kelp_field_upload_elem", slot, dtype, 1, n_planes, hu, hv, hw, n_w_planes);
 }
 
 int kelp_field_clear(kelp_ctx* c, int slot)
@@ -975,8 +665,8 @@ int kelp_step(kelp_ctx* c, const kelp_particles* p, const kelp_step_args* a)
         return fail(KELP_EINVAL, "kelp_step: order must be 1, 2 or 4 (got %d)", a->order);
     if (!(a->dt > 0.0)) return fail(KELP_EINVAL, "kelp_step: dt must be positive (direction goes in time_dir)");
     if (a->time_dir != 1.0 && a->time_dir != -1.0) return fail(KELP_EINVAL, "kelp_step: time_dir must be +1 or -1");
-    if (a->w_method != 0 && a->w_method != 2)
-        return fail(KELP_EINVAL, "kelp_step: w_method must be 0 (nearest) or 2 (barycentric)");
+    if (a->w_method < 0 || a->w_method > 2)
+        return fail(KELP_EINVAL, "kelp_step: w_method must be 0 (nearest), 1 (leastsq) or 2 (barycentric)");
     if (p->n < 0) return fail(KELP_EINVAL, "kelp_step: negative particle count");
     if (p->n == 0) return KELP_OK;
     if (!p->lon || !p->lat || !p->status || !p->nn || !p->fr || !p->u || !p->v)
@@ -998,8 +688,9 @@ int kelp_step(kelp_ctx* c, const kelp_particles* p, const kelp_step_args* a)
     for (const FieldSlot* s : {sb, st}) {
         if (!s) continue;
         if (s->dtype < 0) return fail(KELP_ESTATE, "kelp_step: slot_b/slot_t is empty");
-        if (s->dtype != sa.dtype || s->n_planes != sa.n_planes || s->n_w_planes != sa.n_w_planes)
-            return fail(KELP_EINVAL, "kelp_step: snapshot slots differ in dtype or shape");
+        if (s->dtype != sa.dtype || s->layout != sa.layout || s->n_planes != sa.n_planes ||
+            s->n_w_planes != sa.n_w_planes)
+            return fail(KELP_EINVAL, "kelp_step: snapshot slots differ in dtype, layout or shape");
     }
     if (a->alpha0 != 0.0 && !sb) return fail(KELP_EINVAL, "kelp_step: alpha0 != 0 needs slot_b");
     for (int k = 0; k < a->order; ++k)

@@ -25,12 +25,16 @@ constexpr int kMaxSlots = 16;
 constexpr int kMaxEvents = 16;
 
 // One snapshot in HBM as packed records (see kelp_device.cuh, Snap): 3-D
-// (u, v, w_top, w_bottom) per (layer, node), 2-D (u, v) per node.
+// (u, v, w_top, w_bottom) per (layer, node), 2-D (u, v) per node.  With u/v
+// at element centres (layout 1) the node records carry only the W pair (u, v
+// zero) and `erec` holds (u, v) per (layer, element).
 struct FieldSlot {
     int dtype = -1;  // -1: empty
+    int layout = 0;  // 0: u/v at nodes, 1: u/v at element centres
     int64_t n_planes = 0, n_w_planes = 0;
     bool is3d = false;
     void* rec = nullptr;
+    void* erec = nullptr;
 };
 
 // growable device buffer owned by a context

@@ -1,0 +1,55 @@
+// kelp_step.cuh -- the per-step configuration shared by the C ABI (host) and
+// the step kernels, and the declaration of the kernel launchers.
+//
+// The step kernel is instantiated for ORDER x MAXV x 3-D x field dtype x u/v
+// layout.  The instantiations are spread over several translation units
+// (kelp_step_variant.hip compiled once per (3-D, dtype, layout)), so the
+// library builds in parallel; this header only declares the launchers.
+#pragma once
+
+#include "kelp_device.cuh"
+
+namespace kelp {
+
+// One step's configuration (step_fn keywords, tracking.py:2643-2655, plus the
+// beaching hook's per-run constants, tracking.py:199).
+struct StepCfg {
+    double alpha0;
+    double alphas[4];
+    double dt, time_dir;
+    int exact, w_method, kick, beach;
+    double kh_x, kh_y;
+    uint64_t seed, real_id, step_i;
+    int use_depth, use_coastal, use_stochastic, on_failure;
+    int resuspension, min_wet_steps;
+    double depth_min, zone_depth, t_beach, rewet_hysteresis, t_resuspend, beach_dt;
+    uint64_t beach_seed;
+    int az;           // vertical Az kick, constant source
+    double az_const;
+};
+
+struct PState {
+    double* __restrict__ lon;
+    double* __restrict__ lat;
+    double* __restrict__ z;
+    int32_t* __restrict__ status;
+    int32_t* __restrict__ nn;
+    int8_t* __restrict__ fr;
+    double* __restrict__ u;
+    double* __restrict__ v;
+    double* __restrict__ w;
+    const int64_t* __restrict__ pid;
+    int32_t* __restrict__ ws;
+    unsigned long long* __restrict__ alive_count;
+    int64_t n;
+};
+
+// Launch step_kernel<order, MAXV, IS3D, T, ELEM> over all particles on
+// `stream`.  `f`: the node snapshot records (u, v[, w pair]); `fe`: the
+// element-centre u/v records when ELEM (u/v at element centres, least
+// squares), else unused.
+template <int MAXV, bool IS3D, typename T, bool ELEM>
+void launch_step_v(hipStream_t stream, int order, const DMesh& m, const Snap<T, IS3D>& f,
+                   const Snap<T, false>& fe, const StepCfg& cfg, const PState& ps);
+
+}  // namespace kelp

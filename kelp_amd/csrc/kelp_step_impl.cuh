@@ -1,0 +1,379 @@
+// kelp_step_impl.cuh -- the step kernel: one particle-step per thread.
+//
+// Included only by kelp_step_variant.hip, which instantiates the launchers
+// of one (3-D, dtype, layout) combination per object file.
+#pragma once
+
+#include "kelp_step.cuh"
+
+namespace kelp {
+
+__device__ __forceinline__ double wrap_lon(double x) { return km_pymod360(x + 180.0) - 180.0; }
+
+__device__ __forceinline__ double clamp_lat(double y)
+{
+    if (y > 90.0) return 90.0;
+    if (y < -90.0) return -90.0;
+    return y;
+}
+
+__device__ __forceinline__ double z0_of(double lat)
+{
+    double z0 = km_cos(lat * kDeg2Rad) * kSLM;
+    if (z0 < 1.0) z0 = 1.0;
+    return z0;
+}
+
+__device__ __forceinline__ double clamp_z(const DMesh& m, double z, int nn)
+{
+    if (z < m.zlev[0]) z = m.zlev[0];
+    double bottom = m.depth[nn];
+    if (z > bottom) z = bottom;
+    return z;
+}
+
+// interpolate.py:1341/:1444 _interp_barycentric_core_tinterp(_3d):
+// containment around the anchor, vertex weights, blended vertex values of
+// the layer plane starting at `off`.
+template <int MAXV, typename T, bool IS3D>
+__device__ inline bool interp_uv(const DMesh& m, const Snap<T, IS3D>& f, double alpha0, double alpha,
+                                 double time_dir, double xp, double yp, int nn, int64_t off, double& U,
+                                 double& V)
+{
+    U = 0.0;
+    V = 0.0;
+    int e = find_containing<MAXV>(m, xp, yp, nn);
+    if (e < 0) return false;
+    Weights W;
+    if (!vertex_weights<MAXV>(m, e, xp, yp, W)) return false;
+    double su = 0.0, sv = 0.0;
+#pragma unroll
+    for (int k = 0; k < MAXV; ++k) {
+        if (k < W.n) {
+            double un, vn;
+            uv_blend(f, off + W.v[k], alpha0, alpha, un, vn);
+            su += W.w[k] * un;
+            sv += W.w[k] * vn;
+        }
+    }
+    U = time_dir * su;
+    V = time_dir * sv;
+    return true;
+}
+
+// One evaluation site: anchor, layer at the anchor (vertical.py:40), u/v on
+// that layer (node layout: barycentric in the containing element; element
+// layout: least squares over the anchor's element centres), W by the
+// configured method (integrators.py:1260 _compute_w).
+template <int MAXV, bool IS3D, typename T, bool ELEM>
+__device__ inline bool eval_site(const DMesh& m, const Snap<T, IS3D>& f, const Snap<T, false>& fe,
+                                 const StepCfg& c, double alpha, double lon, double lat, double zp, int& nn,
+                                 double& U, double& V, double& W)
+{
+    nn = site_anchor(m, lon, lat, c.exact);
+    int64_t layer = 0;
+    if (IS3D) layer = find_layer(m, zp, m.depth[nn]);
+    bool ok;
+    if constexpr (ELEM)
+        ok = interp_lsq(m, fe, c.alpha0, alpha, c.time_dir, lon, lat, nn, layer * m.n_elems, U, V);
+    else
+        ok = interp_uv<MAXV>(m, f, c.alpha0, alpha, c.time_dir, lon, lat, nn, layer * m.n_nodes, U, V);
+    W = 0.0;
+    if constexpr (IS3D) {
+        if (c.w_method == 0)
+            W = w_linear(m, f, c.alpha0, alpha, zp, nn, c.time_dir);
+        else if (c.w_method == 1)
+            W = w_lsq<MAXV>(m, f, c.alpha0, alpha, lon, lat, zp, nn, c.time_dir);
+        else
+            W = w_bary<MAXV>(m, f, c.alpha0, alpha, lon, lat, zp, nn, c.time_dir);
+    }
+    return ok;
+}
+
+// The deterministic step of one alive particle, in place.  Euler follows
+// integrators.py:4997 _euler_step_unfused, RK2 the fused barycentric kernels
+// :2522 (2-D) / :3648 (3-D), RK4 :2642 (2-D) and :5760 _rk4_step_unfused
+// (3-D).  Returns the failure code (0 = alive).
+template <int ORDER, int MAXV, bool IS3D, typename T, bool ELEM>
+__device__ inline int det_step(const DMesh& m, const Snap<T, IS3D>& f, const Snap<T, false>& fe, const StepCfg& c,
+                               double& lon, double& lat, double& z, int& nn, double& Uo, double& Vo, double& Wo)
+{
+    const double dt = c.dt;
+    const double lon0 = lon, lat0 = lat, zz0 = z;
+    int node_i;
+    double U1, V1, W1;
+    bool ok = eval_site<MAXV, IS3D, T, ELEM>(m, f, fe, c, c.alphas[0], lon0, lat0, zz0, node_i, U1, V1, W1);
+    Uo = U1;
+    Vo = V1;
+    Wo = W1;
+    double lon_new = lon0, lat_new = lat0, z_new = zz0;
+    if (ok) {
+        if (ORDER == 1) {
+            lon_new = wrap_lon(lon0 + dt * U1 / z0_of(lat0));
+            lat_new = clamp_lat(lat0 + dt * V1 / kSLM);
+            z_new = zz0 + W1 * dt;
+        } else if (ORDER == 2) {
+            double z0 = z0_of(lat0);
+            double lon_mid = wrap_lon(lon0 + 0.5 * dt * U1 / z0);
+            double lat_mid = clamp_lat(lat0 + 0.5 * dt * V1 / kSLM);
+            double z_mid = zz0 + W1 * 0.5 * dt;
+            int nn_mid;
+            double U2, V2, W2;
+            ok = eval_site<MAXV, IS3D, T, ELEM>(m, f, fe, c, c.alphas[1], lon_mid, lat_mid, z_mid, nn_mid, U2, V2, W2);
+            if (ok) {
+                double z0m = z0_of(lat_mid);
+                lon_new = wrap_lon(lon0 + dt * U2 / z0m);
+                lat_new = clamp_lat(lat0 + dt * V2 / kSLM);
+                z_new = zz0 + W2 * dt;
+            }
+        } else {
+            double dlon1 = dt * U1 / z0_of(lat0);
+            double dlat1 = dt * V1 / kSLM;
+            double dz1 = dt * W1;
+            double lon2 = wrap_lon(lon0 + 0.5 * dlon1);
+            double lat2 = clamp_lat(lat0 + 0.5 * dlat1);
+            double z2 = zz0 + 0.5 * dz1;
+            int nn2, nn3, nn4;
+            double U2, V2, W2, U3, V3, W3, U4, V4, W4;
+            ok = eval_site<MAXV, IS3D, T, ELEM>(m, f, fe, c, c.alphas[1], lon2, lat2, z2, nn2, U2, V2, W2);
+            if (ok) {
+                double dlon2 = dt * U2 / z0_of(lat2);
+                double dlat2 = dt * V2 / kSLM;
+                double dz2 = dt * W2;
+                double lon3 = wrap_lon(lon0 + 0.5 * dlon2);
+                double lat3 = clamp_lat(lat0 + 0.5 * dlat2);
+                double z3 = zz0 + 0.5 * dz2;
+                ok = eval_site<MAXV, IS3D, T, ELEM>(m, f, fe, c, c.alphas[2], lon3, lat3, z3, nn3, U3, V3, W3);
+                if (ok) {
+                    double dlon3 = dt * U3 / z0_of(lat3);
+                    double dlat3 = dt * V3 / kSLM;
+                    double dz3 = dt * W3;
+                    double lon4 = wrap_lon(lon0 + dlon3);
+                    double lat4 = clamp_lat(lat0 + dlat3);
+                    double z4 = zz0 + dz3;
+                    ok = eval_site<MAXV, IS3D, T, ELEM>(m, f, fe, c, c.alphas[3], lon4, lat4, z4, nn4, U4, V4, W4);
+                    if (ok) {
+                        double dlon4 = dt * U4 / z0_of(lat4);
+                        double dlat4 = dt * V4 / kSLM;
+                        double dz4 = dt * W4;
+                        double dlon = (dlon1 + 2.0 * dlon2 + 2.0 * dlon3 + dlon4) / 6.0;
+                        double dlat = (dlat1 + 2.0 * dlat2 + 2.0 * dlat3 + dlat4) / 6.0;
+                        double dz = (dz1 + 2.0 * dz2 + 2.0 * dz3 + dz4) / 6.0;
+                        lon_new = wrap_lon(lon0 + dlon);
+                        lat_new = clamp_lat(lat0 + dlat);
+                        z_new = zz0 + dz;
+                    }
+                }
+            }
+        }
+    }
+    if (!ok) {
+        nn = node_i;
+        return kFrInterp;
+    }
+    int nn_new = site_anchor(m, lon_new, lat_new, c.exact);
+    int e = find_containing<MAXV>(m, lon_new, lat_new, nn_new);
+    if (e < 0) {
+        nn = node_i;
+        return kFrOOB;
+    }
+    lon = lon_new;
+    lat = lat_new;
+    if (IS3D) z = clamp_z(m, z_new, nn_new);
+    nn = nn_new;
+    return 0;
+}
+
+// integrators.py:673 _apply_split_diffusion_kick for one particle alive after
+// the deterministic step (uniform Kh, counter RNG keyed by the particle id).
+template <int MAXV, bool IS3D>
+__device__ inline void kick_one(const DMesh& m, const StepCfg& c, uint64_t pid, double pre_lon,
+                                      double pre_lat, double pre_z, int pre_nn, double& lon, double& lat,
+                                      double& z, int32_t& st, int& nn, int8_t& fr)
+{
+    double g1, g2;
+    normal_pair(c.seed, c.real_id, pid, c.step_i, 0, g1, g2);
+    double abs_dt = c.dt >= 0.0 ? c.dt : -c.dt;
+    double sqrt_abs_dt = km_sqrt(abs_dt);
+    double sigma_x = km_sqrt(2.0 * c.kh_x);
+    double sigma_y = km_sqrt(2.0 * c.kh_y);
+    double dx = sigma_x * sqrt_abs_dt * g1;
+    double dy = sigma_y * sqrt_abs_dt * g2;
+    if (c.kh_x != c.kh_y) {
+        double la = lat;
+        if (la < -89.9) la = -89.9;
+        if (la > 89.9) la = 89.9;
+        double corr = (c.kh_x - c.kh_y) * (km_sin(la * kDeg2Rad) / km_cos(la * kDeg2Rad)) / kRMetric *
+                      km_fabs(c.dt);
+        dy = dy + corr;
+    }
+    double lon_k, lat_k;
+    sphere_displace(lon, lat, dx, dy, lon_k, lat_k);
+    int nn_k = site_anchor(m, lon_k, lat_k, c.exact == 1 ? 1 : 2);  // batch find_nearest_nodes
+    int e = find_containing<MAXV>(m, lon_k, lat_k, nn_k);
+    if (e < 0) {
+        lon = pre_lon;
+        lat = pre_lat;
+        z = pre_z;
+        nn = pre_nn;
+        st = kOutside;
+        fr = kFrOOB;
+        return;
+    }
+    lon = lon_k;
+    lat = lat_k;
+    nn = nn_k;
+    if (IS3D) z = clamp_z(m, z, nn_k);
+}
+
+// integrators.py:842 _apply_split_az_kick, constant source on a z-level
+// mesh (numerics/diffusion.py:427 _loop_az_kick_zlevel, gradient 0, and
+// :302 diffusion_az_step_meters): one particle alive after the horizontal
+// kick, folded back into [z_levels[0], depth[nn]].
+__device__ inline void az_one(const DMesh& m, const StepCfg& c, uint64_t pid, int node, double& z)
+{
+    double xi, unused;
+    normal_pair(c.seed, c.real_id, pid, c.step_i, 1, xi, unused);
+    const double sdt = c.time_dir * c.dt;
+    const double abs_dt = sdt >= 0.0 ? sdt : -sdt;
+    const double g = 0.0;
+    const double dz = g * abs_dt + km_sqrt(2.0 * c.az_const * abs_dt) * xi;
+    z = km_reflect_depth(z + dz, m.zlev[0], m.depth[node]);
+}
+
+// tracking.py:199 _apply_beaching_hook + numerics/beaching.py:95
+// evaluate_beaching, one particle (prev = status at the step start).
+template <int MAXV>
+__device__ inline void beach_one(const DMesh& m, const StepCfg& c, uint64_t pid, int32_t prev, double lon,
+                                       double lat, int32_t& st, int& nn, int8_t& fr, int32_t& ws)
+{
+    const bool need_topo = c.use_coastal || c.use_stochastic;
+    const bool newly_failed = (st == kOutside) && (prev == kAlive);
+    if (newly_failed && nn < 0) nn = site_anchor(m, lon, lat, c.exact == 1 ? 1 : 2);
+    const int n = nn;
+    int containing = -1;
+    if (need_topo && (st == kAlive || st == kBeached || newly_failed) && n >= 0)
+        containing = find_containing<MAXV>(m, lon, lat, n);
+    const bool coastal = need_topo && containing >= 0 && m.coastal != nullptr && m.coastal[containing];
+    const double h_raw = n >= 0 ? m.depth[n] : __builtin_inf();
+    if (c.on_failure && newly_failed) {
+        if (h_raw < c.zone_depth || coastal) {
+            st = kBeached;
+            fr = 0;
+        }
+    }
+    const bool alive = st == kAlive, beached = st == kBeached;
+    bool to_beach = false;
+    if (c.use_depth) to_beach |= h_raw < c.depth_min;
+    if (c.use_coastal) to_beach |= coastal;
+    if (c.use_stochastic && c.t_beach > 0) {
+        bool in_zone = h_raw < c.zone_depth || coastal;
+        double p = 1.0 - km_exp(-km_fabs(c.beach_dt) / c.t_beach);
+        if (alive && in_zone && u01(c.beach_seed, c.real_id, pid, c.step_i, 10) < p) to_beach = true;
+    }
+    to_beach = to_beach && alive;
+    bool to_release = false;
+    int32_t w = ws;
+    if (c.resuspension != 0 && beached) {
+        bool rewet = h_raw > (c.depth_min + c.rewet_hysteresis) && n >= 0;
+        w = rewet ? w + 1 : 0;
+        if (c.resuspension == 1 || c.resuspension == 3) to_release |= w >= c.min_wet_steps;
+        if ((c.resuspension == 2 || c.resuspension == 3) && c.t_resuspend > 0) {
+            double p_r = 1.0 - km_exp(-km_fabs(c.beach_dt) / c.t_resuspend);
+            if (rewet && u01(c.beach_seed, c.real_id, pid, c.step_i, 11) < p_r) to_release = true;
+        }
+    }
+    if (to_release) w = 0;
+    if (!beached) w = 0;
+    ws = w;
+    if (to_beach) st = kBeached;
+    if (to_release) {
+        st = kAlive;
+        fr = 0;
+    }
+}
+
+// One full step per particle: deterministic step, split Kh kick, beaching.
+// Waves per SIMD the compiler budgets registers for (0: its own choice).
+#ifndef KELP_STEP_WAVES
+#define KELP_STEP_WAVES 3
+#endif
+#if KELP_STEP_WAVES > 0
+#define KELP_STEP_ATTR __attribute__((amdgpu_waves_per_eu(KELP_STEP_WAVES)))
+#else
+#define KELP_STEP_ATTR
+#endif
+
+template <int ORDER, int MAXV, bool IS3D, typename T, bool ELEM>
+__global__ __launch_bounds__(256) KELP_STEP_ATTR void step_kernel(DMesh m, Snap<T, IS3D> f, Snap<T, false> fe,
+                                                                  StepCfg c, PState s)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.n) return;
+    const int32_t prev = s.status[i];
+    if (s.alive_count) {
+        // particles alive at the step start: one atomic per wavefront
+        unsigned long long alive = __ballot(prev == kAlive);
+        unsigned long long active = __ballot(1);
+        if ((int)(threadIdx.x & 63) == __ffsll((long long)active) - 1 && alive)
+            atomicAdd(s.alive_count, (unsigned long long)__popcll(alive));
+    }
+    int32_t st = prev;
+    double lon = s.lon[i], lat = s.lat[i];
+    double z = IS3D ? s.z[i] : 0.0;
+    int nn = s.nn[i];
+    int8_t fr = s.fr[i];
+    double U = 0.0, V = 0.0, W = 0.0;
+    if (prev == kAlive) {
+        const double pre_lon = lon, pre_lat = lat, pre_z = z;
+        const int pre_nn = nn;
+        int code = det_step<ORDER, MAXV, IS3D, T, ELEM>(m, f, fe, c, lon, lat, z, nn, U, V, W);
+        if (code) {
+            st = kOutside;
+            fr = (int8_t)code;
+        } else if (c.kick && !(c.kh_x == 0.0 && c.kh_y == 0.0)) {
+            kick_one<MAXV, IS3D>(m, c, (uint64_t)s.pid[i], pre_lon, pre_lat, pre_z, pre_nn, lon, lat, z, st, nn, fr);
+        }
+        if (IS3D && c.az && st == kAlive) az_one(m, c, (uint64_t)s.pid[i], nn, z);
+    }
+    if (c.beach) beach_one<MAXV>(m, c, (uint64_t)s.pid[i], prev, lon, lat, st, nn, fr, s.ws[i]);
+    s.lon[i] = lon;
+    s.lat[i] = lat;
+    if (IS3D) s.z[i] = z;
+    s.status[i] = st;
+    s.nn[i] = nn;
+    s.fr[i] = fr;
+    s.u[i] = U;
+    s.v[i] = V;
+    if (IS3D && s.w) s.w[i] = W;
+}
+
+// Never launched.  A second caller of the exact nearest search that carries
+// no occupancy attribute: with the step kernels as its only callers the
+// compiler narrows the out-of-line search to the step kernels' register
+// budget and the step kernels then spill about eight times more (41 -> 329
+// VGPRs for RK4 3-D at 3 waves/SIMD, hipcc -Rpass-analysis=kernel-resource-usage).
+namespace {
+__global__ void anchor_probe_kernel(DMesh m, int64_t n, const double* __restrict__ lon,
+                                    const double* __restrict__ lat, int mode, int32_t* __restrict__ out)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = site_anchor(m, lon[i], lat[i], mode);
+}
+}  // namespace
+
+template <int MAXV, bool IS3D, typename T, bool ELEM>
+void launch_step_v(hipStream_t stream, int order, const DMesh& m, const Snap<T, IS3D>& f,
+                   const Snap<T, false>& fe, const StepCfg& cfg, const PState& ps)
+{
+    const unsigned g = (unsigned)((ps.n + 255) / 256);
+    if (order == 1)
+        step_kernel<1, MAXV, IS3D, T, ELEM><<<g, 256, 0, stream>>>(m, f, fe, cfg, ps);
+    else if (order == 2)
+        step_kernel<2, MAXV, IS3D, T, ELEM><<<g, 256, 0, stream>>>(m, f, fe, cfg, ps);
+    else
+        step_kernel<4, MAXV, IS3D, T, ELEM><<<g, 256, 0, stream>>>(m, f, fe, cfg, ps);
+}
+
+}  // namespace kelp

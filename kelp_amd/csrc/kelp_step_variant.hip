@@ -1,0 +1,27 @@
+// kelp_step_variant.hip -- the step kernels of one (3-D, field dtype, u/v
+// layout) combination, chosen by -DKV_IS3D=0|1 -DKV_F32=0|1 -DKV_ELEM=0|1;
+// the build compiles this file once per combination, in parallel.
+#include "kelp_step_impl.cuh"
+
+#if !defined(KV_IS3D) || !defined(KV_F32) || !defined(KV_ELEM)
+#error "compile with -DKV_IS3D=0|1 -DKV_F32=0|1 -DKV_ELEM=0|1"
+#endif
+
+namespace kelp {
+
+#if KV_F32
+using KvT = float;
+#else
+using KvT = double;
+#endif
+
+template void launch_step_v<3, (bool)KV_IS3D, KvT, (bool)KV_ELEM>(hipStream_t, int, const DMesh&,
+                                                                  const Snap<KvT, (bool)KV_IS3D>&,
+                                                                  const Snap<KvT, false>&, const StepCfg&,
+                                                                  const PState&);
+template void launch_step_v<4, (bool)KV_IS3D, KvT, (bool)KV_ELEM>(hipStream_t, int, const DMesh&,
+                                                                  const Snap<KvT, (bool)KV_IS3D>&,
+                                                                  const Snap<KvT, false>&, const StepCfg&,
+                                                                  const PState&);
+
+}  // namespace kelp

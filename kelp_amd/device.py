@@ -161,6 +161,8 @@ class DeviceContext:
             z=np.ascontiguousarray(mesh.sigma, np.float64),
             coastal=(None if mesh.elem_is_coastal is None
                      else np.ascontiguousarray(mesh.elem_is_coastal, np.uint8)),
+            clon=np.ascontiguousarray(mesh.elem_center_lon, np.float64),
+            clat=np.ascontiguousarray(mesh.elem_center_lat, np.float64),
         )
         d = MeshDesc(
             mesh.n_nodes, mesh.n_elements, mesh.max_v, keep["n2e"].shape[1],
@@ -170,7 +172,8 @@ class DeviceContext:
             mesh.gh_cell_size, mesh.gh_lon_min, mesh.gh_lat_min,
             mesh.gh_nx, mesh.gh_ny, int(mesh.gh_wrap_lon), mesh.gh_far_rows,
             mesh.r_earth, int(keep["z"].size), keep["z"].ctypes.data if keep["z"].size else None,
-            None if keep["coastal"] is None else keep["coastal"].ctypes.data)
+            None if keep["coastal"] is None else keep["coastal"].ctypes.data,
+            keep["clon"].ctypes.data, keep["clat"].ctypes.data)
         check(self.lib.kelp_mesh_upload(self.h, ctypes.byref(d)), "kelp_mesh_upload")
         self.mesh = mesh
         self._slot_keys.clear()
@@ -181,9 +184,10 @@ class DeviceContext:
         return out
 
     # -- fields -------------------------------------------------------------
-    def upload_field(self, slot: int, u, v, w=None, key=None) -> None:
+    def upload_field(self, slot: int, u, v, w=None, key=None, uv_layout="node") -> None:
         """Upload one snapshot into ``slot``; skipped when ``key`` matches the
-        slot's current content."""
+        slot's current content.  ``uv_layout``: "node" (u, v per node) or
+        "element" (u, v per element centre, interpolated by least squares)."""
         if key is not None and self._slot_keys.get(slot) is key:
             return
         u = np.asarray(u)
@@ -192,12 +196,21 @@ class DeviceContext:
         u = np.ascontiguousarray(u, npd)
         v = np.ascontiguousarray(v, npd)
         n_planes = 1 if u.ndim == 1 else u.shape[0]
+        if uv_layout not in ("node", "element"):
+            raise ValueError(f"unknown uv_layout {uv_layout!r}")
+        m = self.mesh.n_nodes if uv_layout == "node" else self.mesh.n_elements
+        if u.shape[-1] != m or v.shape != u.shape:
+            raise ValueError(f"u/v of shape {u.shape}/{v.shape} do not match the mesh "
+                             f"({m} {uv_layout} values per plane)")
         wp, nw = None, 0
         if w is not None:
             w = np.ascontiguousarray(w, npd)
+            if w.shape[-1] != self.mesh.n_nodes:
+                raise ValueError(f"w of shape {w.shape} does not match the mesh's {self.mesh.n_nodes} nodes")
             wp, nw = w.ctypes.data, (1 if w.ndim == 1 else w.shape[0])
-        check(self.lib.kelp_field_upload(self.h, int(slot), dt, n_planes, u.ctypes.data,
-                                         v.ctypes.data, wp, nw), "kelp_field_upload")
+        fn = self.lib.kelp_field_upload if uv_layout == "node" else self.lib.kelp_field_upload_elem
+        check(fn(self.h, int(slot), dt, n_planes, u.ctypes.data, v.ctypes.data, wp, nw),
+              "kelp_field_upload")
         self._slot_keys[slot] = key
 
     # -- particles ----------------------------------------------------------

@@ -19,9 +19,13 @@ with misses healed to the exact nearest (the unfused batch path, and the
 fused paths wherever they are enabled, integrators.py:3289-3320), which is
 anchor mode 2 of the kernel.
 
-Combinations the device path does not implement raise
-``NotImplementedError`` (element-layout u/v, W leastsq/idw, sigma meshes,
-EM/M1 diffusion, an Az kick from an Az field); there is no CPU fallback.
+u/v at the nodes are interpolated barycentrically in the containing element
+(``uv_interp="barycentric"``), u/v at element centres by least squares over
+the anchor's element ring (``uv_interp="leastsq"``, the reference default);
+W by ``nearest``, ``leastsq`` or ``barycentric``.  Combinations the device
+path does not implement raise ``NotImplementedError`` (edge_leastsq, W idw,
+sigma meshes, EM/M1 diffusion, an Az kick from an Az field); there is no CPU
+fallback.
 """
 from __future__ import annotations
 
@@ -37,7 +41,7 @@ from .mesh import Mesh, exact_nearest_on
 from .particles import ParticleState
 
 _C_COEFFS = {1: (0.0,), 2: (0.0, 0.5), 4: (0.0, 0.5, 0.5, 1.0)}
-W_METHODS = {"nearest": 0, "barycentric": 2}
+W_METHODS = {"nearest": 0, "leastsq": 1, "barycentric": 2}
 N_SLOTS = 16
 
 
@@ -101,7 +105,8 @@ class Engine:
         else:
             _, (slot, _, _) = self._slots.popitem(last=False)
         w = None if vel.w is None else np.asarray(vel.w, dtype)
-        self.ctx.upload_field(slot, np.asarray(vel.u, dtype), np.asarray(vel.v, dtype), w)
+        self.ctx.upload_field(slot, np.asarray(vel.u, dtype), np.asarray(vel.v, dtype), w,
+                              uv_layout=vel.uv_layout)
         self._slots[key] = (slot, vel, dtype)
         return slot
 
@@ -127,16 +132,18 @@ def _field_dtype(*fields):
 
 def _check_supported(vel, mesh, uv_interp, w_interp, diffusion, is3d):
     base = vel.vel_a if isinstance(vel, LazyLerpField) else vel
-    if uv_interp != "barycentric" or base.uv_layout != "node":
+    ok = ((uv_interp == "barycentric" and base.uv_layout == "node")
+          or (uv_interp == "leastsq" and base.uv_layout == "element"))
+    if not ok:
         raise NotImplementedError(
             f"device path: uv_interp={uv_interp!r} with uv_layout={base.uv_layout!r} is not "
-            "implemented (node-layout barycentric only)")
+            "implemented (node-layout barycentric, element-layout leastsq)")
     if is3d:
         if mesh.vertical_coord != "zlevel":
             raise NotImplementedError("device path: sigma meshes are not implemented")
         if w_interp not in W_METHODS:
             raise NotImplementedError(f"device path: w_interp={w_interp!r} is not implemented "
-                                      "(nearest, barycentric)")
+                                      "(nearest, leastsq, barycentric)")
         if base.uv_vertical != "layer":
             raise NotImplementedError(f"device path: uv_vertical={base.uv_vertical!r}")
     if diffusion is not None:

@@ -771,6 +771,67 @@ int or_interp_bary_tinterp(const or_mesh_t *m, const or_fields_t *f, double xp, 
     return 1;
 }
 
+/* interpolate.py:207 _interp_leastsq_core_tinterp (== :52
+ * _interp_leastsq_core for a plain field): a plane through the blended
+ * element-centre values of the anchor's 1-ring in particle-centred metric
+ * coordinates, the antimeridian gate on each centre, Sarrus determinant with
+ * the scale-relative singularity guard, Cramer's rule for the constant term.
+ * `layer` selects the (E,) plane of a layered field (< 0: 2-D). */
+int or_interp_lsq_tinterp(const or_mesh_t *m, const or_fields_t *f, double xp, double yp,
+                          int nn, double time_dir, int layer, double alpha,
+                          double *U_o, double *V_o)
+{
+    const double DEG2R = 0.017453292519943295;
+    *U_o = 0.0;
+    *V_o = 0.0;
+    int count = m->n2e_count[nn];
+    if (count < 1) return 0;
+    const int32_t *row = m->n2e + (int64_t)nn * m->max_conn;
+    double z0 = M_COS(yp * DEG2R) * SLM;
+    double xp_m = xp * z0;
+    double yp_m = yp * SLM;
+    double a11 = 0.0, a12 = 0.0, a13 = 0.0, a22 = 0.0, a23 = 0.0, a33 = 0.0;
+    double fUn = 0.0, f1Un = 0.0, f2Un = 0.0, fVn = 0.0, f1Vn = 0.0, f2Vn = 0.0;
+    int64_t off = layer > 0 ? (int64_t)layer * m->n_elems : 0;
+    for (int n = 0; n < count; n++) {
+        int e = row[n];
+        double ex = m->elem_center_lon[e] * z0 - xp_m;
+        double dlon_raw = m->elem_center_lon[e] - xp;
+        if (dlon_raw > 180.0)
+            ex -= 360.0 * z0;
+        else if (dlon_raw < -180.0)
+            ex += 360.0 * z0;
+        double ey = m->elem_center_lat[e] * SLM - yp_m;
+        double u = vblend(f->u_a, f->u_b, f->u_t, f->dtype, off + e, f->alpha0, alpha);
+        double v = vblend(f->v_a, f->v_b, f->v_t, f->dtype, off + e, f->alpha0, alpha);
+        double uu = time_dir * u;
+        double vv = time_dir * v;
+        a11 = a11 + 1.0;
+        a12 = a12 + ex;
+        a13 = a13 + ey;
+        a22 = a22 + ex * ex;
+        a23 = a23 + ex * ey;
+        a33 = a33 + ey * ey;
+        fUn = fUn + uu;
+        f1Un = f1Un + uu * ex;
+        f2Un = f2Un + uu * ey;
+        fVn = fVn + vv;
+        f1Vn = f1Vn + vv * ex;
+        f2Vn = f2Vn + vv * ey;
+    }
+    double a21 = a12, a31 = a13, a32 = a23;
+    double det = (a11 * a22 * a33 + a12 * a23 * a31 + a13 * a21 * a32 - a13 * a22 * a31 -
+                  a12 * a21 * a33 - a11 * a23 * a32);
+    if (fabs(det) < 1e-30 || fabs(det) < 1e-12 * fabs(a11 * a22 * a33)) return 0;
+    double inv = 1.0 / fabs(det);
+    double b11 = a22 * a33 - a23 * a32;
+    double b12 = a13 * a32 - a12 * a33;
+    double b13 = a12 * a23 - a13 * a22;
+    *U_o = (b11 * fUn + b12 * f1Un + b13 * f2Un) * inv;
+    *V_o = (b11 * fVn + b12 * f1Vn + b13 * f2Vn) * inv;
+    return 1;
+}
+
 /* ------------------------------------------------------------------------
  * Vertical (z-level)
  * ---------------------------------------------------------------------- */
@@ -935,6 +996,76 @@ static double w_bary(const or_mesh_t *m, const or_fields_t *f, double alpha, dou
     return w_linear(m, f, alpha, zc, node, time_dir);
 }
 
+/* numerics/vertical.py:493 _interp_w_leastsq_single on the stage field
+ * (interface values blended in time, as the unfused path's materialised
+ * lerp): the unique 1-ring nodes (centre first, then each element's vertices
+ * in order, at most 64 -- vertical.py:446 _collect_neighbour_nodes), W
+ * linear in depth at each (clamped to that node's bottom), a plane fit
+ * through them evaluated at the particle; fewer than 3 nodes or a singular
+ * fit fall back to linear W at the anchor. */
+static double w_leastsq(const or_mesh_t *m, const or_fields_t *f, double alpha, double xp, double yp,
+                        double zp, int node, double time_dir)
+{
+    const double DEG2R = 0.017453292519943295;
+    int buf[64];
+    int count = 0;
+    buf[count++] = node;
+    int n_el = m->n2e_count[node];
+    const int32_t *row = m->n2e + (int64_t)node * m->max_conn;
+    for (int k = 0; k < n_el; k++) {
+        int e = row[k];
+        if (e < 0) continue;
+        for (int v = 0; v < m->max_v; v++) {
+            int nd = m->elem_nodes[(int64_t)e * m->max_v + v];
+            if (nd < 0) continue;
+            int found = 0;
+            for (int j = 0; j < count; j++)
+                if (buf[j] == nd) {
+                    found = 1;
+                    break;
+                }
+            if (!found && count < 64) buf[count++] = nd;
+        }
+    }
+    if (count < 3) {
+        double zc = zp < m->depth[node] ? zp : m->depth[node];
+        return w_linear(m, f, alpha, zc, node, time_dir);
+    }
+    double z0 = M_COS(yp * DEG2R) * SLM;
+    if (z0 < 1.0) z0 = 1.0;
+    double a11 = 0.0, a12 = 0.0, a13 = 0.0, a22 = 0.0, a23 = 0.0, a33 = 0.0;
+    double fw = 0.0, f1w = 0.0, f2w = 0.0;
+    for (int i = 0; i < count; i++) {
+        int nd = buf[i];
+        double bottom = m->depth[nd];
+        double w_val = w_linear(m, f, alpha, zp > bottom ? bottom : zp, nd, time_dir);
+        double dlon = PYMOD360(m->nodes_lon[nd] - xp + 180.0) - 180.0;
+        double ex = dlon * z0;
+        double ey = (m->nodes_lat[nd] - yp) * SLM;
+        a11 += 1.0;
+        a12 += ex;
+        a13 += ey;
+        a22 += ex * ex;
+        a23 += ex * ey;
+        a33 += ey * ey;
+        fw += w_val;
+        f1w += w_val * ex;
+        f2w += w_val * ey;
+    }
+    double a21 = a12, a31 = a13, a32 = a23;
+    double det = (a11 * a22 * a33 + a12 * a23 * a31 + a13 * a21 * a32 - a13 * a22 * a31 -
+                  a12 * a21 * a33 - a11 * a23 * a32);
+    if (fabs(det) < 1e-30 || fabs(det) < 1e-12 * fabs(a11 * a22 * a33)) {
+        double zc = zp < m->depth[node] ? zp : m->depth[node];
+        return w_linear(m, f, alpha, zc, node, time_dir);
+    }
+    double inv = 1.0 / fabs(det);
+    double b11 = a22 * a33 - a23 * a32;
+    double b12 = a13 * a32 - a12 * a33;
+    double b13 = a12 * a23 - a13 * a22;
+    return (b11 * fw + b12 * f1w + b13 * f2w) * inv;
+}
+
 /* ------------------------------------------------------------------------
  * Counter RNG and kicks (numerics/diffusion.py)
  * ---------------------------------------------------------------------- */
@@ -1049,11 +1180,17 @@ static int eval_site(const or_mesh_t *m, const or_fields_t *f, const or_step_arg
     *nn_o = nn;
     int layer = -1;
     if (a->is3d) layer = find_layer(m->z_levels, m->n_levels, zp, m->depth[nn]);
-    int ok = or_interp_bary_tinterp(m, f, lon, lat, nn, a->time_dir, layer, a->alphas[stage], U, V, NULL);
+    int ok;
+    if (f->uv_layout == 1)
+        ok = or_interp_lsq_tinterp(m, f, lon, lat, nn, a->time_dir, layer, a->alphas[stage], U, V);
+    else
+        ok = or_interp_bary_tinterp(m, f, lon, lat, nn, a->time_dir, layer, a->alphas[stage], U, V, NULL);
     *W = 0.0;
     if (a->is3d && want_w) {
         if (a->w_method == 0)
             *W = w_linear(m, f, a->alphas[stage], zp, nn, a->time_dir);
+        else if (a->w_method == 1)
+            *W = w_leastsq(m, f, a->alphas[stage], lon, lat, zp, nn, a->time_dir);
         else
             *W = w_bary(m, f, a->alphas[stage], lon, lat, zp, nn, a->time_dir);
     }

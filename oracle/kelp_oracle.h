@@ -37,6 +37,7 @@ typedef struct {
     int32_t n_levels;
     const double *z_levels;
     const uint8_t *elem_coastal;  /* (E,) coastal-element flags or NULL */
+    const double *elem_center_lon, *elem_center_lat;  /* (E,) element centres */
 } or_mesh_t;
 
 /* Snapshot triple of one step: bracket pair (a, b) blended by alpha0 and
@@ -47,6 +48,7 @@ typedef struct {
     const void *w_a, *w_b, *w_t;   /* 3-D: (K, N) interface W, or NULL */
     int32_t dtype;
     double alpha0;
+    int32_t uv_layout;             /* 0: u/v at nodes, 1: u/v at element centres */
 } or_fields_t;
 
 /* One step's configuration (the keyword arguments of step_fn,
@@ -54,7 +56,7 @@ typedef struct {
 typedef struct {
     int32_t order;     /* 1 Euler, 2 RK2, 4 RK4 */
     int32_t is3d;
-    int32_t w_method;  /* 0 nearest (linear at the anchor), 2 barycentric */
+    int32_t w_method;  /* 0 nearest (linear at the anchor), 1 leastsq, 2 barycentric */
     int32_t exact;     /* mesh.exact_nearest */
     double dt, time_dir;
     double alphas[4];  /* stage fractions towards the target snapshot */

@@ -33,6 +33,7 @@ class OrMesh(ctypes.Structure):
         ("gh_nx", c_int32), ("gh_ny", c_int32), ("gh_wrap", c_int32), ("gh_far_rows", c_int32),
         ("r_earth", c_double),
         ("n_levels", c_int32), ("z_levels", c_void_p), ("elem_coastal", c_void_p),
+        ("elem_center_lon", c_void_p), ("elem_center_lat", c_void_p),
     ]
 
 
@@ -92,7 +93,7 @@ class OrFields(ctypes.Structure):
         ("u_a", c_void_p), ("v_a", c_void_p), ("u_b", c_void_p), ("v_b", c_void_p),
         ("u_t", c_void_p), ("v_t", c_void_p),
         ("w_a", c_void_p), ("w_b", c_void_p), ("w_t", c_void_p),
-        ("dtype", c_int32), ("alpha0", c_double),
+        ("dtype", c_int32), ("alpha0", c_double), ("uv_layout", c_int32),
     ]
 
 
@@ -155,7 +156,8 @@ class OracleMesh:
             off=_c(mesh.gh_offsets, np.int32), ids=_c(mesh.gh_node_ids, np.int32),
             z=_c(mesh.sigma if mesh.sigma.size else np.zeros(1), np.float64),
             coastal=(None if getattr(mesh, "elem_is_coastal", None) is None
-                     else _c(mesh.elem_is_coastal, np.uint8)))
+                     else _c(mesh.elem_is_coastal, np.uint8)),
+            clon=_c(mesh.elem_center_lon, np.float64), clat=_c(mesh.elem_center_lat, np.float64))
         xyz = np.empty((mesh.n_nodes, 3), dtype=np.float64)
         self.L.or_node_xyz(mesh.n_nodes, k["lon"].ctypes.data, k["lat"].ctypes.data, xyz.ctypes.data)
         k["xyz"] = xyz
@@ -168,7 +170,8 @@ class OracleMesh:
             mesh.gh_cell_size, mesh.gh_lon_min, mesh.gh_lat_min,
             mesh.gh_nx, mesh.gh_ny, int(mesh.gh_wrap_lon), mesh.gh_far_rows,
             mesh.r_earth, int(mesh.sigma.size), k["z"].ctypes.data,
-            None if k["coastal"] is None else k["coastal"].ctypes.data)
+            None if k["coastal"] is None else k["coastal"].ctypes.data,
+            k["clon"].ctypes.data, k["clat"].ctypes.data)
 
     def nearest(self, lon, lat, exact=True) -> np.ndarray:
         lon = _c(lon, np.float64)
@@ -244,7 +247,8 @@ class OracleMesh:
     def step(self, state: dict, fields: dict, args: OrStepArgs) -> dict:
         """One full step (deterministic + kick + beaching) on copies of the
         ``state`` arrays (lon, lat, z, status, nn, fr, pid, ws).  ``fields``:
-        u_a, v_a[, w_a] and optionally the b/t snapshots and alpha0."""
+        u_a, v_a[, w_a] and optionally the b/t snapshots, alpha0 and
+        uv_layout ("node", or "element" for u/v at element centres)."""
         ua = np.asarray(fields["u_a"])
         dt_code = 1 if ua.dtype == np.float32 else 0
         npd = np.float32 if dt_code else np.float64
@@ -259,7 +263,8 @@ class OracleMesh:
                 keep[f"{nm}_{sfx}"] = keep[nm + "_a"] if x is None else _c(x, npd)
         ptr = lambda k: keep[k].ctypes.data if k in keep else None
         f = OrFields(ptr("u_a"), ptr("v_a"), ptr("u_b"), ptr("v_b"), ptr("u_t"), ptr("v_t"),
-                     ptr("w_a"), ptr("w_b"), ptr("w_t"), dt_code, float(fields.get("alpha0", 0.0)))
+                     ptr("w_a"), ptr("w_b"), ptr("w_t"), dt_code, float(fields.get("alpha0", 0.0)),
+                     1 if fields.get("uv_layout", "node") == "element" else 0)
         P = len(state["lon"])
         out = dict(lon=np.array(state["lon"], np.float64), lat=np.array(state["lat"], np.float64),
                    z=np.array(state.get("z", np.zeros(P)), np.float64),

@@ -111,3 +111,108 @@ def run_case(name, direction, stepper):
                        ("failure_reason", "fr"), ("u", "u"), ("v", "v"), ("w", "w")):
             rows[k].append(np.array(st[src], copy=True))
     return {k: np.stack(v) for k, v in rows.items()}
+
+
+# ---------------------------------------------------------------------------
+# element-layout cases: leastsq u/v at element centres, leastsq W
+# (tests/_gen_golden_3d.py:154-219 with tests/test_3d_analytical.py:39-101)
+# ---------------------------------------------------------------------------
+
+ELEM_LEVELS = np.array([0.0, 10.0, 25.0, 50.0, 100.0, 200.0])
+ELEM_DT = 3600.0
+ELEM_STEPS = 8
+
+#: case -> (integrator order, snapshot shifts)
+ELEM_CASES = {
+    "fesom2_elem_euler_fused": (1, (0.0,)),
+    "fesom2_elem_rk2_unfused_lerp": (2, (0.0, 1.0)),
+}
+
+
+def elem_mesh():
+    """A 5 x 5 node lattice over 4-6 E, 4-6 N, two triangles per cell
+    ((n0, n1, n2), (n1, n3, n2)), element centres the plain vertex means,
+    bathymetry 60 m (west) .. 200 m (east) with a 35 m centre node."""
+    from kelp_amd.mesh import make_mesh
+    xs = np.linspace(4.0, 6.0, 5)
+    ys = np.linspace(4.0, 6.0, 5)
+    gx, gy = np.meshgrid(xs, ys)
+    lon, lat = gx.ravel(), gy.ravel()
+    elems = []
+    for j in range(4):
+        for i in range(4):
+            n0 = j * 5 + i
+            elems += [[n0, n0 + 1, n0 + 5], [n0 + 1, n0 + 6, n0 + 5]]
+    en = np.array(elems, dtype=np.int32)
+    depth = 60.0 + 70.0 * (lon - 4.0)
+    depth[12] = 35.0
+    mesh = make_mesh("test_local", lon, lat, en, depth=depth, z_levels=ELEM_LEVELS)
+    mesh.elem_center_lon = np.array([lon[e].mean() for e in en])
+    mesh.elem_center_lat = np.array([lat[e].mean() for e in en])
+    return mesh
+
+
+def elem_field(mesh, s):
+    """Element u/v (K-1, E) varying with layer and position, node w (K, N)
+    positive down, zero at the surface; ``s`` shifts the snapshot."""
+    k = np.arange(mesh.sigma.size - 1, dtype=np.float64)[:, None]
+    x = (mesh.elem_center_lon - 5.0)[None, :]
+    y = (mesh.elem_center_lat - 5.0)[None, :]
+    u = (0.35 - 0.04 * k) * (1.0 + 0.4 * x) + 0.15 * y + 0.05 * s
+    v = 0.12 * (1.0 + 0.2 * k) * np.sin(2.0 * x) - 0.08 * y * (1.0 + s)
+    zk = (mesh.sigma / mesh.sigma[-1])[:, None]
+    xn = (mesh.nodes_lon - 5.0)[None, :]
+    w = 1.5e-3 * np.sin(np.pi * zk) * (1.0 + 0.5 * xn) * (1.0 - 0.3 * s)
+    return (np.ascontiguousarray(u, np.float64), np.ascontiguousarray(v, np.float64),
+            np.ascontiguousarray(w, np.float64))
+
+
+def elem_seeds():
+    i = np.arange(32, dtype=np.float64)
+    lon = 4.1 + 1.85 * ((i * 0.618034) % 1.0)
+    lat = 4.1 + 1.8 * ((i * 0.381966 + 0.13) % 1.0)
+    lon[-4:] = [5.96, 5.975, 5.99, 5.998]
+    depths = np.array([0.0, 5.0, 10.0, 17.5, 25.0, 37.0, 50.0, 75.0, 100.0, 150.0, 199.0, 260.0])
+    z = depths[np.arange(32) % depths.size]
+    return lon, lat, z
+
+
+def run_elem_case(name, direction, stepper):
+    """Drive ``stepper(mesh, state, fields, order, alphas, step_i)`` over
+    the element case's steps; ``fields`` carries uv_layout="element"."""
+    order, shifts = ELEM_CASES[name]
+    mesh = elem_mesh()
+    snaps = [elem_field(mesh, s) for s in shifts]
+    a, b = snaps[0], snaps[-1]
+    lon, lat, z = elem_seeds()
+    st = dict(lon=lon, lat=lat, z=z, status=np.ones(lon.size, np.int32),
+              nn=np.full(lon.size, -1, np.int32), fr=np.zeros(lon.size, np.int8))
+    rows = {k: [] for k in ("lon", "lat", "z", "status", "failure_reason", "u", "v", "w")}
+    c = {1: (0.0,), 2: (0.0, 0.5), 4: (0.0, 0.5, 0.5, 1.0)}[order]
+    T = ELEM_STEPS * ELEM_DT
+    t0 = 0.0 if direction > 0 else T
+    for i in range(ELEM_STEPS):
+        if len(snaps) == 1:
+            fields = dict(u_a=a[0], v_a=a[1], w_a=a[2], alpha0=0.0)
+            alphas = [0.0] * len(c)
+        else:
+            s = t0 + direction * i * ELEM_DT
+            alpha = s / T
+            if abs(alpha) < 1e-12:
+                base, alpha0, lerp = a, 0.0, False
+            elif abs(alpha - 1.0) < 1e-12:
+                base, alpha0, lerp = b, 0.0, False
+            else:
+                base, alpha0, lerp = a, alpha, True
+            tgt, dtn = (b, T - s) if direction > 0 else (a, s)
+            fields = dict(u_a=base[0], v_a=base[1], w_a=base[2], u_t=tgt[0], v_t=tgt[1],
+                          w_t=tgt[2], alpha0=alpha0)
+            if lerp:
+                fields.update(u_b=b[0], v_b=b[1], w_b=b[2])
+            alphas = [ck * ELEM_DT / dtn for ck in c]
+        fields["uv_layout"] = "element"
+        st = stepper(mesh, st, fields, order, alphas, i)
+        for k, src in (("lon", "lon"), ("lat", "lat"), ("z", "z"), ("status", "status"),
+                       ("failure_reason", "fr"), ("u", "u"), ("v", "v"), ("w", "w")):
+            rows[k].append(np.array(st[src], copy=True))
+    return {k: np.stack(v) for k, v in rows.items()}

@@ -73,3 +73,32 @@ def test_rk4_3d_tinterp_kick_registry_bitwise():
         ref = om.step(ref, f, step_args(4, dt, 1.0, is3d=True, w_method=2, exact=2, alphas=al,
                                         kick=kick, step_i=k))
     _assert_same(st, ref, True)
+
+
+def test_registry_element_layout_matches_reference_golden():
+    """rk2_step with element-centre u/v (uv_interp="leastsq", the reference
+    default) and leastsq W, driven like the reference's golden generator
+    (tests/_gen_golden_3d.py:200-219), against its stored output."""
+    import os
+    from .golden import reference_cases as rc
+    gold = np.load(os.path.join(os.path.dirname(__file__), "golden", "reference_golden_3d",
+                                "fesom2_elem_rk2_unfused_lerp.npz"), allow_pickle=False)
+    mesh = rc.elem_mesh()
+    a, b = (VelocityField(u=f[0], v=f[1], w=f[2], uv_layout="element")
+            for f in (rc.elem_field(mesh, 0.0), rc.elem_field(mesh, 1.0)))
+    lon, lat, z = rc.elem_seeds()
+    st = make_state(lon, lat, z)
+    T = rc.ELEM_STEPS * rc.ELEM_DT
+    rows = []
+    for i in range(rc.ELEM_STEPS):
+        s = i * rc.ELEM_DT
+        alpha = s / T
+        vel = a if abs(alpha) < 1e-12 else LazyLerpField(a, b, alpha)
+        st = I.rk2_step(st, vel, mesh, rc.ELEM_DT, 1.0, uv_interp="leastsq", w_interp="leastsq",
+                        vel_next=b, dt_to_next=T - s)
+        rows.append(st)
+    for k in ("lon", "lat", "z", "u", "v", "w"):
+        got = np.stack([getattr(r, k) for r in rows])
+        np.testing.assert_allclose(got, gold[k], rtol=1e-9, atol=1e-9, err_msg=k)
+    np.testing.assert_array_equal(np.stack([r.status for r in rows]), gold["status"])
+    np.testing.assert_array_equal(np.stack([r.failure_reason for r in rows]), gold["failure_reason"])

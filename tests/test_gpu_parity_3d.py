@@ -77,7 +77,7 @@ def _assert_bitwise(h, st):
         assert np.array_equal(h[k], st[k]), k
 
 
-@pytest.mark.parametrize("order,w_method", [(4, 2), (2, 2), (2, 0), (1, 2)])
+@pytest.mark.parametrize("order,w_method", [(4, 2), (2, 2), (2, 0), (1, 2), (4, 1), (2, 1)])
 def test_3d_tinterp_bitwise(gpu_ctx, world, order, w_method):
     h, st = _run(gpu_ctx, world, order, w_method, 1.0, 16, None, None)
     _assert_bitwise(h, st)
@@ -148,4 +148,35 @@ def test_3d_az_kick_bitwise(gpu_ctx, world, order):
     column ends), forward and with a time blend."""
     kick = dict(kh_x=10.0, kh_y=10.0, seed=31, az_const=5e-3)
     h, st = _run(gpu_ctx, world, order, 2, 1.0, 12, kick, None)
+    _assert_bitwise(h, st)
+
+
+def test_3d_element_layout_lsq_bitwise(gpu_ctx, world):
+    """u/v at element centres (least squares) with leastsq W, time blend,
+    Kh walk and beaching."""
+    mesh, snaps, lon, lat, z, pid = world
+    gpu_ctx.upload_mesh(mesh)
+    rng = np.random.default_rng(3)
+    K = mesh.sigma.size
+    esnaps = []
+    for s in range(3):
+        ue = (0.3 * rng.standard_normal((K - 1, mesh.n_elements))).astype(np.float32)
+        ve = (0.3 * rng.standard_normal((K - 1, mesh.n_elements))).astype(np.float32)
+        esnaps.append((ue, ve, snaps[s][2]))
+        gpu_ctx.upload_field(s, ue, ve, snaps[s][2], uv_layout="element")
+    p = gpu_ctx.particles(lon, lat, z, pid=pid)
+    om = OracleMesh(mesh, "kmath")
+    st = dict(lon=lon, lat=lat, z=z, status=np.ones(lon.size, np.int32),
+              nn=np.full(lon.size, -1, np.int32), pid=pid)
+    kick = dict(kh_x=10.0, kh_y=10.0, seed=5)
+    for k in range(10):
+        f, (ia, ib, it, a0, dtn) = _fields(esnaps, k * DT, 1.0, np.float32)
+        f["uv_layout"] = "element"
+        al = [c * DT / dtn for c in (0.0, 0.5, 0.5, 1.0)]
+        args = step_args(4, DT, 1.0, is3d=True, w_method=1, alphas=al, kick=kick, beach=BEACH,
+                         step_i=k)
+        st = om.step(st, f, args)
+        gpu_ctx.step(p, 4, ia, DT, 1.0, slot_b=ib, slot_t=it, alpha0=a0, alphas=al, w_method=1,
+                     kick=kick, beach=BEACH, step_i=k)
+    h = p.to_host()
     _assert_bitwise(h, st)

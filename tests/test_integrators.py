@@ -43,6 +43,8 @@ def test_refused_combinations_raise_before_the_device(small):
     st = _state()
     with pytest.raises(NotImplementedError, match="uv_interp"):
         I.rk2_step(st, vel, mesh, 600.0, 1.0, uv_interp="leastsq")
+    with pytest.raises(NotImplementedError, match="uv_interp"):
+        I.rk2_step(st, vel, mesh, 600.0, 1.0, uv_interp="edge_leastsq")
     el = VelocityField(u=np.zeros(mesh.n_elements), v=np.zeros(mesh.n_elements))
     with pytest.raises(NotImplementedError, match="uv_layout"):
         I.rk2_step(st, el, mesh, 600.0, 1.0, uv_interp="barycentric")

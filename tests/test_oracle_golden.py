@@ -58,3 +58,29 @@ def test_oracle_matches_reference_golden(name, direction, flavour):
         np.testing.assert_array_equal(got[k], gold[k], err_msg=k)
     for k in ("lon", "lat", "z", "u", "v", "w"):
         np.testing.assert_allclose(got[k], gold[k], rtol=1e-9, atol=1e-9, err_msg=k)
+
+
+@pytest.mark.parametrize("name,direction", [
+    ("fesom2_elem_euler_fused", 1.0),
+    ("fesom2_elem_rk2_unfused_lerp", 1.0), ("fesom2_elem_rk2_unfused_lerp", -1.0),
+])
+@pytest.mark.parametrize("flavour", ["glibc", "kmath"])
+def test_oracle_matches_reference_golden_element_layout(name, direction, flavour):
+    """Element-centre u/v by least squares and leastsq W (steady fused Euler,
+    time-blended unfused RK2 forward and backward)."""
+    fname = name + ("_bwd" if direction < 0 else "") + ".npz"
+    gold = np.load(os.path.join(GOLD, fname), allow_pickle=False)
+
+    def stepper(mesh, st, fields, order, alphas, i):
+        om = _MESHES.get((id(mesh), flavour))
+        if om is None:
+            om = _MESHES[(id(mesh), flavour)] = OracleMesh(mesh, flavour)
+        args = step_args(order, rc.ELEM_DT, direction, is3d=True, w_method=1, exact=2,
+                         alphas=alphas, step_i=i)
+        return om.step(st, fields, args)
+
+    got = rc.run_elem_case(name, direction, stepper)
+    for k in ("status", "failure_reason"):
+        np.testing.assert_array_equal(got[k], gold[k], err_msg=k)
+    for k in ("lon", "lat", "z", "u", "v", "w"):
+        np.testing.assert_allclose(got[k], gold[k], rtol=1e-9, atol=1e-9, err_msg=k)
