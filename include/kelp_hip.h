@@ -97,6 +97,12 @@ typedef struct {
     int32_t az;                    /* vertical Az kick after the horizontal one
                                       (integrators.py:842), constant source, 3-D */
     double az_const;               /* m^2/s; 0 = no kick */
+    int32_t use_eta;               /* beaching: water depth = depth + eta of the step's
+                                      field (beaching.py:121; needs kelp_field_upload_eta) */
+    int32_t periodic_lon, periodic_lat; /* fold ALIVE particles back into [px0, px1) /
+                                      [py0, py1) after the kicks, before beaching
+                                      (tracking.py:1089 _apply_periodic_wrap) */
+    double px0, px1, py0, py1;
 } kelp_step_args;
 
 const char *kelp_last_error(void);
@@ -134,6 +140,9 @@ int kelp_field_upload(kelp_ctx *ctx, int slot, int dtype, int64_t n_planes,
 int kelp_field_upload_elem(kelp_ctx *ctx, int slot, int dtype, int64_t n_planes,
                            const void *host_u, const void *host_v,
                            const void *host_w, int64_t n_w_planes);
+/* Attach the snapshot's surface elevation (VelocityField.eta, (N,) doubles
+ * in mesh node order) to an uploaded slot; read by beaching with use_eta. */
+int kelp_field_upload_eta(kelp_ctx *ctx, int slot, const double *host_eta);
 int kelp_field_clear(kelp_ctx *ctx, int slot);
 
 /* Advance every particle by one step, in place, on the context stream. */

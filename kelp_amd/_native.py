@@ -60,7 +60,9 @@ class StepArgs(ctypes.Structure):
         ("depth_min", c_double), ("zone_depth", c_double), ("t_beach", c_double),
         ("rewet_hysteresis", c_double), ("t_resuspend", c_double), ("beach_dt", c_double),
         ("beach_seed", ctypes.c_uint64),
-        ("az", c_int32), ("az_const", c_double),
+        ("az", c_int32), ("az_const", c_double), ("use_eta", c_int32),
+        ("periodic_lon", c_int32), ("periodic_lat", c_int32),
+        ("px0", c_double), ("px1", c_double), ("py0", c_double), ("py1", c_double),
     ]
 
 
@@ -80,6 +82,7 @@ SIGNATURES = {
                                   c_void_p, c_int64]),
     "kelp_field_upload_elem": (c_int, [c_void_p, c_int, c_int, c_int64, c_void_p, c_void_p,
                                        c_void_p, c_int64]),
+    "kelp_field_upload_eta": (c_int, [c_void_p, c_int, c_void_p]),
     "kelp_field_clear": (c_int, [c_void_p, c_int]),
     "kelp_step": (c_int, [c_void_p, POINTER(Particles), POINTER(StepArgs)]),
     "kelp_sort_particles": (c_int, [c_void_p, POINTER(Particles)]),

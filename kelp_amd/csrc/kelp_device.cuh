@@ -51,6 +51,7 @@ struct DMesh {
     const double* __restrict__ zlev;      // K interface depths (3-D)
     const uint8_t* __restrict__ coastal;  // (E,) coastal-element flags (beaching)
     const double2* __restrict__ ecent;    // (E,) element centres (lon, lat); element-layout u/v
+    const double* __restrict__ coslat;    // (N,) km_cos(lat * kDeg2Rad) per node
     double cell, lon_min, lat_min, r_earth;
     int32_t nx, ny, wrap, far_rows;
     int32_t n_nodes, n_elems, max_v, n_levels;
@@ -180,15 +181,17 @@ __device__ inline int nearest_3x3(const DMesh& m, int ci, int cj, const Query& q
     return best;
 }
 
-// core/mesh.py:1307 _cap_extents
-__device__ inline void cap_extents(double lon_p, double lat_p, double lon_n, double lat_n,
+// core/mesh.py:1307 _cap_extents, for the query q and a mesh node.  The
+// sines and cosines of both latitudes come from tables instead of being
+// recomputed: the query's are those make_query took of the same argument,
+// the node's sine is the z of its unit vector and its cosine the node's
+// coslat entry, all from km_sincos(lat * kDeg2Rad) -- the same bits.
+__device__ inline void cap_extents(const DMesh& m, const Query& q, double lon_p, double lat_p, int node,
                                    double& theta_deg, double& ext_lon)
 {
-    double lat_pr = lat_p * kDeg2Rad;
-    double lat_nr = lat_n * kDeg2Rad;
-    double sp, cp, sn, cn;
-    km_sincos(lat_pr, &sp, &cp);
-    km_sincos(lat_nr, &sn, &cn);
+    const double lon_n = m.lonlat[node].x;
+    const double sp = q.z, cp = q.cos_lat;
+    const double sn = m.gh_xyz[node].z, cn = m.coslat[node];
     double cosd = (sp * sn + cp * cn * km_cos((lon_p - lon_n) * kDeg2Rad));
     if (cosd > 1.0) cosd = 1.0;
     if (cosd < -1.0) cosd = -1.0;
@@ -217,13 +220,11 @@ __device__ __forceinline__ int pyimod(int a, int n)
     return r < 0 ? r + n : r;
 }
 
-// core/mesh.py:1476 _anchor_provable_rect
-__device__ inline int anchor_provable_rect(const DMesh& m, double lon_p, double lat_p, int node,
-                                           int rx, int ry)
+// core/mesh.py:1476 _anchor_provable_rect with the node's cap (theta_deg,
+// ext_lon) from cap_extents
+__device__ inline int anchor_provable_rect(const DMesh& m, double lon_p, double lat_p, double theta_deg,
+                                           double ext_lon, int rx, int ry)
 {
-    double2 ll = m.lonlat[node];
-    double theta_deg, ext_lon;
-    cap_extents(lon_p, lat_p, ll.x, ll.y, theta_deg, ext_lon);
     if (theta_deg <= 0.0) return 0;
     const double cell = m.cell, lon_min = m.lon_min, lat_min = m.lat_min;
     const int nx = m.nx, ny = m.ny;
@@ -333,11 +334,10 @@ __device__ inline int nearest_exact(const DMesh& m, double lon_p, double lat_p)
     int best = nearest_3x3(m, ci, cj, q, best_d2);
     int code = 0;
     if (best >= 0) {
-        code = anchor_provable_rect(m, lon_p, lat_p, best, 1, 1);
-        if (code == 0) return best;
-        double2 ll = m.lonlat[best];
         double theta_deg, ext;
-        cap_extents(lon_p, lat_p, ll.x, ll.y, theta_deg, ext);
+        cap_extents(m, q, lon_p, lat_p, best, theta_deg, ext);
+        code = anchor_provable_rect(m, lon_p, lat_p, theta_deg, ext, 1, 1);
+        if (code == 0) return best;
         if (theta_deg > far) return best;
     }
     return exact_widen(m, lon_p, lat_p, best, best_d2, code, ci, cj, q);
@@ -365,7 +365,9 @@ __device__ __noinline__ int exact_widen(const DMesh& m, double lon_p, double lat
     const double INF = 1.0e30;
     for (;;) {
         if (best >= 0 && code == 0) {
-            code = anchor_provable_rect(m, lon_p, lat_p, best, rx, ry);
+            double theta_deg, ext;
+            cap_extents(m, q, lon_p, lat_p, best, theta_deg, ext);
+            code = anchor_provable_rect(m, lon_p, lat_p, theta_deg, ext, rx, ry);
             if (code == 0) return best;
         }
         if (best < 0 || 2.0 * km_asin(0.5 * km_sqrt(best_d2)) * kRad2Deg > far) {

@@ -74,6 +74,7 @@ static void free_slot(FieldSlot& s)
 {
     if (s.rec) (void)hipFree(s.rec);
     if (s.erec) (void)hipFree(s.erec);
+    if (s.eta) (void)hipFree(s.eta);
     s = FieldSlot{};
 }
 
@@ -95,13 +96,15 @@ static int dev_alloc(kelp_ctx* c, T** out, size_t count)
 
 // core/mesh.py:713 _node_xyz_fill.  Internal node k is the node in slot k of
 // the hash's CSR order, so the table is already in scan order.
-__global__ void node_xyz_kernel(int64_t n, const double2* __restrict__ lonlat, double4* __restrict__ out)
+__global__ void node_xyz_kernel(int64_t n, const double2* __restrict__ lonlat, double4* __restrict__ out,
+                                double* __restrict__ coslat)
 {
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     double2 ll = lonlat[k];
     double lat_r = ll.y * kDeg2Rad, lon_r = ll.x * kDeg2Rad;
     double c = km_cos(lat_r);
+    coslat[k] = c;
     double4 r;
     r.x = c * km_cos(lon_r);
     r.y = c * km_sin(lon_r);
@@ -451,14 +454,14 @@ int kelp_mesh_upload(kelp_ctx* c, const kelp_mesh_desc* d)
     double4* d_xyz;
     int32_t *d_off, *d_n2e_off, *d_n2e, *d_nodeo, *d_noden, *d_elemo, *d_elemn;
     int4* d_elems;
-    double *d_depth, *d_zlev;
+    double *d_depth, *d_zlev, *d_coslat;
     uint8_t* d_coastal = nullptr;
     double2* d_ecent = nullptr;
     int rc;
     if ((rc = dev_alloc(c, &d_lonlat, N)) || (rc = dev_alloc(c, &d_xyz, N)) ||
         (rc = dev_alloc(c, &d_off, n_cells + 1)) || (rc = dev_alloc(c, &d_elems, E)) ||
         (rc = dev_alloc(c, &d_n2e_off, N + 1)) || (rc = dev_alloc(c, &d_n2e, n2e.size())) ||
-        (rc = dev_alloc(c, &d_depth, N)) || (rc = dev_alloc(c, &d_zlev, d->n_levels > 0 ? d->n_levels : 1)) ||
+        (rc = dev_alloc(c, &d_depth, N)) || (rc = dev_alloc(c, &d_coslat, N)) || (rc = dev_alloc(c, &d_zlev, d->n_levels > 0 ? d->n_levels : 1)) ||
         (rc = dev_alloc(c, &d_nodeo, N)) || (rc = dev_alloc(c, &d_noden, N)) ||
         (rc = dev_alloc(c, &d_elemo, E)) || (rc = dev_alloc(c, &d_elemn, E)) ||
         (d->host_elem_coastal && (rc = dev_alloc(c, &d_coastal, E))) ||
@@ -480,7 +483,7 @@ int kelp_mesh_upload(kelp_ctx* c, const kelp_mesh_desc* d)
         HIP_TRY(hipMemcpy(d_zlev, d->host_z_levels, d->n_levels * sizeof(double), hipMemcpyHostToDevice));
     if (d_coastal) HIP_TRY(hipMemcpy(d_coastal, coastal.data(), E * sizeof(uint8_t), hipMemcpyHostToDevice));
     if (d_ecent) HIP_TRY(hipMemcpy(d_ecent, ecent.data(), E * sizeof(double2), hipMemcpyHostToDevice));
-    node_xyz_kernel<<<grid_for(N, 256), 256, 0, c->stream>>>(N, d_lonlat, d_xyz);
+    node_xyz_kernel<<<grid_for(N, 256), 256, 0, c->stream>>>(N, d_lonlat, d_xyz, d_coslat);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(c->stream));
 
@@ -494,6 +497,7 @@ int kelp_mesh_upload(kelp_ctx* c, const kelp_mesh_desc* d)
     m.zlev = d_zlev;
     m.coastal = d_coastal;
     m.ecent = d_ecent;
+    m.coslat = d_coslat;
     m.cell = d->gh_cell;
     m.lon_min = d->gh_lon_min;
     m.lat_min = d->gh_lat_min;
@@ -648,6 +652,37 @@ int kelp_field_upload_elem(kelp_ctx* c, int slot, int dtype, int64_t n_planes, c
     return field_upload(c, "kelp_field_upload_elem", slot, dtype, 1, n_planes, hu, hv, hw, n_w_planes);
 }
 
+// eta of a snapshot (mesh node order) into internal order
+__global__ void permute_nodes_kernel(int64_t N, const int32_t* __restrict__ old_of_new, const double* __restrict__ in,
+                                     double* __restrict__ out)
+{
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    out[k] = in[old_of_new[k]];
+}
+
+int kelp_field_upload_eta(kelp_ctx* c, int slot, const double* host_eta)
+{
+    if (!c || !host_eta) return fail(KELP_EINVAL, "kelp_field_upload_eta: NULL argument");
+    if (slot < 0 || slot >= kMaxSlots) return fail(KELP_EINVAL, "kelp_field_upload_eta: slot %d out of range", slot);
+    FieldSlot& s = c->slots[slot];
+    if (s.dtype < 0) return fail(KELP_ESTATE, "kelp_field_upload_eta: slot %d holds no snapshot", slot);
+    HIP_TRY(hipSetDevice(c->device));
+    const int64_t N = c->m.n_nodes;
+    if (!s.eta) HIP_TRY(hipMalloc(&s.eta, N * sizeof(double)));
+    double* tmp = nullptr;
+    HIP_TRY(hipMalloc(&tmp, N * sizeof(double)));
+    hipError_t e = hipMemcpyAsync(tmp, host_eta, N * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        permute_nodes_kernel<<<grid_for(N, 256), 256, 0, c->stream>>>(N, c->node_old_of_new, tmp, s.eta);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return fail(KELP_EHIP, "kelp_field_upload_eta: %s", hipGetErrorString(e));
+    return KELP_OK;
+}
+
 int kelp_field_clear(kelp_ctx* c, int slot)
 {
     if (!c || slot < 0 || slot >= kMaxSlots) return fail(KELP_EINVAL, "kelp_field_clear: bad argument");
@@ -740,6 +775,21 @@ int kelp_step(kelp_ctx* c, const kelp_particles* p, const kelp_step_args* a)
     cfg.beach_seed = a->beach_seed;
     cfg.az = a->az && a->az_const > 0.0;
     cfg.az_const = a->az_const;
+    if ((a->periodic_lon && !(a->px1 > a->px0)) || (a->periodic_lat && !(a->py1 > a->py0)))
+        return fail(KELP_EINVAL, "kelp_step: a periodic window needs x0 < x1");
+    cfg.periodic_lon = a->periodic_lon ? 1 : 0;
+    cfg.periodic_lat = a->periodic_lat ? 1 : 0;
+    cfg.px0 = a->px0;
+    cfg.px1 = a->px1;
+    cfg.py0 = a->py0;
+    cfg.py1 = a->py1;
+    if (a->beach && a->use_eta) {
+        if (!sa.eta || (sb && !sb->eta))
+            return fail(KELP_ESTATE, "kelp_step: beaching with use_eta needs eta on the bracket snapshots");
+        cfg.use_eta = 1;
+        cfg.eta_a = sa.eta;
+        cfg.eta_b = sb ? sb->eta : sa.eta;
+    }
     PState ps{p->lon, p->lat, p->z, p->status, p->nn, p->fr, p->u, p->v, p->w, p->pid, p->wet_streak,
               reinterpret_cast<unsigned long long*>(p->alive_count), p->n};
     if (sa.dtype == KELP_DT_F32)

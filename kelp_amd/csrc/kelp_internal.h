@@ -35,6 +35,7 @@ struct FieldSlot {
     bool is3d = false;
     void* rec = nullptr;
     void* erec = nullptr;
+    double* eta = nullptr;  // (N,) surface elevation, internal node order, or null
 };
 
 // growable device buffer owned by a context

@@ -26,6 +26,11 @@ struct StepCfg {
     uint64_t beach_seed;
     int az;           // vertical Az kick, constant source
     double az_const;
+    int use_eta;      // beaching water depth = depth + eta of the step's field
+    const double* eta_a;  // (N,) eta of the bracket pair (internal node order)
+    const double* eta_b;
+    int periodic_lon, periodic_lat;  // post-step fold windows [x0, x1), [y0, y1)
+    double px0, px1, py0, py1;
 };
 
 struct PState {

@@ -241,6 +241,28 @@ __device__ inline void az_one(const DMesh& m, const StepCfg& c, uint64_t pid, in
     z = km_reflect_depth(z + dz, m.zlev[0], m.depth[node]);
 }
 
+// tracking.py:1089 _apply_periodic_wrap for one ALIVE particle: fold a
+// coordinate outside [x0, x1) back with numpy's mod (Python % semantics),
+// strictly inside the window, and re-anchor a folded particle with the batch
+// search (find_nearest_nodes).
+__device__ inline void periodic_one(const DMesh& m, const StepCfg& c, double& lon, double& lat, int& nn)
+{
+    bool moved = false;
+    if (c.periodic_lon && (lon < c.px0 || lon >= c.px1)) {
+        double fo = c.px0 + km_pymod_pos(lon - c.px0, c.px1 - c.px0);
+        if (fo >= c.px1) fo = c.px0;
+        lon = fo;
+        moved = true;
+    }
+    if (c.periodic_lat && (lat < c.py0 || lat >= c.py1)) {
+        double fo = c.py0 + km_pymod_pos(lat - c.py0, c.py1 - c.py0);
+        if (fo >= c.py1) fo = c.py0;
+        lat = fo;
+        moved = true;
+    }
+    if (moved) nn = site_anchor(m, lon, lat, c.exact == 1 ? 1 : 2);
+}
+
 // tracking.py:199 _apply_beaching_hook + numerics/beaching.py:95
 // evaluate_beaching, one particle (prev = status at the step start).
 template <int MAXV>
@@ -255,7 +277,14 @@ __device__ inline void beach_one(const DMesh& m, const StepCfg& c, uint64_t pid,
     if (need_topo && (st == kAlive || st == kBeached || newly_failed) && n >= 0)
         containing = find_containing<MAXV>(m, lon, lat, n);
     const bool coastal = need_topo && containing >= 0 && m.coastal != nullptr && m.coastal[containing];
-    const double h_raw = n >= 0 ? m.depth[n] : __builtin_inf();
+    double h_raw = n >= 0 ? m.depth[n] : __builtin_inf();
+    // + vel.eta[nn] of the step's field: the bracket pair blended by alpha0
+    // (core/io.py:386 LazyLerpField), or the plain snapshot
+    if (c.use_eta && n >= 0) {
+        double eta = c.eta_a[n];
+        if (c.alpha0 != 0.0) eta = (1.0 - c.alpha0) * c.eta_a[n] + c.alpha0 * c.eta_b[n];
+        h_raw = h_raw + eta;
+    }
     if (c.on_failure && newly_failed) {
         if (h_raw < c.zone_depth || coastal) {
             st = kBeached;
@@ -335,6 +364,7 @@ __global__ __launch_bounds__(256) KELP_STEP_ATTR void step_kernel(DMesh m, Snap<
             kick_one<MAXV, IS3D>(m, c, (uint64_t)s.pid[i], pre_lon, pre_lat, pre_z, pre_nn, lon, lat, z, st, nn, fr);
         }
         if (IS3D && c.az && st == kAlive) az_one(m, c, (uint64_t)s.pid[i], nn, z);
+        if ((c.periodic_lon || c.periodic_lat) && st == kAlive) periodic_one(m, c, lon, lat, nn);
     }
     if (c.beach) beach_one<MAXV>(m, c, (uint64_t)s.pid[i], prev, lon, lat, st, nn, fr, s.ws[i]);
     s.lon[i] = lon;

@@ -81,7 +81,8 @@ RESUSPENSION_CODES = {"never": 0, "auto": 1, "stochastic": 2, "auto_or_stochasti
 
 def make_step_args(order, slot_a, dt, time_dir, slot_b=-1, slot_t=-1, alpha0=0.0,
                    alphas=(0.0, 0.0, 0.0, 0.0), exact_nearest=True, w_method=2, kick=None,
-                   beach=None, step_i=0, real_id=0) -> StepArgs:
+                   beach=None, step_i=0, real_id=0, periodic_lon=None,
+                   periodic_lat=None) -> StepArgs:
     a = StepArgs()
     a.order = int(order)
     a.slot_a, a.slot_b, a.slot_t = int(slot_a), int(slot_b), int(slot_t)
@@ -94,6 +95,10 @@ def make_step_args(order, slot_a, dt, time_dir, slot_b=-1, slot_t=-1, alpha0=0.0
     a.w_method = int(w_method)
     a.step_i = int(step_i)
     a.real_id = int(real_id)
+    if periodic_lon is not None:
+        a.periodic_lon, a.px0, a.px1 = 1, float(periodic_lon[0]), float(periodic_lon[1])
+    if periodic_lat is not None:
+        a.periodic_lat, a.py0, a.py1 = 1, float(periodic_lat[0]), float(periodic_lat[1])
     if kick:
         a.kick = 1
         a.kh_x = float(kick["kh_x"])
@@ -118,6 +123,7 @@ def make_step_args(order, slot_a, dt, time_dir, slot_b=-1, slot_t=-1, alpha0=0.0
         a.t_resuspend = float(beach.get("t_resuspend") or 0.0)
         a.beach_dt = float(beach.get("dt", dt))
         a.beach_seed = int(beach.get("seed", a.seed))
+        a.use_eta = int(bool(beach.get("use_eta", False)))
     return a
 
 
@@ -184,6 +190,13 @@ class DeviceContext:
         return out
 
     # -- fields -------------------------------------------------------------
+    def upload_eta(self, slot: int, eta) -> None:
+        """Attach the snapshot's surface elevation (N,) to ``slot``."""
+        eta = np.ascontiguousarray(eta, np.float64)
+        if eta.shape != (self.mesh.n_nodes,):
+            raise ValueError(f"eta of shape {eta.shape} does not match the mesh's {self.mesh.n_nodes} nodes")
+        check(self.lib.kelp_field_upload_eta(self.h, int(slot), eta.ctypes.data), "kelp_field_upload_eta")
+
     def upload_field(self, slot: int, u, v, w=None, key=None, uv_layout="node") -> None:
         """Upload one snapshot into ``slot``; skipped when ``key`` matches the
         slot's current content.  ``uv_layout``: "node" (u, v per node) or
@@ -247,12 +260,13 @@ class DeviceContext:
     def step(self, p: DeviceParticles, order: int, slot_a: int, dt: float, time_dir: float,
              slot_b: int = -1, slot_t: int = -1, alpha0: float = 0.0, alphas=(0.0, 0.0, 0.0, 0.0),
              exact_nearest: bool = True, w_method: int = 2, kick=None, beach=None,
-             step_i: int = 0, real_id: int = 0) -> None:
+             step_i: int = 0, real_id: int = 0, periodic_lon=None, periodic_lat=None) -> None:
         """One step in place.  ``kick``: dict(kh_x, kh_y, seed[, az_const])
         for the split uniform-Kh random walk (and the vertical Az kick);
         ``beach``: dict of the beaching options (criteria, depth_min, ...)."""
         a = make_step_args(order, slot_a, dt, time_dir, slot_b, slot_t, alpha0, alphas,
-                           exact_nearest, w_method, kick, beach, step_i, real_id)
+                           exact_nearest, w_method, kick, beach, step_i, real_id,
+                           periodic_lon, periodic_lat)
         s = p.as_struct()
         check(self.lib.kelp_step(self.h, ctypes.byref(s), ctypes.byref(a)), "kelp_step")
 

@@ -1,0 +1,434 @@
+"""The tracking loop on the device, mirroring src/kelp/tracking.py.
+
+``run_single_realization`` restates the per-realization loop of the
+reference (tracking.py:1805-2833) around ``kelp_step``: the run clock and
+the time bracket of every step (``resolve_time_bracket``, tracking.py:2947,
+and the payload of ``_build_vel_payload``, :3136), the step length that
+lands exactly on a record (the snapshot band, :1667/:2530-2556), delayed
+release (core/release.py:112), the record schedule on the global step and
+the record masks of output format 2 (core/release.py:132/:170, the backward
+velocity flip of tracking.py:2772-2777).  The particle state stays in HBM
+for the whole run; the integrator step, the kicks, the periodic fold and
+the beaching hook of one step are one kernel launch; a record copies the
+state to the host.  ``run_tracking(config)`` is the config-driven entry
+(tracking.py:3288).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional
+
+import numpy as np
+
+from .config import AppConfig, ModeConfig, NumericsConfig, resolve_time_params
+from .fields import VelocityField
+from .integrators import ORDERS, W_METHODS
+from .mesh import Mesh, exact_nearest_on, set_exact_nearest
+from .particles import ParticleState, ParticleStatus
+
+_C = {1: (0.0,), 2: (0.0, 0.5), 4: (0.0, 0.5, 0.5, 1.0)}
+N_SLOTS = 16
+
+
+def snapshot_band(dt) -> float:
+    """tracking.py:1667 _snapshot_band."""
+    return 1e-5 * max(abs(float(dt)), 1.0)
+
+
+def _cycle_period(file_times, n):
+    """The legacy cycle rule: the span of the records plus the trailing
+    interval."""
+    last_dt = file_times[-1] - file_times[-2] if n > 1 else 1.0
+    return (file_times[-1] - file_times[0]) + last_dt
+
+
+def wrap_sim_time(sim_time, file_times, n, on_exhausted):
+    """tracking.py:2919 _wrap_sim_time (legacy cycle rule)."""
+    if on_exhausted != "cycle":
+        return sim_time
+    t0 = file_times[0]
+    return t0 + ((sim_time - t0) % _cycle_period(file_times, n))
+
+
+def resolve_time_bracket(sim_time, file_times, n, on_exhausted, direction=1, snap=0.0):
+    """tracking.py:2947 _resolve_time_bracket (legacy cycle rule):
+    ``(idx_a, idx_b, ta, tb, t_wrapped)`` or None outside the records in
+    stop mode."""
+    t_start, t_end = file_times[0], file_times[-1]
+    tw = wrap_sim_time(sim_time, file_times, n, on_exhausted)
+    if on_exhausted != "cycle" and (tw < t_start - 1e-6 or tw > t_end + 1e-6):
+        return None
+    idx_b = int(np.searchsorted(file_times, tw, side="right"))
+    idx_a = idx_b - 1
+    if on_exhausted == "cycle":
+        ia, ib = idx_a % n, idx_b % n
+    else:
+        ia, ib = max(0, min(idx_a, n - 1)), max(0, min(idx_b, n - 1))
+    ta, tb = file_times[ia], file_times[ib]
+    last_dt = file_times[-1] - file_times[-2] if n > 1 else 1.0
+    if on_exhausted == "cycle" and tb <= ta:
+        tb = ta + last_dt
+    if direction > 0 and snap > 0.0 and ia != ib and tb - tw <= snap:
+        s = ib
+        s_next = (s + 1) % n if on_exhausted == "cycle" else min(s + 1, n - 1)
+        ta_new, tb_new = file_times[s], file_times[s_next]
+        if on_exhausted == "cycle" and tb_new <= ta_new:
+            tb_new = ta_new + last_dt
+        return s, s_next, ta_new, tb_new, ta_new
+    if direction < 0 and tw <= ta + max(1e-6, snap):
+        ib_new = ia
+        ia_new = (ia - 1) % n if on_exhausted == "cycle" else ia - 1
+        if ia_new < 0 or ia_new == ib_new:
+            return ib_new, ib_new, ta, ta, tw
+        ta_new, tb_new = file_times[ia_new], file_times[ib_new]
+        if tb_new <= ta_new:
+            tb_new = ta_new + last_dt
+        return ia_new, ib_new, ta_new, tb_new, tb_new
+    return ia, ib, ta, tb, tw
+
+
+def step_end_on_record(sim_time, pos, dt_to_next, direction, file_times, n, on_exhausted):
+    """tracking.py:2930 _step_end_on_record (legacy cycle rule)."""
+    delta = pos - wrap_sim_time(sim_time, file_times, n, on_exhausted)
+    if on_exhausted == "cycle":
+        period = _cycle_period(file_times, n)
+        delta -= period * np.round(delta / period)
+    return sim_time + delta + dt_to_next * direction
+
+
+@dataclass
+class Payload:
+    """The step's field: bracket snapshots ``ia`` (and ``ib`` blended by
+    ``alpha0``), the target snapshot ``it`` (or None) and ``dt_to_next``."""
+    ia: int
+    ib: Optional[int]
+    alpha0: float
+    it: Optional[int]
+    dt_to_next: Optional[float]
+
+
+def vel_payload(bracket, direction) -> Payload:
+    """tracking.py:3136 _build_vel_payload as snapshot indices."""
+    ia, ib, ta, tb, tw = bracket
+    if ia == ib:
+        return Payload(ia, None, 0.0, None, None)
+    interval = tb - ta
+    alpha = (tw - ta) / interval if interval > 0 else 0.0
+    if abs(alpha) < 1e-12:
+        base = Payload(ia, None, 0.0, None, None)
+    elif abs(alpha - 1.0) < 1e-12:
+        base = Payload(ib, None, 0.0, None, None)
+    else:
+        base = Payload(ia, ib, alpha, None, None)
+    if direction >= 0:
+        base.it, base.dt_to_next = ib, tb - tw
+        return base
+    dt_prev = tw - ta
+    if dt_prev <= 1e-6:
+        return base
+    base.it, base.dt_to_next = ia, dt_prev
+    return base
+
+
+@dataclass
+class TrajectoryRecord:
+    """In-memory trajectories of one realization, ``(particle, record)``
+    arrays with the reference writer's variable names (writers.py:1209)."""
+    time: List[float] = field(default_factory=list)
+    rows: Dict[str, List[np.ndarray]] = field(default_factory=dict)
+
+    def append(self, rec: Dict[str, np.ndarray], time_val: float) -> None:
+        self.time.append(float(time_val))
+        for k, v in rec.items():
+            self.rows.setdefault(k, []).append(np.asarray(v))
+
+    def as_arrays(self) -> Dict[str, np.ndarray]:
+        out = {k: np.stack(v, axis=1) for k, v in self.rows.items()}
+        out["time"] = np.asarray(self.time)
+        return out
+
+    def save_npz(self, path: str) -> None:
+        np.savez(path, **self.as_arrays())
+
+
+class _SnapshotSlots:
+    """Device slots holding the snapshots of the current brackets, keyed by
+    time index, least recently used evicted (the reference's _VelCache,
+    tracking.py:584, on HBM)."""
+
+    def __init__(self, ctx, reader, ds, mesh, mode, numerics, sample_vars, need_eta):
+        self.ctx, self.reader, self.ds, self.mesh = ctx, reader, ds, mesh
+        self.mode, self.numerics = mode, numerics
+        self.sample_vars = sample_vars
+        self.need_eta = need_eta
+        self.map: Dict[int, int] = {}
+        self.order: List[int] = []
+        self.fields: Dict[int, VelocityField] = {}
+
+    def field(self, idx: int) -> VelocityField:
+        if idx not in self.fields:
+            self.fields[idx] = self.reader.read_velocities(
+                self.ds, idx, self.mesh, self.mode, sample_vars=self.sample_vars,
+                velocity_float32=self.numerics.velocity_float32)
+        return self.fields[idx]
+
+    def slot(self, idx: int) -> int:
+        if idx in self.map:
+            self.order.remove(idx)
+            self.order.append(idx)
+            return self.map[idx]
+        if len(self.map) < N_SLOTS:
+            slot = min(set(range(N_SLOTS)) - set(self.map.values()))
+        else:
+            old = self.order.pop(0)
+            slot = self.map.pop(old)
+            self.fields.pop(old, None)
+        vf = self.field(idx)
+        self.ctx.upload_field(slot, vf.u, vf.v, vf.w, uv_layout=vf.uv_layout)
+        if self.need_eta:
+            self.ctx.upload_eta(slot, vf.eta if vf.eta is not None else np.zeros(self.mesh.n_nodes))
+        self.map[idx] = slot
+        self.order.append(idx)
+        return slot
+
+
+def _beach_args(numerics: NumericsConfig, dt):
+    b = numerics.beaching
+    if not b.enabled:
+        return None
+    seed = b.seed if b.seed is not None else numerics.diffusion.seed
+    return dict(criteria=list(b.criteria), depth_min=b.depth_min, use_eta=b.use_eta,
+                zone_depth=b.zone_depth, t_beach=b.t_beach, resuspension=b.resuspension,
+                rewet_hysteresis=b.rewet_hysteresis, min_wet_steps=b.min_wet_steps,
+                t_resuspend=b.t_resuspend, seed=seed, dt=dt)
+
+
+def _kick_args(numerics: NumericsConfig):
+    d = numerics.diffusion
+    if not d.enabled:
+        return None
+    if d.scheme != "uniform":
+        raise NotImplementedError(f"device path: diffusion scheme {d.scheme!r}")
+    if d.az_enabled and d.var_az:
+        raise NotImplementedError("device path: the Az kick from an Az field (var_az)")
+    kick = dict(kh_x=d.kh_zonal, kh_y=d.kh_meridional, seed=d.seed)
+    if d.az_enabled and d.az_const > 0.0:
+        kick["az_const"] = d.az_const
+    return kick
+
+
+def _check_path(numerics: NumericsConfig, mesh: Mesh, is3d: bool, uv_layout: str) -> None:
+    if numerics.integrator not in ORDERS:
+        raise NotImplementedError(f"device path: integrator {numerics.integrator!r}")
+    if numerics.temporal_reconstruction != "linear":
+        raise NotImplementedError("device path: pchip reconstruction")
+    if numerics.elem_search != "local":
+        raise NotImplementedError("device path: elem_search='walk'")
+    want = "node" if numerics.uv_interp == "barycentric" else "element"
+    if numerics.uv_interp not in ("barycentric", "leastsq") or uv_layout != want:
+        raise NotImplementedError(f"device path: uv_interp={numerics.uv_interp!r} with "
+                                  f"{uv_layout}-layout u/v")
+    if is3d:
+        if mesh.vertical_coord != "zlevel":
+            raise NotImplementedError("device path: sigma meshes")
+        if numerics.w_interp not in W_METHODS:
+            raise NotImplementedError(f"device path: w_interp={numerics.w_interp!r}")
+    if numerics.beaching.enabled and numerics.beaching.dry_mask_var:
+        raise NotImplementedError("device path: beaching dry_mask_var")
+
+
+def activate_released(status: np.ndarray, release_step, global_step_i: int):
+    """core/release.py:112: NOT_RELEASED -> ALIVE once the global step
+    reaches the particle's release step."""
+    if release_step is None:
+        return status
+    act = (status == ParticleStatus.NOT_RELEASED) & (int(global_step_i) >= np.asarray(release_step))
+    if not act.any():
+        return status
+    out = status.copy()
+    out[act] = ParticleStatus.ALIVE
+    return out
+
+
+def mask_record(h: Dict[str, np.ndarray], prev_status: np.ndarray, direction: int,
+                is3d: bool) -> Dict[str, np.ndarray]:
+    """core/release.py:132 mask_dormant_for_record + :170
+    mask_inactive_velocity + the backward flip (tracking.py:2772)."""
+    out = dict(h)
+    dormant = h["status"] == ParticleStatus.NOT_RELEASED
+    bad = (h["status"] != ParticleStatus.ALIVE) | (prev_status != ParticleStatus.ALIVE)
+    for k in ("lon", "lat", "z"):
+        a = np.array(h[k], dtype=np.float64, copy=True)
+        a[dormant] = np.nan
+        out[k] = a
+    for k in ("u", "v", "w"):
+        a = np.array(h[k], dtype=np.float64, copy=True)
+        a[bad | dormant] = np.nan
+        if direction < 0:
+            a = -a
+        out[k] = a
+    return out
+
+
+def run_single_realization(mesh: Mesh, ds, reader, init_state: ParticleState, mode: ModeConfig,
+                           numerics: NumericsConfig, *, start_time_idx: int = 0,
+                           n_steps: int, n_time_total: Optional[int] = None,
+                           temporal_interp: Optional[bool] = None, file_times=None,
+                           file_dt: Optional[float] = None, sample_vars=None,
+                           realization_id: int = 0, particle_offset: int = 0,
+                           start_step_i: int = 0, on_exhausted: str = "stop",
+                           start_wet_streak=None, ctx=None, resort_every: int = 0,
+                           writer: Optional[TrajectoryRecord] = None) -> TrajectoryRecord:
+    """One realization on the device (tracking.py:1805).  Returns the
+    records, ``(particle, record)``: lon, lat, z, status, failure_reason,
+    vel_u/vel_v/vel_w (NaN where undefined), time."""
+    from .device import DeviceContext
+    if sample_vars:
+        raise NotImplementedError("device path: record-point sampling")
+    direction = 1 if mode.time_dir >= 0 else -1
+    order = ORDERS.get(numerics.integrator)
+    if file_times is None:
+        from .readers import extract_time_info
+        file_times, ax_dt = extract_time_info(ds)
+        file_dt = file_dt if file_dt is not None else ax_dt
+    file_times = None if file_times is None else np.asarray(file_times, dtype=np.float64)
+    n_time_total = n_time_total if n_time_total is not None else len(file_times)
+    dt = float(numerics.dt if numerics.dt is not None else file_dt)
+    temporal_interp = numerics.temporal_interp if temporal_interp is None else temporal_interp
+    use_tinterp = bool(temporal_interp) and file_times is not None
+    is3d = bool(mode.Fdrift3D)
+    uv_layout = reader.probe_uv_layout(ds, mode, mesh)
+    _check_path(numerics, mesh, is3d, uv_layout)
+    multistage = order > 1
+    rk_snap = use_tinterp and multistage
+    snap = snapshot_band(dt) if rk_snap else 5e-10 * max(dt, 1.0)
+    exact = 1 if exact_nearest_on(mesh) else 2
+    w_method = W_METHODS[numerics.w_interp] if is3d else 2
+    kick = _kick_args(numerics)
+    beach = _beach_args(numerics, dt)
+    periodic = dict(periodic_lon=numerics.periodic_lon, periodic_lat=numerics.periodic_lat)
+
+    if beach is not None and {"coastal_element", "stochastic"} & set(beach["criteria"]):
+        # tracking.py:2085: the coastal-element flags of the mesh boundary
+        if mesh.elem_is_coastal is None:
+            from .mesh import build_boundary_info
+            build_boundary_info(mesh)
+            if ctx is not None and ctx.mesh is mesh:
+                ctx.upload_mesh(mesh)
+    ctx = ctx or DeviceContext(0)
+    if ctx.mesh is not mesh:
+        ctx.upload_mesh(mesh)
+    slots = _SnapshotSlots(ctx, reader, ds, mesh, mode, numerics, None,
+                           need_eta=bool(beach and beach["use_eta"]))
+    P = init_state.n_particles
+    pid = particle_offset + np.arange(P, dtype=np.int64)
+    ws = (np.zeros(P, np.int32) if start_wet_streak is None
+          else np.asarray(start_wet_streak, np.int32))
+    fr0 = (np.zeros(P, np.int8) if init_state.failure_reason is None
+           else init_state.failure_reason)
+    p = ctx.particles(init_state.lon, init_state.lat, init_state.z, status=init_state.status,
+                      nn=init_state.nearest_node, fr=fr0, pid=pid, ws=ws)
+    torch = ctx.torch
+    release_step = init_state.release_step
+    rel_dev = None if release_step is None else torch.as_tensor(
+        np.asarray(release_step, np.int64), device=p.lon.device)
+    rec = writer if writer is not None else TrajectoryRecord()
+    sim_time = None
+    if use_tinterp:
+        if not 0 <= start_time_idx < n_time_total and on_exhausted != "cycle":
+            raise ValueError(f"start_time_idx {start_time_idx} is outside the records [0, {n_time_total})")
+        sim_time = file_times[start_time_idx % n_time_total]
+    t_g0 = file_times[start_time_idx % n_time_total] if file_times is not None else 0.0
+
+    for step_i in range(n_steps):
+        g = start_step_i + step_i
+        if rel_dev is not None:
+            act = (p.status == int(ParticleStatus.NOT_RELEASED)) & (rel_dev[p.pid - particle_offset] <= g)
+            p.status[act] = int(ParticleStatus.ALIVE)
+        prev_status = p.status.clone()
+        if use_tinterp:
+            bracket = resolve_time_bracket(sim_time, file_times, n_time_total, on_exhausted,
+                                           direction, snap)
+            if bracket is None:
+                break
+            pay = vel_payload(bracket, direction)
+        else:
+            file_idx = start_time_idx + step_i * direction
+            if on_exhausted == "cycle":
+                file_idx %= n_time_total
+            elif not 0 <= file_idx < n_time_total:
+                break
+            nxt = file_idx + direction
+            have_next = False
+            if file_times is not None:
+                if on_exhausted == "cycle" and n_time_total > 1:
+                    nxt, have_next = nxt % n_time_total, True
+                elif 0 <= nxt < n_time_total:
+                    have_next = True
+            pay = Payload(file_idx, None, 0.0, None, None)
+            if have_next:
+                ta, tb = file_times[file_idx], file_times[nxt]
+                last_dt = file_times[-1] - file_times[-2] if n_time_total > 1 else 1.0
+                if on_exhausted == "cycle" and direction > 0 and tb <= ta:
+                    tb = ta + last_dt
+                elif direction < 0 and tb >= ta:
+                    tb = ta - last_dt
+                pay.it, pay.dt_to_next = nxt, abs(tb - ta)
+        step_len = dt
+        sim_time_end = None
+        dtn = pay.dt_to_next
+        if rk_snap and dtn is not None and abs(dt - dtn) <= snap:
+            step_len = dtn
+            sim_time_end = step_end_on_record(sim_time, bracket[4], dtn, direction, file_times,
+                                              n_time_total, on_exhausted)
+        elif dtn is not None and dtn < dt <= dtn + snap:
+            dtn = dt
+        if dtn is not None and step_len > dtn:
+            pay.it, dtn = None, None           # steady RK for this step
+        sa = slots.slot(pay.ia)
+        sb = slots.slot(pay.ib) if pay.ib is not None else -1
+        st = slots.slot(pay.it) if pay.it is not None else -1
+        alphas = [0.0] * 4
+        if dtn is not None:
+            alphas = [c * step_len / dtn for c in _C[order]] + [0.0] * (4 - order)
+        if resort_every > 0 and g % resort_every == 0:
+            ctx.sort(p)
+        ctx.step(p, order, sa, step_len, float(direction), slot_b=sb, slot_t=st,
+                 alpha0=pay.alpha0, alphas=alphas, exact_nearest=exact, w_method=w_method,
+                 kick=kick, beach=beach, step_i=g, real_id=realization_id, **periodic)
+        if use_tinterp:
+            sim_time = sim_time_end if sim_time_end is not None else sim_time + dt * direction
+        if (g + 1) % max(int(numerics.output_freq), 1) == 0:
+            h = p.to_host()
+            prev = prev_status.cpu().numpy()
+            order_idx = np.argsort(h["pid"], kind="stable")
+            h = {k: v[order_idx] for k, v in h.items()}
+            prev = prev[order_idx]
+            m = mask_record(h, prev, direction, is3d)
+            rec.append(dict(lon=m["lon"], lat=m["lat"], z=m["z"], status=m["status"],
+                            failure_reason=m["fr"], vel_u=m["u"], vel_v=m["v"], vel_w=m["w"]),
+                       t_g0 + (g + 1) * dt * direction)
+    return rec
+
+
+def run_tracking(config: AppConfig, *, mesh: Mesh, dataset, reader="fesom2_arrays",
+                 init_state: ParticleState, ctx=None) -> List[TrajectoryRecord]:
+    """tracking.py:3288 run_tracking for in-memory inputs: the mesh, the
+    dataset and the seed state are passed in (the file readers of the
+    reference are outside this build); the rest comes from ``config``.
+    Sets the mesh's exact-nearest guard from ``mesh.exact_nearest``
+    (tracking.py:3370) and runs ``time.num_runs`` realizations, each
+    shifted by ``time.run_interval`` records."""
+    from .readers import get_reader
+    rd = get_reader(reader) if isinstance(reader, str) else reader
+    ds = rd.open_dataset(dataset)
+    set_exact_nearest(mesh, bool(config.mesh.exact_nearest))
+    start, n_steps = resolve_time_params(config.time)
+    out = []
+    for r in range(int(config.time.num_runs)):
+        out.append(run_single_realization(
+            mesh, ds, rd, init_state, config.mode, config.numerics,
+            start_time_idx=start + r * int(config.time.run_interval), n_steps=n_steps,
+            on_exhausted=config.time.on_exhausted, realization_id=r,
+            particle_offset=int(config.data.particle_offset or 0), ctx=ctx))
+    return out

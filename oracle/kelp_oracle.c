@@ -1389,10 +1389,32 @@ static void az_one(const or_mesh_t *m, const or_step_args_t *a, uint64_t pid, in
     *z = REFLECT_DEPTH(*z + dz, m->z_levels[0], m->depth[node]);
 }
 
+/* tracking.py:1089 _apply_periodic_wrap for one ALIVE particle: fold a
+ * coordinate outside [x0, x1) back with numpy's mod (Python % semantics),
+ * keep it strictly inside, and re-anchor a folded particle with the batch
+ * search (find_nearest_nodes). */
+static void periodic_one(const or_mesh_t *m, const or_step_args_t *a, double *lon, double *lat, int32_t *nn)
+{
+    int moved = 0;
+    if (a->periodic_lon && (*lon < a->px0 || *lon >= a->px1)) {
+        double fo = a->px0 + PYMOD_POS(*lon - a->px0, a->px1 - a->px0);
+        if (fo >= a->px1) fo = a->px0;
+        *lon = fo;
+        moved = 1;
+    }
+    if (a->periodic_lat && (*lat < a->py0 || *lat >= a->py1)) {
+        double fo = a->py0 + PYMOD_POS(*lat - a->py0, a->py1 - a->py0);
+        if (fo >= a->py1) fo = a->py0;
+        *lat = fo;
+        moved = 1;
+    }
+    if (moved) *nn = or_site_anchor(m, *lon, *lat, a->exact == 1 ? 1 : 2);
+}
+
 /* tracking.py:199 _apply_beaching_hook + beaching.py:95 evaluate_beaching
  * for one particle (prev = status at the step start). */
-static void beach_one(const or_mesh_t *m, const or_step_args_t *a, uint64_t pid, int32_t prev,
-                      double lon, double lat, int32_t *st, int32_t *nn, int8_t *fr, int32_t *ws)
+static void beach_one(const or_mesh_t *m, const or_fields_t *f, const or_step_args_t *a, uint64_t pid,
+                      int32_t prev, double lon, double lat, int32_t *st, int32_t *nn, int8_t *fr, int32_t *ws)
 {
     const int ALIVE = 1, BEACHED = 2, OUTSIDE = 3;
     int need_topo = a->use_coastal || a->use_stochastic;
@@ -1404,6 +1426,13 @@ static void beach_one(const or_mesh_t *m, const or_step_args_t *a, uint64_t pid,
         containing = or_find_containing_local(m, lon, lat, n);
     int coastal = need_topo && containing >= 0 && m->elem_coastal && m->elem_coastal[containing];
     double h_raw = n >= 0 ? m->depth[n] : INFINITY;
+    /* + vel.eta[nn] of the step's field: the bracket pair blended by alpha0
+     * (core/io.py:386 LazyLerpField), or the plain snapshot */
+    if (a->use_eta && n >= 0 && f->eta_a) {
+        double eta = f->eta_a[n];
+        if (f->alpha0 != 0.0 && f->eta_b) eta = (1.0 - f->alpha0) * f->eta_a[n] + f->alpha0 * f->eta_b[n];
+        h_raw = h_raw + eta;
+    }
     if (a->on_failure && newly_failed) {
         int context = h_raw < a->zone_depth || coastal;
         if (context) {
@@ -1467,8 +1496,10 @@ void or_step(const or_mesh_t *m, const or_fields_t *f, const or_step_args_t *a, 
                          &lon[i], &lat[i], &z[i], &status[i], &nn[i], &fr[i]);
             if (a->az && a->is3d && a->az_const > 0.0 && status[i] == ST_ALIVE)
                 az_one(m, a, (uint64_t)pid[i], nn[i], &z[i]);
+            if ((a->periodic_lon || a->periodic_lat) && status[i] == ST_ALIVE)
+                periodic_one(m, a, &lon[i], &lat[i], &nn[i]);
         }
-        if (a->beach) beach_one(m, a, (uint64_t)pid[i], prev, lon[i], lat[i], &status[i], &nn[i], &fr[i], &ws[i]);
+        if (a->beach) beach_one(m, f, a, (uint64_t)pid[i], prev, lon[i], lat[i], &status[i], &nn[i], &fr[i], &ws[i]);
     }
 }
 

@@ -49,6 +49,7 @@ typedef struct {
     int32_t dtype;
     double alpha0;
     int32_t uv_layout;             /* 0: u/v at nodes, 1: u/v at element centres */
+    const double *eta_a, *eta_b;   /* (N,) surface elevation of the bracket pair, or NULL */
 } or_fields_t;
 
 /* One step's configuration (the keyword arguments of step_fn,
@@ -74,6 +75,10 @@ typedef struct {
     /* vertical Az kick, constant source (integrators.py:842) */
     int32_t az;
     double az_const;
+    int32_t use_eta;   /* beaching water depth = depth + eta (beaching.py:121) */
+    /* periodic fold windows [x0, x1) / [y0, y1) (tracking.py:1089) */
+    int32_t periodic_lon, periodic_lat;
+    double px0, px1, py0, py1;
 } or_step_args_t;
 
 #endif

@@ -48,7 +48,9 @@ class OrStepArgs(ctypes.Structure):
         ("depth_min", c_double), ("zone_depth", c_double), ("t_beach", c_double),
         ("rewet_hysteresis", c_double), ("t_resuspend", c_double),
         ("beach_seed", ctypes.c_uint64), ("beach_dt", c_double),
-        ("az", c_int32), ("az_const", c_double),
+        ("az", c_int32), ("az_const", c_double), ("use_eta", c_int32),
+        ("periodic_lon", c_int32), ("periodic_lat", c_int32),
+        ("px0", c_double), ("px1", c_double), ("py0", c_double), ("py1", c_double),
     ]
 
 
@@ -56,13 +58,18 @@ RESUSPENSION_CODES = {"never": 0, "auto": 1, "stochastic": 2, "auto_or_stochasti
 
 
 def step_args(order, dt, time_dir, is3d=False, w_method=2, exact=True, alphas=(0, 0, 0, 0),
-              kick=None, beach=None, step_i=0, real_id=0) -> OrStepArgs:
+              kick=None, beach=None, step_i=0, real_id=0, periodic_lon=None,
+              periodic_lat=None) -> OrStepArgs:
     a = OrStepArgs()
     a.order, a.is3d, a.w_method, a.exact = int(order), int(is3d), int(w_method), int(exact)
     a.dt, a.time_dir = float(dt), float(time_dir)
     for k in range(4):
         a.alphas[k] = float(alphas[k]) if k < len(alphas) else 0.0
     a.step_i, a.real_id = int(step_i), int(real_id)
+    if periodic_lon is not None:
+        a.periodic_lon, a.px0, a.px1 = 1, float(periodic_lon[0]), float(periodic_lon[1])
+    if periodic_lat is not None:
+        a.periodic_lat, a.py0, a.py1 = 1, float(periodic_lat[0]), float(periodic_lat[1])
     if kick:
         a.kick = 1
         a.kh_x, a.kh_y, a.seed = float(kick["kh_x"]), float(kick["kh_y"]), int(kick["seed"])
@@ -85,6 +92,7 @@ def step_args(order, dt, time_dir, is3d=False, w_method=2, exact=True, alphas=(0
         a.t_resuspend = float(beach.get("t_resuspend") or 0.0)
         a.beach_dt = float(beach.get("dt", dt))
         a.beach_seed = int(beach.get("seed", a.seed))
+        a.use_eta = int(bool(beach.get("use_eta", False)))
     return a
 
 
@@ -94,6 +102,7 @@ class OrFields(ctypes.Structure):
         ("u_t", c_void_p), ("v_t", c_void_p),
         ("w_a", c_void_p), ("w_b", c_void_p), ("w_t", c_void_p),
         ("dtype", c_int32), ("alpha0", c_double), ("uv_layout", c_int32),
+        ("eta_a", c_void_p), ("eta_b", c_void_p),
     ]
 
 
@@ -262,9 +271,13 @@ class OracleMesh:
                 x = fields.get(f"{nm}_{sfx}")
                 keep[f"{nm}_{sfx}"] = keep[nm + "_a"] if x is None else _c(x, npd)
         ptr = lambda k: keep[k].ctypes.data if k in keep else None
+        for nm in ("eta_a", "eta_b"):
+            if fields.get(nm) is not None:
+                keep[nm] = _c(fields[nm], np.float64)
         f = OrFields(ptr("u_a"), ptr("v_a"), ptr("u_b"), ptr("v_b"), ptr("u_t"), ptr("v_t"),
                      ptr("w_a"), ptr("w_b"), ptr("w_t"), dt_code, float(fields.get("alpha0", 0.0)),
-                     1 if fields.get("uv_layout", "node") == "element" else 0)
+                     1 if fields.get("uv_layout", "node") == "element" else 0,
+                     ptr("eta_a"), ptr("eta_b"))
         P = len(state["lon"])
         out = dict(lon=np.array(state["lon"], np.float64), lat=np.array(state["lat"], np.float64),
                    z=np.array(state.get("z", np.zeros(P)), np.float64),

@@ -216,3 +216,97 @@ def run_elem_case(name, direction, stepper):
                        ("failure_reason", "fr"), ("u", "u"), ("v", "v"), ("w", "w")):
             rows[k].append(np.array(st[src], copy=True))
     return {k: np.stack(v) for k, v in rows.items()}
+
+
+# ---------------------------------------------------------------------------
+# run-level cases: run_single_realization on an in-memory FESOM2 node dataset
+# (tests/_gen_golden_3d.py:368-470)
+# ---------------------------------------------------------------------------
+
+RUN_LEVELS = np.array([0.0, 10.0, 25.0, 50.0, 100.0])
+RUN_FILE_DT = 3600.0
+RUN_DT = 1800.0
+
+
+def run_mesh(depth_fn):
+    """The 5 x 5 lattice of the element cases with levels 0-100 m and the
+    given bathymetry."""
+    from kelp_amd.mesh import make_mesh
+    xs = np.linspace(4.0, 6.0, 5)
+    gx, gy = np.meshgrid(xs, xs)
+    lon, lat = gx.ravel(), gy.ravel()
+    elems = []
+    for j in range(4):
+        for i in range(4):
+            n0 = j * 5 + i
+            elems += [[n0, n0 + 1, n0 + 5], [n0 + 1, n0 + 6, n0 + 5]]
+    en = np.array(elems, dtype=np.int32)
+    mesh = make_mesh("test_local", lon, lat, en, depth=depth_fn(lon, lat), z_levels=RUN_LEVELS)
+    mesh.elem_center_lon = np.array([lon[e].mean() for e in en])
+    mesh.elem_center_lat = np.array([lat[e].mean() for e in en])
+    return mesh
+
+
+def fesom2_node_ds(mesh, n_time, eta, scalars=False):
+    """FESOM2-shaped node dataset: unod/vnod (time, nz1, nod2), w (time, nz,
+    nod2; file convention positive up), ssh (time, nod2)."""
+    from kelp_amd.readers import ArrayDataset
+    K, N = mesh.sigma.size, mesh.n_nodes
+    t = np.arange(n_time, dtype=np.float64)[:, None, None]
+    k = np.arange(K - 1, dtype=np.float64)[None, :, None]
+    x = (mesh.nodes_lon - 5.0)[None, None, :]
+    y = (mesh.nodes_lat - 5.0)[None, None, :]
+    zk = (mesh.sigma / mesh.sigma[-1])[None, :, None]
+    unod = (0.25 - 0.03 * k) * (1.0 + 0.3 * x) + 0.05 * y + 0.02 * np.sin(t)
+    vnod = 0.1 * (1.0 + 0.2 * k) * np.cos(2.0 * x) - 0.03 * y * (1.0 + 0.1 * t)
+    w = 1e-3 * np.sin(np.pi * zk) * (1.0 + 0.4 * x) * np.cos(0.5 * t)
+    shp_u, shp_w = (n_time, K - 1, N), (n_time, K, N)
+    c = lambda a: np.ascontiguousarray(np.array(a, dtype=np.float64))
+    dvars = {
+        "unod": (("time", "nz1", "nod2"), c(np.broadcast_to(unod, shp_u))),
+        "vnod": (("time", "nz1", "nod2"), c(np.broadcast_to(vnod, shp_u))),
+        "w": (("time", "nz", "nod2"), c(np.broadcast_to(w, shp_w))),
+        "ssh": (("time", "nod2"), c(np.broadcast_to(np.asarray(eta)[:, None], (n_time, N)))),
+    }
+    if scalars:
+        z_ifc = mesh.sigma[None, :, None]
+        z_mid = (0.5 * (mesh.sigma[:-1] + mesh.sigma[1:]))[None, :, None]
+        s_lev = 10.0 + 0.1 * z_ifc + 2.0 * x - y + 0.5 * t
+        s_mid = 35.0 - 0.02 * z_mid + x * y + 0.1 * t
+        dvars["s_lev"] = (("time", "nz", "nod2"), c(np.broadcast_to(s_lev, shp_w)))
+        dvars["s_mid"] = (("time", "nz1", "nod2"), c(np.broadcast_to(s_mid, shp_u)))
+    return ArrayDataset(dvars, time=np.arange(n_time) * RUN_FILE_DT)
+
+
+def beaching_case(direction):
+    """tests/_gen_golden_3d.py:433: a shallow strip falls dry on a falling
+    tide and rewets; depth + stochastic beaching with use_eta,
+    auto-or-stochastic resuspension, node RK2 with a time blend."""
+    from kelp_amd.config import BeachingConfig, ModeConfig, NumericsConfig
+    from kelp_amd.particles import make_state
+
+    def depth(lon, lat):
+        d = np.full(lon.shape, 90.0)
+        d[(lon == 5.5) & (lat >= 4.5) & (lat <= 5.5)] = 1.0
+        return d
+    mesh = run_mesh(depth)
+    eta = [0.0, -0.45, -0.85, -0.85, -0.45, 0.0, 0.3, 0.0]
+    ds = fesom2_node_ds(mesh, len(eta), eta)
+    lon = np.concatenate([np.repeat([5.22, 5.3, 5.45, 5.6], 3),
+                          np.repeat([4.3, 4.7], 4), [5.97, 5.985] * 2])
+    lat = np.concatenate([np.tile([4.6, 5.0, 5.4], 4),
+                          np.tile([4.3, 4.8, 5.2, 5.7], 2), [4.4, 4.4, 5.6, 5.6]])
+    z = np.concatenate([np.tile([0.2, 0.5, 0.8], 4),
+                        np.tile([5.0, 30.0, 60.0, 85.0], 2), [20.0] * 4])
+    beach = BeachingConfig(enabled=True, criteria=["depth", "stochastic"], depth_min=0.3,
+                           use_eta=True, t_beach=10800.0, resuspension="auto_or_stochastic",
+                           t_resuspend=7200.0, min_wet_steps=1, seed=4242)
+    numerics = NumericsConfig(dt=RUN_DT, integrator="rk2", uv_interp="barycentric",
+                              w_interp="barycentric", temporal_interp=True, beaching=beach)
+    mode = ModeConfig(Fdrift2D=False, Fdrift3D=True, time_dir=direction, var_u="unod",
+                      var_v="vnod", var_w="w", var_eta="ssh")
+    n_time = len(eta)
+    return dict(mesh=mesh, ds=ds, init=make_state(lon, lat, z), mode=mode, numerics=numerics,
+                start_time_idx=0 if direction > 0 else n_time - 1, n_steps=14,
+                n_time_total=n_time, file_times=np.arange(n_time) * RUN_FILE_DT,
+                file_dt=RUN_FILE_DT)

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from kelp_amd import synth
-from oracle.oracle import OracleMesh, math_probe
+from oracle.oracle import OracleMesh, math_probe, step_args
 
 from .helpers import global_latlon_mesh
 
@@ -171,3 +171,29 @@ def test_global_wrap_mesh(gpu_ctx):
         assert np.array_equal(h[k].view(np.int64), st[k].view(np.int64)), k
     assert np.array_equal(h["status"], st["status"])
     assert np.array_equal(h["nn"], st["nn"])
+
+
+def test_periodic_fold_bitwise(gpu_ctx, gyre):
+    """The post-step fold into a periodic window (tracking.py:1089) on the
+    device equals the oracle's, with the re-anchored nearest node."""
+    mesh = gyre.mesh
+    gpu_ctx.upload_mesh(mesh)
+    u = (np.full(mesh.n_nodes, 2.0, np.float32) + gyre.u).astype(np.float32)
+    gpu_ctx.upload_field(0, u, gyre.v)
+    om = OracleMesh(mesh, "kmath")
+    lon, lat = gyre.lon[:2000], gyre.lat[:2000]
+    st = dict(lon=lon, lat=lat, z=np.zeros(lon.size), status=np.ones(lon.size, np.int32),
+              nn=np.full(lon.size, -1, np.int32), pid=np.arange(lon.size))
+    p = gpu_ctx.particles(lon, lat)
+    f = dict(u_a=u, v_a=gyre.v, alpha0=0.0)
+    for k in range(30):
+        st = om.step(st, f, step_args(2, 3600.0, 1.0, exact=1, periodic_lon=(5.0, 35.0),
+                                      periodic_lat=(32.0, 58.0)))
+        gpu_ctx.step(p, 2, 0, 3600.0, 1.0, exact_nearest=1, periodic_lon=(5.0, 35.0),
+                     periodic_lat=(32.0, 58.0))
+    h = p.to_host()
+    assert np.all((h["lon"][h["status"] == 1] >= 5.0) & (h["lon"][h["status"] == 1] < 35.0))
+    for k in ("lon", "lat", "u", "v"):
+        assert np.array_equal(h[k].view(np.int64), st[k].view(np.int64)), k
+    for k in ("status", "nn", "fr"):
+        assert np.array_equal(h[k], st[k]), k

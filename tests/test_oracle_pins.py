@@ -327,3 +327,25 @@ def test_stochastic_beaching_probability(meshes):
     beach = dict(criteria=["depth"], depth_min=0.3)
     out = om.step(st, f, step_args(1, dt, 1.0, is3d=False, exact=1, beach=beach))
     assert np.all(out["status"] == 1)
+
+
+def test_periodic_fold_is_python_modulo(meshes):
+    """tracking.py:1089: a particle leaving the window [x0, x1) comes back
+    at x0 + (x - x0) % (x1 - x0); in-window particles are untouched."""
+    mesh = meshes["lattice"]
+    om = OracleMesh(mesh)
+    lon = np.linspace(8.6, 8.99, 30)
+    lat = np.linspace(31, 39, 30)
+    st = dict(lon=lon, lat=lat, z=np.zeros(30), status=np.ones(30, np.int32),
+              nn=np.full(30, -1, np.int32), pid=np.arange(30))
+    u = np.full(mesh.n_nodes, 3.0)
+    v = np.zeros(mesh.n_nodes)
+    f = dict(u_a=u, v_a=v, alpha0=0.0)
+    free = om.step(st, f, step_args(1, 3600.0, 1.0, exact=1))
+    wrapped = om.step(st, f, step_args(1, 3600.0, 1.0, exact=1, periodic_lon=(1.0, 9.0)))
+    out = free["lon"] >= 9.0
+    assert out.any() and (~out).any()
+    want = np.where(out, [1.0 + (float(x) - 1.0) % 8.0 for x in free["lon"]], free["lon"])
+    np.testing.assert_array_equal(wrapped["lon"], want)
+    # the folded particles carry the nearest node of their new position
+    np.testing.assert_array_equal(wrapped["nn"][out], om.nearest(wrapped["lon"][out], wrapped["lat"][out]))

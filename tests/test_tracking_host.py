@@ -1,0 +1,97 @@
+"""Host logic of the tracking loop and the config (CPU): time brackets,
+payloads, record masks, YAML loading."""
+import os
+
+import numpy as np
+import pytest
+
+from kelp_amd import config as C
+from kelp_amd import tracking as T
+from kelp_amd.particles import ParticleStatus
+
+FT = np.arange(8) * 3600.0
+
+
+def test_bracket_inside_and_on_records():
+    # strictly inside: direction-independent
+    assert T.resolve_time_bracket(5400.0, FT, 8, "stop", 1) == (1, 2, 3600.0, 7200.0, 5400.0)
+    assert T.resolve_time_bracket(5400.0, FT, 8, "stop", -1) == (1, 2, 3600.0, 7200.0, 5400.0)
+    # on a record: forward moves into [t_k, t_k+1], backward into [t_k-1, t_k]
+    assert T.resolve_time_bracket(7200.0, FT, 8, "stop", 1)[:2] == (2, 3)
+    b = T.resolve_time_bracket(7200.0, FT, 8, "stop", -1)
+    assert b[:2] == (1, 2) and b[4] == 7200.0
+    # ends: forward at the last record degenerates, backward at the first
+    assert T.resolve_time_bracket(FT[-1], FT, 8, "stop", 1)[:2] == (7, 7)
+    assert T.resolve_time_bracket(0.0, FT, 8, "stop", -1)[:2] == (0, 0)
+    assert T.resolve_time_bracket(FT[-1] + 10.0, FT, 8, "stop", 1) is None
+
+
+def test_bracket_snap_band_and_cycle():
+    band = T.snapshot_band(1800.0)
+    # a query within the band below a record starts on it (forward)
+    b = T.resolve_time_bracket(7200.0 - 0.5 * band, FT, 8, "stop", 1, band)
+    assert b[:2] == (2, 3) and b[4] == 7200.0
+    # cycle: the period is the span plus the trailing interval; the last
+    # bracket runs from the last record to the first of the next pass
+    assert T.wrap_sim_time(8 * 3600.0 + 100.0, FT, 8, "cycle") == 100.0
+    b = T.resolve_time_bracket(7 * 3600.0 + 1800.0, FT, 8, "cycle", 1)
+    assert b[:2] == (7, 0) and b[3] - b[2] == 3600.0
+
+
+def test_payload_endpoints_and_direction():
+    p = T.vel_payload((1, 2, 3600.0, 7200.0, 5400.0), 1)
+    assert (p.ia, p.ib, p.alpha0, p.it, p.dt_to_next) == (1, 2, 0.5, 2, 1800.0)
+    p = T.vel_payload((1, 2, 3600.0, 7200.0, 5400.0), -1)
+    assert (p.it, p.dt_to_next) == (1, 1800.0)
+    p = T.vel_payload((1, 2, 3600.0, 7200.0, 3600.0), 1)      # alpha 0: plain snapshot a
+    assert (p.ia, p.ib, p.alpha0, p.it) == (1, None, 0.0, 2)
+    p = T.vel_payload((1, 2, 3600.0, 7200.0, 7200.0), -1)     # backward on top: plain b
+    assert (p.ia, p.ib, p.it, p.dt_to_next) == (2, None, 1, 3600.0)
+    p = T.vel_payload((3, 3, 0.0, 0.0, 0.0), 1)
+    assert (p.ia, p.it, p.dt_to_next) == (3, None, None)
+
+
+def test_record_masks():
+    A, B, O, N = (int(s) for s in (ParticleStatus.ALIVE, ParticleStatus.BEACHED,
+                                   ParticleStatus.OUTSIDE, ParticleStatus.NOT_RELEASED))
+    h = dict(lon=np.ones(5), lat=np.ones(5), z=np.ones(5), u=np.full(5, 2.0), v=np.full(5, 3.0),
+             w=np.full(5, 4.0), status=np.array([A, A, B, O, N]))
+    prev = np.array([A, B, A, A, N])
+    m = T.mask_record(h, prev, -1, True)
+    # velocity defined only where alive before and after; flipped backward
+    np.testing.assert_array_equal(m["u"], [-2.0, np.nan, np.nan, np.nan, np.nan])
+    # dormant particles have NaN positions
+    assert np.isnan(m["lon"][4]) and not np.isnan(m["lon"][:4]).any()
+
+
+def test_activate_released():
+    st = np.array([4, 4, 1, 3], np.int32)
+    out = T.activate_released(st, np.array([2, 5, 0, 0]), 3)
+    np.testing.assert_array_equal(out, [1, 4, 1, 3])
+
+
+def test_load_reference_style_config(tmp_path):
+    y = tmp_path / "run.yaml"
+    y.write_text("""
+mode: {Fdrift2D: false, Fdrift3D: true, time_dir: -1.0, unknown_key: 1}
+numerics:
+  dt: 1800
+  integrator: rk4
+  uv_interp: barycentric
+  w_interp: barycentric
+  temporal_interp: true
+  periodic_lon: [0.0, 10.0]
+  diffusion: {enabled: true, kh_zonal: 10.0, kh_meridional: 10.0, seed: 7}
+  beaching: {enabled: true, criteria: [on_failure, stochastic], t_beach: 86400}
+mesh: {type: fesom2, meshpath: mesh.nc}
+data: {inicloud: seeds.dat}
+time: {n_steps: 336, on_exhausted: cycle}
+""")
+    cfg = C.load_config(str(y))
+    assert cfg.mode.Fdrift3D and cfg.mode.time_dir == -1.0
+    assert cfg.numerics.integrator == "rk4" and cfg.numerics.periodic_lon == (0.0, 10.0)
+    assert cfg.numerics.diffusion.kh_zonal == 10.0 and cfg.numerics.beaching.enabled
+    assert cfg.mesh.meshpath == os.path.join(str(tmp_path), "mesh.nc")
+    assert C.resolve_time_params(cfg.time) == (0, 336)
+    with pytest.raises(ValueError, match="missing section"):
+        C.config_from_dict({"mode": {}})
