@@ -145,6 +145,18 @@ int kelp_field_upload_elem(kelp_ctx *ctx, int slot, int dtype, int64_t n_planes,
 int kelp_field_upload_eta(kelp_ctx *ctx, int slot, const double *host_eta);
 int kelp_field_clear(kelp_ctx *ctx, int slot);
 
+/* Node scalar `var` (0..7) of a snapshot slot for record-point sampling
+ * (VelocityField.scalars, numerics/sampling.py:727): n_planes x N doubles in
+ * mesh node order, n_planes 1 (2-D), K-1 (mid-layer) or K (interface). */
+int kelp_scalar_upload(kelp_ctx *ctx, int slot, int var, int64_t n_planes, const double *host_values);
+
+/* Sample scalar `var` at every particle (sample_field): the field of slot_a,
+ * or slot_a blended towards slot_b by alpha; method 0 nearest, 1 leastsq,
+ * 2 barycentric, 3 idw.  out: device doubles (n), NaN where the particle is
+ * not alive or has no anchor.  Particle order as in `p`. */
+int kelp_sample(kelp_ctx *ctx, const kelp_particles *p, int var, int slot_a, int slot_b, double alpha,
+                int method, double *out);
+
 /* Advance every particle by one step, in place, on the context stream. */
 int kelp_step(kelp_ctx *ctx, const kelp_particles *p, const kelp_step_args *a);
 

@@ -84,6 +84,9 @@ SIGNATURES = {
                                        c_void_p, c_int64]),
     "kelp_field_upload_eta": (c_int, [c_void_p, c_int, c_void_p]),
     "kelp_field_clear": (c_int, [c_void_p, c_int]),
+    "kelp_scalar_upload": (c_int, [c_void_p, c_int, c_int, c_int64, c_void_p]),
+    "kelp_sample": (c_int, [c_void_p, POINTER(Particles), c_int, c_int, c_int, c_double, c_int,
+                            c_void_p]),
     "kelp_step": (c_int, [c_void_p, POINTER(Particles), POINTER(StepArgs)]),
     "kelp_sort_particles": (c_int, [c_void_p, POINTER(Particles)]),
     "kelp_nearest": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_int, c_void_p]),

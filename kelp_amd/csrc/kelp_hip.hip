@@ -70,11 +70,19 @@ static void free_mesh(kelp_ctx* c)
     c->node_old_of_new = c->node_new_of_old = c->elem_old_of_new = c->elem_new_of_old = nullptr;
 }
 
+void kelp_free_scalar(ScalarField& f)
+{
+    if (f.planes) (void)hipFree(f.planes);
+    if (f.wrec) (void)hipFree(f.wrec);
+    f = ScalarField{};
+}
+
 static void free_slot(FieldSlot& s)
 {
     if (s.rec) (void)hipFree(s.rec);
     if (s.erec) (void)hipFree(s.erec);
     if (s.eta) (void)hipFree(s.eta);
+    for (auto& f : s.scalars) kelp_free_scalar(f);
     s = FieldSlot{};
 }
 

@@ -23,6 +23,17 @@ int kelp_fail(int code, const char* fmt, ...);
 
 constexpr int kMaxSlots = 16;
 constexpr int kMaxEvents = 16;
+constexpr int kMaxScalars = 8;
+
+// A node scalar of one snapshot for record-point sampling: n_planes planes
+// of N doubles in internal node order (1: 2-D, K-1: mid-layer, K:
+// interface); an interface field also as (0, 0, top, bottom) records per
+// (layer, node), the layout the W interpolators read.
+struct ScalarField {
+    int64_t n_planes = 0;
+    double* planes = nullptr;
+    void* wrec = nullptr;
+};
 
 // One snapshot in HBM as packed records (see kelp_device.cuh, Snap): 3-D
 // (u, v, w_top, w_bottom) per (layer, node), 2-D (u, v) per node.  With u/v
@@ -36,6 +47,7 @@ struct FieldSlot {
     void* rec = nullptr;
     void* erec = nullptr;
     double* eta = nullptr;  // (N,) surface elevation, internal node order, or null
+    ScalarField scalars[kMaxScalars];
 };
 
 // growable device buffer owned by a context
@@ -59,6 +71,8 @@ struct kelp_ctx {
     FieldSlot slots[kMaxSlots];
     DevBuf sort_tmp, sort_keys, sort_keys2, sort_idx, sort_perm, sort_stage;
 };
+
+void kelp_free_scalar(ScalarField& f);
 
 // (re)allocate `b` to at least `bytes` (contents not preserved)
 int kelp_reserve(DevBuf& b, size_t bytes);

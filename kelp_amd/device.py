@@ -197,6 +197,29 @@ class DeviceContext:
             raise ValueError(f"eta of shape {eta.shape} does not match the mesh's {self.mesh.n_nodes} nodes")
         check(self.lib.kelp_field_upload_eta(self.h, int(slot), eta.ctypes.data), "kelp_field_upload_eta")
 
+    def upload_scalar(self, slot: int, var: int, values) -> None:
+        """Node scalar ``var`` of the snapshot in ``slot`` for sampling:
+        (N,), (K-1, N) or (K, N) doubles in mesh node order."""
+        values = np.ascontiguousarray(values, np.float64)
+        if values.shape[-1] != self.mesh.n_nodes:
+            raise ValueError(f"scalar of shape {values.shape} does not match the mesh's nodes")
+        n_planes = 1 if values.ndim == 1 else values.shape[0]
+        check(self.lib.kelp_scalar_upload(self.h, int(slot), int(var), n_planes, values.ctypes.data),
+              "kelp_scalar_upload")
+
+    def sample(self, p: "DeviceParticles", var: int, slot_a: int, slot_b: int = -1,
+               alpha: float = 0.0, method: str = "leastsq") -> np.ndarray:
+        """Sampled scalar ``var`` at every particle (host array, particle
+        order of ``p``)."""
+        codes = {"nearest": 0, "leastsq": 1, "barycentric": 2, "idw": 3}
+        t = self.torch
+        out = t.empty((p.n,), dtype=t.float64, device=p.lon.device)
+        s = p.as_struct()
+        check(self.lib.kelp_sample(self.h, ctypes.byref(s), int(var), int(slot_a), int(slot_b),
+                                   float(alpha), codes[method], out.data_ptr()), "kelp_sample")
+        self.synchronize()
+        return out.cpu().numpy()
+
     def upload_field(self, slot: int, u, v, w=None, key=None, uv_layout="node") -> None:
         """Upload one snapshot into ``slot``; skipped when ``key`` matches the
         slot's current content.  ``uv_layout``: "node" (u, v per node) or

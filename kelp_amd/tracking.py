@@ -28,6 +28,7 @@ from .particles import ParticleState, ParticleStatus
 
 _C = {1: (0.0,), 2: (0.0, 0.5), 4: (0.0, 0.5, 0.5, 1.0)}
 N_SLOTS = 16
+SAMPLE_METHODS = {"nearest": 0, "leastsq": 1, "barycentric": 2, "idw": 3}
 
 
 def snapshot_band(dt) -> float:
@@ -187,6 +188,8 @@ class _SnapshotSlots:
         self.ctx.upload_field(slot, vf.u, vf.v, vf.w, uv_layout=vf.uv_layout)
         if self.need_eta:
             self.ctx.upload_eta(slot, vf.eta if vf.eta is not None else np.zeros(self.mesh.n_nodes))
+        for k, name in enumerate(self.sample_vars or ()):
+            self.ctx.upload_scalar(slot, k, vf.scalars[name])
         self.map[idx] = slot
         self.order.append(idx)
         return slot
@@ -283,8 +286,9 @@ def run_single_realization(mesh: Mesh, ds, reader, init_state: ParticleState, mo
     records, ``(particle, record)``: lon, lat, z, status, failure_reason,
     vel_u/vel_v/vel_w (NaN where undefined), time."""
     from .device import DeviceContext
-    if sample_vars:
-        raise NotImplementedError("device path: record-point sampling")
+    sample_vars = list(sample_vars or [])
+    if sample_vars and numerics.sample_interp not in SAMPLE_METHODS:
+        raise ValueError(f"Unknown sample_interp {numerics.sample_interp!r}")
     direction = 1 if mode.time_dir >= 0 else -1
     order = ORDERS.get(numerics.integrator)
     if file_times is None:
@@ -318,7 +322,7 @@ def run_single_realization(mesh: Mesh, ds, reader, init_state: ParticleState, mo
     ctx = ctx or DeviceContext(0)
     if ctx.mesh is not mesh:
         ctx.upload_mesh(mesh)
-    slots = _SnapshotSlots(ctx, reader, ds, mesh, mode, numerics, None,
+    slots = _SnapshotSlots(ctx, reader, ds, mesh, mode, numerics, sample_vars,
                            need_eta=bool(beach and beach["use_eta"]))
     P = init_state.n_particles
     pid = particle_offset + np.arange(P, dtype=np.int64)
@@ -399,15 +403,38 @@ def run_single_realization(mesh: Mesh, ds, reader, init_state: ParticleState, mo
         if use_tinterp:
             sim_time = sim_time_end if sim_time_end is not None else sim_time + dt * direction
         if (g + 1) % max(int(numerics.output_freq), 1) == 0:
+            sampled = {}
+            if sample_vars:
+                # the scalars at the record's instant (tracking.py:2692-2743)
+                ra, rb, ral = sa, sb, pay.alpha0
+                if use_tinterp:
+                    rbr = resolve_time_bracket(sim_time, file_times, n_time_total, on_exhausted,
+                                               direction, snap)
+                    if rbr is not None:
+                        rp = vel_payload(rbr, direction)
+                        ra = slots.slot(rp.ia)
+                        rb = slots.slot(rp.ib) if rp.ib is not None else -1
+                        ral = rp.alpha0
+                else:
+                    nxt = file_idx + direction
+                    if on_exhausted == "cycle":
+                        nxt %= n_time_total
+                    if 0 <= nxt < n_time_total:
+                        ra, rb, ral = slots.slot(nxt), -1, 0.0
+                for k, name in enumerate(sample_vars):
+                    sampled["sample__" + name] = ctx.sample(p, k, ra, rb, ral,
+                                                            numerics.sample_interp)
             h = p.to_host()
             prev = prev_status.cpu().numpy()
             order_idx = np.argsort(h["pid"], kind="stable")
             h = {k: v[order_idx] for k, v in h.items()}
             prev = prev[order_idx]
             m = mask_record(h, prev, direction, is3d)
-            rec.append(dict(lon=m["lon"], lat=m["lat"], z=m["z"], status=m["status"],
-                            failure_reason=m["fr"], vel_u=m["u"], vel_v=m["v"], vel_w=m["w"]),
-                       t_g0 + (g + 1) * dt * direction)
+            row = dict(lon=m["lon"], lat=m["lat"], z=m["z"], status=m["status"],
+                       failure_reason=m["fr"], vel_u=m["u"], vel_v=m["v"], vel_w=m["w"])
+            for k, v in sampled.items():
+                row[k] = v[order_idx]
+            rec.append(row, t_g0 + (g + 1) * dt * direction)
     return rec
 
 

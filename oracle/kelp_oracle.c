@@ -1579,3 +1579,257 @@ void or_math_probe(int fn, int64_t n, const double *x, const double *y, double *
         }
     }
 }
+
+/* ------------------------------------------------------------------------
+ * Record-point sampling of node scalars (numerics/sampling.py)
+ * ---------------------------------------------------------------------- */
+
+/* A node scalar of one record: planes of N values (1: 2-D, K-1: mid-layer,
+ * K: interface), snapshot a blended towards b by alpha (core/io.py:68
+ * _blend on the LazyLerpField scalars; alpha 0 = the plain snapshot). */
+typedef struct {
+    const double *a, *b;
+    double alpha;
+    int n_planes;
+} or_scalar_t;
+
+static inline double sval(const or_scalar_t *s, int64_t idx)
+{
+    if (s->alpha == 0.0) return s->a[idx];
+    return (1.0 - s->alpha) * s->a[idx] + s->alpha * s->b[idx];
+}
+
+/* sampling.py:433 _interp_mid_linear_single at one node */
+static double mid_linear(const or_mesh_t *m, const or_scalar_t *s, double z, int node)
+{
+    const double *zl = m->z_levels;
+    int K1 = m->n_levels - 1;
+    int64_t N = m->n_nodes;
+    if (K1 <= 0) return 0.0;
+    if (K1 == 1) return sval(s, node);
+    double zm0 = 0.5 * (zl[0] + zl[1]);
+    double zml = 0.5 * (zl[K1 - 1] + zl[K1]);
+    if (z <= zm0) return sval(s, node);
+    if (z >= zml) return sval(s, (int64_t)(K1 - 1) * N + node);
+    for (int k = 0; k < K1 - 1; k++) {
+        double zk = 0.5 * (zl[k] + zl[k + 1]);
+        double zk1 = 0.5 * (zl[k + 1] + zl[k + 2]);
+        if (z < zk1) {
+            double dz = zk1 - zk;
+            if (dz < 1e-12) return sval(s, (int64_t)k * N + node);
+            double al = (z - zk) / dz;
+            return (1.0 - al) * sval(s, (int64_t)k * N + node) + al * sval(s, (int64_t)(k + 1) * N + node);
+        }
+    }
+    return sval(s, (int64_t)(K1 - 1) * N + node);
+}
+
+/* value of the field at node nd for the horizontal combination: 2-D the
+ * node value, mid-layer linear in depth at min(z, depth[nd])
+ * (sampling.py:517-520) */
+static inline double node_value(const or_mesh_t *m, const or_scalar_t *s, double z, int nd)
+{
+    if (s->n_planes == 1) return sval(s, nd);
+    double ze = z < m->depth[nd] ? z : m->depth[nd];
+    return mid_linear(m, s, ze, nd);
+}
+
+static int collect_neighbours(const or_mesh_t *m, int node, int *buf)
+{
+    int count = 0;
+    buf[count++] = node;
+    int n_el = m->n2e_count[node];
+    const int32_t *row = m->n2e + (int64_t)node * m->max_conn;
+    for (int k = 0; k < n_el; k++) {
+        int e = row[k];
+        if (e < 0) continue;
+        for (int v = 0; v < m->max_v; v++) {
+            int nd = m->elem_nodes[(int64_t)e * m->max_v + v];
+            if (nd < 0) continue;
+            int found = 0;
+            for (int j = 0; j < count; j++)
+                if (buf[j] == nd) {
+                    found = 1;
+                    break;
+                }
+            if (!found && count < 64) buf[count++] = nd;
+        }
+    }
+    return count;
+}
+
+/* sampling.py:83 _combine_horizontal (0 nearest, 1 leastsq, 2 barycentric,
+ * 3 idw) over the 1-ring values */
+static double combine_horizontal(const or_mesh_t *m, const or_scalar_t *s, double z, const int *neigh,
+                                 int n_neigh, const double *vals, double lon_p, double lat_p, int nn,
+                                 int method)
+{
+    const double DEG2R = 0.017453292519943295;
+    double fallback = vals[0];
+    if (method == 0) return fallback;
+    if (method == 3) {
+        double z0 = M_COS(lat_p * DEG2R) * SLM;
+        if (z0 < 1.0) z0 = 1.0;
+        double sw = 0.0, swv = 0.0;
+        for (int i = 0; i < n_neigh; i++) {
+            int nd = neigh[i];
+            double dlon = PYMOD360(m->nodes_lon[nd] - lon_p + 180.0) - 180.0;
+            double dx = dlon * z0;
+            double dy = (m->nodes_lat[nd] - lat_p) * SLM;
+            double d2 = dx * dx + dy * dy;
+            if (d2 < 1e-12) return vals[i];
+            if (!isfinite(vals[i])) continue;
+            double w = 1.0 / d2;
+            sw += w;
+            swv += w * vals[i];
+        }
+        if (sw < 1e-30) return fallback;
+        return swv / sw;
+    }
+    if (method == 2) {
+        const double R = 6371000.0;
+        int n_el = m->n2e_count[nn];
+        const int32_t *row = m->n2e + (int64_t)nn * m->max_conn;
+        for (int k = 0; k < n_el; k++) {
+            int e = row[k];
+            if (e < 0) continue;
+            int verts[4], nv = 0;
+            for (int v = 0; v < m->max_v; v++) {
+                int nd = m->elem_nodes[(int64_t)e * m->max_v + v];
+                if (nd < 0) continue;
+                if (nv > 0 && nd == verts[0]) continue;
+                verts[nv++] = nd;
+            }
+            if (nv < 3) continue;
+            double lon0 = seam_safe_lon0(m, verts, nv);
+            double lat0 = 0.0;
+            for (int v = 0; v < nv; v++) lat0 += m->nodes_lat[verts[v]];
+            lat0 /= nv;
+            double cos_lat0 = M_COS(lat0 * DEG2R);
+            double vx[4], vy[4];
+            for (int v = 0; v < nv; v++) {
+                double dlon = PYMOD360(m->nodes_lon[verts[v]] - lon0 + 180.0) - 180.0;
+                double dlat = m->nodes_lat[verts[v]] - lat0;
+                vx[v] = dlon * DEG2R * R * cos_lat0;
+                vy[v] = dlat * DEG2R * R;
+            }
+            double dlon_p = PYMOD360(lon_p - lon0 + 180.0) - 180.0;
+            double px = dlon_p * DEG2R * R * cos_lat0;
+            double py = (lat_p - lat0) * DEG2R * R;
+            for (int t = 0; t < nv - 2; t++) {
+                int i1 = t + 1, i2 = t + 2;
+                double ax = vx[0], ay = vy[0], bx = vx[i1], by = vy[i1], cx = vx[i2], cy = vy[i2];
+                double denom = (by - cy) * (ax - cx) + (cx - bx) * (ay - cy);
+                if (fabs(denom) < 1e-14) continue;
+                double w1 = ((by - cy) * (px - cx) + (cx - bx) * (py - cy)) / denom;
+                double w2 = ((cy - ay) * (px - cx) + (ax - cx) * (py - cy)) / denom;
+                double w3 = 1.0 - w1 - w2;
+                const double eps = -1e-6;
+                if (w1 >= eps && w2 >= eps && w3 >= eps) {
+                    if (nv == 4) {
+                        double wts[4];
+                        wachspress_quad(px, py, vx, vy, wts);
+                        double acc = 0.0;
+                        for (int vi = 0; vi < 4; vi++) {
+                            int j;
+                            for (j = 0; j < n_neigh; j++)
+                                if (neigh[j] == verts[vi]) break;
+                            if (j == n_neigh || !isfinite(vals[j])) return fallback;
+                            acc += wts[vi] * vals[j];
+                        }
+                        return acc;
+                    }
+                    int idx[3] = {verts[0], verts[i1], verts[i2]};
+                    double vv[3];
+                    for (int q = 0; q < 3; q++) {
+                        int j;
+                        for (j = 0; j < n_neigh; j++)
+                            if (neigh[j] == idx[q]) break;
+                        if (j == n_neigh) return fallback;
+                        vv[q] = vals[j];
+                    }
+                    if (!(isfinite(vv[0]) && isfinite(vv[1]) && isfinite(vv[2]))) return fallback;
+                    return w1 * vv[0] + w2 * vv[1] + w3 * vv[2];
+                }
+            }
+        }
+        return fallback;
+    }
+    /* leastsq */
+    if (n_neigh < 3) return fallback;
+    double z0 = M_COS(lat_p * DEG2R) * SLM;
+    if (z0 < 1.0) z0 = 1.0;
+    double a11 = 0.0, a12 = 0.0, a13 = 0.0, a22 = 0.0, a23 = 0.0, a33 = 0.0;
+    double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+    int n_fin = 0;
+    for (int i = 0; i < n_neigh; i++) {
+        int nd = neigh[i];
+        double val = vals[i];
+        if (!isfinite(val)) continue;
+        n_fin++;
+        double dlon = PYMOD360(m->nodes_lon[nd] - lon_p + 180.0) - 180.0;
+        double ex = dlon * z0;
+        double ey = (m->nodes_lat[nd] - lat_p) * SLM;
+        a11 += 1.0;
+        a12 += ex;
+        a13 += ey;
+        a22 += ex * ex;
+        a23 += ex * ey;
+        a33 += ey * ey;
+        f0 += val;
+        f1 += val * ex;
+        f2 += val * ey;
+    }
+    if (n_fin < 3) return fallback;
+    double a21 = a12, a31 = a13, a32 = a23;
+    double det = (a11 * a22 * a33 + a12 * a23 * a31 + a13 * a21 * a32 - a13 * a22 * a31 -
+                  a12 * a21 * a33 - a11 * a23 * a32);
+    if (fabs(det) < 1e-12 || fabs(det) < 1e-12 * fabs(a11 * a22 * a33)) return fallback;
+    double inv = 1.0 / fabs(det);
+    double b11 = a22 * a33 - a23 * a32;
+    double b12 = a13 * a32 - a12 * a33;
+    double b13 = a12 * a23 - a13 * a22;
+    (void)s;
+    (void)z;
+    return (b11 * f0 + b12 * f1 + b13 * f2) * inv;
+}
+
+/* sampling.py:727 sample_field on a z-level mesh for each particle:
+ * NaN where not alive or without an anchor; 2-D and mid-layer fields by the
+ * 1-ring combination, interface fields by the W interpolators
+ * (vertical.py:1330 interp_w_spatial, time_dir +1).  method: 0 nearest,
+ * 1 leastsq, 2 barycentric, 3 idw (idw not for interface fields). */
+void or_sample(const or_mesh_t *m, const double *fa, const double *fb, double alpha, int n_planes,
+               int method, int64_t P, const double *lon, const double *lat, const double *z,
+               const int32_t *status, const int32_t *nn, double *out)
+{
+    or_scalar_t s = {fa, fb ? fb : fa, alpha, n_planes};
+    or_fields_t fw;
+    memset(&fw, 0, sizeof fw);
+    fw.w_a = fa;
+    fw.w_b = fb ? fb : fa;
+    fw.w_t = fa;
+    fw.dtype = OR_F64;
+    fw.alpha0 = alpha;
+    for (int64_t i = 0; i < P; i++) {
+        if (status[i] != ST_ALIVE || nn[i] < 0) {
+            out[i] = NAN;
+            continue;
+        }
+        int node = nn[i];
+        if (n_planes == m->n_levels && n_planes > 1) {
+            if (method == 0)
+                out[i] = w_linear(m, &fw, 0.0, z[i], node, 1.0);
+            else if (method == 1)
+                out[i] = w_leastsq(m, &fw, 0.0, lon[i], lat[i], z[i], node, 1.0);
+            else
+                out[i] = w_bary(m, &fw, 0.0, lon[i], lat[i], z[i], node, 1.0);
+            continue;
+        }
+        int buf[64];
+        double vals[64];
+        int count = collect_neighbours(m, node, buf);
+        for (int k = 0; k < count; k++) vals[k] = node_value(m, &s, z ? z[i] : 0.0, buf[k]);
+        out[i] = combine_horizontal(m, &s, z ? z[i] : 0.0, buf, count, vals, lon[i], lat[i], node, method);
+    }
+}

@@ -143,6 +143,9 @@ def lib(flavour: str = "glibc") -> ctypes.CDLL:
                                          c_int, c_double, c_int, c_double,
                                          POINTER(c_double), POINTER(c_double), POINTER(c_int)]
     L.or_interp_bary_tinterp.restype = c_int
+    L.or_sample.argtypes = [POINTER(OrMesh), vp, vp, c_double, c_int, c_int, c_int64, vp, vp, vp,
+                            vp, vp, vp]
+    L.or_sample.restype = None
     _LIBS[flavour] = L
     return L
 
@@ -217,6 +220,23 @@ class OracleMesh:
                 1.0, -1, 0.0, ctypes.byref(a), ctypes.byref(b), ctypes.byref(e)))
             U[i], V[i], el[i] = a.value, b.value, e.value
         return U, V, ok, el
+
+    def sample(self, values_a, lon, lat, z, status, nn, values_b=None, alpha=0.0,
+               method="leastsq"):
+        """numerics/sampling.py:727 sample_field of a node scalar ((N,),
+        (K-1, N) or (K, N)), blended a -> b by ``alpha``, at each particle."""
+        codes = {"nearest": 0, "leastsq": 1, "barycentric": 2, "idw": 3}
+        fa = _c(values_a, np.float64)
+        fb = fa if values_b is None else _c(values_b, np.float64)
+        n_planes = 1 if fa.ndim == 1 else fa.shape[0]
+        lon, lat = _c(lon, np.float64), _c(lat, np.float64)
+        z = _c(np.zeros(lon.size) if z is None else z, np.float64)
+        st, nn_ = _c(status, np.int32), _c(nn, np.int32)
+        out = np.empty(lon.size)
+        self.L.or_sample(ctypes.byref(self.s), fa.ctypes.data, fb.ctypes.data, float(alpha),
+                         int(n_planes), codes[method], lon.size, lon.ctypes.data, lat.ctypes.data,
+                         z.ctypes.data, st.ctypes.data, nn_.ctypes.data, out.ctypes.data)
+        return out
 
     def element_contains(self, e, lon, lat) -> bool:
         return bool(self.L.or_element_contains(ctypes.byref(self.s), int(e), float(lon), float(lat)))

@@ -310,3 +310,25 @@ def beaching_case(direction):
                 start_time_idx=0 if direction > 0 else n_time - 1, n_steps=14,
                 n_time_total=n_time, file_times=np.arange(n_time) * RUN_FILE_DT,
                 file_dt=RUN_FILE_DT)
+
+
+def sampling_case(direction):
+    """tests/_gen_golden_3d.py:463: record-point sampling of an interface
+    (K, N) and a mid-layer (K-1, N) scalar along node RK2 trajectories
+    (leastsq W: the unfused barycentric path)."""
+    from kelp_amd.config import ModeConfig, NumericsConfig
+    from kelp_amd.particles import make_state
+    mesh = run_mesh(lambda lon, lat: 60.0 + 20.0 * (lon - 4.0))
+    ds = fesom2_node_ds(mesh, 5, np.zeros(5), scalars=True)
+    i = np.arange(24, dtype=np.float64)
+    lon = 4.15 + 1.7 * ((i * 0.618034) % 1.0)
+    lat = 4.15 + 1.7 * ((i * 0.381966 + 0.29) % 1.0)
+    z = np.array([0.0, 4.0, 10.0, 17.5, 25.0, 33.0, 50.0, 64.0, 79.0, 99.0, 12.5, 40.0] * 2)
+    numerics = NumericsConfig(dt=RUN_DT, integrator="rk2", uv_interp="barycentric",
+                              w_interp="leastsq", temporal_interp=True)
+    mode = ModeConfig(Fdrift2D=False, Fdrift3D=True, time_dir=direction, var_u="unod",
+                      var_v="vnod", var_w="w", var_eta="ssh")
+    return dict(mesh=mesh, ds=ds, init=make_state(lon, lat, z), mode=mode, numerics=numerics,
+                start_time_idx=0 if direction > 0 else 4, n_steps=8, n_time_total=5,
+                file_times=np.arange(5) * RUN_FILE_DT, file_dt=RUN_FILE_DT,
+                sample_vars=("s_lev", "s_mid"))

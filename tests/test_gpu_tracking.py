@@ -40,3 +40,21 @@ def test_beaching_run_matches_reference_golden(gpu_ctx, direction):
     for k, src in (("lon", "lon"), ("lat", "lat"), ("z", "z"), ("u", "vel_u"), ("v", "vel_v"),
                    ("w", "vel_w")):
         np.testing.assert_allclose(rec[src], gold[k], rtol=1e-9, atol=1e-9, err_msg=k)
+
+
+@pytest.mark.parametrize("direction", [1.0, -1.0])
+def test_sampling_run_matches_reference_golden(gpu_ctx, direction):
+    """Record-point sampling of an interface and a mid-layer scalar
+    (leastsq), re-blended to each record's instant."""
+    case = rc.sampling_case(direction)
+    gold = np.load(os.path.join(GOLD, "scalar_sampling_rk2" + ("_bwd" if direction < 0 else "")
+                                + ".npz"), allow_pickle=False)
+    rec = run_single_realization(
+        case["mesh"], case["ds"], get_reader("fesom2_arrays"), case["init"], case["mode"],
+        case["numerics"], start_time_idx=case["start_time_idx"], n_steps=case["n_steps"],
+        n_time_total=case["n_time_total"], file_times=case["file_times"],
+        file_dt=case["file_dt"], sample_vars=case["sample_vars"], ctx=gpu_ctx).as_arrays()
+    np.testing.assert_array_equal(rec["status"], gold["status"])
+    for k, src in (("lon", "lon"), ("lat", "lat"), ("z", "z"), ("u", "vel_u"), ("w", "vel_w"),
+                   ("sample__s_lev", "sample__s_lev"), ("sample__s_mid", "sample__s_mid")):
+        np.testing.assert_allclose(rec[src], gold[k], rtol=1e-9, atol=1e-9, err_msg=k)
