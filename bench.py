@@ -12,7 +12,8 @@ Workloads:
   global_3d    (default) 1M-node Fibonacci-sphere Delaunay mesh with a land
                mask, 47 z-layers, 8 six-hourly float32 snapshots, 1M particles,
                RK4, dt 1800 s, linear time blend, Kh = 10 m^2/s random walk,
-               beaching (on_failure + stochastic coastal).
+               beaching (on_failure + stochastic coastal), T and S sampled
+               (leastsq) at every step's end, as every step is an output.
   double_gyre  2-D jittered 0.25-degree lattice, steady double gyre, 10,000
                particles, RK2, dt 600 s.
 """
@@ -60,27 +61,45 @@ class Global3D:
         self.kick = dict(kh_x=self.w.kh, kh_y=self.w.kh, seed=20240501)
         self.beach = dict(criteria=["on_failure", "stochastic"], t_beach=86400.0,
                           depth_min=0.3, seed=20240501)
+        self.sample_ts = not args.no_sample
         self.order = 4
         self.config = {"workload": self.name, "mesh_nodes": self.w.mesh.n_nodes,
                        "mesh_elements": self.w.mesh.n_elements, "z_levels": int(self.w.mesh.sigma.size),
                        "particles": int(self.w.lon.size), "integrator": "rk4", "dt_s": self.w.dt,
                        "snapshots": len(self.w.snaps), "snapshot_dt_s": self.w.snap_dt,
                        "field_dtype": "f32", "kh_m2_s": self.w.kh, "w_interp": "barycentric",
-                       "beaching": self.beach["criteria"], "steps_per_run": self.w.n_steps}
+                       "beaching": self.beach["criteria"], "steps_per_run": self.w.n_steps,
+                       "sampled": ["T", "S"] if self.sample_ts else []}
 
     def upload(self, ctx):
         ctx.upload_mesh(self.w.mesh)
-        for s, (U, V, W, _T, _S) in enumerate(self.w.snaps):
+        for s, (U, V, W, T, S) in enumerate(self.w.snaps):
             ctx.upload_field(s, U, V, W)
-        return ctx.particles(self.w.lon, self.w.lat, self.w.z)
+            if self.sample_ts:
+                ctx.upload_scalar(s, 0, T)
+                ctx.upload_scalar(s, 1, S)
+        p = ctx.particles(self.w.lon, self.w.lat, self.w.z)
+        if self.sample_ts:
+            t = ctx.torch
+            self.ts_out = [t.empty((p.n,), dtype=t.float64, device=p.lon.device) for _ in range(2)]
+        return p
 
-    def step(self, ctx, p, k):
+    def step(self, ctx, p, k, ev=None):
         w = self.w
         t = k * w.dt
         ia, ib, a0, it, dtn = self.synth.snapshot_bracket(t, w.snap_dt, len(w.snaps))
         al = [c * w.dt / dtn for c in (0.0, 0.5, 0.5, 1.0)]
+        if ev is not None:
+            ctx.event_record(ev)
         ctx.step(p, 4, ia, w.dt, 1.0, slot_b=ib, slot_t=it, alpha0=a0, alphas=al,
                  exact_nearest=1, w_method=2, kick=self.kick, beach=self.beach, step_i=k)
+        if ev is not None:
+            ctx.event_record(ev + 1)
+        if self.sample_ts:
+            # T and S at the step's end (the record instant)
+            ra, rb, ra0, _, _ = self.synth.snapshot_bracket(t + w.dt, w.snap_dt, len(w.snaps))
+            for var in (0, 1):
+                ctx.sample(p, var, ra, rb, ra0, "leastsq", out=self.ts_out[var])
 
     def cpu_baseline(self, budget_s):
         from oracle.oracle import OracleMesh, step_args
@@ -103,11 +122,16 @@ class Global3D:
             alive_steps += int(np.sum(st["status"] == 1))
             st = om.step(st, f, step_args(4, w.dt, 1.0, is3d=True, w_method=2, exact=1, alphas=al,
                                           kick=self.kick, beach=self.beach, step_i=steps))
+            if self.sample_ts:
+                ra, rb, ra0, _, _ = self.synth.snapshot_bracket(t + w.dt, w.snap_dt, len(w.snaps))
+                for var in (3, 4):
+                    om.sample(w.snaps[ra][var], st["lon"], st["lat"], st["z"], st["status"], st["nn"],
+                              values_b=None if rb < 0 else w.snaps[rb][var], alpha=ra0)
             steps += 1
         el = time.perf_counter() - t0
         return {"value": alive_steps / el, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-                "sample": f"{n} particles x {steps} steps of the same workload, C oracle "
-                          f"(glibc libm), 1 thread, {el:.1f} s"}
+                "sample": f"{n} particles x {steps} steps of the same workload (incl. T/S "
+                          f"sampling), C oracle (glibc libm), 1 thread, {el:.1f} s"}
 
 
 class DoubleGyre:
@@ -130,8 +154,12 @@ class DoubleGyre:
         ctx.upload_field(0, self.w.u, self.w.v)
         return ctx.particles(self.w.lon, self.w.lat)
 
-    def step(self, ctx, p, k):
+    def step(self, ctx, p, k, ev=None):
+        if ev is not None:
+            ctx.event_record(ev)
         ctx.step(p, 2, 0, self.w.dt, 1.0, exact_nearest=1)
+        if ev is not None:
+            ctx.event_record(ev + 1)
 
     def cpu_baseline(self, budget_s):
         from oracle.oracle import OracleMesh
@@ -166,6 +194,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--resort-every", type=int, default=4,
                     help="re-sort particles by grid-hash cell every N steps (0: never)")
+    ap.add_argument("--no-sample", action="store_true",
+                    help="global_3d: leave out the T/S sampling at each step's end")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first copy a buffer of this size 3x (calibrates profiler byte counters)")
     args = ap.parse_args()
@@ -191,11 +221,13 @@ def main():
     p.alive_count = torch.zeros(1, dtype=torch.int64, device="cuda:0")
     log("uploaded; warming up")
 
-    def run_step(k):
+    def run_step(k, ev=None):
         if args.resort_every > 0 and k % args.resort_every == 0:
             ctx.sort(p)
-        wl.step(ctx, p, k)
+        wl.step(ctx, p, k, ev)
 
+    if args.steps > 511:
+        raise SystemExit("--steps is limited to 511 (one event pair per timed step)")
     for k in range(args.warmup):
         run_step(k)
     ctx.synchronize()
@@ -204,14 +236,14 @@ def main():
     torch.cuda.synchronize()
     log(f"timing {args.steps} steps")
     t0 = time.perf_counter()
-    ctx.event_record(0)
-    for k in range(args.warmup, args.warmup + args.steps):
-        run_step(k)
-    ctx.event_record(1)
+    for j, k in enumerate(range(args.warmup, args.warmup + args.steps)):
+        run_step(k, ev=2 + 2 * j)
     ctx.synchronize()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    kern_ms = ctx.event_elapsed_ms(0, 1)
+    # the step kernel's own launches: HIP events on the context stream
+    # around each of them
+    kern_ms = sum(ctx.event_elapsed_ms(2 + 2 * j, 3 + 2 * j) for j in range(args.steps))
     alive_steps = int(p.alive_count.item())
     alive_end = int((p.status == 1).sum().item())
     pss = alive_steps / wall

@@ -181,7 +181,7 @@ int kelp_free(kelp_ctx *ctx, void *ptr);
 int kelp_memcpy_h2d(kelp_ctx *ctx, void *dst, const void *src, int64_t bytes);
 int kelp_memcpy_d2h(kelp_ctx *ctx, void *dst, const void *src, int64_t bytes);
 
-/* Timing on the context stream: record event `slot` (0..15) now; elapsed
+/* Timing on the context stream: record event `slot` (0..1023) now; elapsed
  * milliseconds between two recorded slots (synchronises on `b`). */
 int kelp_event_record(kelp_ctx *ctx, int slot);
 int kelp_event_elapsed(kelp_ctx *ctx, int a, int b, float *ms);

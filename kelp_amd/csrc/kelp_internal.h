@@ -22,7 +22,7 @@ int kelp_fail(int code, const char* fmt, ...);
     } while (0)
 
 constexpr int kMaxSlots = 16;
-constexpr int kMaxEvents = 16;
+constexpr int kMaxEvents = 1024;
 constexpr int kMaxScalars = 8;
 
 // A node scalar of one snapshot for record-point sampling: n_planes planes

@@ -29,36 +29,55 @@ __device__ __forceinline__ double sval(const SField& s, int64_t idx)
     return (1.0 - s.alpha) * s.a[idx] + s.alpha * s.b[idx];
 }
 
-// sampling.py:433 _interp_mid_linear_single (mid-layer depths are the
-// interface midpoints 0.5 * (z_k + z_k+1))
-__device__ double mid_linear(const DMesh& m, const SField& s, double z, int node)
+// The bracket of depth z among the mid-layer depths 0.5 * (z_k + z_k+1) in
+// the branch structure of sampling.py:433 _interp_mid_linear_single: plane
+// `k` alone (al < 0) or planes k and k+1 blended by al.  The mid-layer
+// depths are non-decreasing, so the first k with z < zm[k+1] is found by
+// bisection; a NaN depth ends on the last plane, as the linear scan does.
+struct MidBracket {
+    int k;
+    double al;
+};
+
+__device__ MidBracket mid_bracket(const DMesh& m, double z)
 {
     const double* zl = m.zlev;
     const int K1 = m.n_levels - 1;
-    const int64_t N = m.n_nodes;
-    if (K1 <= 0) return 0.0;
-    if (K1 == 1) return sval(s, node);
-    if (z <= 0.5 * (zl[0] + zl[1])) return sval(s, node);
-    if (z >= 0.5 * (zl[K1 - 1] + zl[K1])) return sval(s, (int64_t)(K1 - 1) * N + node);
-    for (int k = 0; k < K1 - 1; ++k) {
-        const double zk = 0.5 * (zl[k] + zl[k + 1]);
-        const double zk1 = 0.5 * (zl[k + 1] + zl[k + 2]);
-        if (z < zk1) {
-            const double dz = zk1 - zk;
-            if (dz < 1e-12) return sval(s, (int64_t)k * N + node);
-            const double al = (z - zk) / dz;
-            return (1.0 - al) * sval(s, (int64_t)k * N + node) + al * sval(s, (int64_t)(k + 1) * N + node);
-        }
+    if (K1 <= 1 || z <= 0.5 * (zl[0] + zl[1])) return {0, -1.0};
+    if (z >= 0.5 * (zl[K1 - 1] + zl[K1])) return {K1 - 1, -1.0};
+    int lo = 0, hi = K1 - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (z < 0.5 * (zl[mid + 1] + zl[mid + 2]))
+            hi = mid;
+        else
+            lo = mid + 1;
     }
-    return sval(s, (int64_t)(K1 - 1) * N + node);
+    if (lo > K1 - 2) return {K1 - 1, -1.0};
+    const double zk = 0.5 * (zl[lo] + zl[lo + 1]);
+    const double zk1 = 0.5 * (zl[lo + 1] + zl[lo + 2]);
+    const double dz = zk1 - zk;
+    if (dz < 1e-12) return {lo, -1.0};
+    return {lo, (z - zk) / dz};
 }
 
-// the value the horizontal combination uses at node nd (sampling.py:517)
-__device__ __forceinline__ double node_value(const DMesh& m, const SField& s, double z, int nd)
+__device__ __forceinline__ double mid_value(const DMesh& m, const SField& s, const MidBracket& b, int node)
+{
+    const int64_t N = m.n_nodes;
+    if (b.al < 0.0) return sval(s, (int64_t)b.k * N + node);
+    return (1.0 - b.al) * sval(s, (int64_t)b.k * N + node) + b.al * sval(s, (int64_t)(b.k + 1) * N + node);
+}
+
+// The value the horizontal combination uses at node nd (sampling.py:517):
+// 2-D the node value; mid-layer linear in depth at min(z, depth[nd]), with
+// the particle's own bracket `bz` reused for every node deeper than z.
+__device__ __forceinline__ double node_value(const DMesh& m, const SField& s, double z, const MidBracket& bz,
+                                             int nd)
 {
     if (s.n_planes == 1) return sval(s, nd);
     const double dep = m.depth[nd];
-    return mid_linear(m, s, z < dep ? z : dep, nd);
+    if (z < dep) return mid_value(m, s, bz, nd);
+    return mid_value(m, s, mid_bracket(m, dep), nd);
 }
 
 // visit the unique 1-ring nodes of `node` in the order of
@@ -92,7 +111,8 @@ template <int MAXV>
 __device__ double combine(const DMesh& m, const SField& s, double z, double lon_p, double lat_p, int nn,
                           int method)
 {
-    const double fallback = node_value(m, s, z, nn);
+    const MidBracket bz = s.n_planes > 1 ? mid_bracket(m, z) : MidBracket{0, -1.0};
+    const double fallback = node_value(m, s, z, bz, nn);
     if (method == 0) return fallback;
     if (method == 3) {
         double z0 = km_cos(lat_p * kDeg2Rad) * kSLM;
@@ -107,7 +127,7 @@ __device__ double combine(const DMesh& m, const SField& s, double z, double lon_
             const double dx = dlon * z0;
             const double dy = (ll.y - lat_p) * kSLM;
             const double d2 = dx * dx + dy * dy;
-            const double val = node_value(m, s, z, nd);
+            const double val = node_value(m, s, z, bz, nd);
             if (d2 < 1e-12) {
                 hit = true;
                 hit_val = val;
@@ -190,15 +210,15 @@ __device__ double combine(const DMesh& m, const SField& s, double z, double lon_
                         double acc = 0.0;
 #pragma unroll
                         for (int q = 0; q < 4; ++q) {
-                            const double val = node_value(m, s, z, verts[q]);
+                            const double val = node_value(m, s, z, bz, verts[q]);
                             if (!__builtin_isfinite(val)) return fallback;
                             acc += wts[q] * val;
                         }
                         return acc;
                     }
-                    const double v0 = node_value(m, s, z, verts[0]);
-                    const double v1 = node_value(m, s, z, verts[i1]);
-                    const double v2 = node_value(m, s, z, verts[i2]);
+                    const double v0 = node_value(m, s, z, bz, verts[0]);
+                    const double v1 = node_value(m, s, z, bz, verts[i1]);
+                    const double v2 = node_value(m, s, z, bz, verts[i2]);
                     if (!(__builtin_isfinite(v0) && __builtin_isfinite(v1) && __builtin_isfinite(v2)))
                         return fallback;
                     return w1 * v0 + w2 * v1 + w3 * v2;
@@ -214,7 +234,7 @@ __device__ double combine(const DMesh& m, const SField& s, double z, double lon_
     double f0 = 0.0, f1 = 0.0, f2 = 0.0;
     int n_fin = 0;
     const int count = for_neighbours<MAXV>(m, nn, [&](int nd) {
-        const double val = node_value(m, s, z, nd);
+        const double val = node_value(m, s, z, bz, nd);
         if (!__builtin_isfinite(val)) return;
         ++n_fin;
         const double2 ll = m.lonlat[nd];

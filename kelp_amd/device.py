@@ -208,17 +208,19 @@ class DeviceContext:
               "kelp_scalar_upload")
 
     def sample(self, p: "DeviceParticles", var: int, slot_a: int, slot_b: int = -1,
-               alpha: float = 0.0, method: str = "leastsq") -> np.ndarray:
-        """Sampled scalar ``var`` at every particle (host array, particle
-        order of ``p``)."""
+               alpha: float = 0.0, method: str = "leastsq", out=None):
+        """Sampled scalar ``var`` at every particle, particle order of ``p``:
+        a host array, or written into the device tensor ``out`` (no wait)."""
         codes = {"nearest": 0, "leastsq": 1, "barycentric": 2, "idw": 3}
         t = self.torch
-        out = t.empty((p.n,), dtype=t.float64, device=p.lon.device)
+        dev_out = out if out is not None else t.empty((p.n,), dtype=t.float64, device=p.lon.device)
         s = p.as_struct()
         check(self.lib.kelp_sample(self.h, ctypes.byref(s), int(var), int(slot_a), int(slot_b),
-                                   float(alpha), codes[method], out.data_ptr()), "kelp_sample")
+                                   float(alpha), codes[method], dev_out.data_ptr()), "kelp_sample")
+        if out is not None:
+            return out
         self.synchronize()
-        return out.cpu().numpy()
+        return dev_out.cpu().numpy()
 
     def upload_field(self, slot: int, u, v, w=None, key=None, uv_layout="node") -> None:
         """Upload one snapshot into ``slot``; skipped when ``key`` matches the
