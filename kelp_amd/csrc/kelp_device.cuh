@@ -52,6 +52,9 @@ struct DMesh {
     const uint8_t* __restrict__ coastal;  // (E,) coastal-element flags (beaching)
     const double2* __restrict__ ecent;    // (E,) element centres (lon, lat); element-layout u/v
     const double* __restrict__ coslat;    // (N,) km_cos(lat * kDeg2Rad) per node
+    const double4* __restrict__ egeo;     // (2E,) containment frame per element, see elem_geom
+    const double2* __restrict__ exy;      // (4E,) projected vertices per element
+    const uint8_t* __restrict__ eflags;   // (E,) bit 0: the interpolation ring is the containment ring
     double cell, lon_min, lat_min, r_earth;
     int32_t nx, ny, wrap, far_rows;
     int32_t n_nodes, n_elems, max_v, n_levels;
@@ -583,14 +586,42 @@ __device__ __forceinline__ bool in_tri(double px, double py, double ax, double a
     return w1 >= kBaryEps && w2 >= kBaryEps && w3 >= kBaryEps;
 }
 
-// core/mesh.py:2211 _element_contains (bbox prefilter on wrapped deltas,
-// then :2140 _polygon_contains_bool with the :2007 projection)
+// The particle-independent part of core/mesh.py:2211 _element_contains for
+// element c, computed once at mesh upload: the containment ring's first
+// longitude lo0, the bbox of the wrapped longitude deltas (dmin, dmax) and
+// of the latitudes (lamin, lamax), the projection origin of :2007
+// _project_to_local_xy (lon0 seam-gated, lat0, cos(lat0)) and the projected
+// vertices.  Stored as egeo[2c] = (lo0, dmin, dmax, lamin), egeo[2c+1] =
+// (lamax, lon0, lat0, cos_lat0), exy[4c+k] = (x_k, y_k).  A ring of fewer
+// than three vertices gets an empty latitude box (never contains); a
+// triangle in a four-wide mesh gets a NaN fourth vertex (its second fan
+// triangle never contains).  Every value is the expression the per-query
+// test evaluated, so the test's answers are unchanged bit for bit.
 template <int MAXV>
-__device__ inline bool element_contains(const DMesh& m, int c, double lon_p, double lat_p)
+__device__ inline void elem_geom(const DMesh& m, int c, double4& g0, double4& g1, double2 (&xy)[4],
+                                 uint8_t& flags)
 {
     int v[4] = {0, 0, 0, 0};
     const int n = contain_ring<MAXV>(m, c, v);
-    if (n < 3) return false;
+    {
+        // the W and scalar interpolators use the interpolation ring; they
+        // may use this frame only when it is the same vertex sequence
+        int w[4] = {0, 0, 0, 0};
+        const int nw = interp_ring<MAXV>(m.elems[c], w);
+        bool same = nw == n;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (k < n && w[k] != v[k]) same = false;
+        flags = same ? 1 : 0;
+    }
+    const double nan = __builtin_nan("");
+#pragma unroll
+    for (int k = 0; k < 4; ++k) xy[k] = make_double2(nan, nan);
+    if (n < 3) {
+        g0 = make_double4(0.0, 0.0, 0.0, __builtin_inf());
+        g1 = make_double4(-__builtin_inf(), 0.0, 0.0, 0.0);
+        return;
+    }
     double lo[4] = {0, 0, 0, 0}, la[4] = {0, 0, 0, 0};
 #pragma unroll
     for (int k = 0; k < MAXV; ++k) {
@@ -610,8 +641,6 @@ __device__ inline bool element_contains(const DMesh& m, int c, double lon_p, dou
             if (k == 0 || d < dmin) dmin = d;
         }
     }
-    double dlon_p = km_pymod360(lon_p - lon_ref + 180.0) - 180.0;
-    if (dlon_p > dmax || dlon_p < dmin) return false;
     double lamax = la[0], lamin = la[0];
 #pragma unroll
     for (int k = 1; k < MAXV; ++k) {
@@ -620,8 +649,6 @@ __device__ inline bool element_contains(const DMesh& m, int c, double lon_p, dou
             if (la[k] < lamin) lamin = la[k];
         }
     }
-    if (lat_p > lamax || lat_p < lamin) return false;
-    // projection (core/mesh.py:2007)
     double lmin = lo[0], lmax = lo[0];
 #pragma unroll
     for (int k = 0; k < MAXV; ++k) {
@@ -651,23 +678,39 @@ __device__ inline bool element_contains(const DMesh& m, int c, double lon_p, dou
         if (k < n) sl += la[k];
     const double lat0 = sl / nd;
     const double cos_lat0 = km_cos(lat0 * kDeg2Rad);
-    double xn[4] = {0, 0, 0, 0}, yn[4] = {0, 0, 0, 0};
 #pragma unroll
     for (int k = 0; k < MAXV; ++k) {
         if (k < n) {
             double dlon = km_pymod360(lo[k] - lon0 + 180.0) - 180.0;
             double dlat = la[k] - lat0;
-            xn[k] = dlon * kDeg2Rad * kREarthStd * cos_lat0;
-            yn[k] = dlat * kDeg2Rad * kREarthStd;
+            xy[k] = make_double2(dlon * kDeg2Rad * kREarthStd * cos_lat0, dlat * kDeg2Rad * kREarthStd);
         }
     }
-    double dlp = km_pymod360(lon_p - lon0 + 180.0) - 180.0;
-    double px = dlp * kDeg2Rad * kREarthStd * cos_lat0;
-    double py = (lat_p - lat0) * kDeg2Rad * kREarthStd;
-    if (n == 3 || MAXV == 3) return in_tri(px, py, xn[0], yn[0], xn[1], yn[1], xn[2], yn[2]);
-    // quad: fan (0, 1, 2), (0, 2, 3)
-    if (in_tri(px, py, xn[0], yn[0], xn[1], yn[1], xn[2], yn[2])) return true;
-    return in_tri(px, py, xn[0], yn[0], xn[2], yn[2], xn[3], yn[3]);
+    g0 = make_double4(lon_ref, dmin, dmax, lamin);
+    g1 = make_double4(lamax, lon0, lat0, cos_lat0);
+}
+
+// core/mesh.py:2211 _element_contains (bbox prefilter on wrapped deltas,
+// then :2140 _polygon_contains_bool with the :2007 projection) on the
+// element's precomputed frame
+template <int MAXV>
+__device__ inline bool element_contains(const DMesh& m, int c, double lon_p, double lat_p)
+{
+    const double4 g0 = m.egeo[2 * (int64_t)c];
+    const double dlon_p = km_pymod360(lon_p - g0.x + 180.0) - 180.0;
+    if (dlon_p > g0.z || dlon_p < g0.y) return false;
+    const double4 g1 = m.egeo[2 * (int64_t)c + 1];
+    if (lat_p > g1.x || lat_p < g0.w) return false;
+    const double2* xy = m.exy + 4 * (int64_t)c;
+    const double2 a = xy[0], b = xy[1], d = xy[2];
+    const double dlp = km_pymod360(lon_p - g1.y + 180.0) - 180.0;
+    const double px = dlp * kDeg2Rad * kREarthStd * g1.w;
+    const double py = (lat_p - g1.z) * kDeg2Rad * kREarthStd;
+    if (in_tri(px, py, a.x, a.y, b.x, b.y, d.x, d.y)) return true;
+    if (MAXV == 3) return false;
+    // quad: second fan triangle (0, 2, 3); NaN for a triangle never holds
+    const double2 e = xy[3];
+    return in_tri(px, py, a.x, a.y, d.x, d.y, e.x, e.y);
 }
 
 template <int MAXV>
@@ -992,6 +1035,43 @@ __device__ inline int find_layer(const DMesh& m, double zp, double bottom)
     return r <= max_layer ? r : max_layer;
 }
 
+// The vertical bracket of depth zp among the interfaces, in the branch
+// structure of numerics/vertical.py:260 _w_bracket: the record (layer) k and
+// either its top (a = -1: zp at or above the surface level, or a degenerate
+// layer), its bottom (a = 2: zp at or below the last level) or the blend
+// weight a in [0, 1).  Depends on zp only, so one bracket serves every node
+// evaluated at the same depth.
+struct WBracket {
+    int k;
+    double a;
+};
+
+__device__ __forceinline__ WBracket w_bracket(const DMesh& m, double zp)
+{
+    const double* zl = m.zlev;
+    const int K = m.n_levels;
+    if (zp <= zl[0]) return {0, -1.0};
+    if (zp >= zl[K - 1]) return {K - 2, 2.0};
+    int k = first_above(m, zp);  // < K - 1 here unless zp is NaN
+    if (k > K - 2) k = K - 2;
+    const double dz = zl[k + 1] - zl[k];
+    if (dz < 1e-12) return {k, -1.0};
+    return {k, (zp - zl[k]) / dz};
+}
+
+// W of one node at a bracket (numerics/vertical.py:282 _w_at_bracket on the
+// stage field: the interface pair blended in time, then in depth)
+template <typename T>
+__device__ __forceinline__ double w_at(const DMesh& m, const Snap<T, true>& fw, double alpha0, double alpha,
+                                       const WBracket& b, int node, double time_dir)
+{
+    double wt, wb;
+    w_pair_blend(fw, (int64_t)b.k * m.n_nodes + node, alpha0, alpha, wt, wb);
+    if (b.a < 0.0) return time_dir * wt;
+    if (b.a == 2.0) return time_dir * wb;
+    return time_dir * ((1.0 - b.a) * wt + b.a * wb);
+}
+
 // numerics/vertical.py:1065 _interp_w_linear_single_tinterp: W linear in
 // depth between the interfaces bracketing zp at one node (records hold the
 // interface pair of each layer: level k < K-1 is record k's top, level K-1 the
@@ -1000,90 +1080,104 @@ template <typename T>
 __device__ inline double w_linear(const DMesh& m, const Snap<T, true>& fw, double alpha0, double alpha,
                                   double zp, int node, double time_dir)
 {
-    const double* zl = m.zlev;
-    const int K = m.n_levels;
-    const int64_t N = m.n_nodes;
-    double wt, wb;
-    if (zp <= zl[0]) {
-        w_pair_blend(fw, node, alpha0, alpha, wt, wb);
-        return time_dir * wt;
-    }
-    if (zp >= zl[K - 1]) {
-        w_pair_blend(fw, (int64_t)(K - 2) * N + node, alpha0, alpha, wt, wb);
-        return time_dir * wb;
-    }
-    int k = first_above(m, zp);  // < K - 1 here unless zp is NaN
-    if (k > K - 2) k = K - 2;
-    w_pair_blend(fw, (int64_t)k * N + node, alpha0, alpha, wt, wb);
-    double dz = zl[k + 1] - zl[k];
-    if (dz < 1e-12) return time_dir * wt;
-    double a = (zp - zl[k]) / dz;
-    return time_dir * ((1.0 - a) * wt + a * wb);
+    return w_at(m, fw, alpha0, alpha, w_bracket(m, zp), node, time_dir);
+}
+
+// W at node nd, depth clamped to its bottom (the bottom-aware evaluation of
+// the barycentric and least-squares W); bz is the bracket of zp itself
+template <typename T>
+__device__ __forceinline__ double w_clamped(const DMesh& m, const Snap<T, true>& fw, double alpha0, double alpha,
+                                            double zp, const WBracket& bz, int nd, double time_dir)
+{
+    const double dep = m.depth[nd];
+    if (zp < dep) return w_at(m, fw, alpha0, alpha, bz, nd, time_dir);
+    return w_at(m, fw, alpha0, alpha, w_bracket(m, dep), nd, time_dir);
 }
 
 // numerics/vertical.py:745 _interp_w_bary_single_tinterp: first fan
 // sub-triangle of a 1-ring element holding the point (projection around the
 // seam-safe vertex mean, core/mesh.py:1974), bottom-aware linear W per vertex;
-// fallback = linear W at the anchor, depth clamped to its bottom.
+// fallback = linear W at the anchor, depth clamped to its bottom.  The
+// projection is the element's precomputed containment frame (the same
+// expressions) whenever the two vertex rings coincide.
 template <int MAXV, typename T>
 __device__ inline double w_bary(const DMesh& m, const Snap<T, true>& fw, double alpha0, double alpha, double xp,
                                 double yp, double zp, int node, double time_dir)
 {
+    const WBracket bz = w_bracket(m, zp);
     const int k0 = m.n2e_off[node], k1 = m.n2e_off[node + 1];
     for (int k = k0; k < k1; ++k) {
+        const int e = m.n2e[k];
         int verts[4] = {0, 0, 0, 0};
-        const int nv = interp_ring<MAXV>(m.elems[m.n2e[k]], verts);
+        const int nv = interp_ring<MAXV>(m.elems[e], verts);
         if (nv < 3) continue;
-        double lo[4] = {0, 0, 0, 0}, la[4] = {0, 0, 0, 0};
+        double vx[4] = {0, 0, 0, 0}, vy[4] = {0, 0, 0, 0};
+        double px, py;
+        if (m.eflags[e] & 1) {
+            const double4 g1 = m.egeo[2 * (int64_t)e + 1];
+            const double2* xy = m.exy + 4 * (int64_t)e;
 #pragma unroll
-        for (int v = 0; v < MAXV; ++v) {
-            if (v < nv) {
-                double2 ll = m.lonlat[verts[v]];
-                lo[v] = ll.x;
-                la[v] = ll.y;
+            for (int v = 0; v < MAXV; ++v) {
+                if (v < nv) {
+                    const double2 q = xy[v];
+                    vx[v] = q.x;
+                    vy[v] = q.y;
+                }
             }
-        }
-        const double first = lo[0];
-        double lmin = first, lmax = first, ssum = 0.0;
+            const double dlon_p = km_pymod360(xp - g1.y + 180.0) - 180.0;
+            px = dlon_p * kDeg2Rad * kREarthStd * g1.w;
+            py = (yp - g1.z) * kDeg2Rad * kREarthStd;
+        } else {
+            double lo[4] = {0, 0, 0, 0}, la[4] = {0, 0, 0, 0};
 #pragma unroll
-        for (int v = 0; v < MAXV; ++v) {
-            if (v < nv) {
-                ssum += lo[v];
-                if (lo[v] < lmin) lmin = lo[v];
-                if (lo[v] > lmax) lmax = lo[v];
+            for (int v = 0; v < MAXV; ++v) {
+                if (v < nv) {
+                    double2 ll = m.lonlat[verts[v]];
+                    lo[v] = ll.x;
+                    la[v] = ll.y;
+                }
             }
-        }
-        const double nd = (double)nv;
-        double lon0;
-        if (lmax - lmin > 180.0) {
-            double d = 0.0;
+            const double first = lo[0];
+            double lmin = first, lmax = first, ssum = 0.0;
+#pragma unroll
+            for (int v = 0; v < MAXV; ++v) {
+                if (v < nv) {
+                    ssum += lo[v];
+                    if (lo[v] < lmin) lmin = lo[v];
+                    if (lo[v] > lmax) lmax = lo[v];
+                }
+            }
+            const double nd = (double)nv;
+            double lon0;
+            if (lmax - lmin > 180.0) {
+                double d = 0.0;
+#pragma unroll
+                for (int v = 0; v < MAXV; ++v)
+                    if (v < nv) d += km_pymod360(lo[v] - first + 180.0) - 180.0;
+                lon0 = first + d / nd;
+            } else {
+                lon0 = ssum / nd;
+            }
+            double lat0 = 0.0;
 #pragma unroll
             for (int v = 0; v < MAXV; ++v)
-                if (v < nv) d += km_pymod360(lo[v] - first + 180.0) - 180.0;
-            lon0 = first + d / nd;
-        } else {
-            lon0 = ssum / nd;
-        }
-        double lat0 = 0.0;
+                if (v < nv) lat0 += la[v];
+            lat0 /= nd;
+            const double cos_lat0 = km_cos(lat0 * kDeg2Rad);
+            const double R = 6371000.0;
 #pragma unroll
-        for (int v = 0; v < MAXV; ++v)
-            if (v < nv) lat0 += la[v];
-        lat0 /= nd;
-        const double cos_lat0 = km_cos(lat0 * kDeg2Rad);
-        const double R = 6371000.0;
-        double vx[4] = {0, 0, 0, 0}, vy[4] = {0, 0, 0, 0};
-#pragma unroll
-        for (int v = 0; v < MAXV; ++v) {
-            if (v < nv) {
-                double dlon = km_pymod360(lo[v] - lon0 + 180.0) - 180.0;
-                double dlat = la[v] - lat0;
-                vx[v] = dlon * kDeg2Rad * R * cos_lat0;
-                vy[v] = dlat * kDeg2Rad * R;
+            for (int v = 0; v < MAXV; ++v) {
+                if (v < nv) {
+                    double dlon = km_pymod360(lo[v] - lon0 + 180.0) - 180.0;
+                    double dlat = la[v] - lat0;
+                    vx[v] = dlon * kDeg2Rad * R * cos_lat0;
+                    vy[v] = dlat * kDeg2Rad * R;
+                }
             }
+            const double dlon_p = km_pymod360(xp - lon0 + 180.0) - 180.0;
+            px = dlon_p * kDeg2Rad * R * cos_lat0;
+            py = (yp - lat0) * kDeg2Rad * R;
         }
-        const double dlon_p = km_pymod360(xp - lon0 + 180.0) - 180.0;
-        const double px = dlon_p * kDeg2Rad * R * cos_lat0;
-        const double py = (yp - lat0) * kDeg2Rad * R;
 #pragma unroll
         for (int t = 0; t < MAXV - 2; ++t) {
             if (t < nv - 2) {
@@ -1101,26 +1195,20 @@ __device__ inline double w_bary(const DMesh& m, const Snap<T, true>& fw, double 
                             wachspress4(px, py, vx, vy, wts);
                             double acc = 0.0;
 #pragma unroll
-                            for (int q = 0; q < 4; ++q) {
-                                const int nq = verts[q];
-                                const double dep = m.depth[nq];
-                                acc += wts[q] * w_linear(m, fw, alpha0, alpha, zp < dep ? zp : dep, nq, time_dir);
-                            }
+                            for (int q = 0; q < 4; ++q)
+                                acc += wts[q] * w_clamped(m, fw, alpha0, alpha, zp, bz, verts[q], time_dir);
                             return acc;
                         }
-                        const int n0 = verts[0], n1 = verts[i1], n2 = verts[i2];
-                        const double d0 = m.depth[n0], d1 = m.depth[n1], d2 = m.depth[n2];
-                        const double wv0 = w_linear(m, fw, alpha0, alpha, zp < d0 ? zp : d0, n0, time_dir);
-                        const double wv1 = w_linear(m, fw, alpha0, alpha, zp < d1 ? zp : d1, n1, time_dir);
-                        const double wv2 = w_linear(m, fw, alpha0, alpha, zp < d2 ? zp : d2, n2, time_dir);
+                        const double wv0 = w_clamped(m, fw, alpha0, alpha, zp, bz, verts[0], time_dir);
+                        const double wv1 = w_clamped(m, fw, alpha0, alpha, zp, bz, verts[i1], time_dir);
+                        const double wv2 = w_clamped(m, fw, alpha0, alpha, zp, bz, verts[i2], time_dir);
                         return w1 * wv0 + w2 * wv1 + w3 * wv2;
                     }
                 }
             }
         }
     }
-    const double dep = m.depth[node];
-    return w_linear(m, fw, alpha0, alpha, zp < dep ? zp : dep, node, time_dir);
+    return w_clamped(m, fw, alpha0, alpha, zp, bz, node, time_dir);
 }
 
 // Did vertex slot (k, v) of the anchor's element list hold node `nd` at an
@@ -1154,6 +1242,7 @@ __device__ inline double w_lsq(const DMesh& m, const Snap<T, true>& fw, double a
                                double yp, double zp, int node, double time_dir)
 {
     const int k0 = m.n2e_off[node], k1 = m.n2e_off[node + 1];
+    const WBracket bz = w_bracket(m, zp);
     double z0 = km_cos(yp * kDeg2Rad) * kSLM;
     if (z0 < 1.0) z0 = 1.0;
     double a11 = 0.0, a12 = 0.0, a13 = 0.0, a22 = 0.0, a23 = 0.0, a33 = 0.0;
@@ -1177,8 +1266,7 @@ __device__ inline double w_lsq(const DMesh& m, const Snap<T, true>& fw, double a
             if (k >= k0 && (nd == node || seen_before<MAXV>(m, k0, k, v, nd))) continue;
             if (count >= 64) continue;
             ++count;
-            const double bottom = m.depth[nd];
-            const double w_val = w_linear(m, fw, alpha0, alpha, zp > bottom ? bottom : zp, nd, time_dir);
+            const double w_val = w_clamped(m, fw, alpha0, alpha, zp, bz, nd, time_dir);
             const double2 ll = m.lonlat[nd];
             const double dlon = km_pymod360(ll.x - xp + 180.0) - 180.0;
             const double ex = dlon * z0;
@@ -1197,10 +1285,8 @@ __device__ inline double w_lsq(const DMesh& m, const Snap<T, true>& fw, double a
     const double a21 = a12, a31 = a13, a32 = a23;
     const double det = (a11 * a22 * a33 + a12 * a23 * a31 + a13 * a21 * a32 - a13 * a22 * a31 -
                         a12 * a21 * a33 - a11 * a23 * a32);
-    if (count < 3 || km_fabs(det) < 1e-30 || km_fabs(det) < 1e-12 * km_fabs(a11 * a22 * a33)) {
-        const double dep = m.depth[node];
-        return w_linear(m, fw, alpha0, alpha, zp < dep ? zp : dep, node, time_dir);
-    }
+    if (count < 3 || km_fabs(det) < 1e-30 || km_fabs(det) < 1e-12 * km_fabs(a11 * a22 * a33))
+        return w_clamped(m, fw, alpha0, alpha, zp, bz, node, time_dir);
     const double inv = 1.0 / km_fabs(det);
     const double b11 = a22 * a33 - a23 * a32;
     const double b12 = a13 * a32 - a12 * a33;

@@ -121,6 +121,24 @@ __global__ void node_xyz_kernel(int64_t n, const double2* __restrict__ lonlat, d
     out[k] = r;
 }
 
+// containment frames of all elements (elem_geom)
+template <int MAXV>
+__global__ void elem_geom_kernel(DMesh m, double4* __restrict__ egeo, double2* __restrict__ exy,
+                                 uint8_t* __restrict__ eflags)
+{
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.n_elems) return;
+    double4 g0, g1;
+    double2 xy[4];
+    uint8_t fl;
+    elem_geom<MAXV>(m, (int)c, g0, g1, xy, fl);
+    eflags[c] = fl;
+    egeo[2 * c] = g0;
+    egeo[2 * c + 1] = g1;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) exy[4 * c + k] = xy[k];
+}
+
 // node unit vectors back in mesh (original) node order, (N, 3)
 __global__ void gather_xyz_kernel(int64_t n, const double4* __restrict__ xyz, const int32_t* __restrict__ old_of_new,
                                   double* __restrict__ out)
@@ -465,6 +483,9 @@ int kelp_mesh_upload(kelp_ctx* c, const kelp_mesh_desc* d)
     double *d_depth, *d_zlev, *d_coslat;
     uint8_t* d_coastal = nullptr;
     double2* d_ecent = nullptr;
+    double4* d_egeo = nullptr;
+    double2* d_exy = nullptr;
+    uint8_t* d_eflags = nullptr;
     int rc;
     if ((rc = dev_alloc(c, &d_lonlat, N)) || (rc = dev_alloc(c, &d_xyz, N)) ||
         (rc = dev_alloc(c, &d_off, n_cells + 1)) || (rc = dev_alloc(c, &d_elems, E)) ||
@@ -473,7 +494,8 @@ int kelp_mesh_upload(kelp_ctx* c, const kelp_mesh_desc* d)
         (rc = dev_alloc(c, &d_nodeo, N)) || (rc = dev_alloc(c, &d_noden, N)) ||
         (rc = dev_alloc(c, &d_elemo, E)) || (rc = dev_alloc(c, &d_elemn, E)) ||
         (d->host_elem_coastal && (rc = dev_alloc(c, &d_coastal, E))) ||
-        (has_centres && (rc = dev_alloc(c, &d_ecent, E)))) {
+        (has_centres && (rc = dev_alloc(c, &d_ecent, E))) || (rc = dev_alloc(c, &d_egeo, 2 * E)) ||
+        (rc = dev_alloc(c, &d_exy, 4 * E)) || (rc = dev_alloc(c, &d_eflags, E))) {
         free_mesh(c);
         return rc;
     }
@@ -506,6 +528,9 @@ int kelp_mesh_upload(kelp_ctx* c, const kelp_mesh_desc* d)
     m.coastal = d_coastal;
     m.ecent = d_ecent;
     m.coslat = d_coslat;
+    m.egeo = d_egeo;
+    m.exy = d_exy;
+    m.eflags = d_eflags;
     m.cell = d->gh_cell;
     m.lon_min = d->gh_lon_min;
     m.lat_min = d->gh_lat_min;
@@ -518,6 +543,12 @@ int kelp_mesh_upload(kelp_ctx* c, const kelp_mesh_desc* d)
     m.n_elems = (int32_t)E;
     m.max_v = (int32_t)d->max_v;
     m.n_levels = (int32_t)d->n_levels;
+    if (d->max_v == 3)
+        elem_geom_kernel<3><<<grid_for(E, 256), 256, 0, c->stream>>>(m, d_egeo, d_exy, d_eflags);
+    else
+        elem_geom_kernel<4><<<grid_for(E, 256), 256, 0, c->stream>>>(m, d_egeo, d_exy, d_eflags);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));
     c->m = m;
     c->node_old_of_new = d_nodeo;
     c->node_new_of_old = d_noden;
