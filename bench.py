@@ -92,7 +92,8 @@ class Global3D:
         if ev is not None:
             ctx.event_record(ev)
         ctx.step(p, 4, ia, w.dt, 1.0, slot_b=ib, slot_t=it, alpha0=a0, alphas=al,
-                 exact_nearest=1, w_method=2, kick=self.kick, beach=self.beach, step_i=k)
+                 exact_nearest=1, w_method=2, kick=self.kick, beach=self.beach, step_i=k,
+                 reuse_anchor=True)
         if ev is not None:
             ctx.event_record(ev + 1)
         if self.sample_ts:
@@ -157,7 +158,7 @@ class DoubleGyre:
     def step(self, ctx, p, k, ev=None):
         if ev is not None:
             ctx.event_record(ev)
-        ctx.step(p, 2, 0, self.w.dt, 1.0, exact_nearest=1)
+        ctx.step(p, 2, 0, self.w.dt, 1.0, exact_nearest=1, reuse_anchor=True)
         if ev is not None:
             ctx.event_record(ev + 1)
 

@@ -103,6 +103,12 @@ typedef struct {
                                       [py0, py1) after the kicks, before beaching
                                       (tracking.py:1089 _apply_periodic_wrap) */
     double px0, px1, py0, py1;
+    int32_t reuse_anchor;          /* the caller vouches that every ALIVE particle's nn >= 0
+                                      is the anchor of its current position under this
+                                      exact_nearest mode (true after a kelp_step in the same
+                                      mode with the positions left alone; set nn = -1 for
+                                      particles placed or released since): the step skips
+                                      that search.  Ignored in mode 0. */
 } kelp_step_args;
 
 const char *kelp_last_error(void);

@@ -63,6 +63,7 @@ class StepArgs(ctypes.Structure):
         ("az", c_int32), ("az_const", c_double), ("use_eta", c_int32),
         ("periodic_lon", c_int32), ("periodic_lat", c_int32),
         ("px0", c_double), ("px1", c_double), ("py0", c_double), ("py1", c_double),
+        ("reuse_anchor", c_int32),
     ]
 
 

@@ -816,6 +816,9 @@ int kelp_step(kelp_ctx* c, const kelp_particles* p, const kelp_step_args* a)
     cfg.az_const = a->az_const;
     if ((a->periodic_lon && !(a->px1 > a->px0)) || (a->periodic_lat && !(a->py1 > a->py0)))
         return fail(KELP_EINVAL, "kelp_step: a periodic window needs x0 < x1");
+    // the kick and the fold re-anchor in mode 1 (exact) or 2, so a stored
+    // anchor matches the step's own search in those modes only
+    cfg.reuse_anchor = (a->reuse_anchor && a->exact_nearest != 0) ? 1 : 0;
     cfg.periodic_lon = a->periodic_lon ? 1 : 0;
     cfg.periodic_lat = a->periodic_lat ? 1 : 0;
     cfg.px0 = a->px0;

@@ -31,6 +31,7 @@ struct StepCfg {
     const double* eta_b;
     int periodic_lon, periodic_lat;  // post-step fold windows [x0, x1), [y0, y1)
     double px0, px1, py0, py1;
+    int reuse_anchor;  // an ALIVE particle's nn >= 0 is the anchor of its position
 };
 
 struct PState {

@@ -70,7 +70,8 @@ __device__ inline bool eval_site(const DMesh& m, const Snap<T, IS3D>& f, const S
                                  const StepCfg& c, double alpha, double lon, double lat, double zp, int& nn,
                                  double& U, double& V, double& W)
 {
-    nn = site_anchor(m, lon, lat, c.exact);
+    // nn >= 0 on entry: the anchor of (lon, lat) under this mode is known
+    if (nn < 0) nn = site_anchor(m, lon, lat, c.exact);
     int64_t layer = 0;
     if (IS3D) layer = find_layer(m, zp, m.depth[nn]);
     bool ok;
@@ -102,6 +103,9 @@ __device__ inline int det_step(const DMesh& m, const Snap<T, IS3D>& f, const Sna
     const double lon0 = lon, lat0 = lat, zz0 = z;
     int node_i;
     double U1, V1, W1;
+    // site 1 is the particle's position: its anchor is the one the previous
+    // step (or kick, or fold) stored, when the caller vouches for it
+    node_i = (c.reuse_anchor && nn >= 0) ? nn : -1;
     bool ok = eval_site<MAXV, IS3D, T, ELEM>(m, f, fe, c, c.alphas[0], lon0, lat0, zz0, node_i, U1, V1, W1);
     Uo = U1;
     Vo = V1;
@@ -117,7 +121,7 @@ __device__ inline int det_step(const DMesh& m, const Snap<T, IS3D>& f, const Sna
             double lon_mid = wrap_lon(lon0 + 0.5 * dt * U1 / z0);
             double lat_mid = clamp_lat(lat0 + 0.5 * dt * V1 / kSLM);
             double z_mid = zz0 + W1 * 0.5 * dt;
-            int nn_mid;
+            int nn_mid = -1;
             double U2, V2, W2;
             ok = eval_site<MAXV, IS3D, T, ELEM>(m, f, fe, c, c.alphas[1], lon_mid, lat_mid, z_mid, nn_mid, U2, V2, W2);
             if (ok) {
@@ -133,7 +137,7 @@ __device__ inline int det_step(const DMesh& m, const Snap<T, IS3D>& f, const Sna
             double lon2 = wrap_lon(lon0 + 0.5 * dlon1);
             double lat2 = clamp_lat(lat0 + 0.5 * dlat1);
             double z2 = zz0 + 0.5 * dz1;
-            int nn2, nn3, nn4;
+            int nn2 = -1, nn3 = -1, nn4 = -1;
             double U2, V2, W2, U3, V3, W3, U4, V4, W4;
             ok = eval_site<MAXV, IS3D, T, ELEM>(m, f, fe, c, c.alphas[1], lon2, lat2, z2, nn2, U2, V2, W2);
             if (ok) {

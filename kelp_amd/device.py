@@ -82,7 +82,7 @@ RESUSPENSION_CODES = {"never": 0, "auto": 1, "stochastic": 2, "auto_or_stochasti
 def make_step_args(order, slot_a, dt, time_dir, slot_b=-1, slot_t=-1, alpha0=0.0,
                    alphas=(0.0, 0.0, 0.0, 0.0), exact_nearest=True, w_method=2, kick=None,
                    beach=None, step_i=0, real_id=0, periodic_lon=None,
-                   periodic_lat=None) -> StepArgs:
+                   periodic_lat=None, reuse_anchor=False) -> StepArgs:
     a = StepArgs()
     a.order = int(order)
     a.slot_a, a.slot_b, a.slot_t = int(slot_a), int(slot_b), int(slot_t)
@@ -95,6 +95,7 @@ def make_step_args(order, slot_a, dt, time_dir, slot_b=-1, slot_t=-1, alpha0=0.0
     a.w_method = int(w_method)
     a.step_i = int(step_i)
     a.real_id = int(real_id)
+    a.reuse_anchor = int(bool(reuse_anchor))
     if periodic_lon is not None:
         a.periodic_lon, a.px0, a.px1 = 1, float(periodic_lon[0]), float(periodic_lon[1])
     if periodic_lat is not None:
@@ -285,13 +286,14 @@ class DeviceContext:
     def step(self, p: DeviceParticles, order: int, slot_a: int, dt: float, time_dir: float,
              slot_b: int = -1, slot_t: int = -1, alpha0: float = 0.0, alphas=(0.0, 0.0, 0.0, 0.0),
              exact_nearest: bool = True, w_method: int = 2, kick=None, beach=None,
-             step_i: int = 0, real_id: int = 0, periodic_lon=None, periodic_lat=None) -> None:
+             step_i: int = 0, real_id: int = 0, periodic_lon=None, periodic_lat=None,
+             reuse_anchor: bool = False) -> None:
         """One step in place.  ``kick``: dict(kh_x, kh_y, seed[, az_const])
         for the split uniform-Kh random walk (and the vertical Az kick);
         ``beach``: dict of the beaching options (criteria, depth_min, ...)."""
         a = make_step_args(order, slot_a, dt, time_dir, slot_b, slot_t, alpha0, alphas,
                            exact_nearest, w_method, kick, beach, step_i, real_id,
-                           periodic_lon, periodic_lat)
+                           periodic_lon, periodic_lat, reuse_anchor)
         s = p.as_struct()
         check(self.lib.kelp_step(self.h, ctypes.byref(s), ctypes.byref(a)), "kelp_step")
 

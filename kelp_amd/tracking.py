@@ -330,8 +330,10 @@ def run_single_realization(mesh: Mesh, ds, reader, init_state: ParticleState, mo
           else np.asarray(start_wet_streak, np.int32))
     fr0 = (np.zeros(P, np.int8) if init_state.failure_reason is None
            else init_state.failure_reason)
+    # nn starts unknown (-1): every anchor the run stores is then its own,
+    # which lets each step reuse the previous step's end anchor
     p = ctx.particles(init_state.lon, init_state.lat, init_state.z, status=init_state.status,
-                      nn=init_state.nearest_node, fr=fr0, pid=pid, ws=ws)
+                      nn=None, fr=fr0, pid=pid, ws=ws)
     torch = ctx.torch
     release_step = init_state.release_step
     rel_dev = None if release_step is None else torch.as_tensor(
@@ -399,7 +401,8 @@ def run_single_realization(mesh: Mesh, ds, reader, init_state: ParticleState, mo
             ctx.sort(p)
         ctx.step(p, order, sa, step_len, float(direction), slot_b=sb, slot_t=st,
                  alpha0=pay.alpha0, alphas=alphas, exact_nearest=exact, w_method=w_method,
-                 kick=kick, beach=beach, step_i=g, real_id=realization_id, **periodic)
+                 kick=kick, beach=beach, step_i=g, real_id=realization_id, reuse_anchor=True,
+                 **periodic)
         if use_tinterp:
             sim_time = sim_time_end if sim_time_end is not None else sim_time + dt * direction
         if (g + 1) % max(int(numerics.output_freq), 1) == 0:

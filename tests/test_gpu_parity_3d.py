@@ -180,3 +180,30 @@ def test_3d_element_layout_lsq_bitwise(gpu_ctx, world):
                      kick=kick, beach=BEACH, step_i=k)
     h = p.to_host()
     _assert_bitwise(h, st)
+
+
+@pytest.mark.parametrize("exact", [1, 2])
+def test_reused_anchor_is_result_neutral(gpu_ctx, world, exact):
+    """Taking site 1's anchor from the previous step's stored one
+    (reuse_anchor) changes no bit of any output over a run with the Kh walk,
+    beaching and a periodic fold, in both reusable anchor modes."""
+    mesh, snaps, lon, lat, z, pid = world
+    gpu_ctx.upload_mesh(mesh)
+    for s, sn in enumerate(snaps):
+        gpu_ctx.upload_field(s, sn[0], sn[1], sn[2])
+    kick = dict(kh_x=10.0, kh_y=10.0, seed=17)
+    runs = []
+    for reuse in (False, True):
+        p = gpu_ctx.particles(lon, lat, z, pid=pid)
+        for k in range(12):
+            f, (ia, ib, it, a0, dtn) = _fields(snaps, k * DT, 1.0, np.float32)
+            al = [c * DT / dtn for c in (0.0, 0.5, 0.5, 1.0)]
+            gpu_ctx.step(p, 4, ia, DT, 1.0, slot_b=ib, slot_t=it, alpha0=a0, alphas=al,
+                         exact_nearest=exact, w_method=2, kick=kick, beach=BEACH, step_i=k,
+                         periodic_lon=(-170.0, 170.0), reuse_anchor=reuse)
+        runs.append(p.to_host())
+    a, b = runs
+    for k in ("lon", "lat", "z", "u", "v", "w"):
+        assert np.array_equal(a[k].view(np.int64), b[k].view(np.int64)), k
+    for k in ("status", "nn", "fr", "ws"):
+        assert np.array_equal(a[k], b[k]), k
