@@ -81,7 +81,7 @@ class Global3D:
         p = ctx.particles(self.w.lon, self.w.lat, self.w.z)
         if self.sample_ts:
             t = ctx.torch
-            self.ts_out = [t.empty((p.n,), dtype=t.float64, device=p.lon.device) for _ in range(2)]
+            self.ts_out = t.empty((2, p.n), dtype=t.float64, device=p.lon.device)
         return p
 
     def step(self, ctx, p, k, ev=None):
@@ -99,8 +99,7 @@ class Global3D:
         if self.sample_ts:
             # T and S at the step's end (the record instant)
             ra, rb, ra0, _, _ = self.synth.snapshot_bracket(t + w.dt, w.snap_dt, len(w.snaps))
-            for var in (0, 1):
-                ctx.sample(p, var, ra, rb, ra0, "leastsq", out=self.ts_out[var])
+            ctx.sample_vars(p, (0, 1), ra, rb, ra0, "leastsq", out=self.ts_out)
 
     def cpu_baseline(self, budget_s):
         from oracle.oracle import OracleMesh, step_args

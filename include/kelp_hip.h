@@ -163,6 +163,12 @@ int kelp_scalar_upload(kelp_ctx *ctx, int slot, int var, int64_t n_planes, const
 int kelp_sample(kelp_ctx *ctx, const kelp_particles *p, int var, int slot_a, int slot_b, double alpha,
                 int method, double *out);
 
+/* Several variables at once (vars[0..n_vars), host array): row k of `out`
+ * (device, n_vars x n doubles) is variable vars[k] as kelp_sample computes
+ * it; least-squares pairs of 2-D/mid-layer variables share one pass. */
+int kelp_sample_vars(kelp_ctx *ctx, const kelp_particles *p, int n_vars, const int32_t *vars,
+                     int slot_a, int slot_b, double alpha, int method, double *out);
+
 /* Advance every particle by one step, in place, on the context stream. */
 int kelp_step(kelp_ctx *ctx, const kelp_particles *p, const kelp_step_args *a);
 

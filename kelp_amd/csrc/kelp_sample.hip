@@ -300,6 +300,82 @@ __global__ void sample_kernel(DMesh m, SField s, Snap<double, true> fw, int meth
     out[i] = r;
 }
 
+// Two 1-ring scalars by least squares in one pass (the usual T and S pair):
+// the neighbour scan, coordinates and depth brackets are shared, each
+// variable keeps its own sums (non-finite values are filtered per
+// variable), so each output equals combine(..., method 1) of its variable.
+template <int MAXV>
+__global__ void sample2_lsq_kernel(DMesh m, SField s0, SField s1, SampleState st, double* __restrict__ out0,
+                                   double* __restrict__ out1)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= st.n) return;
+    const int nn = st.nn[i];
+    if (st.status[i] != kAlive || nn < 0) {
+        out0[i] = __builtin_nan("");
+        out1[i] = __builtin_nan("");
+        return;
+    }
+    const double lon_p = st.lon[i], lat_p = st.lat[i];
+    const double z = st.z ? st.z[i] : 0.0;
+    const bool mid = s0.n_planes > 1 || s1.n_planes > 1;
+    const MidBracket bz = mid ? mid_bracket(m, z) : MidBracket{0, -1.0};
+    const SField* sv[2] = {&s0, &s1};
+    double z0 = km_cos(lat_p * kDeg2Rad) * kSLM;
+    if (z0 < 1.0) z0 = 1.0;
+    double a[2][6] = {{0, 0, 0, 0, 0, 0}, {0, 0, 0, 0, 0, 0}};
+    double f[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    int n_fin[2] = {0, 0};
+    double fallback[2];
+    const int count = for_neighbours<MAXV>(m, nn, [&](int nd) {
+        MidBracket bn = bz;
+        if (mid) {
+            const double dep = m.depth[nd];
+            if (!(z < dep)) bn = mid_bracket(m, dep);
+        }
+        const double2 ll = m.lonlat[nd];
+        const double dlon = km_pymod360(ll.x - lon_p + 180.0) - 180.0;
+        const double ex = dlon * z0;
+        const double ey = (ll.y - lat_p) * kSLM;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const SField& s = *sv[q];
+            const double val = s.n_planes == 1 ? sval(s, nd) : mid_value(m, s, bn, nd);
+            if (nd == nn) fallback[q] = val;
+            if (!__builtin_isfinite(val)) continue;
+            ++n_fin[q];
+            a[q][0] += 1.0;
+            a[q][1] += ex;
+            a[q][2] += ey;
+            a[q][3] += ex * ex;
+            a[q][4] += ex * ey;
+            a[q][5] += ey * ey;
+            f[q][0] += val;
+            f[q][1] += val * ex;
+            f[q][2] += val * ey;
+        }
+    });
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        double r = fallback[q];
+        if (count >= 3 && n_fin[q] >= 3) {
+            const double a11 = a[q][0], a12 = a[q][1], a13 = a[q][2], a22 = a[q][3], a23 = a[q][4],
+                         a33 = a[q][5];
+            const double a21 = a12, a31 = a13, a32 = a23;
+            const double det = (a11 * a22 * a33 + a12 * a23 * a31 + a13 * a21 * a32 - a13 * a22 * a31 -
+                                a12 * a21 * a33 - a11 * a23 * a32);
+            if (!(km_fabs(det) < 1e-12 || km_fabs(det) < 1e-12 * km_fabs(a11 * a22 * a33))) {
+                const double inv = 1.0 / km_fabs(det);
+                const double b11 = a22 * a33 - a23 * a32;
+                const double b12 = a13 * a32 - a12 * a33;
+                const double b13 = a12 * a23 - a13 * a22;
+                r = (b11 * f[q][0] + b12 * f[q][1] + b13 * f[q][2]) * inv;
+            }
+        }
+        (q == 0 ? out0 : out1)[i] = r;
+    }
+}
+
 __global__ void permute_planes_kernel(int64_t N, int64_t n_planes, const int32_t* __restrict__ old_of_new,
                                       const double* __restrict__ in, double* __restrict__ out)
 {
@@ -396,5 +472,52 @@ extern "C" int kelp_sample(kelp_ctx* c, const kelp_particles* p, int var, int sl
     else
         sample_kernel<4><<<grid_of(p->n), 256, 0, c->stream>>>(c->m, s, fw, method, st, out);
     HIP_TRY(hipGetLastError());
+    return KELP_OK;
+}
+
+// Several variables of one record at once: rows of `out` (n_vars x n), in
+// the order of `vars`; 1-ring least-squares pairs share one pass.
+extern "C" int kelp_sample_vars(kelp_ctx* c, const kelp_particles* p, int n_vars, const int32_t* vars,
+                                int slot_a, int slot_b, double alpha, int method, double* out)
+{
+    if (!c || !p || !vars || !out || n_vars < 0) return kelp_fail(KELP_EINVAL, "kelp_sample_vars: bad argument");
+    if (!c->has_mesh) return kelp_fail(KELP_ESTATE, "kelp_sample_vars: no mesh uploaded");
+    int k = 0;
+    while (k < n_vars) {
+        const int va = vars[k];
+        if (va < 0 || va >= kMaxScalars) return kelp_fail(KELP_EINVAL, "kelp_sample_vars: bad variable %d", va);
+        double* oa = out + (int64_t)k * p->n;
+        const bool pair = method == 1 && k + 1 < n_vars && vars[k + 1] >= 0 && vars[k + 1] < kMaxScalars;
+        if (pair && slot_a >= 0 && slot_a < kMaxSlots && slot_b >= -1 && slot_b < kMaxSlots) {
+            const int vb = vars[k + 1];
+            const ScalarField& fa0 = c->slots[slot_a].scalars[va];
+            const ScalarField& fa1 = c->slots[slot_a].scalars[vb];
+            const ScalarField* fb0 = slot_b >= 0 ? &c->slots[slot_b].scalars[va] : nullptr;
+            const ScalarField* fb1 = slot_b >= 0 ? &c->slots[slot_b].scalars[vb] : nullptr;
+            const int64_t K = c->m.n_levels;
+            const auto ring = [&](const ScalarField& f) { return f.planes && !(f.n_planes == K && K >= 2); };
+            const auto blend_ok = [&](const ScalarField& f, const ScalarField* g) {
+                return alpha == 0.0 || (g && g->planes && g->n_planes == f.n_planes);
+            };
+            if (ring(fa0) && ring(fa1) && blend_ok(fa0, fb0) && blend_ok(fa1, fb1) && p->n > 0 &&
+                p->lon && p->lat && p->status && p->nn && (p->z || (fa0.n_planes == 1 && fa1.n_planes == 1))) {
+                HIP_TRY(hipSetDevice(c->device));
+                SField s0{fa0.planes, (fb0 && fb0->planes) ? fb0->planes : fa0.planes, alpha, fa0.n_planes};
+                SField s1{fa1.planes, (fb1 && fb1->planes) ? fb1->planes : fa1.planes, alpha, fa1.n_planes};
+                SampleState st{p->lon, p->lat, p->z, p->status, p->nn, p->n};
+                double* ob = out + (int64_t)(k + 1) * p->n;
+                if (c->m.max_v == 3)
+                    sample2_lsq_kernel<3><<<grid_of(p->n), 256, 0, c->stream>>>(c->m, s0, s1, st, oa, ob);
+                else
+                    sample2_lsq_kernel<4><<<grid_of(p->n), 256, 0, c->stream>>>(c->m, s0, s1, st, oa, ob);
+                HIP_TRY(hipGetLastError());
+                k += 2;
+                continue;
+            }
+        }
+        int rc = kelp_sample(c, p, va, slot_a, slot_b, alpha, method, oa);
+        if (rc) return rc;
+        k += 1;
+    }
     return KELP_OK;
 }

@@ -223,6 +223,21 @@ class DeviceContext:
         self.synchronize()
         return dev_out.cpu().numpy()
 
+    def sample_vars(self, p: "DeviceParticles", variables, slot_a: int, slot_b: int = -1,
+                    alpha: float = 0.0, method: str = "leastsq", out=None):
+        """Several scalars at once: a (len(variables), n) device tensor
+        (``out`` if given), row k = variable variables[k]; no wait."""
+        codes = {"nearest": 0, "leastsq": 1, "barycentric": 2, "idw": 3}
+        t = self.torch
+        vs = np.ascontiguousarray(variables, np.int32)
+        if out is None:
+            out = t.empty((vs.size, p.n), dtype=t.float64, device=p.lon.device)
+        s = p.as_struct()
+        check(self.lib.kelp_sample_vars(self.h, ctypes.byref(s), int(vs.size), vs.ctypes.data,
+                                        int(slot_a), int(slot_b), float(alpha), codes[method],
+                                        out.data_ptr()), "kelp_sample_vars")
+        return out
+
     def upload_field(self, slot: int, u, v, w=None, key=None, uv_layout="node") -> None:
         """Upload one snapshot into ``slot``; skipped when ``key`` matches the
         slot's current content.  ``uv_layout``: "node" (u, v per node) or

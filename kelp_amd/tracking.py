@@ -424,9 +424,12 @@ def run_single_realization(mesh: Mesh, ds, reader, init_state: ParticleState, mo
                         nxt %= n_time_total
                     if 0 <= nxt < n_time_total:
                         ra, rb, ral = slots.slot(nxt), -1, 0.0
+                rows = ctx.sample_vars(p, range(len(sample_vars)), ra, rb, ral,
+                                       numerics.sample_interp)
+                ctx.synchronize()
+                rows = rows.cpu().numpy()
                 for k, name in enumerate(sample_vars):
-                    sampled["sample__" + name] = ctx.sample(p, k, ra, rb, ral,
-                                                            numerics.sample_interp)
+                    sampled["sample__" + name] = rows[k]
             h = p.to_host()
             prev = prev_status.cpu().numpy()
             order_idx = np.argsort(h["pid"], kind="stable")
