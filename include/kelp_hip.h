@@ -163,7 +163,15 @@ int kelp_scalar_upload(kelp_ctx *ctx, int slot, int var, int64_t n_planes, const
 int kelp_sample(kelp_ctx *ctx, const kelp_particles *p, int var, int slot_a, int slot_b, double alpha,
                 int method, double *out);
 
-/* Several variables at once (vars[0..n_vars), host array): row k of `out`
+/* Release (core/release.py:112 activate_released): status NOT_RELEASED (4)
+ * becomes ALIVE where step >= release_step[pid - pid_offset] (device int64,
+ * n_release entries; NULL = no release schedule).  prev_status (device,
+ * n int32, or NULL) receives every particle's status after activation,
+ * the step-start status the record masks of core/release.py:132 use. */
+int kelp_activate(kelp_ctx *ctx, const kelp_particles *p, const int64_t *release_step, int64_t pid_offset,
+                  int64_t n_release, int64_t step, int32_t *prev_status);
+
+/* Several variables at once/* Several variables at once (vars[0..n_vars), host array): row k of `out`
  * (device, n_vars x n doubles) is variable vars[k] as kelp_sample computes
  * it; least-squares pairs of 2-D/mid-layer variables share one pass. */
 int kelp_sample_vars(kelp_ctx *ctx, const kelp_particles *p, int n_vars, const int32_t *vars,

@@ -88,6 +88,8 @@ SIGNATURES = {
     "kelp_scalar_upload": (c_int, [c_void_p, c_int, c_int, c_int64, c_void_p]),
     "kelp_sample": (c_int, [c_void_p, POINTER(Particles), c_int, c_int, c_int, c_double, c_int,
                             c_void_p]),
+    "kelp_activate": (c_int, [c_void_p, POINTER(Particles), c_void_p, c_int64, c_int64, c_int64,
+                              c_void_p]),
     "kelp_sample_vars": (c_int, [c_void_p, POINTER(Particles), c_int, c_void_p, c_int, c_int,
                                  c_double, c_int, c_void_p]),
     "kelp_step": (c_int, [c_void_p, POINTER(Particles), POINTER(StepArgs)]),

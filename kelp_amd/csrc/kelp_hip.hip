@@ -594,6 +594,42 @@ int kelp_nodes_from_mesh_ids(kelp_ctx* c, int64_t n, int32_t* ids)
     return KELP_OK;
 }
 
+// core/release.py:112 activate_released: NOT_RELEASED -> ALIVE once the
+// global step reaches the particle's release step (indexed by pid -
+// pid_offset, so the particle order on the device does not matter); the
+// status at the step start is copied to prev_status for the record masks.
+__global__ void activate_kernel(int64_t n, int32_t* __restrict__ status, const int64_t* __restrict__ pid,
+                                const int64_t* __restrict__ release_step, int64_t pid_offset, int64_t n_release,
+                                int64_t step, int32_t* __restrict__ prev_status)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int32_t s = status[i];
+    if (release_step && s == 4) {
+        const int64_t k = pid[i] - pid_offset;
+        if (k >= 0 && k < n_release && step >= release_step[k]) {
+            s = kAlive;
+            status[i] = s;
+        }
+    }
+    if (prev_status) prev_status[i] = s;
+}
+
+int kelp_activate(kelp_ctx* c, const kelp_particles* p, const int64_t* release_step, int64_t pid_offset,
+                  int64_t n_release, int64_t step, int32_t* prev_status)
+{
+    if (!c || !p) return fail(KELP_EINVAL, "kelp_activate: NULL argument");
+    if (p->n < 0 || !p->status) return fail(KELP_EINVAL, "kelp_activate: particles need status");
+    if (release_step && (!p->pid || n_release < 0))
+        return fail(KELP_EINVAL, "kelp_activate: release steps need pid and n_release >= 0");
+    if (p->n == 0 || (!release_step && !prev_status)) return KELP_OK;
+    HIP_TRY(hipSetDevice(c->device));
+    activate_kernel<<<grid_for(p->n, 256), 256, 0, c->stream>>>(p->n, p->status, p->pid, release_step, pid_offset,
+                                                               n_release, step, prev_status);
+    HIP_TRY(hipGetLastError());
+    return KELP_OK;
+}
+
 // Upload one snapshot: u/v planes at the nodes (layout 0) or at the element
 // centres (layout 1), W planes at the nodes; packed on the device.
 static int field_upload(kelp_ctx* c, const char* fn, int slot, int dtype, int layout, int64_t n_planes,

@@ -312,6 +312,19 @@ class DeviceContext:
         s = p.as_struct()
         check(self.lib.kelp_step(self.h, ctypes.byref(s), ctypes.byref(a)), "kelp_step")
 
+    def activate(self, p: DeviceParticles, release_step, pid_offset: int, step: int,
+                 prev_status=None) -> None:
+        """Release NOT_RELEASED particles whose release step has come
+        (``release_step``: device int64 tensor indexed by pid - pid_offset,
+        or None) and copy the resulting status into ``prev_status`` (device
+        int32 tensor or None); no wait."""
+        s = p.as_struct()
+        rs = 0 if release_step is None else release_step.data_ptr()
+        nr = 0 if release_step is None else int(release_step.numel())
+        ps = 0 if prev_status is None else prev_status.data_ptr()
+        check(self.lib.kelp_activate(self.h, ctypes.byref(s), rs, int(pid_offset), nr, int(step), ps),
+              "kelp_activate")
+
     def sort(self, p: DeviceParticles) -> None:
         """Reorder the particle arrays by grid-hash cell (see kelp_sort_particles)."""
         s = p.as_struct()

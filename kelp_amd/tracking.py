@@ -338,6 +338,7 @@ def run_single_realization(mesh: Mesh, ds, reader, init_state: ParticleState, mo
     release_step = init_state.release_step
     rel_dev = None if release_step is None else torch.as_tensor(
         np.asarray(release_step, np.int64), device=p.lon.device)
+    prev_status = torch.empty_like(p.status)
     rec = writer if writer is not None else TrajectoryRecord()
     sim_time = None
     if use_tinterp:
@@ -348,10 +349,7 @@ def run_single_realization(mesh: Mesh, ds, reader, init_state: ParticleState, mo
 
     for step_i in range(n_steps):
         g = start_step_i + step_i
-        if rel_dev is not None:
-            act = (p.status == int(ParticleStatus.NOT_RELEASED)) & (rel_dev[p.pid - particle_offset] <= g)
-            p.status[act] = int(ParticleStatus.ALIVE)
-        prev_status = p.status.clone()
+        ctx.activate(p, rel_dev, particle_offset, g, prev_status)
         if use_tinterp:
             bracket = resolve_time_bracket(sim_time, file_times, n_time_total, on_exhausted,
                                            direction, snap)

@@ -58,3 +58,65 @@ def test_sampling_run_matches_reference_golden(gpu_ctx, direction):
     for k, src in (("lon", "lon"), ("lat", "lat"), ("z", "z"), ("u", "vel_u"), ("w", "vel_w"),
                    ("sample__s_lev", "sample__s_lev"), ("sample__s_mid", "sample__s_mid")):
         np.testing.assert_allclose(rec[src], gold[k], rtol=1e-9, atol=1e-9, err_msg=k)
+
+
+def test_activate_kernel_matches_release_rule(gpu_ctx):
+    """kelp_activate against core/release.py:112 (activate_released), with
+    the particles in shuffled pid order on the device."""
+    from kelp_amd import synth
+    from kelp_amd.tracking import activate_released
+    mesh = synth.sphere_mesh(2000)
+    gpu_ctx.upload_mesh(mesh)
+    rng = np.random.default_rng(3)
+    P, off = 5000, 700
+    rel = rng.integers(0, 20, P).astype(np.int64)
+    status = rng.choice([1, 2, 3, 4, 4, 4], P).astype(np.int32)
+    perm = rng.permutation(P)
+    lon, lat = np.zeros(P), np.zeros(P)
+    p = gpu_ctx.particles(lon, lat, None, status=status[perm], pid=(off + perm).astype(np.int64))
+    torch = gpu_ctx.torch
+    rel_dev = torch.as_tensor(rel, device=p.lon.device)
+    prev = torch.empty_like(p.status)
+    want = status.copy()
+    for g in (0, 5, 5, 19, 30):
+        gpu_ctx.activate(p, rel_dev, off, g, prev)
+        gpu_ctx.synchronize()
+        want = activate_released(want, rel, g)
+        np.testing.assert_array_equal(p.status.cpu().numpy(), want[perm])
+        np.testing.assert_array_equal(prev.cpu().numpy(), want[perm])
+
+
+def test_staggered_release_run(gpu_ctx):
+    """A release schedule in the device loop: dormant particles are masked
+    and do not move, particles released at step 0 follow the run without a
+    schedule bit for bit (the RNG is keyed by pid), later ones start from
+    their seed position at their release step."""
+    case = rc.beaching_case(1.0)
+    init = case["init"]
+    P = init.n_particles
+    rel = np.zeros(P, np.int64)
+    rel[1::3] = 4
+    rel[2::3] = 9
+
+    def run(state):
+        return run_single_realization(
+            case["mesh"], case["ds"], get_reader("fesom2_arrays"), state, case["mode"],
+            case["numerics"], start_time_idx=case["start_time_idx"], n_steps=case["n_steps"],
+            n_time_total=case["n_time_total"], file_times=case["file_times"],
+            file_dt=case["file_dt"], ctx=gpu_ctx).as_arrays()
+
+    status0 = init.status.copy()
+    status0[rel > 0] = 4
+    base = run(init)
+    st = run(init.replace(status=status0, release_step=rel))
+    now = rel == 0
+    for k in ("lon", "lat", "z", "status", "vel_u", "vel_w"):
+        np.testing.assert_array_equal(st[k][now], base[k][now], err_msg=k)
+    for r in (4, 9):
+        sel = rel == r
+        # record j holds the state after global step j (records every step)
+        assert (st["status"][sel, :r] == 4).all()
+        assert np.isnan(st["lon"][sel, :r]).all() and np.isnan(st["vel_u"][sel, :r]).all()
+        assert (st["status"][sel, r] != 4).all()
+        moved = st["lon"][sel, r] != init.lon[sel]
+        assert moved[st["status"][sel, r] == 1].all()
