@@ -63,9 +63,12 @@ __device__ MidBracket mid_bracket(const DMesh& m, double z)
 
 __device__ __forceinline__ double mid_value(const DMesh& m, const SField& s, const MidBracket& b, int node)
 {
-    const int64_t N = m.n_nodes;
-    if (b.al < 0.0) return sval(s, (int64_t)b.k * N + node);
-    return (1.0 - b.al) * sval(s, (int64_t)b.k * N + node) + b.al * sval(s, (int64_t)(b.k + 1) * N + node);
+    // mid-layer planes are stored per node (column layout): planes k and
+    // k + 1 of one node share a cache line, and particles near one node at
+    // nearby depths share its column
+    const int64_t col = (int64_t)node * s.n_planes;
+    if (b.al < 0.0) return sval(s, col + b.k);
+    return (1.0 - b.al) * sval(s, col + b.k) + b.al * sval(s, col + b.k + 1);
 }
 
 // The value the horizontal combination uses at node nd (sampling.py:517):
@@ -376,13 +379,20 @@ __global__ void sample2_lsq_kernel(DMesh m, SField s0, SField s1, SampleState st
     }
 }
 
+// host planes (n_planes x N, mesh order) into internal node order: plane
+// layout (n_planes x N) or, with `column`, one column per node (N x n_planes)
 __global__ void permute_planes_kernel(int64_t N, int64_t n_planes, const int32_t* __restrict__ old_of_new,
-                                      const double* __restrict__ in, double* __restrict__ out)
+                                      const double* __restrict__ in, double* __restrict__ out, int column)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N * n_planes) return;
-    const int64_t L = i / N, k = i - L * N;
-    out[i] = in[L * N + old_of_new[k]];
+    if (column) {
+        const int64_t k = i / n_planes, L = i - k * n_planes;
+        out[i] = in[L * N + old_of_new[k]];
+    } else {
+        const int64_t L = i / N, k = i - L * N;
+        out[i] = in[L * N + old_of_new[k]];
+    }
 }
 
 // interface planes (internal order) into (0, 0, top, bottom) records per layer
@@ -423,7 +433,7 @@ extern "C" int kelp_scalar_upload(kelp_ctx* c, int slot, int var, int64_t n_plan
     if (e == hipSuccess) e = hipMemcpyAsync(tmp, host_values, bytes, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) {
         permute_planes_kernel<<<grid_of(N * n_planes), 256, 0, c->stream>>>(N, n_planes, c->node_old_of_new, tmp,
-                                                                          f.planes);
+                                                                          f.planes, (!ifc && n_planes > 1) ? 1 : 0);
         if (ifc)
             pair_records_kernel<<<grid_of(N * (K - 1)), 256, 0, c->stream>>>(N, K - 1, f.planes,
                                                                            (double4*)f.wrec);
