@@ -53,6 +53,12 @@ typedef struct {
     const uint8_t *host_elem_coastal;              /* (E,) coastal flags (core/mesh.py:2765) or NULL */
     const double *host_elem_center_lon;            /* (E,) element centres (Mesh.elem_center_lon/lat); */
     const double *host_elem_center_lat;            /*   needed only for element-centre u/v */
+    int64_t vertical_coord;                        /* 0 z-level (host_z_levels = interface
+                                                      depths, non-decreasing); 1 sigma
+                                                      (Mesh.vertical_coord == "sigma",
+                                                      host_z_levels = sigma_lev, 1 at the
+                                                      surface .. 0 at the bottom,
+                                                      non-increasing) */
 } kelp_mesh_desc;
 
 /* Particle state, device memory, structure of arrays (the fields of the
@@ -83,7 +89,7 @@ typedef struct {
                                       0 guard off, fused kernels' widened window;
                                       2 guard off, batch 3x3 window healed exactly */
     int32_t w_method;              /* 3-D W: 0 linear at the anchor ("nearest"), 1 leastsq,
-                                      2 barycentric */
+                                      2 barycentric; sigma meshes also 3 idw */
     int32_t kick;                  /* split uniform-Kh random walk (integrators.py:673) */
     int32_t beach;                 /* beaching hook */
     double kh_x, kh_y;             /* m^2/s */
@@ -109,6 +115,20 @@ typedef struct {
                                       mode with the positions left alone; set nn = -1 for
                                       particles placed or released since): the step skips
                                       that search.  Ignored in mode 0. */
+    /* Sigma meshes (integrators.py:1310-1441: the step integrates the sigma
+     * fraction s = z / H_p with H_p = max(depth + eta, Dmin) interpolated like
+     * W; vertical_sigma.py).  The surface elevation at the step start is
+     * the time blend of the eta of slots (a, b, t) with fractions
+     * (alpha0, alpha) -- the nested blend of the velocities; b/t -1 when
+     * unused -- and likewise at the step end (sig_end_a = -1: the start
+     * surface).  Stage surfaces follow the stages' velocity blend, or on a
+     * steady step (slot_t = -1) the blend of the start and end surfaces. */
+    int32_t sig_start_a, sig_start_b, sig_start_t;
+    double sig_start_alpha0, sig_start_alpha;
+    int32_t sig_end_a, sig_end_b, sig_end_t;
+    double sig_end_alpha0, sig_end_alpha;
+    double Dmin;                   /* minimum column height (m) */
+    double z0b_min;                /* distance kept above the bed (m) */
 } kelp_step_args;
 
 const char *kelp_last_error(void);
@@ -171,7 +191,7 @@ int kelp_sample(kelp_ctx *ctx, const kelp_particles *p, int var, int slot_a, int
 int kelp_activate(kelp_ctx *ctx, const kelp_particles *p, const int64_t *release_step, int64_t pid_offset,
                   int64_t n_release, int64_t step, int32_t *prev_status);
 
-/* Several variables at once/* Several variables at once (vars[0..n_vars), host array): row k of `out`
+/* Several variables at once (vars[0..n_vars), host array): row k of `out`
  * (device, n_vars x n doubles) is variable vars[k] as kelp_sample computes
  * it; least-squares pairs of 2-D/mid-layer variables share one pass. */
 int kelp_sample_vars(kelp_ctx *ctx, const kelp_particles *p, int n_vars, const int32_t *vars,

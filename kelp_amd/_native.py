@@ -33,6 +33,7 @@ class MeshDesc(ctypes.Structure):
         ("n_levels", c_int64), ("host_z_levels", c_void_p),
         ("host_elem_coastal", c_void_p),
         ("host_elem_center_lon", c_void_p), ("host_elem_center_lat", c_void_p),
+        ("vertical_coord", c_int64),
     ]
 
 
@@ -64,6 +65,11 @@ class StepArgs(ctypes.Structure):
         ("periodic_lon", c_int32), ("periodic_lat", c_int32),
         ("px0", c_double), ("px1", c_double), ("py0", c_double), ("py1", c_double),
         ("reuse_anchor", c_int32),
+        ("sig_start_a", c_int32), ("sig_start_b", c_int32), ("sig_start_t", c_int32),
+        ("sig_start_alpha0", c_double), ("sig_start_alpha", c_double),
+        ("sig_end_a", c_int32), ("sig_end_b", c_int32), ("sig_end_t", c_int32),
+        ("sig_end_alpha0", c_double), ("sig_end_alpha", c_double),
+        ("Dmin", c_double), ("z0b_min", c_double),
     ]
 
 

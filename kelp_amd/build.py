@@ -31,6 +31,8 @@ HIPCC_FLAGS = [
 #: (object name, source, extra defines)
 UNITS = [("kelp_hip", "kelp_hip.hip", []), ("kelp_sort", "kelp_sort.hip", []),
          ("kelp_sample", "kelp_sample.hip", [])] + [
+    (f"kelp_sigma_{'f32' if f32 else 'f64'}_{'elem' if el else 'node'}", "kelp_step_sigma.hip",
+     [f"-DKS_F32={f32}", f"-DKS_ELEM={el}"]) for f32 in (0, 1) for el in (0, 1)] + [
     (f"kelp_step_{'3d' if d3 else '2d'}_{'f32' if f32 else 'f64'}_{'elem' if el else 'node'}",
      "kelp_step_variant.hip", [f"-DKV_IS3D={d3}", f"-DKV_F32={f32}", f"-DKV_ELEM={el}"])
     for d3 in (0, 1) for f32 in (0, 1) for el in (0, 1)

@@ -68,6 +68,8 @@ static void free_mesh(kelp_ctx* c)
     c->has_mesh = false;
     c->m = DMesh{};
     c->node_old_of_new = c->node_new_of_old = c->elem_old_of_new = c->elem_new_of_old = nullptr;
+    c->vertical_coord = 0;
+    c->sig_eta = nullptr;
 }
 
 void kelp_free_scalar(ScalarField& f)
@@ -289,11 +291,30 @@ static void launch_step_t(kelp_ctx* c, int order, const FieldSlot& a, const Fiel
         launch_step_v<4, IS3D, T, ELEM>(c->stream, order, c->m, f, fe, cfg, ps);
 }
 
+template <typename T, bool ELEM>
+static void launch_sigma_t(kelp_ctx* c, int order, const FieldSlot& a, const FieldSlot* b, const FieldSlot* t,
+                           const StepCfg& cfg, const PState& ps)
+{
+    Snap<T, true> f = snap_of<T, true>(a.rec, b ? b->rec : nullptr, t ? t->rec : nullptr);
+    Snap<T, false> fe = snap_of<T, false>(a.erec, b ? b->erec : nullptr, t ? t->erec : nullptr);
+    if (c->m.max_v == 3)
+        launch_sigma_step<3, T, ELEM>(c->stream, order, c->m, f, fe, cfg, ps);
+    else
+        launch_sigma_step<4, T, ELEM>(c->stream, order, c->m, f, fe, cfg, ps);
+}
+
 template <typename T>
 static void launch_step(kelp_ctx* c, int order, bool is3d, const FieldSlot& a, const FieldSlot* b,
                         const FieldSlot* t, const StepCfg& cfg, const PState& ps)
 {
     const bool elem = a.layout == 1;
+    if (c->vertical_coord == 1) {
+        if (elem)
+            launch_sigma_t<T, true>(c, order, a, b, t, cfg, ps);
+        else
+            launch_sigma_t<T, false>(c, order, a, b, t, cfg, ps);
+        return;
+    }
     if (is3d && elem)
         launch_step_t<true, T, true>(c, order, a, b, t, cfg, ps);
     else if (is3d)
@@ -392,9 +413,14 @@ int kelp_mesh_upload(kelp_ctx* c, const kelp_mesh_desc* d)
         return fail(KELP_EINVAL, "kelp_mesh_upload: grid hash offsets do not span the nodes");
     if (d->n_levels < 0 || (d->n_levels > 0 && !d->host_z_levels))
         return fail(KELP_EINVAL, "kelp_mesh_upload: bad vertical levels");
-    for (int64_t k = 1; k < d->n_levels; ++k)
-        if (!(d->host_z_levels[k] >= d->host_z_levels[k - 1]))
+    if (d->vertical_coord != 0 && d->vertical_coord != 1)
+        return fail(KELP_EINVAL, "kelp_mesh_upload: vertical_coord must be 0 (z-level) or 1 (sigma)");
+    for (int64_t k = 1; k < d->n_levels; ++k) {
+        if (d->vertical_coord == 0 && !(d->host_z_levels[k] >= d->host_z_levels[k - 1]))
             return fail(KELP_EINVAL, "kelp_mesh_upload: z levels must be non-decreasing");
+        if (d->vertical_coord == 1 && !(d->host_z_levels[k] <= d->host_z_levels[k - 1]))
+            return fail(KELP_EINVAL, "kelp_mesh_upload: sigma levels must be non-increasing (1 at the surface)");
+    }
     HIP_TRY(hipSetDevice(c->device));
     free_mesh(c);
 
@@ -486,6 +512,7 @@ int kelp_mesh_upload(kelp_ctx* c, const kelp_mesh_desc* d)
     double4* d_egeo = nullptr;
     double2* d_exy = nullptr;
     uint8_t* d_eflags = nullptr;
+    double* d_sig_eta = nullptr;
     int rc;
     if ((rc = dev_alloc(c, &d_lonlat, N)) || (rc = dev_alloc(c, &d_xyz, N)) ||
         (rc = dev_alloc(c, &d_off, n_cells + 1)) || (rc = dev_alloc(c, &d_elems, E)) ||
@@ -495,7 +522,8 @@ int kelp_mesh_upload(kelp_ctx* c, const kelp_mesh_desc* d)
         (rc = dev_alloc(c, &d_elemo, E)) || (rc = dev_alloc(c, &d_elemn, E)) ||
         (d->host_elem_coastal && (rc = dev_alloc(c, &d_coastal, E))) ||
         (has_centres && (rc = dev_alloc(c, &d_ecent, E))) || (rc = dev_alloc(c, &d_egeo, 2 * E)) ||
-        (rc = dev_alloc(c, &d_exy, 4 * E)) || (rc = dev_alloc(c, &d_eflags, E))) {
+        (rc = dev_alloc(c, &d_exy, 4 * E)) || (rc = dev_alloc(c, &d_eflags, E)) ||
+        (d->vertical_coord == 1 && (rc = dev_alloc(c, &d_sig_eta, 5 * N)))) {
         free_mesh(c);
         return rc;
     }
@@ -554,6 +582,8 @@ int kelp_mesh_upload(kelp_ctx* c, const kelp_mesh_desc* d)
     c->node_new_of_old = d_noden;
     c->elem_old_of_new = d_elemo;
     c->elem_new_of_old = d_elemn;
+    c->vertical_coord = (int)d->vertical_coord;
+    c->sig_eta = d_sig_eta;
     c->has_mesh = true;
     return KELP_OK;
 }
@@ -767,6 +797,72 @@ int kelp_field_clear(kelp_ctx* c, int slot)
     return KELP_OK;
 }
 
+// The eta of slot k (internal node order), or null when the slot or its eta
+// is missing.
+static const double* slot_eta(kelp_ctx* c, int k)
+{
+    if (k < 0 || k >= kMaxSlots || c->slots[k].dtype < 0) return nullptr;
+    return c->slots[k].eta;
+}
+
+// A nested blend of slot surfaces; every source the blend reads must exist.
+static int eta_source(kelp_ctx* c, const char* what, int sa, int sb, int st, double alpha0, double alpha,
+                      EtaSrc& out)
+{
+    out = EtaSrc{slot_eta(c, sa), slot_eta(c, sb), slot_eta(c, st), alpha0, alpha};
+    const bool need_t = alpha >= 1e-12;
+    const bool need_a = alpha <= 1.0 - 1e-12;
+    const bool need_b = need_a && alpha0 != 0.0;
+    if ((need_a && !out.a) || (need_b && !out.b) || (need_t && !out.t))
+        return fail(KELP_ESTATE, "kelp_step: the sigma %s surface needs eta on slots (%d, %d, %d)", what, sa, sb, st);
+    if (!out.a) out.a = out.t;
+    if (!out.b) out.b = out.a;
+    if (!out.t) out.t = out.a;
+    return KELP_OK;
+}
+
+// Sigma meshes: check the step, build the stage surfaces of this step in
+// the context's buffer and point the configuration at them.
+static int sigma_prepare(kelp_ctx* c, const kelp_step_args* a, bool is3d, const FieldSlot* sb,
+                         const FieldSlot* st, StepCfg& cfg)
+{
+    if (!is3d) return fail(KELP_EINVAL, "kelp_step: a sigma mesh runs 3-D fields (W on the sigma interfaces)");
+    if (a->kick || a->az || a->beach)
+        return fail(KELP_EINVAL, "kelp_step: the diffusion kicks and beaching on sigma meshes are not implemented");
+    if (!(a->Dmin > 0.0) || !(a->z0b_min >= 0.0))
+        return fail(KELP_EINVAL, "kelp_step: sigma steps need Dmin > 0 and z0b_min >= 0");
+    const bool steady = a->slot_t < 0;
+    EtaSrc start, end, stages;
+    int rc = eta_source(c, "start", a->sig_start_a, a->sig_start_b, a->sig_start_t, a->sig_start_alpha0,
+                        a->sig_start_alpha, start);
+    if (rc) return rc;
+    const bool end_is_start = a->sig_end_a < 0;
+    if (!end_is_start) {
+        rc = eta_source(c, "end", a->sig_end_a, a->sig_end_b, a->sig_end_t, a->sig_end_alpha0, a->sig_end_alpha,
+                        end);
+        if (rc) return rc;
+    } else {
+        end = start;
+    }
+    stages = start;
+    if (!steady) {
+        // stage k's surface is the eta of stage k's velocity blend
+        for (int k = 1; k < a->order; ++k) {
+            rc = eta_source(c, "stage", a->slot_a, sb ? a->slot_b : -1, a->slot_t, a->alpha0, a->alphas[k], stages);
+            if (rc) return rc;
+        }
+    }
+    (void)st;
+    const int64_t N = c->m.n_nodes;
+    launch_sigma_eta(c->stream, N, a->order, steady ? 1 : 0, start, end, end_is_start ? 1 : 0, stages, a->alphas,
+                     c->sig_eta);
+    HIP_TRY(hipGetLastError());
+    for (int k = 0; k < 5; ++k) cfg.sig_eta[k] = c->sig_eta + (int64_t)k * N;
+    cfg.Dmin = a->Dmin;
+    cfg.z0b_min = a->z0b_min;
+    return KELP_OK;
+}
+
 int kelp_step(kelp_ctx* c, const kelp_particles* p, const kelp_step_args* a)
 {
     if (!c || !p || !a) return fail(KELP_EINVAL, "kelp_step: NULL argument");
@@ -775,8 +871,10 @@ int kelp_step(kelp_ctx* c, const kelp_particles* p, const kelp_step_args* a)
         return fail(KELP_EINVAL, "kelp_step: order must be 1, 2 or 4 (got %d)", a->order);
     if (!(a->dt > 0.0)) return fail(KELP_EINVAL, "kelp_step: dt must be positive (direction goes in time_dir)");
     if (a->time_dir != 1.0 && a->time_dir != -1.0) return fail(KELP_EINVAL, "kelp_step: time_dir must be +1 or -1");
-    if (a->w_method < 0 || a->w_method > 2)
-        return fail(KELP_EINVAL, "kelp_step: w_method must be 0 (nearest), 1 (leastsq) or 2 (barycentric)");
+    const bool sigma = c->vertical_coord == 1;
+    if (a->w_method < 0 || a->w_method > (sigma ? 3 : 2))
+        return fail(KELP_EINVAL, "kelp_step: w_method must be 0 (nearest), 1 (leastsq), 2 (barycentric)%s",
+                    sigma ? " or 3 (idw)" : "");
     if (p->n < 0) return fail(KELP_EINVAL, "kelp_step: negative particle count");
     if (p->n == 0) return KELP_OK;
     if (!p->lon || !p->lat || !p->status || !p->nn || !p->fr || !p->u || !p->v)
@@ -867,6 +965,10 @@ int kelp_step(kelp_ctx* c, const kelp_particles* p, const kelp_step_args* a)
         cfg.use_eta = 1;
         cfg.eta_a = sa.eta;
         cfg.eta_b = sb ? sb->eta : sa.eta;
+    }
+    if (sigma) {
+        int rc = sigma_prepare(c, a, is3d, sb, st, cfg);
+        if (rc) return rc;
     }
     PState ps{p->lon, p->lat, p->z, p->status, p->nn, p->fr, p->u, p->v, p->w, p->pid, p->wet_streak,
               reinterpret_cast<unsigned long long*>(p->alive_count), p->n};

@@ -68,6 +68,8 @@ struct kelp_ctx {
     int32_t* node_new_of_old = nullptr;
     int32_t* elem_old_of_new = nullptr;
     int32_t* elem_new_of_old = nullptr;
+    int vertical_coord = 0;        // 0 z-level, 1 sigma
+    double* sig_eta = nullptr;     // sigma meshes: 5 x N surfaces of one step (mesh-owned)
     FieldSlot slots[kMaxSlots];
     DevBuf sort_tmp, sort_keys, sort_keys2, sort_idx, sort_perm, sort_stage;
 };

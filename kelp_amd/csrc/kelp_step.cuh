@@ -32,6 +32,10 @@ struct StepCfg {
     int periodic_lon, periodic_lat;  // post-step fold windows [x0, x1), [y0, y1)
     double px0, px1, py0, py1;
     int reuse_anchor;  // an ALIVE particle's nn >= 0 is the anchor of its position
+    // sigma meshes: surface elevation per node of stages 0..3 and of the step
+    // end (internal node order), column floor and bed distance
+    const double* sig_eta[5];
+    double Dmin, z0b_min;
 };
 
 struct PState {
@@ -57,5 +61,21 @@ struct PState {
 template <int MAXV, bool IS3D, typename T, bool ELEM>
 void launch_step_v(hipStream_t stream, int order, const DMesh& m, const Snap<T, IS3D>& f,
                    const Snap<T, false>& fe, const StepCfg& cfg, const PState& ps);
+
+// The sigma-mesh step (3-D): launch sigma_step_kernel<order, MAXV, T, ELEM>.
+template <int MAXV, typename T, bool ELEM>
+void launch_sigma_step(hipStream_t stream, int order, const DMesh& m, const Snap<T, true>& f,
+                       const Snap<T, false>& fe, const StepCfg& cfg, const PState& ps);
+
+// Surfaces of one sigma step into out[0..4] (stages 0..3, end), N each:
+// start/end/stage sources are nested time blends of slot eta arrays.
+struct EtaSrc {
+    const double* a;
+    const double* b;
+    const double* t;
+    double alpha0, alpha;
+};
+void launch_sigma_eta(hipStream_t stream, int64_t n_nodes, int order, int steady, EtaSrc start, EtaSrc end,
+                      int end_is_start, EtaSrc stages, const double* stage_alphas, double* out);
 
 }  // namespace kelp

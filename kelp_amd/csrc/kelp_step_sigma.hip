@@ -1,0 +1,267 @@
+// kelp_step_sigma.hip -- the 3-D step on sigma meshes (FESOM-C).
+//
+// The reference runs sigma meshes through its multi-pass steppers
+// (integrators.py:4997 _euler_step_unfused, :5380 _rk2_step_unfused, :5760
+// _rk4_step_unfused, sigma branches): every stage anchors, interpolates the
+// column height H_p like W (vertical_sigma.py:1063 sigma_column_height),
+// looks the layer up in that column, reads u/v on the layer and the
+// dia-sigma flux W, and the step integrates the sigma fraction s = z / H_p;
+// the particle ends at z = s_new H_p(end) clamped into its end column.  Here
+// one thread carries one particle through all stages.
+//
+// Compiled once per (field dtype, u/v layout): -DKS_F32=0|1 -DKS_ELEM=0|1;
+// the (double, node) unit also holds the stage-surface kernel.
+#include "kelp_sigma.cuh"
+#include "kelp_step_impl.cuh"
+
+#if !defined(KS_F32) || !defined(KS_ELEM)
+#error "compile with -DKS_F32=0|1 -DKS_ELEM=0|1"
+#endif
+
+namespace kelp {
+
+namespace {
+
+// One stage: anchor, column height with the stage surface, the stage depth
+// (z itself, or s H_p), layer, u/v and W.
+template <int MAXV, typename T, bool ELEM>
+__device__ inline bool sig_site(const DMesh& m, const Snap<T, true>& f, const Snap<T, false>& fe,
+                                const StepCfg& c, int k, double lon, double lat, bool from_s, double s_or_z,
+                                int& nn, double& H, double& U, double& V, double& W)
+{
+    if (nn < 0) nn = site_anchor(m, lon, lat, c.exact);
+    const double* eta = c.sig_eta[k];
+    H = column_height<MAXV>(m, eta, c.Dmin, c.w_method, lon, lat, nn);
+    const double zs = from_s ? s_or_z * H : s_or_z;
+    const int layer = find_layer_sigma(m, zs, H < c.Dmin ? c.Dmin : H);
+    bool ok;
+    if constexpr (ELEM)
+        ok = interp_lsq(m, fe, c.alpha0, c.alphas[k], c.time_dir, lon, lat, nn, (int64_t)layer * m.n_elems, U, V);
+    else
+        ok = interp_uv<MAXV>(m, f, c.alpha0, c.alphas[k], c.time_dir, lon, lat, nn, (int64_t)layer * m.n_nodes, U,
+                             V);
+    W = w_sigma<MAXV>(m, f, eta, c.Dmin, c.w_method, c.alpha0, c.alphas[k], lon, lat, zs, nn, c.time_dir);
+    return ok;
+}
+
+// Returns the failure code (0 = alive); on failure the position is kept and
+// nn is the start position's anchor.
+template <int ORDER, int MAXV, typename T, bool ELEM>
+__device__ inline int sigma_det_step(const DMesh& m, const Snap<T, true>& f, const Snap<T, false>& fe,
+                                     const StepCfg& c, double& lon, double& lat, double& z, int& nn, double& Uo,
+                                     double& Vo, double& Wo)
+{
+    const double dt = c.dt;
+    const double lon0 = lon, lat0 = lat, zz0 = z;
+    int n1 = (c.reuse_anchor && nn >= 0) ? nn : -1;
+    double H1, U1, V1, W1;
+    bool ok = sig_site<MAXV, T, ELEM>(m, f, fe, c, 0, lon0, lat0, false, zz0, n1, H1, U1, V1, W1);
+    Uo = U1;
+    Vo = V1;
+    Wo = W1;
+    double lon_new = lon0, lat_new = lat0, s_new = 0.0;
+    if (ok) {
+        if (ORDER == 1) {
+            lon_new = wrap_lon(lon0 + dt * U1 / z0_of(lat0));
+            lat_new = clamp_lat(lat0 + dt * V1 / kSLM);
+            s_new = (zz0 + W1 * dt) / H1;
+        } else if (ORDER == 2) {
+            const double z0 = z0_of(lat0);
+            const double lon_mid = wrap_lon(lon0 + 0.5 * dt * U1 / z0);
+            const double lat_mid = clamp_lat(lat0 + 0.5 * dt * V1 / kSLM);
+            const double s_mid = (zz0 + W1 * 0.5 * dt) / H1;
+            int n2 = -1;
+            double H2, U2, V2, W2;
+            ok = sig_site<MAXV, T, ELEM>(m, f, fe, c, 1, lon_mid, lat_mid, true, s_mid, n2, H2, U2, V2, W2);
+            if (ok) {
+                lon_new = wrap_lon(lon0 + dt * U2 / z0_of(lat_mid));
+                lat_new = clamp_lat(lat0 + dt * V2 / kSLM);
+                s_new = zz0 / H1 + dt * W2 / H2;
+            }
+        } else {
+            const double dlon1 = dt * U1 / z0_of(lat0);
+            const double dlat1 = dt * V1 / kSLM;
+            const double s0 = zz0 / H1;
+            const double f1 = W1 / H1;
+            const double lon2 = wrap_lon(lon0 + 0.5 * dlon1);
+            const double lat2 = clamp_lat(lat0 + 0.5 * dlat1);
+            int n2 = -1, n3 = -1, n4 = -1;
+            double H2, U2, V2, W2, H3, U3, V3, W3, H4, U4, V4, W4;
+            ok = sig_site<MAXV, T, ELEM>(m, f, fe, c, 1, lon2, lat2, true, s0 + 0.5 * dt * f1, n2, H2, U2, V2, W2);
+            if (ok) {
+                const double f2 = W2 / H2;
+                const double dlon2 = dt * U2 / z0_of(lat2);
+                const double dlat2 = dt * V2 / kSLM;
+                const double lon3 = wrap_lon(lon0 + 0.5 * dlon2);
+                const double lat3 = clamp_lat(lat0 + 0.5 * dlat2);
+                ok = sig_site<MAXV, T, ELEM>(m, f, fe, c, 2, lon3, lat3, true, s0 + 0.5 * dt * f2, n3, H3, U3, V3,
+                                             W3);
+                if (ok) {
+                    const double f3 = W3 / H3;
+                    const double dlon3 = dt * U3 / z0_of(lat3);
+                    const double dlat3 = dt * V3 / kSLM;
+                    const double lon4 = wrap_lon(lon0 + dlon3);
+                    const double lat4 = clamp_lat(lat0 + dlat3);
+                    ok = sig_site<MAXV, T, ELEM>(m, f, fe, c, 3, lon4, lat4, true, s0 + dt * f3, n4, H4, U4, V4,
+                                                 W4);
+                    if (ok) {
+                        const double f4 = W4 / H4;
+                        const double dlon4 = dt * U4 / z0_of(lat4);
+                        const double dlat4 = dt * V4 / kSLM;
+                        const double dlon = (dlon1 + 2.0 * dlon2 + 2.0 * dlon3 + dlon4) / 6.0;
+                        const double dlat = (dlat1 + 2.0 * dlat2 + 2.0 * dlat3 + dlat4) / 6.0;
+                        lon_new = wrap_lon(lon0 + dlon);
+                        lat_new = clamp_lat(lat0 + dlat);
+                        s_new = s0 + dt * (f1 + 2.0 * f2 + 2.0 * f3 + f4) / 6.0;
+                    }
+                }
+            }
+        }
+    }
+    if (!ok) {
+        nn = n1;
+        return kFrInterp;
+    }
+    const int nn_new = site_anchor(m, lon_new, lat_new, c.exact);
+    if (find_containing<MAXV>(m, lon_new, lat_new, nn_new) < 0) {
+        nn = n1;
+        return kFrOOB;
+    }
+    lon = lon_new;
+    lat = lat_new;
+    const double H_end = column_height<MAXV>(m, c.sig_eta[4], c.Dmin, c.w_method, lon_new, lat_new, nn_new);
+    z = clamp_sigma(s_new * H_end, H_end, c.Dmin, c.z0b_min);
+    nn = nn_new;
+    return 0;
+}
+
+template <int ORDER, int MAXV, typename T, bool ELEM>
+__global__ __launch_bounds__(256) void sigma_step_kernel(DMesh m, Snap<T, true> f, Snap<T, false> fe, StepCfg c,
+                                                         PState s)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.n) return;
+    const int32_t prev = s.status[i];
+    if (s.alive_count) {
+        unsigned long long alive = __ballot(prev == kAlive);
+        unsigned long long active = __ballot(1);
+        if ((int)(threadIdx.x & 63) == __ffsll((long long)active) - 1 && alive)
+            atomicAdd(s.alive_count, (unsigned long long)__popcll(alive));
+    }
+    double U = 0.0, V = 0.0, W = 0.0;
+    if (prev != kAlive) {
+        // the diagnostic velocity of a particle that does not move is 0
+        s.u[i] = U;
+        s.v[i] = V;
+        if (s.w) s.w[i] = W;
+        return;
+    }
+    int32_t st = prev;
+    double lon = s.lon[i], lat = s.lat[i], z = s.z[i];
+    int nn = s.nn[i];
+    int8_t fr = s.fr[i];
+    const int code = sigma_det_step<ORDER, MAXV, T, ELEM>(m, f, fe, c, lon, lat, z, nn, U, V, W);
+    if (code) {
+        st = kOutside;
+        fr = (int8_t)code;
+    }
+    if ((c.periodic_lon || c.periodic_lat) && st == kAlive) periodic_one(m, c, lon, lat, nn);
+    s.lon[i] = lon;
+    s.lat[i] = lat;
+    s.z[i] = z;
+    s.status[i] = st;
+    s.nn[i] = nn;
+    s.fr[i] = fr;
+    s.u[i] = U;
+    s.v[i] = V;
+    if (s.w) s.w[i] = W;
+}
+
+}  // namespace
+
+template <int MAXV, typename T, bool ELEM>
+void launch_sigma_step(hipStream_t stream, int order, const DMesh& m, const Snap<T, true>& f,
+                       const Snap<T, false>& fe, const StepCfg& cfg, const PState& ps)
+{
+    const unsigned g = (unsigned)((ps.n + 255) / 256);
+    if (order == 1)
+        sigma_step_kernel<1, MAXV, T, ELEM><<<g, 256, 0, stream>>>(m, f, fe, cfg, ps);
+    else if (order == 2)
+        sigma_step_kernel<2, MAXV, T, ELEM><<<g, 256, 0, stream>>>(m, f, fe, cfg, ps);
+    else
+        sigma_step_kernel<4, MAXV, T, ELEM><<<g, 256, 0, stream>>>(m, f, fe, cfg, ps);
+}
+
+#if KS_F32
+using KsT = float;
+#else
+using KsT = double;
+#endif
+template void launch_sigma_step<3, KsT, (bool)KS_ELEM>(hipStream_t, int, const DMesh&, const Snap<KsT, true>&,
+                                                       const Snap<KsT, false>&, const StepCfg&, const PState&);
+template void launch_sigma_step<4, KsT, (bool)KS_ELEM>(hipStream_t, int, const DMesh&, const Snap<KsT, true>&,
+                                                       const Snap<KsT, false>&, const StepCfg&, const PState&);
+
+#if !KS_F32 && !KS_ELEM
+namespace {
+// core/io.py:68 _blend chains of one node: the value of the stage field
+// (nested time blend, the same short-circuits as tblend), sources read only
+// where the blend uses them.
+__device__ __forceinline__ double eta_of(const EtaSrc& e, int64_t n)
+{
+    if (e.alpha < 1e-12) {
+        if (e.alpha0 == 0.0) return e.a[n];
+        return (1.0 - e.alpha0) * e.a[n] + e.alpha0 * e.b[n];
+    }
+    if (e.alpha > 1.0 - 1e-12) return e.t[n];
+    if (e.alpha0 == 0.0) return (1.0 - e.alpha) * e.a[n] + e.alpha * e.t[n];
+    const double a0 = (1.0 - e.alpha0) * e.a[n] + e.alpha0 * e.b[n];
+    return (1.0 - e.alpha) * a0 + e.alpha * e.t[n];
+}
+
+// integrators.py:1369 _eta_blend
+__device__ __forceinline__ double eta_blend(double e0, double e1, double c)
+{
+    if (c <= 0.0) return e0;
+    if (c >= 1.0) return e1;
+    return (1.0 - c) * e0 + c * e1;
+}
+
+// the stage surfaces of integrators.py:5426-5498 (RK2) and :5806-5816 (RK4)
+__global__ void sigma_eta_kernel(int64_t N, int order, int steady, EtaSrc start, EtaSrc end, int end_is_start,
+                                 EtaSrc stages, double a1, double a2, double a3, double* __restrict__ out)
+{
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    const double e0 = eta_of(start, n);
+    const double ee = end_is_start ? e0 : eta_of(end, n);
+    out[n] = e0;
+    out[4 * N + n] = ee;
+    const double al[3] = {a1, a2, a3};
+    const double cc[3] = {0.5, 0.5, 1.0};
+    for (int k = 1; k < order; ++k) {
+        double v;
+        if (steady) {
+            v = eta_blend(e0, ee, order == 2 ? 0.5 : cc[k - 1]);
+        } else {
+            EtaSrc s = stages;
+            s.alpha = al[k - 1];
+            v = eta_of(s, n);
+        }
+        out[(int64_t)k * N + n] = v;
+    }
+}
+
+}  // namespace
+
+void launch_sigma_eta(hipStream_t stream, int64_t n_nodes, int order, int steady, EtaSrc start, EtaSrc end,
+                      int end_is_start, EtaSrc stages, const double* stage_alphas, double* out)
+{
+    const unsigned g = (unsigned)((n_nodes + 255) / 256);
+    sigma_eta_kernel<<<g, 256, 0, stream>>>(n_nodes, order, steady, start, end, end_is_start, stages,
+                                            stage_alphas[1], stage_alphas[2], stage_alphas[3], out);
+}
+
+#endif
+
+}  // namespace kelp

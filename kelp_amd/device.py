@@ -82,7 +82,7 @@ RESUSPENSION_CODES = {"never": 0, "auto": 1, "stochastic": 2, "auto_or_stochasti
 def make_step_args(order, slot_a, dt, time_dir, slot_b=-1, slot_t=-1, alpha0=0.0,
                    alphas=(0.0, 0.0, 0.0, 0.0), exact_nearest=True, w_method=2, kick=None,
                    beach=None, step_i=0, real_id=0, periodic_lon=None,
-                   periodic_lat=None, reuse_anchor=False) -> StepArgs:
+                   periodic_lat=None, reuse_anchor=False, sigma=None) -> StepArgs:
     a = StepArgs()
     a.order = int(order)
     a.slot_a, a.slot_b, a.slot_t = int(slot_a), int(slot_b), int(slot_t)
@@ -96,6 +96,19 @@ def make_step_args(order, slot_a, dt, time_dir, slot_b=-1, slot_t=-1, alpha0=0.0
     a.step_i = int(step_i)
     a.real_id = int(real_id)
     a.reuse_anchor = int(bool(reuse_anchor))
+    a.sig_start_a = a.sig_start_b = a.sig_start_t = -1
+    a.sig_end_a = a.sig_end_b = a.sig_end_t = -1
+    if sigma is not None:
+        # surfaces as (slot_a, slot_b, slot_t, alpha0, alpha) nested blends
+        sa_, sb_, st_, a0_, al_ = sigma["start"]
+        a.sig_start_a, a.sig_start_b, a.sig_start_t = int(sa_), int(sb_), int(st_)
+        a.sig_start_alpha0, a.sig_start_alpha = float(a0_), float(al_)
+        if sigma.get("end") is not None:
+            ea_, eb_, et_, e0_, el_ = sigma["end"]
+            a.sig_end_a, a.sig_end_b, a.sig_end_t = int(ea_), int(eb_), int(et_)
+            a.sig_end_alpha0, a.sig_end_alpha = float(e0_), float(el_)
+        a.Dmin = float(sigma.get("Dmin", 0.5))
+        a.z0b_min = float(sigma.get("z0b_min", 0.1))
     if periodic_lon is not None:
         a.periodic_lon, a.px0, a.px1 = 1, float(periodic_lon[0]), float(periodic_lon[1])
     if periodic_lat is not None:
@@ -180,7 +193,8 @@ class DeviceContext:
             mesh.gh_nx, mesh.gh_ny, int(mesh.gh_wrap_lon), mesh.gh_far_rows,
             mesh.r_earth, int(keep["z"].size), keep["z"].ctypes.data if keep["z"].size else None,
             None if keep["coastal"] is None else keep["coastal"].ctypes.data,
-            keep["clon"].ctypes.data, keep["clat"].ctypes.data)
+            keep["clon"].ctypes.data, keep["clat"].ctypes.data,
+            1 if mesh.vertical_coord == "sigma" else 0)
         check(self.lib.kelp_mesh_upload(self.h, ctypes.byref(d)), "kelp_mesh_upload")
         self.mesh = mesh
         self._slot_keys.clear()
@@ -302,13 +316,16 @@ class DeviceContext:
              slot_b: int = -1, slot_t: int = -1, alpha0: float = 0.0, alphas=(0.0, 0.0, 0.0, 0.0),
              exact_nearest: bool = True, w_method: int = 2, kick=None, beach=None,
              step_i: int = 0, real_id: int = 0, periodic_lon=None, periodic_lat=None,
-             reuse_anchor: bool = False) -> None:
+             reuse_anchor: bool = False, sigma=None) -> None:
         """One step in place.  ``kick``: dict(kh_x, kh_y, seed[, az_const])
         for the split uniform-Kh random walk (and the vertical Az kick);
-        ``beach``: dict of the beaching options (criteria, depth_min, ...)."""
+        ``beach``: dict of the beaching options (criteria, depth_min, ...);
+        ``sigma`` (sigma meshes): dict(start=(slot_a, slot_b, slot_t, alpha0,
+        alpha), end=same or None, Dmin, z0b_min), the surfaces at the step's
+        start and end as blends of the slots' eta."""
         a = make_step_args(order, slot_a, dt, time_dir, slot_b, slot_t, alpha0, alphas,
                            exact_nearest, w_method, kick, beach, step_i, real_id,
-                           periodic_lon, periodic_lat, reuse_anchor)
+                           periodic_lon, periodic_lat, reuse_anchor, sigma)
         s = p.as_struct()
         check(self.lib.kelp_step(self.h, ctypes.byref(s), ctypes.byref(a)), "kelp_step")
 

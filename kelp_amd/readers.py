@@ -168,3 +168,147 @@ class Fesom2ArrayReader:
                     raise KeyError(f"sample variable {name!r} not in the dataset")
                 scalars[name] = np.asarray(ds.record(name, timestep), dtype=np.float64)
         return VelocityField(u=u, v=v, eta=eta, w=w, uv_layout=layout, scalars=scalars)
+
+
+def sanitize_connectivity(nv) -> np.ndarray:
+    """readers/fesomc.py:52 _sanitize_connectivity: 1-based padded
+    connectivity (a triangle repeats its first vertex) to 0-based with -1
+    fill and no repeated vertex."""
+    conn = np.asarray(nv, dtype=np.int64)
+    conn = np.where(conn <= 0, -1, conn - 1)
+    clean = -np.ones(conn.shape, dtype=np.int32)
+    for i in range(conn.shape[0]):
+        seen = []
+        for node in conn[i]:
+            node = int(node)
+            if node < 0 or node in seen:
+                continue
+            seen.append(node)
+        clean[i, :len(seen)] = seen
+    return clean
+
+
+def _layer_major(arr, dims, n_layers, n_items, name, vert=("sigma", "sigmam1", "nz", "nz1", "depth",
+                                                            "level")):
+    """readers/fesomc.py:166/:206: a (layer, item) array from either axis
+    order, by dimension name first, else by shape."""
+    arr = np.asarray(arr)
+    if arr.ndim != 2:
+        raise ValueError(f"{name} must be 2-D in 3-D mode, got shape {arr.shape}")
+    if dims is not None and len(dims) == 2:
+        if dims[0] in vert and dims[1] not in vert:
+            out = arr
+        elif dims[1] in vert and dims[0] not in vert:
+            out = arr.T
+        else:
+            out = None
+        if out is not None:
+            if out.shape != (n_layers, n_items):
+                raise ValueError(f"{name}: expected ({n_layers}, {n_items}), got {out.shape}")
+            return np.ascontiguousarray(out)
+    if arr.shape == (n_layers, n_items):
+        return np.ascontiguousarray(arr)
+    if arr.shape == (n_items, n_layers):
+        return np.ascontiguousarray(arr.T)
+    raise ValueError(f"{name} has shape {arr.shape}; expected ({n_layers}, {n_items}) or the transpose")
+
+
+@register("fesomc_arrays")
+class FesomCArrayReader:
+    """FESOM-C variable conventions (readers/fesomc.py) over an
+    :class:`ArrayDataset`: nodes ``lon``/``lat``/``depth``, connectivity
+    ``nv`` (1-based, triangles repeating their first vertex), element
+    centres ``lon_elem``/``lat_elem``, sigma interfaces ``sigma_lev`` (1 at
+    the surface .. 0 at the bottom), u/v per element and layer, the
+    dia-sigma flux ``w`` on the interfaces (file convention positive up,
+    flipped to positive down), surface elevation ``eta``."""
+
+    @staticmethod
+    def open_dataset(source):
+        if isinstance(source, ArrayDataset):
+            return source
+        raise TypeError("fesomc_arrays reads ArrayDataset objects (no file formats in this build)")
+
+    @staticmethod
+    def load_mesh(ds, r_earth=None, cell_size=0.0):
+        """readers/fesomc.py:246 load_mesh: a sigma mesh with the grid hash
+        built (Earth radius 6,400 km unless given, the FESOM-C value)."""
+        from .mesh import Mesh, build_grid_hash, build_node_to_elems
+        lon = np.asarray(ds.record("lon", 0), dtype=np.float64)
+        lat = np.asarray(ds.record("lat", 0), dtype=np.float64)
+        depth = (np.asarray(ds.record("depth", 0), dtype=np.float64) if "depth" in ds
+                 else np.zeros_like(lon))
+        conn = sanitize_connectivity(ds.record("nv", 0))
+        if "lon_elem" in ds and "lat_elem" in ds:
+            clon = np.asarray(ds.record("lon_elem", 0), dtype=np.float64)
+            clat = np.asarray(ds.record("lat_elem", 0), dtype=np.float64)
+        else:
+            clon = np.empty(conn.shape[0])
+            clat = np.empty(conn.shape[0])
+            for i, row in enumerate(conn):
+                v = row[row >= 0]
+                lo = lon[v]
+                if lo.max() - lo.min() <= 180.0:
+                    clon[i] = lo.mean()
+                else:
+                    d = ((lo - lo[0] + 180.0) % 360.0) - 180.0
+                    clon[i] = ((lo[0] + d.mean() + 180.0) % 360.0) - 180.0
+                clat[i] = lat[v].mean()
+        sigma = (np.asarray(ds.record("sigma_lev", 0), dtype=np.float64) if "sigma_lev" in ds
+                 else np.array([0.0]))
+        n2e, cnt = build_node_to_elems(lon.size, conn)
+        mesh = Mesh(name="fesomc", nodes_lon=lon, nodes_lat=lat, depth=depth, elem_nodes=conn,
+                    elem_center_lon=clon, elem_center_lat=clat, node_to_elems=n2e,
+                    node_to_elems_count=cnt, sigma=sigma, vertical_coord="sigma",
+                    r_earth=float(r_earth or 6_400_000.0),
+                    metadata={"sigma_transform": "linear_fesomc"})
+        build_grid_hash(mesh, cell_size)
+        return mesh
+
+    @staticmethod
+    def _uv_names(ds, mode_cfg):
+        vu = getattr(mode_cfg, "var_u", None)
+        vv = getattr(mode_cfg, "var_v", None)
+        return (vu or "u"), (vv or "v")
+
+    @classmethod
+    def probe_uv_layout(cls, ds, mode_cfg, mesh) -> str:
+        u_name, _ = cls._uv_names(ds, mode_cfg)
+        dims = ds.dims(u_name)
+        if "nele" in dims or "elem" in dims:
+            return "element"
+        if "node" in dims or "nod2" in dims:
+            return "node"
+        return Fesom2ArrayReader.probe_uv_layout(ds, mode_cfg, mesh)
+
+    @classmethod
+    def read_velocities(cls, ds, timestep, mesh, mode_cfg=None, diffusion=None, sample_vars=None,
+                        velocity_float32=False) -> VelocityField:
+        """readers/fesomc.py:514 read_velocities (3-D)."""
+        if not getattr(mode_cfg, "Fdrift3D", False):
+            raise NotImplementedError("fesomc_arrays: 2-D reductions of sigma fields")
+        vd = np.float32 if velocity_float32 else np.float64
+        e_name = getattr(mode_cfg, "var_eta", None) or "eta"
+        eta = (np.asarray(ds.record(e_name, timestep), dtype=np.float64) if e_name in ds
+               else np.zeros(mesh.n_nodes))
+        u_name, v_name = cls._uv_names(ds, mode_cfg)
+        w_name = getattr(mode_cfg, "var_w", None) or "w"
+        if w_name == "w_cart":
+            raise ValueError("FESOM-C 3-D: 'w_cart' lacks the dia-sigma part; track with 'w'")
+        layout = cls.probe_uv_layout(ds, mode_cfg, mesh)
+        n_items = mesh.n_nodes if layout == "node" else mesh.n_elements
+        K = mesh.sigma.size
+        drop_time = lambda name: ds.dims(name)[1:] if ds.dims(name)[0] == "time" else ds.dims(name)
+        u = _layer_major(ds.record(u_name, timestep), drop_time(u_name), K - 1, n_items, u_name)
+        v = _layer_major(ds.record(v_name, timestep), drop_time(v_name), K - 1, n_items, v_name)
+        w = _layer_major(ds.record(w_name, timestep), drop_time(w_name), K, mesh.n_nodes, w_name)
+        pos = getattr(mode_cfg, "w_positive", "auto")
+        if pos == "auto":
+            pos = ds.attrs.get(w_name, {}).get("positive", "up")
+        w = (-1.0 if pos == "up" else 1.0) * np.asarray(w, dtype=np.float64)
+        scalars = None
+        if sample_vars:
+            scalars = {name: np.asarray(ds.record(name, timestep), dtype=np.float64)
+                       for name in sample_vars}
+        return VelocityField(u=u.astype(vd, copy=False), v=v.astype(vd, copy=False), eta=eta,
+                             w=w.astype(vd, copy=False), uv_layout=layout, scalars=scalars)

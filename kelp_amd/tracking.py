@@ -29,6 +29,24 @@ from .particles import ParticleState, ParticleStatus
 _C = {1: (0.0,), 2: (0.0, 0.5), 4: (0.0, 0.5, 0.5, 1.0)}
 N_SLOTS = 16
 SAMPLE_METHODS = {"nearest": 0, "leastsq": 1, "barycentric": 2, "idw": 3}
+SIGMA_W_METHODS = {"nearest": 0, "leastsq": 1, "barycentric": 2, "idw": 3}
+
+
+def eta_at_time(t, file_times, n, on_exhausted, direction, snap, slots):
+    """tracking.py:3097 _eta_at_time (linear): the surface at time ``t`` as
+    a blend ``(slot_a, slot_b, -1, alpha, 0)`` of snapshot surfaces, or
+    None outside the records."""
+    br = resolve_time_bracket(t, file_times, n, on_exhausted, direction, snap)
+    if br is None:
+        return None
+    ia, ib, ta, tb, tw = br
+    interval = tb - ta
+    alpha = (tw - ta) / interval if interval > 0 else 0.0
+    if ia == ib or abs(alpha) < 1e-12:
+        return (slots.slot(ia), -1, -1, 0.0, 0.0)
+    if abs(alpha - 1.0) < 1e-12:
+        return (slots.slot(ib), -1, -1, 0.0, 0.0)
+    return (slots.slot(ia), slots.slot(ib), -1, alpha, 0.0)
 
 
 def snapshot_band(dt) -> float:
@@ -232,12 +250,43 @@ def _check_path(numerics: NumericsConfig, mesh: Mesh, is3d: bool, uv_layout: str
         raise NotImplementedError(f"device path: uv_interp={numerics.uv_interp!r} with "
                                   f"{uv_layout}-layout u/v")
     if is3d:
-        if mesh.vertical_coord != "zlevel":
-            raise NotImplementedError("device path: sigma meshes")
-        if numerics.w_interp not in W_METHODS:
+        if mesh.vertical_coord not in ("zlevel", "sigma"):
+            raise NotImplementedError(f"device path: vertical_coord {mesh.vertical_coord!r}")
+        methods = SIGMA_W_METHODS if mesh.vertical_coord == "sigma" else W_METHODS
+        if numerics.w_interp not in methods:
             raise NotImplementedError(f"device path: w_interp={numerics.w_interp!r}")
+        if mesh.vertical_coord == "sigma" and (numerics.diffusion.enabled or numerics.beaching.enabled):
+            raise NotImplementedError("device path: diffusion or beaching on a sigma mesh")
     if numerics.beaching.enabled and numerics.beaching.dry_mask_var:
         raise NotImplementedError("device path: beaching dry_mask_var")
+
+
+def _sigma_surfaces(use_tinterp, sim_time, sim_time_end, dt, direction, file_times, n,
+                    on_exhausted, snap, slots, pay, order, sa, sb, st, alphas, step_len, dtn,
+                    file_idx):
+    """The surfaces a sigma step starts and ends on (tracking.py:2562-2594:
+    ``sigma_eta_start``/``sigma_eta_end``, one function of time), with the
+    integrators' defaults where the tracking layer supplies none
+    (integrators.py:1381 _sigma_end_eta, :5806-5812)."""
+    start = end = None
+    if use_tinterp:
+        start = eta_at_time(sim_time, file_times, n, on_exhausted, direction, snap, slots)
+        t_end = sim_time_end if sim_time_end is not None else sim_time + dt * direction
+        end = eta_at_time(t_end, file_times, n, on_exhausted, direction, snap, slots)
+    else:
+        nxt = file_idx + direction
+        if on_exhausted == "cycle":
+            nxt %= n
+        if 0 <= nxt < n:
+            end = (slots.slot(nxt), -1, -1, 0.0, 0.0)
+    if start is None:
+        # the step field's own surface (stage 0 of the velocity blend)
+        start = (sa, sb, st, pay.alpha0, 0.0)
+    if end is None and st >= 0:
+        # not steady: the velocity blend at the step end
+        a_end = alphas[3] if order == 4 else step_len / dtn
+        end = (sa, sb, st, pay.alpha0, a_end)
+    return dict(start=start, end=end)
 
 
 def activate_released(status: np.ndarray, release_step, global_step_i: int):
@@ -307,7 +356,8 @@ def run_single_realization(mesh: Mesh, ds, reader, init_state: ParticleState, mo
     rk_snap = use_tinterp and multistage
     snap = snapshot_band(dt) if rk_snap else 5e-10 * max(dt, 1.0)
     exact = 1 if exact_nearest_on(mesh) else 2
-    w_method = W_METHODS[numerics.w_interp] if is3d else 2
+    sig3d = is3d and mesh.vertical_coord == "sigma"
+    w_method = ((SIGMA_W_METHODS if sig3d else W_METHODS)[numerics.w_interp] if is3d else 2)
     kick = _kick_args(numerics)
     beach = _beach_args(numerics, dt)
     periodic = dict(periodic_lon=numerics.periodic_lon, periodic_lat=numerics.periodic_lat)
@@ -323,7 +373,7 @@ def run_single_realization(mesh: Mesh, ds, reader, init_state: ParticleState, mo
     if ctx.mesh is not mesh:
         ctx.upload_mesh(mesh)
     slots = _SnapshotSlots(ctx, reader, ds, mesh, mode, numerics, sample_vars,
-                           need_eta=bool(beach and beach["use_eta"]))
+                           need_eta=bool(beach and beach["use_eta"]) or sig3d)
     P = init_state.n_particles
     pid = particle_offset + np.arange(P, dtype=np.int64)
     ws = (np.zeros(P, np.int32) if start_wet_streak is None
@@ -395,12 +445,18 @@ def run_single_realization(mesh: Mesh, ds, reader, init_state: ParticleState, mo
         alphas = [0.0] * 4
         if dtn is not None:
             alphas = [c * step_len / dtn for c in _C[order]] + [0.0] * (4 - order)
+        sigma = None
+        if sig3d:
+            sigma = _sigma_surfaces(use_tinterp, sim_time, sim_time_end, dt, direction, file_times,
+                                    n_time_total, on_exhausted, snap, slots, pay, order, sa, sb,
+                                    st, alphas, step_len, dtn, file_idx if not use_tinterp else None)
+            sigma.update(Dmin=numerics.Dmin, z0b_min=numerics.z0b_min)
         if resort_every > 0 and g % resort_every == 0:
             ctx.sort(p)
         ctx.step(p, order, sa, step_len, float(direction), slot_b=sb, slot_t=st,
                  alpha0=pay.alpha0, alphas=alphas, exact_nearest=exact, w_method=w_method,
                  kick=kick, beach=beach, step_i=g, real_id=realization_id, reuse_anchor=True,
-                 **periodic)
+                 sigma=sigma, **periodic)
         if use_tinterp:
             sim_time = sim_time_end if sim_time_end is not None else sim_time + dt * direction
         if (g + 1) % max(int(numerics.output_freq), 1) == 0:

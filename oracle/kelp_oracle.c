@@ -1304,6 +1304,412 @@ static int det_step(const or_mesh_t *m, const or_fields_t *f, const or_step_args
     return 0;
 }
 
+/* ---------------------------------------------------------------------
+ * Sigma meshes (numerics/vertical_sigma.py and the sigma branches of the
+ * multi-pass steppers, integrators.py:1310-1441, :4997, :5380, :5760).
+ * The mesh's levels are the sigma interfaces; interface k of a column of
+ * height H lies at z = (1 - sigma_k) H.
+ * --------------------------------------------------------------------- */
+
+static int collect_neighbours(const or_mesh_t *m, int node, int *buf);
+
+/* vertical_sigma.py:473 _col_H */
+static inline double col_H(const or_mesh_t *m, const double *eta, double Dmin, int nd)
+{
+    double H = m->depth[nd] + eta[nd];
+    if (H < Dmin) H = Dmin;
+    return H;
+}
+
+/* vertical_sigma.py:482 _sigma_frac */
+static inline double sigma_frac(double z, double Hp)
+{
+    double f = z / Hp;
+    if (f < 0.0)
+        f = 0.0;
+    else if (f > 1.0)
+        f = 1.0;
+    return f;
+}
+
+/* vertical_sigma.py:74 find_layer_sigma */
+static int find_layer_sigma(const double *s, int K, double z, double H)
+{
+    if (K < 2) return 0;
+    double z_top = (1.0 - s[0]) * H;
+    if (z <= z_top) return 0;
+    double z_bot = (1.0 - s[K - 1]) * H;
+    if (z >= z_bot) return K - 2;
+    for (int k = 0; k < K - 1; k++) {
+        double z_next = (1.0 - s[k + 1]) * H;
+        if (z < z_next) return k;
+    }
+    return K - 2;
+}
+
+/* vertical_sigma.py:297 _interp_w_sigma_linear_single on the stage field */
+static double w_sig_linear(const or_mesh_t *m, const or_fields_t *f, double alpha, double z, int node,
+                           double H, double time_dir)
+{
+    const double *s = m->z_levels;
+    int K = m->n_levels;
+    int64_t N = m->n_nodes;
+    if (K < 2) return time_dir * wlev(f, node, alpha);
+    double z_top = (1.0 - s[0]) * H;
+    double z_bot = (1.0 - s[K - 1]) * H;
+    if (z <= z_top) return time_dir * wlev(f, node, alpha);
+    if (z >= z_bot) return time_dir * wlev(f, (int64_t)(K - 1) * N + node, alpha);
+    int k = K - 2;
+    for (int j = 0; j < K - 1; j++) {
+        double z_next = (1.0 - s[j + 1]) * H;
+        if (z < z_next) {
+            k = j;
+            break;
+        }
+    }
+    double z_k = (1.0 - s[k]) * H;
+    double z_k1 = (1.0 - s[k + 1]) * H;
+    double dz = z_k1 - z_k;
+    if (dz < 1e-12) return time_dir * wlev(f, (int64_t)k * N + node, alpha);
+    double al = (z - z_k) / dz;
+    double w_val = (1.0 - al) * wlev(f, (int64_t)k * N + node, alpha) + al * wlev(f, (int64_t)(k + 1) * N + node, alpha);
+    return time_dir * w_val;
+}
+
+/* the own column at min(z, H): every horizontal method's fallback */
+static double w_sig_own(const or_mesh_t *m, const or_fields_t *f, const double *eta, double Dmin, double alpha,
+                        double z, int node, double time_dir)
+{
+    double H = col_H(m, eta, Dmin, node);
+    double zc = z < H ? z : H;
+    return w_sig_linear(m, f, alpha, zc, node, H, time_dir);
+}
+
+/* vertical_sigma.py:566 _bary_locate: the containing element among the
+ * node's elements and its vertex weights; returns n_valid (0: none) */
+static int bary_locate(const or_mesh_t *m, double xp, double yp, int node, int *verts, double *wts)
+{
+    const double DEG2R = 0.017453292519943295;
+    int n_el = m->n2e_count[node];
+    const int32_t *row = m->n2e + (int64_t)node * m->max_conn;
+    for (int k = 0; k < n_el; k++) {
+        int e = row[k];
+        if (e < 0) continue;
+        int nv = 0;
+        for (int v = 0; v < m->max_v; v++) {
+            int nd = m->elem_nodes[(int64_t)e * m->max_v + v];
+            if (nd < 0) continue;
+            if (nv > 0 && nd == verts[0]) continue;
+            verts[nv++] = nd;
+        }
+        if (nv < 3) continue;
+        double lon0 = seam_safe_lon0(m, verts, nv);
+        double lat0 = 0.0;
+        for (int v = 0; v < nv; v++) lat0 += m->nodes_lat[verts[v]];
+        lat0 /= nv;
+        double cos_lat0 = M_COS(lat0 * DEG2R);
+        const double R = 6371000.0;
+        double vx[4], vy[4];
+        for (int v = 0; v < nv; v++) {
+            double dlon = PYMOD360(m->nodes_lon[verts[v]] - lon0 + 180.0) - 180.0;
+            vx[v] = dlon * DEG2R * R * cos_lat0;
+            vy[v] = (m->nodes_lat[verts[v]] - lat0) * DEG2R * R;
+        }
+        double dlon_p = PYMOD360(xp - lon0 + 180.0) - 180.0;
+        double px = dlon_p * DEG2R * R * cos_lat0;
+        double py = (yp - lat0) * DEG2R * R;
+        for (int t = 0; t < nv - 2; t++) {
+            double ax = vx[0], ay = vy[0], bx = vx[t + 1], by = vy[t + 1], cx = vx[t + 2], cy = vy[t + 2];
+            double denom = (by - cy) * (ax - cx) + (cx - bx) * (ay - cy);
+            if (fabs(denom) < 1e-14) continue;
+            double w1 = ((by - cy) * (px - cx) + (cx - bx) * (py - cy)) / denom;
+            double w2 = ((cy - ay) * (px - cx) + (ax - cx) * (py - cy)) / denom;
+            double w3 = 1.0 - w1 - w2;
+            const double eps = -1e-6;
+            if (w1 >= eps && w2 >= eps && w3 >= eps) {
+                for (int v = 0; v < 4; v++) wts[v] = 0.0;
+                if (nv == 4) {
+                    wachspress_quad(px, py, vx, vy, wts);
+                } else {
+                    wts[0] = w1;
+                    wts[t + 1] = w2;
+                    wts[t + 2] = w3;
+                }
+                return nv;
+            }
+        }
+    }
+    return 0;
+}
+
+/* vertical_sigma.py:1035 _loop_sigma_column_height for one particle
+ * (:493 leastsq, :636 barycentric, :534 idw, else the anchor's column) */
+static double column_height(const or_mesh_t *m, const double *eta, double Dmin, int method, double xp,
+                            double yp, int node)
+{
+    const double DEG2R = 0.017453292519943295;
+    if (method == 1) {
+        int buf[64];
+        int count = collect_neighbours(m, node, buf);
+        if (count < 3) return col_H(m, eta, Dmin, node);
+        double z0 = M_COS(yp * DEG2R) * SLM;
+        if (z0 < 1.0) z0 = 1.0;
+        double a11 = 0, a12 = 0, a13 = 0, a22 = 0, a23 = 0, a33 = 0, fw = 0, f1w = 0, f2w = 0;
+        for (int i = 0; i < count; i++) {
+            int nd = buf[i];
+            double h = col_H(m, eta, Dmin, nd);
+            double dlon = PYMOD360(m->nodes_lon[nd] - xp + 180.0) - 180.0;
+            double ex = dlon * z0;
+            double ey = (m->nodes_lat[nd] - yp) * SLM;
+            a11 += 1.0; a12 += ex; a13 += ey;
+            a22 += ex * ex; a23 += ex * ey; a33 += ey * ey;
+            fw += h; f1w += h * ex; f2w += h * ey;
+        }
+        double a21 = a12, a31 = a13, a32 = a23;
+        double det = (a11 * a22 * a33 + a12 * a23 * a31 + a13 * a21 * a32 - a13 * a22 * a31 - a12 * a21 * a33 -
+                      a11 * a23 * a32);
+        if (fabs(det) < 1e-30 || fabs(det) < 1e-12 * fabs(a11 * a22 * a33)) return col_H(m, eta, Dmin, node);
+        double inv = 1.0 / fabs(det);
+        double b11 = a22 * a33 - a23 * a32;
+        double b12 = a13 * a32 - a12 * a33;
+        double b13 = a12 * a23 - a13 * a22;
+        double H = (b11 * fw + b12 * f1w + b13 * f2w) * inv;
+        if (H < Dmin) H = Dmin;
+        return H;
+    }
+    if (method == 2) {
+        int verts[4];
+        double wts[4];
+        int nv = bary_locate(m, xp, yp, node, verts, wts);
+        if (nv == 0) return col_H(m, eta, Dmin, node);
+        double H = 0.0;
+        for (int v = 0; v < nv; v++) H += wts[v] * col_H(m, eta, Dmin, verts[v]);
+        return H;
+    }
+    if (method == 3) {
+        int buf[64];
+        int count = collect_neighbours(m, node, buf);
+        if (count < 1) return col_H(m, eta, Dmin, node);
+        double z0 = M_COS(yp * DEG2R) * SLM;
+        if (z0 < 1.0) z0 = 1.0;
+        double sw = 0.0, swh = 0.0;
+        for (int i = 0; i < count; i++) {
+            int nd = buf[i];
+            double h = col_H(m, eta, Dmin, nd);
+            double dlon = PYMOD360(m->nodes_lon[nd] - xp + 180.0) - 180.0;
+            double dx = dlon * z0;
+            double dy = (m->nodes_lat[nd] - yp) * SLM;
+            double d2 = dx * dx + dy * dy;
+            if (d2 < 1.0) return h;
+            sw += 1.0 / d2;
+            swh += h / d2;
+        }
+        if (sw < 1e-30) return col_H(m, eta, Dmin, node);
+        return swh / sw;
+    }
+    return col_H(m, eta, Dmin, node);
+}
+
+/* vertical_sigma.py:971 interp_w_sigma_spatial for one particle */
+static double w_sigma(const or_mesh_t *m, const or_fields_t *f, const double *eta, double Dmin, int method,
+                      double alpha, double xp, double yp, double z, int node, double time_dir)
+{
+    const double DEG2R = 0.017453292519943295;
+    if (method == 0) return w_sig_linear(m, f, alpha, z, node, col_H(m, eta, Dmin, node), time_dir);
+    if (method == 2) {
+        int verts[4];
+        double wts[4];
+        int nv = bary_locate(m, xp, yp, node, verts, wts);
+        if (nv == 0) return w_sig_own(m, f, eta, Dmin, alpha, z, node, time_dir);
+        double Hp = 0.0;
+        for (int v = 0; v < nv; v++) Hp += wts[v] * col_H(m, eta, Dmin, verts[v]);
+        double frac = sigma_frac(z, Hp);
+        double acc = 0.0;
+        for (int v = 0; v < nv; v++) {
+            double Hv = col_H(m, eta, Dmin, verts[v]);
+            acc += wts[v] * w_sig_linear(m, f, alpha, frac * Hv, verts[v], Hv, time_dir);
+        }
+        return acc;
+    }
+    int buf[64];
+    int count = collect_neighbours(m, node, buf);
+    if (method == 3) {
+        if (count < 1) return w_sig_own(m, f, eta, Dmin, alpha, z, node, time_dir);
+        double z0 = M_COS(yp * DEG2R) * SLM;
+        if (z0 < 1.0) z0 = 1.0;
+        double frac = sigma_frac(z, column_height(m, eta, Dmin, 3, xp, yp, node));
+        double sum_w = 0.0, sum_wv = 0.0;
+        for (int i = 0; i < count; i++) {
+            int nd = buf[i];
+            double H = col_H(m, eta, Dmin, nd);
+            double wv = w_sig_linear(m, f, alpha, frac * H, nd, H, time_dir);
+            double dlon = PYMOD360(m->nodes_lon[nd] - xp + 180.0) - 180.0;
+            double dx = dlon * z0;
+            double dy = (m->nodes_lat[nd] - yp) * SLM;
+            double d2 = dx * dx + dy * dy;
+            if (d2 < 1.0) return wv;
+            double weight = 1.0 / d2;
+            sum_w += weight;
+            sum_wv += weight * wv;
+        }
+        if (sum_w < 1e-30) return w_sig_own(m, f, eta, Dmin, alpha, z, node, time_dir);
+        return sum_wv / sum_w;
+    }
+    /* vertical_sigma.py:656 leastsq */
+    if (count < 3) return w_sig_own(m, f, eta, Dmin, alpha, z, node, time_dir);
+    double z0 = M_COS(yp * DEG2R) * SLM;
+    if (z0 < 1.0) z0 = 1.0;
+    double frac = sigma_frac(z, column_height(m, eta, Dmin, 1, xp, yp, node));
+    double a11 = 0, a12 = 0, a13 = 0, a22 = 0, a23 = 0, a33 = 0, fw = 0, f1w = 0, f2w = 0;
+    for (int i = 0; i < count; i++) {
+        int nd = buf[i];
+        double H = col_H(m, eta, Dmin, nd);
+        double wv = w_sig_linear(m, f, alpha, frac * H, nd, H, time_dir);
+        double dlon = PYMOD360(m->nodes_lon[nd] - xp + 180.0) - 180.0;
+        double ex = dlon * z0;
+        double ey = (m->nodes_lat[nd] - yp) * SLM;
+        a11 += 1.0; a12 += ex; a13 += ey;
+        a22 += ex * ex; a23 += ex * ey; a33 += ey * ey;
+        fw += wv; f1w += wv * ex; f2w += wv * ey;
+    }
+    double a21 = a12, a31 = a13, a32 = a23;
+    double det = (a11 * a22 * a33 + a12 * a23 * a31 + a13 * a21 * a32 - a13 * a22 * a31 - a12 * a21 * a33 -
+                  a11 * a23 * a32);
+    if (fabs(det) < 1e-30 || fabs(det) < 1e-12 * fabs(a11 * a22 * a33))
+        return w_sig_own(m, f, eta, Dmin, alpha, z, node, time_dir);
+    double inv = 1.0 / fabs(det);
+    double b11 = a22 * a33 - a23 * a32;
+    double b12 = a13 * a32 - a12 * a33;
+    double b13 = a12 * a23 - a13 * a22;
+    return (b11 * fw + b12 * f1w + b13 * f2w) * inv;
+}
+
+/* One sigma stage: anchor, column height with the stage surface, the
+ * stage depth (z, or s H_p), the layer in that column, u/v, W. */
+static int sig_site(const or_mesh_t *m, const or_fields_t *f, const or_step_args_t *a, int stage, double lon,
+                    double lat, int from_s, double s_or_z, int *nn_o, double *H_o, double *U, double *V,
+                    double *W)
+{
+    int nn = or_site_anchor(m, lon, lat, a->exact);
+    *nn_o = nn;
+    const double *eta = a->sig_eta[stage];
+    double H = column_height(m, eta, a->Dmin, a->w_method, lon, lat, nn);
+    *H_o = H;
+    double zs = from_s ? s_or_z * H : s_or_z;
+    double Hl = H < a->Dmin ? a->Dmin : H;
+    int layer = find_layer_sigma(m->z_levels, m->n_levels, zs, Hl);
+    int ok;
+    if (f->uv_layout == 1)
+        ok = or_interp_lsq_tinterp(m, f, lon, lat, nn, a->time_dir, layer, a->alphas[stage], U, V);
+    else
+        ok = or_interp_bary_tinterp(m, f, lon, lat, nn, a->time_dir, layer, a->alphas[stage], U, V, NULL);
+    *W = w_sigma(m, f, eta, a->Dmin, a->w_method, a->alphas[stage], lon, lat, zs, nn, a->time_dir);
+    return ok;
+}
+
+/* vertical_sigma.py:237 _loop_clamp_depth_sigma_H */
+static double clamp_sigma(double z, double H, double Dmin, double z0b_min)
+{
+    if (H < Dmin) H = Dmin;
+    if (z < 0.0) z = 0.0;
+    double bottom = H - z0b_min;
+    if (bottom < 0.0) bottom = 0.0;
+    if (z > bottom) z = bottom;
+    return z;
+}
+
+/* The sigma step of one alive particle: Euler :4997, RK2 :5380, RK4 :5760
+ * (sigma branches), in place; returns 0 alive, else the failure code. */
+static int sigma_det_step(const or_mesh_t *m, const or_fields_t *f, const or_step_args_t *a, double *lon,
+                          double *lat, double *z, int32_t *st, int32_t *nn, int8_t *fr, double *Uo, double *Vo,
+                          double *Wo)
+{
+    const double dt = a->dt;
+    const double lon0 = *lon, lat0 = *lat, zz0 = *z;
+    int n1;
+    double H1, U1, V1, W1;
+    int ok = sig_site(m, f, a, 0, lon0, lat0, 0, zz0, &n1, &H1, &U1, &V1, &W1);
+    *Uo = U1;
+    *Vo = V1;
+    *Wo = W1;
+    double lon_new = lon0, lat_new = lat0, s_new = 0.0;
+    if (ok) {
+        if (a->order == 1) {
+            lon_new = wrap_lon(lon0 + dt * U1 / z0_of(lat0));
+            lat_new = clamp_lat(lat0 + dt * V1 / SLM);
+            s_new = (zz0 + W1 * dt) / H1;
+        } else if (a->order == 2) {
+            double z0 = z0_of(lat0);
+            double lon_mid = wrap_lon(lon0 + 0.5 * dt * U1 / z0);
+            double lat_mid = clamp_lat(lat0 + 0.5 * dt * V1 / SLM);
+            double s_mid = (zz0 + W1 * 0.5 * dt) / H1;
+            int n2;
+            double H2, U2, V2, W2;
+            ok = sig_site(m, f, a, 1, lon_mid, lat_mid, 1, s_mid, &n2, &H2, &U2, &V2, &W2);
+            if (ok) {
+                lon_new = wrap_lon(lon0 + dt * U2 / z0_of(lat_mid));
+                lat_new = clamp_lat(lat0 + dt * V2 / SLM);
+                s_new = zz0 / H1 + dt * W2 / H2;
+            }
+        } else {
+            double dlon1 = dt * U1 / z0_of(lat0);
+            double dlat1 = dt * V1 / SLM;
+            double s0 = zz0 / H1;
+            double f1 = W1 / H1;
+            double lon2 = wrap_lon(lon0 + 0.5 * dlon1);
+            double lat2 = clamp_lat(lat0 + 0.5 * dlat1);
+            int n2, n3, n4;
+            double H2, U2, V2, W2, H3, U3, V3, W3, H4, U4, V4, W4;
+            ok = sig_site(m, f, a, 1, lon2, lat2, 1, s0 + 0.5 * dt * f1, &n2, &H2, &U2, &V2, &W2);
+            if (ok) {
+                double f2 = W2 / H2;
+                double dlon2 = dt * U2 / z0_of(lat2);
+                double dlat2 = dt * V2 / SLM;
+                double lon3 = wrap_lon(lon0 + 0.5 * dlon2);
+                double lat3 = clamp_lat(lat0 + 0.5 * dlat2);
+                ok = sig_site(m, f, a, 2, lon3, lat3, 1, s0 + 0.5 * dt * f2, &n3, &H3, &U3, &V3, &W3);
+                if (ok) {
+                    double f3 = W3 / H3;
+                    double dlon3 = dt * U3 / z0_of(lat3);
+                    double dlat3 = dt * V3 / SLM;
+                    double lon4 = wrap_lon(lon0 + dlon3);
+                    double lat4 = clamp_lat(lat0 + dlat3);
+                    ok = sig_site(m, f, a, 3, lon4, lat4, 1, s0 + dt * f3, &n4, &H4, &U4, &V4, &W4);
+                    if (ok) {
+                        double f4 = W4 / H4;
+                        double dlon4 = dt * U4 / z0_of(lat4);
+                        double dlat4 = dt * V4 / SLM;
+                        double dlon = (dlon1 + 2.0 * dlon2 + 2.0 * dlon3 + dlon4) / 6.0;
+                        double dlat = (dlat1 + 2.0 * dlat2 + 2.0 * dlat3 + dlat4) / 6.0;
+                        lon_new = wrap_lon(lon0 + dlon);
+                        lat_new = clamp_lat(lat0 + dlat);
+                        s_new = s0 + dt * (f1 + 2.0 * f2 + 2.0 * f3 + f4) / 6.0;
+                    }
+                }
+            }
+        }
+    }
+    if (!ok) {
+        *st = ST_OUTSIDE;
+        *fr = FR_INTERP;
+        *nn = n1;
+        return FR_INTERP;
+    }
+    int nn_new = or_site_anchor(m, lon_new, lat_new, a->exact);
+    if (or_find_containing_local(m, lon_new, lat_new, nn_new) < 0) {
+        *st = ST_OUTSIDE;
+        *fr = FR_OOB;
+        *nn = n1;
+        return FR_OOB;
+    }
+    *lon = lon_new;
+    *lat = lat_new;
+    double H_end = column_height(m, a->sig_eta[4], a->Dmin, a->w_method, lon_new, lat_new, nn_new);
+    *z = clamp_sigma(s_new * H_end, H_end, a->Dmin, a->z0b_min);
+    *nn = nn_new;
+    return 0;
+}
+
 /* integrators.py:673 _apply_split_diffusion_kick for one particle alive
  * after the deterministic step; pre_* is the start-of-step state. */
 static void kick_one(const or_mesh_t *m, const or_step_args_t *a, uint64_t pid,
@@ -1489,7 +1895,11 @@ void or_step(const or_mesh_t *m, const or_fields_t *f, const or_step_args_t *a, 
         U[i] = 0.0;
         V[i] = 0.0;
         W[i] = 0.0;
-        if (prev == ST_ALIVE) {
+        if (prev == ST_ALIVE && a->is3d && m->vertical_coord == 1) {
+            sigma_det_step(m, f, a, &lon[i], &lat[i], &z[i], &status[i], &nn[i], &fr[i], &U[i], &V[i], &W[i]);
+            if ((a->periodic_lon || a->periodic_lat) && status[i] == ST_ALIVE)
+                periodic_one(m, a, &lon[i], &lat[i], &nn[i]);
+        } else if (prev == ST_ALIVE) {
             det_step(m, f, a, &lon[i], &lat[i], &z[i], &status[i], &nn[i], &fr[i], &U[i], &V[i], &W[i]);
             if (a->kick && status[i] == ST_ALIVE && !(a->kh_x == 0.0 && a->kh_y == 0.0))
                 kick_one(m, a, (uint64_t)pid[i], pre_lon, pre_lat, pre_z, pre_nn,

@@ -38,6 +38,7 @@ typedef struct {
     const double *z_levels;
     const uint8_t *elem_coastal;  /* (E,) coastal-element flags or NULL */
     const double *elem_center_lon, *elem_center_lat;  /* (E,) element centres */
+    int32_t vertical_coord;      /* 0 z-level; 1 sigma: z_levels holds sigma_lev (1 .. 0) */
 } or_mesh_t;
 
 /* Snapshot triple of one step: bracket pair (a, b) blended by alpha0 and
@@ -79,6 +80,11 @@ typedef struct {
     /* periodic fold windows [x0, x1) / [y0, y1) (tracking.py:1089) */
     int32_t periodic_lon, periodic_lat;
     double px0, px1, py0, py1;
+    /* sigma meshes: the surface elevation of stages 0..3 and of the step
+     * end, (N,) each (integrators.py:1381-1441), the column floor and the
+     * bed distance */
+    const double *sig_eta[5];
+    double Dmin, z0b_min;
 } or_step_args_t;
 
 #endif

@@ -34,6 +34,7 @@ class OrMesh(ctypes.Structure):
         ("r_earth", c_double),
         ("n_levels", c_int32), ("z_levels", c_void_p), ("elem_coastal", c_void_p),
         ("elem_center_lon", c_void_p), ("elem_center_lat", c_void_p),
+        ("vertical_coord", c_int32),
     ]
 
 
@@ -51,6 +52,7 @@ class OrStepArgs(ctypes.Structure):
         ("az", c_int32), ("az_const", c_double), ("use_eta", c_int32),
         ("periodic_lon", c_int32), ("periodic_lat", c_int32),
         ("px0", c_double), ("px1", c_double), ("py0", c_double), ("py1", c_double),
+        ("sig_eta", c_void_p * 5), ("Dmin", c_double), ("z0b_min", c_double),
     ]
 
 
@@ -59,8 +61,18 @@ RESUSPENSION_CODES = {"never": 0, "auto": 1, "stochastic": 2, "auto_or_stochasti
 
 def step_args(order, dt, time_dir, is3d=False, w_method=2, exact=True, alphas=(0, 0, 0, 0),
               kick=None, beach=None, step_i=0, real_id=0, periodic_lon=None,
-              periodic_lat=None) -> OrStepArgs:
+              periodic_lat=None, sigma=None) -> OrStepArgs:
+    """``sigma`` (sigma meshes): dict(etas=[stage 0..3 surfaces, end
+    surface] as (N,) arrays, Dmin, z0b_min); the arrays are kept alive by
+    the returned structure."""
     a = OrStepArgs()
+    if sigma is not None:
+        etas = [np.ascontiguousarray(e, dtype=np.float64) for e in sigma["etas"]]
+        a._sig_keep = etas
+        for k in range(5):
+            a.sig_eta[k] = etas[k].ctypes.data
+        a.Dmin = float(sigma.get("Dmin", 0.5))
+        a.z0b_min = float(sigma.get("z0b_min", 0.1))
     a.order, a.is3d, a.w_method, a.exact = int(order), int(is3d), int(w_method), int(exact)
     a.dt, a.time_dir = float(dt), float(time_dir)
     for k in range(4):
@@ -183,7 +195,8 @@ class OracleMesh:
             mesh.gh_nx, mesh.gh_ny, int(mesh.gh_wrap_lon), mesh.gh_far_rows,
             mesh.r_earth, int(mesh.sigma.size), k["z"].ctypes.data,
             None if k["coastal"] is None else k["coastal"].ctypes.data,
-            k["clon"].ctypes.data, k["clat"].ctypes.data)
+            k["clon"].ctypes.data, k["clat"].ctypes.data,
+            1 if getattr(mesh, "vertical_coord", "zlevel") == "sigma" else 0)
 
     def nearest(self, lon, lat, exact=True) -> np.ndarray:
         lon = _c(lon, np.float64)
@@ -317,4 +330,39 @@ def math_probe(fn: int, x, y=None, flavour="glibc") -> np.ndarray:
     y = x if y is None else _c(y, np.float64)
     out = np.empty_like(x)
     lib(flavour).or_math_probe(int(fn), x.size, x.ctypes.data, y.ctypes.data, out.ctypes.data)
+    return out
+
+
+def _blend(a, b, alpha):
+    """core/io.py:68 _blend: (1 - alpha) a + alpha b."""
+    return (1.0 - alpha) * np.asarray(a, dtype=np.float64) + alpha * np.asarray(b, dtype=np.float64)
+
+
+def stage_field(a, b, t, alpha0, alpha):
+    """A stage's field value (integrators.py:1187 _make_stage_vels over the
+    bracket blend core/io.py:386): the bracket (a, b) blended by alpha0,
+    then towards t by alpha with the 1e-12 short-circuits."""
+    base = a if (b is None or alpha0 == 0.0) else _blend(a, b, alpha0)
+    if alpha < 1e-12:
+        return np.asarray(base, dtype=np.float64)
+    if alpha > 1.0 - 1e-12:
+        return np.asarray(t, dtype=np.float64)
+    return _blend(base, t, alpha)
+
+
+def sigma_stage_etas(order, steady, eta_start, eta_end, a=None, b=None, t=None, alpha0=0.0,
+                     alphas=(0.0, 0.0, 0.0, 0.0)):
+    """The surfaces of one sigma step (integrators.py:5426-5498 RK2,
+    :5806-5816 RK4): stage 0 = the start surface; later stages the stage
+    field's eta, or on a steady step integrators.py:1369 _eta_blend of the
+    start and end surfaces at c; [stage 0..3, end]."""
+    e0 = np.asarray(eta_start, dtype=np.float64)
+    ee = e0 if eta_end is None else np.asarray(eta_end, dtype=np.float64)
+    out = [e0, e0, e0, e0, ee]
+    cs = {2: (0.5,), 4: (0.5, 0.5, 1.0)}.get(int(order), ())
+    for k, c in enumerate(cs, start=1):
+        if steady:
+            out[k] = e0 if c <= 0.0 else (ee if c >= 1.0 else _blend(e0, ee, c))
+        else:
+            out[k] = stage_field(a, b, t, alpha0, alphas[k])
     return out

@@ -332,3 +332,64 @@ def sampling_case(direction):
                 start_time_idx=0 if direction > 0 else 4, n_steps=8, n_time_total=5,
                 file_times=np.arange(5) * RUN_FILE_DT, file_dt=RUN_FILE_DT,
                 sample_vars=("s_lev", "s_mid"))
+
+
+def sigma_case(direction):
+    """tests/_gen_golden_3d.py:527 (fesomc_sigma_rk2): FESOM-C sigma
+    coordinates on the 3 x 3 node grid of tests/test_fesomc_w_contract.py:49
+    -- stretched sigma 1 - linspace(0, 1, 11)^2, a sloping bottom, a tide, a
+    dia-sigma flux and layered element u/v; RK2 with a linear time blend,
+    W and the column height by least squares."""
+    from kelp_amd.config import ModeConfig, NumericsConfig
+    from kelp_amd.particles import make_state
+    from kelp_amd.readers import ArrayDataset, FesomCArrayReader
+    xs, ys = np.array([4.0, 5.0, 6.0]), np.array([-1.0, 0.0, 1.0])
+    gx, gy = np.meshgrid(xs, ys)
+    lon_n, lat_n = gx.ravel(), gy.ravel()
+    tris = []
+    for j in range(2):
+        for i in range(2):
+            n0 = j * 3 + i
+            tris += [[n0, n0 + 1, n0 + 3], [n0 + 1, n0 + 4, n0 + 3]]
+    tris = np.array(tris)
+    nv = np.column_stack([tris + 1, tris[:, :1] + 1]).astype(np.int32)
+    sig = 1.0 - np.linspace(0.0, 1.0, 11) ** 2
+    K, E = sig.size, tris.shape[0]
+    n_time = 7
+    times = np.arange(n_time) * 900.0
+    tide = 0.8 * np.sin(2.0 * np.pi * times / 5400.0)
+    eta = tide[:, None] * (1.0 + 0.1 * (lon_n - 5.0))[None, :]
+    depth = 12.0 + 6.0 * (lat_n + 1.0) + 2.0 * (lon_n - 4.0)
+    H = depth[None, :] + eta
+    wvel = 2e-5 * H[:, None, :] * np.sin(np.pi * (1.0 - sig))[None, :, None]
+    wvel[:, 0, :] = 0.0
+    wvel[:, -1, :] = 0.0
+    kk = ((np.arange(K - 1) + 0.5) / (K - 1))[None, None, :]
+    lonc = lon_n[tris].mean(1)[None, :, None]
+    latc = lat_n[tris].mean(1)[None, :, None]
+    ph = 2.0 * np.pi * times[:, None, None] / 5400.0
+    u_e = 0.25 * (1.0 - 0.5 * kk) * (1.0 + 0.3 * (lonc - 5.0)) + 0.05 * np.cos(ph)
+    v_e = 0.15 * (1.0 + 0.4 * kk) * (1.0 + 0.2 * latc) - 0.04 * np.sin(ph)
+    shp = (n_time, E, K - 1)
+    ds = ArrayDataset({
+        "lon": (["node"], lon_n), "lat": (["node"], lat_n), "depth": (["node"], depth),
+        "nv": (["nele", "four"], nv),
+        "lon_elem": (["nele"], lon_n[tris].mean(1)), "lat_elem": (["nele"], lat_n[tris].mean(1)),
+        "sigma_lev": (["sigma"], sig),
+        "u": (["time", "nele", "sigmam1"], np.ascontiguousarray(np.broadcast_to(u_e, shp))),
+        "v": (["time", "nele", "sigmam1"], np.ascontiguousarray(np.broadcast_to(v_e, shp))),
+        "w": (["time", "node", "sigma"], np.ascontiguousarray(np.transpose(wvel, (0, 2, 1)))),
+        "eta": (["time", "node"], eta),
+    }, time=times)
+    mesh = FesomCArrayReader.load_mesh(ds)
+    lon = np.concatenate([np.repeat([4.6, 5.0, 5.4], 3), [5.996, 5.998]])
+    lat = np.concatenate([np.tile([-0.4, 0.0, 0.4], 3), [0.3, -0.4]])
+    frac = np.concatenate([np.tile([0.1, 0.4, 0.8], 3), [0.5, 0.95]])
+    h0 = 12.0 + 6.0 * (lat + 1.0) + 2.0 * (lon - 4.0)
+    numerics = NumericsConfig(dt=450.0, integrator="rk2", output_freq=1,
+                              temporal_reconstruction="linear", w_interp="leastsq",
+                              temporal_interp=True)
+    mode = ModeConfig(Fdrift2D=False, Fdrift3D=True, time_dir=direction)
+    return dict(mesh=mesh, ds=ds, init=make_state(lon, lat, frac * h0), mode=mode,
+                numerics=numerics, start_time_idx=0 if direction > 0 else n_time - 1, n_steps=12,
+                n_time_total=n_time, file_times=times, file_dt=900.0)

@@ -120,3 +120,24 @@ def test_staggered_release_run(gpu_ctx):
         assert (st["status"][sel, r] != 4).all()
         moved = st["lon"][sel, r] != init.lon[sel]
         assert moved[st["status"][sel, r] == 1].all()
+
+
+@pytest.mark.parametrize("direction", [1.0, -1.0])
+def test_sigma_run_matches_reference_golden(gpu_ctx, direction):
+    """FESOM-C sigma coordinates (tests/_gen_golden_3d.py:527): stretched
+    sigma, sloping bottom, a tide and a dia-sigma flux, element u/v by least
+    squares, W and the column height by least squares, RK2 with a linear
+    time blend; particles leave through the east edge."""
+    case = rc.sigma_case(direction)
+    gold = np.load(os.path.join(GOLD, "fesomc_sigma_rk2" + ("_bwd" if direction < 0 else "")
+                                + ".npz"), allow_pickle=False)
+    rec = run_single_realization(
+        case["mesh"], case["ds"], get_reader("fesomc_arrays"), case["init"], case["mode"],
+        case["numerics"], start_time_idx=case["start_time_idx"], n_steps=case["n_steps"],
+        n_time_total=case["n_time_total"], file_times=case["file_times"],
+        file_dt=case["file_dt"], ctx=gpu_ctx).as_arrays()
+    np.testing.assert_array_equal(rec["status"], gold["status"])
+    np.testing.assert_array_equal(rec["failure_reason"], gold["failure_reason"])
+    for k, src in (("lon", "lon"), ("lat", "lat"), ("z", "z"), ("u", "vel_u"), ("v", "vel_v"),
+                   ("w", "vel_w")):
+        np.testing.assert_allclose(rec[src], gold[k], rtol=1e-9, atol=1e-9, err_msg=k)

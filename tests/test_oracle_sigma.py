@@ -1,0 +1,207 @@
+"""The oracle's sigma-coordinate step (CPU).
+
+Pins the sigma branch of the oracle (numerics/vertical_sigma.py and the
+sigma branches of the multi-pass steppers, integrators.py:1310-1441) against
+the reference's run-level golden ``fesomc_sigma_rk2`` (forward and
+backward): a FESOM-C file on a 3 x 3 node grid with a stretched sigma
+coordinate, sloping bottom, a tide and a dia-sigma flux, element u/v by
+least squares, W and the column height by least squares, RK2 with a linear
+time blend, particles leaving through the east edge.  The loop below walks
+the reference's clock for this case (tracking.py:2384-2833: the time
+bracket, the record landing, the surfaces at each step's start and end from
+one function of time, the output masks); the oracle does the step.  Bar:
+integers exact, floats within rtol = atol = 1e-9 (the reference's
+cross-environment rule).
+
+Closed forms (the reference's own contract tests,
+tests/test_fesomc_w_contract.py): with no dia-sigma flux a particle keeps its
+sigma fraction z / H exactly while the surface moves; with a uniform
+d(sigma)/dt = c the fraction drifts by c t.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from kelp_amd.readers import get_reader
+from kelp_amd.tracking import (mask_record, resolve_time_bracket, snapshot_band,
+                               step_end_on_record, vel_payload)
+from oracle.oracle import OracleMesh, _blend, sigma_stage_etas, stage_field, step_args
+
+from .golden import reference_cases as rc
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_golden_3d")
+_C = {1: (0.0,), 2: (0.0, 0.5), 4: (0.0, 0.5, 0.5, 1.0)}
+
+
+def _eta_at(t, fields, file_times, n, direction, snap):
+    """tracking.py:3097 _eta_at_time (linear, stop mode)."""
+    br = resolve_time_bracket(t, file_times, n, "stop", direction, snap)
+    if br is None:
+        return None
+    ia, ib, ta, tb, tw = br
+    interval = tb - ta
+    alpha = (tw - ta) / interval if interval > 0 else 0.0
+    if ia == ib or abs(alpha) < 1e-12:
+        return fields[ia].eta
+    if abs(alpha - 1.0) < 1e-12:
+        return fields[ib].eta
+    return _blend(fields[ia].eta, fields[ib].eta, alpha)
+
+
+def run_sigma_oracle(case, order=None, w_interp=None, flavour="glibc", uv_override=None):
+    """The case's tracking loop with the oracle doing each step; returns the
+    masked records ``(particle, record)``."""
+    mesh, ds, mode, numerics = case["mesh"], case["ds"], case["mode"], case["numerics"]
+    order = order or {"euler": 1, "rk2": 2, "rk4": 4}[numerics.integrator]
+    w_method = {"nearest": 0, "leastsq": 1, "barycentric": 2, "idw": 3}[w_interp or numerics.w_interp]
+    reader = get_reader("fesomc_arrays")
+    file_times = np.asarray(case["file_times"], dtype=np.float64)
+    n = case["n_time_total"]
+    fields = {i: reader.read_velocities(ds, i, mesh, mode) for i in range(n)}
+    if uv_override is not None:
+        for i, f in fields.items():
+            f.u, f.v = uv_override(i, f)
+    om = OracleMesh(mesh, flavour)
+    direction = 1 if mode.time_dir >= 0 else -1
+    dt = float(numerics.dt)
+    snap = snapshot_band(dt) if order > 1 else 5e-10 * max(dt, 1.0)
+    init = case["init"]
+    P = init.n_particles
+    state = dict(lon=init.lon.copy(), lat=init.lat.copy(), z=init.z.copy(),
+                 status=np.ones(P, np.int32), nn=np.zeros(P, np.int32), fr=np.zeros(P, np.int8))
+    sim_time = file_times[case["start_time_idx"]]
+    rows = []
+    for _ in range(case["n_steps"]):
+        br = resolve_time_bracket(sim_time, file_times, n, "stop", direction, snap)
+        if br is None:
+            break
+        pay = vel_payload(br, direction)
+        step_len, sim_time_end, dtn = dt, None, pay.dt_to_next
+        if order > 1 and dtn is not None and abs(dt - dtn) <= snap:
+            step_len = dtn
+            sim_time_end = step_end_on_record(sim_time, br[4], dtn, direction, file_times, n, "stop")
+        elif dtn is not None and dtn < dt <= dtn + snap:
+            dtn = dt
+        if dtn is not None and step_len > dtn:
+            pay.it, dtn = None, None
+        alphas = [0.0] * 4
+        if dtn is not None:
+            alphas = [c * step_len / dtn for c in _C[order]] + [0.0] * (4 - order)
+        fa = fields[pay.ia]
+        fb = fields[pay.ib] if pay.ib is not None else fa
+        ft = fields[pay.it] if pay.it is not None else fa
+        e_start = _eta_at(sim_time, fields, file_times, n, direction, snap)
+        t_end = sim_time_end if sim_time_end is not None else sim_time + dt * direction
+        e_end = _eta_at(t_end, fields, file_times, n, direction, snap)
+        steady = pay.it is None
+        if e_end is None and not steady:
+            e_end = stage_field(fa.eta, fb.eta, ft.eta, pay.alpha0,
+                                alphas[3] if order == 4 else step_len / dtn)
+        etas = sigma_stage_etas(order, steady, e_start, e_end, fa.eta, fb.eta, ft.eta, pay.alpha0,
+                                alphas)
+        flds = dict(u_a=fa.u, v_a=fa.v, w_a=fa.w, u_b=fb.u, v_b=fb.v, w_b=fb.w, u_t=ft.u,
+                    v_t=ft.v, w_t=ft.w, alpha0=pay.alpha0, uv_layout=fa.uv_layout)
+        args = step_args(order, step_len, float(direction), is3d=True, w_method=w_method, exact=2,
+                         alphas=alphas, sigma=dict(etas=etas, Dmin=numerics.Dmin,
+                                                   z0b_min=numerics.z0b_min))
+        prev = state["status"].copy()
+        state = om.step(state, flds, args)
+        rows.append(mask_record(state, prev, direction, True))
+        sim_time = sim_time_end if sim_time_end is not None else sim_time + dt * direction
+    return {k: np.stack([r[k] for r in rows], axis=1) for k in ("lon", "lat", "z", "status", "fr",
+                                                               "u", "v", "w")}
+
+
+@pytest.mark.parametrize("flavour", ["glibc", "kmath"])
+@pytest.mark.parametrize("direction", [1.0, -1.0])
+def test_sigma_oracle_matches_reference_golden(direction, flavour):
+    gold = np.load(os.path.join(GOLD, "fesomc_sigma_rk2" + ("_bwd" if direction < 0 else "")
+                                + ".npz"), allow_pickle=False)
+    rec = run_sigma_oracle(rc.sigma_case(direction), flavour=flavour)
+    np.testing.assert_array_equal(rec["status"], gold["status"])
+    np.testing.assert_array_equal(rec["fr"], gold["failure_reason"])
+    for k in ("lon", "lat", "z", "u", "v", "w"):
+        np.testing.assert_allclose(rec[k], gold[k], rtol=1e-9, atol=1e-9, err_msg=k)
+
+
+def _column_case(wvel_fn, integrator, direction, flat=False):
+    """The golden's grid with u = v = 0, a moving surface and a chosen
+    dia-sigma flux (positive up in the file); ``flat``: a 20 m bed and a
+    spatially uniform tide."""
+    case = rc.sigma_case(direction)
+    ds, mesh = case["ds"], case["mesh"]
+    K = mesh.sigma.size
+    for name in ("u", "v"):
+        dims, arr = ds.vars[name]
+        ds.vars[name] = (dims, np.zeros_like(arr))
+    if flat:
+        mesh.depth[:] = 20.0
+        dims, eta = ds.vars["eta"]
+        ds.vars["eta"] = (dims, np.ascontiguousarray(np.broadcast_to(eta[:, 4:5], eta.shape)))
+    dims, w = ds.vars["w"]
+    H = mesh.depth[None, :] + ds.vars["eta"][1]
+    ds.vars["w"] = (dims, np.ascontiguousarray(np.transpose(wvel_fn(H, K), (0, 2, 1))))
+    case["numerics"].integrator = integrator
+    return case
+
+
+def _tide(t):
+    """The recipe's tide, linear in time between the 900 s records as the
+    tracker sees it."""
+    rec_t = np.arange(7) * 900.0
+    return np.interp(t, rec_t, 0.8 * np.sin(2.0 * np.pi * rec_t / 5400.0))
+
+
+@pytest.mark.parametrize("integrator", ["euler", "rk2", "rk4"])
+@pytest.mark.parametrize("w_interp,flat", [("leastsq", False), ("barycentric", False),
+                                           ("nearest", True), ("idw", True)])
+def test_sigma_fraction_kept_without_flux(integrator, w_interp, flat):
+    """No dia-sigma flux and no horizontal flow: z / H stays at its start
+    value while the tide moves the surface.  On the sloping grid H is linear
+    in lon/lat, which leastsq and barycentric reproduce exactly; nearest and
+    idw on a flat bed with a uniform tide."""
+    case = _column_case(lambda H, K: np.zeros((H.shape[0], K, H.shape[1])), integrator, 1.0, flat)
+    rec = run_sigma_oracle(case, w_interp=w_interp)
+    init = case["init"]
+    assert (rec["status"] == 1).all()
+    lon, lat = init.lon, init.lat
+    t_rec = 450.0 * np.arange(1, rec["z"].shape[1] + 1)
+    if flat:
+        depth = np.full(lon.shape, 20.0)
+        slope = np.ones(lon.shape)
+    else:
+        depth = 12.0 + 6.0 * (lat + 1.0) + 2.0 * (lon - 4.0)
+        slope = 1.0 + 0.1 * (lon - 5.0)
+    H = depth[:, None] + _tide(t_rec)[None, :] * slope[:, None]
+    s0 = init.z / depth
+    np.testing.assert_allclose(rec["z"] / H, np.broadcast_to(s0[:, None], H.shape), rtol=1e-9)
+    # u = v = 0: only the longitude fold's rounding moves a particle
+    np.testing.assert_allclose(rec["lon"], np.broadcast_to(lon[:, None], H.shape), atol=1e-12)
+
+
+@pytest.mark.parametrize("integrator", ["euler", "rk2", "rk4"])
+def test_sigma_uniform_flux_closed_form(integrator):
+    """A uniform d(sigma)/dt = c (file Wvel = c H on the inner interfaces, 0
+    at the surface and the bed; tests/test_fesomc_w_contract.py:282): the
+    fraction moves by -c t for every integrator while it stays between the
+    inner interfaces (flat bed, uniform tide, nearest columns)."""
+    c = 1e-5
+
+    def wvel(H, K):
+        w = c * np.broadcast_to(H[:, None, :], (H.shape[0], K, H.shape[1])).copy()
+        w[:, 0, :] = 0.0
+        w[:, -1, :] = 0.0
+        return w
+    case = _column_case(wvel, integrator, 1.0, flat=True)
+    init = case["init"]
+    rec = run_sigma_oracle(case, w_interp="nearest")
+    assert (rec["status"] == 1).all()
+    t = 450.0 * np.arange(1, rec["z"].shape[1] + 1)
+    H = 20.0 + _tide(t)
+    s0 = init.z / 20.0
+    # the inner interfaces of 1 - linspace(0, 1, 11)^2 span s in [0.01, 0.81]
+    keep = (s0 < 0.8) & (s0 - c * t[-1] > 0.02)
+    assert keep.sum() >= 4
+    s_exact = s0[keep][:, None] - c * t[None, :]
+    np.testing.assert_allclose(rec["z"][keep] / H[None, :], s_exact, rtol=1e-9)
