@@ -827,8 +827,6 @@ static int sigma_prepare(kelp_ctx* c, const kelp_step_args* a, bool is3d, const 
                          const FieldSlot* st, StepCfg& cfg)
 {
     if (!is3d) return fail(KELP_EINVAL, "kelp_step: a sigma mesh runs 3-D fields (W on the sigma interfaces)");
-    if (a->kick || a->az || a->beach)
-        return fail(KELP_EINVAL, "kelp_step: the diffusion kicks and beaching on sigma meshes are not implemented");
     if (!(a->Dmin > 0.0) || !(a->z0b_min >= 0.0))
         return fail(KELP_EINVAL, "kelp_step: sigma steps need Dmin > 0 and z0b_min >= 0");
     const bool steady = a->slot_t < 0;

@@ -4,6 +4,7 @@
 // of one (3-D, dtype, layout) combination per object file.
 #pragma once
 
+#include "kelp_sigma.cuh"
 #include "kelp_step.cuh"
 
 namespace kelp {
@@ -190,7 +191,10 @@ __device__ inline int det_step(const DMesh& m, const Snap<T, IS3D>& f, const Sna
 
 // integrators.py:673 _apply_split_diffusion_kick for one particle alive after
 // the deterministic step (uniform Kh, counter RNG keyed by the particle id).
-template <int MAXV, bool IS3D>
+// SIGMA: the particle keeps its sigma fraction in the end-of-step surface,
+// z / H_p(post-step position) * H_p(kicked position), clamped in the new
+// column (integrators.py:784-801).
+template <int MAXV, bool IS3D, bool SIGMA = false>
 __device__ inline void kick_one(const DMesh& m, const StepCfg& c, uint64_t pid, double pre_lon,
                                       double pre_lat, double pre_z, int pre_nn, double& lon, double& lat,
                                       double& z, int32_t& st, int& nn, int8_t& fr)
@@ -224,10 +228,16 @@ __device__ inline void kick_one(const DMesh& m, const StepCfg& c, uint64_t pid, 
         fr = kFrOOB;
         return;
     }
+    if constexpr (SIGMA) {
+        const double Hp = column_height<MAXV>(m, c.sig_eta[4], c.Dmin, c.w_method, lon, lat, nn);
+        const double Hk = column_height<MAXV>(m, c.sig_eta[4], c.Dmin, c.w_method, lon_k, lat_k, nn_k);
+        z = clamp_sigma(z / Hp * Hk, Hk, c.Dmin, c.z0b_min);
+    } else if (IS3D) {
+        z = clamp_z(m, z, nn_k);
+    }
     lon = lon_k;
     lat = lat_k;
     nn = nn_k;
-    if (IS3D) z = clamp_z(m, z, nn_k);
 }
 
 // integrators.py:842 _apply_split_az_kick, constant source on a z-level
@@ -243,6 +253,26 @@ __device__ inline void az_one(const DMesh& m, const StepCfg& c, uint64_t pid, in
     const double g = 0.0;
     const double dz = g * abs_dt + km_sqrt(2.0 * c.az_const * abs_dt) * xi;
     z = km_reflect_depth(z + dz, m.zlev[0], m.depth[node]);
+}
+
+// integrators.py:842 _apply_split_az_kick on a sigma mesh (constant source,
+// numerics/diffusion.py:459 _loop_az_kick_sigma): folded into
+// [0, max(H_p, Dmin) - z0b_min] of the end-of-step column.
+template <int MAXV>
+__device__ inline void az_sigma_one(const DMesh& m, const StepCfg& c, uint64_t pid, double lon, double lat, int node,
+                                    double& z)
+{
+    double H = column_height<MAXV>(m, c.sig_eta[4], c.Dmin, c.w_method, lon, lat, node);
+    if (H < c.Dmin) H = c.Dmin;
+    double bot = H - c.z0b_min;
+    if (bot < 0.0) bot = 0.0;
+    double xi, unused;
+    normal_pair(c.seed, c.real_id, pid, c.step_i, 1, xi, unused);
+    const double sdt = c.time_dir * c.dt;
+    const double abs_dt = sdt >= 0.0 ? sdt : -sdt;
+    const double g = 0.0;
+    const double dz = g * abs_dt + km_sqrt(2.0 * c.az_const * abs_dt) * xi;
+    z = km_reflect_depth(z + dz, 0.0, bot);
 }
 
 // tracking.py:1089 _apply_periodic_wrap for one ALIVE particle: fold a

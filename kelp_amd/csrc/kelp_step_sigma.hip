@@ -150,7 +150,17 @@ __global__ __launch_bounds__(256) void sigma_step_kernel(DMesh m, Snap<T, true> 
     }
     double U = 0.0, V = 0.0, W = 0.0;
     if (prev != kAlive) {
-        // the diagnostic velocity of a particle that does not move is 0
+        // the diagnostic velocity of a particle that does not move is 0;
+        // beaching still looks at beached particles (resuspension)
+        int32_t st = prev;
+        int nn = s.nn[i];
+        int8_t fr = s.fr[i];
+        if (c.beach) {
+            beach_one<MAXV>(m, c, (uint64_t)s.pid[i], prev, s.lon[i], s.lat[i], st, nn, fr, s.ws[i]);
+            s.status[i] = st;
+            s.nn[i] = nn;
+            s.fr[i] = fr;
+        }
         s.u[i] = U;
         s.v[i] = V;
         if (s.w) s.w[i] = W;
@@ -160,12 +170,19 @@ __global__ __launch_bounds__(256) void sigma_step_kernel(DMesh m, Snap<T, true> 
     double lon = s.lon[i], lat = s.lat[i], z = s.z[i];
     int nn = s.nn[i];
     int8_t fr = s.fr[i];
+    const double pre_lon = lon, pre_lat = lat, pre_z = z;
+    const int pre_nn = nn;
     const int code = sigma_det_step<ORDER, MAXV, T, ELEM>(m, f, fe, c, lon, lat, z, nn, U, V, W);
     if (code) {
         st = kOutside;
         fr = (int8_t)code;
+    } else if (c.kick && !(c.kh_x == 0.0 && c.kh_y == 0.0)) {
+        kick_one<MAXV, true, true>(m, c, (uint64_t)s.pid[i], pre_lon, pre_lat, pre_z, pre_nn, lon, lat, z, st, nn,
+                                   fr);
     }
+    if (c.az && st == kAlive) az_sigma_one<MAXV>(m, c, (uint64_t)s.pid[i], lon, lat, nn, z);
     if ((c.periodic_lon || c.periodic_lat) && st == kAlive) periodic_one(m, c, lon, lat, nn);
+    if (c.beach) beach_one<MAXV>(m, c, (uint64_t)s.pid[i], prev, lon, lat, st, nn, fr, s.ws[i]);
     s.lon[i] = lon;
     s.lat[i] = lat;
     s.z[i] = z;

@@ -255,8 +255,6 @@ def _check_path(numerics: NumericsConfig, mesh: Mesh, is3d: bool, uv_layout: str
         methods = SIGMA_W_METHODS if mesh.vertical_coord == "sigma" else W_METHODS
         if numerics.w_interp not in methods:
             raise NotImplementedError(f"device path: w_interp={numerics.w_interp!r}")
-        if mesh.vertical_coord == "sigma" and (numerics.diffusion.enabled or numerics.beaching.enabled):
-            raise NotImplementedError("device path: diffusion or beaching on a sigma mesh")
     if numerics.beaching.enabled and numerics.beaching.dry_mask_var:
         raise NotImplementedError("device path: beaching dry_mask_var")
 

@@ -1744,10 +1744,18 @@ static void kick_one(const or_mesh_t *m, const or_step_args_t *a, uint64_t pid,
         *fr = FR_OOB;
         return;
     }
+    if (a->is3d && m->vertical_coord == 1) {
+        /* integrators.py:784-801: keep s = z / H_p in the end-of-step
+         * surface, clamp in the kicked column */
+        double Hp = column_height(m, a->sig_eta[4], a->Dmin, a->w_method, *lon, *lat, *nn);
+        double Hk = column_height(m, a->sig_eta[4], a->Dmin, a->w_method, lon_k, lat_k, nn_k);
+        *z = clamp_sigma(*z / Hp * Hk, Hk, a->Dmin, a->z0b_min);
+    } else if (a->is3d) {
+        *z = clamp_z(m, *z, nn_k);
+    }
     *lon = lon_k;
     *lat = lat_k;
     *nn = nn_k;
-    if (a->is3d) *z = clamp_z(m, *z, nn_k);
 }
 
 #ifdef ORACLE_KMATH
@@ -1783,6 +1791,25 @@ static double REFLECT_DEPTH(double z, double top, double bot)
  * mesh: numerics/diffusion.py:427 _loop_az_kick_zlevel (gradient 0) and
  * :302 diffusion_az_step_meters, one particle alive after the horizontal
  * kick.  The single normal is the first Box-Muller half of component 1. */
+/* numerics/diffusion.py:459 _loop_az_kick_sigma (integrators.py:909-928):
+ * folded into [0, max(H_p, Dmin) - z0b_min] of the end-of-step column */
+static void az_sigma_one(const or_mesh_t *m, const or_step_args_t *a, uint64_t pid, double lon, double lat,
+                         int32_t node, double *z)
+{
+    double az = a->az_const < 0.0 ? 0.0 : a->az_const;
+    double H = column_height(m, a->sig_eta[4], a->Dmin, a->w_method, lon, lat, node);
+    if (H < a->Dmin) H = a->Dmin;
+    double bot = H - a->z0b_min;
+    if (bot < 0.0) bot = 0.0;
+    double g = 0.0;
+    double xi, unused;
+    or_normal_pair(a->seed, a->real_id, pid, a->step_i, 1, &xi, &unused);
+    double sdt = a->time_dir * a->dt;
+    double abs_dt = sdt >= 0.0 ? sdt : -sdt;
+    double dz = g * abs_dt + sqrt(2.0 * az * abs_dt) * xi;
+    *z = REFLECT_DEPTH(*z + dz, 0.0, bot);
+}
+
 static void az_one(const or_mesh_t *m, const or_step_args_t *a, uint64_t pid, int32_t node, double *z)
 {
     double az = a->az_const < 0.0 ? 0.0 : a->az_const;
@@ -1897,6 +1924,11 @@ void or_step(const or_mesh_t *m, const or_fields_t *f, const or_step_args_t *a, 
         W[i] = 0.0;
         if (prev == ST_ALIVE && a->is3d && m->vertical_coord == 1) {
             sigma_det_step(m, f, a, &lon[i], &lat[i], &z[i], &status[i], &nn[i], &fr[i], &U[i], &V[i], &W[i]);
+            if (a->kick && status[i] == ST_ALIVE && !(a->kh_x == 0.0 && a->kh_y == 0.0))
+                kick_one(m, a, (uint64_t)pid[i], pre_lon, pre_lat, pre_z, pre_nn,
+                         &lon[i], &lat[i], &z[i], &status[i], &nn[i], &fr[i]);
+            if (a->az && a->az_const > 0.0 && status[i] == ST_ALIVE)
+                az_sigma_one(m, a, (uint64_t)pid[i], lon[i], lat[i], nn[i], &z[i]);
             if ((a->periodic_lon || a->periodic_lat) && status[i] == ST_ALIVE)
                 periodic_one(m, a, &lon[i], &lat[i], &nn[i]);
         } else if (prev == ST_ALIVE) {

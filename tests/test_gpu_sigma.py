@@ -129,3 +129,46 @@ def test_sigma_step_bitwise(gpu_ctx, world, order, w_method, layout):
         for k in ("lon", "lat", "z", "u", "v", "w"):
             np.testing.assert_array_equal(got[k].view(np.int64), want[k].view(np.int64),
                                           err_msg=f"{k} {tag}")
+
+
+@pytest.mark.parametrize("order,w_method,layout", [(2, 1, "element"), (4, 2, "node"), (1, 3, "element")])
+def test_sigma_kicks_and_beaching_bitwise(gpu_ctx, world, order, w_method, layout):
+    """The split Kh kick (sigma fraction kept in the end surface), the Az
+    kick folded into the end column and depth/on-failure beaching with the
+    surface, on time-blended sigma steps, three steps in a row."""
+    mesh, snaps, lon, lat, z0, pid = world
+    gpu_ctx.upload_mesh(mesh)
+    uk, vk = ("un", "vn") if layout == "node" else ("ue", "ve")
+    for s, sn in enumerate(snaps):
+        gpu_ctx.upload_field(s, sn[uk], sn[vk], sn["w"], uv_layout=layout)
+        gpu_ctx.upload_eta(s, sn["eta"])
+    om = OracleMesh(mesh, "kmath")
+    kick = dict(kh_x=20.0, kh_y=8.0, seed=123, az_const=5e-4)
+    beach = dict(criteria=["depth", "on_failure"], depth_min=1.2, use_eta=True, resuspension="auto",
+                 min_wet_steps=1, dt=DT)
+    p = gpu_ctx.particles(lon, lat, z0, pid=pid)
+    st = dict(lon=lon, lat=lat, z=z0, status=np.ones(lon.size, np.int32),
+              nn=np.full(lon.size, -1, np.int32), pid=pid)
+    eta = [sn["eta"] for sn in snaps]
+    a0 = 0.3
+    al = [0.0, 0.2, 0.2, 0.4][:order] + [0.0] * (4 - order)
+    for k in range(3):
+        start = (0, 1, -1, 0.25, 0.0)
+        end = (1, 2, -1, 0.6, 0.0)
+        gpu_ctx.step(p, order, 0, DT, 1.0, slot_b=1, slot_t=2, alpha0=a0, alphas=al,
+                     exact_nearest=2, w_method=w_method, kick=kick, beach=beach, step_i=k,
+                     sigma=dict(start=start, end=end, Dmin=0.5, z0b_min=0.1))
+        etas = sigma_stage_etas(order, False, _blend(eta[0], eta[1], 0.25), _blend(eta[1], eta[2], 0.6),
+                                eta[0], eta[1], eta[2], a0, al)
+        flds = dict(u_a=snaps[0][uk], v_a=snaps[0][vk], w_a=snaps[0]["w"], u_b=snaps[1][uk],
+                    v_b=snaps[1][vk], w_b=snaps[1]["w"], u_t=snaps[2][uk], v_t=snaps[2][vk],
+                    w_t=snaps[2]["w"], alpha0=a0, uv_layout=layout, eta_a=eta[0], eta_b=eta[1])
+        args = step_args(order, DT, 1.0, is3d=True, w_method=w_method, exact=2, alphas=al, kick=kick,
+                         beach=beach, step_i=k, sigma=dict(etas=etas, Dmin=0.5, z0b_min=0.1))
+        st = om.step(st, flds, args)
+    got = p.to_host()
+    assert (st["status"] == 2).any() and (st["status"] == 1).mean() > 0.5
+    for key in ("status", "fr", "nn", "ws"):
+        np.testing.assert_array_equal(got[key], st[key], err_msg=key)
+    for key in ("lon", "lat", "z", "u", "v", "w"):
+        np.testing.assert_array_equal(got[key].view(np.int64), st[key].view(np.int64), err_msg=key)

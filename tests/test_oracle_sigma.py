@@ -205,3 +205,63 @@ def test_sigma_uniform_flux_closed_form(integrator):
     assert keep.sum() >= 4
     s_exact = s0[keep][:, None] - c * t[None, :]
     np.testing.assert_allclose(rec["z"][keep] / H[None, :], s_exact, rtol=1e-9)
+
+
+def _kick_case(kh, az):
+    """Flat 20 m bed, uniform tide, no flow: only the kicks move particles."""
+    case = _column_case(lambda H, K: np.zeros((H.shape[0], K, H.shape[1])), "rk2", 1.0, flat=True)
+    d = case["numerics"].diffusion
+    d.enabled, d.kh_zonal, d.kh_meridional, d.seed = kh > 0, kh, kh, 77
+    d.az_enabled, d.az_const = az > 0, az
+    return case
+
+
+def _oracle_kicked(case, n_particles=4000, seed=3):
+    """One RK2 step of many particles near the grid centre through the oracle
+    with the case's kicks; returns (start state, end state, H at start/end)."""
+    mesh = case["mesh"]
+    rng = np.random.default_rng(seed)
+    lon = rng.uniform(4.8, 5.2, n_particles)
+    lat = rng.uniform(-0.2, 0.2, n_particles)
+    z = rng.uniform(0.1, 0.9, n_particles) * 20.0
+    reader = get_reader("fesomc_arrays")
+    f = [reader.read_velocities(case["ds"], i, mesh, case["mode"]) for i in (0, 1)]
+    om = OracleMesh(mesh, "glibc")
+    d = case["numerics"].diffusion
+    kick = dict(kh_x=d.kh_zonal, kh_y=d.kh_meridional, seed=d.seed)
+    if d.az_enabled:
+        kick["az_const"] = d.az_const
+    etas = sigma_stage_etas(2, True, f[0].eta, f[1].eta)
+    args = step_args(2, 900.0, 1.0, is3d=True, w_method=1, exact=2, kick=kick,
+                     sigma=dict(etas=etas, Dmin=0.5, z0b_min=0.1))
+    st = dict(lon=lon, lat=lat, z=z, status=np.ones(lon.size, np.int32),
+              nn=np.zeros(lon.size, np.int32), pid=np.arange(lon.size, dtype=np.int64))
+    out = om.step(st, dict(u_a=f[0].u, v_a=f[0].v, w_a=f[0].w, uv_layout=f[0].uv_layout), args)
+    return st, out, 20.0 + f[0].eta[4], 20.0 + f[1].eta[4]
+
+
+def test_sigma_kick_keeps_fraction_and_spreads():
+    """Horizontal Kh kick on a flat sigma column: the particle keeps its
+    sigma fraction in the end-of-step surface (integrators.py:784-801) and
+    the kicks spread by 2 Kh dt per axis."""
+    kh = 5.0
+    st, out, H0, H1 = _oracle_kicked(_kick_case(kh, 0.0))
+    ok = out["status"] == 1
+    assert ok.mean() > 0.99
+    np.testing.assert_allclose(out["z"][ok] / H1, st["z"][ok] / H0, rtol=1e-12)
+    dy = (out["lat"][ok] - st["lat"][ok]) * 111120.0
+    assert abs(dy.var() / (2.0 * kh * 900.0) - 1.0) < 0.08
+
+
+def test_sigma_az_kick_variance_and_bounds():
+    """Vertical Az kick on a sigma column (numerics/diffusion.py:459): the
+    depth stays in [0, H - z0b_min] of the end column and, away from the
+    boundaries, spreads by 2 Az dt."""
+    az = 2e-3
+    st, out, H0, H1 = _oracle_kicked(_kick_case(0.0, az))
+    ok = out["status"] == 1
+    assert ok.all()
+    assert (out["z"] >= 0.0).all() and (out["z"] <= H1 - 0.1 + 1e-12).all()
+    moved = out["z"] - st["z"] / H0 * H1
+    interior = (st["z"] / H0 > 0.3) & (st["z"] / H0 < 0.7)
+    assert abs(moved[interior].var() / (2.0 * az * 900.0) - 1.0) < 0.08
