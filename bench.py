@@ -37,6 +37,21 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def profiled_traffic(path):
+    """HBM bytes per step-kernel launch measured by rocprofv3 (FETCH_SIZE and
+    WRITE_SIZE, each scaled by a copy of known size) and recorded in the
+    workload's profile summary (line ``traffic_bytes_per_launch: N``); None
+    when there is no such record."""
+    try:
+        with open(path) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        return None
+    return None
+
+
 class Global3D:
     """RK4 3-D with time blend, random walk and beaching.
 
@@ -49,6 +64,8 @@ class Global3D:
     """
     name = "global_3d_rk4_kh_beach"
     b_alg = 1049.0
+    # rocprofv3 FETCH_SIZE/WRITE_SIZE summary of this workload's step kernel
+    profile = os.path.join(ROOT, "profiles", "step_kernel_rk4_3d_global_1M.md")
 
     def __init__(self, args):
         from kelp_amd import synth
@@ -250,6 +267,10 @@ def main():
     avg_launch_s = kern_ms / 1e3 / args.steps
     per_launch_units = alive_steps / args.steps
     achieved = wl.b_alg * per_launch_units / avg_launch_s / 1e9
+    # measured HBM bytes per launch, valid for the default workload size only
+    traffic = None
+    if getattr(wl, "profile", None) and not (args.particles or args.nodes):
+        traffic = profiled_traffic(wl.profile)
     line = {
         "metric": "particle-steps/s", "value": pss, "unit": "particle-steps/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -257,7 +278,7 @@ def main():
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": dict(wl.config, alive_end=alive_end, resort_every=args.resort_every),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "step_kernel", "kernel_avg_us": avg_launch_s * 1e6,
                      "b_alg_per_particle_step": wl.b_alg,
                      "particle_steps_per_launch": per_launch_units},
