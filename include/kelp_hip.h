@@ -185,7 +185,8 @@ int kelp_sample(kelp_ctx *ctx, const kelp_particles *p, int var, int slot_a, int
 
 /* Release (core/release.py:112 activate_released): status NOT_RELEASED (4)
  * becomes ALIVE where step >= release_step[pid - pid_offset] (device int64,
- * n_release entries; NULL = no release schedule).  prev_status (device,
+ * n_release entries; NULL = no release schedule); a released particle's nn
+ * becomes -1 (its first step anchors it).  prev_status (device,
  * n int32, or NULL) receives every particle's status after activation,
  * the step-start status the record masks of core/release.py:132 use. */
 int kelp_activate(kelp_ctx *ctx, const kelp_particles *p, const int64_t *release_step, int64_t pid_offset,

@@ -628,9 +628,10 @@ int kelp_nodes_from_mesh_ids(kelp_ctx* c, int64_t n, int32_t* ids)
 // global step reaches the particle's release step (indexed by pid -
 // pid_offset, so the particle order on the device does not matter); the
 // status at the step start is copied to prev_status for the record masks.
-__global__ void activate_kernel(int64_t n, int32_t* __restrict__ status, const int64_t* __restrict__ pid,
-                                const int64_t* __restrict__ release_step, int64_t pid_offset, int64_t n_release,
-                                int64_t step, int32_t* __restrict__ prev_status)
+__global__ void activate_kernel(int64_t n, int32_t* __restrict__ status, int32_t* __restrict__ nn,
+                                const int64_t* __restrict__ pid, const int64_t* __restrict__ release_step,
+                                int64_t pid_offset, int64_t n_release, int64_t step,
+                                int32_t* __restrict__ prev_status)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -640,6 +641,7 @@ __global__ void activate_kernel(int64_t n, int32_t* __restrict__ status, const i
         if (k >= 0 && k < n_release && step >= release_step[k]) {
             s = kAlive;
             status[i] = s;
+            if (nn) nn[i] = -1;  // anchored afresh by its first step
         }
     }
     if (prev_status) prev_status[i] = s;
@@ -654,8 +656,8 @@ int kelp_activate(kelp_ctx* c, const kelp_particles* p, const int64_t* release_s
         return fail(KELP_EINVAL, "kelp_activate: release steps need pid and n_release >= 0");
     if (p->n == 0 || (!release_step && !prev_status)) return KELP_OK;
     HIP_TRY(hipSetDevice(c->device));
-    activate_kernel<<<grid_for(p->n, 256), 256, 0, c->stream>>>(p->n, p->status, p->pid, release_step, pid_offset,
-                                                               n_release, step, prev_status);
+    activate_kernel<<<grid_for(p->n, 256), 256, 0, c->stream>>>(p->n, p->status, p->nn, p->pid, release_step,
+                                                               pid_offset, n_release, step, prev_status);
     HIP_TRY(hipGetLastError());
     return KELP_OK;
 }

@@ -106,6 +106,33 @@ def resolve_time_bracket(sim_time, file_times, n, on_exhausted, direction=1, sna
     return ia, ib, ta, tb, tw
 
 
+def index_to_sim_time(idx, file_times, n, on_exhausted, dt=None) -> float:
+    """tracking.py:2868 _index_to_sim_time (legacy cycle rule): the time of
+    the unwrapped record index ``idx``."""
+    if n <= 0:
+        return 0.0
+    if on_exhausted == "cycle":
+        period = (float(dt) if dt is not None else 1.0) if n == 1 else _cycle_period(file_times, n)
+        n_full, rem = divmod(int(idx), n)
+        return n_full * period + float(file_times[rem])
+    return float(file_times[max(0, min(int(idx), n - 1))])
+
+
+def record_clock_origin(use_tinterp, sim_time, start_time_idx, file_times, n, on_exhausted, dt,
+                        direction, start_step_i) -> float:
+    """tracking.py:1197 _record_clock_origin: the time of the start of
+    global step 0, so the record after global step g is labelled
+    ``T_g0 + (g + 1) dt direction`` in a resumed run as in the whole run."""
+    if use_tinterp:
+        return float(sim_time) - start_step_i * dt * direction
+    idx0 = int(start_time_idx) - int(start_step_i) * int(direction)
+    if file_times is not None and len(file_times) >= n and n > 0:
+        if n == 1:
+            return float(file_times[0]) + float(idx0) * float(dt)
+        return index_to_sim_time(idx0, file_times, n, on_exhausted)
+    return float(idx0) * float(dt)
+
+
 def step_end_on_record(sim_time, pos, dt_to_next, direction, file_times, n, on_exhausted):
     """tracking.py:2930 _step_end_on_record (legacy cycle rule)."""
     delta = pos - wrap_sim_time(sim_time, file_times, n, on_exhausted)
@@ -328,10 +355,19 @@ def run_single_realization(mesh: Mesh, ds, reader, init_state: ParticleState, mo
                            realization_id: int = 0, particle_offset: int = 0,
                            start_step_i: int = 0, on_exhausted: str = "stop",
                            start_wet_streak=None, ctx=None, resort_every: int = 0,
-                           writer: Optional[TrajectoryRecord] = None) -> TrajectoryRecord:
+                           writer: Optional[TrajectoryRecord] = None,
+                           sim_time_start: Optional[float] = None,
+                           checkpoint_freq: Optional[int] = None,
+                           checkpoint_path: Optional[str] = None) -> TrajectoryRecord:
     """One realization on the device (tracking.py:1805).  Returns the
     records, ``(particle, record)``: lon, lat, z, status, failure_reason,
-    vel_u/vel_v/vel_w (NaN where undefined), time."""
+    vel_u/vel_v/vel_w (NaN where undefined), time.
+
+    Checkpoints (tracking.py:2785-2862): with ``checkpoint_path`` and
+    ``checkpoint_freq`` (default ``numerics.checkpoint_freq``) > 0 the
+    device state is written every ``checkpoint_freq`` global steps and after
+    the last step that ran (:func:`write_checkpoint`); :func:`load_restart`
+    returns the arguments that resume the run bit for bit."""
     from .device import DeviceContext
     sample_vars = list(sample_vars or [])
     if sample_vars and numerics.sample_interp not in SAMPLE_METHODS:
@@ -378,10 +414,17 @@ def run_single_realization(mesh: Mesh, ds, reader, init_state: ParticleState, mo
           else np.asarray(start_wet_streak, np.int32))
     fr0 = (np.zeros(P, np.int8) if init_state.failure_reason is None
            else init_state.failure_reason)
-    # nn starts unknown (-1): every anchor the run stores is then its own,
-    # which lets each step reuse the previous step's end anchor
+    # moving particles start with nn unknown (-1): every anchor the run
+    # stores is then its own, which lets each step reuse the previous step's
+    # end anchor; beached and outside particles keep theirs (the beaching
+    # hook reads it, e.g. in a run resumed from a checkpoint)
+    nn0 = None
+    if init_state.nearest_node is not None:
+        moving = ((init_state.status == ParticleStatus.ALIVE)
+                  | (init_state.status == ParticleStatus.NOT_RELEASED))
+        nn0 = np.where(moving, -1, init_state.nearest_node).astype(np.int32)
     p = ctx.particles(init_state.lon, init_state.lat, init_state.z, status=init_state.status,
-                      nn=None, fr=fr0, pid=pid, ws=ws)
+                      nn=nn0, fr=fr0, pid=pid, ws=ws)
     torch = ctx.torch
     release_step = init_state.release_step
     rel_dev = None if release_step is None else torch.as_tensor(
@@ -392,8 +435,19 @@ def run_single_realization(mesh: Mesh, ds, reader, init_state: ParticleState, mo
     if use_tinterp:
         if not 0 <= start_time_idx < n_time_total and on_exhausted != "cycle":
             raise ValueError(f"start_time_idx {start_time_idx} is outside the records [0, {n_time_total})")
-        sim_time = file_times[start_time_idx % n_time_total]
-    t_g0 = file_times[start_time_idx % n_time_total] if file_times is not None else 0.0
+        sim_time = (file_times[start_time_idx % n_time_total] if sim_time_start is None
+                    else float(sim_time_start))
+    t_g0 = record_clock_origin(use_tinterp, sim_time, start_time_idx, file_times, n_time_total,
+                               on_exhausted, dt, direction, start_step_i)
+
+    ck_freq = int(numerics.checkpoint_freq if checkpoint_freq is None else checkpoint_freq)
+    ck_on = bool(checkpoint_path) and ck_freq > 0
+    last_g = None
+
+    def checkpoint(g_done, next_idx):
+        write_checkpoint(checkpoint_path, p.to_host(), g_done, next_idx,
+                         sim_time if use_tinterp else None,
+                         release_step=release_step, particle_offset=particle_offset)
 
     for step_i in range(n_steps):
         g = start_step_i + step_i
@@ -493,7 +547,69 @@ def run_single_realization(mesh: Mesh, ds, reader, init_state: ParticleState, mo
             for k, v in sampled.items():
                 row[k] = v[order_idx]
             rec.append(row, t_g0 + (g + 1) * dt * direction)
+        last_g = g
+        next_idx = (start_time_idx + (step_i + 1) * direction) if not use_tinterp else None
+        if use_tinterp:
+            nb = resolve_time_bracket(sim_time, file_times, n_time_total, on_exhausted, direction, snap)
+            next_idx = nb[0] if nb is not None else None
+        if ck_on and (g + 1) % ck_freq == 0:
+            checkpoint(g, next_idx)
+    if ck_on and last_g is not None and (last_g + 1) % ck_freq != 0:
+        checkpoint(last_g, next_idx)
     return rec
+
+
+def write_checkpoint(path: str, h: Dict[str, np.ndarray], step_i: int, next_time_idx,
+                     sim_time=None, release_step=None, particle_offset: int = 0) -> None:
+    """writers.py:567 write_checkpoint, as a ``.npz``: the device state in
+    particle-id order (``h`` as ``DeviceParticles.to_host`` returns it), the
+    last fully executed global step, the record index the next step reads
+    (``restart_file_idx``, the ``start_time_idx`` of a resume) and, for a
+    time-interpolated run, the clock.  Written to a temporary file, synced
+    and renamed, so an interrupted checkpoint never replaces a valid one
+    (the role of ``restart_valid``, writers.py:581)."""
+    import os
+    order = np.argsort(h["pid"], kind="stable")
+    arrays = {"restart_" + k: np.asarray(h[src])[order] for k, src in (
+        ("lon", "lon"), ("lat", "lat"), ("z", "z"), ("status", "status"),
+        ("nearest_node", "nn"), ("failure_reason", "fr"), ("wet_streak", "ws"), ("pid", "pid"))}
+    arrays.update(restart_valid=np.int32(1), restart_step_i=np.int64(step_i),
+                  restart_file_idx=np.int64(-1 if next_time_idx is None else next_time_idx),
+                  restart_sim_time=np.float64(np.nan if sim_time is None else sim_time),
+                  restart_particle_offset=np.int64(particle_offset))
+    if release_step is not None:
+        arrays["restart_release_step"] = np.asarray(release_step, np.int64)
+    tmp = path + ".tmp"
+    with open(tmp, "wb") as fh:
+        np.savez(fh, **arrays)
+        fh.flush()
+        os.fsync(fh.fileno())
+    os.replace(tmp, path)
+
+
+def load_restart(path: str) -> Dict[str, object]:
+    """writers.py:1541 load_restart for :func:`write_checkpoint` files:
+    ``state`` (ParticleState, particle-id order), ``start_step_i`` (the
+    next global step), ``start_time_idx``, ``sim_time_start`` (None for a
+    discrete run), ``start_wet_streak``, ``particle_offset``: the keyword
+    arguments of :func:`run_single_realization` that continue the run."""
+    with np.load(path, allow_pickle=False) as z:
+        if "restart_valid" not in z.files or int(z["restart_valid"]) != 1:
+            raise ValueError(f"No valid restart checkpoint in {path}")
+        d = {k: z[k] for k in z.files}
+    st = ParticleState(lon=d["restart_lon"], lat=d["restart_lat"], z=d["restart_z"],
+                       status=d["restart_status"].astype(np.int32),
+                       nearest_node=d["restart_nearest_node"].astype(np.int32),
+                       failure_reason=d["restart_failure_reason"].astype(np.int8))
+    if "restart_release_step" in d:
+        st = st.replace(release_step=d["restart_release_step"])
+    sim = float(d["restart_sim_time"])
+    idx = int(d["restart_file_idx"])
+    return dict(state=st, start_step_i=int(d["restart_step_i"]) + 1,
+                start_time_idx=(idx if idx >= 0 else 0),
+                sim_time_start=None if np.isnan(sim) else sim,
+                start_wet_streak=d["restart_wet_streak"].astype(np.int32),
+                particle_offset=int(d["restart_particle_offset"]))
 
 
 def run_tracking(config: AppConfig, *, mesh: Mesh, dataset, reader="fesom2_arrays",

@@ -141,3 +141,42 @@ def test_sigma_run_matches_reference_golden(gpu_ctx, direction):
     for k, src in (("lon", "lon"), ("lat", "lat"), ("z", "z"), ("u", "vel_u"), ("v", "vel_v"),
                    ("w", "vel_w")):
         np.testing.assert_allclose(rec[src], gold[k], rtol=1e-9, atol=1e-9, err_msg=k)
+
+
+@pytest.mark.parametrize("split,tinterp", [(6, True), (9, True), (5, False)])
+def test_checkpoint_resume_is_bitwise(gpu_ctx, tmp_path, split, tinterp):
+    """A run checkpointed after ``split`` steps and resumed from the file
+    (tracking.py:2785 checkpoints, writers.py:1541 load_restart) records
+    exactly what the uninterrupted run records: positions, status, failure
+    codes, velocities -- through stochastic beaching and resuspension, whose
+    draws and wet streaks continue on the global step."""
+    from kelp_amd.tracking import load_restart
+    case = rc.beaching_case(1.0)
+    if not tinterp:
+        # discrete: one record per step
+        case["numerics"].temporal_interp = False
+        case["numerics"].dt = float(case["file_dt"])
+        case["n_steps"] = case["n_time_total"] - 1
+    args = dict(n_time_total=case["n_time_total"], file_times=case["file_times"],
+                file_dt=case["file_dt"], ctx=gpu_ctx)
+    reader = get_reader("fesom2_arrays")
+    full = run_single_realization(case["mesh"], case["ds"], reader, case["init"], case["mode"],
+                                  case["numerics"], start_time_idx=case["start_time_idx"],
+                                  n_steps=case["n_steps"], **args).as_arrays()
+    ck = str(tmp_path / "restart.npz")
+    first = run_single_realization(case["mesh"], case["ds"], reader, case["init"], case["mode"],
+                                   case["numerics"], start_time_idx=case["start_time_idx"],
+                                   n_steps=split, checkpoint_freq=split, checkpoint_path=ck,
+                                   **args).as_arrays()
+    r = load_restart(ck)
+    rest = run_single_realization(case["mesh"], case["ds"], reader, r["state"], case["mode"],
+                                  case["numerics"], start_time_idx=r["start_time_idx"],
+                                  sim_time_start=r["sim_time_start"],
+                                  start_step_i=r["start_step_i"],
+                                  start_wet_streak=r["start_wet_streak"],
+                                  particle_offset=r["particle_offset"],
+                                  n_steps=case["n_steps"] - split, **args).as_arrays()
+    assert r["start_step_i"] == split
+    for k in ("lon", "lat", "z", "status", "failure_reason", "vel_u", "vel_v", "vel_w", "time"):
+        both = np.concatenate([first[k], rest[k]], axis=-1)
+        np.testing.assert_array_equal(both, full[k], err_msg=k)

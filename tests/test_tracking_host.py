@@ -95,3 +95,42 @@ time: {n_steps: 336, on_exhausted: cycle}
     assert C.resolve_time_params(cfg.time) == (0, 336)
     with pytest.raises(ValueError, match="missing section"):
         C.config_from_dict({"mode": {}})
+
+
+def test_checkpoint_file_protocol(tmp_path):
+    """write_checkpoint / load_restart: state in particle-id order, the
+    resume arguments, and no valid checkpoint without the flag."""
+    P = 5
+    pid = np.array([3, 0, 4, 1, 2], np.int64)
+    h = dict(lon=pid * 1.5, lat=pid * -2.0, z=pid + 0.25, status=np.int32(pid % 3 + 1),
+             nn=np.int32(pid * 7), fr=np.int8(pid % 2), ws=np.int32(pid * 2), pid=pid,
+             u=np.zeros(P), v=np.zeros(P), w=np.zeros(P))
+    path = str(tmp_path / "ck.npz")
+    T.write_checkpoint(path, h, 11, 4, 5400.0, release_step=np.arange(P), particle_offset=100)
+    r = T.load_restart(path)
+    np.testing.assert_array_equal(r["state"].lon, np.arange(P) * 1.5)
+    np.testing.assert_array_equal(r["state"].nearest_node, np.arange(P) * 7)
+    np.testing.assert_array_equal(r["start_wet_streak"], np.arange(P) * 2)
+    np.testing.assert_array_equal(r["state"].release_step, np.arange(P))
+    assert (r["start_step_i"], r["start_time_idx"], r["sim_time_start"], r["particle_offset"]) == \
+        (12, 4, 5400.0, 100)
+    assert not os.path.exists(path + ".tmp")
+    T.write_checkpoint(path, h, 3, 2, None)
+    assert T.load_restart(path)["sim_time_start"] is None
+    bad = str(tmp_path / "bad.npz")
+    np.savez(bad, restart_lon=np.zeros(2))
+    with pytest.raises(ValueError, match="No valid restart"):
+        T.load_restart(bad)
+
+
+def test_record_clock_origin_of_a_resumed_run():
+    """tracking.py:1197: labels anchor on global step 0, whatever the cut."""
+    ft = np.arange(6) * 3600.0
+    # discrete: a segment resumed at record 5 after 3 steps started at record 2
+    assert T.record_clock_origin(False, None, 5, ft, 6, "stop", 3600.0, 1, 3) == 7200.0
+    # backward
+    assert T.record_clock_origin(False, None, 1, ft, 6, "stop", 3600.0, -1, 3) == 14400.0
+    # interpolated: the clock minus the steps done
+    assert T.record_clock_origin(True, 9000.0, 0, ft, 6, "stop", 1800.0, 1, 4) == 1800.0
+    # cycle: unwrapped index past the axis
+    assert T.index_to_sim_time(8, ft, 6, "cycle") == 6 * 3600.0 + 2 * 3600.0
