@@ -79,6 +79,15 @@ class Global3D:
         self.beach = dict(criteria=["on_failure", "stochastic"], t_beach=86400.0,
                           depth_min=0.3, seed=20240501)
         self.sample_ts = not args.no_sample
+        # snapshot ring: `ring` slots, the next snapshot refilled
+        # asynchronously while steps run (or synchronously at the bracket
+        # change with --ring-sync); all snapshots resident when 0
+        self.ring = int(args.ring)
+        self.ring_sync = bool(args.ring_sync)
+        if self.ring:
+            if self.ring < 3:
+                raise SystemExit("--ring needs at least 3 slots (bracket pair + the one refilling)")
+            self.sample_ts = False
         self.order = 4
         self.config = {"workload": self.name, "mesh_nodes": self.w.mesh.n_nodes,
                        "mesh_elements": self.w.mesh.n_elements, "z_levels": int(self.w.mesh.sigma.size),
@@ -87,9 +96,20 @@ class Global3D:
                        "field_dtype": "f32", "kh_m2_s": self.w.kh, "w_interp": "barycentric",
                        "beaching": self.beach["criteria"], "steps_per_run": self.w.n_steps,
                        "sampled": ["T", "S"] if self.sample_ts else []}
+        if self.ring:
+            self.config.update(ring_slots=self.ring,
+                               ring_refill="sync" if self.ring_sync else "async prefetch")
 
     def upload(self, ctx):
         ctx.upload_mesh(self.w.mesh)
+        if self.ring:
+            # the host side of the ring: every snapshot page-locked, as a
+            # reader decoding into pinned buffers would leave it
+            self.pin = [(ctx.pinned(U), ctx.pinned(V), ctx.pinned(W)) for U, V, W, _, _ in self.w.snaps]
+            self.resident = {}
+            for e in range(self.ring):
+                self._ensure(ctx, e, sync=True)
+            return ctx.particles(self.w.lon, self.w.lat, self.w.z)
         for s, (U, V, W, T, S) in enumerate(self.w.snaps):
             ctx.upload_field(s, U, V, W)
             if self.sample_ts:
@@ -101,11 +121,31 @@ class Global3D:
             self.ts_out = t.empty((2, p.n), dtype=t.float64, device=p.lon.device)
         return p
 
+    def _ensure(self, ctx, e, sync):
+        """Ring slot e % ring holds the snapshot of epoch e (unwrapped
+        snapshot count), refilled now if it does not."""
+        s = e % self.ring
+        if self.resident.get(s) == e:
+            return
+        U, V, W = self.pin[e % len(self.w.snaps)]
+        if sync:
+            ctx.upload_field(s, U, V, W)
+        else:
+            ctx.upload_field_async(s, U, V, W)
+        self.resident[s] = e
+
     def step(self, ctx, p, k, ev=None):
         w = self.w
         t = k * w.dt
         ia, ib, a0, it, dtn = self.synth.snapshot_bracket(t, w.snap_dt, len(w.snaps))
         al = [c * w.dt / dtn for c in (0.0, 0.5, 0.5, 1.0)]
+        if self.ring:
+            ep = int(np.floor(t / w.snap_dt))
+            self._ensure(ctx, ep, sync=True)
+            self._ensure(ctx, ep + 1, sync=True)
+            ia = ep % self.ring
+            it = (ep + 1) % self.ring
+            ib = it if ib >= 0 else -1
         if ev is not None:
             ctx.event_record(ev)
         ctx.step(p, 4, ia, w.dt, 1.0, slot_b=ib, slot_t=it, alpha0=a0, alphas=al,
@@ -113,6 +153,9 @@ class Global3D:
                  reuse_anchor=True)
         if ev is not None:
             ctx.event_record(ev + 1)
+        if self.ring and not self.ring_sync:
+            # the snapshot after the bracket streams in behind this step
+            self._ensure(ctx, int(np.floor(t / w.snap_dt)) + 2, sync=False)
         if self.sample_ts:
             # T and S at the step's end (the record instant)
             ra, rb, ra0, _, _ = self.synth.snapshot_bracket(t + w.dt, w.snap_dt, len(w.snaps))
@@ -211,6 +254,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--resort-every", type=int, default=4,
                     help="re-sort particles by grid-hash cell every N steps (0: never)")
+    ap.add_argument("--ring", type=int, default=0,
+                    help="global_3d: keep only this many snapshot slots (>= 3) and stream the "
+                         "snapshots through them (no T/S sampling)")
+    ap.add_argument("--ring-sync", action="store_true",
+                    help="with --ring: refill at the bracket change, synchronously")
     ap.add_argument("--no-sample", action="store_true",
                     help="global_3d: leave out the T/S sampling at each step's end")
     ap.add_argument("--copy-probe-mib", type=int, default=0,

@@ -192,6 +192,24 @@ int kelp_sample(kelp_ctx *ctx, const kelp_particles *p, int var, int slot_a, int
 int kelp_activate(kelp_ctx *ctx, const kelp_particles *p, const int64_t *release_step, int64_t pid_offset,
                   int64_t n_release, int64_t step, int32_t *prev_status);
 
+/* Refill a slot asynchronously (the snapshot ring: the next snapshot
+ * streams in while steps run).  The slot must already hold a snapshot of
+ * the same dtype, layout and shape (one kelp_field_upload fills it first);
+ * host_eta (N doubles, or NULL) also refills its surface.  The copies and
+ * the packing run on the context's copy stream after every call already
+ * enqueued on the compute stream (a step still reading the old snapshot
+ * finishes first); steps and sampling that read the slot wait for the
+ * refill on the device.  The host arrays must stay valid until the refill
+ * has run (kelp_field_upload_wait); page-locked host memory makes the
+ * copies truly asynchronous.  No device allocation after the first refill
+ * of a slot.  Replaces the reference's _VelCache prefetch
+ * (tracking.py:584-735), which reads ahead on a host thread. */
+int kelp_field_upload_async(kelp_ctx *ctx, int slot, int dtype, int layout, int64_t n_planes,
+                            const void *host_u, const void *host_v, const void *host_w, int64_t n_w_planes,
+                            const double *host_eta);
+/* Block the host until the last refill of `slot` has run. */
+int kelp_field_upload_wait(kelp_ctx *ctx, int slot);
+
 /* Several variables at once (vars[0..n_vars), host array): row k of `out`
  * (device, n_vars x n doubles) is variable vars[k] as kelp_sample computes
  * it; least-squares pairs of 2-D/mid-layer variables share one pass. */

@@ -85,7 +85,18 @@ static void free_slot(FieldSlot& s)
     if (s.erec) (void)hipFree(s.erec);
     if (s.eta) (void)hipFree(s.eta);
     for (auto& f : s.scalars) kelp_free_scalar(f);
+    for (void* p : s.stage)
+        if (p) (void)hipFree(p);
+    if (s.ready) (void)hipEventDestroy(s.ready);
     s = FieldSlot{};
+}
+
+int kelp_wait_slot(kelp_ctx* c, FieldSlot& s)
+{
+    if (!s.pending) return KELP_OK;
+    HIP_TRY(hipStreamWaitEvent(c->stream, s.ready, 0));
+    s.pending = false;
+    return KELP_OK;
 }
 
 template <typename T>
@@ -358,6 +369,12 @@ int kelp_ctx_create(int device, kelp_ctx** out)
         delete c;
         return fail(KELP_EHIP, "hipStreamCreate: %s", hipGetErrorString(e));
     }
+    e = hipStreamCreate(&c->copy_stream);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->compute_mark, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        kelp_ctx_destroy(c);
+        return fail(KELP_EHIP, "copy stream: %s", hipGetErrorString(e));
+    }
     for (int k = 0; k < kMaxEvents; ++k) {
         e = hipEventCreate(&c->events[k]);
         if (e != hipSuccess) {
@@ -374,12 +391,15 @@ int kelp_ctx_destroy(kelp_ctx* c)
     if (!c) return KELP_OK;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);
     free_mesh(c);
     for (auto& s : c->slots) free_slot(s);
     for (DevBuf* b : {&c->sort_tmp, &c->sort_keys, &c->sort_keys2, &c->sort_idx, &c->sort_perm, &c->sort_stage})
         kelp_release(*b);
     for (int k = 0; k < kMaxEvents; ++k)
         if (c->events[k]) (void)hipEventDestroy(c->events[k]);
+    if (c->compute_mark) (void)hipEventDestroy(c->compute_mark);
+    if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return KELP_OK;
@@ -662,6 +682,45 @@ int kelp_activate(kelp_ctx* c, const kelp_particles* p, const int64_t* release_s
     return KELP_OK;
 }
 
+// Pack one snapshot's planes (device, mesh order) into the slot's records on
+// `stream` (pack3d/pack2d for node records, pack_elem for element u/v).
+static hipError_t launch_pack(kelp_ctx* c, hipStream_t stream, FieldSlot& s, int dtype, int layout, bool is3d,
+                              int64_t n_planes, const void* tu, const void* tv, const void* tw)
+{
+    const int64_t N = c->m.n_nodes, E = c->m.n_elems;
+    const bool f32 = dtype == KELP_DT_F32;
+    // node records: u/v (node layout) and the W pair (3-D)
+    const void* nu = layout == 0 ? tu : nullptr;
+    const void* nv = layout == 0 ? tv : nullptr;
+    if (is3d) {
+        const unsigned g = grid_for(N * n_planes, 256);
+        if (f32)
+            pack3d_kernel<float><<<g, 256, 0, stream>>>(N, n_planes, c->node_old_of_new, (const float*)nu,
+                                                        (const float*)nv, (const float*)tw, (float4*)s.rec);
+        else
+            pack3d_kernel<double><<<g, 256, 0, stream>>>(N, n_planes, c->node_old_of_new, (const double*)nu,
+                                                         (const double*)nv, (const double*)tw, (double4*)s.rec);
+    } else if (layout == 0) {
+        const unsigned g = grid_for(N, 256);
+        if (f32)
+            pack2d_kernel<float><<<g, 256, 0, stream>>>(N, c->node_old_of_new, (const float*)tu, (const float*)tv,
+                                                        (float2*)s.rec);
+        else
+            pack2d_kernel<double><<<g, 256, 0, stream>>>(N, c->node_old_of_new, (const double*)tu,
+                                                         (const double*)tv, (double2*)s.rec);
+    }
+    if (layout == 1) {
+        const unsigned g = grid_for(E * n_planes, 256);
+        if (f32)
+            pack_elem_kernel<float><<<g, 256, 0, stream>>>(E, n_planes, c->elem_old_of_new, (const float*)tu,
+                                                           (const float*)tv, (float2*)s.erec);
+        else
+            pack_elem_kernel<double><<<g, 256, 0, stream>>>(E, n_planes, c->elem_old_of_new, (const double*)tu,
+                                                            (const double*)tv, (double2*)s.erec);
+    }
+    return hipGetLastError();
+}
+
 // Upload one snapshot: u/v planes at the nodes (layout 0) or at the element
 // centres (layout 1), W planes at the nodes; packed on the device.
 static int field_upload(kelp_ctx* c, const char* fn, int slot, int dtype, int layout, int64_t n_planes,
@@ -679,6 +738,8 @@ static int field_upload(kelp_ctx* c, const char* fn, int slot, int dtype, int la
         return fail(KELP_ESTATE, "%s: element-centre u/v need the mesh's element centres", fn);
     HIP_TRY(hipSetDevice(c->device));
     FieldSlot& s = c->slots[slot];
+    if (s.pending) HIP_TRY(hipStreamSynchronize(c->copy_stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
     free_slot(s);
     const int64_t N = c->m.n_nodes, E = c->m.n_elems;
     const int64_t M = layout == 1 ? E : N;  // u/v values per plane
@@ -695,42 +756,7 @@ static int field_upload(kelp_ctx* c, const char* fn, int slot, int dtype, int la
     if (e == hipSuccess) e = hipMemcpyAsync(tu, hu, uv_bytes, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(tv, hv, uv_bytes, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess && is3d) e = hipMemcpyAsync(tw, hw, w_bytes, hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess) {
-        const bool f32 = dtype == KELP_DT_F32;
-        // node records: u/v (node layout) and the W pair (3-D)
-        const void* nu = layout == 0 ? tu : nullptr;
-        const void* nv = layout == 0 ? tv : nullptr;
-        if (is3d) {
-            const unsigned g = grid_for(N * n_planes, 256);
-            if (f32)
-                pack3d_kernel<float><<<g, 256, 0, c->stream>>>(N, n_planes, c->node_old_of_new, (const float*)nu,
-                                                               (const float*)nv, (const float*)tw, (float4*)s.rec);
-            else
-                pack3d_kernel<double><<<g, 256, 0, c->stream>>>(N, n_planes, c->node_old_of_new,
-                                                                (const double*)nu, (const double*)nv,
-                                                                (const double*)tw, (double4*)s.rec);
-        } else if (layout == 0) {
-            const unsigned g = grid_for(N, 256);
-            if (f32)
-                pack2d_kernel<float><<<g, 256, 0, c->stream>>>(N, c->node_old_of_new, (const float*)tu,
-                                                               (const float*)tv, (float2*)s.rec);
-            else
-                pack2d_kernel<double><<<g, 256, 0, c->stream>>>(N, c->node_old_of_new, (const double*)tu,
-                                                                (const double*)tv, (double2*)s.rec);
-        }
-        if (layout == 1) {
-            const unsigned g = grid_for(E * n_planes, 256);
-            if (f32)
-                pack_elem_kernel<float><<<g, 256, 0, c->stream>>>(E, n_planes, c->elem_old_of_new,
-                                                                  (const float*)tu, (const float*)tv,
-                                                                  (float2*)s.erec);
-            else
-                pack_elem_kernel<double><<<g, 256, 0, c->stream>>>(E, n_planes, c->elem_old_of_new,
-                                                                   (const double*)tu, (const double*)tv,
-                                                                   (double2*)s.erec);
-        }
-        e = hipGetLastError();
-    }
+    if (e == hipSuccess) e = launch_pack(c, c->stream, s, dtype, layout, is3d, n_planes, tu, tv, tw);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (tu) (void)hipFree(tu);
     if (tv) (void)hipFree(tv);
@@ -775,6 +801,7 @@ int kelp_field_upload_eta(kelp_ctx* c, int slot, const double* host_eta)
     FieldSlot& s = c->slots[slot];
     if (s.dtype < 0) return fail(KELP_ESTATE, "kelp_field_upload_eta: slot %d holds no snapshot", slot);
     HIP_TRY(hipSetDevice(c->device));
+    if (s.pending) HIP_TRY(hipStreamSynchronize(c->copy_stream));
     const int64_t N = c->m.n_nodes;
     if (!s.eta) HIP_TRY(hipMalloc(&s.eta, N * sizeof(double)));
     double* tmp = nullptr;
@@ -795,7 +822,79 @@ int kelp_field_clear(kelp_ctx* c, int slot)
     if (!c || slot < 0 || slot >= kMaxSlots) return fail(KELP_EINVAL, "kelp_field_clear: bad argument");
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    (void)hipStreamSynchronize(c->copy_stream);
     free_slot(c->slots[slot]);
+    return KELP_OK;
+}
+
+// Device staging buffer k of a slot, at least `bytes` (grown with hipMalloc
+// only when too small; the copy stream has finished with it by then).
+static int slot_stage(kelp_ctx* c, FieldSlot& s, int k, size_t bytes, void** out)
+{
+    if (s.stage_bytes[k] < bytes) {
+        HIP_TRY(hipStreamSynchronize(c->copy_stream));
+        if (s.stage[k]) (void)hipFree(s.stage[k]);
+        s.stage[k] = nullptr;
+        s.stage_bytes[k] = 0;
+        HIP_TRY(hipMalloc(&s.stage[k], bytes));
+        s.stage_bytes[k] = bytes;
+    }
+    *out = s.stage[k];
+    return KELP_OK;
+}
+
+int kelp_field_upload_async(kelp_ctx* c, int slot, int dtype, int layout, int64_t n_planes, const void* host_u,
+                            const void* host_v, const void* host_w, int64_t n_w_planes, const double* host_eta)
+{
+    if (!c) return fail(KELP_EINVAL, "kelp_field_upload_async: NULL context");
+    if (slot < 0 || slot >= kMaxSlots) return fail(KELP_EINVAL, "kelp_field_upload_async: slot %d out of range", slot);
+    if (!host_u || !host_v) return fail(KELP_EINVAL, "kelp_field_upload_async: u/v missing");
+    FieldSlot& s = c->slots[slot];
+    const bool is3d = host_w != nullptr;
+    if (s.dtype != dtype || s.layout != layout || s.n_planes != n_planes || s.is3d != is3d ||
+        (is3d && s.n_w_planes != n_w_planes))
+        return fail(KELP_ESTATE, "kelp_field_upload_async: slot %d must already hold a snapshot of this shape "
+                                 "(fill it once with kelp_field_upload)", slot);
+    if (host_eta && !s.eta)
+        return fail(KELP_ESTATE, "kelp_field_upload_async: slot %d has no eta to refill (kelp_field_upload_eta)",
+                    slot);
+    HIP_TRY(hipSetDevice(c->device));
+    const int64_t N = c->m.n_nodes, E = c->m.n_elems;
+    const int64_t M = layout == 1 ? E : N;
+    const size_t es = dtype == KELP_DT_F64 ? 8 : 4;
+    const size_t uv_bytes = (size_t)n_planes * M * es;
+    const size_t w_bytes = is3d ? (size_t)n_w_planes * N * es : 0;
+    void *tu, *tv, *tw = nullptr, *te = nullptr;
+    int rc;
+    if ((rc = slot_stage(c, s, 0, uv_bytes, &tu)) || (rc = slot_stage(c, s, 1, uv_bytes, &tv)) ||
+        (is3d && (rc = slot_stage(c, s, 2, w_bytes, &tw))) ||
+        (host_eta && (rc = slot_stage(c, s, 3, (size_t)N * sizeof(double), &te))))
+        return rc;
+    if (!s.ready) HIP_TRY(hipEventCreateWithFlags(&s.ready, hipEventDisableTiming));
+    // the refill overwrites the slot only after every step enqueued so far,
+    // which may read it, has run
+    HIP_TRY(hipEventRecord(c->compute_mark, c->stream));
+    HIP_TRY(hipStreamWaitEvent(c->copy_stream, c->compute_mark, 0));
+    HIP_TRY(hipMemcpyAsync(tu, host_u, uv_bytes, hipMemcpyHostToDevice, c->copy_stream));
+    HIP_TRY(hipMemcpyAsync(tv, host_v, uv_bytes, hipMemcpyHostToDevice, c->copy_stream));
+    if (is3d) HIP_TRY(hipMemcpyAsync(tw, host_w, w_bytes, hipMemcpyHostToDevice, c->copy_stream));
+    if (host_eta) HIP_TRY(hipMemcpyAsync(te, host_eta, N * sizeof(double), hipMemcpyHostToDevice, c->copy_stream));
+    HIP_TRY(launch_pack(c, c->copy_stream, s, dtype, layout, is3d, n_planes, tu, tv, tw));
+    if (host_eta) {
+        permute_nodes_kernel<<<grid_for(N, 256), 256, 0, c->copy_stream>>>(N, c->node_old_of_new,
+                                                                           (const double*)te, s.eta);
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipEventRecord(s.ready, c->copy_stream));
+    s.pending = true;
+    return KELP_OK;
+}
+
+int kelp_field_upload_wait(kelp_ctx* c, int slot)
+{
+    if (!c || slot < 0 || slot >= kMaxSlots) return fail(KELP_EINVAL, "kelp_field_upload_wait: bad argument");
+    FieldSlot& s = c->slots[slot];
+    if (s.ready) HIP_TRY(hipEventSynchronize(s.ready));
     return KELP_OK;
 }
 
@@ -917,6 +1016,12 @@ int kelp_step(kelp_ctx* c, const kelp_particles* p, const kelp_step_args* a)
         return fail(KELP_EINVAL, "kelp_step: a layered u/v field needs W (3-D)");
     }
     HIP_TRY(hipSetDevice(c->device));
+    for (int k : {a->slot_a, a->slot_b, a->slot_t, a->sig_start_a, a->sig_start_b, a->sig_start_t, a->sig_end_a,
+                  a->sig_end_b, a->sig_end_t})
+        if (k >= 0 && k < kMaxSlots) {
+            int rc = kelp_wait_slot(c, c->slots[k]);
+            if (rc) return rc;
+        }
     StepCfg cfg{};
     cfg.alpha0 = a->alpha0;
     for (int k = 0; k < 4; ++k) cfg.alphas[k] = a->alphas[k];

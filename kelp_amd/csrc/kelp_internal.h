@@ -48,6 +48,13 @@ struct FieldSlot {
     void* erec = nullptr;
     double* eta = nullptr;  // (N,) surface elevation, internal node order, or null
     ScalarField scalars[kMaxScalars];
+    // asynchronous refill (kelp_field_upload_async): device staging of the
+    // host planes, and the copy stream's event the readers of the slot wait
+    // on while `pending`
+    void* stage[4] = {nullptr, nullptr, nullptr, nullptr};
+    size_t stage_bytes[4] = {0, 0, 0, 0};
+    hipEvent_t ready = nullptr;
+    bool pending = false;
 };
 
 // growable device buffer owned by a context
@@ -59,6 +66,8 @@ struct DevBuf {
 struct kelp_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t copy_stream = nullptr;  // snapshot refills (default priority)
+    hipEvent_t compute_mark = nullptr;  // "work enqueued so far" on `stream`
     hipEvent_t events[kMaxEvents] = {};
     bool has_mesh = false;
     kelp::DMesh m{};
@@ -75,6 +84,9 @@ struct kelp_ctx {
 };
 
 void kelp_free_scalar(ScalarField& f);
+
+// Order `c->stream` after a pending refill of slot s (no-op otherwise).
+int kelp_wait_slot(kelp_ctx* c, FieldSlot& s);
 
 // (re)allocate `b` to at least `bytes` (contents not preserved)
 int kelp_reserve(DevBuf& b, size_t bytes);

@@ -471,6 +471,11 @@ extern "C" int kelp_sample(kelp_ctx* c, const kelp_particles* p, int var, int sl
     if (!p->lon || !p->lat || !p->status || !p->nn) return kelp_fail(KELP_EINVAL, "kelp_sample: missing particle array");
     if (p->n <= 0) return KELP_OK;
     HIP_TRY(hipSetDevice(c->device));
+    for (int k : {slot_a, slot_b})
+        if (k >= 0) {
+            int rc = kelp_wait_slot(c, c->slots[k]);
+            if (rc) return rc;
+        }
     SField s{fa.planes, (fb && fb->planes) ? fb->planes : fa.planes, alpha, fa.n_planes};
     Snap<double, true> fw;
     fw.a = static_cast<const double4*>(fa.wrec);
@@ -512,6 +517,11 @@ extern "C" int kelp_sample_vars(kelp_ctx* c, const kelp_particles* p, int n_vars
             if (ring(fa0) && ring(fa1) && blend_ok(fa0, fb0) && blend_ok(fa1, fb1) && p->n > 0 &&
                 p->lon && p->lat && p->status && p->nn && (p->z || (fa0.n_planes == 1 && fa1.n_planes == 1))) {
                 HIP_TRY(hipSetDevice(c->device));
+                for (int k : {slot_a, slot_b})
+                    if (k >= 0) {
+                        int rc = kelp_wait_slot(c, c->slots[k]);
+                        if (rc) return rc;
+                    }
                 SField s0{fa0.planes, (fb0 && fb0->planes) ? fb0->planes : fa0.planes, alpha, fa0.n_planes};
                 SField s1{fa1.planes, (fb1 && fb1->planes) ? fb1->planes : fa1.planes, alpha, fa1.n_planes};
                 SampleState st{p->lon, p->lat, p->z, p->status, p->nn, p->n};

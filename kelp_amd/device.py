@@ -281,6 +281,46 @@ class DeviceContext:
               "kelp_field_upload")
         self._slot_keys[slot] = key
 
+    def pinned(self, a):
+        """A page-locked host copy of ``a`` (torch tensor), the form in which
+        :meth:`upload_field_async` copies run asynchronously."""
+        t = self.torch
+        return t.from_numpy(np.ascontiguousarray(a)).pin_memory()
+
+    def upload_field_async(self, slot: int, u, v, w=None, eta=None, uv_layout="node", key=None) -> None:
+        """Refill ``slot`` (already holding a snapshot of this shape) on the
+        copy stream while steps run (kelp_field_upload_async).  Arrays:
+        numpy or pinned torch CPU tensors (see :meth:`pinned`); they are kept
+        alive until the next refill of the slot."""
+        t = self.torch
+
+        def host(a, dtype=None):
+            if a is None:
+                return None
+            if isinstance(a, t.Tensor):
+                return a
+            return np.ascontiguousarray(a, dtype)
+
+        u = host(u)
+        f32 = (u.dtype == t.float32) if isinstance(u, t.Tensor) else (u.dtype == np.float32)
+        npd = np.float32 if f32 else np.float64
+        u, v, w = host(u, npd), host(v, npd), host(w, npd)
+        e = host(eta, np.float64)
+        n_planes = 1 if u.ndim == 1 else int(u.shape[0])
+        nw = 0 if w is None else (1 if w.ndim == 1 else int(w.shape[0]))
+        ptr = lambda a: None if a is None else (a.data_ptr() if isinstance(a, t.Tensor) else a.ctypes.data)
+        keep = getattr(self, "_async_keep", None)
+        if keep is None:
+            keep = self._async_keep = {}
+        if slot in keep:
+            check(self.lib.kelp_field_upload_wait(self.h, int(slot)), "kelp_field_upload_wait")
+        layout = 0 if uv_layout == "node" else 1
+        check(self.lib.kelp_field_upload_async(self.h, int(slot), KELP_DT_F32 if f32 else KELP_DT_F64,
+                                               layout, n_planes, ptr(u), ptr(v), ptr(w), nw, ptr(e)),
+              "kelp_field_upload_async")
+        keep[slot] = (u, v, w, e)
+        self._slot_keys[slot] = key
+
     # -- particles ----------------------------------------------------------
     def particles(self, lon, lat, z=None, status=None, nn=None, fr=None, pid=None,
                   ws=None) -> DeviceParticles:
