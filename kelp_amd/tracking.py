@@ -210,6 +210,7 @@ class _SnapshotSlots:
         self.map: Dict[int, int] = {}
         self.order: List[int] = []
         self.fields: Dict[int, VelocityField] = {}
+        self.in_use: set = set()
 
     def field(self, idx: int) -> VelocityField:
         if idx not in self.fields:
@@ -219,25 +220,59 @@ class _SnapshotSlots:
         return self.fields[idx]
 
     def slot(self, idx: int) -> int:
+        self.in_use.add(idx)
         if idx in self.map:
             self.order.remove(idx)
             self.order.append(idx)
             return self.map[idx]
+        slot = self._victim()
+        self._fill(slot, idx, refill=False)
+        return slot
+
+    def begin_step(self) -> None:
+        """A new step: the snapshots it acquires are protected from the
+        prefetch's eviction until the next step."""
+        self.in_use = set()
+
+    def _victim(self) -> int:
         if len(self.map) < N_SLOTS:
-            slot = min(set(range(N_SLOTS)) - set(self.map.values()))
-        else:
-            old = self.order.pop(0)
-            slot = self.map.pop(old)
-            self.fields.pop(old, None)
+            return min(set(range(N_SLOTS)) - set(self.map.values()))
+        for old in self.order:
+            if old not in self.in_use:
+                self.order.remove(old)
+                self.fields.pop(old, None)
+                return self.map.pop(old)
+        raise RuntimeError("every snapshot slot is in use by this step")
+
+    def _fill(self, slot: int, idx: int, refill: bool) -> None:
         vf = self.field(idx)
-        self.ctx.upload_field(slot, vf.u, vf.v, vf.w, uv_layout=vf.uv_layout)
+        eta = None
         if self.need_eta:
-            self.ctx.upload_eta(slot, vf.eta if vf.eta is not None else np.zeros(self.mesh.n_nodes))
+            eta = vf.eta if vf.eta is not None else np.zeros(self.mesh.n_nodes)
+        if refill:
+            # the snapshot ring: the copy runs behind the enqueued steps
+            self.ctx.upload_field_async(slot, vf.u, vf.v, vf.w, eta=eta, uv_layout=vf.uv_layout)
+        else:
+            self.ctx.upload_field(slot, vf.u, vf.v, vf.w, uv_layout=vf.uv_layout)
+            if eta is not None:
+                self.ctx.upload_eta(slot, eta)
         for k, name in enumerate(self.sample_vars or ()):
             self.ctx.upload_scalar(slot, k, vf.scalars[name])
         self.map[idx] = slot
         self.order.append(idx)
-        return slot
+
+    def prefetch(self, idx: int) -> None:
+        """tracking.py:584 _VelCache read-ahead on the device: snapshot
+        ``idx`` is uploaded into a slot no snapshot of this step uses, as an
+        asynchronous refill when that slot already holds a snapshot of the
+        same shape (the usual case once the run has filled its slots)."""
+        if idx in self.map:
+            return
+        full = len(self.map) >= N_SLOTS
+        if full and all(old in self.in_use for old in self.order):
+            return          # every slot serves this step: no read-ahead
+        slot = self._victim()
+        self._fill(slot, idx, refill=full)
 
 
 def _beach_args(numerics: NumericsConfig, dt):
@@ -449,8 +484,10 @@ def run_single_realization(mesh: Mesh, ds, reader, init_state: ParticleState, mo
                          sim_time if use_tinterp else None,
                          release_step=release_step, particle_offset=particle_offset)
 
+    prefetch = int(getattr(numerics, "prefetch_size", 1) or 0) > 0
     for step_i in range(n_steps):
         g = start_step_i + step_i
+        slots.begin_step()
         ctx.activate(p, rel_dev, particle_offset, g, prev_status)
         if use_tinterp:
             bracket = resolve_time_bracket(sim_time, file_times, n_time_total, on_exhausted,
@@ -509,6 +546,15 @@ def run_single_realization(mesh: Mesh, ds, reader, init_state: ParticleState, mo
                  alpha0=pay.alpha0, alphas=alphas, exact_nearest=exact, w_method=w_method,
                  kick=kick, beach=beach, step_i=g, real_id=realization_id, reuse_anchor=True,
                  sigma=sigma, **periodic)
+        if prefetch and n_time_total and n_time_total > 1:
+            # the snapshot after this step's bracket streams in while it runs
+            ahead = max(i for i in (pay.ia, pay.ib, pay.it) if i is not None) if direction > 0 else \
+                min(i for i in (pay.ia, pay.ib, pay.it) if i is not None)
+            nxt = ahead + direction
+            if on_exhausted == "cycle":
+                nxt %= n_time_total
+            if 0 <= nxt < n_time_total:
+                slots.prefetch(nxt)
         if use_tinterp:
             sim_time = sim_time_end if sim_time_end is not None else sim_time + dt * direction
         if (g + 1) % max(int(numerics.output_freq), 1) == 0:

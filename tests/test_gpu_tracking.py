@@ -180,3 +180,21 @@ def test_checkpoint_resume_is_bitwise(gpu_ctx, tmp_path, split, tinterp):
     for k in ("lon", "lat", "z", "status", "failure_reason", "vel_u", "vel_v", "vel_w", "time"):
         both = np.concatenate([first[k], rest[k]], axis=-1)
         np.testing.assert_array_equal(both, full[k], err_msg=k)
+
+
+def test_beaching_golden_through_a_three_slot_ring(gpu_ctx, monkeypatch):
+    """The tracking loop with only three device slots: snapshots are evicted
+    and the read-ahead refills them asynchronously behind the running steps
+    (eta included); the reference's run-level golden still holds."""
+    import kelp_amd.tracking as T
+    monkeypatch.setattr(T, "N_SLOTS", 3)
+    case = rc.beaching_case(1.0)
+    gold = np.load(os.path.join(GOLD, "beaching_resuspension_rk2.npz"), allow_pickle=False)
+    rec = run_single_realization(
+        case["mesh"], case["ds"], get_reader("fesom2_arrays"), case["init"], case["mode"],
+        case["numerics"], start_time_idx=case["start_time_idx"], n_steps=case["n_steps"],
+        n_time_total=case["n_time_total"], file_times=case["file_times"],
+        file_dt=case["file_dt"], ctx=gpu_ctx).as_arrays()
+    np.testing.assert_array_equal(rec["status"], gold["status"])
+    for k, src in (("lon", "lon"), ("lat", "lat"), ("z", "z"), ("u", "vel_u"), ("w", "vel_w")):
+        np.testing.assert_allclose(rec[src], gold[k], rtol=1e-9, atol=1e-9, err_msg=k)
