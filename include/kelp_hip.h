@@ -59,6 +59,10 @@ typedef struct {
                                                       host_z_levels = sigma_lev, 1 at the
                                                       surface .. 0 at the bottom,
                                                       non-increasing) */
+    const int32_t *host_elem_neighbors;            /* (E, max_v) core/mesh.py:2824
+                                                      build_elem_neighbors (mesh ids, -1
+                                                      boundary) or NULL; needed by the
+                                                      edge walk (elem_search: walk) */
 } kelp_mesh_desc;
 
 /* Particle state, device memory, structure of arrays (the fields of the
@@ -73,6 +77,8 @@ typedef struct {
     int64_t *pid;      /* global particle id = RNG key; needed with the kick, beaching, sorting */
     int32_t *wet_streak; /* beaching wet streak (tracking loop state); needed with beaching */
     uint64_t *alive_count; /* optional: += particles alive at the step start */
+    int32_t *elem;     /* carried containing element (internal numbering, -1 unknown), with
+                          elem_search walk (ParticleState.elem); NULL otherwise */
 } kelp_particles;
 
 /* One step (the arguments of step_fn at tracking.py:2643-2655 and the
@@ -129,6 +135,9 @@ typedef struct {
     double sig_end_alpha0, sig_end_alpha;
     double Dmin;                   /* minimum column height (m) */
     double z0b_min;                /* distance kept above the bed (m) */
+    int32_t walk;                  /* elem_search walk (core/mesh.py:2489): every site's
+                                      containing element by the edge walk from the carried
+                                      one, the domain verdict too; node-layout u/v, z-level */
 } kelp_step_args;
 
 const char *kelp_last_error(void);
@@ -150,6 +159,10 @@ int kelp_mesh_node_xyz(kelp_ctx *ctx, double *host_out);
  * in place between the mesh's numbering and the internal one (-1 kept).  No
  * reference counterpart (the reference works in mesh numbering throughout). */
 int kelp_nodes_to_mesh_ids(kelp_ctx *ctx, int64_t n, int32_t *ids);
+/* Element ids (device int32, n) between the mesh's and the internal
+ * numbering (kelp_particles.elem), in place. */
+int kelp_elems_to_mesh_ids(kelp_ctx *ctx, int64_t n, int32_t *ids);
+int kelp_elems_from_mesh_ids(kelp_ctx *ctx, int64_t n, int32_t *ids);
 int kelp_nodes_from_mesh_ids(kelp_ctx *ctx, int64_t n, int32_t *ids);
 
 /* Field snapshot slot (the reference's VelocityField, core/io.py:180).

@@ -33,7 +33,7 @@ class MeshDesc(ctypes.Structure):
         ("n_levels", c_int64), ("host_z_levels", c_void_p),
         ("host_elem_coastal", c_void_p),
         ("host_elem_center_lon", c_void_p), ("host_elem_center_lat", c_void_p),
-        ("vertical_coord", c_int64),
+        ("vertical_coord", c_int64), ("host_elem_neighbors", c_void_p),
     ]
 
 
@@ -44,6 +44,7 @@ class Particles(ctypes.Structure):
         ("status", c_void_p), ("nn", c_void_p), ("fr", c_void_p),
         ("u", c_void_p), ("v", c_void_p), ("w", c_void_p),
         ("pid", c_void_p), ("wet_streak", c_void_p), ("alive_count", c_void_p),
+        ("elem", c_void_p),
     ]
 
 
@@ -69,7 +70,7 @@ class StepArgs(ctypes.Structure):
         ("sig_start_alpha0", c_double), ("sig_start_alpha", c_double),
         ("sig_end_a", c_int32), ("sig_end_b", c_int32), ("sig_end_t", c_int32),
         ("sig_end_alpha0", c_double), ("sig_end_alpha", c_double),
-        ("Dmin", c_double), ("z0b_min", c_double),
+        ("Dmin", c_double), ("z0b_min", c_double), ("walk", c_int32),
     ]
 
 
@@ -84,6 +85,8 @@ SIGNATURES = {
     "kelp_mesh_upload": (c_int, [c_void_p, POINTER(MeshDesc)]),
     "kelp_mesh_node_xyz": (c_int, [c_void_p, c_void_p]),
     "kelp_nodes_to_mesh_ids": (c_int, [c_void_p, c_int64, c_void_p]),
+    "kelp_elems_to_mesh_ids": (c_int, [c_void_p, c_int64, c_void_p]),
+    "kelp_elems_from_mesh_ids": (c_int, [c_void_p, c_int64, c_void_p]),
     "kelp_nodes_from_mesh_ids": (c_int, [c_void_p, c_int64, c_void_p]),
     "kelp_field_upload": (c_int, [c_void_p, c_int, c_int, c_int64, c_void_p, c_void_p,
                                   c_void_p, c_int64]),

@@ -55,6 +55,8 @@ struct DMesh {
     const double4* __restrict__ egeo;     // (2E,) containment frame per element, see elem_geom
     const double2* __restrict__ exy;      // (4E,) projected vertices per element
     const uint8_t* __restrict__ eflags;   // (E,) bit 0: the interpolation ring is the containment ring
+    const int4* __restrict__ enbr;        // (E,) neighbour across edge k of the compacted ring, -1 boundary
+                                          //   (core/mesh.py:2824 build_elem_neighbors); null without it
     double cell, lon_min, lat_min, r_earth;
     int32_t nx, ny, wrap, far_rows;
     int32_t n_nodes, n_elems, max_v, n_levels;
@@ -756,6 +758,149 @@ __device__ inline int find_containing(const DMesh& m, double lon_p, double lat_p
         if (element_contains<MAXV>(m, c, lon_p, lat_p)) return c;
     }
     return containing_2ring<MAXV>(m, lon_p, lat_p, nn);
+}
+
+// ---------------------------------------------------------------------------
+// Edge walk (core/mesh.py:2489 _walk_containing_element, :2582
+// _loop_walk_containing): elem_search = walk
+// ---------------------------------------------------------------------------
+
+constexpr int kWalkMaxSteps = 256;     // WALK_MAX_STEPS, core/mesh.py:2473
+constexpr double kWalkEps = 1e-6;      // _WALK_EPS, core/mesh.py:2478
+constexpr double kWalkVerify = 1e-2;   // _WALK_VERIFY_BAND, core/mesh.py:2485
+
+// core/mesh.py:2007 _project_to_local_xy of the compacted ring v[0..n)
+template <int MAXV>
+__device__ inline void project_ring(const DMesh& m, const int (&v)[4], int n, double lon_p, double lat_p,
+                                    double (&x)[4], double (&y)[4], double& px, double& py)
+{
+    double lo[4] = {0, 0, 0, 0}, la[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < MAXV; ++k) {
+        if (k < n) {
+            const double2 ll = m.lonlat[v[k]];
+            lo[k] = ll.x;
+            la[k] = ll.y;
+        }
+    }
+    double lmin = lo[0], lmax = lo[0];
+#pragma unroll
+    for (int k = 0; k < MAXV; ++k) {
+        if (k < n) {
+            if (lo[k] < lmin) lmin = lo[k];
+            if (lo[k] > lmax) lmax = lo[k];
+        }
+    }
+    const double nd = (double)n;
+    double lon0;
+    if (lmax - lmin > 180.0) {
+        double dsum = 0.0;
+#pragma unroll
+        for (int k = 0; k < MAXV; ++k)
+            if (k < n) dsum += km_pymod360(lo[k] - lo[0] + 180.0) - 180.0;
+        lon0 = lo[0] + dsum / nd;
+    } else {
+        double ssum = 0.0;
+#pragma unroll
+        for (int k = 0; k < MAXV; ++k)
+            if (k < n) ssum += lo[k];
+        lon0 = ssum / nd;
+    }
+    double sl = 0.0;
+#pragma unroll
+    for (int k = 0; k < MAXV; ++k)
+        if (k < n) sl += la[k];
+    const double lat0 = sl / nd;
+    const double cosl = km_cos(lat0 * kDeg2Rad);
+#pragma unroll
+    for (int k = 0; k < MAXV; ++k) {
+        x[k] = 0.0;
+        y[k] = 0.0;
+        if (k < n) {
+            x[k] = (km_pymod360(lo[k] - lon0 + 180.0) - 180.0) * kDeg2Rad * kREarthStd * cosl;
+            y[k] = (la[k] - lat0) * kDeg2Rad * kREarthStd;
+        }
+    }
+    px = (km_pymod360(lon_p - lon0 + 180.0) - 180.0) * kDeg2Rad * kREarthStd * cosl;
+    py = (lat_p - lat0) * kDeg2Rad * kREarthStd;
+}
+
+template <int MAXV>
+__device__ inline int walk_element(const DMesh& m, double lon_p, double lat_p, int start)
+{
+    int e = start;
+    if (e < 0 || e >= m.n_elems) return -1;
+    int prev = -1;
+    for (int it = 0; it < kWalkMaxSteps; ++it) {
+        int v[4] = {0, 0, 0, 0};
+        const int4 en = m.elems[e];
+        const int nv = interp_ring<MAXV>(en, v);
+        if (nv < 3) return -1;
+        double x[4], y[4], px, py;
+        if (m.eflags[e] & 1) {
+            // the element's precomputed frame is this projection
+            const double4 g1 = m.egeo[2 * (int64_t)e + 1];
+            const double2* xy = m.exy + 4 * (int64_t)e;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                x[k] = k < nv ? xy[k].x : 0.0;
+                y[k] = k < nv ? xy[k].y : 0.0;
+            }
+            px = (km_pymod360(lon_p - g1.y + 180.0) - 180.0) * kDeg2Rad * kREarthStd * g1.w;
+            py = (lat_p - g1.z) * kDeg2Rad * kREarthStd;
+        } else {
+            project_ring<MAXV>(m, v, nv, lon_p, lat_p, x, y, px, py);
+        }
+        double S = 0.0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (k < nv) {
+                const int j = (k + 1 == nv) ? 0 : k + 1;
+                S += x[k] * y[j] - x[j] * y[k];
+            }
+        }
+        const double absS = km_fabs(S);
+        if (absS < kAreaTol) return -1;
+        const double sgn = S > 0.0 ? 1.0 : -1.0;
+        double worst = 1.0e100;
+        int wk = -1;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (k < nv) {
+                const int j = (k + 1 == nv) ? 0 : k + 1;
+                const double A = (x[j] - x[k]) * (py - y[k]) - (px - x[k]) * (y[j] - y[k]);
+                const double w = A * sgn / absS;
+                if (w < worst) {
+                    worst = w;
+                    wk = k;
+                }
+            }
+        }
+        if (worst >= -kWalkEps) {
+            if (worst >= kWalkVerify) return e;
+            return element_contains<MAXV>(m, e, lon_p, lat_p) ? e : -1;
+        }
+        const int4 nb4 = m.enbr[e];
+        const int nb = wk == 0 ? nb4.x : (wk == 1 ? nb4.y : (wk == 2 ? nb4.z : nb4.w));
+        if (nb < 0 || nb == prev) return -1;
+        prev = e;
+        e = nb;
+    }
+    return -1;
+}
+
+// One particle of _loop_walk_containing: the walk from the carried element,
+// else the local search around the widened grid-hash nearest node.
+template <int MAXV>
+__device__ inline int walk_locate(const DMesh& m, double lon_p, double lat_p, int start)
+{
+    if (start >= 0) {
+        const int e = walk_element<MAXV>(m, lon_p, lat_p, start);
+        if (e >= 0) return e;
+    }
+    const int nn = nearest_widened(m, lon_p, lat_p);
+    if (nn < 0) return -1;
+    return find_containing<MAXV>(m, lon_p, lat_p, nn);
 }
 
 // ---------------------------------------------------------------------------

@@ -488,6 +488,7 @@ int kelp_mesh_upload(kelp_ctx* c, const kelp_mesh_desc* d)
     std::vector<uint8_t> coastal(d->host_elem_coastal ? E : 0);
     const bool has_centres = d->host_elem_center_lon && d->host_elem_center_lat;
     std::vector<double2> ecent(has_centres ? E : 0);
+    std::vector<int4> enbr(d->host_elem_neighbors ? E : 0);
     for (int64_t en = 0; en < E; ++en) {
         int64_t eo = e_old_of_new[en];
         int v[4] = {-1, -1, -1, -1};
@@ -496,6 +497,15 @@ int kelp_mesh_upload(kelp_ctx* c, const kelp_mesh_desc* d)
             v[k] = x >= 0 ? new_of_old[x] : -1;
         }
         elems[en] = make_int4(v[0], v[1], v[2], v[3]);
+        if (d->host_elem_neighbors) {
+            int b[4] = {-1, -1, -1, -1};
+            for (int k = 0; k < d->max_v; ++k) {
+                int x = d->host_elem_neighbors[eo * d->max_v + k];
+                if (x < -1 || x >= E) return fail(KELP_EINVAL, "kelp_mesh_upload: element neighbour out of range");
+                b[k] = x >= 0 ? e_new_of_old[x] : -1;
+            }
+            enbr[en] = make_int4(b[0], b[1], b[2], b[3]);
+        }
         if (d->host_elem_coastal) coastal[en] = d->host_elem_coastal[eo] ? 1 : 0;
         if (has_centres) ecent[en] = make_double2(d->host_elem_center_lon[eo], d->host_elem_center_lat[eo]);
     }
@@ -533,6 +543,7 @@ int kelp_mesh_upload(kelp_ctx* c, const kelp_mesh_desc* d)
     double2* d_exy = nullptr;
     uint8_t* d_eflags = nullptr;
     double* d_sig_eta = nullptr;
+    int4* d_enbr = nullptr;
     int rc;
     if ((rc = dev_alloc(c, &d_lonlat, N)) || (rc = dev_alloc(c, &d_xyz, N)) ||
         (rc = dev_alloc(c, &d_off, n_cells + 1)) || (rc = dev_alloc(c, &d_elems, E)) ||
@@ -543,7 +554,8 @@ int kelp_mesh_upload(kelp_ctx* c, const kelp_mesh_desc* d)
         (d->host_elem_coastal && (rc = dev_alloc(c, &d_coastal, E))) ||
         (has_centres && (rc = dev_alloc(c, &d_ecent, E))) || (rc = dev_alloc(c, &d_egeo, 2 * E)) ||
         (rc = dev_alloc(c, &d_exy, 4 * E)) || (rc = dev_alloc(c, &d_eflags, E)) ||
-        (d->vertical_coord == 1 && (rc = dev_alloc(c, &d_sig_eta, 5 * N)))) {
+        (d->vertical_coord == 1 && (rc = dev_alloc(c, &d_sig_eta, 5 * N))) ||
+        (d->host_elem_neighbors && (rc = dev_alloc(c, &d_enbr, E)))) {
         free_mesh(c);
         return rc;
     }
@@ -561,6 +573,7 @@ int kelp_mesh_upload(kelp_ctx* c, const kelp_mesh_desc* d)
         HIP_TRY(hipMemcpy(d_zlev, d->host_z_levels, d->n_levels * sizeof(double), hipMemcpyHostToDevice));
     if (d_coastal) HIP_TRY(hipMemcpy(d_coastal, coastal.data(), E * sizeof(uint8_t), hipMemcpyHostToDevice));
     if (d_ecent) HIP_TRY(hipMemcpy(d_ecent, ecent.data(), E * sizeof(double2), hipMemcpyHostToDevice));
+    if (d_enbr) HIP_TRY(hipMemcpy(d_enbr, enbr.data(), E * sizeof(int4), hipMemcpyHostToDevice));
     node_xyz_kernel<<<grid_for(N, 256), 256, 0, c->stream>>>(N, d_lonlat, d_xyz, d_coslat);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -579,6 +592,7 @@ int kelp_mesh_upload(kelp_ctx* c, const kelp_mesh_desc* d)
     m.egeo = d_egeo;
     m.exy = d_exy;
     m.eflags = d_eflags;
+    m.enbr = d_enbr;
     m.cell = d->gh_cell;
     m.lon_min = d->gh_lon_min;
     m.lat_min = d->gh_lat_min;
@@ -630,6 +644,26 @@ int kelp_nodes_to_mesh_ids(kelp_ctx* c, int64_t n, int32_t* ids)
     if (!c->has_mesh) return fail(KELP_ESTATE, "kelp_nodes_to_mesh_ids: no mesh uploaded");
     if (n == 0) return KELP_OK;
     map_ids_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(n, c->node_old_of_new, ids);
+    HIP_TRY(hipGetLastError());
+    return KELP_OK;
+}
+
+int kelp_elems_to_mesh_ids(kelp_ctx* c, int64_t n, int32_t* ids)
+{
+    if (!c || n < 0 || (n > 0 && !ids)) return fail(KELP_EINVAL, "kelp_elems_to_mesh_ids: bad argument");
+    if (!c->has_mesh) return fail(KELP_ESTATE, "kelp_elems_to_mesh_ids: no mesh uploaded");
+    if (n == 0) return KELP_OK;
+    map_ids_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(n, c->elem_old_of_new, ids);
+    HIP_TRY(hipGetLastError());
+    return KELP_OK;
+}
+
+int kelp_elems_from_mesh_ids(kelp_ctx* c, int64_t n, int32_t* ids)
+{
+    if (!c || n < 0 || (n > 0 && !ids)) return fail(KELP_EINVAL, "kelp_elems_from_mesh_ids: bad argument");
+    if (!c->has_mesh) return fail(KELP_ESTATE, "kelp_elems_from_mesh_ids: no mesh uploaded");
+    if (n == 0) return KELP_OK;
+    map_ids_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(n, c->elem_new_of_old, ids);
     HIP_TRY(hipGetLastError());
     return KELP_OK;
 }
@@ -971,6 +1005,11 @@ int kelp_step(kelp_ctx* c, const kelp_particles* p, const kelp_step_args* a)
     if (!(a->dt > 0.0)) return fail(KELP_EINVAL, "kelp_step: dt must be positive (direction goes in time_dir)");
     if (a->time_dir != 1.0 && a->time_dir != -1.0) return fail(KELP_EINVAL, "kelp_step: time_dir must be +1 or -1");
     const bool sigma = c->vertical_coord == 1;
+    if (a->walk) {
+        if (!p->elem) return fail(KELP_EINVAL, "kelp_step: elem_search walk needs the carried elements (elem)");
+        if (!c->m.enbr) return fail(KELP_ESTATE, "kelp_step: elem_search walk needs the element neighbours");
+        if (sigma) return fail(KELP_EINVAL, "kelp_step: elem_search walk on sigma meshes is not implemented");
+    }
     if (a->w_method < 0 || a->w_method > (sigma ? 3 : 2))
         return fail(KELP_EINVAL, "kelp_step: w_method must be 0 (nearest), 1 (leastsq), 2 (barycentric)%s",
                     sigma ? " or 3 (idw)" : "");
@@ -1058,6 +1097,9 @@ int kelp_step(kelp_ctx* c, const kelp_particles* p, const kelp_step_args* a)
     // the kick and the fold re-anchor in mode 1 (exact) or 2, so a stored
     // anchor matches the step's own search in those modes only
     cfg.reuse_anchor = (a->reuse_anchor && a->exact_nearest != 0) ? 1 : 0;
+    cfg.walk = a->walk ? 1 : 0;
+    if (cfg.walk && sa.layout != 0)
+        return fail(KELP_EINVAL, "kelp_step: elem_search walk with element-centre u/v is not implemented");
     cfg.periodic_lon = a->periodic_lon ? 1 : 0;
     cfg.periodic_lat = a->periodic_lat ? 1 : 0;
     cfg.px0 = a->px0;
@@ -1076,7 +1118,7 @@ int kelp_step(kelp_ctx* c, const kelp_particles* p, const kelp_step_args* a)
         if (rc) return rc;
     }
     PState ps{p->lon, p->lat, p->z, p->status, p->nn, p->fr, p->u, p->v, p->w, p->pid, p->wet_streak,
-              reinterpret_cast<unsigned long long*>(p->alive_count), p->n};
+              reinterpret_cast<unsigned long long*>(p->alive_count), p->n, p->elem};
     if (sa.dtype == KELP_DT_F32)
         launch_step<float>(c, a->order, is3d, sa, sb, st, cfg, ps);
     else

@@ -100,7 +100,7 @@ extern "C" int kelp_sort_particles(kelp_ctx* c, const kelp_particles* p)
         (rc = permute(c, n, perm, p->nn)) || (rc = permute(c, n, perm, p->fr)) ||
         (rc = permute(c, n, perm, p->u)) || (rc = permute(c, n, perm, p->v)) ||
         (rc = permute(c, n, perm, p->w)) || (rc = permute(c, n, perm, p->pid)) ||
-        (rc = permute(c, n, perm, p->wet_streak)))
+        (rc = permute(c, n, perm, p->wet_streak)) || (rc = permute(c, n, perm, p->elem)))
         return rc;
     return KELP_OK;
 }

@@ -32,6 +32,7 @@ struct StepCfg {
     int periodic_lon, periodic_lat;  // post-step fold windows [x0, x1), [y0, y1)
     double px0, px1, py0, py1;
     int reuse_anchor;  // an ALIVE particle's nn >= 0 is the anchor of its position
+    int walk;          // elem_search = walk: containing elements by the edge walk (PState.elem carried)
     // sigma meshes: surface elevation per node of stages 0..3 and of the step
     // end (internal node order), column floor and bed distance
     const double* sig_eta[5];
@@ -52,6 +53,7 @@ struct PState {
     int32_t* __restrict__ ws;
     unsigned long long* __restrict__ alive_count;
     int64_t n;
+    int32_t* __restrict__ elem;  // carried containing element (internal numbering) or null
 };
 
 // Launch step_kernel<order, MAXV, IS3D, T, ELEM> over all particles on

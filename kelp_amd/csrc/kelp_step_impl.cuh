@@ -39,11 +39,13 @@ __device__ __forceinline__ double clamp_z(const DMesh& m, double z, int nn)
 template <int MAXV, typename T, bool IS3D>
 __device__ inline bool interp_uv(const DMesh& m, const Snap<T, IS3D>& f, double alpha0, double alpha,
                                  double time_dir, double xp, double yp, int nn, int64_t off, double& U,
-                                 double& V)
+                                 double& V, int ehint = -1)
 {
     U = 0.0;
     V = 0.0;
-    int e = find_containing<MAXV>(m, xp, yp, nn);
+    // a containing element validated for this position (the edge walk,
+    // interpolate.py:1266) skips the search
+    int e = ehint >= 0 ? ehint : find_containing<MAXV>(m, xp, yp, nn);
     if (e < 0) return false;
     Weights W;
     if (!vertex_weights<MAXV>(m, e, xp, yp, W)) return false;
@@ -69,7 +71,7 @@ __device__ inline bool interp_uv(const DMesh& m, const Snap<T, IS3D>& f, double 
 template <int MAXV, bool IS3D, typename T, bool ELEM>
 __device__ inline bool eval_site(const DMesh& m, const Snap<T, IS3D>& f, const Snap<T, false>& fe,
                                  const StepCfg& c, double alpha, double lon, double lat, double zp, int& nn,
-                                 double& U, double& V, double& W)
+                                 double& U, double& V, double& W, int ehint = -1)
 {
     // nn >= 0 on entry: the anchor of (lon, lat) under this mode is known
     if (nn < 0) nn = site_anchor(m, lon, lat, c.exact);
@@ -79,7 +81,7 @@ __device__ inline bool eval_site(const DMesh& m, const Snap<T, IS3D>& f, const S
     if constexpr (ELEM)
         ok = interp_lsq(m, fe, c.alpha0, alpha, c.time_dir, lon, lat, nn, layer * m.n_elems, U, V);
     else
-        ok = interp_uv<MAXV>(m, f, c.alpha0, alpha, c.time_dir, lon, lat, nn, layer * m.n_nodes, U, V);
+        ok = interp_uv<MAXV>(m, f, c.alpha0, alpha, c.time_dir, lon, lat, nn, layer * m.n_nodes, U, V, ehint);
     W = 0.0;
     if constexpr (IS3D) {
         if (c.w_method == 0)
@@ -93,12 +95,26 @@ __device__ inline bool eval_site(const DMesh& m, const Snap<T, IS3D>& f, const S
 }
 
 // The deterministic step of one alive particle, in place.  Euler follows
+// The walk kept out of line: it runs only with elem_search = walk.
+template <int MAXV>
+__device__ __noinline__ int walk_at(const DMesh& m, double lon, double lat, int start)
+{
+    return walk_locate<MAXV>(m, lon, lat, start);
+}
+
 // integrators.py:4997 _euler_step_unfused, RK2 the fused barycentric kernels
 // :2522 (2-D) / :3648 (3-D), RK4 :2642 (2-D) and :5760 _rk4_step_unfused
 // (3-D).  Returns the failure code (0 = alive).
+//
+// With elem_search = walk (c.walk, node-layout u/v; the reference's
+// multi-pass steps with carried elements, integrators.py:5020-5022,
+// :5415-5417, :5792-5794) every site's containing element comes from the
+// edge walk started at the previous site's (the carried element at site 1)
+// and the end-of-step domain verdict from the walk too; `elem` is carried.
 template <int ORDER, int MAXV, bool IS3D, typename T, bool ELEM>
 __device__ inline int det_step(const DMesh& m, const Snap<T, IS3D>& f, const Snap<T, false>& fe, const StepCfg& c,
-                               double& lon, double& lat, double& z, int& nn, double& Uo, double& Vo, double& Wo)
+                               double& lon, double& lat, double& z, int& nn, double& Uo, double& Vo, double& Wo,
+                               int& elem)
 {
     const double dt = c.dt;
     const double lon0 = lon, lat0 = lat, zz0 = z;
@@ -107,7 +123,9 @@ __device__ inline int det_step(const DMesh& m, const Snap<T, IS3D>& f, const Sna
     // site 1 is the particle's position: its anchor is the one the previous
     // step (or kick, or fold) stored, when the caller vouches for it
     node_i = (c.reuse_anchor && nn >= 0) ? nn : -1;
-    bool ok = eval_site<MAXV, IS3D, T, ELEM>(m, f, fe, c, c.alphas[0], lon0, lat0, zz0, node_i, U1, V1, W1);
+    const int e1 = c.walk ? walk_at<MAXV>(m, lon0, lat0, elem) : -1;
+    int e_last = e1;
+    bool ok = eval_site<MAXV, IS3D, T, ELEM>(m, f, fe, c, c.alphas[0], lon0, lat0, zz0, node_i, U1, V1, W1, e1);
     Uo = U1;
     Vo = V1;
     Wo = W1;
@@ -124,7 +142,9 @@ __device__ inline int det_step(const DMesh& m, const Snap<T, IS3D>& f, const Sna
             double z_mid = zz0 + W1 * 0.5 * dt;
             int nn_mid = -1;
             double U2, V2, W2;
-            ok = eval_site<MAXV, IS3D, T, ELEM>(m, f, fe, c, c.alphas[1], lon_mid, lat_mid, z_mid, nn_mid, U2, V2, W2);
+            if (c.walk) e_last = walk_at<MAXV>(m, lon_mid, lat_mid, e_last);
+            ok = eval_site<MAXV, IS3D, T, ELEM>(m, f, fe, c, c.alphas[1], lon_mid, lat_mid, z_mid, nn_mid, U2, V2, W2,
+                                                e_last);
             if (ok) {
                 double z0m = z0_of(lat_mid);
                 lon_new = wrap_lon(lon0 + dt * U2 / z0m);
@@ -140,7 +160,8 @@ __device__ inline int det_step(const DMesh& m, const Snap<T, IS3D>& f, const Sna
             double z2 = zz0 + 0.5 * dz1;
             int nn2 = -1, nn3 = -1, nn4 = -1;
             double U2, V2, W2, U3, V3, W3, U4, V4, W4;
-            ok = eval_site<MAXV, IS3D, T, ELEM>(m, f, fe, c, c.alphas[1], lon2, lat2, z2, nn2, U2, V2, W2);
+            if (c.walk) e_last = walk_at<MAXV>(m, lon2, lat2, e_last);
+            ok = eval_site<MAXV, IS3D, T, ELEM>(m, f, fe, c, c.alphas[1], lon2, lat2, z2, nn2, U2, V2, W2, e_last);
             if (ok) {
                 double dlon2 = dt * U2 / z0_of(lat2);
                 double dlat2 = dt * V2 / kSLM;
@@ -148,7 +169,9 @@ __device__ inline int det_step(const DMesh& m, const Snap<T, IS3D>& f, const Sna
                 double lon3 = wrap_lon(lon0 + 0.5 * dlon2);
                 double lat3 = clamp_lat(lat0 + 0.5 * dlat2);
                 double z3 = zz0 + 0.5 * dz2;
-                ok = eval_site<MAXV, IS3D, T, ELEM>(m, f, fe, c, c.alphas[2], lon3, lat3, z3, nn3, U3, V3, W3);
+                if (c.walk) e_last = walk_at<MAXV>(m, lon3, lat3, e_last);
+                ok = eval_site<MAXV, IS3D, T, ELEM>(m, f, fe, c, c.alphas[2], lon3, lat3, z3, nn3, U3, V3, W3,
+                                                    e_last);
                 if (ok) {
                     double dlon3 = dt * U3 / z0_of(lat3);
                     double dlat3 = dt * V3 / kSLM;
@@ -156,7 +179,9 @@ __device__ inline int det_step(const DMesh& m, const Snap<T, IS3D>& f, const Sna
                     double lon4 = wrap_lon(lon0 + dlon3);
                     double lat4 = clamp_lat(lat0 + dlat3);
                     double z4 = zz0 + dz3;
-                    ok = eval_site<MAXV, IS3D, T, ELEM>(m, f, fe, c, c.alphas[3], lon4, lat4, z4, nn4, U4, V4, W4);
+                    if (c.walk) e_last = walk_at<MAXV>(m, lon4, lat4, e_last);
+                    ok = eval_site<MAXV, IS3D, T, ELEM>(m, f, fe, c, c.alphas[3], lon4, lat4, z4, nn4, U4, V4,
+                                                        W4, e_last);
                     if (ok) {
                         double dlon4 = dt * U4 / z0_of(lat4);
                         double dlat4 = dt * V4 / kSLM;
@@ -174,18 +199,21 @@ __device__ inline int det_step(const DMesh& m, const Snap<T, IS3D>& f, const Sna
     }
     if (!ok) {
         nn = node_i;
+        if (c.walk) elem = e1;
         return kFrInterp;
     }
     int nn_new = site_anchor(m, lon_new, lat_new, c.exact);
-    int e = find_containing<MAXV>(m, lon_new, lat_new, nn_new);
+    int e = c.walk ? walk_at<MAXV>(m, lon_new, lat_new, e_last) : find_containing<MAXV>(m, lon_new, lat_new, nn_new);
     if (e < 0) {
         nn = node_i;
+        if (c.walk) elem = e1;
         return kFrOOB;
     }
     lon = lon_new;
     lat = lat_new;
     if (IS3D) z = clamp_z(m, z_new, nn_new);
     nn = nn_new;
+    if (c.walk) elem = e;
     return 0;
 }
 
@@ -279,7 +307,8 @@ __device__ inline void az_sigma_one(const DMesh& m, const StepCfg& c, uint64_t p
 // coordinate outside [x0, x1) back with numpy's mod (Python % semantics),
 // strictly inside the window, and re-anchor a folded particle with the batch
 // search (find_nearest_nodes).
-__device__ inline void periodic_one(const DMesh& m, const StepCfg& c, double& lon, double& lat, int& nn)
+__device__ inline void periodic_one(const DMesh& m, const StepCfg& c, double& lon, double& lat, int& nn,
+                                    int& elem)
 {
     bool moved = false;
     if (c.periodic_lon && (lon < c.px0 || lon >= c.px1)) {
@@ -294,7 +323,10 @@ __device__ inline void periodic_one(const DMesh& m, const StepCfg& c, double& lo
         lat = fo;
         moved = true;
     }
-    if (moved) nn = site_anchor(m, lon, lat, c.exact == 1 ? 1 : 2);
+    if (moved) {
+        nn = site_anchor(m, lon, lat, c.exact == 1 ? 1 : 2);
+        elem = -1;   // tracking.py:1143: a folded particle's carried element is dropped
+    }
 }
 
 // tracking.py:199 _apply_beaching_hook + numerics/beaching.py:95
@@ -386,11 +418,12 @@ __global__ __launch_bounds__(256) KELP_STEP_ATTR void step_kernel(DMesh m, Snap<
     double z = IS3D ? s.z[i] : 0.0;
     int nn = s.nn[i];
     int8_t fr = s.fr[i];
+    int elem = s.elem ? s.elem[i] : -1;
     double U = 0.0, V = 0.0, W = 0.0;
     if (prev == kAlive) {
         const double pre_lon = lon, pre_lat = lat, pre_z = z;
         const int pre_nn = nn;
-        int code = det_step<ORDER, MAXV, IS3D, T, ELEM>(m, f, fe, c, lon, lat, z, nn, U, V, W);
+        int code = det_step<ORDER, MAXV, IS3D, T, ELEM>(m, f, fe, c, lon, lat, z, nn, U, V, W, elem);
         if (code) {
             st = kOutside;
             fr = (int8_t)code;
@@ -398,9 +431,10 @@ __global__ __launch_bounds__(256) KELP_STEP_ATTR void step_kernel(DMesh m, Snap<
             kick_one<MAXV, IS3D>(m, c, (uint64_t)s.pid[i], pre_lon, pre_lat, pre_z, pre_nn, lon, lat, z, st, nn, fr);
         }
         if (IS3D && c.az && st == kAlive) az_one(m, c, (uint64_t)s.pid[i], nn, z);
-        if ((c.periodic_lon || c.periodic_lat) && st == kAlive) periodic_one(m, c, lon, lat, nn);
+        if ((c.periodic_lon || c.periodic_lat) && st == kAlive) periodic_one(m, c, lon, lat, nn, elem);
     }
     if (c.beach) beach_one<MAXV>(m, c, (uint64_t)s.pid[i], prev, lon, lat, st, nn, fr, s.ws[i]);
+    if (s.elem) s.elem[i] = elem;
     s.lon[i] = lon;
     s.lat[i] = lat;
     if (IS3D) s.z[i] = z;

@@ -181,7 +181,8 @@ __global__ __launch_bounds__(256) void sigma_step_kernel(DMesh m, Snap<T, true> 
                                    fr);
     }
     if (c.az && st == kAlive) az_sigma_one<MAXV>(m, c, (uint64_t)s.pid[i], lon, lat, nn, z);
-    if ((c.periodic_lon || c.periodic_lat) && st == kAlive) periodic_one(m, c, lon, lat, nn);
+    int elem = -1;
+    if ((c.periodic_lon || c.periodic_lat) && st == kAlive) periodic_one(m, c, lon, lat, nn, elem);
     if (c.beach) beach_one<MAXV>(m, c, (uint64_t)s.pid[i], prev, lon, lat, st, nn, fr, s.ws[i]);
     s.lon[i] = lon;
     s.lat[i] = lat;

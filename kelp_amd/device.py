@@ -49,6 +49,7 @@ class DeviceParticles:
     ws: object
     alive_count: object = None
     ctx: object = None      # owning DeviceContext (node id translation)
+    elem: object = None     # carried containing element (elem_search walk)
 
     @property
     def n(self) -> int:
@@ -57,7 +58,7 @@ class DeviceParticles:
     def as_struct(self) -> Particles:
         return Particles(self.n, _ptr(self.lon), _ptr(self.lat), _ptr(self.z), _ptr(self.status),
                          _ptr(self.nn), _ptr(self.fr), _ptr(self.u), _ptr(self.v), _ptr(self.w),
-                         _ptr(self.pid), _ptr(self.ws), _ptr(self.alive_count))
+                         _ptr(self.pid), _ptr(self.ws), _ptr(self.alive_count), _ptr(self.elem))
 
     def to_host(self) -> Dict[str, np.ndarray]:
         """Host copies of every array; ``nn`` in the mesh's node numbering.
@@ -73,6 +74,13 @@ class DeviceParticles:
                   "kelp_nodes_to_mesh_ids")
             self.ctx.synchronize()
         out["nn"] = nn.cpu().numpy()
+        if self.elem is not None:
+            el = self.elem.clone()
+            if self.ctx is not None:
+                check(self.ctx.lib.kelp_elems_to_mesh_ids(self.ctx.h, el.numel(), el.data_ptr()),
+                      "kelp_elems_to_mesh_ids")
+                self.ctx.synchronize()
+            out["elem"] = el.cpu().numpy()
         return out
 
 
@@ -82,7 +90,7 @@ RESUSPENSION_CODES = {"never": 0, "auto": 1, "stochastic": 2, "auto_or_stochasti
 def make_step_args(order, slot_a, dt, time_dir, slot_b=-1, slot_t=-1, alpha0=0.0,
                    alphas=(0.0, 0.0, 0.0, 0.0), exact_nearest=True, w_method=2, kick=None,
                    beach=None, step_i=0, real_id=0, periodic_lon=None,
-                   periodic_lat=None, reuse_anchor=False, sigma=None) -> StepArgs:
+                   periodic_lat=None, reuse_anchor=False, sigma=None, walk=False) -> StepArgs:
     a = StepArgs()
     a.order = int(order)
     a.slot_a, a.slot_b, a.slot_t = int(slot_a), int(slot_b), int(slot_t)
@@ -96,6 +104,7 @@ def make_step_args(order, slot_a, dt, time_dir, slot_b=-1, slot_t=-1, alpha0=0.0
     a.step_i = int(step_i)
     a.real_id = int(real_id)
     a.reuse_anchor = int(bool(reuse_anchor))
+    a.walk = int(bool(walk))
     a.sig_start_a = a.sig_start_b = a.sig_start_t = -1
     a.sig_end_a = a.sig_end_b = a.sig_end_t = -1
     if sigma is not None:
@@ -183,6 +192,8 @@ class DeviceContext:
                      else np.ascontiguousarray(mesh.elem_is_coastal, np.uint8)),
             clon=np.ascontiguousarray(mesh.elem_center_lon, np.float64),
             clat=np.ascontiguousarray(mesh.elem_center_lat, np.float64),
+            nbr=(None if getattr(mesh, "elem_neighbors", None) is None
+                 else np.ascontiguousarray(mesh.elem_neighbors, np.int32)),
         )
         d = MeshDesc(
             mesh.n_nodes, mesh.n_elements, mesh.max_v, keep["n2e"].shape[1],
@@ -194,7 +205,8 @@ class DeviceContext:
             mesh.r_earth, int(keep["z"].size), keep["z"].ctypes.data if keep["z"].size else None,
             None if keep["coastal"] is None else keep["coastal"].ctypes.data,
             keep["clon"].ctypes.data, keep["clat"].ctypes.data,
-            1 if mesh.vertical_coord == "sigma" else 0)
+            1 if mesh.vertical_coord == "sigma" else 0,
+            None if keep["nbr"] is None else keep["nbr"].ctypes.data)
         check(self.lib.kelp_mesh_upload(self.h, ctypes.byref(d)), "kelp_mesh_upload")
         self.mesh = mesh
         self._slot_keys.clear()
@@ -323,7 +335,9 @@ class DeviceContext:
 
     # -- particles ----------------------------------------------------------
     def particles(self, lon, lat, z=None, status=None, nn=None, fr=None, pid=None,
-                  ws=None) -> DeviceParticles:
+                  ws=None, elem=None) -> DeviceParticles:
+        """Device particle state; ``nn`` and ``elem`` (carried elements for
+        elem_search walk, -1 = unknown) in the mesh's numbering."""
         t = self.torch
         dev = f"cuda:{self.device}"
         P = len(lon)
@@ -347,16 +361,22 @@ class DeviceContext:
             check(self.lib.kelp_nodes_from_mesh_ids(self.h, nn_t.numel(), nn_t.data_ptr()),
                   "kelp_nodes_from_mesh_ids")
             self.synchronize()
+        el_t = None
+        if elem is not None:
+            el_t = i32(elem, -1)
+            check(self.lib.kelp_elems_from_mesh_ids(self.h, el_t.numel(), el_t.data_ptr()),
+                  "kelp_elems_from_mesh_ids")
+            self.synchronize()
         return DeviceParticles(lon=f64(lon), lat=f64(lat), z=f64(z), status=i32(status, 1),
                                nn=nn_t, fr=fr_t, u=f64(None), v=f64(None), w=f64(None),
-                               pid=pid_t, ws=i32(ws, 0), ctx=self)
+                               pid=pid_t, ws=i32(ws, 0), ctx=self, elem=el_t)
 
     # -- operations ---------------------------------------------------------
     def step(self, p: DeviceParticles, order: int, slot_a: int, dt: float, time_dir: float,
              slot_b: int = -1, slot_t: int = -1, alpha0: float = 0.0, alphas=(0.0, 0.0, 0.0, 0.0),
              exact_nearest: bool = True, w_method: int = 2, kick=None, beach=None,
              step_i: int = 0, real_id: int = 0, periodic_lon=None, periodic_lat=None,
-             reuse_anchor: bool = False, sigma=None) -> None:
+             reuse_anchor: bool = False, sigma=None, walk: bool = False) -> None:
         """One step in place.  ``kick``: dict(kh_x, kh_y, seed[, az_const])
         for the split uniform-Kh random walk (and the vertical Az kick);
         ``beach``: dict of the beaching options (criteria, depth_min, ...);
@@ -365,7 +385,7 @@ class DeviceContext:
         start and end as blends of the slots' eta."""
         a = make_step_args(order, slot_a, dt, time_dir, slot_b, slot_t, alpha0, alphas,
                            exact_nearest, w_method, kick, beach, step_i, real_id,
-                           periodic_lon, periodic_lat, reuse_anchor, sigma)
+                           periodic_lon, periodic_lat, reuse_anchor, sigma, walk)
         s = p.as_struct()
         check(self.lib.kelp_step(self.h, ctypes.byref(s), ctypes.byref(a)), "kelp_step")
 

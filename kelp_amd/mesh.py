@@ -60,6 +60,7 @@ class Mesh:
     gh_far_rows: int = 16
     gh_max_edge_deg: Optional[float] = None
     elem_is_coastal: Optional[np.ndarray] = None
+    elem_neighbors: Optional[np.ndarray] = None   # (E, max_v), build_elem_neighbors
 
     @property
     def n_nodes(self) -> int:
@@ -318,3 +319,55 @@ def make_mesh(name, nodes_lon, nodes_lat, elem_nodes, depth=None, z_levels=None,
         r_earth=float(r_earth), metadata=dict(metadata or {}))
     build_grid_hash(mesh, cell_size)
     return mesh
+
+
+def build_elem_neighbors(mesh: Mesh) -> np.ndarray:
+    """core/mesh.py:2824 build_elem_neighbors: slot k of row e is the element
+    sharing the edge (verts[k], verts[(k+1) % nv]) of e's compacted vertex
+    ring, -1 on a boundary edge (and in unused slots); an edge shared by
+    more than two elements raises.  Cached as ``mesh.elem_neighbors``."""
+    cached = getattr(mesh, "elem_neighbors", None)
+    if cached is not None:
+        return cached
+    en = np.asarray(mesh.elem_nodes, dtype=np.int64)
+    E, mv = en.shape if en.size else (0, 0)
+    out = np.full((E, mv), -1, dtype=np.int32)
+    rows = []
+    for e in range(E):
+        verts = []
+        for nd in en[e]:
+            nd = int(nd)
+            if nd < 0 or (verts and nd == verts[0]):
+                continue
+            verts.append(nd)
+        nv = len(verts)
+        if nv < 3:
+            continue
+        seen = set()
+        for k in range(nv):
+            a, b = verts[k], verts[(k + 1) % nv]
+            if a == b:
+                continue
+            ek = (a, b) if a < b else (b, a)
+            if ek in seen:
+                continue
+            seen.add(ek)
+            rows.append((ek[0], ek[1], e, k))
+    if rows:
+        arr = np.asarray(rows, dtype=np.int64)
+        key = arr[:, 0] * np.int64(mesh.n_nodes + 1) + arr[:, 1]
+        order = np.argsort(key, kind="stable")
+        key_s, arr_s = key[order], arr[order]
+        starts = np.flatnonzero(np.r_[True, key_s[1:] != key_s[:-1]])
+        counts = np.diff(np.r_[starts, key_s.size])
+        if (counts > 2).any():
+            bad = arr_s[starts[counts > 2][0]]
+            raise ValueError(f"non-manifold mesh: edge ({bad[0]}, {bad[1]}) is shared by more "
+                             "than two elements")
+        two = starts[counts == 2]
+        e1, k1 = arr_s[two, 2], arr_s[two, 3]
+        e2, k2 = arr_s[two + 1, 2], arr_s[two + 1, 3]
+        out[e1, k1] = e2
+        out[e2, k2] = e1
+    mesh.elem_neighbors = out
+    return out

@@ -305,8 +305,11 @@ def _check_path(numerics: NumericsConfig, mesh: Mesh, is3d: bool, uv_layout: str
         raise NotImplementedError(f"device path: integrator {numerics.integrator!r}")
     if numerics.temporal_reconstruction != "linear":
         raise NotImplementedError("device path: pchip reconstruction")
-    if numerics.elem_search != "local":
-        raise NotImplementedError("device path: elem_search='walk'")
+    if numerics.elem_search not in ("local", "walk"):
+        raise ValueError(f"numerics.elem_search: unknown {numerics.elem_search!r}")
+    if numerics.elem_search == "walk" and (uv_layout != "node" or mesh.vertical_coord == "sigma"):
+        raise NotImplementedError("device path: elem_search='walk' with element-layout u/v or on a "
+                                  "sigma mesh")
     want = "node" if numerics.uv_interp == "barycentric" else "element"
     if numerics.uv_interp not in ("barycentric", "leastsq") or uv_layout != want:
         raise NotImplementedError(f"device path: uv_interp={numerics.uv_interp!r} with "
@@ -431,6 +434,12 @@ def run_single_realization(mesh: Mesh, ds, reader, init_state: ParticleState, mo
     beach = _beach_args(numerics, dt)
     periodic = dict(periodic_lon=numerics.periodic_lon, periodic_lat=numerics.periodic_lat)
 
+    walk = numerics.elem_search == "walk"
+    if walk and mesh.elem_neighbors is None:
+        from .mesh import build_elem_neighbors
+        build_elem_neighbors(mesh)
+        if ctx is not None and ctx.mesh is mesh:
+            ctx.upload_mesh(mesh)
     if beach is not None and {"coastal_element", "stochastic"} & set(beach["criteria"]):
         # tracking.py:2085: the coastal-element flags of the mesh boundary
         if mesh.elem_is_coastal is None:
@@ -458,8 +467,16 @@ def run_single_realization(mesh: Mesh, ds, reader, init_state: ParticleState, mo
         moving = ((init_state.status == ParticleStatus.ALIVE)
                   | (init_state.status == ParticleStatus.NOT_RELEASED))
         nn0 = np.where(moving, -1, init_state.nearest_node).astype(np.int32)
+    elem0 = None
+    if walk:
+        # tracking.py:2236-2253: the carried elements are seeded by one
+        # global search (a resumed state brings its own)
+        elem0 = init_state.elem
+        if elem0 is None:
+            nn_s = ctx.nearest(init_state.lon, init_state.lat, exact=exact == 1)
+            elem0 = ctx.containing(init_state.lon, init_state.lat, nn_s)
     p = ctx.particles(init_state.lon, init_state.lat, init_state.z, status=init_state.status,
-                      nn=nn0, fr=fr0, pid=pid, ws=ws)
+                      nn=nn0, fr=fr0, pid=pid, ws=ws, elem=elem0)
     torch = ctx.torch
     release_step = init_state.release_step
     rel_dev = None if release_step is None else torch.as_tensor(
@@ -545,7 +562,7 @@ def run_single_realization(mesh: Mesh, ds, reader, init_state: ParticleState, mo
         ctx.step(p, order, sa, step_len, float(direction), slot_b=sb, slot_t=st,
                  alpha0=pay.alpha0, alphas=alphas, exact_nearest=exact, w_method=w_method,
                  kick=kick, beach=beach, step_i=g, real_id=realization_id, reuse_anchor=True,
-                 sigma=sigma, **periodic)
+                 sigma=sigma, walk=walk, **periodic)
         if prefetch and n_time_total and n_time_total > 1:
             # the snapshot after this step's bracket streams in while it runs
             ahead = max(i for i in (pay.ia, pay.ib, pay.it) if i is not None) if direction > 0 else \
@@ -619,6 +636,8 @@ def write_checkpoint(path: str, h: Dict[str, np.ndarray], step_i: int, next_time
     arrays = {"restart_" + k: np.asarray(h[src])[order] for k, src in (
         ("lon", "lon"), ("lat", "lat"), ("z", "z"), ("status", "status"),
         ("nearest_node", "nn"), ("failure_reason", "fr"), ("wet_streak", "ws"), ("pid", "pid"))}
+    if h.get("elem") is not None:
+        arrays["restart_elem"] = np.asarray(h["elem"])[order]
     arrays.update(restart_valid=np.int32(1), restart_step_i=np.int64(step_i),
                   restart_file_idx=np.int64(-1 if next_time_idx is None else next_time_idx),
                   restart_sim_time=np.float64(np.nan if sim_time is None else sim_time),
@@ -649,6 +668,8 @@ def load_restart(path: str) -> Dict[str, object]:
                        failure_reason=d["restart_failure_reason"].astype(np.int8))
     if "restart_release_step" in d:
         st = st.replace(release_step=d["restart_release_step"])
+    if "restart_elem" in d:
+        st = st.replace(elem=d["restart_elem"].astype(np.int32))
     sim = float(d["restart_sim_time"])
     idx = int(d["restart_file_idx"])
     return dict(state=st, start_step_i=int(d["restart_step_i"]) + 1,

@@ -692,6 +692,66 @@ static int elem_ring(const or_mesh_t *m, int e, int *verts)
     return nv;
 }
 
+/* core/mesh.py:2489 _walk_containing_element: (element, 1) or (-1, 0) */
+static int walk_element(const or_mesh_t *m, double lon_p, double lat_p, int start)
+{
+    int e = start;
+    if (e < 0 || e >= m->n_elems) return -1;
+    int prev = -1;
+    for (int it = 0; it < 256; it++) {
+        int verts[4];
+        int nv = elem_ring(m, e, verts);
+        if (nv < 3) return -1;
+        double lon_n[4], lat_n[4], x[4], y[4], px, py;
+        for (int k = 0; k < nv; k++) {
+            lon_n[k] = m->nodes_lon[verts[k]];
+            lat_n[k] = m->nodes_lat[verts[k]];
+        }
+        project_to_local_xy(lon_n, lat_n, nv, lon_p, lat_p, x, y, &px, &py);
+        double S = 0.0;
+        for (int k = 0; k < nv; k++) {
+            int j = (k + 1) % nv;
+            S += x[k] * y[j] - x[j] * y[k];
+        }
+        double absS = fabs(S);
+        if (absS < AREA_TOL) return -1;
+        double sgn = S > 0.0 ? 1.0 : -1.0;
+        double worst = 1.0e100;
+        int worst_k = -1;
+        for (int k = 0; k < nv; k++) {
+            int j = (k + 1) % nv;
+            double A = (x[j] - x[k]) * (py - y[k]) - (px - x[k]) * (y[j] - y[k]);
+            double w = A * sgn / absS;
+            if (w < worst) {
+                worst = w;
+                worst_k = k;
+            }
+        }
+        if (worst >= -1e-6) {
+            if (worst >= 1e-2) return e;
+            return or_element_contains(m, e, lon_p, lat_p) ? e : -1;
+        }
+        int nb = m->elem_neighbors[(int64_t)e * m->max_v + worst_k];
+        if (nb < 0 || nb == prev) return -1;
+        prev = e;
+        e = nb;
+    }
+    return -1;
+}
+
+/* core/mesh.py:2582 _loop_walk_containing for one query: the walk, else
+ * the local search around the widened grid-hash nearest node */
+int or_walk_locate(const or_mesh_t *m, double lon_p, double lat_p, int start)
+{
+    if (start >= 0) {
+        int e = walk_element(m, lon_p, lat_p, start);
+        if (e >= 0) return e;
+    }
+    int nn = or_nearest_widened(m, lon_p, lat_p);
+    if (nn < 0) return -1;
+    return or_find_containing_local(m, lon_p, lat_p, nn);
+}
+
 /* Vertex weights of (xp, yp) in element e: the projection of
  * interpolate.py:1387-1404; returns 0 when the tri sentinel fires. */
 static int elem_vertex_weights(const or_mesh_t *m, int e, double xp, double yp,
@@ -745,13 +805,15 @@ static inline double vblend(const void *a, const void *b, const void *t, int dty
 
 /* interpolate.py:1341 _interp_barycentric_core_tinterp (layer < 0: 2-D)
  * and :1444 _interp_barycentric_core_tinterp_3d.  Returns ok. */
-int or_interp_bary_tinterp(const or_mesh_t *m, const or_fields_t *f, double xp, double yp,
-                           int nn, double time_dir, int layer, double alpha,
-                           double *U_o, double *V_o, int *elem_o)
+/* ehint >= 0: a containing element validated for this position (the edge
+ * walk), the search is skipped (interpolate.py:1266) */
+static int interp_bary_hint(const or_mesh_t *m, const or_fields_t *f, double xp, double yp,
+                            int nn, double time_dir, int layer, double alpha,
+                            double *U_o, double *V_o, int *elem_o, int ehint)
 {
     *U_o = 0.0;
     *V_o = 0.0;
-    int e = or_find_containing_local(m, xp, yp, nn);
+    int e = ehint >= 0 ? ehint : or_find_containing_local(m, xp, yp, nn);
     if (elem_o) *elem_o = e;
     if (e < 0) return 0;
     int verts[4], nv;
@@ -769,6 +831,13 @@ int or_interp_bary_tinterp(const or_mesh_t *m, const or_fields_t *f, double xp, 
     *U_o = time_dir * U;
     *V_o = time_dir * V;
     return 1;
+}
+
+int or_interp_bary_tinterp(const or_mesh_t *m, const or_fields_t *f, double xp, double yp,
+                           int nn, double time_dir, int layer, double alpha,
+                           double *U_o, double *V_o, int *elem_o)
+{
+    return interp_bary_hint(m, f, xp, yp, nn, time_dir, layer, alpha, U_o, V_o, elem_o, -1);
 }
 
 /* interpolate.py:207 _interp_leastsq_core_tinterp (== :52
@@ -1174,7 +1243,7 @@ static inline double z0_of(double lat)
  * _compute_w).  Returns ok of the u/v interpolation. */
 static int eval_site(const or_mesh_t *m, const or_fields_t *f, const or_step_args_t *a, int stage,
                      double lon, double lat, double zp, int *nn_o, double *U, double *V, double *W,
-                     int want_w)
+                     int want_w, int ehint)
 {
     int nn = or_site_anchor(m, lon, lat, a->exact);
     *nn_o = nn;
@@ -1184,7 +1253,7 @@ static int eval_site(const or_mesh_t *m, const or_fields_t *f, const or_step_arg
     if (f->uv_layout == 1)
         ok = or_interp_lsq_tinterp(m, f, lon, lat, nn, a->time_dir, layer, a->alphas[stage], U, V);
     else
-        ok = or_interp_bary_tinterp(m, f, lon, lat, nn, a->time_dir, layer, a->alphas[stage], U, V, NULL);
+        ok = interp_bary_hint(m, f, lon, lat, nn, a->time_dir, layer, a->alphas[stage], U, V, NULL, ehint);
     *W = 0.0;
     if (a->is3d && want_w) {
         if (a->w_method == 0)
@@ -1212,13 +1281,17 @@ static inline double clamp_z(const or_mesh_t *m, double z, int nn)
  * failure code. */
 static int det_step(const or_mesh_t *m, const or_fields_t *f, const or_step_args_t *a,
                     double *lon, double *lat, double *z, int32_t *st, int32_t *nn, int8_t *fr,
-                    double *Uo, double *Vo, double *Wo)
+                    double *Uo, double *Vo, double *Wo, int32_t *elem)
 {
     const double dt = a->dt;
     const double lon0 = *lon, lat0 = *lat, zz0 = *z;
     int node_i;
     double U1, V1, W1;
-    int ok = eval_site(m, f, a, 0, lon0, lat0, zz0, &node_i, &U1, &V1, &W1, 1);
+    /* elem_search walk (integrators.py:5020, :5415, :5792): each site's
+     * element walked from the previous site's, the carried one first */
+    const int e1 = a->walk ? or_walk_locate(m, lon0, lat0, *elem) : -1;
+    int e_last = e1;
+    int ok = eval_site(m, f, a, 0, lon0, lat0, zz0, &node_i, &U1, &V1, &W1, 1, e1);
     *Uo = U1;
     *Vo = V1;
     *Wo = W1;
@@ -1235,7 +1308,8 @@ static int det_step(const or_mesh_t *m, const or_fields_t *f, const or_step_args
             double z_mid = zz0 + W1 * 0.5 * dt;
             int nn_mid;
             double U2, V2, W2;
-            ok = eval_site(m, f, a, 1, lon_mid, lat_mid, z_mid, &nn_mid, &U2, &V2, &W2, 1);
+            if (a->walk) e_last = or_walk_locate(m, lon_mid, lat_mid, e_last);
+            ok = eval_site(m, f, a, 1, lon_mid, lat_mid, z_mid, &nn_mid, &U2, &V2, &W2, 1, e_last);
             if (ok) {
                 double z0m = z0_of(lat_mid);
                 lon_new = wrap_lon(lon0 + dt * U2 / z0m);
@@ -1251,7 +1325,8 @@ static int det_step(const or_mesh_t *m, const or_fields_t *f, const or_step_args
             double z2 = zz0 + 0.5 * dz1;
             int nn2, nn3, nn4;
             double U2, V2, W2, U3, V3, W3, U4, V4, W4;
-            ok = eval_site(m, f, a, 1, lon2, lat2, z2, &nn2, &U2, &V2, &W2, 1);
+            if (a->walk) e_last = or_walk_locate(m, lon2, lat2, e_last);
+            ok = eval_site(m, f, a, 1, lon2, lat2, z2, &nn2, &U2, &V2, &W2, 1, e_last);
             if (ok) {
                 double dlon2 = dt * U2 / z0_of(lat2);
                 double dlat2 = dt * V2 / SLM;
@@ -1259,7 +1334,8 @@ static int det_step(const or_mesh_t *m, const or_fields_t *f, const or_step_args
                 double lon3 = wrap_lon(lon0 + 0.5 * dlon2);
                 double lat3 = clamp_lat(lat0 + 0.5 * dlat2);
                 double z3 = zz0 + 0.5 * dz2;
-                ok = eval_site(m, f, a, 2, lon3, lat3, z3, &nn3, &U3, &V3, &W3, 1);
+                if (a->walk) e_last = or_walk_locate(m, lon3, lat3, e_last);
+                ok = eval_site(m, f, a, 2, lon3, lat3, z3, &nn3, &U3, &V3, &W3, 1, e_last);
                 if (ok) {
                     double dlon3 = dt * U3 / z0_of(lat3);
                     double dlat3 = dt * V3 / SLM;
@@ -1267,7 +1343,8 @@ static int det_step(const or_mesh_t *m, const or_fields_t *f, const or_step_args
                     double lon4 = wrap_lon(lon0 + dlon3);
                     double lat4 = clamp_lat(lat0 + dlat3);
                     double z4 = zz0 + dz3;
-                    ok = eval_site(m, f, a, 3, lon4, lat4, z4, &nn4, &U4, &V4, &W4, 1);
+                    if (a->walk) e_last = or_walk_locate(m, lon4, lat4, e_last);
+                    ok = eval_site(m, f, a, 3, lon4, lat4, z4, &nn4, &U4, &V4, &W4, 1, e_last);
                     if (ok) {
                         double dlon4 = dt * U4 / z0_of(lat4);
                         double dlat4 = dt * V4 / SLM;
@@ -1287,20 +1364,24 @@ static int det_step(const or_mesh_t *m, const or_fields_t *f, const or_step_args
         *st = ST_OUTSIDE;
         *fr = FR_INTERP;
         *nn = node_i;
+        if (a->walk) *elem = e1;
         return FR_INTERP;
     }
     int nn_new = or_site_anchor(m, lon_new, lat_new, a->exact);
-    int e = or_find_containing_local(m, lon_new, lat_new, nn_new);
+    int e = a->walk ? or_walk_locate(m, lon_new, lat_new, e_last)
+                    : or_find_containing_local(m, lon_new, lat_new, nn_new);
     if (e < 0) {
         *st = ST_OUTSIDE;
         *fr = FR_OOB;
         *nn = node_i;
+        if (a->walk) *elem = e1;
         return FR_OOB;
     }
     *lon = lon_new;
     *lat = lat_new;
     if (a->is3d) *z = clamp_z(m, z_new, nn_new);
     *nn = nn_new;
+    if (a->walk) *elem = e;
     return 0;
 }
 
@@ -1826,7 +1907,8 @@ static void az_one(const or_mesh_t *m, const or_step_args_t *a, uint64_t pid, in
  * coordinate outside [x0, x1) back with numpy's mod (Python % semantics),
  * keep it strictly inside, and re-anchor a folded particle with the batch
  * search (find_nearest_nodes). */
-static void periodic_one(const or_mesh_t *m, const or_step_args_t *a, double *lon, double *lat, int32_t *nn)
+static void periodic_one(const or_mesh_t *m, const or_step_args_t *a, double *lon, double *lat, int32_t *nn,
+                         int32_t *elem)
 {
     int moved = 0;
     if (a->periodic_lon && (*lon < a->px0 || *lon >= a->px1)) {
@@ -1841,7 +1923,10 @@ static void periodic_one(const or_mesh_t *m, const or_step_args_t *a, double *lo
         *lat = fo;
         moved = 1;
     }
-    if (moved) *nn = or_site_anchor(m, *lon, *lat, a->exact == 1 ? 1 : 2);
+    if (moved) {
+        *nn = or_site_anchor(m, *lon, *lat, a->exact == 1 ? 1 : 2);
+        *elem = -1; /* tracking.py:1143: the carried element is dropped */
+    }
 }
 
 /* tracking.py:199 _apply_beaching_hook + beaching.py:95 evaluate_beaching
@@ -1913,10 +1998,12 @@ static void beach_one(const or_mesh_t *m, const or_fields_t *f, const or_step_ar
  * beaching wet streak. */
 void or_step(const or_mesh_t *m, const or_fields_t *f, const or_step_args_t *a, int64_t P,
              double *lon, double *lat, double *z, int32_t *status, int32_t *nn, int8_t *fr,
-             double *U, double *V, double *W, const int64_t *pid, int32_t *ws)
+             double *U, double *V, double *W, const int64_t *pid, int32_t *ws, int32_t *elem)
 {
     for (int64_t i = 0; i < P; i++) {
         int32_t prev = status[i];
+        int32_t el_dummy = -1;
+        int32_t *el = elem ? &elem[i] : &el_dummy;
         double pre_lon = lon[i], pre_lat = lat[i], pre_z = z[i];
         int32_t pre_nn = nn[i];
         U[i] = 0.0;
@@ -1930,16 +2017,16 @@ void or_step(const or_mesh_t *m, const or_fields_t *f, const or_step_args_t *a, 
             if (a->az && a->az_const > 0.0 && status[i] == ST_ALIVE)
                 az_sigma_one(m, a, (uint64_t)pid[i], lon[i], lat[i], nn[i], &z[i]);
             if ((a->periodic_lon || a->periodic_lat) && status[i] == ST_ALIVE)
-                periodic_one(m, a, &lon[i], &lat[i], &nn[i]);
+                periodic_one(m, a, &lon[i], &lat[i], &nn[i], el);
         } else if (prev == ST_ALIVE) {
-            det_step(m, f, a, &lon[i], &lat[i], &z[i], &status[i], &nn[i], &fr[i], &U[i], &V[i], &W[i]);
+            det_step(m, f, a, &lon[i], &lat[i], &z[i], &status[i], &nn[i], &fr[i], &U[i], &V[i], &W[i], el);
             if (a->kick && status[i] == ST_ALIVE && !(a->kh_x == 0.0 && a->kh_y == 0.0))
                 kick_one(m, a, (uint64_t)pid[i], pre_lon, pre_lat, pre_z, pre_nn,
                          &lon[i], &lat[i], &z[i], &status[i], &nn[i], &fr[i]);
             if (a->az && a->is3d && a->az_const > 0.0 && status[i] == ST_ALIVE)
                 az_one(m, a, (uint64_t)pid[i], nn[i], &z[i]);
             if ((a->periodic_lon || a->periodic_lat) && status[i] == ST_ALIVE)
-                periodic_one(m, a, &lon[i], &lat[i], &nn[i]);
+                periodic_one(m, a, &lon[i], &lat[i], &nn[i], el);
         }
         if (a->beach) beach_one(m, f, a, (uint64_t)pid[i], prev, lon[i], lat[i], &status[i], &nn[i], &fr[i], &ws[i]);
     }
@@ -1969,8 +2056,9 @@ void or_step2d(const or_mesh_t *m, const or_fields_t *f, int order, const double
         fr_out[i] = 0;
         if (status[i] != ST_ALIVE) continue;
         double zz = 0.0, ww;
+        int32_t el = -1;
         det_step(m, f, &a, &lon_out[i], &lat_out[i], &zz, &status_out[i], &nn_out[i], &fr_out[i],
-                 &U_out[i], &V_out[i], &ww);
+                 &U_out[i], &V_out[i], &ww, &el);
     }
 }
 

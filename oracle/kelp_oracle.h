@@ -39,6 +39,7 @@ typedef struct {
     const uint8_t *elem_coastal;  /* (E,) coastal-element flags or NULL */
     const double *elem_center_lon, *elem_center_lat;  /* (E,) element centres */
     int32_t vertical_coord;      /* 0 z-level; 1 sigma: z_levels holds sigma_lev (1 .. 0) */
+    const int32_t *elem_neighbors;  /* (E, max_v) core/mesh.py:2824, or NULL */
 } or_mesh_t;
 
 /* Snapshot triple of one step: bracket pair (a, b) blended by alpha0 and
@@ -85,6 +86,7 @@ typedef struct {
      * bed distance */
     const double *sig_eta[5];
     double Dmin, z0b_min;
+    int32_t walk;      /* elem_search walk: containing elements by the edge walk */
 } or_step_args_t;
 
 #endif

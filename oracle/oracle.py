@@ -34,7 +34,7 @@ class OrMesh(ctypes.Structure):
         ("r_earth", c_double),
         ("n_levels", c_int32), ("z_levels", c_void_p), ("elem_coastal", c_void_p),
         ("elem_center_lon", c_void_p), ("elem_center_lat", c_void_p),
-        ("vertical_coord", c_int32),
+        ("vertical_coord", c_int32), ("elem_neighbors", c_void_p),
     ]
 
 
@@ -53,6 +53,7 @@ class OrStepArgs(ctypes.Structure):
         ("periodic_lon", c_int32), ("periodic_lat", c_int32),
         ("px0", c_double), ("px1", c_double), ("py0", c_double), ("py1", c_double),
         ("sig_eta", c_void_p * 5), ("Dmin", c_double), ("z0b_min", c_double),
+        ("walk", c_int32),
     ]
 
 
@@ -61,11 +62,12 @@ RESUSPENSION_CODES = {"never": 0, "auto": 1, "stochastic": 2, "auto_or_stochasti
 
 def step_args(order, dt, time_dir, is3d=False, w_method=2, exact=True, alphas=(0, 0, 0, 0),
               kick=None, beach=None, step_i=0, real_id=0, periodic_lon=None,
-              periodic_lat=None, sigma=None) -> OrStepArgs:
+              periodic_lat=None, sigma=None, walk=False) -> OrStepArgs:
     """``sigma`` (sigma meshes): dict(etas=[stage 0..3 surfaces, end
     surface] as (N,) arrays, Dmin, z0b_min); the arrays are kept alive by
     the returned structure."""
     a = OrStepArgs()
+    a.walk = int(bool(walk))
     if sigma is not None:
         etas = [np.ascontiguousarray(e, dtype=np.float64) for e in sigma["etas"]]
         a._sig_keep = etas
@@ -133,7 +135,7 @@ def lib(flavour: str = "glibc") -> ctypes.CDLL:
                             c_int, c_int64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
     L.or_step2d.restype = None
     L.or_step.argtypes = [POINTER(OrMesh), POINTER(OrFields), POINTER(OrStepArgs), c_int64,
-                          vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+                          vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
     L.or_step.restype = None
     L.or_hash5.argtypes = [ctypes.c_uint64] * 5
     L.or_hash5.restype = ctypes.c_uint64
@@ -181,7 +183,9 @@ class OracleMesh:
             z=_c(mesh.sigma if mesh.sigma.size else np.zeros(1), np.float64),
             coastal=(None if getattr(mesh, "elem_is_coastal", None) is None
                      else _c(mesh.elem_is_coastal, np.uint8)),
-            clon=_c(mesh.elem_center_lon, np.float64), clat=_c(mesh.elem_center_lat, np.float64))
+            clon=_c(mesh.elem_center_lon, np.float64), clat=_c(mesh.elem_center_lat, np.float64),
+            nbr=(None if getattr(mesh, "elem_neighbors", None) is None
+                 else _c(mesh.elem_neighbors, np.int32)))
         xyz = np.empty((mesh.n_nodes, 3), dtype=np.float64)
         self.L.or_node_xyz(mesh.n_nodes, k["lon"].ctypes.data, k["lat"].ctypes.data, xyz.ctypes.data)
         k["xyz"] = xyz
@@ -196,7 +200,8 @@ class OracleMesh:
             mesh.r_earth, int(mesh.sigma.size), k["z"].ctypes.data,
             None if k["coastal"] is None else k["coastal"].ctypes.data,
             k["clon"].ctypes.data, k["clat"].ctypes.data,
-            1 if getattr(mesh, "vertical_coord", "zlevel") == "sigma" else 0)
+            1 if getattr(mesh, "vertical_coord", "zlevel") == "sigma" else 0,
+            None if k["nbr"] is None else k["nbr"].ctypes.data)
 
     def nearest(self, lon, lat, exact=True) -> np.ndarray:
         lon = _c(lon, np.float64)
@@ -319,10 +324,23 @@ class OracleMesh:
                    u=np.zeros(P), v=np.zeros(P), w=np.zeros(P),
                    pid=np.array(state.get("pid", np.arange(P)), np.int64),
                    ws=np.array(state.get("ws", np.zeros(P)), np.int32))
+        if state.get("elem") is not None:
+            out["elem"] = np.array(state["elem"], np.int32)
         self.L.or_step(ctypes.byref(self.s), ctypes.byref(f), ctypes.byref(args), P,
                        *[out[k].ctypes.data for k in ("lon", "lat", "z", "status", "nn", "fr",
-                                                      "u", "v", "w", "pid", "ws")])
+                                                      "u", "v", "w", "pid", "ws")],
+                       out["elem"].ctypes.data if "elem" in out else None)
         return out
+
+    def walk_locate(self, lon, lat, start) -> np.ndarray:
+        """core/mesh.py:2631 find_containing_elements_walk (elements only)."""
+        lon = _c(lon, np.float64)
+        lat = _c(lat, np.float64)
+        start = _c(start, np.int32)
+        self.L.or_walk_locate.restype = ctypes.c_int
+        self.L.or_walk_locate.argtypes = [POINTER(OrMesh), c_double, c_double, ctypes.c_int]
+        return np.array([self.L.or_walk_locate(ctypes.byref(self.s), float(x), float(y), int(e))
+                         for x, y, e in zip(lon, lat, start)], np.int32)
 
 
 def math_probe(fn: int, x, y=None, flavour="glibc") -> np.ndarray:

@@ -198,3 +198,20 @@ def test_beaching_golden_through_a_three_slot_ring(gpu_ctx, monkeypatch):
     np.testing.assert_array_equal(rec["status"], gold["status"])
     for k, src in (("lon", "lon"), ("lat", "lat"), ("z", "z"), ("u", "vel_u"), ("w", "vel_w")):
         np.testing.assert_allclose(rec[src], gold[k], rtol=1e-9, atol=1e-9, err_msg=k)
+
+
+def test_beaching_golden_with_elem_search_walk(gpu_ctx):
+    """elem_search: walk (carried elements seeded by one global search,
+    tracking.py:2236) only changes which element holds a point on a shared
+    edge: the reference's local-search golden holds at its own bar."""
+    case = rc.beaching_case(1.0)
+    case["numerics"].elem_search = "walk"
+    gold = np.load(os.path.join(GOLD, "beaching_resuspension_rk2.npz"), allow_pickle=False)
+    rec = run_single_realization(
+        case["mesh"], case["ds"], get_reader("fesom2_arrays"), case["init"], case["mode"],
+        case["numerics"], start_time_idx=case["start_time_idx"], n_steps=case["n_steps"],
+        n_time_total=case["n_time_total"], file_times=case["file_times"],
+        file_dt=case["file_dt"], ctx=gpu_ctx).as_arrays()
+    np.testing.assert_array_equal(rec["status"], gold["status"])
+    for k, src in (("lon", "lon"), ("lat", "lat"), ("z", "z"), ("u", "vel_u"), ("w", "vel_w")):
+        np.testing.assert_allclose(rec[src], gold[k], rtol=1e-9, atol=1e-9, err_msg=k)

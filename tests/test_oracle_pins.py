@@ -349,3 +349,52 @@ def test_periodic_fold_is_python_modulo(meshes):
     np.testing.assert_array_equal(wrapped["lon"], want)
     # the folded particles carry the nearest node of their new position
     np.testing.assert_array_equal(wrapped["nn"][out], om.nearest(wrapped["lon"][out], wrapped["lat"][out]))
+
+
+# ---------------------------------------------------------------------------
+# edge walk (elem_search: walk)
+# ---------------------------------------------------------------------------
+
+def test_element_neighbours_are_symmetric_and_bound_the_mesh(meshes):
+    """core/mesh.py:2824: the neighbour across edge k of e has e across the
+    same edge; -1 marks exactly the edges owned by one element."""
+    from kelp_amd.mesh import build_elem_neighbors
+    mesh = meshes["lattice"]
+    nb = build_elem_neighbors(mesh)
+    en = mesh.elem_nodes
+    edges = {}
+    for e in range(mesh.n_elements):
+        v = [x for x in en[e] if x >= 0]
+        for k in range(len(v)):
+            a, b = v[k], v[(k + 1) % len(v)]
+            edges.setdefault((min(a, b), max(a, b)), []).append((e, k))
+    for owners in edges.values():
+        if len(owners) == 1:
+            e, k = owners[0]
+            assert nb[e, k] == -1
+        else:
+            (e1, k1), (e2, k2) = owners
+            assert nb[e1, k1] == e2 and nb[e2, k2] == e1
+
+
+@pytest.mark.parametrize("name", ["lattice", "sphere"])
+def test_walk_verdict_matches_brute_force(meshes, name):
+    """The walk (or its fallback) finds a containing element exactly when
+    one exists, from any starting element (far, near, invalid)."""
+    from kelp_amd.mesh import build_elem_neighbors
+    mesh = meshes[name]
+    build_elem_neighbors(mesh)
+    om = OracleMesh(mesh)
+    rng = np.random.default_rng(4)
+    if name == "lattice":
+        lon, lat = rng.uniform(-0.3, 10.3, 600), rng.uniform(29.7, 40.3, 600)
+    else:
+        lon = rng.uniform(-180, 180, 600)
+        lat = np.degrees(np.arcsin(rng.uniform(-1, 1, 600)))
+    start = rng.integers(-1, mesh.n_elements, lon.size)
+    e = om.walk_locate(lon, lat, start)
+    for i in range(lon.size):
+        hits = [k for k in range(mesh.n_elements) if om.element_contains(k, lon[i], lat[i])]
+        assert (e[i] >= 0) == bool(hits), i
+        if e[i] >= 0:
+            assert e[i] in hits
