@@ -220,6 +220,16 @@ int kelp_activate(kelp_ctx *ctx, const kelp_particles *p, const int64_t *release
 int kelp_field_upload_async(kelp_ctx *ctx, int slot, int dtype, int layout, int64_t n_planes,
                             const void *host_u, const void *host_v, const void *host_w, int64_t n_w_planes,
                             const double *host_eta);
+/* 4-snapshot PCHIP temporal reconstruction (numerics/temporal_reconstruct.py:
+ * 218 reconstruct_pchip, the per-value arithmetic of :184
+ * _pchip_value_scalar): slot `dst` receives the monotone cubic of slots
+ * src[0..3] (same dtype, layout and shape) at times[0..3] (strictly
+ * increasing), evaluated at tq in [times[1], times[2]] (within 1e-9 of the
+ * interval), every u, v and W value, stored as float64 (allocated at the
+ * first call, or when the shape changes).  A step reads a stage field from
+ * it like any snapshot.  Runs on the context stream. */
+int kelp_field_reconstruct(kelp_ctx *ctx, int dst, const int32_t *src, const double *times, double tq);
+
 /* Block the host until the last refill of `slot` has run. */
 int kelp_field_upload_wait(kelp_ctx *ctx, int slot);
 

@@ -102,6 +102,7 @@ SIGNATURES = {
     "kelp_field_upload_async": (c_int, [c_void_p, c_int, c_int, c_int, c_int64, c_void_p, c_void_p,
                                         c_void_p, c_int64, c_void_p]),
     "kelp_field_upload_wait": (c_int, [c_void_p, c_int]),
+    "kelp_field_reconstruct": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_double]),
     "kelp_sample_vars": (c_int, [c_void_p, POINTER(Particles), c_int, c_void_p, c_int, c_int,
                                  c_double, c_int, c_void_p]),
     "kelp_step": (c_int, [c_void_p, POINTER(Particles), POINTER(StepArgs)]),

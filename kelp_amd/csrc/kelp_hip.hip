@@ -346,6 +346,54 @@ static const FieldSlot* opt_slot(kelp_ctx* c, int k)
 // C ABI
 // ---------------------------------------------------------------------------
 
+// numerics/temporal_reconstruct.py:168 _fc_slope_scalar
+__device__ __forceinline__ double fc_slope(double d_prev, double d_next, double h_prev, double h_next)
+{
+    if (d_prev * d_next > 0.0) {
+        const double w1 = 2.0 * h_next + h_prev;
+        const double w2 = h_next + 2.0 * h_prev;
+        return (w1 + w2) / (w1 / d_prev + w2 / d_next);
+    }
+    return 0.0;
+}
+
+// temporal_reconstruct.py:184 _pchip_value_scalar: the monotone cubic through
+// four snapshot values at t0 < t1 < t2 < t3, evaluated at tq in [t1, t2]
+__device__ __forceinline__ double pchip4(double f0, double f1, double f2, double f3, double t0, double t1, double t2,
+                                         double t3, double tq)
+{
+    const double h0 = t1 - t0, h1 = t2 - t1, h2 = t3 - t2;
+    const double d0 = (f1 - f0) / h0;
+    const double d1 = (f2 - f1) / h1;
+    const double d2 = (f3 - f2) / h2;
+    const double m1 = fc_slope(d0, d1, h0, h1);
+    const double m2 = fc_slope(d1, d2, h1, h2);
+    const double s = (tq - t1) / h1;
+    const double s2 = s * s;
+    const double s3 = s2 * s;
+    const double h00 = 2.0 * s3 - 3.0 * s2 + 1.0;
+    const double h10 = s3 - 2.0 * s2 + s;
+    const double h01 = -2.0 * s3 + 3.0 * s2;
+    const double h11 = s3 - s2;
+    return h00 * f1 + h10 * h1 * m1 + h01 * f2 + h11 * h1 * m2;
+}
+
+struct PchipTimes {
+    double t0, t1, t2, t3, tq;
+};
+
+// every component of n records of width W (4: node 3-D, 2: node 2-D or
+// element) of four same-shaped snapshots into f64 records
+template <typename T, int W>
+__global__ void pchip_records_kernel(int64_t n, const T* __restrict__ a, const T* __restrict__ b,
+                                     const T* __restrict__ c, const T* __restrict__ d, PchipTimes t,
+                                     double* __restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n * W) return;
+    out[i] = pchip4((double)a[i], (double)b[i], (double)c[i], (double)d[i], t.t0, t.t1, t.t2, t.t3, t.tq);
+}
+
 extern "C" {
 
 const char* kelp_last_error(void) { return g_err; }
@@ -929,6 +977,89 @@ int kelp_field_upload_wait(kelp_ctx* c, int slot)
     if (!c || slot < 0 || slot >= kMaxSlots) return fail(KELP_EINVAL, "kelp_field_upload_wait: bad argument");
     FieldSlot& s = c->slots[slot];
     if (s.ready) HIP_TRY(hipEventSynchronize(s.ready));
+    return KELP_OK;
+}
+
+int kelp_field_reconstruct(kelp_ctx* c, int dst, const int32_t* src, const double* times, double tq)
+{
+    if (!c || !src || !times) return fail(KELP_EINVAL, "kelp_field_reconstruct: NULL argument");
+    if (dst < 0 || dst >= kMaxSlots) return fail(KELP_EINVAL, "kelp_field_reconstruct: bad destination slot");
+    for (int k = 0; k < 4; ++k) {
+        if (src[k] < 0 || src[k] >= kMaxSlots || c->slots[src[k]].dtype < 0)
+            return fail(KELP_ESTATE, "kelp_field_reconstruct: source slot %d is empty", src[k]);
+        if (src[k] == dst) return fail(KELP_EINVAL, "kelp_field_reconstruct: the destination is a source");
+    }
+    const FieldSlot& s0 = c->slots[src[0]];
+    for (int k = 1; k < 4; ++k) {
+        const FieldSlot& sk = c->slots[src[k]];
+        if (sk.dtype != s0.dtype || sk.layout != s0.layout || sk.n_planes != s0.n_planes ||
+            sk.n_w_planes != s0.n_w_planes || sk.is3d != s0.is3d)
+            return fail(KELP_EINVAL, "kelp_field_reconstruct: the four snapshots differ in dtype, layout or shape");
+    }
+    const double t0 = times[0], t1 = times[1], t2 = times[2], t3 = times[3];
+    if (!(t0 < t1 && t1 < t2 && t2 < t3))
+        return fail(KELP_EINVAL, "kelp_field_reconstruct: times must be strictly increasing");
+    const double tol = 1e-9 * (t2 - t1);
+    if (!(t1 - tol <= tq && tq <= t2 + tol))
+        return fail(KELP_EINVAL, "kelp_field_reconstruct: query time outside the central interval");
+    HIP_TRY(hipSetDevice(c->device));
+    for (int k = 0; k < 4; ++k) {
+        int rc = kelp_wait_slot(c, c->slots[src[k]]);
+        if (rc) return rc;
+    }
+    FieldSlot& d = c->slots[dst];
+    const int64_t N = c->m.n_nodes, E = c->m.n_elems;
+    const int64_t n_node_rec = s0.is3d ? N * s0.n_planes : (s0.layout == 0 ? N : 0);
+    const int wn = s0.is3d ? 4 : 2;
+    const int64_t n_elem_rec = s0.layout == 1 ? E * s0.n_planes : 0;
+    const bool fits = d.dtype == KELP_DT_F64 && d.layout == s0.layout && d.n_planes == s0.n_planes &&
+                      d.n_w_planes == s0.n_w_planes && d.is3d == s0.is3d;
+    if (!fits) {
+        if (d.pending) HIP_TRY(hipStreamSynchronize(c->copy_stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        free_slot(d);
+        hipError_t e = hipSuccess;
+        if (n_node_rec) e = hipMalloc(&d.rec, (size_t)n_node_rec * wn * sizeof(double));
+        if (e == hipSuccess && n_elem_rec) e = hipMalloc(&d.erec, (size_t)n_elem_rec * 2 * sizeof(double));
+        if (e != hipSuccess) {
+            free_slot(d);
+            return fail(KELP_ENOMEM, "kelp_field_reconstruct: %s", hipGetErrorString(e));
+        }
+        d.dtype = KELP_DT_F64;
+        d.layout = s0.layout;
+        d.n_planes = s0.n_planes;
+        d.n_w_planes = s0.n_w_planes;
+        d.is3d = s0.is3d;
+    }
+    const PchipTimes pt{t0, t1, t2, t3, tq};
+    const FieldSlot *a = &c->slots[src[0]], *b = &c->slots[src[1]], *cc = &c->slots[src[2]],
+                    *dd = &c->slots[src[3]];
+    auto launch = [&](int64_t n, int w, const void* pa, const void* pb, const void* pc, const void* pd,
+                      void* out) {
+        const unsigned g = grid_for(n * w, 256);
+        if (s0.dtype == KELP_DT_F32) {
+            if (w == 4)
+                pchip_records_kernel<float, 4><<<g, 256, 0, c->stream>>>(n, (const float*)pa, (const float*)pb,
+                                                                         (const float*)pc, (const float*)pd, pt,
+                                                                         (double*)out);
+            else
+                pchip_records_kernel<float, 2><<<g, 256, 0, c->stream>>>(n, (const float*)pa, (const float*)pb,
+                                                                         (const float*)pc, (const float*)pd, pt,
+                                                                         (double*)out);
+        } else {
+            if (w == 4)
+                pchip_records_kernel<double, 4><<<g, 256, 0, c->stream>>>(n, (const double*)pa, (const double*)pb,
+                                                                          (const double*)pc, (const double*)pd, pt,
+                                                                          (double*)out);
+            else
+                pchip_records_kernel<double, 2><<<g, 256, 0, c->stream>>>(n, (const double*)pa, (const double*)pb,
+                                                                          (const double*)pc, (const double*)pd, pt,
+                                                                          (double*)out);
+        }
+    };
+    if (n_node_rec) launch(n_node_rec, wn, a->rec, b->rec, cc->rec, dd->rec, d.rec);
+    if (n_elem_rec) launch(n_elem_rec, 2, a->erec, b->erec, cc->erec, dd->erec, d.erec);
+    HIP_TRY(hipGetLastError());
     return KELP_OK;
 }
 

@@ -333,6 +333,17 @@ class DeviceContext:
         keep[slot] = (u, v, w, e)
         self._slot_keys[slot] = key
 
+    def reconstruct(self, dst: int, src, times, tq: float) -> None:
+        """Slot ``dst`` <- the PCHIP reconstruction of slots ``src`` (4) at
+        ``times`` evaluated at ``tq`` (kelp_field_reconstruct); no wait."""
+        s = np.ascontiguousarray(src, np.int32)
+        t = np.ascontiguousarray(times, np.float64)
+        if s.size != 4 or t.size != 4:
+            raise ValueError("reconstruct needs four slots and four times")
+        check(self.lib.kelp_field_reconstruct(self.h, int(dst), s.ctypes.data, t.ctypes.data, float(tq)),
+              "kelp_field_reconstruct")
+        self._slot_keys[dst] = None
+
     # -- particles ----------------------------------------------------------
     def particles(self, lon, lat, z=None, status=None, nn=None, fr=None, pid=None,
                   ws=None, elem=None) -> DeviceParticles:

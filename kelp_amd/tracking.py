@@ -15,6 +15,7 @@ state to the host.  ``run_tracking(config)`` is the config-driven entry
 """
 from __future__ import annotations
 
+import warnings
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional
 
@@ -28,6 +29,10 @@ from .particles import ParticleState, ParticleStatus
 
 _C = {1: (0.0,), 2: (0.0, 0.5), 4: (0.0, 0.5, 0.5, 1.0)}
 N_SLOTS = 16
+#: slots of the library (kMaxSlots); with PCHIP reconstruction the last two
+#: hold the reconstructed stage fields and the snapshot ring uses the rest
+MAX_SLOTS = 16
+RECON_SLOTS = (MAX_SLOTS - 2, MAX_SLOTS - 1)
 SAMPLE_METHODS = {"nearest": 0, "leastsq": 1, "barycentric": 2, "idw": 3}
 SIGMA_W_METHODS = {"nearest": 0, "leastsq": 1, "barycentric": 2, "idw": 3}
 
@@ -202,8 +207,10 @@ class _SnapshotSlots:
     time index, least recently used evicted (the reference's _VelCache,
     tracking.py:584, on HBM)."""
 
-    def __init__(self, ctx, reader, ds, mesh, mode, numerics, sample_vars, need_eta):
+    def __init__(self, ctx, reader, ds, mesh, mode, numerics, sample_vars, need_eta,
+                 reserve_recon=False):
         self.ctx, self.reader, self.ds, self.mesh = ctx, reader, ds, mesh
+        self.n_slots = min(N_SLOTS, RECON_SLOTS[0]) if reserve_recon else N_SLOTS
         self.mode, self.numerics = mode, numerics
         self.sample_vars = sample_vars
         self.need_eta = need_eta
@@ -235,8 +242,8 @@ class _SnapshotSlots:
         self.in_use = set()
 
     def _victim(self) -> int:
-        if len(self.map) < N_SLOTS:
-            return min(set(range(N_SLOTS)) - set(self.map.values()))
+        if len(self.map) < self.n_slots:
+            return min(set(range(self.n_slots)) - set(self.map.values()))
         for old in self.order:
             if old not in self.in_use:
                 self.order.remove(old)
@@ -268,7 +275,7 @@ class _SnapshotSlots:
         same shape (the usual case once the run has filled its slots)."""
         if idx in self.map:
             return
-        full = len(self.map) >= N_SLOTS
+        full = len(self.map) >= self.n_slots
         if full and all(old in self.in_use for old in self.order):
             return          # every slot serves this step: no read-ahead
         slot = self._victim()
@@ -303,8 +310,8 @@ def _kick_args(numerics: NumericsConfig):
 def _check_path(numerics: NumericsConfig, mesh: Mesh, is3d: bool, uv_layout: str) -> None:
     if numerics.integrator not in ORDERS:
         raise NotImplementedError(f"device path: integrator {numerics.integrator!r}")
-    if numerics.temporal_reconstruction != "linear":
-        raise NotImplementedError("device path: pchip reconstruction")
+    if numerics.temporal_reconstruction not in ("linear", "pchip"):
+        raise ValueError(f"numerics.temporal_reconstruction: unknown {numerics.temporal_reconstruction!r}")
     if numerics.elem_search not in ("local", "walk"):
         raise ValueError(f"numerics.elem_search: unknown {numerics.elem_search!r}")
     if numerics.elem_search == "walk" and (uv_layout != "node" or mesh.vertical_coord == "sigma"):
@@ -322,6 +329,17 @@ def _check_path(numerics: NumericsConfig, mesh: Mesh, is3d: bool, uv_layout: str
             raise NotImplementedError(f"device path: w_interp={numerics.w_interp!r}")
     if numerics.beaching.enabled and numerics.beaching.dry_mask_var:
         raise NotImplementedError("device path: beaching dry_mask_var")
+
+
+def resolve_recon_stencil(bracket, n):
+    """tracking.py:3075 _resolve_recon_stencil: the four snapshots
+    ``(ia - 1, ia, ib, ib + 1)`` around a bracket and its query time, or None
+    on a degenerate bracket, across a cycle seam or at the first/last
+    interval of the records (the step then blends linearly)."""
+    ia, ib, _ta, _tb, q = bracket
+    if ib != ia + 1 or ia - 1 < 0 or ib + 1 >= n:
+        return None
+    return (ia - 1, ia, ib, ib + 1), q
 
 
 def _sigma_surfaces(use_tinterp, sim_time, sim_time_end, dt, direction, file_times, n,
@@ -429,6 +447,16 @@ def run_single_realization(mesh: Mesh, ds, reader, init_state: ParticleState, mo
     snap = snapshot_band(dt) if rk_snap else 5e-10 * max(dt, 1.0)
     exact = 1 if exact_nearest_on(mesh) else 2
     sig3d = is3d and mesh.vertical_coord == "sigma"
+    # tracking.py:2159: PCHIP in time, on multistage steps of a time-
+    # interpolated run; each stage reads its own reconstructed field
+    recon_on = use_tinterp and multistage and numerics.temporal_reconstruction == "pchip"
+    if recon_on:
+        if order != 2:
+            raise NotImplementedError("device path: pchip reconstruction with rk4 (rk2 only)")
+        if sig3d:
+            raise NotImplementedError("device path: pchip reconstruction on a sigma mesh")
+        if numerics.beaching.enabled and numerics.beaching.use_eta:
+            raise NotImplementedError("device path: pchip reconstruction with beaching use_eta")
     w_method = ((SIGMA_W_METHODS if sig3d else W_METHODS)[numerics.w_interp] if is3d else 2)
     kick = _kick_args(numerics)
     beach = _beach_args(numerics, dt)
@@ -451,7 +479,8 @@ def run_single_realization(mesh: Mesh, ds, reader, init_state: ParticleState, mo
     if ctx.mesh is not mesh:
         ctx.upload_mesh(mesh)
     slots = _SnapshotSlots(ctx, reader, ds, mesh, mode, numerics, sample_vars,
-                           need_eta=bool(beach and beach["use_eta"]) or sig3d)
+                           need_eta=bool(beach and beach["use_eta"]) or sig3d,
+                           reserve_recon=recon_on)
     P = init_state.n_particles
     pid = particle_offset + np.arange(P, dtype=np.int64)
     ws = (np.zeros(P, np.int32) if start_wet_streak is None
@@ -502,6 +531,7 @@ def run_single_realization(mesh: Mesh, ds, reader, init_state: ParticleState, mo
                          release_step=release_step, particle_offset=particle_offset)
 
     prefetch = int(getattr(numerics, "prefetch_size", 1) or 0) > 0
+    recon_warned = False
     for step_i in range(n_steps):
         g = start_step_i + step_i
         slots.begin_step()
@@ -543,8 +573,15 @@ def run_single_realization(mesh: Mesh, ds, reader, init_state: ParticleState, mo
                                               n_time_total, on_exhausted)
         elif dtn is not None and dtn < dt <= dtn + snap:
             dtn = dt
+        stencil = resolve_recon_stencil(bracket, n_time_total) if recon_on else None
+        if recon_on and stencil is None and not recon_warned:
+            warnings.warn("temporal_reconstruction='pchip': fewer than 4 snapshots at the range "
+                          "edge; this step uses linear time interpolation. Further occurrences in "
+                          "this realization suppressed.", RuntimeWarning, stacklevel=2)
+            recon_warned = True
         if dtn is not None and step_len > dtn:
             pay.it, dtn = None, None           # steady RK for this step
+            stencil = None                     # tracking.py:2560: no PCHIP either
         sa = slots.slot(pay.ia)
         sb = slots.slot(pay.ib) if pay.ib is not None else -1
         st = slots.slot(pay.it) if pay.it is not None else -1
@@ -559,14 +596,27 @@ def run_single_realization(mesh: Mesh, ds, reader, init_state: ParticleState, mo
             sigma.update(Dmin=numerics.Dmin, z0b_min=numerics.z0b_min)
         if resort_every > 0 and g % resort_every == 0:
             ctx.sort(p)
-        ctx.step(p, order, sa, step_len, float(direction), slot_b=sb, slot_t=st,
-                 alpha0=pay.alpha0, alphas=alphas, exact_nearest=exact, w_method=w_method,
+        step_a, step_b, step_t, step_a0 = sa, sb, st, pay.alpha0
+        if stencil is not None:
+            # temporal_reconstruct.py:298 make_stage_vels_recon: stage c at
+            # q + direction c dt, reconstructed on the device from the four
+            # resident snapshots; alpha 0 / 1 select the two stage fields
+            idx4, q = stencil
+            src = [slots.slot(i) for i in idx4]
+            t4 = [float(file_times[i]) for i in idx4]
+            for dst, c in zip(RECON_SLOTS, _C[2]):
+                ctx.reconstruct(dst, src, t4, float(q) + float(direction) * c * step_len)
+            step_a, step_b, step_t, step_a0 = RECON_SLOTS[0], -1, RECON_SLOTS[1], 0.0
+            alphas = [0.0, 1.0, 0.0, 0.0]
+        ctx.step(p, order, step_a, step_len, float(direction), slot_b=step_b, slot_t=step_t,
+                 alpha0=step_a0, alphas=alphas, exact_nearest=exact, w_method=w_method,
                  kick=kick, beach=beach, step_i=g, real_id=realization_id, reuse_anchor=True,
                  sigma=sigma, walk=walk, **periodic)
         if prefetch and n_time_total and n_time_total > 1:
             # the snapshot after this step's bracket streams in while it runs
-            ahead = max(i for i in (pay.ia, pay.ib, pay.it) if i is not None) if direction > 0 else \
-                min(i for i in (pay.ia, pay.ib, pay.it) if i is not None)
+            held = [i for i in (pay.ia, pay.ib, pay.it) if i is not None]
+            held += list(stencil[0]) if stencil is not None else []
+            ahead = max(held) if direction > 0 else min(held)
             nxt = ahead + direction
             if on_exhausted == "cycle":
                 nxt %= n_time_total

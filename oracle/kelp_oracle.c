@@ -2363,3 +2363,37 @@ void or_sample(const or_mesh_t *m, const double *fa, const double *fb, double al
         out[i] = combine_horizontal(m, &s, z ? z[i] : 0.0, buf, count, vals, lon[i], lat[i], node, method);
     }
 }
+
+/* numerics/temporal_reconstruct.py:168 _fc_slope_scalar */
+static double fc_slope(double d_prev, double d_next, double h_prev, double h_next)
+{
+    if (d_prev * d_next > 0.0) {
+        double w1 = 2.0 * h_next + h_prev;
+        double w2 = h_next + 2.0 * h_prev;
+        return (w1 + w2) / (w1 / d_prev + w2 / d_next);
+    }
+    return 0.0;
+}
+
+/* temporal_reconstruct.py:184 _pchip_value_scalar over n values of four
+ * snapshots (float64 inputs) at t0 < t1 < t2 < t3, evaluated at tq */
+void or_pchip4(int64_t n, const double *f0, const double *f1, const double *f2, const double *f3, double t0,
+               double t1, double t2, double t3, double tq, double *out)
+{
+    for (int64_t i = 0; i < n; i++) {
+        double h0 = t1 - t0, h1 = t2 - t1, h2 = t3 - t2;
+        double d0 = (f1[i] - f0[i]) / h0;
+        double d1 = (f2[i] - f1[i]) / h1;
+        double d2 = (f3[i] - f2[i]) / h2;
+        double m1 = fc_slope(d0, d1, h0, h1);
+        double m2 = fc_slope(d1, d2, h1, h2);
+        double s = (tq - t1) / h1;
+        double s2 = s * s;
+        double s3 = s2 * s;
+        double h00 = 2.0 * s3 - 3.0 * s2 + 1.0;
+        double h10 = s3 - 2.0 * s2 + s;
+        double h01 = -2.0 * s3 + 3.0 * s2;
+        double h11 = s3 - s2;
+        out[i] = h00 * f1[i] + h10 * h1 * m1 + h01 * f2[i] + h11 * h1 * m2;
+    }
+}

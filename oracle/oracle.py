@@ -384,3 +384,16 @@ def sigma_stage_etas(order, steady, eta_start, eta_end, a=None, b=None, t=None, 
         else:
             out[k] = stage_field(a, b, t, alpha0, alphas[k])
     return out
+
+
+def pchip4(f0, f1, f2, f3, times, tq, flavour="glibc"):
+    """temporal_reconstruct.py:184 _pchip_value_scalar over whole arrays (the
+    C oracle's or_pchip4); float64 result of f1's shape."""
+    L = lib(flavour)
+    arrs = [np.ascontiguousarray(f, np.float64) for f in (f0, f1, f2, f3)]
+    out = np.empty(arrs[1].shape, np.float64)
+    L.or_pchip4.argtypes = [c_int64] + [ctypes.c_void_p] * 4 + [c_double] * 5 + [ctypes.c_void_p]
+    L.or_pchip4.restype = None
+    L.or_pchip4(out.size, *[a.ctypes.data for a in arrs], *[float(t) for t in times], float(tq),
+                out.ctypes.data)
+    return out

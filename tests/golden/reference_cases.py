@@ -113,6 +113,31 @@ def run_case(name, direction, stepper):
     return {k: np.stack(v) for k, v in rows.items()}
 
 
+def run_pchip_case(direction, stepper):
+    """tests/_gen_golden_3d.py:305 (fesom2_node_rk2_fused_pchip): four node
+    snapshots at 0, T, 2T, 3T (T = the run's span), the run inside the
+    central interval; every RK2 stage reads the monotone-cubic field at
+    tq = q + direction c dt (temporal_reconstruct.py:298).  Drives
+    ``stepper(mesh, state, snaps, times, tqs, step_i)``."""
+    mesh = dual_mesh()
+    snaps = [field(mesh, s) for s in (11, 12, 13, 14)]
+    T = NODE_STEPS * NODE_DT
+    times = [0.0, T, 2.0 * T, 3.0 * T]
+    t0 = T if direction > 0 else 2.0 * T
+    lon, lat, z = seeds()
+    st = dict(lon=lon, lat=lat, z=z, status=np.ones(lon.size, np.int32),
+              nn=np.full(lon.size, -1, np.int32), fr=np.zeros(lon.size, np.int8))
+    rows = {k: [] for k in ("lon", "lat", "z", "status", "failure_reason", "u", "v", "w")}
+    for i in range(NODE_STEPS):
+        q = t0 + direction * i * NODE_DT
+        tqs = [float(q) + float(direction) * c * NODE_DT for c in (0.0, 0.5)]
+        st = stepper(mesh, st, snaps, times, tqs, i)
+        for k, src in (("lon", "lon"), ("lat", "lat"), ("z", "z"), ("status", "status"),
+                       ("failure_reason", "fr"), ("u", "u"), ("v", "v"), ("w", "w")):
+            rows[k].append(np.array(st[src], copy=True))
+    return {k: np.stack(v) for k, v in rows.items()}
+
+
 # ---------------------------------------------------------------------------
 # element-layout cases: leastsq u/v at element centres, leastsq W
 # (tests/_gen_golden_3d.py:154-219 with tests/test_3d_analytical.py:39-101)

@@ -102,3 +102,48 @@ def test_device_matches_reference_golden_element_layout(gpu_ctx, name, direction
         np.testing.assert_array_equal(got[k], gold[k], err_msg=k)
     for k in ("lon", "lat", "z", "u", "v", "w"):
         np.testing.assert_allclose(got[k], gold[k], rtol=1e-9, atol=1e-9, err_msg=k)
+
+
+@pytest.mark.parametrize("direction", [1.0, -1.0])
+def test_device_matches_reference_golden_pchip(gpu_ctx, direction):
+    """4-snapshot PCHIP in time: the four snapshots resident in slots 3-6,
+    each RK2 stage's field reconstructed on the device
+    (kelp_field_reconstruct) into slots 0 and 2, then the step.  Also
+    bit-identical to the kmath oracle doing the same in float64."""
+    from oracle.oracle import OracleMesh, pchip4, step_args
+    gold = np.load(os.path.join(GOLD, "fesom2_node_rk2_fused_pchip" + ("_bwd" if direction < 0 else "")
+                                + ".npz"), allow_pickle=False)
+    state = {}
+
+    def stepper(mesh, st, snaps, times, tqs, i):
+        if state.get("mesh") is not mesh:
+            gpu_ctx.upload_mesh(mesh)
+            for k, s in enumerate(snaps):
+                gpu_ctx.upload_field(3 + k, s[0], s[1], s[2])
+            state["mesh"] = mesh
+            state["om"] = OracleMesh(mesh, "kmath")
+            state["ost"] = dict(st)
+        for dst, tq in zip((0, 2), tqs):
+            gpu_ctx.reconstruct(dst, [3, 4, 5, 6], times, tq)
+        p = gpu_ctx.particles(st["lon"], st["lat"], st["z"], status=st["status"], nn=st["nn"], fr=st["fr"])
+        gpu_ctx.step(p, 2, 0, rc.NODE_DT, direction, slot_b=-1, slot_t=2, alpha0=0.0, alphas=[0.0, 1.0],
+                     exact_nearest=2, w_method=2, step_i=i)
+        out = p.to_host()
+        R = [[pchip4(*(s[c] for s in snaps), times, tq, "kmath") for c in range(3)] for tq in tqs]
+        fields = dict(u_a=R[0][0], v_a=R[0][1], w_a=R[0][2], u_t=R[1][0], v_t=R[1][1], w_t=R[1][2],
+                      alpha0=0.0)
+        want = state["om"].step(state["ost"], fields, step_args(2, rc.NODE_DT, direction, is3d=True,
+                                                                w_method=2, exact=2, alphas=(0.0, 1.0),
+                                                                step_i=i))
+        state["ost"] = want
+        for k in ("status", "fr", "nn"):
+            np.testing.assert_array_equal(out[k], want[k], err_msg=f"{k} step {i}")
+        for k in ("lon", "lat", "z", "u", "v", "w"):
+            np.testing.assert_array_equal(out[k].view(np.int64), want[k].view(np.int64), err_msg=f"{k} step {i}")
+        return out
+
+    got = rc.run_pchip_case(direction, stepper)
+    for k in ("status", "failure_reason"):
+        np.testing.assert_array_equal(got[k], gold[k], err_msg=k)
+    for k in ("lon", "lat", "z", "u", "v", "w"):
+        np.testing.assert_allclose(got[k], gold[k], rtol=1e-9, atol=1e-9, err_msg=k)

@@ -215,3 +215,65 @@ def test_beaching_golden_with_elem_search_walk(gpu_ctx):
     np.testing.assert_array_equal(rec["status"], gold["status"])
     for k, src in (("lon", "lon"), ("lat", "lat"), ("z", "z"), ("u", "vel_u"), ("w", "vel_w")):
         np.testing.assert_allclose(rec[src], gold[k], rtol=1e-9, atol=1e-9, err_msg=k)
+
+
+def _pchip_runs(gpu_ctx, direction, linear_data):
+    """The sampling workload run twice, with temporal_reconstruction pchip
+    and linear; with ``linear_data`` every variable is replaced by its
+    record-0 value plus a per-record increment (linear in time)."""
+    import copy
+    import warnings
+    from kelp_amd.readers import ArrayDataset
+    case = rc.sampling_case(direction)
+    if linear_data:
+        ds = case["ds"]
+        nt = ds.time.size
+        t = np.arange(nt, dtype=np.float64)
+        rng = np.random.default_rng(5)
+        dv = {}
+        for name, (dims, arr) in ds.vars.items():
+            if dims and dims[0] == "time":
+                step = 0.05 * np.abs(arr[0]).max() * rng.standard_normal(arr.shape[1:])
+                arr = arr[0][None] + t.reshape((nt,) + (1,) * (arr.ndim - 1)) * step[None]
+            dv[name] = (dims, np.ascontiguousarray(arr))
+        case["ds"] = ArrayDataset(dv, time=ds.time, attrs=ds.attrs)
+    out = {}
+    for recon in ("pchip", "linear"):
+        num = copy.deepcopy(case["numerics"])
+        num.temporal_reconstruction = recon
+        with warnings.catch_warnings(record=True) as caught:
+            warnings.simplefilter("always")
+            out[recon] = run_single_realization(
+                case["mesh"], case["ds"], get_reader("fesom2_arrays"), case["init"], case["mode"],
+                num, start_time_idx=case["start_time_idx"], n_steps=case["n_steps"],
+                n_time_total=case["n_time_total"], file_times=case["file_times"],
+                file_dt=case["file_dt"], sample_vars=case["sample_vars"], ctx=gpu_ctx).as_arrays()
+        out[recon + "_warned"] = any("fewer than 4" in str(w.message) for w in caught)
+    return out
+
+
+@pytest.mark.parametrize("direction", [1.0, -1.0])
+def test_pchip_run_reduces_to_linear_on_linear_data(gpu_ctx, direction):
+    """On data linear in time the monotone cubic through four records is the
+    straight line, so every stage field the loop reconstructs (its four
+    snapshots, their times, the stage times q + direction c dt) must equal
+    the linear blend's: the PCHIP run equals the linear run to rounding."""
+    r = _pchip_runs(gpu_ctx, direction, linear_data=True)
+    np.testing.assert_array_equal(r["pchip"]["status"], r["linear"]["status"])
+    for k in ("lon", "lat", "z", "vel_u", "vel_v", "vel_w"):
+        np.testing.assert_allclose(r["pchip"][k], r["linear"][k], rtol=0, atol=1e-10, err_msg=k)
+
+
+@pytest.mark.parametrize("direction", [1.0, -1.0])
+def test_pchip_run_reconstructs_interior_steps(gpu_ctx, direction):
+    """On the sampling workload (u and w vary as sin t, cos t/2) the steps of
+    the first record interval have no four-snapshot stencil and blend
+    linearly (bit-identical to the linear run, one warning); the steps of
+    the two interior intervals move differently under PCHIP."""
+    r = _pchip_runs(gpu_ctx, direction, linear_data=False)
+    assert r["pchip_warned"] and not r["linear_warned"]
+    a, b = r["pchip"], r["linear"]
+    for k in ("lon", "lat", "z", "vel_u", "vel_w"):
+        np.testing.assert_array_equal(a[k][:, :2], b[k][:, :2], err_msg=k)
+    assert np.nanmax(np.abs(a["lon"][:, 2:6] - b["lon"][:, 2:6])) > 1e-7
+    assert np.nanmax(np.abs(a["vel_u"][:, 2:6] - b["vel_u"][:, 2:6])) > 1e-6

@@ -84,3 +84,31 @@ def test_oracle_matches_reference_golden_element_layout(name, direction, flavour
         np.testing.assert_array_equal(got[k], gold[k], err_msg=k)
     for k in ("lon", "lat", "z", "u", "v", "w"):
         np.testing.assert_allclose(got[k], gold[k], rtol=1e-9, atol=1e-9, err_msg=k)
+
+
+@pytest.mark.parametrize("direction", [1.0, -1.0])
+@pytest.mark.parametrize("flavour", ["glibc", "kmath"])
+def test_oracle_matches_reference_golden_pchip(direction, flavour):
+    """4-snapshot PCHIP in time (the fused 3-D barycentric RK2 PCHIP kernel,
+    integrators.py:3859): the stage fields reconstructed by the oracle's
+    _pchip_value_scalar restatement, then the RK2 step on them."""
+    from oracle.oracle import pchip4
+    gold = np.load(os.path.join(GOLD, "fesom2_node_rk2_fused_pchip" + ("_bwd" if direction < 0 else "")
+                                + ".npz"), allow_pickle=False)
+
+    def stepper(mesh, st, snaps, times, tqs, i):
+        om = _MESHES.get((id(mesh), flavour))
+        if om is None:
+            om = _MESHES[(id(mesh), flavour)] = OracleMesh(mesh, flavour)
+        R = [[pchip4(*(s[c] for s in snaps), times, tq, flavour) for c in range(3)] for tq in tqs]
+        fields = dict(u_a=R[0][0], v_a=R[0][1], w_a=R[0][2], u_t=R[1][0], v_t=R[1][1], w_t=R[1][2],
+                      alpha0=0.0)
+        args = step_args(2, rc.NODE_DT, direction, is3d=True, w_method=2, exact=2,
+                         alphas=(0.0, 1.0), step_i=i)
+        return om.step(st, fields, args)
+
+    got = rc.run_pchip_case(direction, stepper)
+    for k in ("status", "failure_reason"):
+        np.testing.assert_array_equal(got[k], gold[k], err_msg=k)
+    for k in ("lon", "lat", "z", "u", "v", "w"):
+        np.testing.assert_allclose(got[k], gold[k], rtol=1e-9, atol=1e-9, err_msg=k)

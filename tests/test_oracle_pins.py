@@ -398,3 +398,29 @@ def test_walk_verdict_matches_brute_force(meshes, name):
         assert (e[i] >= 0) == bool(hits), i
         if e[i] >= 0:
             assert e[i] in hits
+
+
+@pytest.mark.parametrize("flavour", ["glibc", "kmath"])
+def test_pchip_is_scipy_pchip_on_the_central_interval(flavour):
+    """temporal_reconstruct.py:184 on [t1, t2] against scipy's
+    PchipInterpolator (the same Fritsch-Carlson interior slopes; its end
+    slopes do not enter the central interval), on uneven record spacing,
+    monotone and non-monotone data, flat and sign-changing differences;
+    and the interpolation conditions at t1 and t2 exactly."""
+    from scipy.interpolate import PchipInterpolator
+    from oracle.oracle import pchip4
+    rng = np.random.default_rng(3)
+    times = np.array([0.0, 2100.0, 3600.0, 8000.0])
+    f = rng.standard_normal((4, 400))
+    f[:, :50] = np.cumsum(np.abs(f[:, :50]), axis=0)      # monotone
+    f[1, 50:60] = f[0, 50:60]                             # flat left difference
+    f[2, 60:70] = f[1, 60:70]                             # flat central difference
+    for tq in (2100.0, 2500.0, 2850.5, 3599.0, 3600.0):
+        got = pchip4(f[0], f[1], f[2], f[3], times, tq, flavour)
+        want = PchipInterpolator(times, f, axis=0)(tq)
+        np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12)
+    np.testing.assert_array_equal(pchip4(f[0], f[1], f[2], f[3], times, 2100.0, flavour), f[1])
+    np.testing.assert_array_equal(pchip4(f[0], f[1], f[2], f[3], times, 3600.0, flavour), f[2])
+    # monotone data stay within the bracket values
+    mid = pchip4(f[0, :50], f[1, :50], f[2, :50], f[3, :50], times, 2900.0, flavour)
+    assert np.all((mid >= f[1, :50]) & (mid <= f[2, :50]))

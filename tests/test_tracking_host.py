@@ -134,3 +134,24 @@ def test_record_clock_origin_of_a_resumed_run():
     assert T.record_clock_origin(True, 9000.0, 0, ft, 6, "stop", 1800.0, 1, 4) == 1800.0
     # cycle: unwrapped index past the axis
     assert T.index_to_sim_time(8, ft, 6, "cycle") == 6 * 3600.0 + 2 * 3600.0
+
+
+def test_recon_stencil_rule():
+    """tracking.py:3075: four consecutive interior records around the
+    bracket, else None (first/last interval, on a record, cycle seam)."""
+    assert T.resolve_recon_stencil((1, 2, 3600.0, 7200.0, 5000.0), 5) == ((0, 1, 2, 3), 5000.0)
+    assert T.resolve_recon_stencil((0, 1, 0.0, 3600.0, 10.0), 5) is None
+    assert T.resolve_recon_stencil((3, 4, 0.0, 3600.0, 10.0), 5) is None
+    assert T.resolve_recon_stencil((2, 2, 7200.0, 7200.0, 7200.0), 5) is None
+    assert T.resolve_recon_stencil((4, 0, 14400.0, 18000.0, 15000.0), 5) is None
+
+
+def test_pchip_config_checks():
+    mesh = T.Mesh.__new__(T.Mesh)
+    mesh.vertical_coord = "zlevel"
+    num = C.NumericsConfig(integrator="rk2", uv_interp="barycentric", w_interp="nearest",
+                           temporal_reconstruction="pchip")
+    T._check_path(num, mesh, True, "node")
+    num.temporal_reconstruction = "cubic"
+    with pytest.raises(ValueError, match="temporal_reconstruction"):
+        T._check_path(num, mesh, True, "node")
