@@ -138,6 +138,15 @@ typedef struct {
     int32_t walk;                  /* elem_search walk (core/mesh.py:2489): every site's
                                       containing element by the edge walk from the carried
                                       one, the domain verdict too; node-layout u/v, z-level */
+    /* Az from a field (integrators.py:842 with vel.az, numerics/diffusion.py:427
+     * _loop_az_kick_zlevel): with `az` and az_field != 0 the kick takes Az and
+     * dAz/dz on the nearest node's column of slot az_slot_a's Az, blended with
+     * az_slot_b's by az_alpha0 when az_slot_b >= 0 (the step payload's bracket,
+     * core/io.py:68), instead of az_const; z-level 3-D meshes, both slots filled
+     * by kelp_field_upload_az */
+    int32_t az_field;
+    int32_t az_slot_a, az_slot_b;
+    double az_alpha0;
 } kelp_step_args;
 
 const char *kelp_last_error(void);
@@ -229,6 +238,13 @@ int kelp_field_upload_async(kelp_ctx *ctx, int slot, int dtype, int layout, int6
  * first call, or when the shape changes).  A step reads a stage field from
  * it like any snapshot.  Runs on the context stream. */
 int kelp_field_reconstruct(kelp_ctx *ctx, int dst, const int32_t *src, const double *times, double tq);
+
+/* The vertical diffusivity of a snapshot (readers/fesom2.py:1596-1636: the
+ * `var_az` record, float64, NaN as 0; core/io.py:246 VelocityField.az): K
+ * interface planes of N values in mesh node order, stored one column per
+ * node.  Re-uploading the slot's snapshot (kelp_field_upload*,
+ * kelp_field_reconstruct) drops it. */
+int kelp_field_upload_az(kelp_ctx *ctx, int slot, int64_t n_planes, const double *host_az);
 
 /* Block the host until the last refill of `slot` has run. */
 int kelp_field_upload_wait(kelp_ctx *ctx, int slot);

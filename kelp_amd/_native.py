@@ -71,6 +71,7 @@ class StepArgs(ctypes.Structure):
         ("sig_end_a", c_int32), ("sig_end_b", c_int32), ("sig_end_t", c_int32),
         ("sig_end_alpha0", c_double), ("sig_end_alpha", c_double),
         ("Dmin", c_double), ("z0b_min", c_double), ("walk", c_int32),
+        ("az_field", c_int32), ("az_slot_a", c_int32), ("az_slot_b", c_int32), ("az_alpha0", c_double),
     ]
 
 
@@ -103,6 +104,7 @@ SIGNATURES = {
                                         c_void_p, c_int64, c_void_p]),
     "kelp_field_upload_wait": (c_int, [c_void_p, c_int]),
     "kelp_field_reconstruct": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_double]),
+    "kelp_field_upload_az": (c_int, [c_void_p, c_int, c_int64, c_void_p]),
     "kelp_sample_vars": (c_int, [c_void_p, POINTER(Particles), c_int, c_void_p, c_int, c_int,
                                  c_double, c_int, c_void_p]),
     "kelp_step": (c_int, [c_void_p, POINTER(Particles), POINTER(StepArgs)]),

@@ -32,6 +32,23 @@ constexpr double kSLM = 111120.0;                  // METERS_PER_DEG_LAT, core/g
 constexpr double kREarthStd = 6371000.0;           // R_EARTH_STANDARD, core/geometry.py:30
 constexpr double kBaryEps = -1e-6;                 // _BARY_EPS, core/mesh.py:33
 constexpr double kAreaTol = 1e-10;                 // _AREA_TOL, core/mesh.py:32
+
+// MI355X: 8 XCDs, each with its own L2.  The dispatcher hands workgroup b to
+// XCD b % 8, so consecutive workgroups -- neighbouring particles once they
+// are sorted by grid-hash cell -- would land in eight different L2s.
+constexpr int kXcds = 8;
+
+// Global thread index with the workgroups regrouped by XCD: the blocks XCD x
+// receives (x, x + 8, x + 16, ...) take the x-th contiguous run of the
+// block range, so each L2 serves one stretch of sorted particles.  A
+// bijection of [0, gridDim.x) onto itself for any grid size.
+__device__ __forceinline__ int64_t xcd_thread_index()
+{
+    const int64_t nb = gridDim.x, b = blockIdx.x;
+    const int64_t per = nb / kXcds, rem = nb % kXcds, x = b % kXcds;
+    const int64_t lb = x * per + (x < rem ? x : rem) + b / kXcds;
+    return lb * (int64_t)blockDim.x + threadIdx.x;
+}
 constexpr int kShortRowCells = 32;                 // _GH_EXACT_SHORT_ROW_CELLS, core/mesh.py:1608
 
 constexpr int32_t kAlive = 1;

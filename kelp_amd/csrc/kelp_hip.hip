@@ -84,6 +84,7 @@ static void free_slot(FieldSlot& s)
     if (s.rec) (void)hipFree(s.rec);
     if (s.erec) (void)hipFree(s.erec);
     if (s.eta) (void)hipFree(s.eta);
+    if (s.az) (void)hipFree(s.az);
     for (auto& f : s.scalars) kelp_free_scalar(f);
     for (void* p : s.stage)
         if (p) (void)hipFree(p);
@@ -969,6 +970,7 @@ int kelp_field_upload_async(kelp_ctx* c, int slot, int dtype, int layout, int64_
     }
     HIP_TRY(hipEventRecord(s.ready, c->copy_stream));
     s.pending = true;
+    s.az_k = 0;  // the new snapshot brings its own Az, if any
     return KELP_OK;
 }
 
@@ -1031,6 +1033,7 @@ int kelp_field_reconstruct(kelp_ctx* c, int dst, const int32_t* src, const doubl
         d.n_w_planes = s0.n_w_planes;
         d.is3d = s0.is3d;
     }
+    d.az_k = 0;
     const PchipTimes pt{t0, t1, t2, t3, tq};
     const FieldSlot *a = &c->slots[src[0]], *b = &c->slots[src[1]], *cc = &c->slots[src[2]],
                     *dd = &c->slots[src[3]];
@@ -1180,14 +1183,25 @@ int kelp_step(kelp_ctx* c, const kelp_particles* p, const kelp_step_args* a)
             return fail(KELP_EINVAL, "kelp_step: 3-D field shapes (%lld, %lld) do not match K=%d",
                         (long long)sa.n_planes, (long long)sa.n_w_planes, K);
         if (!p->z) return fail(KELP_EINVAL, "kelp_step: 3-D step needs z");
-    } else if (a->az && a->az_const > 0.0) {
+    } else if (a->az && (a->az_const > 0.0 || a->az_field)) {
         return fail(KELP_EINVAL, "kelp_step: the Az kick needs a 3-D run (integrators.py:893)");
     } else if (sa.n_planes != 1) {
         return fail(KELP_EINVAL, "kelp_step: a layered u/v field needs W (3-D)");
     }
+    const bool az_field = a->az && a->az_field;
+    if (az_field) {
+        // integrators.py:902-906: a field source on sigma meshes is refused
+        if (sigma) return fail(KELP_EINVAL, "kelp_step: an Az field on a sigma mesh is not supported");
+        for (int k : {a->az_slot_a, a->az_slot_b}) {
+            if (k < -1 || k >= kMaxSlots) return fail(KELP_EINVAL, "kelp_step: Az slot %d out of range", k);
+            if (k >= 0 && c->slots[k].az_k != K)
+                return fail(KELP_ESTATE, "kelp_step: slot %d carries no Az (kelp_field_upload_az)", k);
+        }
+        if (a->az_slot_a < 0) return fail(KELP_EINVAL, "kelp_step: an Az field needs az_slot_a");
+    }
     HIP_TRY(hipSetDevice(c->device));
     for (int k : {a->slot_a, a->slot_b, a->slot_t, a->sig_start_a, a->sig_start_b, a->sig_start_t, a->sig_end_a,
-                  a->sig_end_b, a->sig_end_t})
+                  a->sig_end_b, a->sig_end_t, az_field ? a->az_slot_a : -1, az_field ? a->az_slot_b : -1})
         if (k >= 0 && k < kMaxSlots) {
             int rc = kelp_wait_slot(c, c->slots[k]);
             if (rc) return rc;
@@ -1221,8 +1235,13 @@ int kelp_step(kelp_ctx* c, const kelp_particles* p, const kelp_step_args* a)
     cfg.t_resuspend = a->t_resuspend;
     cfg.beach_dt = a->beach_dt;
     cfg.beach_seed = a->beach_seed;
-    cfg.az = a->az && a->az_const > 0.0;
+    cfg.az = a->az && (a->az_const > 0.0 || az_field);
     cfg.az_const = a->az_const;
+    if (az_field) {
+        cfg.az_a = c->slots[a->az_slot_a].az;
+        cfg.az_b = a->az_slot_b >= 0 ? c->slots[a->az_slot_b].az : nullptr;
+        cfg.az_alpha0 = a->az_alpha0;
+    }
     if ((a->periodic_lon && !(a->px1 > a->px0)) || (a->periodic_lat && !(a->py1 > a->py0)))
         return fail(KELP_EINVAL, "kelp_step: a periodic window needs x0 < x1");
     // the kick and the fold re-anchor in mode 1 (exact) or 2, so a stored

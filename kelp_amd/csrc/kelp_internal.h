@@ -47,6 +47,8 @@ struct FieldSlot {
     void* rec = nullptr;
     void* erec = nullptr;
     double* eta = nullptr;  // (N,) surface elevation, internal node order, or null
+    double* az = nullptr;   // Az at the interfaces, one column of az_k per node, or null
+    int64_t az_k = 0;       // 0: no Az for the slot's current snapshot
     ScalarField scalars[kMaxScalars];
     // asynchronous refill (kelp_field_upload_async): device staging of the
     // host planes, and the copy stream's event the readers of the slot wait

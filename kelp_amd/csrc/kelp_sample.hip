@@ -279,7 +279,7 @@ template <int MAXV>
 __global__ void sample_kernel(DMesh m, SField s, Snap<double, true> fw, int method, SampleState st,
                               double* __restrict__ out)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = xcd_thread_index();
     if (i >= st.n) return;
     const int node = st.nn[i];
     if (st.status[i] != kAlive || node < 0) {
@@ -311,7 +311,7 @@ template <int MAXV>
 __global__ void sample2_lsq_kernel(DMesh m, SField s0, SField s1, SampleState st, double* __restrict__ out0,
                                    double* __restrict__ out1)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = xcd_thread_index();
     if (i >= st.n) return;
     const int nn = st.nn[i];
     if (st.status[i] != kAlive || nn < 0) {
@@ -447,6 +447,41 @@ extern "C" int kelp_scalar_upload(kelp_ctx* c, int slot, int var, int64_t n_plan
                          hipGetErrorString(e));
     }
     f.n_planes = n_planes;
+    return KELP_OK;
+}
+
+extern "C" int kelp_field_upload_az(kelp_ctx* c, int slot, int64_t n_planes, const double* host_az)
+{
+    if (!c || !host_az) return kelp_fail(KELP_EINVAL, "kelp_field_upload_az: NULL argument");
+    if (slot < 0 || slot >= kMaxSlots)
+        return kelp_fail(KELP_EINVAL, "kelp_field_upload_az: slot %d out of range", slot);
+    FieldSlot& fs = c->slots[slot];
+    if (fs.dtype < 0 || !fs.is3d)
+        return kelp_fail(KELP_ESTATE, "kelp_field_upload_az: slot %d holds no 3-D snapshot", slot);
+    const int64_t K = c->m.n_levels, N = c->m.n_nodes;
+    // readers/fesom2.py:1607-1632: Az lives on the interfaces (the W layout)
+    if (n_planes != K)
+        return kelp_fail(KELP_EINVAL, "kelp_field_upload_az: %lld planes, Az needs the K = %lld interfaces",
+                         (long long)n_planes, (long long)K);
+    HIP_TRY(hipSetDevice(c->device));
+    const size_t bytes = (size_t)K * N * sizeof(double);
+    hipError_t e = hipSuccess;
+    if (!fs.az) e = hipMalloc(&fs.az, bytes);
+    double* tmp = nullptr;
+    if (e == hipSuccess) e = hipMalloc(&tmp, bytes);
+    // on the context stream: after every step enqueued so far, which may
+    // read the slot's previous Az
+    if (e == hipSuccess) e = hipMemcpyAsync(tmp, host_az, bytes, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        permute_planes_kernel<<<grid_of(N * K), 256, 0, c->stream>>>(N, K, c->node_old_of_new, tmp, fs.az, 1);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (tmp) (void)hipFree(tmp);
+    if (e != hipSuccess)
+        return kelp_fail(e == hipErrorOutOfMemory ? KELP_ENOMEM : KELP_EHIP, "kelp_field_upload_az: %s",
+                         hipGetErrorString(e));
+    fs.az_k = K;
     return KELP_OK;
 }
 

@@ -24,8 +24,11 @@ struct StepCfg {
     int resuspension, min_wet_steps;
     double depth_min, zone_depth, t_beach, rewet_hysteresis, t_resuspend, beach_dt;
     uint64_t beach_seed;
-    int az;           // vertical Az kick, constant source
-    double az_const;
+    int az;           // vertical Az kick
+    double az_const;  // the constant source when az_a is null
+    const double* az_a;  // Az columns (n_levels per node, internal order) of the
+    const double* az_b;  // step payload's bracket, blended by az_alpha0 when az_b
+    double az_alpha0;    // is not null
     int use_eta;      // beaching water depth = depth + eta of the step's field
     const double* eta_a;  // (N,) eta of the bracket pair (internal node order)
     const double* eta_b;

@@ -268,18 +268,77 @@ __device__ inline void kick_one(const DMesh& m, const StepCfg& c, uint64_t pid, 
     nn = nn_k;
 }
 
-// integrators.py:842 _apply_split_az_kick, constant source on a z-level
-// mesh (numerics/diffusion.py:427 _loop_az_kick_zlevel, gradient 0, and
-// :302 diffusion_az_step_meters): one particle alive after the horizontal
-// kick, folded back into [z_levels[0], depth[nn]].
+// numerics/diffusion.py:378 az_column_value_gradient on the column of `node`
+// of the step payload's Az (the bracket pair blended per value,
+// core/io.py:68): the depth clamped to [z_levels[0], min(depth, last
+// level)], the value linear between interfaces (:357), the gradient the
+// slope of the segment the layer search picks.
+__device__ inline void az_column(const DMesh& m, const double* __restrict__ az_a, const double* __restrict__ az_b,
+                                 double alpha0, int node, double z, double& val, double& grad)
+{
+    const int K = m.n_levels;
+    const double* zl = m.zlev;
+    const double* ca = az_a + (int64_t)node * K;
+    const double* cb = az_b ? az_b + (int64_t)node * K : nullptr;
+    const double beta = 1.0 - alpha0;
+    auto A = [&](int k) { return cb ? beta * ca[k] + alpha0 * cb[k] : ca[k]; };
+    const double top = zl[0];
+    const double dn = m.depth[node];
+    double bot = dn < zl[K - 1] ? dn : zl[K - 1];
+    if (bot < top) bot = top;
+    double ze = z;
+    if (ze < top) ze = top;
+    if (ze > bot) ze = bot;
+    if (K < 2) {
+        val = A(0);
+        grad = 0.0;
+        return;
+    }
+    // the first segment whose lower interface is at or below ze (the last
+    // one when none is): the gradient's segment, and the value's inside
+    int k = K - 2;
+    for (int kk = 0; kk < K - 1; ++kk)
+        if (ze <= zl[kk + 1]) {
+            k = kk;
+            break;
+        }
+    if (ze <= zl[0]) {
+        val = A(0);
+    } else if (ze >= zl[K - 1]) {
+        val = A(K - 1);
+    } else {
+        const double dzl = zl[k + 1] - zl[k];
+        if (dzl <= 0.0) {
+            val = A(k);
+        } else {
+            const double frac = (ze - zl[k]) / dzl;
+            val = (1.0 - frac) * A(k) + frac * A(k + 1);
+        }
+    }
+    const double dz = zl[k + 1] - zl[k];
+    grad = dz <= 0.0 ? 0.0 : (A(k + 1) - A(k)) / dz;
+}
+
+// integrators.py:842 _apply_split_az_kick on a z-level mesh
+// (numerics/diffusion.py:427 _loop_az_kick_zlevel and :302
+// diffusion_az_step_meters): the constant source (gradient 0) or the
+// payload's Az field, one particle alive after the horizontal kick, folded
+// back into [z_levels[0], depth[nn]].
 __device__ inline void az_one(const DMesh& m, const StepCfg& c, uint64_t pid, int node, double& z)
 {
+    double az = c.az_const, g = 0.0;
+    if (c.az_a) {
+        az_column(m, c.az_a, c.az_b, c.az_alpha0, node, z, az, g);
+        if (az < 0.0) {  // numerics/diffusion.py:448: a negative value kills the kick
+            az = 0.0;
+            g = 0.0;
+        }
+    }
     double xi, unused;
     normal_pair(c.seed, c.real_id, pid, c.step_i, 1, xi, unused);
     const double sdt = c.time_dir * c.dt;
     const double abs_dt = sdt >= 0.0 ? sdt : -sdt;
-    const double g = 0.0;
-    const double dz = g * abs_dt + km_sqrt(2.0 * c.az_const * abs_dt) * xi;
+    const double dz = g * abs_dt + km_sqrt(2.0 * az * abs_dt) * xi;
     z = km_reflect_depth(z + dz, m.zlev[0], m.depth[node]);
 }
 
@@ -403,7 +462,7 @@ template <int ORDER, int MAXV, bool IS3D, typename T, bool ELEM>
 __global__ __launch_bounds__(256) KELP_STEP_ATTR void step_kernel(DMesh m, Snap<T, IS3D> f, Snap<T, false> fe,
                                                                   StepCfg c, PState s)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t i = xcd_thread_index();
     if (i >= s.n) return;
     const int32_t prev = s.status[i];
     if (s.alive_count) {

@@ -139,7 +139,7 @@ template <int ORDER, int MAXV, typename T, bool ELEM>
 __global__ __launch_bounds__(256) void sigma_step_kernel(DMesh m, Snap<T, true> f, Snap<T, false> fe, StepCfg c,
                                                          PState s)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = xcd_thread_index();
     if (i >= s.n) return;
     const int32_t prev = s.status[i];
     if (s.alive_count) {

@@ -129,6 +129,11 @@ def make_step_args(order, slot_a, dt, time_dir, slot_b=-1, slot_t=-1, alpha0=0.0
         a.seed = int(kick["seed"])
         if kick.get("az_const"):
             a.az, a.az_const = 1, float(kick["az_const"])
+        if kick.get("az_field") is not None:
+            # (slot_a, slot_b or -1, alpha0): the step payload's Az bracket
+            fa, fb, fal = kick["az_field"]
+            a.az, a.az_field = 1, 1
+            a.az_slot_a, a.az_slot_b, a.az_alpha0 = int(fa), int(fb), float(fal)
     if beach:
         a.beach = 1
         crit = set(beach["criteria"])
@@ -223,6 +228,15 @@ class DeviceContext:
         if eta.shape != (self.mesh.n_nodes,):
             raise ValueError(f"eta of shape {eta.shape} does not match the mesh's {self.mesh.n_nodes} nodes")
         check(self.lib.kelp_field_upload_eta(self.h, int(slot), eta.ctypes.data), "kelp_field_upload_eta")
+
+    def upload_az(self, slot: int, az) -> None:
+        """Attach the snapshot's vertical diffusivity (K, N) at the
+        interfaces (mesh node order) to ``slot`` (kelp_field_upload_az)."""
+        az = np.ascontiguousarray(az, np.float64)
+        if az.ndim != 2 or az.shape[1] != self.mesh.n_nodes:
+            raise ValueError(f"Az of shape {az.shape} is not (K, {self.mesh.n_nodes})")
+        check(self.lib.kelp_field_upload_az(self.h, int(slot), az.shape[0], az.ctypes.data),
+              "kelp_field_upload_az")
 
     def upload_scalar(self, slot: int, var: int, values) -> None:
         """Node scalar ``var`` of the snapshot in ``slot`` for sampling:

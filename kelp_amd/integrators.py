@@ -107,6 +107,8 @@ class Engine:
         w = None if vel.w is None else np.asarray(vel.w, dtype)
         self.ctx.upload_field(slot, np.asarray(vel.u, dtype), np.asarray(vel.v, dtype), w,
                               uv_layout=vel.uv_layout)
+        if getattr(vel, "az", None) is not None and w is not None:
+            self.ctx.upload_az(slot, vel.az)
         self._slots[key] = (slot, vel, dtype)
         return slot
 
@@ -128,6 +130,16 @@ def _field_dtype(*fields):
         if f is not None and np.asarray(f.u).dtype != np.float32:
             return np.float64
     return np.float32
+
+
+def _payload_az(vel):
+    """The step payload's Az: a bracket's only when both snapshots carry
+    one (core/io.py:475-492), else the snapshot's own."""
+    if isinstance(vel, LazyLerpField):
+        if vel.vel_a.az is None or vel.vel_b.az is None:
+            return None
+        return vel.vel_a.az
+    return getattr(vel, "az", None)
 
 
 def _check_supported(vel, mesh, uv_interp, w_interp, diffusion, is3d):
@@ -152,10 +164,12 @@ def _check_supported(vel, mesh, uv_interp, w_interp, diffusion, is3d):
         if diffusion.composition not in ("split", "split_rk"):
             raise NotImplementedError(f"device path: composition {diffusion.composition!r}")
         if diffusion.az_enabled:
-            if diffusion.az_requires_field or getattr(base, "az", None) is not None:
-                raise NotImplementedError("device path: the Az kick from an Az field is not "
-                                          "implemented (constant az_const only)")
-            if not is3d and diffusion.az_const > 0.0:
+            has_field = _payload_az(vel) is not None
+            if diffusion.az_requires_field and not has_field:
+                # integrators.py:884-889
+                raise ValueError("diffusion.var_az is configured but the velocity payload carries "
+                                 "no az field")
+            if not is3d and (has_field or diffusion.az_const > 0.0):
                 # integrators.py:893
                 raise ValueError("diffusion.az_enabled requires a 3-D run (Fdrift3D)")
 
@@ -208,6 +222,9 @@ def _step(order: int, state: ParticleState, vel, mesh: Mesh, dt: float, time_dir
         kick = dict(kh_x=diffusion.kh_x, kh_y=diffusion.kh_y, seed=diffusion.seed)
         if diffusion.az_enabled and diffusion.az_const > 0.0:
             kick["az_const"] = float(diffusion.az_const)
+        if diffusion.az_enabled and is3d and _payload_az(vel) is not None:
+            # integrators.py:873 vel.az: the payload's bracket, blended per value
+            kick["az_field"] = (slot_a, slot_b, alpha0)
         step_i, real_id = int(diffusion.step_i), int(diffusion.real_id)
     exact = 1 if exact_nearest_on(mesh) else 2
     ctx.step(p, order, slot_a, dt, time_dir, slot_b=slot_b, slot_t=slot_t, alpha0=alpha0,

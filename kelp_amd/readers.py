@@ -167,7 +167,24 @@ class Fesom2ArrayReader:
                 if name not in ds:
                     raise KeyError(f"sample variable {name!r} not in the dataset")
                 scalars[name] = np.asarray(ds.record(name, timestep), dtype=np.float64)
-        return VelocityField(u=u, v=v, eta=eta, w=w, uv_layout=layout, scalars=scalars)
+        az = None
+        var_az = (getattr(diffusion, "var_az", None)
+                  if diffusion is not None and getattr(diffusion, "enabled", False)
+                  and getattr(diffusion, "az_enabled", False) else None)
+        if is3d and var_az:
+            # readers/fesom2.py:1596-1636: interface-based (the W layout),
+            # float64, NaN and infinities as 0
+            if var_az not in ds:
+                raise ValueError(f"diffusion.var_az={var_az!r} not found in the dataset")
+            if "nz1" in ds.dims(var_az):
+                raise ValueError(f"diffusion.var_az={var_az!r} lives on the mid-layers -- Az must be "
+                                 "interface-based (dim nz, same layout as w)")
+            az = np.asarray(ds.record(var_az, timestep), dtype=np.float64)
+            if az.shape != (mesh.sigma.size, mesh.n_nodes):
+                raise ValueError(f"diffusion.var_az={var_az!r}: expected ({mesh.sigma.size}, "
+                                 f"{mesh.n_nodes}), got {az.shape}")
+            az = np.ascontiguousarray(np.nan_to_num(az, nan=0.0, posinf=0.0, neginf=0.0))
+        return VelocityField(u=u, v=v, eta=eta, w=w, uv_layout=layout, scalars=scalars, az=az)
 
 
 def sanitize_connectivity(nv) -> np.ndarray:

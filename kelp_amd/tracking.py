@@ -222,8 +222,8 @@ class _SnapshotSlots:
     def field(self, idx: int) -> VelocityField:
         if idx not in self.fields:
             self.fields[idx] = self.reader.read_velocities(
-                self.ds, idx, self.mesh, self.mode, sample_vars=self.sample_vars,
-                velocity_float32=self.numerics.velocity_float32)
+                self.ds, idx, self.mesh, self.mode, diffusion=self.numerics.diffusion,
+                sample_vars=self.sample_vars, velocity_float32=self.numerics.velocity_float32)
         return self.fields[idx]
 
     def slot(self, idx: int) -> int:
@@ -263,6 +263,8 @@ class _SnapshotSlots:
             self.ctx.upload_field(slot, vf.u, vf.v, vf.w, uv_layout=vf.uv_layout)
             if eta is not None:
                 self.ctx.upload_eta(slot, eta)
+        if vf.az is not None and vf.w is not None:
+            self.ctx.upload_az(slot, vf.az)
         for k, name in enumerate(self.sample_vars or ()):
             self.ctx.upload_scalar(slot, k, vf.scalars[name])
         self.map[idx] = slot
@@ -299,8 +301,6 @@ def _kick_args(numerics: NumericsConfig):
         return None
     if d.scheme != "uniform":
         raise NotImplementedError(f"device path: diffusion scheme {d.scheme!r}")
-    if d.az_enabled and d.var_az:
-        raise NotImplementedError("device path: the Az kick from an Az field (var_az)")
     kick = dict(kh_x=d.kh_zonal, kh_y=d.kh_meridional, seed=d.seed)
     if d.az_enabled and d.az_const > 0.0:
         kick["az_const"] = d.az_const
@@ -459,6 +459,12 @@ def run_single_realization(mesh: Mesh, ds, reader, init_state: ParticleState, mo
             raise NotImplementedError("device path: pchip reconstruction with beaching use_eta")
     w_method = ((SIGMA_W_METHODS if sig3d else W_METHODS)[numerics.w_interp] if is3d else 2)
     kick = _kick_args(numerics)
+    d = numerics.diffusion
+    az_field = bool(is3d and d.enabled and d.az_enabled and d.var_az)
+    if az_field and sig3d:
+        # integrators.py:902-906
+        raise ValueError("diffusion.var_az is not supported on sigma meshes (v1) -- use the "
+                         "constant az_const source instead")
     beach = _beach_args(numerics, dt)
     periodic = dict(periodic_lon=numerics.periodic_lon, periodic_lat=numerics.periodic_lat)
 
@@ -608,9 +614,13 @@ def run_single_realization(mesh: Mesh, ds, reader, init_state: ParticleState, mo
                 ctx.reconstruct(dst, src, t4, float(q) + float(direction) * c * step_len)
             step_a, step_b, step_t, step_a0 = RECON_SLOTS[0], -1, RECON_SLOTS[1], 0.0
             alphas = [0.0, 1.0, 0.0, 0.0]
+        step_kick = kick
+        if az_field:
+            # integrators.py:873: the Az of the step payload (the bracket)
+            step_kick = dict(kick, az_field=(sa, sb, pay.alpha0))
         ctx.step(p, order, step_a, step_len, float(direction), slot_b=step_b, slot_t=step_t,
                  alpha0=step_a0, alphas=alphas, exact_nearest=exact, w_method=w_method,
-                 kick=kick, beach=beach, step_i=g, real_id=realization_id, reuse_anchor=True,
+                 kick=step_kick, beach=beach, step_i=g, real_id=realization_id, reuse_anchor=True,
                  sigma=sigma, walk=walk, **periodic)
         if prefetch and n_time_total and n_time_total > 1:
             # the snapshot after this step's bracket streams in while it runs

@@ -1891,10 +1891,67 @@ static void az_sigma_one(const or_mesh_t *m, const or_step_args_t *a, uint64_t p
     *z = REFLECT_DEPTH(*z + dz, 0.0, bot);
 }
 
+/* numerics/diffusion.py:378 az_column_value_gradient: Az and dAz/dz on the
+ * column of `node` at depth z clamped to [z_levels[0], min(depth, last
+ * level)]; the value linear between interfaces (:357 _az_interp_column),
+ * the gradient the slope of the segment the layer search picks (the one
+ * above an interface the depth sits on). */
+static void az_column(const or_mesh_t *m, const double *azf, int32_t node, double z, double *val, double *grad)
+{
+    const int K = m->n_levels;
+    const double *zl = m->z_levels;
+    const int64_t N = m->n_nodes;
+#define AZ(k) azf[(int64_t)(k) * N + node]
+    double top = zl[0];
+    double bot = m->depth[node] < zl[K - 1] ? m->depth[node] : zl[K - 1];
+    if (bot < top) bot = top;
+    double ze = z;
+    if (ze < top) ze = top;
+    if (ze > bot) ze = bot;
+    if (K < 2) {
+        *val = AZ(0);
+        *grad = 0.0;
+        return;
+    }
+    if (ze <= zl[0]) {
+        *val = AZ(0);
+    } else if (ze >= zl[K - 1]) {
+        *val = AZ(K - 1);
+    } else {
+        int k = 0;
+        while (!(ze <= zl[k + 1])) ++k;
+        double dzl = zl[k + 1] - zl[k];
+        if (dzl <= 0.0) {
+            *val = AZ(k);
+        } else {
+            double frac = (ze - zl[k]) / dzl;
+            *val = (1.0 - frac) * AZ(k) + frac * AZ(k + 1);
+        }
+    }
+    int ks = K - 2;
+    for (int kk = 0; kk < K - 1; ++kk) {
+        if (ze <= zl[kk + 1]) {
+            ks = kk;
+            break;
+        }
+    }
+    double dz = zl[ks + 1] - zl[ks];
+    *grad = dz <= 0.0 ? 0.0 : (AZ(ks + 1) - AZ(ks)) / dz;
+#undef AZ
+}
+
 static void az_one(const or_mesh_t *m, const or_step_args_t *a, uint64_t pid, int32_t node, double *z)
 {
     double az = a->az_const < 0.0 ? 0.0 : a->az_const;
     double g = 0.0;
+    if (a->az_field) {
+        /* numerics/diffusion.py:448-454: a negative value kills the kick */
+        az_column(m, a->az_field, node, *z, &az, &g);
+        if (az < 0.0) {
+            az = 0.0;
+            g = 0.0;
+        }
+    }
     double xi, unused;
     or_normal_pair(a->seed, a->real_id, pid, a->step_i, 1, &xi, &unused);
     double sdt = a->time_dir * a->dt;
@@ -2023,7 +2080,7 @@ void or_step(const or_mesh_t *m, const or_fields_t *f, const or_step_args_t *a, 
             if (a->kick && status[i] == ST_ALIVE && !(a->kh_x == 0.0 && a->kh_y == 0.0))
                 kick_one(m, a, (uint64_t)pid[i], pre_lon, pre_lat, pre_z, pre_nn,
                          &lon[i], &lat[i], &z[i], &status[i], &nn[i], &fr[i]);
-            if (a->az && a->is3d && a->az_const > 0.0 && status[i] == ST_ALIVE)
+            if (a->az && a->is3d && (a->az_field || a->az_const > 0.0) && status[i] == ST_ALIVE)
                 az_one(m, a, (uint64_t)pid[i], nn[i], &z[i]);
             if ((a->periodic_lon || a->periodic_lat) && status[i] == ST_ALIVE)
                 periodic_one(m, a, &lon[i], &lat[i], &nn[i], el);

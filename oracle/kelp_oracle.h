@@ -74,9 +74,12 @@ typedef struct {
     double depth_min, zone_depth, t_beach, rewet_hysteresis, t_resuspend;
     uint64_t beach_seed;
     double beach_dt;
-    /* vertical Az kick, constant source (integrators.py:842) */
+    /* vertical Az kick (integrators.py:842): the constant source, or the
+     * step payload's Az field (K, N) at the interfaces (vel.az, already
+     * blended over the bracket) when az_field is not NULL */
     int32_t az;
     double az_const;
+    const double *az_field;
     int32_t use_eta;   /* beaching water depth = depth + eta (beaching.py:121) */
     /* periodic fold windows [x0, x1) / [y0, y1) (tracking.py:1089) */
     int32_t periodic_lon, periodic_lat;

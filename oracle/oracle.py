@@ -49,7 +49,7 @@ class OrStepArgs(ctypes.Structure):
         ("depth_min", c_double), ("zone_depth", c_double), ("t_beach", c_double),
         ("rewet_hysteresis", c_double), ("t_resuspend", c_double),
         ("beach_seed", ctypes.c_uint64), ("beach_dt", c_double),
-        ("az", c_int32), ("az_const", c_double), ("use_eta", c_int32),
+        ("az", c_int32), ("az_const", c_double), ("az_field", c_void_p), ("use_eta", c_int32),
         ("periodic_lon", c_int32), ("periodic_lat", c_int32),
         ("px0", c_double), ("px1", c_double), ("py0", c_double), ("py1", c_double),
         ("sig_eta", c_void_p * 5), ("Dmin", c_double), ("z0b_min", c_double),
@@ -89,6 +89,11 @@ def step_args(order, dt, time_dir, is3d=False, w_method=2, exact=True, alphas=(0
         a.kh_x, a.kh_y, a.seed = float(kick["kh_x"]), float(kick["kh_y"]), int(kick["seed"])
         if kick.get("az_const"):
             a.az, a.az_const = 1, float(kick["az_const"])
+        if kick.get("az_field") is not None:
+            # the step payload's Az (K, N) at the interfaces, mesh order
+            azf = np.ascontiguousarray(kick["az_field"], dtype=np.float64)
+            a._az_keep = azf
+            a.az, a.az_field = 1, azf.ctypes.data
     if beach:
         a.beach = 1
         crit = set(beach["criteria"])

@@ -207,3 +207,42 @@ def test_reused_anchor_is_result_neutral(gpu_ctx, world, exact):
         assert np.array_equal(a[k].view(np.int64), b[k].view(np.int64)), k
     for k in ("status", "nn", "fr", "ws"):
         assert np.array_equal(a[k], b[k]), k
+
+
+@pytest.mark.parametrize("order,direction", [(1, 1.0), (4, 1.0), (2, -1.0)])
+def test_3d_az_field_kick_bitwise(gpu_ctx, world, order, direction):
+    """The vertical kick from an Az field (kelp_field_upload_az): per
+    snapshot, depth- and node-dependent, a patch of negative values (the
+    kick switched off there), blended over the step's bracket like the
+    payload (core/io.py:68); after the Kh walk, twelve steps."""
+    mesh, snaps, lon, lat, z, pid = world
+    gpu_ctx.upload_mesh(mesh)
+    K, N = mesh.sigma.size, mesh.n_nodes
+    zl = np.asarray(mesh.sigma)[:, None]
+    azs = []
+    for s, sn in enumerate(snaps):
+        gpu_ctx.upload_field(s, sn[0].astype(np.float32), sn[1].astype(np.float32), sn[2].astype(np.float32))
+        az = (2e-3 * (1.0 + 0.5 * s) * np.exp(-zl / 800.0) * (1.2 + np.sin(np.radians(mesh.nodes_lon)))[None, :]
+              + 1e-5 * zl / 100.0)
+        az[:, mesh.nodes_lat > 50.0] *= -1.0
+        azs.append(az)
+        gpu_ctx.upload_az(s, az)
+    p = gpu_ctx.particles(lon, lat, z, pid=pid)
+    om = OracleMesh(mesh, "kmath")
+    st = dict(lon=lon, lat=lat, z=z, status=np.ones(lon.size, np.int32),
+              nn=np.full(lon.size, -1, np.int32), pid=pid)
+    kick = dict(kh_x=10.0, kh_y=5.0, seed=41)
+    t0 = 0.0 if direction > 0 else 2 * synth.SNAP_DT
+    for k in range(12):
+        f, (ia, ib, it, a0, dtn) = _fields(snaps, t0 + direction * k * DT, direction, np.float32)
+        c = (0.0, 0.5, 0.5, 1.0) if order == 4 else (0.0, 0.5) if order == 2 else (0.0,)
+        al = [ck * DT / dtn for ck in c]
+        az_pay = azs[ia] if ib < 0 else (1.0 - a0) * azs[ia] + a0 * azs[ib]
+        st = om.step(st, f, step_args(order, DT, direction, is3d=True, w_method=2, alphas=al,
+                                      kick=dict(kick, az_field=az_pay), step_i=k))
+        gpu_ctx.step(p, order, ia, DT, direction, slot_b=ib, slot_t=it, alpha0=a0, alphas=al, w_method=2,
+                     kick=dict(kick, az_field=(ia, ib, a0)), step_i=k)
+    gpu_ctx.synchronize()
+    h = p.to_host()
+    assert (st["status"] == 1).mean() > 0.8
+    _assert_bitwise(h, st)

@@ -277,3 +277,56 @@ def test_pchip_run_reconstructs_interior_steps(gpu_ctx, direction):
         np.testing.assert_array_equal(a[k][:, :2], b[k][:, :2], err_msg=k)
     assert np.nanmax(np.abs(a["lon"][:, 2:6] - b["lon"][:, 2:6])) > 1e-7
     assert np.nanmax(np.abs(a["vel_u"][:, 2:6] - b["vel_u"][:, 2:6])) > 1e-6
+
+
+@pytest.mark.parametrize("direction", [1.0, -1.0])
+def test_az_field_run_follows_the_payload_bracket(gpu_ctx, direction):
+    """diffusion.var_az through the tracking loop: the reader's Az record
+    rides in the snapshot slots and each step's kick takes the payload's
+    bracket blend (integrators.py:873 vel.az).  With no flow, Az uniform in
+    space and depth but different in every record, and particles far from
+    the column ends, step g moves z by exactly sqrt(2 c_g dt) xi_g: c_g the
+    blend of the records around the step's start, xi_g the counter-RNG
+    normal of (seed, 0, pid, g, 1)."""
+    import ctypes
+    from kelp_amd.config import DiffusionConfig, ModeConfig, NumericsConfig
+    from kelp_amd.particles import make_state
+    from kelp_amd.readers import ArrayDataset
+    from oracle.oracle import lib
+    mesh = rc.run_mesh(lambda lon, lat: np.full(lon.shape, 90.0))
+    K, N, T = mesh.sigma.size, mesh.n_nodes, 5
+    cs = np.array([4e-4, 1.1e-3, 2.5e-4, 7e-4, 1.6e-3])
+    z3 = lambda k: np.zeros((T, k, N))
+    ds = ArrayDataset({"unod": (("time", "nz1", "nod2"), z3(K - 1)), "vnod": (("time", "nz1", "nod2"), z3(K - 1)),
+                       "w": (("time", "nz", "nod2"), z3(K)), "ssh": (("time", "nod2"), np.zeros((T, N))),
+                       "Av": (("time", "nz", "nod2"), np.ascontiguousarray(
+                           np.broadcast_to(cs[:, None, None], (T, K, N))))},
+                      time=np.arange(T) * rc.RUN_FILE_DT)
+    i = np.arange(16, dtype=np.float64)
+    lon, lat = 4.3 + 1.4 * ((i * 0.618034) % 1.0), 4.3 + 1.4 * ((i * 0.381966) % 1.0)
+    z0 = 30.0 + 30.0 * ((i * 0.7548) % 1.0)
+    seed = 1234
+    numerics = NumericsConfig(dt=rc.RUN_DT, integrator="rk2", uv_interp="barycentric", w_interp="barycentric",
+                              temporal_interp=True,
+                              diffusion=DiffusionConfig(enabled=True, seed=seed, az_enabled=True, var_az="Av"))
+    mode = ModeConfig(Fdrift2D=False, Fdrift3D=True, time_dir=direction, var_u="unod", var_v="vnod", var_w="w")
+    n_steps, start = 8, (0 if direction > 0 else T - 1)
+    rec = run_single_realization(mesh, ds, get_reader("fesom2_arrays"), make_state(lon, lat, z0), mode, numerics,
+                                 start_time_idx=start, n_steps=n_steps, n_time_total=T,
+                                 file_times=np.arange(T) * rc.RUN_FILE_DT, file_dt=rc.RUN_FILE_DT,
+                                 ctx=gpu_ctx).as_arrays()
+    L = lib("glibc")
+    a, b = ctypes.c_double(), ctypes.c_double()
+    z = z0.copy()
+    for g in range(n_steps):
+        t = start * rc.RUN_FILE_DT + direction * g * rc.RUN_DT
+        ia = int(np.floor(t / rc.RUN_FILE_DT + 1e-9))
+        if direction < 0 and abs(t - ia * rc.RUN_FILE_DT) < 1e-6:
+            ia -= 1          # backward on a record: the interval below it
+        al = (t - ia * rc.RUN_FILE_DT) / rc.RUN_FILE_DT
+        c = cs[ia] if abs(al) < 1e-12 else (1.0 - al) * cs[ia] + al * cs[ia + 1]
+        for j in range(z.size):
+            L.or_normal_pair(seed, 0, j, g, 1, ctypes.byref(a), ctypes.byref(b))
+            z[j] = z[j] + np.sqrt(2.0 * c * rc.RUN_DT) * a.value
+        np.testing.assert_allclose(rec["z"][:, g], z, rtol=0, atol=1e-9, err_msg=f"step {g}")
+    assert (rec["status"] == 1).all()

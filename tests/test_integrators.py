@@ -55,7 +55,7 @@ def test_refused_combinations_raise_before_the_device(small):
     with pytest.raises(ValueError, match="3-D"):
         I.euler_step(st, vel, mesh, 600.0, 1.0, uv_interp="barycentric", diffusion=d)
     d = DiffusionStepArgs("uniform", 1.0, 1.0, 1, 0, 0, az_enabled=True, az_requires_field=True)
-    with pytest.raises(NotImplementedError, match="Az field"):
+    with pytest.raises(ValueError, match="no az field"):   # integrators.py:884
         I.euler_step(st, vel, mesh, 600.0, 1.0, uv_interp="barycentric", diffusion=d)
     with pytest.raises(ValueError, match="together"):
         I.rk4_step(st, vel, mesh, 600.0, 1.0, uv_interp="barycentric", vel_next=vel)

@@ -155,3 +155,29 @@ def test_pchip_config_checks():
     num.temporal_reconstruction = "cubic"
     with pytest.raises(ValueError, match="temporal_reconstruction"):
         T._check_path(num, mesh, True, "node")
+
+
+def test_reader_reads_var_az_on_the_interfaces():
+    """readers/fesom2.py:1596-1636: Az only when the vertical kick is on,
+    float64 with NaN as 0, refused on the mid-layers."""
+    from kelp_amd.readers import ArrayDataset, get_reader
+    from tests.golden import reference_cases as rc
+    mesh = rc.run_mesh(lambda lon, lat: np.full(lon.shape, 90.0))
+    K, N = mesh.sigma.size, mesh.n_nodes
+    av = np.full((2, K, N), 2e-3, np.float32)
+    av[1, 2, 3] = np.nan
+    ds = ArrayDataset({"unod": (("time", "nz1", "nod2"), np.zeros((2, K - 1, N))),
+                       "vnod": (("time", "nz1", "nod2"), np.zeros((2, K - 1, N))),
+                       "w": (("time", "nz", "nod2"), np.zeros((2, K, N))),
+                       "Av": (("time", "nz", "nod2"), av),
+                       "Avmid": (("time", "nz1", "nod2"), np.zeros((2, K - 1, N)))}, time=[0.0, 3600.0])
+    mode = C.ModeConfig(Fdrift2D=False, Fdrift3D=True, var_u="unod", var_v="vnod", var_w="w")
+    R = get_reader("fesom2_arrays")
+    d = C.DiffusionConfig(enabled=True, az_enabled=True, var_az="Av")
+    vf = R.read_velocities(ds, 1, mesh, mode, diffusion=d)
+    assert vf.az.dtype == np.float64 and vf.az.shape == (K, N)
+    assert vf.az[2, 3] == 0.0 and vf.az[0, 0] == np.float64(np.float32(2e-3))
+    assert R.read_velocities(ds, 1, mesh, mode, diffusion=C.DiffusionConfig(var_az="Av")).az is None
+    with pytest.raises(ValueError, match="mid-layers"):
+        R.read_velocities(ds, 0, mesh, mode, diffusion=C.DiffusionConfig(enabled=True, az_enabled=True,
+                                                                            var_az="Avmid"))
