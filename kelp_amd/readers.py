@@ -329,3 +329,142 @@ class FesomCArrayReader:
                        for name in sample_vars}
         return VelocityField(u=u.astype(vd, copy=False), v=v.astype(vd, copy=False), eta=eta,
                              w=w.astype(vd, copy=False), uv_layout=layout, scalars=scalars)
+
+
+_HP_U = ("avg_uoe", "uo", "uos", "u")
+_HP_V = ("avg_von", "vo", "vos", "v")
+_HP_W = ("avg_wo", "wo", "w", "wvel")
+_HP_ETA = ("avg_zos", "zos", "ssh", "eta", "sossheig")
+_HP_DEPTH = ("deptho", "bathymetry", "bathy_meter")
+
+
+@register("healpix_arrays")
+class HealpixArrayReader:
+    """readers/healpix.py:1291 HealpixReader over an ``ArrayDataset`` of a
+    HEALPix store: u/v ``(time, level, value)`` with land and below-bottom
+    levels NaN (finite-prefix columns), W ON the interfaces (positive up,
+    like the products), surface elevation per cell; the NEST ordering and
+    the geographic frame (no ``mesh.rotation``).  Nodes are the wet pixel
+    centres of the dual mesh (``healpix.dual_mesh``), data gathered at the
+    node pixels."""
+
+    @staticmethod
+    def _name(ds, aliases, override=None):
+        if override:
+            if override not in ds:
+                raise KeyError(f"healpix store: variable {override!r} not found")
+            return override
+        return _first(ds, aliases)
+
+    @classmethod
+    def load_mesh(cls, ds, levels=None, nside=None, order="nest", mode_cfg=None, name="healpix"):
+        """readers/healpix.py:1295 load_mesh: the wet mask = finite u and v
+        at the first record on the tracked level (the surface unless
+        ``mode.sigma_layer``), the bathymetry from a depth variable (which
+        also refines the mask) or else ``levels[min(finite count, K-1)]``
+        per cell, then the dual mesh on those pixels."""
+        from . import healpix
+        crs = ds.attrs.get("crs", {})
+        nside = int(nside if nside is not None else crs.get("healpix_nside"))
+        order = crs.get("healpix_order", order)
+        if order != "nest":
+            raise NotImplementedError("healpix_arrays: RING-ordered stores")
+        u_name = cls._name(ds, _HP_U, getattr(mode_cfg, "var_u", None))
+        v_name = cls._name(ds, _HP_V, getattr(mode_cfg, "var_v", None))
+        if u_name is None or v_name is None:
+            raise KeyError("healpix store: no u/v variable found")
+        u0, v0 = np.asarray(ds.record(u_name, 0), np.float64), np.asarray(ds.record(v_name, 0), np.float64)
+        lev = getattr(mode_cfg, "sigma_layer", None) or 0
+        u0s = u0[lev] if u0.ndim == 2 else u0
+        v0s = v0[lev] if v0.ndim == 2 else v0
+        npix = 12 * nside * nside
+        if u0s.size != npix:
+            raise ValueError(f"healpix store: {u_name!r} has {u0s.size} cells, expected {npix}")
+        wet = np.isfinite(u0s) & np.isfinite(v0s)
+        levels = None if levels is None else np.abs(np.asarray(levels, np.float64))
+        if getattr(mode_cfg, "Fdrift3D", False) and levels is None:
+            raise ValueError("healpix 3-D mode needs the vertical interface depths (mesh.levels)")
+        dep = None
+        d_name = _first(ds, _HP_DEPTH)
+        if d_name is not None:
+            dd = np.asarray(ds.vars[d_name][1], np.float64).reshape(-1)
+            dep = np.abs(np.nan_to_num(dd, nan=0.0))
+            wet &= dep > 0.0
+        elif levels is not None and u0.ndim == 2:
+            nfin = np.minimum(np.isfinite(u0).sum(axis=0), levels.size - 1)
+            dep = levels[nfin]
+        mesh = healpix.dual_mesh(nside, wet_mask=wet, depth=dep, z_levels=levels, name=name)
+        mesh.metadata["healpix_npix"] = npix
+        return mesh
+
+    @staticmethod
+    def probe_uv_layout(ds, mode_cfg=None, mesh=None) -> str:
+        return "node"
+
+    @classmethod
+    def read_velocities(cls, ds, timestep, mesh, mode_cfg=None, diffusion=None, sample_vars=None,
+                        velocity_float32=False) -> VelocityField:
+        """readers/healpix.py:1872 read_velocities (dual geometry): K-1
+        u/v layers and K W interfaces gathered at the node pixels (a layer
+        variable's K-th plane must be empty there), W negated when the
+        store is positive up, every non-finite velocity zeroed (ground
+        below each column's finite prefix), eta NaN as 0."""
+        if sample_vars:
+            raise NotImplementedError("healpix_arrays: record-point sampling")
+        idx = np.asarray(mesh.metadata["healpix_node_pix"], np.int64)
+        is3d = bool(getattr(mode_cfg, "Fdrift3D", False))
+        u_name = cls._name(ds, _HP_U, getattr(mode_cfg, "var_u", None))
+        v_name = cls._name(ds, _HP_V, getattr(mode_cfg, "var_v", None))
+        vd = np.float32 if velocity_float32 else np.float64
+        u = np.asarray(ds.record(u_name, timestep), np.float64)
+        v = np.asarray(ds.record(v_name, timestep), np.float64)
+        w = None
+        if is3d:
+            K = int(mesh.sigma.size)
+            w_name = cls._name(ds, _HP_W, getattr(mode_cfg, "var_w", None))
+            if w_name is None:
+                raise KeyError("healpix store: no vertical velocity")
+            wr = np.asarray(ds.record(w_name, timestep), np.float64)
+            if wr.shape[0] < K:
+                raise ValueError(f"healpix reader: {w_name!r} has {wr.shape[0]} levels for {K} interfaces")
+            w = wr[:K][:, idx]
+            pos = getattr(mode_cfg, "w_positive", "auto")
+            if pos not in ("up", "down"):
+                pos = ds.attrs.get(w_name, {}).get("positive", "up")
+            if pos == "up":
+                w = np.negative(w)
+            cols = []
+            for name, a in ((u_name, u), (v_name, v)):
+                if a.shape[0] > K - 1 and np.isfinite(a[K - 1][idx]).any():
+                    raise ValueError(f"healpix reader: {name!r} carries data in the plane below the "
+                                     "deepest interface at tracked nodes")
+                c = a[:K - 1][:, idx]
+                fin = np.isfinite(c)
+                if (~fin[:-1] & fin[1:]).any():
+                    raise ValueError(f"healpix reader: {name!r} has finite data below a non-finite level")
+                cols.append(c)
+            u, v = cols
+            w = w.astype(vd)
+        else:
+            lev = getattr(mode_cfg, "sigma_layer", None) or 0
+            u = (u[lev] if u.ndim == 2 else u)[idx]
+            v = (v[lev] if v.ndim == 2 else v)[idx]
+        surf = (u[0] if is3d else u)
+        surf_v = (v[0] if is3d else v)
+        bad = ~(np.isfinite(surf) & np.isfinite(surf_v))
+        if bad.mean() > 0.05:
+            raise ValueError(f"healpix reader: {bad.mean():.1%} of wet nodes carry non-finite u/v at "
+                             f"timestep {timestep}")
+        u = u.astype(vd)
+        v = v.astype(vd)
+        u[~np.isfinite(u)] = 0.0
+        v[~np.isfinite(v)] = 0.0
+        if w is not None:
+            w[~np.isfinite(w)] = 0.0
+        eta = np.zeros(mesh.n_nodes)
+        e_name = cls._name(ds, _HP_ETA, getattr(mode_cfg, "var_eta", None))
+        if e_name is not None:
+            eta = np.nan_to_num(np.asarray(ds.record(e_name, timestep), np.float64).reshape(-1)[idx],
+                                nan=0.0, posinf=0.0, neginf=0.0)
+        return VelocityField(u=np.ascontiguousarray(u), v=np.ascontiguousarray(v), eta=eta,
+                             w=None if w is None else np.ascontiguousarray(w), uv_layout="node")

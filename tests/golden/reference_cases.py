@@ -418,3 +418,66 @@ def sigma_case(direction):
     return dict(mesh=mesh, ds=ds, init=make_state(lon, lat, frac * h0), mode=mode,
                 numerics=numerics, start_time_idx=0 if direction > 0 else n_time - 1, n_steps=12,
                 n_time_total=n_time, file_times=times, file_dt=900.0)
+
+
+HP_LEVELS = [0.0, 10.0, 30.0, 70.0]      # tests/test_healpix_reader.py:28 LEVELS_M
+HP_NTIME, HP_NLEV = 3, 4                 # :27 NTIME, :26 NLEV
+
+
+def healpix_store(nside=8):
+    """tests/test_healpix_reader.py:45 _make_uv_store(nside=8, with_eta,
+    with_w, n_finite_levels as tests/_gen_golden_3d.py:636-637, w_value
+    1e-4) with u/v rewritten per level, cell and record
+    (tests/_gen_golden_3d.py:594 _rewrite_healpix_uv), as arrays: the
+    northern cap above 60 N is land, wet columns hold 3 finite layers south
+    of the equator, 2 north of it and 1 at every fifth pixel; the deepest
+    level is NaN padding for u/v; W sits on the interfaces (one more finite
+    entry than u); eta 0.25 on the wet cells; daily records."""
+    from kelp_amd import healpix
+    from kelp_amd.readers import ArrayDataset
+    npix = 12 * nside * nside
+    plon, plat = healpix.pix2ang_nest(nside, np.arange(npix), lonlat=True)
+    plon = np.mod(plon, 360.0)
+    wet = plat < 60.0
+    nfin = np.where(wet, np.where(plat < 0, 3, 2), 0)
+    nfin[::5] = np.where(wet[::5], 1, 0)
+    T, L = HP_NTIME, HP_NLEV
+    t = np.arange(T, dtype=np.float64)[:, None, None]
+    k = np.arange(L, dtype=np.float64)[None, :, None]
+    x, y = (plon / 360.0)[None, None, :], (plat / 60.0)[None, None, :]
+    alive = nfin[None, None, :] > k
+    u = np.where(alive, 0.4 - 0.12 * k + 0.03 * y + 0.01 * t, np.nan)
+    v = np.where(alive, 0.05 - 0.05 * k + 0.02 * x - 0.005 * t, np.nan)
+    on_ifc = (nfin[None, None, :] >= k) & (nfin[None, None, :] > 0)
+    w = np.where(on_ifc, 1e-4, np.nan) + np.zeros((T, 1, 1))
+    eta = np.repeat(np.where(wet, 0.25, np.nan)[None, :], T, axis=0)
+    c = lambda a: np.ascontiguousarray(np.broadcast_to(a, (T, L, npix)), dtype=np.float64)
+    ds = ArrayDataset({"avg_uoe": (("time", "level", "value"), c(u)),
+                       "avg_von": (("time", "level", "value"), c(v)),
+                       "avg_wo": (("time", "level", "value"), c(w)),
+                       "avg_zos": (("time", "value"), np.ascontiguousarray(eta))},
+                      time=np.arange(T) * 86400.0,
+                      attrs={"crs": {"grid_mapping_name": "healpix", "healpix_nside": nside,
+                                     "healpix_order": "nest"}})
+    return ds
+
+
+def healpix_case(direction):
+    """tests/_gen_golden_3d.py:625 healpix_3d_rk2: the native HEALPix
+    reader's dual mesh and records, float32 velocities, node RK2 with
+    barycentric u/v and W and a linear time blend, 6-hour steps over the
+    three daily records."""
+    from kelp_amd.config import ModeConfig, NumericsConfig
+    from kelp_amd.particles import make_state
+    from kelp_amd.readers import get_reader
+    ds = healpix_store(8)
+    mode = ModeConfig(Fdrift2D=False, Fdrift3D=True, time_dir=direction)
+    mesh = get_reader("healpix_arrays").load_mesh(ds, levels=HP_LEVELS, mode_cfg=mode)
+    lon = np.repeat([-150.0, -60.0, 0.0, 45.0, 120.0, 170.0], 4)
+    lat = np.tile([-8.0, -2.0, 3.0, 51.0], 6)
+    z = np.array([6.3, 15.0, 25.0, 29.5, 35.0, 47.6, 65.0, 69.0] * 3)
+    numerics = NumericsConfig(dt=21600.0, integrator="rk2", uv_interp="barycentric",
+                              w_interp="barycentric", temporal_interp=True, velocity_float32=True)
+    return dict(mesh=mesh, ds=ds, init=make_state(lon, lat, z), mode=mode, numerics=numerics,
+                start_time_idx=0 if direction > 0 else HP_NTIME - 1, n_steps=8,
+                n_time_total=HP_NTIME, file_times=np.arange(HP_NTIME) * 86400.0, file_dt=86400.0)

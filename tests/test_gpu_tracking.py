@@ -330,3 +330,25 @@ def test_az_field_run_follows_the_payload_bracket(gpu_ctx, direction):
             z[j] = z[j] + np.sqrt(2.0 * c * rc.RUN_DT) * a.value
         np.testing.assert_allclose(rec["z"][:, g], z, rtol=0, atol=1e-9, err_msg=f"step {g}")
     assert (rec["status"] == 1).all()
+
+
+@pytest.mark.parametrize("direction", [1.0, -1.0])
+def test_healpix_run_matches_reference_golden(gpu_ctx, direction):
+    """The native HEALPix reader's store (finite-prefix columns of 1-3
+    layers, NaN land and padding, W on the interfaces, float32) on its wet
+    dual mesh, node RK2 with a time blend: the reference's run-level golden
+    ``healpix_3d_rk2`` through the device tracking loop."""
+    case = rc.healpix_case(direction)
+    gold = np.load(os.path.join(GOLD, "healpix_3d_rk2" + ("_bwd" if direction < 0 else "") + ".npz"),
+                   allow_pickle=False)
+    rec = run_single_realization(
+        case["mesh"], case["ds"], get_reader("healpix_arrays"), case["init"], case["mode"],
+        case["numerics"], start_time_idx=case["start_time_idx"], n_steps=case["n_steps"],
+        n_time_total=case["n_time_total"], file_times=case["file_times"],
+        file_dt=case["file_dt"], ctx=gpu_ctx).as_arrays()
+    assert (gold["status"] == 3).any() and (gold["status"] == 1).any()
+    np.testing.assert_array_equal(rec["status"], gold["status"])
+    np.testing.assert_array_equal(rec["failure_reason"], gold["failure_reason"])
+    for k, src in (("lon", "lon"), ("lat", "lat"), ("z", "z"), ("u", "vel_u"), ("v", "vel_v"),
+                   ("w", "vel_w")):
+        np.testing.assert_allclose(rec[src], gold[k], rtol=1e-9, atol=1e-9, err_msg=k)
