@@ -187,6 +187,7 @@ class TrajectoryRecord:
     arrays with the reference writer's variable names (writers.py:1209)."""
     time: List[float] = field(default_factory=list)
     rows: Dict[str, List[np.ndarray]] = field(default_factory=dict)
+    meta: Dict[str, object] = field(default_factory=dict)
 
     def append(self, rec: Dict[str, np.ndarray], time_val: float) -> None:
         self.time.append(float(time_val))
@@ -196,6 +197,7 @@ class TrajectoryRecord:
     def as_arrays(self) -> Dict[str, np.ndarray]:
         out = {k: np.stack(v, axis=1) for k, v in self.rows.items()}
         out["time"] = np.asarray(self.time)
+        out.update({k: v for k, v in self.meta.items() if isinstance(v, np.ndarray)})
         return out
 
     def save_npz(self, path: str) -> None:
@@ -739,24 +741,85 @@ def load_restart(path: str) -> Dict[str, object]:
                 particle_offset=int(d["restart_particle_offset"]))
 
 
+def shift_to_midpoints(file_times: np.ndarray) -> np.ndarray:
+    """tracking.py:1427 _shift_to_midpoints: end-of-window stamps of
+    time-averaged records to the midpoints of their windows (the first one
+    extrapolated by half the first interval)."""
+    t = np.asarray(file_times, dtype=np.float64)
+    if t.size < 2:
+        return t.copy()
+    out = np.empty_like(t)
+    out[1:] = 0.5 * (t[:-1] + t[1:])
+    out[0] = t[0] - 0.5 * (t[1] - t[0])
+    return out
+
+
+def resolve_record_times(config: AppConfig, ds, mesh_type: str = "fesom2"):
+    """tracking.py:3416-3500: the record times of ``ds`` (seconds since its
+    first record), shifted to the window midpoints under
+    ``data.time_averaging: mean`` (the FESOM2 and HEALPix default when
+    unset, with a warning; FESOM-C and the others default to instant), the
+    snapshot interval (``numerics.file_dt`` wins) and the step (``dt``, or
+    the snapshot interval)."""
+    from .readers import extract_time_info
+    file_times, file_dt = extract_time_info(ds)
+    avg = config.data.time_averaging
+    if avg is None:
+        avg = "mean" if ("fesom2" in mesh_type or "healpix" in mesh_type) else "instant"
+        warnings.warn(f"data.time_averaging is not set -- assuming {avg!r}. Please set "
+                      "data.time_averaging explicitly to 'mean' or 'instant' in your config.",
+                      UserWarning, stacklevel=3)
+    elif avg not in ("mean", "instant"):
+        raise ValueError(f"data.time_averaging must be 'mean' or 'instant', got {avg!r}")
+    if avg == "mean" and file_times is not None and len(file_times) > 1:
+        file_times = shift_to_midpoints(file_times)
+    if config.numerics.file_dt is not None:
+        file_dt = float(config.numerics.file_dt)
+        if file_times is None:
+            file_times = np.arange(ds.time.size, dtype=np.float64) * file_dt
+    dt = config.numerics.dt if config.numerics.dt is not None else file_dt
+    if dt is None:
+        raise ValueError("numerics.dt is not set and the dataset has no time step to take it from")
+    return file_times, file_dt, float(dt), avg
+
+
 def run_tracking(config: AppConfig, *, mesh: Mesh, dataset, reader="fesom2_arrays",
                  init_state: ParticleState, ctx=None) -> List[TrajectoryRecord]:
     """tracking.py:3288 run_tracking for in-memory inputs: the mesh, the
     dataset and the seed state are passed in (the file readers of the
-    reference are outside this build); the rest comes from ``config``.
-    Sets the mesh's exact-nearest guard from ``mesh.exact_nearest``
-    (tracking.py:3370) and runs ``time.num_runs`` realizations, each
-    shifted by ``time.run_interval`` records."""
+    reference are outside this build); the rest comes from ``config``:
+    the record times (:func:`resolve_record_times`), the exact-nearest
+    guard (tracking.py:3370), the sampled variables (``mode.sample_vars``)
+    and ``time.num_runs`` realizations, each shifted by
+    ``time.run_interval`` records.  Every record set carries the release
+    positions and time (``release_*``, writers.py:1209)."""
     from .readers import get_reader
     rd = get_reader(reader) if isinstance(reader, str) else reader
     ds = rd.open_dataset(dataset)
     set_exact_nearest(mesh, bool(config.mesh.exact_nearest))
     start, n_steps = resolve_time_params(config.time)
+    file_times, file_dt, dt, avg = resolve_record_times(config, ds, str(config.mesh.type).lower())
+    numerics = config.numerics
+    if numerics.dt is None:
+        import copy
+        numerics = copy.copy(numerics)
+        numerics.dt = dt
+    n = int(ds.time.size)
     out = []
     for r in range(int(config.time.num_runs)):
+        s0 = start + r * int(config.time.run_interval)
+        rec = TrajectoryRecord()
+        rec.meta.update(release_lon=np.asarray(init_state.lon, np.float64).copy(),
+                        release_lat=np.asarray(init_state.lat, np.float64).copy(),
+                        release_z=np.asarray(init_state.z, np.float64).copy(),
+                        release_time=np.full(init_state.n_particles,
+                                             index_to_sim_time(s0, file_times, n, config.time.on_exhausted,
+                                                               dt) if file_times is not None else 0.0),
+                        time_averaging=avg)
         out.append(run_single_realization(
-            mesh, ds, rd, init_state, config.mode, config.numerics,
-            start_time_idx=start + r * int(config.time.run_interval), n_steps=n_steps,
+            mesh, ds, rd, init_state, config.mode, numerics, start_time_idx=s0, n_steps=n_steps,
+            n_time_total=n, file_times=file_times, file_dt=file_dt,
+            sample_vars=list(config.mode.sample_vars or []),
             on_exhausted=config.time.on_exhausted, realization_id=r,
-            particle_offset=int(config.data.particle_offset or 0), ctx=ctx))
+            particle_offset=int(config.data.particle_offset or 0), ctx=ctx, writer=rec))
     return out

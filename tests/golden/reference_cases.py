@@ -481,3 +481,60 @@ def healpix_case(direction):
     return dict(mesh=mesh, ds=ds, init=make_state(lon, lat, z), mode=mode, numerics=numerics,
                 start_time_idx=0 if direction > 0 else HP_NTIME - 1, n_steps=8,
                 n_time_total=HP_NTIME, file_times=np.arange(HP_NTIME) * 86400.0, file_dt=86400.0)
+
+
+MY_YEARS = {1958: 365, 1959: 2}              # tests/_gen_golden_3d.py:750
+MY_FACTOR = {1958: 1.0, 1959: 1.25}          # :689 MULTIYEAR_YEAR
+MY_W_UP = np.array([1.5e-5, 2.25e-5, 3.0e-5])  # :690 MULTIYEAR_W_UP
+
+
+def multiyear_case():
+    """tests/_gen_golden_3d.py:740 _multiyear_config: run_tracking on the
+    FESOM2 unit-square mesh of tests/_fesom2_synth.py:16-24 (4 nodes, 2
+    triangles, interfaces 0/10/30 m, bed 30 m, the FESOM2 Earth radius) over
+    daily records stamped at 23:15 in the files of 1958 (365) and 1959 (2),
+    read as one axis; unod/vnod/temp per layer and node, w on the
+    interfaces (positive up), every record of a year alike (:693
+    _multiyear_field), ssh 0.1, all stored float32; daily means shifted to
+    the window midpoints; RK2, 6-hour steps from record 362 across the year
+    boundary, temp sampled, failure reasons tracked, prefetch 2."""
+    from kelp_amd.config import (AppConfig, DataConfig, MeshConfig, ModeConfig, NumericsConfig,
+                                 TimeConfig)
+    from kelp_amd.mesh import make_mesh
+    from kelp_amd.particles import make_state
+    from kelp_amd.readers import ArrayDataset
+    lon_n, lat_n = np.array([0.0, 1.0, 1.0, 0.0]), np.array([0.0, 0.0, 1.0, 1.0])
+    mesh = make_mesh("fesom2", lon_n, lat_n, np.array([[0, 1, 2], [0, 2, 3]], np.int32),
+                     depth=np.full(4, 30.0), z_levels=np.array([0.0, 10.0, 30.0]),
+                     r_earth=6_367_500.0)
+    x, y = lon_n[None, :], lat_n[None, :]
+
+    def fld(var, year):
+        f = MY_FACTOR[year]
+        if var == "unod":
+            return (np.array([0.06, 0.02])[:, None] + 0.01 * x - 0.008 * y) * f
+        if var == "vnod":
+            return (np.array([0.03, -0.02])[:, None] + 0.006 * x + 0.004 * y) * f
+        if var == "temp":
+            return np.array([12.0, 6.0])[:, None] + 0.5 * x - 0.3 * y + 0.5 * (year - 1958)
+        return np.broadcast_to(MY_W_UP[:, None] * f, (3, 4))
+
+    data = {}
+    for var, dim in (("unod", "nz1"), ("vnod", "nz1"), ("temp", "nz1"), ("w", "nz")):
+        parts = [np.broadcast_to(fld(var, yr).astype(np.float32)[None], (n,) + fld(var, yr).shape)
+                 for yr, n in MY_YEARS.items()]
+        data[var] = (("time", dim, "nod2"), np.ascontiguousarray(np.concatenate(parts)))
+    nt = sum(MY_YEARS.values())
+    data["ssh"] = (("time", "nod2"), np.full((nt, 4), 0.1, np.float32))
+    ds = ArrayDataset(data, time=np.arange(nt) * 86400.0)      # one record a day, from the first
+    seeds = np.array([(0.2, 0.3, 12.3), (0.5, 0.5, 17.6), (0.7, 0.2, 24.1),
+                      (0.3, 0.8, 27.4), (0.72, 0.55, 8.7), (0.98, 0.97, 14.2)])
+    cfg = AppConfig(
+        mode=ModeConfig(Fdrift2D=False, Fdrift3D=True, var_u="unod", var_v="vnod", var_w="w",
+                        var_eta="ssh", sample_vars=["temp"], track_failure_reason=True),
+        numerics=NumericsConfig(dt=21600.0, integrator="rk2", uv_interp="barycentric",
+                                w_interp="barycentric", temporal_interp=True, prefetch_size=2),
+        mesh=MeshConfig(type="fesom2"),
+        data=DataConfig(time_averaging="mean"),
+        time=TimeConfig(start_time_idx=362, n_steps=12))
+    return dict(mesh=mesh, ds=ds, config=cfg, init=make_state(seeds[:, 0], seeds[:, 1], seeds[:, 2]))

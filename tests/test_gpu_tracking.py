@@ -352,3 +352,26 @@ def test_healpix_run_matches_reference_golden(gpu_ctx, direction):
     for k, src in (("lon", "lon"), ("lat", "lat"), ("z", "z"), ("u", "vel_u"), ("v", "vel_v"),
                    ("w", "vel_w")):
         np.testing.assert_allclose(rec[src], gold[k], rtol=1e-9, atol=1e-9, err_msg=k)
+
+
+def test_run_tracking_matches_reference_multiyear_golden(gpu_ctx):
+    """run_tracking(config) end to end (tests/_gen_golden_3d.py:740): two
+    yearly files of daily means read as one axis and shifted to the window
+    midpoints, RK2 across the year boundary, temp sampled at every record,
+    failure reasons tracked, read-ahead of two; every trajectory variable of
+    the reference's output file, its time axis and release metadata."""
+    from kelp_amd.tracking import run_tracking
+    case = rc.multiyear_case()
+    gold = np.load(os.path.join(GOLD, "run_tracking_fesom2_multiyear_node.npz"), allow_pickle=False)
+    (rec,) = run_tracking(case["config"], mesh=case["mesh"], dataset=case["ds"], reader="fesom2_arrays",
+                          init_state=case["init"], ctx=gpu_ctx)
+    out = rec.as_arrays()
+    np.testing.assert_array_equal(out["status"], gold["status"])
+    np.testing.assert_array_equal(out["failure_reason"], gold["failure_reason"])
+    np.testing.assert_array_equal(out["time"], gold["time"])
+    for k in ("release_lon", "release_lat", "release_z", "release_time"):
+        np.testing.assert_array_equal(out[k], gold[k], err_msg=k)
+    assert rec.meta["time_averaging"] == str(gold["attr__time_averaging"])
+    for k, g in (("lon", "lon"), ("lat", "lat"), ("z", "z"), ("vel_u", "vel_u"), ("vel_v", "vel_v"),
+                 ("vel_w", "vel_w"), ("sample__temp", "temp")):
+        np.testing.assert_allclose(out[k], gold[g], rtol=1e-9, atol=1e-9, err_msg=k)
