@@ -69,8 +69,7 @@ struct DMesh {
     const uint8_t* __restrict__ coastal;  // (E,) coastal-element flags (beaching)
     const double2* __restrict__ ecent;    // (E,) element centres (lon, lat); element-layout u/v
     const double* __restrict__ coslat;    // (N,) km_cos(lat * kDeg2Rad) per node
-    const double4* __restrict__ egeo;     // (2E,) containment frame per element, see elem_geom
-    const double2* __restrict__ exy;      // (4E,) projected vertices per element
+    const double4* __restrict__ eframe;   // (4E,) containment frame per element, one 128-byte line each
     const uint8_t* __restrict__ eflags;   // (E,) bit 0: the interpolation ring is the containment ring
     const int4* __restrict__ enbr;        // (E,) neighbour across edge k of the compacted ring, -1 boundary
                                           //   (core/mesh.py:2824 build_elem_neighbors); null without it
@@ -610,12 +609,19 @@ __device__ __forceinline__ bool in_tri(double px, double py, double ax, double a
 // longitude lo0, the bbox of the wrapped longitude deltas (dmin, dmax) and
 // of the latitudes (lamin, lamax), the projection origin of :2007
 // _project_to_local_xy (lon0 seam-gated, lat0, cos(lat0)) and the projected
-// vertices.  Stored as egeo[2c] = (lo0, dmin, dmax, lamin), egeo[2c+1] =
-// (lamax, lon0, lat0, cos_lat0), exy[4c+k] = (x_k, y_k).  A ring of fewer
+// vertices.  Stored in one 128-byte line per element: eframe[4c] = (lo0,
+// dmin, dmax, lamin), eframe[4c+1] = (lamax, lon0, lat0, cos_lat0), then the
+// projected vertices (x_k, y_k) as double2 pairs (frame_xy).  A ring of fewer
 // than three vertices gets an empty latitude box (never contains); a
 // triangle in a four-wide mesh gets a NaN fourth vertex (its second fan
 // triangle never contains).  Every value is the expression the per-query
 // test evaluated, so the test's answers are unchanged bit for bit.
+// the projected vertices of element c's frame: the second half of its line
+__device__ __forceinline__ const double2* frame_xy(const DMesh& m, int64_t c)
+{
+    return reinterpret_cast<const double2*>(m.eframe + 4 * c + 2);
+}
+
 template <int MAXV>
 __device__ inline void elem_geom(const DMesh& m, int c, double4& g0, double4& g1, double2 (&xy)[4],
                                  uint8_t& flags)
@@ -715,12 +721,12 @@ __device__ inline void elem_geom(const DMesh& m, int c, double4& g0, double4& g1
 template <int MAXV>
 __device__ inline bool element_contains(const DMesh& m, int c, double lon_p, double lat_p)
 {
-    const double4 g0 = m.egeo[2 * (int64_t)c];
+    const double4 g0 = m.eframe[4 * (int64_t)c];
     const double dlon_p = km_pymod360(lon_p - g0.x + 180.0) - 180.0;
     if (dlon_p > g0.z || dlon_p < g0.y) return false;
-    const double4 g1 = m.egeo[2 * (int64_t)c + 1];
+    const double4 g1 = m.eframe[4 * (int64_t)c + 1];
     if (lat_p > g1.x || lat_p < g0.w) return false;
-    const double2* xy = m.exy + 4 * (int64_t)c;
+    const double2* xy = frame_xy(m, c);
     const double2 a = xy[0], b = xy[1], d = xy[2];
     const double dlp = km_pymod360(lon_p - g1.y + 180.0) - 180.0;
     const double px = dlp * kDeg2Rad * kREarthStd * g1.w;
@@ -856,8 +862,8 @@ __device__ inline int walk_element(const DMesh& m, double lon_p, double lat_p, i
         double x[4], y[4], px, py;
         if (m.eflags[e] & 1) {
             // the element's precomputed frame is this projection
-            const double4 g1 = m.egeo[2 * (int64_t)e + 1];
-            const double2* xy = m.exy + 4 * (int64_t)e;
+            const double4 g1 = m.eframe[4 * (int64_t)e + 1];
+            const double2* xy = frame_xy(m, e);
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 x[k] = k < nv ? xy[k].x : 0.0;
@@ -1276,8 +1282,8 @@ __device__ inline double w_bary(const DMesh& m, const Snap<T, true>& fw, double 
         double vx[4] = {0, 0, 0, 0}, vy[4] = {0, 0, 0, 0};
         double px, py;
         if (m.eflags[e] & 1) {
-            const double4 g1 = m.egeo[2 * (int64_t)e + 1];
-            const double2* xy = m.exy + 4 * (int64_t)e;
+            const double4 g1 = m.eframe[4 * (int64_t)e + 1];
+            const double2* xy = frame_xy(m, e);
 #pragma unroll
             for (int v = 0; v < MAXV; ++v) {
                 if (v < nv) {

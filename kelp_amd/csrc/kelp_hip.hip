@@ -137,8 +137,7 @@ __global__ void node_xyz_kernel(int64_t n, const double2* __restrict__ lonlat, d
 
 // containment frames of all elements (elem_geom)
 template <int MAXV>
-__global__ void elem_geom_kernel(DMesh m, double4* __restrict__ egeo, double2* __restrict__ exy,
-                                 uint8_t* __restrict__ eflags)
+__global__ void elem_geom_kernel(DMesh m, double4* __restrict__ eframe, uint8_t* __restrict__ eflags)
 {
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= m.n_elems) return;
@@ -147,10 +146,10 @@ __global__ void elem_geom_kernel(DMesh m, double4* __restrict__ egeo, double2* _
     uint8_t fl;
     elem_geom<MAXV>(m, (int)c, g0, g1, xy, fl);
     eflags[c] = fl;
-    egeo[2 * c] = g0;
-    egeo[2 * c + 1] = g1;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) exy[4 * c + k] = xy[k];
+    eframe[4 * c] = g0;
+    eframe[4 * c + 1] = g1;
+    eframe[4 * c + 2] = make_double4(xy[0].x, xy[0].y, xy[1].x, xy[1].y);
+    eframe[4 * c + 3] = make_double4(xy[2].x, xy[2].y, xy[3].x, xy[3].y);
 }
 
 // node unit vectors back in mesh (original) node order, (N, 3)
@@ -588,8 +587,7 @@ int kelp_mesh_upload(kelp_ctx* c, const kelp_mesh_desc* d)
     double *d_depth, *d_zlev, *d_coslat;
     uint8_t* d_coastal = nullptr;
     double2* d_ecent = nullptr;
-    double4* d_egeo = nullptr;
-    double2* d_exy = nullptr;
+    double4* d_eframe = nullptr;
     uint8_t* d_eflags = nullptr;
     double* d_sig_eta = nullptr;
     int4* d_enbr = nullptr;
@@ -601,8 +599,8 @@ int kelp_mesh_upload(kelp_ctx* c, const kelp_mesh_desc* d)
         (rc = dev_alloc(c, &d_nodeo, N)) || (rc = dev_alloc(c, &d_noden, N)) ||
         (rc = dev_alloc(c, &d_elemo, E)) || (rc = dev_alloc(c, &d_elemn, E)) ||
         (d->host_elem_coastal && (rc = dev_alloc(c, &d_coastal, E))) ||
-        (has_centres && (rc = dev_alloc(c, &d_ecent, E))) || (rc = dev_alloc(c, &d_egeo, 2 * E)) ||
-        (rc = dev_alloc(c, &d_exy, 4 * E)) || (rc = dev_alloc(c, &d_eflags, E)) ||
+        (has_centres && (rc = dev_alloc(c, &d_ecent, E))) || (rc = dev_alloc(c, &d_eframe, 4 * E)) ||
+        (rc = dev_alloc(c, &d_eflags, E)) ||
         (d->vertical_coord == 1 && (rc = dev_alloc(c, &d_sig_eta, 5 * N))) ||
         (d->host_elem_neighbors && (rc = dev_alloc(c, &d_enbr, E)))) {
         free_mesh(c);
@@ -638,8 +636,7 @@ int kelp_mesh_upload(kelp_ctx* c, const kelp_mesh_desc* d)
     m.coastal = d_coastal;
     m.ecent = d_ecent;
     m.coslat = d_coslat;
-    m.egeo = d_egeo;
-    m.exy = d_exy;
+    m.eframe = d_eframe;
     m.eflags = d_eflags;
     m.enbr = d_enbr;
     m.cell = d->gh_cell;
@@ -655,9 +652,9 @@ int kelp_mesh_upload(kelp_ctx* c, const kelp_mesh_desc* d)
     m.max_v = (int32_t)d->max_v;
     m.n_levels = (int32_t)d->n_levels;
     if (d->max_v == 3)
-        elem_geom_kernel<3><<<grid_for(E, 256), 256, 0, c->stream>>>(m, d_egeo, d_exy, d_eflags);
+        elem_geom_kernel<3><<<grid_for(E, 256), 256, 0, c->stream>>>(m, d_eframe, d_eflags);
     else
-        elem_geom_kernel<4><<<grid_for(E, 256), 256, 0, c->stream>>>(m, d_egeo, d_exy, d_eflags);
+        elem_geom_kernel<4><<<grid_for(E, 256), 256, 0, c->stream>>>(m, d_eframe, d_eflags);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->m = m;
