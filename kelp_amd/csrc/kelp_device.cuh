@@ -622,6 +622,19 @@ __device__ __forceinline__ const double2* frame_xy(const DMesh& m, int64_t c)
     return reinterpret_cast<const double2*>(m.eframe + 4 * c + 2);
 }
 
+// Element e's vertex ids (elems[e]).  On a triangle mesh the frame's fourth
+// vertex slot is free and holds them, so the interpolation after a
+// containment test reads the line that test has just brought in instead of
+// a second one.
+template <int MAXV>
+__device__ __forceinline__ int4 elem_nodes(const DMesh& m, int64_t e)
+{
+    if constexpr (MAXV == 3)
+        return *reinterpret_cast<const int4*>(reinterpret_cast<const char*>(m.eframe + 4 * e + 3) + 16);
+    else
+        return m.elems[e];
+}
+
 template <int MAXV>
 __device__ inline void elem_geom(const DMesh& m, int c, double4& g0, double4& g1, double2 (&xy)[4],
                                  uint8_t& flags)
@@ -856,7 +869,7 @@ __device__ inline int walk_element(const DMesh& m, double lon_p, double lat_p, i
     int prev = -1;
     for (int it = 0; it < kWalkMaxSteps; ++it) {
         int v[4] = {0, 0, 0, 0};
-        const int4 en = m.elems[e];
+        const int4 en = elem_nodes<MAXV>(m, e);
         const int nv = interp_ring<MAXV>(en, v);
         if (nv < 3) return -1;
         double x[4], y[4], px, py;
@@ -962,7 +975,7 @@ __device__ inline bool vertex_weights(const DMesh& m, int e, double xp, double y
 {
     W.v[0] = W.v[1] = W.v[2] = W.v[3] = 0;
     W.w[0] = W.w[1] = W.w[2] = W.w[3] = 0.0;
-    const int nv = interp_ring<MAXV>(m.elems[e], W.v);
+    const int nv = interp_ring<MAXV>(elem_nodes<MAXV>(m, e), W.v);
     W.n = nv;
     if (nv < 3) return false;
     const double R = m.r_earth;
@@ -1277,7 +1290,7 @@ __device__ inline double w_bary(const DMesh& m, const Snap<T, true>& fw, double 
     for (int k = k0; k < k1; ++k) {
         const int e = m.n2e[k];
         int verts[4] = {0, 0, 0, 0};
-        const int nv = interp_ring<MAXV>(m.elems[e], verts);
+        const int nv = interp_ring<MAXV>(elem_nodes<MAXV>(m, e), verts);
         if (nv < 3) continue;
         double vx[4] = {0, 0, 0, 0}, vy[4] = {0, 0, 0, 0};
         double px, py;

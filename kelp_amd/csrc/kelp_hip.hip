@@ -149,7 +149,14 @@ __global__ void elem_geom_kernel(DMesh m, double4* __restrict__ eframe, uint8_t*
     eframe[4 * c] = g0;
     eframe[4 * c + 1] = g1;
     eframe[4 * c + 2] = make_double4(xy[0].x, xy[0].y, xy[1].x, xy[1].y);
-    eframe[4 * c + 3] = make_double4(xy[2].x, xy[2].y, xy[3].x, xy[3].y);
+    double4 last = make_double4(xy[2].x, xy[2].y, xy[3].x, xy[3].y);
+    if (MAXV == 3) {
+        // no fourth vertex: the slot carries the vertex ids (elem_nodes)
+        const int4 en = m.elems[c];
+        last.z = __hiloint2double(en.y, en.x);
+        last.w = __hiloint2double(en.w, en.z);
+    }
+    eframe[4 * c + 3] = last;
 }
 
 // node unit vectors back in mesh (original) node order, (N, 3)
