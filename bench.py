@@ -60,7 +60,8 @@ class Global3D:
     row 28 B + 3 vertices x 20 B of ids/coordinates) + 4 stages x 104 B (u, v
     at 3 vertices x 2 snapshots x 4 B = 48 B, W at 3 vertices x 2 interfaces x
     2 snapshots x 4 B = 48 B, 8 B node depth) + the kick's anchor and element
-    96 B + beaching 8 B = 1049 B.
+    96 B + beaching 8 B = 1049 B.  With bfloat16 snapshots (``--storage
+    bf16``) the field values are 2 B: 4 stages x 56 B, 857 B in all.
     """
     name = "global_3d_rk4_kh_beach"
     b_alg = 1049.0
@@ -79,6 +80,10 @@ class Global3D:
         self.beach = dict(criteria=["on_failure", "stochastic"], t_beach=86400.0,
                           depth_min=0.3, seed=20240501)
         self.sample_ts = not args.no_sample
+        self.storage = getattr(args, "storage", "f32")
+        if self.storage == "bf16":
+            self.b_alg = 857.0
+            self.profile = None         # the profiled traffic is the float32 workload's
         # snapshot ring: `ring` slots, the next snapshot refilled
         # asynchronously while steps run (or synchronously at the bracket
         # change with --ring-sync); all snapshots resident when 0
@@ -93,7 +98,7 @@ class Global3D:
                        "mesh_elements": self.w.mesh.n_elements, "z_levels": int(self.w.mesh.sigma.size),
                        "particles": int(self.w.lon.size), "integrator": "rk4", "dt_s": self.w.dt,
                        "snapshots": len(self.w.snaps), "snapshot_dt_s": self.w.snap_dt,
-                       "field_dtype": "f32", "kh_m2_s": self.w.kh, "w_interp": "barycentric",
+                       "field_dtype": self.storage, "kh_m2_s": self.w.kh, "w_interp": "barycentric",
                        "beaching": self.beach["criteria"], "steps_per_run": self.w.n_steps,
                        "sampled": ["T", "S"] if self.sample_ts else []}
         if self.ring:
@@ -111,7 +116,7 @@ class Global3D:
                 self._ensure(ctx, e, sync=True)
             return ctx.particles(self.w.lon, self.w.lat, self.w.z)
         for s, (U, V, W, T, S) in enumerate(self.w.snaps):
-            ctx.upload_field(s, U, V, W)
+            ctx.upload_field(s, U, V, W, storage="bf16" if self.storage == "bf16" else None)
             if self.sample_ts:
                 ctx.upload_scalar(s, 0, T)
                 ctx.upload_scalar(s, 1, S)
@@ -128,10 +133,11 @@ class Global3D:
         if self.resident.get(s) == e:
             return
         U, V, W = self.pin[e % len(self.w.snaps)]
+        st = "bf16" if self.storage == "bf16" else None
         if sync:
-            ctx.upload_field(s, U, V, W)
+            ctx.upload_field(s, U, V, W, storage=st)
         else:
-            ctx.upload_field_async(s, U, V, W)
+            ctx.upload_field_async(s, U, V, W, storage=st)
         self.resident[s] = e
 
     def step(self, ctx, p, k, ev=None):
@@ -259,6 +265,8 @@ def main():
                          "snapshots through them (no T/S sampling)")
     ap.add_argument("--ring-sync", action="store_true",
                     help="with --ring: refill at the bracket change, synchronously")
+    ap.add_argument("--storage", default="f32", choices=["f32", "bf16"],
+                    help="snapshot storage of the 3-D workload (bf16: rounded on upload)")
     ap.add_argument("--no-sample", action="store_true",
                     help="global_3d: leave out the T/S sampling at each step's end")
     ap.add_argument("--copy-probe-mib", type=int, default=0,

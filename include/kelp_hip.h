@@ -31,6 +31,11 @@ extern "C" {
 
 #define KELP_DT_F64 0
 #define KELP_DT_F32 1
+/* bfloat16 storage of node-layout z-level snapshots (kelp_field_upload*: the
+ * host planes are float32, rounded to nearest even on the device); read
+ * back exactly as float32 values.  No counterpart in the reference, which
+ * keeps float32 or float64 snapshots: halves the records' HBM footprint. */
+#define KELP_DT_BF16 2
 
 typedef struct kelp_ctx kelp_ctx;
 

@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libkelp_hip
 KELP_OK = 0
 KELP_DT_F64 = 0
 KELP_DT_F32 = 1
+KELP_DT_BF16 = 2
 
 
 class KelpError(RuntimeError):

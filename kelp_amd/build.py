@@ -36,6 +36,9 @@ UNITS = [("kelp_hip", "kelp_hip.hip", []), ("kelp_sort", "kelp_sort.hip", []),
     (f"kelp_step_{'3d' if d3 else '2d'}_{'f32' if f32 else 'f64'}_{'elem' if el else 'node'}",
      "kelp_step_variant.hip", [f"-DKV_IS3D={d3}", f"-DKV_F32={f32}", f"-DKV_ELEM={el}"])
     for d3 in (0, 1) for f32 in (0, 1) for el in (0, 1)
+] + [
+    (f"kelp_step_{'3d' if d3 else '2d'}_bf16_node", "kelp_step_variant.hip",
+     [f"-DKV_IS3D={d3}", "-DKV_F32=0", "-DKV_BF16=1", "-DKV_ELEM=0"]) for d3 in (0, 1)
 ]
 
 

@@ -1050,6 +1050,40 @@ struct RecOf<double, false> {
     using type = double2;
 };
 
+// bfloat16 storage: the upper half of an IEEE float32, widened exactly
+// (through float) to double wherever a record value is read, so the
+// arithmetic downstream is that of a float32 field holding the same values
+struct Bf16 {
+    uint16_t bits;
+    __device__ __forceinline__ operator double() const
+    {
+        return (double)__uint_as_float((uint32_t)bits << 16);
+    }
+};
+struct __align__(8) Bf16x4 {
+    Bf16 x, y, z, w;
+};
+struct __align__(4) Bf16x2 {
+    Bf16 x, y;
+};
+template <>
+struct RecOf<Bf16, true> {
+    using type = Bf16x4;
+};
+template <>
+struct RecOf<Bf16, false> {
+    using type = Bf16x2;
+};
+
+// float32 -> bfloat16, round to nearest even (a NaN stays a quiet NaN)
+__device__ __forceinline__ Bf16 to_bf16(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return Bf16{(uint16_t)((u >> 16) | 0x40u)};
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return Bf16{(uint16_t)(u >> 16)};
+}
+
 // bracket pair (a, b) blended by alpha0 and the stage target t
 template <typename T, bool IS3D>
 struct Snap {

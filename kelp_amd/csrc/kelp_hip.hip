@@ -203,6 +203,35 @@ __global__ void pack2d_kernel(int64_t N, const int32_t* __restrict__ old_of_new,
     rec[k] = r;
 }
 
+// float32 planes into bfloat16 records (KELP_DT_BF16), 3-D and 2-D
+__global__ void pack3d_bf16_kernel(int64_t N, int64_t n_layers, const int32_t* __restrict__ old_of_new,
+                                   const float* __restrict__ u, const float* __restrict__ v,
+                                   const float* __restrict__ w, Bf16x4* __restrict__ rec)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N * n_layers) return;
+    int64_t L = i / N, k = i - L * N;
+    int64_t o = old_of_new[k];
+    Bf16x4 r;
+    r.x = to_bf16(u[L * N + o]);
+    r.y = to_bf16(v[L * N + o]);
+    r.z = to_bf16(w[L * N + o]);
+    r.w = to_bf16(w[(L + 1) * N + o]);
+    rec[i] = r;
+}
+
+__global__ void pack2d_bf16_kernel(int64_t N, const int32_t* __restrict__ old_of_new, const float* __restrict__ u,
+                                   const float* __restrict__ v, Bf16x2* __restrict__ rec)
+{
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    int64_t o = old_of_new[k];
+    Bf16x2 r;
+    r.x = to_bf16(u[o]);
+    r.y = to_bf16(v[o]);
+    rec[k] = r;
+}
+
 // element-centre u/v planes (mesh element order) into (u, v) records per
 // (layer, element) in internal element order
 template <typename T>
@@ -779,6 +808,16 @@ static hipError_t launch_pack(kelp_ctx* c, hipStream_t stream, FieldSlot& s, int
     // node records: u/v (node layout) and the W pair (3-D)
     const void* nu = layout == 0 ? tu : nullptr;
     const void* nv = layout == 0 ? tv : nullptr;
+    if (dtype == KELP_DT_BF16) {  // node layout only (checked by the callers)
+        if (is3d)
+            pack3d_bf16_kernel<<<grid_for(N * n_planes, 256), 256, 0, stream>>>(
+                N, n_planes, c->node_old_of_new, (const float*)tu, (const float*)tv, (const float*)tw,
+                (Bf16x4*)s.rec);
+        else
+            pack2d_bf16_kernel<<<grid_for(N, 256), 256, 0, stream>>>(N, c->node_old_of_new, (const float*)tu,
+                                                                     (const float*)tv, (Bf16x2*)s.rec);
+        return hipGetLastError();
+    }
     if (is3d) {
         const unsigned g = grid_for(N * n_planes, 256);
         if (f32)
@@ -816,7 +855,10 @@ static int field_upload(kelp_ctx* c, const char* fn, int slot, int dtype, int la
     if (!c) return fail(KELP_EINVAL, "%s: NULL context", fn);
     if (!c->has_mesh) return fail(KELP_ESTATE, "%s: upload the mesh first", fn);
     if (slot < 0 || slot >= kMaxSlots) return fail(KELP_EINVAL, "%s: slot %d not in [0, %d)", fn, slot, kMaxSlots);
-    if (dtype != KELP_DT_F64 && dtype != KELP_DT_F32) return fail(KELP_EINVAL, "%s: bad dtype %d", fn, dtype);
+    if (dtype != KELP_DT_F64 && dtype != KELP_DT_F32 && dtype != KELP_DT_BF16)
+        return fail(KELP_EINVAL, "%s: bad dtype %d", fn, dtype);
+    if (dtype == KELP_DT_BF16 && (layout != 0 || c->vertical_coord == 1))
+        return fail(KELP_EINVAL, "%s: bfloat16 storage is for node-layout z-level snapshots", fn);
     if (n_planes < 1 || !hu || !hv) return fail(KELP_EINVAL, "%s: u/v missing", fn);
     const bool is3d = hw != nullptr;
     if (is3d && n_w_planes != n_planes + 1) return fail(KELP_EINVAL, "%s: W needs n_planes + 1 interface planes", fn);
@@ -830,12 +872,13 @@ static int field_upload(kelp_ctx* c, const char* fn, int slot, int dtype, int la
     free_slot(s);
     const int64_t N = c->m.n_nodes, E = c->m.n_elems;
     const int64_t M = layout == 1 ? E : N;  // u/v values per plane
-    const size_t es = dtype == KELP_DT_F64 ? 8 : 4;
+    const size_t es = dtype == KELP_DT_F64 ? 8 : 4;                     // host values
+    const size_t vs = dtype == KELP_DT_F64 ? 8 : dtype == KELP_DT_F32 ? 4 : 2;  // stored values
     const size_t uv_bytes = (size_t)n_planes * M * es;
     const size_t w_bytes = is3d ? (size_t)n_w_planes * N * es : 0;
     void *tu = nullptr, *tv = nullptr, *tw = nullptr;
     hipError_t e = hipSuccess;
-    if (is3d || layout == 0) e = hipMalloc(&s.rec, (size_t)n_planes * N * es * (is3d ? 4 : 2));
+    if (is3d || layout == 0) e = hipMalloc(&s.rec, (size_t)n_planes * N * vs * (is3d ? 4 : 2));
     if (e == hipSuccess && layout == 1) e = hipMalloc(&s.erec, (size_t)n_planes * E * es * 2);
     if (e == hipSuccess) e = hipMalloc(&tu, uv_bytes);
     if (e == hipSuccess) e = hipMalloc(&tv, uv_bytes);
@@ -996,6 +1039,8 @@ int kelp_field_reconstruct(kelp_ctx* c, int dst, const int32_t* src, const doubl
         if (src[k] == dst) return fail(KELP_EINVAL, "kelp_field_reconstruct: the destination is a source");
     }
     const FieldSlot& s0 = c->slots[src[0]];
+    if (s0.dtype != KELP_DT_F32 && s0.dtype != KELP_DT_F64)
+        return fail(KELP_EINVAL, "kelp_field_reconstruct: the snapshots must be stored as float32 or float64");
     for (int k = 1; k < 4; ++k) {
         const FieldSlot& sk = c->slots[src[k]];
         if (sk.dtype != s0.dtype || sk.layout != s0.layout || sk.n_planes != s0.n_planes ||
@@ -1273,10 +1318,18 @@ int kelp_step(kelp_ctx* c, const kelp_particles* p, const kelp_step_args* a)
     }
     PState ps{p->lon, p->lat, p->z, p->status, p->nn, p->fr, p->u, p->v, p->w, p->pid, p->wet_streak,
               reinterpret_cast<unsigned long long*>(p->alive_count), p->n, p->elem};
-    if (sa.dtype == KELP_DT_F32)
+    if (sa.dtype == KELP_DT_BF16) {
+        // node-layout z-level only (refused at upload otherwise)
+        if (sigma) return fail(KELP_EINVAL, "kelp_step: bfloat16 snapshots on a sigma mesh");
+        if (is3d)
+            launch_step_t<true, Bf16, false>(c, a->order, sa, sb, st, cfg, ps);
+        else
+            launch_step_t<false, Bf16, false>(c, a->order, sa, sb, st, cfg, ps);
+    } else if (sa.dtype == KELP_DT_F32) {
         launch_step<float>(c, a->order, is3d, sa, sb, st, cfg, ps);
-    else
+    } else {
         launch_step<double>(c, a->order, is3d, sa, sb, st, cfg, ps);
+    }
     HIP_TRY(hipGetLastError());
     return KELP_OK;
 }

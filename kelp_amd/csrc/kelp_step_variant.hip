@@ -1,6 +1,7 @@
 // kelp_step_variant.hip -- the step kernels of one (3-D, field dtype, u/v
-// layout) combination, chosen by -DKV_IS3D=0|1 -DKV_F32=0|1 -DKV_ELEM=0|1;
-// the build compiles this file once per combination, in parallel.
+// layout) combination, chosen by -DKV_IS3D=0|1 -DKV_F32=0|1 -DKV_ELEM=0|1
+// (-DKV_BF16=1: bfloat16 records, node layout); the build compiles this file
+// once per combination, in parallel.
 #include "kelp_step_impl.cuh"
 
 #if !defined(KV_IS3D) || !defined(KV_F32) || !defined(KV_ELEM)
@@ -9,7 +10,9 @@
 
 namespace kelp {
 
-#if KV_F32
+#if defined(KV_BF16) && KV_BF16
+using KvT = Bf16;
+#elif KV_F32
 using KvT = float;
 #else
 using KvT = double;

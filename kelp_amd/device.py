@@ -14,7 +14,7 @@ from typing import Dict, Optional
 import numpy as np
 
 from . import _native
-from ._native import KELP_DT_F32, KELP_DT_F64, MeshDesc, Particles, StepArgs, check
+from ._native import KELP_DT_BF16, KELP_DT_F32, KELP_DT_F64, MeshDesc, Particles, StepArgs, check
 from .mesh import Mesh
 
 
@@ -278,15 +278,21 @@ class DeviceContext:
                                         out.data_ptr()), "kelp_sample_vars")
         return out
 
-    def upload_field(self, slot: int, u, v, w=None, key=None, uv_layout="node") -> None:
+    def upload_field(self, slot: int, u, v, w=None, key=None, uv_layout="node", storage=None) -> None:
         """Upload one snapshot into ``slot``; skipped when ``key`` matches the
         slot's current content.  ``uv_layout``: "node" (u, v per node) or
-        "element" (u, v per element centre, interpolated by least squares)."""
+        "element" (u, v per element centre, interpolated by least squares).
+        ``storage="bf16"``: the values are rounded to bfloat16 on the device
+        (node layout, z-level meshes); else stored in their own dtype."""
         if key is not None and self._slot_keys.get(slot) is key:
             return
         u = np.asarray(u)
         dt = KELP_DT_F32 if u.dtype == np.float32 else KELP_DT_F64
-        npd = np.float32 if dt == KELP_DT_F32 else np.float64
+        if storage == "bf16":
+            dt = KELP_DT_BF16
+        elif storage not in (None, "native"):
+            raise ValueError(f"unknown storage {storage!r}")
+        npd = np.float64 if dt == KELP_DT_F64 else np.float32
         u = np.ascontiguousarray(u, npd)
         v = np.ascontiguousarray(v, npd)
         n_planes = 1 if u.ndim == 1 else u.shape[0]
@@ -313,7 +319,8 @@ class DeviceContext:
         t = self.torch
         return t.from_numpy(np.ascontiguousarray(a)).pin_memory()
 
-    def upload_field_async(self, slot: int, u, v, w=None, eta=None, uv_layout="node", key=None) -> None:
+    def upload_field_async(self, slot: int, u, v, w=None, eta=None, uv_layout="node", key=None,
+                           storage=None) -> None:
         """Refill ``slot`` (already holding a snapshot of this shape) on the
         copy stream while steps run (kelp_field_upload_async).  Arrays:
         numpy or pinned torch CPU tensors (see :meth:`pinned`); they are kept
@@ -329,6 +336,7 @@ class DeviceContext:
 
         u = host(u)
         f32 = (u.dtype == t.float32) if isinstance(u, t.Tensor) else (u.dtype == np.float32)
+        f32 = f32 or storage == "bf16"
         npd = np.float32 if f32 else np.float64
         u, v, w = host(u, npd), host(v, npd), host(w, npd)
         e = host(eta, np.float64)
@@ -341,7 +349,8 @@ class DeviceContext:
         if slot in keep:
             check(self.lib.kelp_field_upload_wait(self.h, int(slot)), "kelp_field_upload_wait")
         layout = 0 if uv_layout == "node" else 1
-        check(self.lib.kelp_field_upload_async(self.h, int(slot), KELP_DT_F32 if f32 else KELP_DT_F64,
+        code = KELP_DT_BF16 if storage == "bf16" else KELP_DT_F32 if f32 else KELP_DT_F64
+        check(self.lib.kelp_field_upload_async(self.h, int(slot), code,
                                                layout, n_planes, ptr(u), ptr(v), ptr(w), nw, ptr(e)),
               "kelp_field_upload_async")
         keep[slot] = (u, v, w, e)

@@ -402,3 +402,14 @@ def pchip4(f0, f1, f2, f3, times, tq, flavour="glibc"):
     L.or_pchip4(out.size, *[a.ctypes.data for a in arrs], *[float(t) for t in times], float(tq),
                 out.ctypes.data)
     return out
+
+
+def bf16_round(a) -> np.ndarray:
+    """float32 -> bfloat16 -> float32, round to nearest even (IEEE 754
+    roundTiesToEven on the top 16 bits of the float32 pattern; a NaN stays a
+    quiet NaN): the values a bfloat16 snapshot holds, as float32 arrays."""
+    u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    nan = (u & 0x7FFFFFFF) > 0x7F800000
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) & 0xFFFF
+    r = np.where(nan, (u >> 16) | 0x40, r)
+    return (r.astype(np.uint32) << 16).view(np.float32)

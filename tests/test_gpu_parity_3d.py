@@ -246,3 +246,93 @@ def test_3d_az_field_kick_bitwise(gpu_ctx, world, order, direction):
     h = p.to_host()
     assert (st["status"] == 1).mean() > 0.8
     _assert_bitwise(h, st)
+
+
+def _bf16_world(world):
+    from oracle.oracle import bf16_round
+    mesh, snaps, lon, lat, z, pid = world
+    q = [tuple(bf16_round(a.astype(np.float32)) for a in sn[:3]) for sn in snaps]
+    return q
+
+
+@pytest.mark.parametrize("order,direction", [(4, 1.0), (2, -1.0), (1, 1.0)])
+def test_3d_bf16_storage_bitwise(gpu_ctx, world, order, direction):
+    """bfloat16 snapshots (KELP_DT_BF16: float32 planes rounded to nearest
+    even on the device, widened exactly on every read) against the oracle
+    fed the same rounded values as float32: bit-identical, through a time
+    blend, the Kh walk and beaching, twelve steps."""
+    mesh, snaps, lon, lat, z, pid = world
+    q = _bf16_world(world)
+    gpu_ctx.upload_mesh(mesh)
+    for s, sn in enumerate(snaps):
+        gpu_ctx.upload_field(s, sn[0].astype(np.float32), sn[1].astype(np.float32), sn[2].astype(np.float32),
+                             storage="bf16")
+    p = gpu_ctx.particles(lon, lat, z, pid=pid)
+    om = OracleMesh(mesh, "kmath")
+    st = dict(lon=lon, lat=lat, z=z, status=np.ones(lon.size, np.int32),
+              nn=np.full(lon.size, -1, np.int32), pid=pid)
+    kick = dict(kh_x=10.0, kh_y=10.0, seed=17)
+    t0 = 0.0 if direction > 0 else 2 * synth.SNAP_DT
+    for k in range(12):
+        f, (ia, ib, it, a0, dtn) = _fields(q, t0 + direction * k * DT, direction, np.float32)
+        c = (0.0, 0.5, 0.5, 1.0) if order == 4 else (0.0, 0.5) if order == 2 else (0.0,)
+        al = [ck * DT / dtn for ck in c]
+        st = om.step(st, f, step_args(order, DT, direction, is3d=True, w_method=2, alphas=al, kick=kick,
+                                      beach=BEACH, step_i=k))
+        gpu_ctx.step(p, order, ia, DT, direction, slot_b=ib, slot_t=it, alpha0=a0, alphas=al, w_method=2,
+                     kick=kick, beach=BEACH, step_i=k)
+    gpu_ctx.synchronize()
+    assert (st["status"] == 1).mean() > 0.6
+    _assert_bitwise(p.to_host(), st)
+
+
+def test_2d_bf16_storage_bitwise(gpu_ctx, world):
+    mesh, snaps, lon, lat, z, pid = world
+    q = _bf16_world(world)
+    gpu_ctx.upload_mesh(mesh)
+    gpu_ctx.upload_field(0, snaps[0][0][0].astype(np.float32), snaps[0][1][0].astype(np.float32), storage="bf16")
+    p = gpu_ctx.particles(lon, lat, z, pid=pid)
+    om = OracleMesh(mesh, "kmath")
+    st = dict(lon=lon, lat=lat, z=z, status=np.ones(lon.size, np.int32),
+              nn=np.full(lon.size, -1, np.int32), pid=pid)
+    for k in range(10):
+        st = om.step(st, dict(u_a=q[0][0][0], v_a=q[0][1][0], alpha0=0.0),
+                     step_args(2, DT, 1.0, is3d=False, step_i=k))
+        gpu_ctx.step(p, 2, 0, DT, 1.0, step_i=k)
+    gpu_ctx.synchronize()
+    h = p.to_host()
+    for key in ("lon", "lat", "u", "v"):
+        assert np.array_equal(h[key].view(np.int64), st[key].view(np.int64)), key
+    for key in ("status", "nn", "fr"):
+        assert np.array_equal(h[key], st[key]), key
+
+
+def test_bf16_storage_error_against_float32(gpu_ctx, world):
+    """What bfloat16 storage costs in position: the same 24-step RK4 run from
+    float32 and from bfloat16 snapshots (no kicks, so only the field
+    differs); the great-circle separation stays far below one mesh spacing
+    (about 150 km here) and the run is not identical."""
+    mesh, snaps, lon, lat, z, pid = world
+    gpu_ctx.upload_mesh(mesh)
+    ends = []
+    for storage in (None, "bf16"):
+        for s, sn in enumerate(snaps):
+            gpu_ctx.upload_field(s, sn[0].astype(np.float32), sn[1].astype(np.float32), sn[2].astype(np.float32),
+                                 storage=storage)
+        p = gpu_ctx.particles(lon, lat, z, pid=pid)
+        for k in range(24):
+            f, (ia, ib, it, a0, dtn) = _fields(snaps, k * DT, 1.0, np.float32)
+            gpu_ctx.step(p, 4, ia, DT, 1.0, slot_b=ib, slot_t=it, alpha0=a0,
+                         alphas=[ck * DT / dtn for ck in (0.0, 0.5, 0.5, 1.0)], w_method=2, step_i=k)
+        gpu_ctx.synchronize()
+        ends.append(p.to_host())
+    a, b = ends
+    ok = (a["status"] == 1) & (b["status"] == 1)
+    assert ok.mean() > 0.8
+    la1, la2 = np.radians(a["lat"][ok]), np.radians(b["lat"][ok])
+    dlon = np.radians(a["lon"][ok] - b["lon"][ok])
+    d = 6371e3 * np.arccos(np.clip(np.sin(la1) * np.sin(la2) + np.cos(la1) * np.cos(la2) * np.cos(dlon), -1, 1))
+    print(f"bf16 vs f32 after 24 RK4 steps: median {np.median(d):.2f} m, p99 {np.percentile(d, 99):.2f} m, "
+          f"max {d.max():.2f} m; depth p99 {np.percentile(np.abs(a['z'][ok] - b['z'][ok]), 99):.4f} m")
+    assert d.max() > 0.0
+    assert np.percentile(d, 99) < 5000.0

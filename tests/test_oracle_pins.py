@@ -424,3 +424,28 @@ def test_pchip_is_scipy_pchip_on_the_central_interval(flavour):
     # monotone data stay within the bracket values
     mid = pchip4(f[0, :50], f[1, :50], f[2, :50], f[3, :50], times, 2900.0, flavour)
     assert np.all((mid >= f[1, :50]) & (mid <= f[2, :50]))
+
+
+def test_bf16_round_is_round_to_nearest_even():
+    """The bfloat16 rounding the snapshot upload applies, against a value
+    restatement: of the two bfloat16 neighbours of x (its float32 pattern
+    truncated to 16 bits, and the next one up in magnitude) the nearer in
+    exact float64 arithmetic, a tie going to the even pattern; overflow past
+    the largest bfloat16 to infinity."""
+    from oracle.oracle import bf16_round
+    rng = np.random.default_rng(8)
+    x = np.concatenate([rng.standard_normal(20000) * 10.0 ** rng.integers(-30, 30, 20000),
+                        [1.0 + 2.0 ** -8, 1.0 + 3 * 2.0 ** -8, -(1.0 + 2.0 ** -8), 0.0, -0.0,
+                         3.3e38, 1e-39]]).astype(np.float32)
+    got = bf16_round(x)
+    bits = x.view(np.uint32)
+    lo = (bits & 0xFFFF0000).view(np.float32).astype(np.float64)
+    hi_bits = (bits & 0xFFFF0000) + 0x10000
+    hi = hi_bits.view(np.float32).astype(np.float64)
+    xd = x.astype(np.float64)
+    dl, dh = np.abs(xd - lo), np.abs(hi - xd)
+    even_lo = ((bits >> 16) & 1) == 0
+    want = np.where(dl < dh, lo, np.where(dh < dl, hi, np.where(even_lo, lo, hi)))
+    np.testing.assert_array_equal(got.astype(np.float64), want)
+    assert np.array_equal(bf16_round(np.array([np.inf, -np.inf], np.float32)), [np.inf, -np.inf])
+    assert np.isnan(bf16_round(np.array([np.nan], np.float32)))[0]
