@@ -85,6 +85,7 @@ static void free_slot(FieldSlot& s)
     if (s.erec) (void)hipFree(s.erec);
     if (s.eta) (void)hipFree(s.eta);
     if (s.az) (void)hipFree(s.az);
+    if (s.pair) (void)hipFree(s.pair);
     for (auto& f : s.scalars) kelp_free_scalar(f);
     for (void* p : s.stage)
         if (p) (void)hipFree(p);

@@ -48,6 +48,10 @@ struct FieldSlot {
     void* erec = nullptr;
     double* eta = nullptr;  // (N,) surface elevation, internal node order, or null
     double* az = nullptr;   // Az at the interfaces, one column of az_k per node, or null
+    // two mid-layer scalars interleaved per (node, layer) for paired
+    // sampling, built on first use; pair_vars = {-1, -1} when stale
+    double* pair = nullptr;
+    int pair_vars[2] = {-1, -1};
     int64_t az_k = 0;       // 0: no Az for the slot's current snapshot
     ScalarField scalars[kMaxScalars];
     // asynchronous refill (kelp_field_upload_async): device staging of the

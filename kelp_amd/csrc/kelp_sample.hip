@@ -16,17 +16,21 @@ using namespace kelp;
 
 namespace {
 
+// value idx of a field is a[idx * stride] (stride 2: one variable of an
+// interleaved pair, kelp_sample_vars)
 struct SField {
     const double* __restrict__ a;
     const double* __restrict__ b;
     double alpha;
     int64_t n_planes;
+    int64_t stride = 1;
 };
 
 __device__ __forceinline__ double sval(const SField& s, int64_t idx)
 {
-    if (s.alpha == 0.0) return s.a[idx];
-    return (1.0 - s.alpha) * s.a[idx] + s.alpha * s.b[idx];
+    const int64_t j = idx * s.stride;
+    if (s.alpha == 0.0) return s.a[j];
+    return (1.0 - s.alpha) * s.a[j] + s.alpha * s.b[j];
 }
 
 // The bracket of depth z among the mid-layer depths 0.5 * (z_k + z_k+1) in
@@ -407,6 +411,37 @@ __global__ void pair_records_kernel(int64_t N, int64_t n_layers, const double* _
 
 unsigned grid_of(int64_t n) { return (unsigned)((n + 255) / 256); }
 
+__global__ void interleave_kernel(int64_t n, const double* __restrict__ a, const double* __restrict__ b,
+                                  double* __restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[2 * i] = a[i];
+    out[2 * i + 1] = b[i];
+}
+
+// The slot's two mid-layer variables va, vb interleaved per (node, layer),
+// so one line holds both at the two layers a depth brackets; built once per
+// upload of either variable.  Null when it cannot be allocated (the caller
+// then reads the separate columns).
+double* slot_pair(kelp_ctx* c, FieldSlot& fs, int va, int vb)
+{
+    const ScalarField& f0 = fs.scalars[va];
+    const ScalarField& f1 = fs.scalars[vb];
+    if (fs.pair && fs.pair_vars[0] == va && fs.pair_vars[1] == vb) return fs.pair;
+    const int64_t n = c->m.n_nodes * f0.n_planes;
+    if (!fs.pair && hipMalloc(&fs.pair, (size_t)n * 2 * sizeof(double)) != hipSuccess) {
+        fs.pair = nullptr;
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    interleave_kernel<<<grid_of(n), 256, 0, c->stream>>>(n, f0.planes, f1.planes, fs.pair);
+    if (hipGetLastError() != hipSuccess) return nullptr;
+    fs.pair_vars[0] = va;
+    fs.pair_vars[1] = vb;
+    return fs.pair;
+}
+
 }  // namespace
 
 extern "C" int kelp_scalar_upload(kelp_ctx* c, int slot, int var, int64_t n_planes, const double* host_values)
@@ -423,6 +458,7 @@ extern "C" int kelp_scalar_upload(kelp_ctx* c, int slot, int var, int64_t n_plan
                          (long long)n_planes, (long long)K);
     HIP_TRY(hipSetDevice(c->device));
     ScalarField& f = fs.scalars[var];
+    if (fs.pair_vars[0] == var || fs.pair_vars[1] == var) fs.pair_vars[0] = fs.pair_vars[1] = -1;
     kelp_free_scalar(f);
     const size_t bytes = (size_t)n_planes * N * sizeof(double);
     double* tmp = nullptr;
@@ -559,6 +595,19 @@ extern "C" int kelp_sample_vars(kelp_ctx* c, const kelp_particles* p, int n_vars
                     }
                 SField s0{fa0.planes, (fb0 && fb0->planes) ? fb0->planes : fa0.planes, alpha, fa0.n_planes};
                 SField s1{fa1.planes, (fb1 && fb1->planes) ? fb1->planes : fa1.planes, alpha, fa1.n_planes};
+                if (fa0.n_planes > 1 && fa1.n_planes == fa0.n_planes) {
+                    // two mid-layer columns: read them interleaved (same values)
+                    const bool use_b = alpha != 0.0 && slot_b >= 0;
+                    if (fa0.planes == fa1.planes) {
+                        // one variable asked twice: nothing to interleave
+                    } else if (double* pa = slot_pair(c, c->slots[slot_a], va, vb)) {
+                        double* pb = use_b ? slot_pair(c, c->slots[slot_b], va, vb) : pa;
+                        if (pb) {
+                            s0 = SField{pa, pb, alpha, fa0.n_planes, 2};
+                            s1 = SField{pa + 1, pb + 1, alpha, fa1.n_planes, 2};
+                        }
+                    }
+                }
                 SampleState st{p->lon, p->lat, p->z, p->status, p->nn, p->n};
                 double* ob = out + (int64_t)(k + 1) * p->n;
                 if (c->m.max_v == 3)

@@ -81,10 +81,17 @@ def test_sample_vars_matches_single(gpu_ctx, world, shapes, method):
     status = np.where(rng.random(lon.size) < 0.1, 3, 1).astype(np.int32)
     nn = OracleMesh(mesh, "kmath").nearest(lon, lat)
     p = gpu_ctx.particles(lon, lat, z, status=status, nn=nn)
-    for alpha in (0.0, 0.3):
-        both = gpu_ctx.sample_vars(p, (0, 1), 0, 1, alpha, method)
-        gpu_ctx.synchronize()
-        rows = both.cpu().numpy()
-        for var in (0, 1):
-            one = gpu_ctx.sample(p, var, 0, 1, alpha, method)
-            assert _same(rows[var], one), (shapes, method, alpha, var)
+    for rnd in range(2):
+        if rnd == 1:
+            # a new record of variable 1 in slot 0: the paired (interleaved)
+            # copy of the two columns must follow it
+            planes = {"2d": 1, "mid": K - 1, "ifc": K}[shapes[1]]
+            fn = rng.standard_normal((planes, N)) if planes > 1 else rng.standard_normal(N)
+            gpu_ctx.upload_scalar(0, 1, fn)
+        for alpha in (0.0, 0.3):
+            both = gpu_ctx.sample_vars(p, (0, 1), 0, 1, alpha, method)
+            gpu_ctx.synchronize()
+            rows = both.cpu().numpy()
+            for var in (0, 1):
+                one = gpu_ctx.sample(p, var, 0, 1, alpha, method)
+                assert _same(rows[var], one), (shapes, method, alpha, var, rnd)
