@@ -258,7 +258,7 @@ def main():
     ap.add_argument("--nodes", type=int, default=0)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--resort-every", type=int, default=4,
+    ap.add_argument("--resort-every", type=int, default=8,
                     help="re-sort particles by grid-hash cell every N steps (0: never)")
     ap.add_argument("--ring", type=int, default=0,
                     help="global_3d: keep only this many snapshot slots (>= 3) and stream the "
