@@ -152,6 +152,13 @@ typedef struct {
     int32_t az_field;
     int32_t az_slot_a, az_slot_b;
     double az_alpha0;
+    /* order 5 = rk45 (integrators.py:6448 rk45_step): adaptive Dormand-Prince
+     * 5(4) per particle over exactly dt, 2-D, u/v at element centres (least
+     * squares), steady or blended towards slot_t; the embedded error in
+     * metres per substep against rk45_tol_m (<= 0: 1 m), substeps never below
+     * rk45_h_min seconds (<= 0: dt 1e-4); rk45_inv_dtn = 1 / dt_to_next (0:
+     * steady), stage fractions (t / dt_to_next) take the place of alphas */
+    double rk45_tol_m, rk45_h_min, rk45_inv_dtn;
 } kelp_step_args;
 
 const char *kelp_last_error(void);

@@ -73,6 +73,7 @@ class StepArgs(ctypes.Structure):
         ("sig_end_alpha0", c_double), ("sig_end_alpha", c_double),
         ("Dmin", c_double), ("z0b_min", c_double), ("walk", c_int32),
         ("az_field", c_int32), ("az_slot_a", c_int32), ("az_slot_b", c_int32), ("az_alpha0", c_double),
+        ("rk45_tol_m", c_double), ("rk45_h_min", c_double), ("rk45_inv_dtn", c_double),
     ]
 
 

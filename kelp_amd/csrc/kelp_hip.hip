@@ -1184,11 +1184,16 @@ int kelp_step(kelp_ctx* c, const kelp_particles* p, const kelp_step_args* a)
 {
     if (!c || !p || !a) return fail(KELP_EINVAL, "kelp_step: NULL argument");
     if (!c->has_mesh) return fail(KELP_ESTATE, "kelp_step: no mesh uploaded");
-    if (a->order != 1 && a->order != 2 && a->order != 4)
-        return fail(KELP_EINVAL, "kelp_step: order must be 1, 2 or 4 (got %d)", a->order);
+    if (a->order != 1 && a->order != 2 && a->order != 4 && a->order != 5)
+        return fail(KELP_EINVAL, "kelp_step: order must be 1, 2, 4 or 5 (rk45) (got %d)", a->order);
     if (!(a->dt > 0.0)) return fail(KELP_EINVAL, "kelp_step: dt must be positive (direction goes in time_dir)");
     if (a->time_dir != 1.0 && a->time_dir != -1.0) return fail(KELP_EINVAL, "kelp_step: time_dir must be +1 or -1");
     const bool sigma = c->vertical_coord == 1;
+    // rk45 reads no stage fractions (alphas): refuse what it does not take
+    // before anything loops over the stages
+    if (a->order == 5 && (sigma || a->walk))
+        return fail(KELP_EINVAL, "kelp_step: rk45 takes 2-D u/v at element centres (no 3-D, sigma or walk)");
+    const int n_stage = a->order == 5 ? 0 : a->order;
     if (a->walk) {
         if (!p->elem) return fail(KELP_EINVAL, "kelp_step: elem_search walk needs the carried elements (elem)");
         if (!c->m.enbr) return fail(KELP_ESTATE, "kelp_step: elem_search walk needs the element neighbours");
@@ -1223,7 +1228,7 @@ int kelp_step(kelp_ctx* c, const kelp_particles* p, const kelp_step_args* a)
             return fail(KELP_EINVAL, "kelp_step: snapshot slots differ in dtype, layout or shape");
     }
     if (a->alpha0 != 0.0 && !sb) return fail(KELP_EINVAL, "kelp_step: alpha0 != 0 needs slot_b");
-    for (int k = 0; k < a->order; ++k)
+    for (int k = 0; k < n_stage; ++k)
         if (a->alphas[k] >= 1e-12 && !st) return fail(KELP_EINVAL, "kelp_step: stage alpha > 0 needs slot_t");
     const bool is3d = sa.is3d;
     const int K = c->m.n_levels;
@@ -1319,6 +1324,34 @@ int kelp_step(kelp_ctx* c, const kelp_particles* p, const kelp_step_args* a)
     }
     PState ps{p->lon, p->lat, p->z, p->status, p->nn, p->fr, p->u, p->v, p->w, p->pid, p->wet_streak,
               reinterpret_cast<unsigned long long*>(p->alive_count), p->n, p->elem};
+    if (a->order == 5) {
+        // integrators.py:6485-6513: rk45 is 2-D, element-layout least squares
+        if (is3d || sa.layout != 1 || sigma || cfg.walk)
+            return fail(KELP_EINVAL, "kelp_step: rk45 takes 2-D u/v at element centres (no 3-D, sigma or walk)");
+        if (!(a->dt > 0.0)) return fail(KELP_EINVAL, "kelp_step: rk45 needs dt > 0");
+        cfg.rk45_tol = a->rk45_tol_m > 0.0 ? a->rk45_tol_m : 1.0;
+        cfg.rk45_h_min = a->rk45_h_min > 0.0 ? a->rk45_h_min : a->dt * 1e-4;
+        cfg.rk45_inv_dtn = a->rk45_inv_dtn;
+        if (!(cfg.rk45_inv_dtn >= 0.0) || (cfg.rk45_inv_dtn > 0.0 && !st))
+            return fail(KELP_EINVAL, "kelp_step: rk45 blending towards slot_t needs slot_t and rk45_inv_dtn >= 0");
+        // :6531: the cap exceeds the accepts of a whole step at h_min and their rejections
+        const double nmin = a->dt / cfg.rk45_h_min;
+        cfg.rk45_max_sub = nmin < 1e12 ? (int64_t)(2.0 * (double)(int64_t)nmin + 16.0) : (int64_t)1 << 40;
+        if (cfg.rk45_max_sub < 10000) cfg.rk45_max_sub = 10000;
+        Snap<float, false> ff{};
+        Snap<double, false> fd{};
+        if (sa.dtype == KELP_DT_F32) {
+            ff = snap_of<float, false>(sa.erec, sb ? sb->erec : nullptr, st ? st->erec : nullptr);
+            if (c->m.max_v == 3) launch_rk45_v<3, float>(c->stream, c->m, ff, cfg, ps);
+            else launch_rk45_v<4, float>(c->stream, c->m, ff, cfg, ps);
+        } else {
+            fd = snap_of<double, false>(sa.erec, sb ? sb->erec : nullptr, st ? st->erec : nullptr);
+            if (c->m.max_v == 3) launch_rk45_v<3, double>(c->stream, c->m, fd, cfg, ps);
+            else launch_rk45_v<4, double>(c->stream, c->m, fd, cfg, ps);
+        }
+        HIP_TRY(hipGetLastError());
+        return KELP_OK;
+    }
     if (sa.dtype == KELP_DT_BF16) {
         // node-layout z-level only (refused at upload otherwise)
         if (sigma) return fail(KELP_EINVAL, "kelp_step: bfloat16 snapshots on a sigma mesh");

@@ -14,6 +14,10 @@ namespace kelp {
 // One step's configuration (step_fn keywords, tracking.py:2643-2655, plus the
 // beaching hook's per-run constants, tracking.py:199).
 struct StepCfg {
+    // rk45 (order 5): tolerance (m) per substep, smallest substep (s),
+    // 1 / dt_to_next (0: steady), substep cap
+    double rk45_tol, rk45_h_min, rk45_inv_dtn;
+    int64_t rk45_max_sub;
     double alpha0;
     double alphas[4];
     double dt, time_dir;
@@ -66,6 +70,12 @@ struct PState {
 template <int MAXV, bool IS3D, typename T, bool ELEM>
 void launch_step_v(hipStream_t stream, int order, const DMesh& m, const Snap<T, IS3D>& f,
                    const Snap<T, false>& fe, const StepCfg& cfg, const PState& ps);
+
+// The adaptive Dormand-Prince 5(4) step (2-D, u/v at element centres):
+// launch rk45_kernel<MAXV, T>.
+template <int MAXV, typename T>
+void launch_rk45_v(hipStream_t stream, const DMesh& m, const Snap<T, false>& fe, const StepCfg& cfg,
+                   const PState& ps);
 
 // The sigma-mesh step (3-D): launch sigma_step_kernel<order, MAXV, T, ELEM>.
 template <int MAXV, typename T, bool ELEM>

@@ -505,6 +505,220 @@ __global__ __launch_bounds__(256) KELP_STEP_ATTR void step_kernel(DMesh m, Snap<
     if (IS3D && s.w) s.w[i] = W;
 }
 
+// ---------------------------------------------------------------------------
+// rk45: integrators.py:6233 _rk45_kernel_2d, per-particle adaptive
+// Dormand-Prince 5(4) over exactly dt (2-D, u/v at element centres by least
+// squares, steady or time-blended), then rk45_step's re-validation
+// (:6586-6611).  Tableau of Dormand & Prince (1980), error weights b5 - b4
+// as exact fractions.
+
+// a trial position: lon wrapped only when it left [-180, 180), lat clamped
+// (integrators.py:6182 _norm_lonlat)
+__device__ __forceinline__ void norm_lonlat(double& lon, double& lat)
+{
+    if (lon >= 180.0 || lon < -180.0) lon = km_pymod360(lon + 180.0) - 180.0;
+    if (lat > 90.0)
+        lat = 90.0;
+    else if (lat < -90.0)
+        lat = -90.0;
+}
+
+// integrators.py:6197 _rk45_eval: velocity and degree rates at (lon, lat)
+template <int MAXV, typename T>
+__device__ inline bool rk45_eval(const DMesh& m, const Snap<T, false>& fe, const StepCfg& c, double lon, double lat,
+                                 double alpha, double& U, double& V, double& kx, double& ky)
+{
+    U = V = kx = ky = 0.0;
+    const int nn = site_anchor(m, lon, lat, c.exact == 1 ? 1 : 0);
+    if (nn < 0) return false;
+    double u, v;
+    if (!interp_lsq(m, fe, c.alpha0, alpha, c.time_dir, lon, lat, nn, 0, u, v)) return false;
+    const double z0 = z0_of(lat);
+    U = u;
+    V = v;
+    kx = u / z0;
+    ky = v / kSLM;
+    return true;
+}
+
+__device__ __forceinline__ double rk45_alpha(double t, double inv_dtn)
+{
+    const double al = t * inv_dtn;
+    return al > 1.0 ? 1.0 : al;
+}
+
+template <int MAXV, typename T>
+__device__ __noinline__ void rk45_det(const DMesh& m, const Snap<T, false>& fe, const StepCfg& c, double& lon,
+                                      double& lat, int32_t& st, int& nn, int8_t& fr, double& Uo, double& Vo)
+{
+    const double dt = c.dt, tol = c.rk45_tol, h_min = c.rk45_h_min, inv = c.rk45_inv_dtn;
+    double y0 = lon, y1 = lat, t = 0.0, h = dt;
+    double Uv, Vv, k1x, k1y;
+    bool ok_out = true;
+    Uo = Vo = 0.0;
+    if (!rk45_eval<MAXV, T>(m, fe, c, y0, y1, rk45_alpha(t, inv), Uv, Vv, k1x, k1y)) {
+        ok_out = false;
+    } else {
+        int64_t nsub = 0;
+        double k2x = 0, k2y = 0, k3x = 0, k3y = 0, k4x = 0, k4y = 0, k5x = 0, k5y = 0, k6x = 0, k6y = 0, k7x = 0,
+               k7y = 0, U7 = 0, V7 = 0, y5x = y0, y5y = y1, du, dv;
+        while (t < dt * (1.0 - 1e-12)) {
+            if (h > dt - t) h = dt - t;
+            bool clamped = false;
+            if (h < h_min) {
+                h = h_min;
+                if (h > dt - t) h = dt - t;
+                clamped = true;
+            }
+            double px, py;
+            px = y0 + h * (1.0 / 5.0) * k1x;
+            py = y1 + h * (1.0 / 5.0) * k1y;
+            norm_lonlat(px, py);
+            bool okst = rk45_eval<MAXV, T>(m, fe, c, px, py, rk45_alpha(t + (1.0 / 5.0) * h, inv), du, dv, k2x, k2y);
+            if (okst) {
+                px = y0 + h * ((3.0 / 40.0) * k1x + (9.0 / 40.0) * k2x);
+                py = y1 + h * ((3.0 / 40.0) * k1y + (9.0 / 40.0) * k2y);
+                norm_lonlat(px, py);
+                okst = rk45_eval<MAXV, T>(m, fe, c, px, py, rk45_alpha(t + (3.0 / 10.0) * h, inv), du, dv, k3x, k3y);
+            }
+            if (okst) {
+                px = y0 + h * ((44.0 / 45.0) * k1x + (-56.0 / 15.0) * k2x + (32.0 / 9.0) * k3x);
+                py = y1 + h * ((44.0 / 45.0) * k1y + (-56.0 / 15.0) * k2y + (32.0 / 9.0) * k3y);
+                norm_lonlat(px, py);
+                okst = rk45_eval<MAXV, T>(m, fe, c, px, py, rk45_alpha(t + (4.0 / 5.0) * h, inv), du, dv, k4x, k4y);
+            }
+            if (okst) {
+                px = y0 + h * ((19372.0 / 6561.0) * k1x + (-25360.0 / 2187.0) * k2x + (64448.0 / 6561.0) * k3x +
+                               (-212.0 / 729.0) * k4x);
+                py = y1 + h * ((19372.0 / 6561.0) * k1y + (-25360.0 / 2187.0) * k2y + (64448.0 / 6561.0) * k3y +
+                               (-212.0 / 729.0) * k4y);
+                norm_lonlat(px, py);
+                okst = rk45_eval<MAXV, T>(m, fe, c, px, py, rk45_alpha(t + (8.0 / 9.0) * h, inv), du, dv, k5x, k5y);
+            }
+            const double a6 = rk45_alpha(t + h, inv);
+            if (okst) {
+                px = y0 + h * ((9017.0 / 3168.0) * k1x + (-355.0 / 33.0) * k2x + (46732.0 / 5247.0) * k3x +
+                               (49.0 / 176.0) * k4x + (-5103.0 / 18656.0) * k5x);
+                py = y1 + h * ((9017.0 / 3168.0) * k1y + (-355.0 / 33.0) * k2y + (46732.0 / 5247.0) * k3y +
+                               (49.0 / 176.0) * k4y + (-5103.0 / 18656.0) * k5y);
+                norm_lonlat(px, py);
+                okst = rk45_eval<MAXV, T>(m, fe, c, px, py, a6, du, dv, k6x, k6y);
+            }
+            if (okst) {
+                y5x = y0 + h * ((35.0 / 384.0) * k1x + (500.0 / 1113.0) * k3x + (125.0 / 192.0) * k4x +
+                                (-2187.0 / 6784.0) * k5x + (11.0 / 84.0) * k6x);
+                y5y = y1 + h * ((35.0 / 384.0) * k1y + (500.0 / 1113.0) * k3y + (125.0 / 192.0) * k4y +
+                                (-2187.0 / 6784.0) * k5y + (11.0 / 84.0) * k6y);
+                norm_lonlat(y5x, y5y);
+                okst = rk45_eval<MAXV, T>(m, fe, c, y5x, y5y, a6, U7, V7, k7x, k7y);
+            }
+            nsub += 1;
+            if (!okst) {
+                // a failed stage is a rejected attempt, terminal only at the
+                // h_min floor or the cap
+                if (nsub > c.rk45_max_sub || h <= h_min * (1.0 + 1e-12)) {
+                    ok_out = false;
+                    break;
+                }
+                h = h * 0.2;
+                if (h < h_min) h = h_min;
+                continue;
+            }
+            const double errx = h * ((71.0 / 57600.0) * k1x + (-71.0 / 16695.0) * k3x + (71.0 / 1920.0) * k4x +
+                                     (-17253.0 / 339200.0) * k5x + (22.0 / 525.0) * k6x + (-1.0 / 40.0) * k7x);
+            const double erry = h * ((71.0 / 57600.0) * k1y + (-71.0 / 16695.0) * k3y + (71.0 / 1920.0) * k4y +
+                                     (-17253.0 / 339200.0) * k5y + (22.0 / 525.0) * k6y + (-1.0 / 40.0) * k7y);
+            const double z0m = z0_of(y1);
+            const double ex = errx * z0m, ey = erry * kSLM;
+            const double err = km_sqrt(ex * ex + ey * ey);
+            if (nsub > c.rk45_max_sub) {
+                ok_out = false;
+                break;
+            }
+            if (err <= tol || clamped) {
+                t += h;
+                y0 = y5x;
+                y1 = y5y;
+                k1x = k7x;
+                k1y = k7y;
+                Uv = U7;
+                Vv = V7;
+            }
+            double fac;
+            if (err > 0.0) {
+                fac = 0.9 * km_exp(0.2 * km_log(tol / err));
+                if (fac < 0.2)
+                    fac = 0.2;
+                else if (fac > 5.0)
+                    fac = 5.0;
+            } else {
+                fac = 5.0;
+            }
+            h = h * fac;
+        }
+    }
+    if (!ok_out) {
+        st = kOutside;
+        fr = 1;
+        return;
+    }
+    // rk45_step's re-validation: the batch nearest node, then the domain
+    const int nn_new = site_anchor(m, y0, y1, c.exact == 1 ? 1 : 2);
+    if (find_containing<MAXV>(m, y0, y1, nn_new) < 0) {
+        st = kOutside;
+        fr = 2;
+        return;
+    }
+    lon = y0;
+    lat = y1;
+    nn = nn_new;
+    Uo = Uv;
+    Vo = Vv;
+}
+
+template <int MAXV, typename T>
+__global__ __launch_bounds__(256) void rk45_kernel(DMesh m, Snap<T, false> fe, StepCfg c, PState s)
+{
+    const int64_t i = xcd_thread_index();
+    if (i >= s.n) return;
+    const int32_t prev = s.status[i];
+    if (s.alive_count) {
+        unsigned long long alive = __ballot(prev == kAlive);
+        unsigned long long active = __ballot(1);
+        if ((int)(threadIdx.x & 63) == __ffsll((long long)active) - 1 && alive)
+            atomicAdd(s.alive_count, (unsigned long long)__popcll(alive));
+    }
+    int32_t st = prev;
+    double lon = s.lon[i], lat = s.lat[i];
+    int nn = s.nn[i];
+    int8_t fr = s.fr[i];
+    int elem = -1;
+    double U = 0.0, V = 0.0, z = 0.0;
+    if (prev == kAlive) {
+        const double pre_lon = lon, pre_lat = lat;
+        const int pre_nn = nn;
+        rk45_det<MAXV, T>(m, fe, c, lon, lat, st, nn, fr, U, V);
+        if (st == kAlive && c.kick && !(c.kh_x == 0.0 && c.kh_y == 0.0))
+            kick_one<MAXV, false>(m, c, (uint64_t)s.pid[i], pre_lon, pre_lat, 0.0, pre_nn, lon, lat, z, st, nn, fr);
+        if ((c.periodic_lon || c.periodic_lat) && st == kAlive) periodic_one(m, c, lon, lat, nn, elem);
+    }
+    if (c.beach) beach_one<MAXV>(m, c, (uint64_t)s.pid[i], prev, lon, lat, st, nn, fr, s.ws[i]);
+    s.lon[i] = lon;
+    s.lat[i] = lat;
+    s.status[i] = st;
+    s.nn[i] = nn;
+    s.fr[i] = fr;
+    s.u[i] = U;
+    s.v[i] = V;
+}
+
+template <int MAXV, typename T>
+void launch_rk45_v(hipStream_t stream, const DMesh& m, const Snap<T, false>& fe, const StepCfg& cfg,
+                   const PState& ps)
+{
+    rk45_kernel<MAXV, T><<<(unsigned)((ps.n + 255) / 256), 256, 0, stream>>>(m, fe, cfg, ps);
+}
+
 // Never launched.  A second caller of the exact nearest search that carries
 // no occupancy attribute: with the step kernels as its only callers the
 // compiler narrows the out-of-line search to the step kernels' register

@@ -27,4 +27,12 @@ template void launch_step_v<4, (bool)KV_IS3D, KvT, (bool)KV_ELEM>(hipStream_t, i
                                                                   const Snap<KvT, false>&, const StepCfg&,
                                                                   const PState&);
 
+#if !KV_IS3D && KV_ELEM && !(defined(KV_BF16) && KV_BF16)
+// rk45 (2-D, element-centre u/v) lives with the 2-D element-layout steps
+template void launch_rk45_v<3, KvT>(hipStream_t, const DMesh&, const Snap<KvT, false>&, const StepCfg&,
+                                    const PState&);
+template void launch_rk45_v<4, KvT>(hipStream_t, const DMesh&, const Snap<KvT, false>&, const StepCfg&,
+                                    const PState&);
+#endif
+
 }  // namespace kelp

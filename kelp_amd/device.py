@@ -90,9 +90,14 @@ RESUSPENSION_CODES = {"never": 0, "auto": 1, "stochastic": 2, "auto_or_stochasti
 def make_step_args(order, slot_a, dt, time_dir, slot_b=-1, slot_t=-1, alpha0=0.0,
                    alphas=(0.0, 0.0, 0.0, 0.0), exact_nearest=True, w_method=2, kick=None,
                    beach=None, step_i=0, real_id=0, periodic_lon=None,
-                   periodic_lat=None, reuse_anchor=False, sigma=None, walk=False) -> StepArgs:
+                   periodic_lat=None, reuse_anchor=False, sigma=None, walk=False, rk45=None) -> StepArgs:
     a = StepArgs()
     a.order = int(order)
+    if rk45 is not None:
+        # order 5: dict(tol_m, h_min, inv_dtn); 0 or missing = the defaults
+        a.rk45_tol_m = float(rk45.get("tol_m") or 0.0)
+        a.rk45_h_min = float(rk45.get("h_min") or 0.0)
+        a.rk45_inv_dtn = float(rk45.get("inv_dtn") or 0.0)
     a.slot_a, a.slot_b, a.slot_t = int(slot_a), int(slot_b), int(slot_t)
     a.alpha0 = float(alpha0)
     for k in range(4):
@@ -410,16 +415,17 @@ class DeviceContext:
              slot_b: int = -1, slot_t: int = -1, alpha0: float = 0.0, alphas=(0.0, 0.0, 0.0, 0.0),
              exact_nearest: bool = True, w_method: int = 2, kick=None, beach=None,
              step_i: int = 0, real_id: int = 0, periodic_lon=None, periodic_lat=None,
-             reuse_anchor: bool = False, sigma=None, walk: bool = False) -> None:
+             reuse_anchor: bool = False, sigma=None, walk: bool = False, rk45=None) -> None:
         """One step in place.  ``kick``: dict(kh_x, kh_y, seed[, az_const])
         for the split uniform-Kh random walk (and the vertical Az kick);
         ``beach``: dict of the beaching options (criteria, depth_min, ...);
         ``sigma`` (sigma meshes): dict(start=(slot_a, slot_b, slot_t, alpha0,
         alpha), end=same or None, Dmin, z0b_min), the surfaces at the step's
-        start and end as blends of the slots' eta."""
+        start and end as blends of the slots' eta.  ``order=5`` is rk45
+        with ``rk45=dict(tol_m, h_min, inv_dtn)``."""
         a = make_step_args(order, slot_a, dt, time_dir, slot_b, slot_t, alpha0, alphas,
                            exact_nearest, w_method, kick, beach, step_i, real_id,
-                           periodic_lon, periodic_lat, reuse_anchor, sigma, walk)
+                           periodic_lon, periodic_lat, reuse_anchor, sigma, walk, rk45)
         s = p.as_struct()
         check(self.lib.kelp_step(self.h, ctypes.byref(s), ctypes.byref(a)), "kelp_step")
 

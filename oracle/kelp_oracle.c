@@ -24,6 +24,7 @@
 #define M_ATAN2 km_atan2
 #define M_LOG km_log
 #define M_EXP km_exp
+#define M_POW(x, y) km_exp((y) * km_log(x))
 #define PYMOD360 km_pymod360
 #else
 #define M_SIN sin
@@ -33,6 +34,7 @@
 #define M_ATAN2 atan2
 #define M_LOG log
 #define M_EXP exp
+#define M_POW pow
 /* Python float `%` by 360 (CPython float_rem): fmod, then the divisor's
  * sign; a zero remainder is +0.0. */
 static double PYMOD360(double x)
@@ -1386,6 +1388,176 @@ static int det_step(const or_mesh_t *m, const or_fields_t *f, const or_step_args
 }
 
 /* ---------------------------------------------------------------------
+ * rk45: integrators.py:6233 _rk45_kernel_2d -- per-particle adaptive
+ * Dormand-Prince 5(4) over exactly dt (2-D, u/v at element centres by least
+ * squares, steady or time-blended), then the re-validation of rk45_step
+ * (:6586-6611).  The tableau is Dormand & Prince (1980); the error weights
+ * are b5 - b4 as exact fractions.
+ * --------------------------------------------------------------------- */
+#define DP_C2 (1.0 / 5.0)
+#define DP_C3 (3.0 / 10.0)
+#define DP_C4 (4.0 / 5.0)
+#define DP_C5 (8.0 / 9.0)
+
+/* a trial position: lon wrapped only when it left [-180, 180), lat clamped
+ * (integrators.py:6182 _norm_lonlat) */
+static void norm_lonlat(double *lon, double *lat)
+{
+    if (*lon >= 180.0 || *lon < -180.0) *lon = PYMOD360(*lon + 180.0) - 180.0;
+    if (*lat > 90.0) *lat = 90.0;
+    else if (*lat < -90.0) *lat = -90.0;
+}
+
+/* integrators.py:6197 _rk45_eval: velocity and degree rates at (lon, lat)
+ * with the stage fraction alpha */
+static int rk45_eval(const or_mesh_t *m, const or_fields_t *f, const or_step_args_t *a, double lon, double lat,
+                     double alpha, double *U, double *V, double *kx, double *ky)
+{
+    int nn = or_site_anchor(m, lon, lat, a->exact == 1 ? 1 : 0);
+    *U = *V = *kx = *ky = 0.0;
+    if (nn < 0) return 0;
+    double u, v;
+    if (!or_interp_lsq_tinterp(m, f, lon, lat, nn, a->time_dir, -1, alpha, &u, &v)) return 0;
+    double z0 = z0_of(lat);
+    *U = u;
+    *V = v;
+    *kx = u / z0;
+    *ky = v / SLM;
+    return 1;
+}
+
+static inline double stage_alpha(double t, double inv_dtn)
+{
+    double al = t * inv_dtn;
+    return al > 1.0 ? 1.0 : al;
+}
+
+static void rk45_det(const or_mesh_t *m, const or_fields_t *f, const or_step_args_t *a, double *lon, double *lat,
+                     int32_t *st, int32_t *nn, int8_t *fr, double *Uo, double *Vo)
+{
+    const double dt = a->dt, tol = a->rk45_tol_m, h_min = a->rk45_h_min, inv = a->rk45_inv_dtn;
+    double y0 = *lon, y1 = *lat, t = 0.0, h = dt;
+    double Uv, Vv, k1x, k1y;
+    int ok_out = 1;
+    *Uo = *Vo = 0.0;
+    if (!rk45_eval(m, f, a, y0, y1, stage_alpha(t, inv), &Uv, &Vv, &k1x, &k1y)) {
+        ok_out = 0;
+    } else {
+        int64_t nsub = 0;
+        double k2x = 0, k2y = 0, k3x = 0, k3y = 0, k4x = 0, k4y = 0, k5x = 0, k5y = 0, k6x = 0, k6y = 0, k7x = 0,
+               k7y = 0, U7 = 0, V7 = 0, y5x = y0, y5y = y1, du, dv;
+        while (t < dt * (1.0 - 1e-12)) {
+            if (h > dt - t) h = dt - t;
+            int clamped = 0;
+            if (h < h_min) {
+                h = h_min;
+                if (h > dt - t) h = dt - t;
+                clamped = 1;
+            }
+            int okst = 1;
+            double px, py;
+            px = y0 + h * (1.0 / 5.0) * k1x;
+            py = y1 + h * (1.0 / 5.0) * k1y;
+            norm_lonlat(&px, &py);
+            okst = rk45_eval(m, f, a, px, py, stage_alpha(t + DP_C2 * h, inv), &du, &dv, &k2x, &k2y);
+            if (okst) {
+                px = y0 + h * ((3.0 / 40.0) * k1x + (9.0 / 40.0) * k2x);
+                py = y1 + h * ((3.0 / 40.0) * k1y + (9.0 / 40.0) * k2y);
+                norm_lonlat(&px, &py);
+                okst = rk45_eval(m, f, a, px, py, stage_alpha(t + DP_C3 * h, inv), &du, &dv, &k3x, &k3y);
+            }
+            if (okst) {
+                px = y0 + h * ((44.0 / 45.0) * k1x + (-56.0 / 15.0) * k2x + (32.0 / 9.0) * k3x);
+                py = y1 + h * ((44.0 / 45.0) * k1y + (-56.0 / 15.0) * k2y + (32.0 / 9.0) * k3y);
+                norm_lonlat(&px, &py);
+                okst = rk45_eval(m, f, a, px, py, stage_alpha(t + DP_C4 * h, inv), &du, &dv, &k4x, &k4y);
+            }
+            if (okst) {
+                px = y0 + h * ((19372.0 / 6561.0) * k1x + (-25360.0 / 2187.0) * k2x + (64448.0 / 6561.0) * k3x +
+                               (-212.0 / 729.0) * k4x);
+                py = y1 + h * ((19372.0 / 6561.0) * k1y + (-25360.0 / 2187.0) * k2y + (64448.0 / 6561.0) * k3y +
+                               (-212.0 / 729.0) * k4y);
+                norm_lonlat(&px, &py);
+                okst = rk45_eval(m, f, a, px, py, stage_alpha(t + DP_C5 * h, inv), &du, &dv, &k5x, &k5y);
+            }
+            const double a6 = stage_alpha(t + h, inv);
+            if (okst) {
+                px = y0 + h * ((9017.0 / 3168.0) * k1x + (-355.0 / 33.0) * k2x + (46732.0 / 5247.0) * k3x +
+                               (49.0 / 176.0) * k4x + (-5103.0 / 18656.0) * k5x);
+                py = y1 + h * ((9017.0 / 3168.0) * k1y + (-355.0 / 33.0) * k2y + (46732.0 / 5247.0) * k3y +
+                               (49.0 / 176.0) * k4y + (-5103.0 / 18656.0) * k5y);
+                norm_lonlat(&px, &py);
+                okst = rk45_eval(m, f, a, px, py, a6, &du, &dv, &k6x, &k6y);
+            }
+            if (okst) {
+                y5x = y0 + h * ((35.0 / 384.0) * k1x + (500.0 / 1113.0) * k3x + (125.0 / 192.0) * k4x +
+                                (-2187.0 / 6784.0) * k5x + (11.0 / 84.0) * k6x);
+                y5y = y1 + h * ((35.0 / 384.0) * k1y + (500.0 / 1113.0) * k3y + (125.0 / 192.0) * k4y +
+                                (-2187.0 / 6784.0) * k5y + (11.0 / 84.0) * k6y);
+                norm_lonlat(&y5x, &y5y);
+                okst = rk45_eval(m, f, a, y5x, y5y, a6, &U7, &V7, &k7x, &k7y);
+            }
+            nsub += 1;
+            if (!okst) {
+                /* a failed stage is a rejected attempt, terminal only at the
+                 * h_min floor or the cap */
+                if (nsub > a->rk45_max_sub || h <= h_min * (1.0 + 1e-12)) {
+                    ok_out = 0;
+                    break;
+                }
+                h = h * 0.2;
+                if (h < h_min) h = h_min;
+                continue;
+            }
+            double errx = h * ((71.0 / 57600.0) * k1x + (-71.0 / 16695.0) * k3x + (71.0 / 1920.0) * k4x +
+                               (-17253.0 / 339200.0) * k5x + (22.0 / 525.0) * k6x + (-1.0 / 40.0) * k7x);
+            double erry = h * ((71.0 / 57600.0) * k1y + (-71.0 / 16695.0) * k3y + (71.0 / 1920.0) * k4y +
+                               (-17253.0 / 339200.0) * k5y + (22.0 / 525.0) * k6y + (-1.0 / 40.0) * k7y);
+            double z0m = z0_of(y1);
+            double err = sqrt(sq(errx * z0m) + sq(erry * SLM));
+            if (nsub > a->rk45_max_sub) {
+                ok_out = 0;
+                break;
+            }
+            if (err <= tol || clamped) {
+                t += h;
+                y0 = y5x;
+                y1 = y5y;
+                k1x = k7x;
+                k1y = k7y;
+                Uv = U7;
+                Vv = V7;
+            }
+            double fac;
+            if (err > 0.0) {
+                fac = 0.9 * M_POW(tol / err, 0.2);
+                if (fac < 0.2) fac = 0.2;
+                else if (fac > 5.0) fac = 5.0;
+            } else {
+                fac = 5.0;
+            }
+            h = h * fac;
+        }
+    }
+    if (!ok_out) {
+        *st = ST_OUTSIDE;
+        *fr = FR_INTERP;
+        return;
+    }
+    int nn_new = or_site_anchor(m, y0, y1, a->exact == 1 ? 1 : 2);
+    if (or_find_containing_local(m, y0, y1, nn_new) < 0) {
+        *st = ST_OUTSIDE;
+        *fr = FR_OOB;
+        return;
+    }
+    *lon = y0;
+    *lat = y1;
+    *nn = nn_new;
+    *Uo = Uv;
+    *Vo = Vv;
+}
+
+/* ---------------------------------------------------------------------
  * Sigma meshes (numerics/vertical_sigma.py and the sigma branches of the
  * multi-pass steppers, integrators.py:1310-1441, :4997, :5380, :5760).
  * The mesh's levels are the sigma interfaces; interface k of a column of
@@ -2076,7 +2248,10 @@ void or_step(const or_mesh_t *m, const or_fields_t *f, const or_step_args_t *a, 
             if ((a->periodic_lon || a->periodic_lat) && status[i] == ST_ALIVE)
                 periodic_one(m, a, &lon[i], &lat[i], &nn[i], el);
         } else if (prev == ST_ALIVE) {
-            det_step(m, f, a, &lon[i], &lat[i], &z[i], &status[i], &nn[i], &fr[i], &U[i], &V[i], &W[i], el);
+            if (a->order == 5)
+                rk45_det(m, f, a, &lon[i], &lat[i], &status[i], &nn[i], &fr[i], &U[i], &V[i]);
+            else
+                det_step(m, f, a, &lon[i], &lat[i], &z[i], &status[i], &nn[i], &fr[i], &U[i], &V[i], &W[i], el);
             if (a->kick && status[i] == ST_ALIVE && !(a->kh_x == 0.0 && a->kh_y == 0.0))
                 kick_one(m, a, (uint64_t)pid[i], pre_lon, pre_lat, pre_z, pre_nn,
                          &lon[i], &lat[i], &z[i], &status[i], &nn[i], &fr[i]);

@@ -90,6 +90,10 @@ typedef struct {
     const double *sig_eta[5];
     double Dmin, z0b_min;
     int32_t walk;      /* elem_search walk: containing elements by the edge walk */
+    /* order 5 = rk45 (integrators.py:6448): error tolerance (m) per substep,
+     * smallest substep (s), 1 / dt_to_next (0: steady), substep cap */
+    double rk45_tol_m, rk45_h_min, rk45_inv_dtn;
+    int64_t rk45_max_sub;
 } or_step_args_t;
 
 #endif

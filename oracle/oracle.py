@@ -54,6 +54,8 @@ class OrStepArgs(ctypes.Structure):
         ("px0", c_double), ("px1", c_double), ("py0", c_double), ("py1", c_double),
         ("sig_eta", c_void_p * 5), ("Dmin", c_double), ("z0b_min", c_double),
         ("walk", c_int32),
+        ("rk45_tol_m", c_double), ("rk45_h_min", c_double), ("rk45_inv_dtn", c_double),
+        ("rk45_max_sub", ctypes.c_int64),
     ]
 
 
@@ -62,12 +64,20 @@ RESUSPENSION_CODES = {"never": 0, "auto": 1, "stochastic": 2, "auto_or_stochasti
 
 def step_args(order, dt, time_dir, is3d=False, w_method=2, exact=True, alphas=(0, 0, 0, 0),
               kick=None, beach=None, step_i=0, real_id=0, periodic_lon=None,
-              periodic_lat=None, sigma=None, walk=False) -> OrStepArgs:
+              periodic_lat=None, sigma=None, walk=False, rk45=None) -> OrStepArgs:
     """``sigma`` (sigma meshes): dict(etas=[stage 0..3 surfaces, end
     surface] as (N,) arrays, Dmin, z0b_min); the arrays are kept alive by
     the returned structure."""
     a = OrStepArgs()
     a.walk = int(bool(walk))
+    if int(order) == 5:
+        # integrators.py:6524-6533: tolerance 1 m, h_min dt 1e-4, the cap
+        r = dict(rk45 or {})
+        a.rk45_tol_m = float(r.get("tol_m", 1.0))
+        h_min = r.get("h_min")
+        a.rk45_h_min = float(h_min) if h_min is not None else float(dt) * 1e-4
+        a.rk45_max_sub = max(10_000, 2 * int(float(dt) / a.rk45_h_min) + 16)
+        a.rk45_inv_dtn = float(r.get("inv_dtn", 0.0))
     if sigma is not None:
         etas = [np.ascontiguousarray(e, dtype=np.float64) for e in sigma["etas"]]
         a._sig_keep = etas
