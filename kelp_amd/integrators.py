@@ -63,6 +63,8 @@ def stage_alphas(order: int, dt: float, dt_to_next: Optional[float]):
     dt_to_next`` (integrators.py:1187 _make_stage_vels); the kernel applies
     the reference's endpoint short-circuits (alpha < 1e-12 -> the step-start
     field, alpha > 1 - 1e-12 -> the target)."""
+    if order == 5:
+        return [0.0] * 4          # rk45 takes t / dt_to_next per substep instead
     cs = _C_COEFFS[order]
     if dt_to_next is None:
         return [0.0] * 4
@@ -187,6 +189,20 @@ def _step(order: int, state: ParticleState, vel, mesh: Mesh, dt: float, time_dir
     diffusion: Optional[DiffusionStepArgs] = kwargs.get("diffusion")
     _validate_time_aware_kwargs(vel_next, dt_to_next)
     is3d = _is_3d(vel)
+    rk45 = None
+    if order == 5:
+        # integrators.py:6485-6513: 2-D element-layout least squares only
+        base = vel.vel_a if isinstance(vel, LazyLerpField) else vel
+        if uv_interp != "leastsq":
+            raise NotImplementedError(f"rk45 supports uv_interp='leastsq' only (v1), got {uv_interp!r}")
+        if is3d:
+            raise NotImplementedError("rk45 is 2-D only (v1)")
+        if base.uv_layout != "element":
+            raise ValueError("rk45 requires element-layout velocities (v1)")
+        tol = float(kwargs.get("rk45_tol_m", 1.0))
+        h_min = kwargs.get("rk45_h_min")
+        rk45 = dict(tol_m=tol, h_min=None if h_min is None else float(h_min),
+                    inv_dtn=0.0 if dt_to_next is None else 1.0 / float(dt_to_next))
     _check_supported(vel, mesh, uv_interp, w_interp, diffusion, is3d)
     if dt <= 0:
         raise ValueError(f"dt must be positive (direction goes in time_dir), got {dt}")
@@ -229,7 +245,7 @@ def _step(order: int, state: ParticleState, vel, mesh: Mesh, dt: float, time_dir
     exact = 1 if exact_nearest_on(mesh) else 2
     ctx.step(p, order, slot_a, dt, time_dir, slot_b=slot_b, slot_t=slot_t, alpha0=alpha0,
              alphas=alphas, exact_nearest=exact, w_method=W_METHODS.get(w_interp, 2),
-             kick=kick, step_i=step_i, real_id=real_id)
+             kick=kick, step_i=step_i, real_id=real_id, rk45=rk45)
     h = p.to_host()
     return state.replace(lon=h["lon"], lat=h["lat"], z=h["z"] if is3d else state.z,
                          status=h["status"], nearest_node=h["nn"], failure_reason=h["fr"],
@@ -251,21 +267,35 @@ def rk4_step(state, vel, mesh, dt, time_dir, layer=0, **kwargs):
     return _step(4, state, vel, mesh, dt, time_dir, kwargs)
 
 
+def rk45_step(state, vel, mesh, dt, time_dir, layer=0, **kwargs):
+    """integrators.py:6448 rk45_step: adaptive Dormand-Prince 5(4) per
+    particle over exactly ``dt`` (``rk45_tol_m`` metres per substep, default
+    1; ``rk45_h_min`` seconds, default dt 1e-4), 2-D element-layout least
+    squares, steady or blended towards ``vel_next``."""
+    h_min = kwargs.get("rk45_h_min")
+    if h_min is not None and float(h_min) <= 0.0:
+        raise ValueError("numerics.rk45_h_min must be > 0 seconds")
+    if float(kwargs.get("rk45_tol_m", 1.0)) <= 0.0:
+        raise ValueError("numerics.rk45_tol_m must be > 0 m")
+    return _step(5, state, vel, mesh, dt, time_dir, kwargs)
+
+
 INTEGRATORS = {
     "euler": euler_step,
     "rk2": rk2_step,
     "rk4": rk4_step,
+    "rk45": rk45_step,
 }
 
-ORDERS = {"euler": 1, "rk2": 2, "rk4": 4}
+ORDERS = {"euler": 1, "rk2": 2, "rk4": 4, "rk45": 5}
 
 
 def get_integrator(name: str):
-    """integrators.py:6650 get_integrator (rk45 is not on the device path)."""
+    """integrators.py:6650 get_integrator."""
     if name not in INTEGRATORS:
         raise ValueError(f"Unknown integrator '{name}'. Available: {sorted(INTEGRATORS)}")
     return INTEGRATORS[name]
 
 
-__all__ = ["euler_step", "rk2_step", "rk4_step", "get_integrator", "INTEGRATORS",
+__all__ = ["euler_step", "rk2_step", "rk4_step", "rk45_step", "get_integrator", "INTEGRATORS",
            "stage_alphas", "Engine", "get_engine"]

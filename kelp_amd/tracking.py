@@ -27,7 +27,7 @@ from .integrators import ORDERS, W_METHODS
 from .mesh import Mesh, exact_nearest_on, set_exact_nearest
 from .particles import ParticleState, ParticleStatus
 
-_C = {1: (0.0,), 2: (0.0, 0.5), 4: (0.0, 0.5, 0.5, 1.0)}
+_C = {1: (0.0,), 2: (0.0, 0.5), 4: (0.0, 0.5, 0.5, 1.0), 5: ()}
 N_SLOTS = 16
 #: slots of the library (kMaxSlots); with PCHIP reconstruction the last two
 #: hold the reconstructed stage fields and the snapshot ring uses the rest
@@ -316,6 +316,12 @@ def _check_path(numerics: NumericsConfig, mesh: Mesh, is3d: bool, uv_layout: str
         raise ValueError(f"numerics.temporal_reconstruction: unknown {numerics.temporal_reconstruction!r}")
     if numerics.elem_search not in ("local", "walk"):
         raise ValueError(f"numerics.elem_search: unknown {numerics.elem_search!r}")
+    if numerics.integrator == "rk45":
+        # integrators.py:6485-6513
+        if is3d or numerics.uv_interp != "leastsq" or uv_layout != "element":
+            raise NotImplementedError("rk45 is 2-D element-layout least squares only (v1)")
+        if numerics.elem_search == "walk":
+            raise ValueError("rk45 does not support elem_search='walk' (v1)")
     if numerics.elem_search == "walk" and (uv_layout != "node" or mesh.vertical_coord == "sigma"):
         raise NotImplementedError("device path: elem_search='walk' with element-layout u/v or on a "
                                   "sigma mesh")
@@ -454,7 +460,7 @@ def run_single_realization(mesh: Mesh, ds, reader, init_state: ParticleState, mo
     recon_on = use_tinterp and multistage and numerics.temporal_reconstruction == "pchip"
     if recon_on:
         if order != 2:
-            raise NotImplementedError("device path: pchip reconstruction with rk4 (rk2 only)")
+            raise NotImplementedError(f"device path: pchip reconstruction with {numerics.integrator} (rk2 only)")
         if sig3d:
             raise NotImplementedError("device path: pchip reconstruction on a sigma mesh")
         if numerics.beaching.enabled and numerics.beaching.use_eta:
@@ -594,8 +600,13 @@ def run_single_realization(mesh: Mesh, ds, reader, init_state: ParticleState, mo
         sb = slots.slot(pay.ib) if pay.ib is not None else -1
         st = slots.slot(pay.it) if pay.it is not None else -1
         alphas = [0.0] * 4
-        if dtn is not None:
+        if dtn is not None and order != 5:
             alphas = [c * step_len / dtn for c in _C[order]] + [0.0] * (4 - order)
+        rk45 = None
+        if order == 5:
+            # integrators.py:6533-6545: the substep fractions t / dt_to_next
+            rk45 = dict(tol_m=numerics.rk45_tol_m, h_min=numerics.rk45_h_min,
+                        inv_dtn=0.0 if dtn is None else 1.0 / dtn)
         sigma = None
         if sig3d:
             sigma = _sigma_surfaces(use_tinterp, sim_time, sim_time_end, dt, direction, file_times,
@@ -623,7 +634,7 @@ def run_single_realization(mesh: Mesh, ds, reader, init_state: ParticleState, mo
         ctx.step(p, order, step_a, step_len, float(direction), slot_b=step_b, slot_t=step_t,
                  alpha0=step_a0, alphas=alphas, exact_nearest=exact, w_method=w_method,
                  kick=step_kick, beach=beach, step_i=g, real_id=realization_id, reuse_anchor=True,
-                 sigma=sigma, walk=walk, **periodic)
+                 sigma=sigma, walk=walk, rk45=rk45, **periodic)
         if prefetch and n_time_total and n_time_total > 1:
             # the snapshot after this step's bracket streams in while it runs
             held = [i for i in (pay.ia, pay.ib, pay.it) if i is not None]

@@ -12,9 +12,11 @@ from kelp_amd.particles import make_state
 
 def test_registry_names():
     assert I.get_integrator("rk2") is I.rk2_step
-    assert set(I.INTEGRATORS) == {"euler", "rk2", "rk4"}
+    # integrators.py:6635 INTEGRATORS
+    assert set(I.INTEGRATORS) == {"euler", "rk2", "rk4", "rk45"}
+    assert I.get_integrator("rk45") is I.rk45_step
     with pytest.raises(ValueError, match="Unknown integrator"):
-        I.get_integrator("rk45")
+        I.get_integrator("rk5")
 
 
 def test_stage_alphas_match_reference_formula():
