@@ -1392,7 +1392,10 @@ static int det_step(const or_mesh_t *m, const or_fields_t *f, const or_step_args
  * Dormand-Prince 5(4) over exactly dt (2-D, u/v at element centres by least
  * squares, steady or time-blended), then the re-validation of rk45_step
  * (:6586-6611).  The tableau is Dormand & Prince (1980); the error weights
- * are b5 - b4 as exact fractions.
+ * are b5 - b4 as exact fractions.  No reference golden holds an rk45 run:
+ * pinned by the uniform-flow closed form (one exact substep) and by agreeing
+ * with RK4 on 64 substeps at a tight tolerance, steady and blended, forward
+ * and backward (tests/test_oracle_rk45.py).
  * --------------------------------------------------------------------- */
 #define DP_C2 (1.0 / 5.0)
 #define DP_C3 (3.0 / 10.0)
@@ -2067,7 +2070,9 @@ static void az_sigma_one(const or_mesh_t *m, const or_step_args_t *a, uint64_t p
  * column of `node` at depth z clamped to [z_levels[0], min(depth, last
  * level)]; the value linear between interfaces (:357 _az_interp_column),
  * the gradient the slope of the segment the layer search picks (the one
- * above an interface the depth sits on). */
+ * above an interface the depth sits on).  No reference golden holds an
+ * Az-field run: pinned by the linear-profile closed form, the constant-field
+ * identity and the well-mixed condition (tests/test_oracle_az.py). */
 static void az_column(const or_mesh_t *m, const double *azf, int32_t node, double z, double *val, double *grad)
 {
     const int K = m->n_levels;
@@ -2608,7 +2613,9 @@ static double fc_slope(double d_prev, double d_next, double h_prev, double h_nex
 }
 
 /* temporal_reconstruct.py:184 _pchip_value_scalar over n values of four
- * snapshots (float64 inputs) at t0 < t1 < t2 < t3, evaluated at tq */
+ * snapshots (float64 inputs) at t0 < t1 < t2 < t3, evaluated at tq.
+ * pinned by the reference golden fesom2_node_rk2_fused_pchip (fwd/bwd) and
+ * by scipy's PchipInterpolator on the central interval. */
 void or_pchip4(int64_t n, const double *f0, const double *f1, const double *f2, const double *f3, double t0,
                double t1, double t2, double t3, double tq, double *out)
 {
