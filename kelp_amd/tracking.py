@@ -833,4 +833,18 @@ def run_tracking(config: AppConfig, *, mesh: Mesh, dataset, reader="fesom2_array
             sample_vars=list(config.mode.sample_vars or []),
             on_exhausted=config.time.on_exhausted, realization_id=r,
             particle_offset=int(config.data.particle_offset or 0), ctx=ctx, writer=rec))
+        if config.data.output_path:
+            # tracking.py:4015-4023: one file per realization (_r<i> when several)
+            import os
+            from .writers import run_attributes, write_trajectories
+            path = str(config.data.output_path)
+            if int(config.time.num_runs) > 1:
+                stem, ext = os.path.splitext(path)
+                path = f"{stem}_r{r}{ext}"
+            arrays = rec.as_arrays()
+            if not config.mode.Fdrift3D:
+                arrays.pop("vel_w", None)      # no vel_w in 2-D
+            origin = ds.attrs.get("time_origin") if isinstance(ds.attrs, dict) else None
+            write_trajectories(path, arrays, f"seconds since {origin}" if origin else "seconds",
+                               "standard", run_attributes(config, s0, n_steps, avg, mesh, numerics=numerics))
     return out

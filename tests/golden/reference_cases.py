@@ -526,7 +526,8 @@ def multiyear_case():
         data[var] = (("time", dim, "nod2"), np.ascontiguousarray(np.concatenate(parts)))
     nt = sum(MY_YEARS.values())
     data["ssh"] = (("time", "nod2"), np.full((nt, 4), 0.1, np.float32))
-    ds = ArrayDataset(data, time=np.arange(nt) * 86400.0)      # one record a day, from the first
+    # one record a day, from the first, stamped at 23:15 (tests/_fesom2_synth.py:304)
+    ds = ArrayDataset(data, time=np.arange(nt) * 86400.0, attrs={"time_origin": "1958-01-01 23:15:00"})
     seeds = np.array([(0.2, 0.3, 12.3), (0.5, 0.5, 17.6), (0.7, 0.2, 24.1),
                       (0.3, 0.8, 27.4), (0.72, 0.55, 8.7), (0.98, 0.97, 14.2)])
     cfg = AppConfig(

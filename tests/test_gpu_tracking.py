@@ -354,17 +354,27 @@ def test_healpix_run_matches_reference_golden(gpu_ctx, direction):
         np.testing.assert_allclose(rec[src], gold[k], rtol=1e-9, atol=1e-9, err_msg=k)
 
 
-def test_run_tracking_matches_reference_multiyear_golden(gpu_ctx):
+def test_run_tracking_matches_reference_multiyear_golden(gpu_ctx, tmp_path):
     """run_tracking(config) end to end (tests/_gen_golden_3d.py:740): two
     yearly files of daily means read as one axis and shifted to the window
     midpoints, RK2 across the year boundary, temp sampled at every record,
     failure reasons tracked, read-ahead of two; every trajectory variable of
     the reference's output file, its time axis and release metadata."""
     from kelp_amd.tracking import run_tracking
+    from kelp_amd.writers import read_trajectories
     case = rc.multiyear_case()
     gold = np.load(os.path.join(GOLD, "run_tracking_fesom2_multiyear_node.npz"), allow_pickle=False)
+    case["config"].data.output_path = str(tmp_path / "traj.nc")
     (rec,) = run_tracking(case["config"], mesh=case["mesh"], dataset=case["ds"], reader="fesom2_arrays",
                           init_state=case["init"], ctx=gpu_ctx)
+    # the output file: every variable the reference's file holds
+    f = read_trajectories(case["config"].data.output_path)
+    assert set(f) == {k for k in gold.files if not k.startswith(("attr__", "dims__"))}
+    for k in f:
+        if f[k].dtype.kind == "f":
+            np.testing.assert_allclose(f[k], gold[k], rtol=1e-9, atol=1e-9, err_msg=k)
+        else:
+            np.testing.assert_array_equal(f[k], gold[k], err_msg=k)
     out = rec.as_arrays()
     np.testing.assert_array_equal(out["status"], gold["status"])
     np.testing.assert_array_equal(out["failure_reason"], gold["failure_reason"])

@@ -48,3 +48,55 @@ def test_multiyear_oracle_matches_reference_golden(flavour):
     for k, g in (("lon", "lon"), ("lat", "lat"), ("z", "z"), ("u", "vel_u"), ("v", "vel_v"),
                  ("w", "vel_w")):
         np.testing.assert_allclose(rec[k], gold[g], rtol=1e-9, atol=1e-9, err_msg=k)
+
+
+def test_output_file_matches_the_reference_file(tmp_path):
+    """Trajectory output format 2 written to a file (classic NetCDF here)
+    against the reference's own output file of this run, as the golden
+    holds it: every variable's values (trajectories from the oracle, the
+    sampled temp and release data from the golden's run), dimensions and
+    attributes, and the run's global attributes; only the build identity
+    (kelp_version, history, w_contract) differs by design."""
+    from kelp_amd.writers import read_trajectories, run_attributes, write_trajectories
+    from kelp_amd.tracking import mask_record  # noqa: F401  (records come masked)
+    from scipy.io import netcdf_file
+    case = rc.multiyear_case()
+    cfg = case["config"]
+    gold = np.load(os.path.join(GOLD, "run_tracking_fesom2_multiyear_node.npz"), allow_pickle=False)
+    ft, _, _, avg = resolve_record_times(cfg, case["ds"])
+    set_exact_nearest(case["mesh"], True)
+    run = dict(mesh=case["mesh"], ds=case["ds"], mode=cfg.mode, numerics=cfg.numerics, init=case["init"],
+               file_times=ft, n_time_total=ft.size, start_time_idx=cfg.time.start_time_idx,
+               n_steps=cfg.time.n_steps)
+    rec = run_zlevel_oracle(run, "fesom2_arrays", "glibc")
+    arrays = dict(lon=rec["lon"], lat=rec["lat"], z=rec["z"], status=rec["status"],
+                  failure_reason=rec["fr"], vel_u=rec["u"], vel_v=rec["v"], vel_w=rec["w"],
+                  sample__temp=gold["temp"], time=ft[362] + np.arange(1, 13) * 21600.0,
+                  release_lon=case["init"].lon, release_lat=case["init"].lat, release_z=case["init"].z,
+                  release_time=np.full(6, ft[362]))
+    path = str(tmp_path / "traj.nc")
+    write_trajectories(path, arrays, time_units=str(gold["attr__time__units"]), calendar="standard",
+                       attrs=run_attributes(cfg, 362, 12, avg, case["mesh"]))
+    got = read_trajectories(path)
+    assert set(got) == {k for k in gold.files if not k.startswith(("attr__", "dims__"))}
+    for k in got:
+        if got[k].dtype.kind == "f":
+            np.testing.assert_allclose(got[k], gold[k], rtol=1e-9, atol=1e-9, err_msg=k)
+        else:
+            np.testing.assert_array_equal(got[k], gold[k], err_msg=k)
+    with netcdf_file(path, "r", mmap=False) as nc:
+        for name, var in nc.variables.items():
+            assert " ".join(var.dimensions) == str(gold["dims__" + name]), name
+            want = {k.split("__", 2)[2]: str(gold[k]) for k in gold.files
+                    if k.startswith(f"attr__{name}__")}
+            have = {k: (str(np.asarray(v)) if not isinstance(v, bytes) else v.decode())
+                    for k, v in var._attributes.items()}
+            assert have == want, (name, have, want)
+        build_identity = {"kelp_version", "history", "w_contract"}
+        want = {k[len("attr__"):]: str(gold[k]) for k in gold.files
+                if k.startswith("attr__") and k.count("__") == 1}
+        have = {k: (v.decode() if isinstance(v, bytes) else str(np.asarray(v).item() if np.ndim(v) == 0 or
+                                                                 np.size(v) == 1 else np.asarray(v)))
+                for k, v in nc._attributes.items()}
+        for k in set(want) - build_identity:
+            assert k in have and have[k] == want[k], (k, have.get(k), want[k])
