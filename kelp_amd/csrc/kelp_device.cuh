@@ -5,7 +5,7 @@
 // mesh layout built by the library at upload time:
 //
 //   lonlat[N]      double2 (lon, lat) per node: one 16-byte load per vertex
-//   gh_xyz[N]      double4 (x, y, z, node id bits) in grid-hash CSR order, so
+//   gh_xyz[N]      double4 (x, y, z, element-list range) in grid-hash CSR order, so
 //                  a row segment of the hash is one contiguous stream
 //   gh_offsets     int32 CSR offsets over cells (cj * nx + ci)
 //   elems[E]       int4 vertex ids (-1 padded): one 16-byte load per element
@@ -80,9 +80,14 @@ struct DMesh {
 
 __device__ __forceinline__ double sqd(double x) { return x * x; }
 
-__device__ __forceinline__ int32_t node_id_of(double4 p)
+// Node nn's element list n2e[k0 .. k1), from the w of its grid-hash entry
+// (internal node id = CSR slot, so gh_xyz[nn] is the node's entry): the
+// line the anchor search has just read, not a separate offsets array.
+__device__ __forceinline__ void n2e_range(const DMesh& m, int nn, int& k0, int& k1)
 {
-    return (int32_t)__double_as_longlong(p.w);
+    const double w = m.gh_xyz[nn].w;
+    k0 = __double2loint(w);
+    k1 = __double2hiint(w);
 }
 
 // core/mesh.py:1193 _gh_fold_dlon
@@ -141,7 +146,7 @@ __device__ inline void scan_range(const DMesh& m, int cj2, int c_lo, int c_hi, c
         double d2 = sqd(p.x - q.x) + sqd(p.y - q.y) + sqd(p.z - q.z);
         if (d2 < best_d2) {
             best_d2 = d2;
-            best = node_id_of(p);
+            best = k;  // the internal node id is the CSR slot
         }
     }
 }
@@ -764,7 +769,8 @@ __device__ __forceinline__ bool elem_touches(const DMesh& m, int c, int nn)
 template <int MAXV>
 __device__ __noinline__ int containing_2ring(const DMesh& m, double lon_p, double lat_p, int nn)
 {
-    int k0 = m.n2e_off[nn], k1 = m.n2e_off[nn + 1];
+    int k0, k1;
+    n2e_range(m, nn, k0, k1);
     for (int k = k0; k < k1; ++k) {
         int e = m.n2e[k];
         int4 en = m.elems[e];
@@ -772,7 +778,8 @@ __device__ __noinline__ int containing_2ring(const DMesh& m, double lon_p, doubl
         for (int j = 0; j < MAXV; ++j) {
             int v = elem_slot(en, j);
             if (v < 0 || v == nn) continue;
-            int q0 = m.n2e_off[v], q1 = m.n2e_off[v + 1];
+            int q0, q1;
+            n2e_range(m, v, q0, q1);
             for (int kk = q0; kk < q1; ++kk) {
                 int c = m.n2e[kk];
                 if (elem_touches<MAXV>(m, c, nn)) continue;
@@ -787,7 +794,8 @@ __device__ __noinline__ int containing_2ring(const DMesh& m, double lon_p, doubl
 template <int MAXV>
 __device__ inline int find_containing(const DMesh& m, double lon_p, double lat_p, int nn)
 {
-    int k0 = m.n2e_off[nn], k1 = m.n2e_off[nn + 1];
+    int k0, k1;
+    n2e_range(m, nn, k0, k1);
     if (k1 == k0) return -1;
     for (int k = k0; k < k1; ++k) {
         int c = m.n2e[k];
@@ -1168,7 +1176,8 @@ __device__ inline bool interp_lsq(const DMesh& m, const Snap<T, false>& fe, doub
 {
     U = 0.0;
     V = 0.0;
-    const int k0 = m.n2e_off[nn], k1 = m.n2e_off[nn + 1];
+    int k0, k1;
+    n2e_range(m, nn, k0, k1);
     if (k1 - k0 < 1) return false;
     const double z0 = km_cos(yp * kDeg2Rad) * kSLM;
     const double xp_m = xp * z0;
@@ -1320,7 +1329,8 @@ __device__ inline double w_bary(const DMesh& m, const Snap<T, true>& fw, double 
                                 double yp, double zp, int node, double time_dir)
 {
     const WBracket bz = w_bracket(m, zp);
-    const int k0 = m.n2e_off[node], k1 = m.n2e_off[node + 1];
+    int k0, k1;
+    n2e_range(m, node, k0, k1);
     for (int k = k0; k < k1; ++k) {
         const int e = m.n2e[k];
         int verts[4] = {0, 0, 0, 0};
@@ -1456,7 +1466,8 @@ template <int MAXV, typename T>
 __device__ inline double w_lsq(const DMesh& m, const Snap<T, true>& fw, double alpha0, double alpha, double xp,
                                double yp, double zp, int node, double time_dir)
 {
-    const int k0 = m.n2e_off[node], k1 = m.n2e_off[node + 1];
+    int k0, k1;
+    n2e_range(m, node, k0, k1);
     const WBracket bz = w_bracket(m, zp);
     double z0 = km_cos(yp * kDeg2Rad) * kSLM;
     if (z0 < 1.0) z0 = 1.0;

@@ -119,8 +119,8 @@ static int dev_alloc(kelp_ctx* c, T** out, size_t count)
 
 // core/mesh.py:713 _node_xyz_fill.  Internal node k is the node in slot k of
 // the hash's CSR order, so the table is already in scan order.
-__global__ void node_xyz_kernel(int64_t n, const double2* __restrict__ lonlat, double4* __restrict__ out,
-                                double* __restrict__ coslat)
+__global__ void node_xyz_kernel(int64_t n, const double2* __restrict__ lonlat, const int32_t* __restrict__ n2e_off,
+                                double4* __restrict__ out, double* __restrict__ coslat)
 {
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
@@ -132,7 +132,8 @@ __global__ void node_xyz_kernel(int64_t n, const double2* __restrict__ lonlat, d
     r.x = c * km_cos(lon_r);
     r.y = c * km_sin(lon_r);
     r.z = km_sin(lat_r);
-    r.w = __longlong_as_double((long long)k);
+    // the node's element-list range [start, end) (n2e_range)
+    r.w = __hiloint2double(n2e_off[k + 1], n2e_off[k]);
     out[k] = r;
 }
 
@@ -658,7 +659,7 @@ int kelp_mesh_upload(kelp_ctx* c, const kelp_mesh_desc* d)
     if (d_coastal) HIP_TRY(hipMemcpy(d_coastal, coastal.data(), E * sizeof(uint8_t), hipMemcpyHostToDevice));
     if (d_ecent) HIP_TRY(hipMemcpy(d_ecent, ecent.data(), E * sizeof(double2), hipMemcpyHostToDevice));
     if (d_enbr) HIP_TRY(hipMemcpy(d_enbr, enbr.data(), E * sizeof(int4), hipMemcpyHostToDevice));
-    node_xyz_kernel<<<grid_for(N, 256), 256, 0, c->stream>>>(N, d_lonlat, d_xyz, d_coslat);
+    node_xyz_kernel<<<grid_for(N, 256), 256, 0, c->stream>>>(N, d_lonlat, d_n2e_off, d_xyz, d_coslat);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(c->stream));
 

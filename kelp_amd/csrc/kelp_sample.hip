@@ -92,7 +92,8 @@ __device__ __forceinline__ double node_value(const DMesh& m, const SField& s, do
 template <int MAXV, typename F>
 __device__ inline int for_neighbours(const DMesh& m, int node, F&& fn)
 {
-    const int k0 = m.n2e_off[node], k1 = m.n2e_off[node + 1];
+    int k0, k1;
+    n2e_range(m, node, k0, k1);
     int count = 0;
     fn(node);
     ++count;
@@ -151,7 +152,8 @@ __device__ double combine(const DMesh& m, const SField& s, double z, double lon_
     }
     if (method == 2) {
         const double R = 6371000.0;
-        const int k0 = m.n2e_off[nn], k1 = m.n2e_off[nn + 1];
+        int k0, k1;
+        n2e_range(m, nn, k0, k1);
         for (int k = k0; k < k1; ++k) {
             const int e = m.n2e[k];
             if (e < 0) continue;

@@ -61,7 +61,8 @@ __device__ __forceinline__ double clamp_sigma(double z, double H, double Dmin, d
 template <int MAXV, typename F>
 __device__ inline int ring_nodes(const DMesh& m, int node, F&& fn)
 {
-    const int k0 = m.n2e_off[node], k1 = m.n2e_off[node + 1];
+    int k0, k1;
+    n2e_range(m, node, k0, k1);
     int count = 0;
     fn(node);
     ++count;
@@ -93,7 +94,8 @@ struct SigLoc {
 template <int MAXV>
 __device__ inline bool sigma_bary_locate(const DMesh& m, double xp, double yp, int node, SigLoc& L)
 {
-    const int k0 = m.n2e_off[node], k1 = m.n2e_off[node + 1];
+    int k0, k1;
+    n2e_range(m, node, k0, k1);
     for (int k = k0; k < k1; ++k) {
         const int e = m.n2e[k];
         if (e < 0) continue;
