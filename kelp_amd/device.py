@@ -70,6 +70,9 @@ class DeviceParticles:
                ("lon", "lat", "z", "status", "fr", "u", "v", "w", "pid", "ws")}
         nn = self.nn.clone()
         if self.ctx is not None:
+            # the clone ran on torch's stream, the id map runs on the
+            # context's: order them explicitly
+            self.ctx.torch.cuda.current_stream().synchronize()
             check(self.ctx.lib.kelp_nodes_to_mesh_ids(self.ctx.h, nn.numel(), nn.data_ptr()),
                   "kelp_nodes_to_mesh_ids")
             self.ctx.synchronize()
@@ -77,6 +80,7 @@ class DeviceParticles:
         if self.elem is not None:
             el = self.elem.clone()
             if self.ctx is not None:
+                self.ctx.torch.cuda.current_stream().synchronize()
                 check(self.ctx.lib.kelp_elems_to_mesh_ids(self.ctx.h, el.numel(), el.data_ptr()),
                       "kelp_elems_to_mesh_ids")
                 self.ctx.synchronize()
@@ -396,6 +400,9 @@ class DeviceContext:
         pid_t = (t.arange(P, dtype=t.int64, device=dev) if pid is None
                  else t.as_tensor(np.ascontiguousarray(pid, np.int64), device=dev))
         nn_t = i32(nn, -1)
+        # the tensors above were filled on torch's stream; the id maps below
+        # and every later call run on the context's stream
+        t.cuda.current_stream().synchronize()
         if nn is not None:
             check(self.lib.kelp_nodes_from_mesh_ids(self.h, nn_t.numel(), nn_t.data_ptr()),
                   "kelp_nodes_from_mesh_ids")

@@ -549,6 +549,11 @@ def run_single_realization(mesh: Mesh, ds, reader, init_state: ParticleState, mo
     for step_i in range(n_steps):
         g = start_step_i + step_i
         slots.begin_step()
+        # the re-sort permutes the particle arrays, so it runs before the
+        # activation writes the step-start status (prev_status is indexed
+        # by device slot and is not among the arrays the sort permutes)
+        if resort_every > 0 and g % resort_every == 0:
+            ctx.sort(p)
         ctx.activate(p, rel_dev, particle_offset, g, prev_status)
         if use_tinterp:
             bracket = resolve_time_bracket(sim_time, file_times, n_time_total, on_exhausted,
@@ -613,8 +618,6 @@ def run_single_realization(mesh: Mesh, ds, reader, init_state: ParticleState, mo
                                     n_time_total, on_exhausted, snap, slots, pay, order, sa, sb,
                                     st, alphas, step_len, dtn, file_idx if not use_tinterp else None)
             sigma.update(Dmin=numerics.Dmin, z0b_min=numerics.z0b_min)
-        if resort_every > 0 and g % resort_every == 0:
-            ctx.sort(p)
         step_a, step_b, step_t, step_a0 = sa, sb, st, pay.alpha0
         if stencil is not None:
             # temporal_reconstruct.py:298 make_stage_vels_recon: stage c at

@@ -1766,6 +1766,18 @@ static double column_height(const or_mesh_t *m, const double *eta, double Dmin, 
     return col_H(m, eta, Dmin, node);
 }
 
+/* vertical_sigma.py:1063 sigma_column_height over P particles (method 0
+ * nearest, 1 leastsq, 2 barycentric, 3 idw); entries of particles that are
+ * not alive are 1.0, as there.  Test entry point for the pins of the
+ * column-height interpolators. */
+void or_column_height_batch(const or_mesh_t *m, const double *eta, double Dmin, int method, int64_t P,
+                            const double *lon, const double *lat, const int32_t *nn, const int32_t *alive,
+                            double *out)
+{
+    for (int64_t i = 0; i < P; i++)
+        out[i] = alive[i] ? column_height(m, eta, Dmin, method, lon[i], lat[i], nn[i]) : 1.0;
+}
+
 /* vertical_sigma.py:971 interp_w_sigma_spatial for one particle */
 static double w_sigma(const or_mesh_t *m, const or_fields_t *f, const double *eta, double Dmin, int method,
                       double alpha, double xp, double yp, double z, int node, double time_dir)

@@ -175,6 +175,9 @@ def lib(flavour: str = "glibc") -> ctypes.CDLL:
     L.or_sample.argtypes = [POINTER(OrMesh), vp, vp, c_double, c_int, c_int, c_int64, vp, vp, vp,
                             vp, vp, vp]
     L.or_sample.restype = None
+    L.or_column_height_batch.argtypes = [POINTER(OrMesh), vp, c_double, c_int, c_int64, vp, vp, vp,
+                                         vp, vp]
+    L.or_column_height_batch.restype = None
     _LIBS[flavour] = L
     return L
 
@@ -269,6 +272,21 @@ class OracleMesh:
         self.L.or_sample(ctypes.byref(self.s), fa.ctypes.data, fb.ctypes.data, float(alpha),
                          int(n_planes), codes[method], lon.size, lon.ctypes.data, lat.ctypes.data,
                          z.ctypes.data, st.ctypes.data, nn_.ctypes.data, out.ctypes.data)
+        return out
+
+    def column_height(self, lon, lat, nn, eta, Dmin=0.5, method="idw", alive=None):
+        """vertical_sigma.py:1063 sigma_column_height: max(depth + eta,
+        Dmin) interpolated at each particle by ``method`` around anchor
+        ``nn`` (mesh ids); 1.0 where ``alive`` is False."""
+        codes = {"nearest": 0, "leastsq": 1, "barycentric": 2, "idw": 3}
+        lon, lat = _c(lon, np.float64), _c(lat, np.float64)
+        nn_ = _c(nn, np.int32)
+        al = _c(np.ones(lon.size) if alive is None else alive, np.int32)
+        e = _c(eta, np.float64)
+        out = np.empty(lon.size)
+        self.L.or_column_height_batch(ctypes.byref(self.s), e.ctypes.data, float(Dmin), codes[method],
+                                      lon.size, lon.ctypes.data, lat.ctypes.data, nn_.ctypes.data,
+                                      al.ctypes.data, out.ctypes.data)
         return out
 
     def element_contains(self, e, lon, lat) -> bool:

@@ -122,6 +122,38 @@ def test_staggered_release_run(gpu_ctx):
         assert moved[st["status"][sel, r] == 1].all()
 
 
+def test_resort_every_step_leaves_records_unchanged(gpu_ctx):
+    """The tracking loop's periodic re-sort (resort_every=1) with staggered
+    releases, beaching and resuspension: every record -- positions, status,
+    failure reasons and the velocities the record masks keep or blank from
+    the step-start status -- is bit-identical to the run that never sorts."""
+    case = rc.beaching_case(1.0)
+    init = case["init"]
+    P = init.n_particles
+    rel = np.zeros(P, np.int64)
+    rel[1::3] = 3
+    rel[2::3] = 7
+    status0 = init.status.copy()
+    status0[rel > 0] = 4
+    state = init.replace(status=status0, release_step=rel)
+
+    def run(every):
+        return run_single_realization(
+            case["mesh"], case["ds"], get_reader("fesom2_arrays"), state, case["mode"],
+            case["numerics"], start_time_idx=case["start_time_idx"], n_steps=case["n_steps"],
+            n_time_total=case["n_time_total"], file_times=case["file_times"],
+            file_dt=case["file_dt"], ctx=gpu_ctx, resort_every=every).as_arrays()
+
+    base, srt = run(0), run(1)
+    assert (base["status"] == 2).any(), "the case should beach particles"
+    for k in ("status", "failure_reason"):
+        np.testing.assert_array_equal(srt[k], base[k], err_msg=k)
+    for k in ("lon", "lat", "z", "vel_u", "vel_v", "vel_w"):
+        a, b = np.asarray(base[k], np.float64), np.asarray(srt[k], np.float64)
+        np.testing.assert_array_equal(np.isnan(a), np.isnan(b), err_msg=k + " mask")
+        np.testing.assert_array_equal(a.view(np.int64), b.view(np.int64), err_msg=k)
+
+
 @pytest.mark.parametrize("direction", [1.0, -1.0])
 def test_sigma_run_matches_reference_golden(gpu_ctx, direction):
     """FESOM-C sigma coordinates (tests/_gen_golden_3d.py:527): stretched

@@ -1,0 +1,217 @@
+"""Pins of the oracle's record-point sampling (numerics/sampling.py:83
+_combine_horizontal, :450 _interp_mid_linear_single) and of the sigma
+column height by inverse distance (numerics/vertical_sigma.py:534
+_H_idw_single) by properties the mathematics fixes, independent of the
+oracle's own expressions:
+
+* nearest returns the anchor node's value (2-D), and on a mid-layer field
+  linear in depth the anchor's value at the bottom-clamped depth;
+* barycentric (triangles) and Wachspress (quads) reproduce a field linear in
+  longitude and latitude (linear precision: the per-element projection is
+  an affine map of (lon, lat), and both weight families reproduce affine
+  functions);
+* inverse distance: a constant field comes back unchanged; a point on a
+  node returns that node's raw value (NaN included); the weight of a node is
+  proportional to 1/d^2 in the local metric (dlon cos(lat) 111120 m,
+  dlat 111120 m), so on a regular lattice the weights of two nodes on the
+  particle's own row stand in the ratio of their squared longitude
+  offsets, and a node on the particle's meridian against the anchor in the
+  closed form that carries cos(lat); a non-finite value is left out of both
+  sums, which fixes the result in terms of the full-ring weights;
+* the sigma column height by inverse distance obeys the same weight law on
+  a sloped, non-flat bed, returns the node's own column within 1 m of a
+  node, and floors every column at Dmin.
+
+Each planted mistake of a plausible kind (1/d for 1/d^2, no cos(lat) factor,
+NaN treated as 0, a swapped vertex weight) fails at least one of them.
+"""
+import numpy as np
+import pytest
+
+from kelp_amd import synth
+from kelp_amd.mesh import make_mesh
+from oracle.oracle import OracleMesh
+
+SLM = 111120.0
+STEP = 0.5
+
+
+def _quad_mesh(jitter=0.05, seed=4):
+    lon = np.arange(0.0, 10.0 + 1e-9, STEP)
+    lat = np.arange(30.0, 40.0 + 1e-9, STEP)
+    LON, LAT = np.meshgrid(lon, lat)
+    rng = np.random.default_rng(seed)
+    LON = LON + rng.uniform(-jitter, jitter, LON.shape)
+    LAT = LAT + rng.uniform(-jitter, jitter, LAT.shape)
+    nlat, nlon = LON.shape
+    nid = np.arange(nlat * nlon).reshape(nlat, nlon)
+    q = np.stack([nid[:-1, :-1].ravel(), nid[:-1, 1:].ravel(), nid[1:, 1:].ravel(),
+                  nid[1:, :-1].ravel()], 1).astype(np.int32)
+    return make_mesh("quads", LON.ravel(), LAT.ravel(), q)
+
+
+@pytest.fixture(scope="module")
+def meshes():
+    reg = synth.lattice_mesh(0.0, 10.0, 30.0, 40.0, STEP, jitter=0.0, name="regular")
+    jit = synth.lattice_mesh(0.0, 10.0, 30.0, 40.0, STEP, jitter=0.1, seed=9, name="jittered")
+    return dict(reg=reg, jit=jit, quad=_quad_mesh())
+
+
+def _near_nodes(mesh, n, seed, frac=0.25):
+    """Points within frac * STEP of random interior nodes (so the node is the
+    anchor and the containing element touches it)."""
+    rng = np.random.default_rng(seed)
+    lon, lat = mesh.nodes_lon, mesh.nodes_lat
+    inner = np.where((lon > 1.0) & (lon < 9.0) & (lat > 31.0) & (lat < 39.0))[0]
+    nodes = rng.choice(inner, n)
+    r = frac * STEP * np.sqrt(rng.random(n))
+    th = rng.uniform(0, 2 * np.pi, n)
+    return lon[nodes] + r * np.cos(th), lat[nodes] + r * np.sin(th)
+
+
+def _sample(om, values, lon, lat, method, z=None):
+    nn = om.nearest(lon, lat, exact=True)
+    st = np.ones(lon.size, np.int32)
+    return om.sample(values, lon, lat, z, st, nn, method=method), nn
+
+
+@pytest.mark.parametrize("flavour", ["glibc", "kmath"])
+def test_nearest_is_the_anchor_value(meshes, flavour):
+    m = meshes["jit"]
+    om = OracleMesh(m, flavour)
+    vals = np.random.default_rng(1).normal(size=m.n_nodes)
+    lon, lat = _near_nodes(m, 400, 2, frac=0.45)
+    out, nn = _sample(om, vals, lon, lat, "nearest")
+    assert np.array_equal(out, vals[nn])
+
+
+def test_nearest_mid_layer_linear_in_depth():
+    """A mid-layer field c_n + b z_mid: the anchor's value at
+    clamp(min(z, depth_n), z_mid[0], z_mid[-1])."""
+    base = synth.lattice_mesh(0.0, 10.0, 30.0, 40.0, STEP, jitter=0.1, seed=9)
+    zl = np.array([0.0, 10.0, 30.0, 70.0, 150.0, 310.0])
+    rng = np.random.default_rng(3)
+    depth = rng.uniform(20.0, 400.0, base.n_nodes)
+    m = make_mesh("layers", base.nodes_lon, base.nodes_lat, base.elem_nodes, depth=depth, z_levels=zl)
+    om = OracleMesh(m, "glibc")
+    zmid = 0.5 * (zl[:-1] + zl[1:])
+    c = rng.normal(size=m.n_nodes) * 10.0
+    b = 0.37
+    field = c[None, :] + b * zmid[:, None]
+    lon, lat = _near_nodes(m, 500, 4, frac=0.45)
+    z = rng.uniform(0.0, 400.0, lon.size)
+    out, nn = _sample(om, field, lon, lat, "nearest", z=z)
+    ze = np.clip(np.minimum(z, depth[nn]), zmid[0], zmid[-1])
+    np.testing.assert_allclose(out, c[nn] + b * ze, rtol=0, atol=1e-11)
+
+
+@pytest.mark.parametrize("kind", ["jit", "quad"])
+def test_barycentric_reproduces_linear_fields(meshes, kind):
+    m = meshes[kind]
+    om = OracleMesh(m, "glibc")
+    f = lambda lo, la: 3.0 + 0.7 * lo - 1.3 * la
+    vals = f(m.nodes_lon, m.nodes_lat)
+    lon, lat = _near_nodes(m, 600, 5)
+    out, nn = _sample(om, vals, lon, lat, "barycentric")
+    np.testing.assert_allclose(out, f(lon, lat), rtol=0, atol=1e-10)
+    # the fallback (the anchor's value) was not what passed
+    assert np.mean(np.abs(out - vals[nn]) > 1e-6) > 0.95
+
+
+def test_idw_constant_and_on_node(meshes):
+    m = meshes["jit"]
+    om = OracleMesh(m, "glibc")
+    lon, lat = _near_nodes(m, 300, 6)
+    out, _ = _sample(om, np.full(m.n_nodes, 3.25), lon, lat, "idw")
+    np.testing.assert_allclose(out, 3.25, rtol=4 * np.finfo(float).eps, atol=0)
+    # on a node: its raw value, also a NaN
+    vals = np.random.default_rng(7).normal(size=m.n_nodes)
+    nodes = np.random.default_rng(8).choice(m.n_nodes, 50)
+    vals[nodes[:10]] = np.nan
+    out, nn = _sample(om, vals, m.nodes_lon[nodes], m.nodes_lat[nodes], "idw")
+    assert np.array_equal(nn, nodes)
+    assert np.isnan(out[:10]).all()
+    assert np.array_equal(out[10:], vals[nodes[10:]])
+
+
+def _row_case(m, i_off=0.3):
+    """A particle on lattice row j at dlon = i_off * STEP east of node (i, j)
+    of the regular lattice: returns the particle and the node ids of (i, j),
+    (i-1, j), (i+1, j), (i, j+1)."""
+    nlon = int(round(10.0 / STEP)) + 1
+    i, j = 8, 9
+    nid = lambda a, b: b * nlon + a
+    lon_p = m.nodes_lon[nid(i, j)] + i_off * STEP
+    lat_p = m.nodes_lat[nid(i, j)]
+    return lon_p, lat_p, nid(i, j), nid(i - 1, j), nid(i + 1, j), nid(i, j + 1)
+
+
+def test_idw_weight_law(meshes):
+    m = meshes["reg"]
+    om = OracleMesh(m, "glibc")
+    lon_p, lat_p, n0, nw, ne, nn_ = _row_case(m)
+    lo, la = np.array([lon_p]), np.array([lat_p])
+
+    def weight(node, nan_at=None):
+        v = np.zeros(m.n_nodes)
+        v[node] = 1.0
+        if nan_at is not None:
+            v[nan_at] = np.nan
+        out, anchor = _sample(om, v, lo, la, "idw")
+        assert anchor[0] == n0
+        return out[0]
+
+    w0, ww, we, wn = weight(n0), weight(nw), weight(ne), weight(nn_)
+    assert min(w0, ww, we, wn) > 0
+    # same row: d^2 = (dlon z0)^2, z0 cancels
+    np.testing.assert_allclose(we / ww, (1.3 / 0.7) ** 2, rtol=1e-12)
+    np.testing.assert_allclose(w0 / we, (0.7 / 0.3) ** 2, rtol=1e-12)
+    # the node north of the anchor: (0.3 STEP z0)^2 + (STEP SLM)^2 against (0.3 STEP z0)^2
+    z0 = np.cos(np.radians(lat_p)) * SLM
+    dx = 0.3 * STEP * z0
+    np.testing.assert_allclose(wn / w0, dx ** 2 / (dx ** 2 + (STEP * SLM) ** 2), rtol=1e-12)
+    # a non-finite node leaves both sums: w_k' = w_k / (1 - w_m)
+    np.testing.assert_allclose(weight(ne, nan_at=nw), we / (1.0 - ww), rtol=1e-12)
+
+
+def test_sigma_idw_column_height_weight_law(meshes):
+    """H_p by inverse distance on a sloped bed with one deeper column: the
+    excess over the slope's idw is the deep node's idw weight, which obeys
+    the 1/d^2 law; within 1 m of a node the node's own column; Dmin floors."""
+    m = meshes["reg"]
+    om = OracleMesh(m, "glibc")
+    lon_p, lat_p, n0, nw, ne, nn_ = _row_case(m)
+    lo, la, anc = np.array([lon_p]), np.array([lat_p]), np.array([n0], np.int32)
+    slope = 40.0 + 6.0 * m.nodes_lon + 2.0 * m.nodes_lat
+    eta = 0.1 * np.sin(m.nodes_lon)
+
+    def H_with(depth):
+        om2 = OracleMesh(make_mesh("bed", m.nodes_lon, m.nodes_lat, m.elem_nodes, depth=depth), "glibc")
+        return om2.column_height(lo, la, anc, eta, Dmin=0.5, method="idw")[0]
+
+    h_slope = H_with(slope)
+    delta = 25.0
+    exc = {}
+    for k in (nw, ne, n0, nn_):
+        d = slope.copy()
+        d[k] += delta
+        exc[k] = (H_with(d) - h_slope) / delta
+    np.testing.assert_allclose(exc[ne] / exc[nw], (1.3 / 0.7) ** 2, rtol=1e-9)
+    z0 = np.cos(np.radians(lat_p)) * SLM
+    dx = 0.3 * STEP * z0
+    np.testing.assert_allclose(exc[nn_] / exc[n0], dx ** 2 / (dx ** 2 + (STEP * SLM) ** 2), rtol=1e-9)
+    # the weights sum to one: raising every column by delta raises H by delta
+    np.testing.assert_allclose(H_with(slope + delta) - h_slope, delta, rtol=1e-9)
+    # within 1 m of the anchor (1e-6 deg of longitude is ~0.09 m): the
+    # anchor's own column, exactly
+    om_s = OracleMesh(make_mesh("bed", m.nodes_lon, m.nodes_lat, m.elem_nodes, depth=slope), "glibc")
+    on = om_s.column_height(np.array([m.nodes_lon[n0] + 1e-6]), np.array([m.nodes_lat[n0]]), anc, eta,
+                            method="idw")
+    assert on[0] == slope[n0] + eta[n0]
+    # Dmin floors a dry column (depth + eta below Dmin) at the node itself
+    dry = slope.copy()
+    dry[n0] = -5.0
+    om3 = OracleMesh(make_mesh("dry", m.nodes_lon, m.nodes_lat, m.elem_nodes, depth=dry), "glibc")
+    h_on = om3.column_height(np.array([m.nodes_lon[n0]]), np.array([m.nodes_lat[n0]]), anc, eta,
+                             Dmin=0.5, method="idw")
+    assert h_on[0] == 0.5
