@@ -278,6 +278,17 @@ int kelp_step(kelp_ctx *ctx, const kelp_particles *p, const kelp_step_args *a);
  * on release order (numerics/integrators.py prange loops). */
 int kelp_sort_particles(kelp_ctx *ctx, const kelp_particles *p);
 
+/* kelp_sort_particles without any device allocation, for callers that
+ * cannot allocate (code under HIP graph capture, serving loops that forbid
+ * hipMalloc).  No counterpart in the reference.  The keys, permutation,
+ * staging and the radix sort's temporary storage live in a static device
+ * array (48 MiB per GPU, held until the process ends); at most 1,048,576
+ * particles per call (KELP_EINVAL above that).  All calls on one GPU share
+ * that array: calls that overlap on one GPU (several contexts or threads)
+ * race and must be serialised by the caller.  Same results as
+ * kelp_sort_particles. */
+int kelp_sort_particles_noalloc(kelp_ctx *ctx, const kelp_particles *p);
+
 /* Batch queries (core/mesh.py:1896 find_nearest_nodes and :2289
  * find_containing_elements) on device arrays. */
 int kelp_nearest(kelp_ctx *ctx, int64_t n, const double *lon, const double *lat,

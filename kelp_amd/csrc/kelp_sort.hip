@@ -15,6 +15,8 @@
 // by its slot.  Callers restore the release order from pid.
 #include <rocprim/device/device_radix_sort.hpp>
 
+#include <mutex>
+
 #include "kelp_internal.h"
 
 using namespace kelp;
@@ -61,6 +63,108 @@ int permute(kelp_ctx* c, int64_t n, const int32_t* perm, T* arr)
 }
 
 }  // namespace
+
+// ---------------------------------------------------------------------------
+// The same re-sort without any device allocation: keys, permutation,
+// staging and the radix sort's temporary storage live in one file-scope
+// device array.  Each device holds its own copy of it for the life of the
+// process; its address on a device comes from hipGetSymbolAddress (the
+// array's name in host code is a stand-in, not a device address).
+// ---------------------------------------------------------------------------
+
+namespace {
+
+constexpr int64_t kNoAllocMaxParticles = int64_t(1) << 20;
+// keys, keys2, idx, perm (4 B each) + staging (8 B) per particle, the rest
+// for the radix sort's temporary storage
+constexpr size_t kScratchBytes = size_t(48) << 20;
+constexpr int kMaxDevices = 64;
+
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+// this device's copy of `scratch` (looked up once per device, with the
+// device current)
+int scratch_on(int device, unsigned char** out)
+{
+    if (device < 0 || device >= kMaxDevices)
+        return kelp_fail(KELP_EINVAL, "kelp_sort_particles_noalloc: device %d out of range", device);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* ptr = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) return kelp_fail(KELP_EHIP, "kelp_sort_particles_noalloc: no address for the scratch array");
+        g_scratch_ptr[device] = static_cast<unsigned char*>(ptr);
+    }
+    *out = g_scratch_ptr[device];
+    return KELP_OK;
+}
+
+size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+template <typename T>
+int permute_via(kelp_ctx* c, int64_t n, const int32_t* perm, T* arr, void* stage_p)
+{
+    if (!arr) return KELP_OK;
+    T* stage = static_cast<T*>(stage_p);
+    gather_kernel<T><<<grid_of(n), 256, 0, c->stream>>>(n, perm, arr, stage);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(arr, stage, (size_t)n * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
+    return KELP_OK;
+}
+
+}  // namespace
+
+extern "C" int kelp_sort_particles_noalloc(kelp_ctx* c, const kelp_particles* p)
+{
+    if (!c || !p) return kelp_fail(KELP_EINVAL, "kelp_sort_particles_noalloc: NULL argument");
+    if (!c->has_mesh) return kelp_fail(KELP_ESTATE, "kelp_sort_particles_noalloc: no mesh uploaded");
+    const int64_t n = p->n;
+    if (n < 0) return kelp_fail(KELP_EINVAL, "kelp_sort_particles_noalloc: bad particle count");
+    if (n > kNoAllocMaxParticles)
+        return kelp_fail(KELP_EINVAL, "kelp_sort_particles_noalloc: %lld particles exceed the maximum of %lld",
+                         (long long)n, (long long)kNoAllocMaxParticles);
+    if (n <= 1) return KELP_OK;
+    if (!p->lon || !p->lat || !p->status)
+        return kelp_fail(KELP_EINVAL, "kelp_sort_particles_noalloc: lon, lat and status are required");
+    const uint64_t n_cells = (uint64_t)c->m.nx * (uint64_t)c->m.ny;
+    if (3 * n_cells >= (1ull << 32))
+        return kelp_fail(KELP_EINVAL, "kelp_sort_particles_noalloc: too many hash cells");
+    unsigned end_bit = 1;
+    while (end_bit < 32 && (1ull << end_bit) < 3 * n_cells) ++end_bit;
+    HIP_TRY(hipSetDevice(c->device));
+    unsigned char* base = nullptr;
+    int rc = scratch_on(c->device, &base);
+    if (rc) return rc;
+    const size_t b4 = align256((size_t)n * 4), b8 = align256((size_t)n * 8);
+    uint32_t* keys = reinterpret_cast<uint32_t*>(base);
+    uint32_t* keys2 = reinterpret_cast<uint32_t*>(base + b4);
+    int32_t* idx = reinterpret_cast<int32_t*>(base + 2 * b4);
+    int32_t* perm = reinterpret_cast<int32_t*>(base + 3 * b4);
+    void* stage = base + 4 * b4;
+    unsigned char* tmp = base + 4 * b4 + b8;
+    const size_t used = 4 * b4 + b8;
+    size_t tmp_bytes = 0;
+    HIP_TRY(rocprim::radix_sort_pairs(nullptr, tmp_bytes, keys, keys2, idx, perm, (unsigned int)n, 0u, end_bit,
+                                      c->stream));
+    if (used + tmp_bytes > kScratchBytes)
+        return kelp_fail(KELP_EINVAL, "kelp_sort_particles_noalloc: %lld particles need %zu scratch bytes (%zu held)",
+                         (long long)n, used + tmp_bytes, kScratchBytes);
+    sort_key_kernel<<<grid_of(n), 256, 0, c->stream>>>(c->m, n, p->lon, p->lat, p->status, keys, idx);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(rocprim::radix_sort_pairs(tmp, tmp_bytes, keys, keys2, idx, perm, (unsigned int)n, 0u, end_bit,
+                                      c->stream));
+    if ((rc = permute_via(c, n, perm, p->lon, stage)) || (rc = permute_via(c, n, perm, p->lat, stage)) ||
+        (rc = permute_via(c, n, perm, p->z, stage)) || (rc = permute_via(c, n, perm, p->status, stage)) ||
+        (rc = permute_via(c, n, perm, p->nn, stage)) || (rc = permute_via(c, n, perm, p->fr, stage)) ||
+        (rc = permute_via(c, n, perm, p->u, stage)) || (rc = permute_via(c, n, perm, p->v, stage)) ||
+        (rc = permute_via(c, n, perm, p->w, stage)) || (rc = permute_via(c, n, perm, p->pid, stage)) ||
+        (rc = permute_via(c, n, perm, p->wet_streak, stage)) || (rc = permute_via(c, n, perm, p->elem, stage)))
+        return rc;
+    return KELP_OK;
+}
 
 extern "C" int kelp_sort_particles(kelp_ctx* c, const kelp_particles* p)
 {
