@@ -6,7 +6,8 @@ counted on the device, over the wall time of the K timed steps) with the
 mesh, the snapshots and the particle state already resident in HBM; the
 roofline of the step kernel (algorithmic bytes per launch over its average
 duration, HIP events on the stream the kernel runs on); and a CPU baseline
-(the oracle, one host core, on a bounded sample of the same workload).
+(the C oracle with OpenMP over particles, on every host thread the process
+may use and on one, on a bounded sample of the same workload).
 
 Workloads:
   global_3d    (default) 1M-node Fibonacci-sphere Delaunay mesh with a land
@@ -52,6 +53,47 @@ def profiled_traffic(path):
     return None
 
 
+def host_threads():
+    """Host threads the CPU baseline may use: OMP_NUM_THREADS when set (the
+    process's CPU share), else the CPUs this process may run on."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0))
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_legs(make_oracle, run, n_total, n_per_thread, budget_s, what):
+    """The C oracle (glibc libm, OpenMP over particles) timed on all host
+    threads and on one: ``run(oracle, n, budget) -> (particle-steps/s,
+    steps)`` over the first ``n`` particles.  The all-thread leg takes
+    about two thirds of the budget and ``n_per_thread`` particles per
+    thread (up to the workload's count)."""
+    from oracle.oracle import set_threads
+    om = make_oracle()
+    threads = set_threads(host_threads())
+    n_all = min(n_total, n_per_thread * threads)
+    v_all, steps_all = run(om, n_all, budget_s * 2.0 / 3.0)
+    set_threads(1)
+    n_one = min(n_total, n_per_thread)
+    v_one, steps_one = run(om, n_one, budget_s / 3.0)
+    return {"value": v_all, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(), "value_1_thread": v_one,
+            "sample": f"{n_all} particles x {steps_all} steps {what} on {threads} threads; "
+                      f"{n_one} particles x {steps_one} steps on 1 thread; C oracle (glibc libm), "
+                      f"OpenMP over particles"}
+
+
 class Global3D:
     """RK4 3-D with time blend, random walk and beaching.
 
@@ -81,6 +123,7 @@ class Global3D:
                           depth_min=0.3, seed=20240501)
         self.sample_ts = not args.no_sample
         self.storage = getattr(args, "storage", "f32")
+        self.field_dtype = self.storage
         if self.storage == "bf16":
             self.b_alg = 857.0
             self.profile = None         # the profiled traffic is the float32 workload's
@@ -170,34 +213,34 @@ class Global3D:
     def cpu_baseline(self, budget_s):
         from oracle.oracle import OracleMesh, step_args
         w = self.w
-        om = OracleMesh(w.mesh, "glibc")
-        n = min(w.lon.size, 20_000)
-        st = dict(lon=w.lon[:n], lat=w.lat[:n], z=w.z[:n], status=np.ones(n, np.int32),
-                  nn=np.full(n, -1, np.int32))
-        steps, alive_steps = 0, 0
-        t0 = time.perf_counter()
-        while time.perf_counter() - t0 < budget_s and steps < w.n_steps:
-            t = steps * w.dt
-            ia, ib, a0, it, dtn = self.synth.snapshot_bracket(t, w.snap_dt, len(w.snaps))
-            A, T = w.snaps[ia], w.snaps[it]
-            f = dict(u_a=A[0], v_a=A[1], w_a=A[2], u_t=T[0], v_t=T[1], w_t=T[2], alpha0=a0)
-            if ib >= 0:
-                B = w.snaps[ib]
-                f.update(u_b=B[0], v_b=B[1], w_b=B[2])
-            al = [c * w.dt / dtn for c in (0.0, 0.5, 0.5, 1.0)]
-            alive_steps += int(np.sum(st["status"] == 1))
-            st = om.step(st, f, step_args(4, w.dt, 1.0, is3d=True, w_method=2, exact=1, alphas=al,
-                                          kick=self.kick, beach=self.beach, step_i=steps))
-            if self.sample_ts:
-                ra, rb, ra0, _, _ = self.synth.snapshot_bracket(t + w.dt, w.snap_dt, len(w.snaps))
-                for var in (3, 4):
-                    om.sample(w.snaps[ra][var], st["lon"], st["lat"], st["z"], st["status"], st["nn"],
-                              values_b=None if rb < 0 else w.snaps[rb][var], alpha=ra0)
-            steps += 1
-        el = time.perf_counter() - t0
-        return {"value": alive_steps / el, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-                "sample": f"{n} particles x {steps} steps of the same workload (incl. T/S "
-                          f"sampling), C oracle (glibc libm), 1 thread, {el:.1f} s"}
+
+        def run(om, n, budget):
+            st = dict(lon=w.lon[:n].copy(), lat=w.lat[:n].copy(), z=w.z[:n].copy(),
+                      status=np.ones(n, np.int32), nn=np.full(n, -1, np.int32))
+            steps, alive_steps = 0, 0
+            t0 = time.perf_counter()
+            while time.perf_counter() - t0 < budget and steps < w.n_steps:
+                t = steps * w.dt
+                ia, ib, a0, it, dtn = self.synth.snapshot_bracket(t, w.snap_dt, len(w.snaps))
+                A, T = w.snaps[ia], w.snaps[it]
+                f = dict(u_a=A[0], v_a=A[1], w_a=A[2], u_t=T[0], v_t=T[1], w_t=T[2], alpha0=a0)
+                if ib >= 0:
+                    B = w.snaps[ib]
+                    f.update(u_b=B[0], v_b=B[1], w_b=B[2])
+                al = [c * w.dt / dtn for c in (0.0, 0.5, 0.5, 1.0)]
+                alive_steps += int(np.sum(st["status"] == 1))
+                st = om.step(st, f, step_args(4, w.dt, 1.0, is3d=True, w_method=2, exact=1, alphas=al,
+                                              kick=self.kick, beach=self.beach, step_i=steps))
+                if self.sample_ts:
+                    ra, rb, ra0, _, _ = self.synth.snapshot_bracket(t + w.dt, w.snap_dt, len(w.snaps))
+                    for var in (3, 4):
+                        om.sample(w.snaps[ra][var], st["lon"], st["lat"], st["z"], st["status"],
+                                  st["nn"], values_b=None if rb < 0 else w.snaps[rb][var], alpha=ra0)
+                steps += 1
+            return alive_steps / (time.perf_counter() - t0), steps
+
+        return cpu_legs(lambda: OracleMesh(w.mesh, "glibc_omp"), run, w.lon.size, 20_000, budget_s,
+                        "of the same workload (incl. T/S sampling)")
 
 
 class DoubleGyre:
@@ -230,22 +273,228 @@ class DoubleGyre:
     def cpu_baseline(self, budget_s):
         from oracle.oracle import OracleMesh
         w = self.w
-        om = OracleMesh(w.mesh, "glibc")
-        n = min(w.lon.size, 10_000)
-        st = dict(lon=w.lon[:n].copy(), lat=w.lat[:n].copy(), status=np.ones(n, np.int32),
-                  nn=np.full(n, -1, np.int32))
-        steps, alive_steps = 0, 0
-        t0 = time.perf_counter()
-        while time.perf_counter() - t0 < budget_s:
-            alive_steps += int(np.sum(st["status"] == 1))
-            st = om.step2d(st, 2, w.u, w.v, w.dt, 1.0)
-            steps += 1
-        el = time.perf_counter() - t0
-        return {"value": alive_steps / el, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-                "sample": f"{n} particles x {steps} steps, C oracle (glibc libm), 1 thread, {el:.1f} s"}
+
+        def run(om, n, budget):
+            st = dict(lon=w.lon[:n].copy(), lat=w.lat[:n].copy(), status=np.ones(n, np.int32),
+                      nn=np.full(n, -1, np.int32))
+            steps, alive_steps = 0, 0
+            t0 = time.perf_counter()
+            while time.perf_counter() - t0 < budget:
+                alive_steps += int(np.sum(st["status"] == 1))
+                st = om.step2d(st, 2, w.u, w.v, w.dt, 1.0)
+                steps += 1
+            return alive_steps / (time.perf_counter() - t0), steps
+
+        return cpu_legs(lambda: OracleMesh(w.mesh, "glibc_omp"), run, w.lon.size, 10_000, budget_s,
+                        "of the same workload")
 
 
 WORKLOADS = {"global_3d": Global3D, "double_gyre": DoubleGyre}
+
+
+def _alone_rate(ctx, specs, axis3, axis2, snap_dt, order, dt, budget_s, sigma=None, beach=None, ctx2d=None):
+    """The plain path on the same clouds, one cloud at a time (each step one
+    kelp_step launch over that cloud), for up to ``budget_s`` seconds:
+    particle-steps/s (alive at the step start) and how much was run.  2-D
+    clouds run on ``ctx2d`` when given (a sigma mesh's plain step takes 3-D
+    fields only)."""
+    from kelp_amd import synth
+    C = {1: (0.0,), 2: (0.0, 0.5), 4: (0.0, 0.5, 0.5, 1.0)}
+    done_steps, n_clouds = 0, 0
+    t0 = time.perf_counter()
+    for spec in specs:
+        if time.perf_counter() - t0 > budget_s:
+            break
+        cx = ctx2d if (ctx2d is not None and not spec.is3d) else ctx
+        t = cx.torch
+        p = cx.particles(spec.lon, spec.lat, spec.z if spec.is3d else None, pid=np.arange(spec.n))
+        p.alive_count = t.zeros(1, dtype=t.int64, device=p.lon.device)
+        slot0, n = axis3 if spec.is3d else axis2
+        kick = dict(kh_x=spec.kh, kh_y=spec.kh, seed=spec.seed) if spec.kh > 0 else None
+        for k in range(spec.n_steps):
+            if n <= 1:
+                sa, sb, st, a0, al = slot0, -1, -1, 0.0, [0.0] * 4
+            else:
+                tt = spec.t0 + spec.direction * (k * dt)
+                ia, ib, a0, it, dtn = synth.snapshot_bracket(tt, snap_dt, n, spec.direction)
+                sa, sb = slot0 + ia, (slot0 + ib if ib >= 0 else -1)
+                st, al = ((-1, [0.0] * 4) if dt > dtn else
+                          (slot0 + it, [c * dt / dtn for c in C[order]] + [0.0] * (4 - order)))
+            cx.step(p, order, sa, dt, spec.direction, slot_b=sb, slot_t=st, alpha0=a0, alphas=al,
+                    exact_nearest=1, w_method=2, kick=kick, step_i=k, reuse_anchor=True,
+                    beach=None if beach is None else dict(beach, seed=spec.seed),
+                    sigma=sigma if spec.is3d else None)
+        cx.synchronize()
+        done_steps += int(p.alive_count.item())
+        n_clouds += 1
+    el = time.perf_counter() - t0
+    return done_steps / el, n_clouds, el
+
+
+def run_batch(args):
+    """The batching workload: 256 clouds on the mixed triangle/quad lattice
+    with 20 sigma layers, all advanced together until the longest is done."""
+    from kelp_amd import synth
+    from kelp_amd.batching import Batch, CloudSpec, autotune
+    from kelp_amd.device import DeviceContext
+    t = time.time()
+    mesh = synth.mixed_lattice_mesh()
+    u, v, w, eta, u2, v2 = synth.mixed_lattice_fields(mesh)
+    clouds = synth.batch_clouds(mesh, n_clouds=args.clouds)
+    specs = [CloudSpec(**c) for c in clouds]
+    P = sum(s.n for s in specs)
+    log(f"batch workload built in {time.time() - t:.1f} s: {mesh.n_nodes} nodes, {mesh.n_elements} "
+        f"elements, {len(specs)} clouds, {P} particles")
+    ctx = DeviceContext(0)
+    torch = ctx.torch
+    ctx.upload_mesh(mesh)
+    ctx.upload_field(0, u, v, w)
+    ctx.upload_eta(0, eta)
+    ctx.upload_field(1, u2, v2)
+    sigma = dict(start=(0, -1, -1, 0.0, 0.0), end=None, Dmin=0.5, z0b_min=0.1)
+    order, dt = 2, synth.DAY
+    kw = dict(order=order, dt=dt, exact_nearest=1, w_method=2, sigma=sigma)
+    b = Batch(ctx, P, axis3=(0, 1), axis2=(1, 1), max_clouds=len(specs), **kw)
+    for s_ in specs:
+        b.add(s_)
+    choice = autotune(b, steps=8, warmup=2, blocks=(64, 128, 256), resort=(0, 4, 8, 16), graph=(1, 8))
+    log(f"autotuner: block {choice.block}, re-sort every {choice.resort_every}, graph of "
+        f"{choice.graph_steps} steps ({choice.rate / 1e6:.1f} M particle-steps/s)")
+    n_total = max(s_.n_steps for s_ in specs)
+    b.alive_steps(reset=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.steps(n_total)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    alive = b.alive_steps()
+    pss = alive / wall
+    log("plain path, cloud by cloud, for comparison")
+    import copy
+    m2 = copy.copy(mesh)            # the horizontal mesh as a z-level mesh for the 2-D clouds
+    m2.sigma = np.array([], dtype=np.float64)
+    m2.vertical_coord = "zlevel"
+    ctx2 = DeviceContext(0)
+    ctx2.upload_mesh(m2)
+    ctx2.upload_field(1, u2, v2)
+    plain, n_pl, el_pl = _alone_rate(ctx, specs, (0, 1), (1, 1), 0.0, order, dt, args.cpu_budget, sigma=sigma,
+                                     ctx2d=ctx2)
+    ctx2.close()
+    line = {
+        "metric": "particle-steps/s", "value": pss, "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": n_total, "warmup": 0, "ms_per_step": wall / n_total * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "field_dtype": "f32", "data": "synthetic",
+        "config": {"workload": "batch_256_clouds_mixed_lattice_sigma20", "mesh_nodes": mesh.n_nodes,
+                   "mesh_elements": mesh.n_elements, "quads": int((mesh.elem_nodes[:, 3] >= 0).sum()),
+                   "sigma_layers": int(mesh.sigma.size - 1), "clouds": len(specs), "particles": P,
+                   "integrator": "rk2", "dt_s": dt, "fields": "steady",
+                   "clouds_3d": sum(s_.is3d for s_ in specs), "clouds_backward": sum(s_.direction < 0 for s_ in specs),
+                   "clouds_kh": sum(s_.kh > 0 for s_ in specs), "alive_particle_steps": alive,
+                   "autotune": {"block": choice.block, "resort_every": choice.resort_every,
+                                "graph_steps": choice.graph_steps, "candidates": choice.table}},
+        "plain_path": {"value": plain, "unit": "particle-steps/s", "clouds": n_pl, "seconds": el_pl,
+                       "what": "the same clouds one at a time through kelp_step"},
+        "speedup_vs_plain": pss / plain if plain > 0 else None,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+def run_serving(args):
+    """The serving workload on the 3-D global mesh's fields (resident, cycled
+    six-hourly snapshots): Poisson request arrivals against the wall clock,
+    continuous batching, latency percentiles."""
+    from kelp_amd import synth
+    from kelp_amd.batching import CloudSpec, TuneChoice
+    from kelp_amd.device import DeviceContext
+    from kelp_amd.serving import Pool, Request, Server, latency_summary, poisson_stream
+    t = time.time()
+    w = synth.global_3d_workload(n_nodes=args.nodes or 1_000_000, n_particles=1)
+    log(f"fields built in {time.time() - t:.1f} s")
+    ctx = DeviceContext(0)
+    ctx.upload_mesh(w.mesh)
+    st = "bf16" if args.storage == "bf16" else None
+    for s_, (U, V, W, _, _) in enumerate(w.snaps):
+        ctx.upload_field(s_, U, V, W, storage=st)
+    ctx.upload_field(len(w.snaps), w.snaps[0][0][0].copy(), w.snaps[0][1][0].copy(), storage=st)
+    mesh = w.mesh
+    n_snap = len(w.snaps)
+    dt = 3600.0
+
+    def make(rid, rng, arrival):
+        n = int(min(500_000, max(1, round(5000.0 * np.exp(args.size_sigma * rng.standard_normal())))))
+        # a random spherical cap of radius 1-10 degrees, seeds on wet elements inside it
+        rad = np.radians(rng.uniform(1.0, 10.0))
+        cz = rng.uniform(-1.0, 1.0)
+        clon = rng.uniform(-180.0, 180.0)
+        c = np.array([np.sqrt(1 - cz * cz) * np.cos(np.radians(clon)),
+                      np.sqrt(1 - cz * cz) * np.sin(np.radians(clon)), cz])
+        lon_out, lat_out = [], []
+        need = n
+        while need > 0:
+            m = max(4 * need, 256)
+            zc = rng.uniform(np.cos(rad), 1.0, m)
+            ph = rng.uniform(0, 2 * np.pi, m)
+            e1 = np.cross(c, [0.0, 0.0, 1.0] if abs(c[2]) < 0.9 else [1.0, 0.0, 0.0])
+            e1 /= np.linalg.norm(e1)
+            e2 = np.cross(c, e1)
+            sz = np.sqrt(1 - zc * zc)
+            p = zc[:, None] * c + (sz * np.cos(ph))[:, None] * e1 + (sz * np.sin(ph))[:, None] * e2
+            lo = np.degrees(np.arctan2(p[:, 1], p[:, 0]))
+            la = np.degrees(np.arcsin(np.clip(p[:, 2], -1, 1)))
+            nn = ctx.nearest(lo, la, exact=False)
+            inside = ctx.containing(lo, la, nn) >= 0
+            lon_out.append(lo[inside][:need])
+            lat_out.append(la[inside][:need])
+            need -= int(min(inside.sum(), need))
+            if m > 50 * n and need == n:
+                break           # a cap on land: resample the centre next time
+        lon = np.concatenate(lon_out) if lon_out else np.zeros(0)
+        lat = np.concatenate(lat_out) if lat_out else np.zeros(0)
+        if lon.size == 0:
+            return make(rid, rng, arrival)
+        days = rng.uniform(1.0, 90.0)
+        return Request(rid=rid, lon=lon, lat=lat, z=rng.uniform(0.0, 1000.0, lon.size),
+                       n_steps=int(round(days * 24)), direction=float(rng.choice([1.0, -1.0])),
+                       is3d=bool(rng.random() < 0.5), kh=float(rng.choice([0.0, 10.0])),
+                       seed=int(rng.integers(1, 2 ** 62)), t0=float(rng.uniform(0, n_snap * w.snap_dt)),
+                       arrival=arrival)
+
+    t = time.time()
+    reqs = poisson_stream(args.rate, args.serve_seconds, 5050, make)
+    log(f"{len(reqs)} requests generated in {time.time() - t:.1f} s "
+        f"({sum(r.n for r in reqs)} particles, {sum(r.n * r.n_steps for r in reqs) / 1e9:.2f} G particle-steps)")
+    pool = Pool(ctx, axis3=(0, n_snap), axis2=(n_snap, 1), snap_dt=w.snap_dt, max_capacity=args.pool,
+                max_clouds=8192, choice=TuneChoice(256, 8, 4), order=2, dt=dt, exact_nearest=1, w_method=2)
+    srv = Server({"c2": pool}, chunk=4)
+    for r in reqs:
+        r.pool = "c2"
+    res = srv.run(reqs, realtime=True)
+    lat = latency_summary(res)
+    pss = srv.alive_steps / srv.wall
+    log(f"served {len(res)} requests in {srv.wall:.1f} s: {pss / 1e6:.1f} M particle-steps/s, "
+        f"p50 {lat['p50_s']:.2f} s, p99 {lat['p99_s']:.2f} s")
+    # the plain path on the first requests, one at a time
+    specs = [CloudSpec(lon=r.lon, lat=r.lat, z=r.z, n_steps=r.n_steps, direction=r.direction, is3d=r.is3d,
+                       kh=r.kh, seed=r.seed, t0=r.t0) for r in sorted(reqs, key=lambda q: q.rid)]
+    plain, n_pl, el_pl = _alone_rate(ctx, specs, (0, n_snap), (n_snap, 1), w.snap_dt, 2, dt, args.cpu_budget)
+    line = {
+        "metric": "particle-steps/s", "value": pss, "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": None, "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "field_dtype": args.storage, "data": "synthetic",
+        "config": {"workload": "serving_poisson_global_3d", "mesh_nodes": mesh.n_nodes, "rate_hz": args.rate,
+                   "arrival_window_s": args.serve_seconds, "requests": len(reqs),
+                   "size": "lognormal median 5000, sigma %.2f, cap 500000" % args.size_sigma,
+                   "duration": "uniform 1-90 days at 1 h", "integrator": "rk2", "dt_s": dt,
+                   "pool_capacity": pool.batch.capacity},
+        "latency": lat,
+        "wall_s": srv.wall,
+        "plain_path": {"value": plain, "unit": "particle-steps/s", "requests": n_pl, "seconds": el_pl,
+                       "what": "the same requests one at a time through kelp_step"},
+        "speedup_vs_plain": pss / plain if plain > 0 else None,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
 
 
 def main():
@@ -253,7 +502,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=120)
     ap.add_argument("--warmup", type=int, default=12)
-    ap.add_argument("--workload", default="global_3d", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="global_3d", choices=sorted(WORKLOADS) + ["batch", "serving"])
     ap.add_argument("--particles", type=int, default=0)
     ap.add_argument("--nodes", type=int, default=0)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
@@ -271,6 +520,11 @@ def main():
                     help="global_3d: leave out the T/S sampling at each step's end")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first copy a buffer of this size 3x (calibrates profiler byte counters)")
+    ap.add_argument("--clouds", type=int, default=256, help="batch: number of release clouds")
+    ap.add_argument("--rate", type=float, default=50.0, help="serving: request arrivals per second")
+    ap.add_argument("--serve-seconds", type=float, default=20.0, help="serving: arrival window (s)")
+    ap.add_argument("--size-sigma", type=float, default=1.0, help="serving: lognormal sigma of cloud sizes")
+    ap.add_argument("--pool", type=int, default=8_000_000, help="serving: particle slots at most")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
@@ -278,6 +532,10 @@ def main():
         sys.exit(2)
 
     from kelp_amd.device import DeviceContext
+    if args.workload == "batch":
+        return run_batch(args)
+    if args.workload == "serving":
+        return run_serving(args)
     wl = WORKLOADS[args.workload](args)
     ctx = DeviceContext(0)
     torch = ctx.torch
@@ -323,18 +581,26 @@ def main():
     avg_launch_s = kern_ms / 1e3 / args.steps
     per_launch_units = alive_steps / args.steps
     achieved = wl.b_alg * per_launch_units / avg_launch_s / 1e9
-    # measured HBM bytes per launch, valid for the default workload size only
-    traffic = None
-    if getattr(wl, "profile", None) and not (args.particles or args.nodes):
+    # measured HBM bytes per launch: the profile summary's figure, valid only
+    # for the configuration it was measured on (default sizes, f32 storage,
+    # T/S sampling, re-sort every 8 steps, all snapshots resident)
+    traffic, traffic_src = None, None
+    profiled_cfg = (not (args.particles or args.nodes) and args.resort_every == 8
+                    and not args.no_sample and not args.ring and args.storage == "f32")
+    if getattr(wl, "profile", None) and profiled_cfg:
         traffic = profiled_traffic(wl.profile)
+        if traffic is not None:
+            traffic_src = os.path.relpath(wl.profile, ROOT)
     line = {
         "metric": "particle-steps/s", "value": pss, "unit": "particle-steps/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "vs_baseline": None, "dtype": "f64", "field_dtype": getattr(wl, "field_dtype", "f32"),
+        "data": "synthetic",
         "config": dict(wl.config, alive_end=alive_end, resort_every=args.resort_every),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "kernel": "step_kernel", "kernel_avg_us": avg_launch_s * 1e6,
                      "b_alg_per_particle_step": wl.b_alg,
                      "particle_steps_per_launch": per_launch_units},

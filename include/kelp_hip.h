@@ -36,6 +36,17 @@ extern "C" {
  * back exactly as float32 values.  No counterpart in the reference, which
  * keeps float32 or float64 snapshots: halves the records' HBM footprint. */
 #define KELP_DT_BF16 2
+/* float16 storage with a power-of-two scale per level (node-layout z-level
+ * snapshots; host planes float32): every u/v layer and every W interface
+ * gets the scale 2^(q - 15) of its largest finite |value| m = f 2^q (u and v
+ * of a layer share the larger), the values are stored as half(value /
+ * scale) rounded to nearest even and read back exactly as half * scale.
+ * No counterpart in the reference (float32 / float64 snapshots). */
+#define KELP_DT_F16S 3
+
+/* kelp_step_args.sig_start_a / sig_end_a: the surface given by
+ * kelp_sigma_surface_upload instead of a blend of slot surfaces. */
+#define KELP_SURFACE_GIVEN (-2)
 
 typedef struct kelp_ctx kelp_ctx;
 
@@ -159,7 +170,83 @@ typedef struct {
      * rk45_h_min seconds (<= 0: dt 1e-4); rk45_inv_dtn = 1 / dt_to_next (0:
      * steady), stage fractions (t / dt_to_next) take the place of alphas */
     double rk45_tol_m, rk45_h_min, rk45_inv_dtn;
+    int32_t block_size;            /* threads per workgroup of the batched step (kelp_batch_*):
+                                      64, 128 or 256; 0 = 256.  kelp_step runs 256. */
 } kelp_step_args;
+
+/* ---------------------------------------------------------------------------
+ * Batched clouds: many independent releases advanced together, one kernel
+ * per step.  No single counterpart in the reference: it runs each release
+ * (a realization, utils/split_seeds.py:89, tracking.py:1805
+ * run_single_realization) as its own loop; here every cloud is such a run,
+ * and its particles' results equal that run's (same integrator, dt, fields
+ * and draws).  A particle belongs to cloud pid >> KELP_CLOUD_SHIFT and its
+ * random draws are keyed by the cloud-local part pid & (2^40 - 1) with the
+ * cloud's seed and the cloud-local step, so a cloud's results do not depend
+ * on which other clouds share the batch.
+ * ------------------------------------------------------------------------- */
+#define KELP_CLOUD_SHIFT 40
+#define KELP_CLOUD_VALID 1      /* the entry describes a cloud */
+#define KELP_CLOUD_3D 2         /* 3-D cloud (the 3-D snapshot axis), else 2-D */
+#define KELP_CLOUD_KICK 4       /* split uniform-Kh random walk */
+#define KELP_MAX_CLOUDS 65536
+
+typedef struct {
+    int32_t flags;       /* KELP_CLOUD_* */
+    int32_t n_steps;     /* steps the cloud runs */
+    int64_t start_step;  /* batch step of its first step */
+    double time_dir;     /* +1 forward, -1 backward */
+    double t0;           /* time of its first step on the snapshot axis (s) */
+    double kh_x, kh_y;   /* Kh of its random walk (m^2/s) */
+    uint64_t seed, real_id; /* counter-RNG key of its kicks and beaching */
+} kelp_cloud;
+
+/* The snapshot axes the clouds read: slots slot0 .. slot0 + n_snap - 1 hold
+ * snapshots at 0, snap_dt, 2 snap_dt, ... cycled (time.on_exhausted:
+ * cycle); n_snap 1 = a steady field.  A cloud at time t steps with the
+ * bracket of t (the step payload of tracking.py:3136 _build_vel_payload:
+ * alpha0 snapped to 0 / 1 within 1e-12, the stage target b forward and a
+ * backward, stage fractions c_k dt / dt_to_next).  2-D clouds read the 2-D
+ * axis, 3-D clouds the 3-D one (unused axis: n_snap 0). */
+typedef struct {
+    int32_t slot0_3d, n_snap_3d;
+    int32_t slot0_2d, n_snap_2d;
+    double snap_dt;
+} kelp_batch_axis;
+
+/* Write cloud entries [first, first + n) of the context's cloud table
+ * (grown as needed, at most KELP_MAX_CLOUDS; entries past the old end are
+ * cleared) and set the axes.  Takes effect at the next batched step. */
+int kelp_batch_set_clouds(kelp_ctx *ctx, int64_t first, int64_t n, const kelp_cloud *host_clouds,
+                          const kelp_batch_axis *axis);
+/* Start a new table of n_clouds empty entries (no cloud runs) and set the
+ * step counter to 0. */
+int kelp_batch_reset(kelp_ctx *ctx, int64_t n_clouds);
+/* Set the batch step counter (the step the next kelp_batch_step runs). */
+int kelp_batch_set_step(kelp_ctx *ctx, int64_t step);
+/* Read the batch step counter (waits for the context stream). */
+int kelp_batch_get_step(kelp_ctx *ctx, int64_t *step);
+/* One batched step: every cloud with start_step <= step < start_step +
+ * n_steps advances its particles by one step (the base arguments give the
+ * integrator, dt, anchoring, W method, beaching and, on sigma meshes, the
+ * surfaces; slot and time fractions come from each cloud's bracket, and
+ * time_dir, kick, seed, real_id and step_i from the cloud); particles of
+ * other clouds, and entries whose cloud is not in the table (free pool
+ * entries), are left alone.  Then the step counter advances.  base->slot_a
+ * names a slot of the axes (dtype and shape); on sigma meshes the 3-D axis
+ * must be steady.  Node-layout u/v. */
+int kelp_batch_step(kelp_ctx *ctx, const kelp_particles *p, const kelp_step_args *base);
+/* n_steps batched steps as one HIP graph launch on the context stream: the
+ * graph is captured at the first call and reused while the particle
+ * arrays, the base arguments, n_steps and the cloud table's allocation stay
+ * the same (the table's contents may change between launches). */
+int kelp_batch_run_graph(kelp_ctx *ctx, const kelp_particles *p, const kelp_step_args *base, int n_steps);
+/* Copy the particles of cloud `cloud` into host-visible or device arrays
+ * indexed by their cloud-local id (pid & (2^40 - 1) < n_out): lon, lat, z
+ * (may be NULL), status, fr; then, when `release` is set, mark them free
+ * (status 0, pid of no cloud), so a re-sort moves them to the end. */
+int kelp_batch_collect(kelp_ctx *ctx, const kelp_particles *p, int64_t cloud, int64_t n_out, double *lon,
+                       double *lat, double *z, int32_t *status, int8_t *fr, int release);
 
 const char *kelp_last_error(void);
 int kelp_version(void);
@@ -204,6 +291,14 @@ int kelp_field_upload_elem(kelp_ctx *ctx, int slot, int dtype, int64_t n_planes,
  * in mesh node order) to an uploaded slot; read by beaching with use_eta. */
 int kelp_field_upload_eta(kelp_ctx *ctx, int slot, const double *host_eta);
 int kelp_field_clear(kelp_ctx *ctx, int slot);
+
+/* Sigma meshes: the surface elevation a step starts (which 0) or ends
+ * (which 1) on, given as an array -- the sigma_eta_start / sigma_eta_end
+ * keywords of the reference's step functions (numerics/integrators.py:
+ * 5642-5643, used at :1381 _sigma_end_eta and :1423 _sigma_given).  (N,)
+ * doubles in mesh node order; NULL forgets the surface.  A step selects it
+ * with sig_start_a / sig_end_a = KELP_SURFACE_GIVEN. */
+int kelp_sigma_surface_upload(kelp_ctx *ctx, int which, const double *host_eta);
 
 /* Node scalar `var` (0..7) of a snapshot slot for record-point sampling
  * (VelocityField.scalars, numerics/sampling.py:727): n_planes x N doubles in

@@ -15,6 +15,7 @@ KELP_OK = 0
 KELP_DT_F64 = 0
 KELP_DT_F32 = 1
 KELP_DT_BF16 = 2
+KELP_DT_F16S = 3
 
 
 class KelpError(RuntimeError):
@@ -74,7 +75,30 @@ class StepArgs(ctypes.Structure):
         ("Dmin", c_double), ("z0b_min", c_double), ("walk", c_int32),
         ("az_field", c_int32), ("az_slot_a", c_int32), ("az_slot_b", c_int32), ("az_alpha0", c_double),
         ("rk45_tol_m", c_double), ("rk45_h_min", c_double), ("rk45_inv_dtn", c_double),
+        ("block_size", c_int32),
     ]
+
+
+class Cloud(ctypes.Structure):
+    """kelp_cloud: one cloud of a batch."""
+    _fields_ = [
+        ("flags", c_int32), ("n_steps", c_int32), ("start_step", c_int64),
+        ("time_dir", c_double), ("t0", c_double), ("kh_x", c_double), ("kh_y", c_double),
+        ("seed", ctypes.c_uint64), ("real_id", ctypes.c_uint64),
+    ]
+
+
+class BatchAxis(ctypes.Structure):
+    """kelp_batch_axis: the snapshot axes the clouds of a batch read."""
+    _fields_ = [
+        ("slot0_3d", c_int32), ("n_snap_3d", c_int32), ("slot0_2d", c_int32), ("n_snap_2d", c_int32),
+        ("snap_dt", c_double),
+    ]
+
+
+CLOUD_SHIFT = 40
+CLOUD_VALID, CLOUD_3D, CLOUD_KICK = 1, 2, 4
+MAX_CLOUDS = 65536
 
 
 #: every symbol include/kelp_hip.h declares, with its (restype, argtypes)
@@ -96,6 +120,7 @@ SIGNATURES = {
     "kelp_field_upload_elem": (c_int, [c_void_p, c_int, c_int, c_int64, c_void_p, c_void_p,
                                        c_void_p, c_int64]),
     "kelp_field_upload_eta": (c_int, [c_void_p, c_int, c_void_p]),
+    "kelp_sigma_surface_upload": (c_int, [c_void_p, c_int, c_void_p]),
     "kelp_field_clear": (c_int, [c_void_p, c_int]),
     "kelp_scalar_upload": (c_int, [c_void_p, c_int, c_int, c_int64, c_void_p]),
     "kelp_sample": (c_int, [c_void_p, POINTER(Particles), c_int, c_int, c_int, c_double, c_int,
@@ -121,6 +146,14 @@ SIGNATURES = {
     "kelp_event_record": (c_int, [c_void_p, c_int]),
     "kelp_event_elapsed": (c_int, [c_void_p, c_int, c_int, POINTER(c_float)]),
     "kelp_math_probe": (c_int, [c_void_p, c_int, c_int64, c_void_p, c_void_p, c_void_p]),
+    "kelp_batch_set_clouds": (c_int, [c_void_p, c_int64, c_int64, c_void_p, POINTER(BatchAxis)]),
+    "kelp_batch_reset": (c_int, [c_void_p, c_int64]),
+    "kelp_batch_set_step": (c_int, [c_void_p, c_int64]),
+    "kelp_batch_get_step": (c_int, [c_void_p, POINTER(c_int64)]),
+    "kelp_batch_step": (c_int, [c_void_p, POINTER(Particles), POINTER(StepArgs)]),
+    "kelp_batch_run_graph": (c_int, [c_void_p, POINTER(Particles), POINTER(StepArgs), c_int]),
+    "kelp_batch_collect": (c_int, [c_void_p, POINTER(Particles), c_int64, c_int64, c_void_p, c_void_p,
+                                   c_void_p, c_void_p, c_void_p, c_int]),
 }
 
 _LIB = None

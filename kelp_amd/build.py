@@ -39,6 +39,14 @@ UNITS = [("kelp_hip", "kelp_hip.hip", []), ("kelp_sort", "kelp_sort.hip", []),
 ] + [
     (f"kelp_step_{'3d' if d3 else '2d'}_bf16_node", "kelp_step_variant.hip",
      [f"-DKV_IS3D={d3}", "-DKV_F32=0", "-DKV_BF16=1", "-DKV_ELEM=0"]) for d3 in (0, 1)
+] + [
+    (f"kelp_step_{'3d' if d3 else '2d'}_f16s_node", "kelp_step_variant.hip",
+     [f"-DKV_IS3D={d3}", "-DKV_F32=0", "-DKV_F16S=1", "-DKV_ELEM=0"]) for d3 in (0, 1)
+] + [("kelp_batch_host", "kelp_batch_host.hip", [])] + [
+    (f"kelp_batch_{dt}_{'sigma' if sg else 'zlevel'}", "kelp_batch.hip",
+     [f"-DKB_F32={int(dt == 'f32')}", f"-DKB_BF16={int(dt == 'bf16')}", f"-DKB_F16S={int(dt == 'f16s')}",
+      f"-DKB_SIGMA={sg}"])
+    for dt, sg in (("f32", 0), ("f32", 1), ("f64", 0), ("f64", 1), ("bf16", 0), ("f16s", 0))
 ]
 
 
@@ -56,16 +64,46 @@ def _run(cmd):
     return r.stderr
 
 
-def build_hip(verbose: bool = False, jobs: int = 0) -> str:
+def _up_to_date(obj: str, dep: str, cmd) -> bool:
+    """True when ``obj`` is newer than every file its dependency list names
+    and was built by the same command line."""
+    try:
+        with open(dep) as fh:
+            text = fh.read()
+        with open(obj + ".cmd") as fh:
+            if fh.read() != " ".join(cmd):
+                return False
+        t = os.path.getmtime(obj)
+    except OSError:
+        return False
+    files = text.replace("\\\n", " ").split(":", 1)[-1].split()
+    return all(os.path.exists(f) and os.path.getmtime(f) <= t for f in files)
+
+
+def build_hip(verbose: bool = False, jobs: int = 0, incremental: bool = False) -> str:
+    """Compile every unit and link.  ``incremental``: skip units whose
+    object is newer than the sources and headers it was built from."""
     os.makedirs(OBJ_DIR, exist_ok=True)
     jobs = jobs or min(len(UNITS), max(1, os.cpu_count() or 1))
     cmds, objs = [], []
     for name, src, defs in UNITS:
         obj = os.path.join(OBJ_DIR, name + ".o")
+        dep = os.path.join(OBJ_DIR, name + ".d")
         objs.append(obj)
-        cmds.append([hipcc(), *HIPCC_FLAGS, *defs, "-c", "-o", obj, os.path.join(CSRC, src)])
+        cmd = [hipcc(), *HIPCC_FLAGS, *defs, "-c", "-o", obj, os.path.join(CSRC, src)]
+        if incremental and _up_to_date(obj, dep, cmd):
+            continue
+        cmds.append(cmd + ["-MD", "-MF", dep])
+
+    def _compile(cmd):
+        out = _run(cmd)
+        obj = cmd[cmd.index("-o") + 1]
+        with open(obj + ".cmd", "w") as fh:
+            fh.write(" ".join(cmd[:-3]))
+        return out
+
     with ThreadPoolExecutor(jobs) as ex:
-        logs = list(ex.map(_run, cmds))
+        logs = list(ex.map(_compile, cmds))
     tmp = OUT + ".tmp"
     logs.append(_run([hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]))
     if verbose:
@@ -80,11 +118,11 @@ def build_oracle() -> None:
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
 
 
-def build(verbose: bool = False) -> str:
-    out = build_hip(verbose)
+def build(verbose: bool = False, incremental: bool = False) -> str:
+    out = build_hip(verbose, incremental=incremental)
     build_oracle()
     return out
 
 
 if __name__ == "__main__":
-    print(build(verbose=True))
+    print(build(verbose=True, incremental="--incremental" in sys.argv))

@@ -1092,6 +1092,36 @@ __device__ __forceinline__ Bf16 to_bf16(float f)
     return Bf16{(uint16_t)(u >> 16)};
 }
 
+// float16 storage with a power-of-two scale per level (KELP_DT_F16S): a
+// record holds (u, v[, W top, W bottom]) divided by their level's scale as
+// IEEE half values; the scales of record row L (one row of n_nodes records
+// per layer) are the float4 (s_uv, s_uv, s_w(L), s_w(L + 1)) of the slot's
+// scale table.  A value reads back as half -> float -> double times its
+// scale, exactly (the scale is a power of two), so the arithmetic downstream
+// is that of a float32 field holding the decoded values.
+struct F16s {
+    uint16_t bits;
+};
+struct __align__(8) F16sx4 {
+    uint16_t x, y, z, w;
+};
+struct __align__(4) F16sx2 {
+    uint16_t x, y;
+};
+template <>
+struct RecOf<F16s, true> {
+    using type = F16sx4;
+};
+template <>
+struct RecOf<F16s, false> {
+    using type = F16sx2;
+};
+
+__device__ __forceinline__ double f16_val(uint16_t h, float scale)
+{
+    return (double)(float)__builtin_bit_cast(_Float16, h) * (double)scale;
+}
+
 // bracket pair (a, b) blended by alpha0 and the stage target t
 template <typename T, bool IS3D>
 struct Snap {
@@ -1099,6 +1129,19 @@ struct Snap {
     const R* __restrict__ a;
     const R* __restrict__ b;
     const R* __restrict__ t;
+};
+
+// float16 records carry their slots' per-row scale tables
+template <bool IS3D>
+struct Snap<F16s, IS3D> {
+    using R = typename RecOf<F16s, IS3D>::type;
+    const R* __restrict__ a;
+    const R* __restrict__ b;
+    const R* __restrict__ t;
+    const float4* __restrict__ sa;
+    const float4* __restrict__ sb;
+    const float4* __restrict__ st;
+    uint32_t row_len;  // records per row (the mesh's node count)
 };
 
 // numerics/vertical.py:1043 _tblend_scalar == interpolate.py:1419-1437: the
@@ -1152,6 +1195,49 @@ __device__ __forceinline__ void uv_blend(const Snap<T, IS3D>& f, int64_t idx, do
     v = (1.0 - alpha) * va0 + alpha * (double)rt.y;
 }
 
+// the same for float16 records: each value times its row's scale
+template <bool IS3D>
+__device__ __forceinline__ void uv_blend(const Snap<F16s, IS3D>& f, int64_t idx, double alpha0, double alpha,
+                                         double& u, double& v)
+{
+    const uint32_t row = (uint32_t)idx / f.row_len;
+    if (alpha < 1e-12) {
+        const auto ra = f.a[idx];
+        const float4 sa = f.sa[row];
+        if (alpha0 == 0.0) {
+            u = f16_val(ra.x, sa.x);
+            v = f16_val(ra.y, sa.y);
+            return;
+        }
+        const auto rb = f.b[idx];
+        const float4 sb = f.sb[row];
+        u = (1.0 - alpha0) * f16_val(ra.x, sa.x) + alpha0 * f16_val(rb.x, sb.x);
+        v = (1.0 - alpha0) * f16_val(ra.y, sa.y) + alpha0 * f16_val(rb.y, sb.y);
+        return;
+    }
+    if (alpha > 1.0 - 1e-12) {
+        const auto rt = f.t[idx];
+        const float4 st = f.st[row];
+        u = f16_val(rt.x, st.x);
+        v = f16_val(rt.y, st.y);
+        return;
+    }
+    const auto ra = f.a[idx];
+    const auto rt = f.t[idx];
+    const float4 sa = f.sa[row], st = f.st[row];
+    if (alpha0 == 0.0) {
+        u = (1.0 - alpha) * f16_val(ra.x, sa.x) + alpha * f16_val(rt.x, st.x);
+        v = (1.0 - alpha) * f16_val(ra.y, sa.y) + alpha * f16_val(rt.y, st.y);
+        return;
+    }
+    const auto rb = f.b[idx];
+    const float4 sb = f.sb[row];
+    double ua0 = (1.0 - alpha0) * f16_val(ra.x, sa.x) + alpha0 * f16_val(rb.x, sb.x);
+    double va0 = (1.0 - alpha0) * f16_val(ra.y, sa.y) + alpha0 * f16_val(rb.y, sb.y);
+    u = (1.0 - alpha) * ua0 + alpha * f16_val(rt.x, st.x);
+    v = (1.0 - alpha) * va0 + alpha * f16_val(rt.y, st.y);
+}
+
 // blended W of the two interfaces bounding record idx (w_top, w_bottom)
 template <typename T>
 __device__ __forceinline__ void w_pair_blend(const Snap<T, true>& f, int64_t idx, double alpha0, double alpha,
@@ -1162,6 +1248,18 @@ __device__ __forceinline__ void w_pair_blend(const Snap<T, true>& f, int64_t idx
     auto rt = f.t[idx];
     w_top = tblend((double)ra.z, (double)rb.z, (double)rt.z, alpha0, alpha);
     w_bot = tblend((double)ra.w, (double)rb.w, (double)rt.w, alpha0, alpha);
+}
+
+__device__ __forceinline__ void w_pair_blend(const Snap<F16s, true>& f, int64_t idx, double alpha0, double alpha,
+                                             double& w_top, double& w_bot)
+{
+    const uint32_t row = (uint32_t)idx / f.row_len;
+    const auto ra = f.a[idx];
+    const auto rb = f.b[idx];
+    const auto rt = f.t[idx];
+    const float4 sa = f.sa[row], sb = f.sb[row], st = f.st[row];
+    w_top = tblend(f16_val(ra.z, sa.z), f16_val(rb.z, sb.z), f16_val(rt.z, st.z), alpha0, alpha);
+    w_bot = tblend(f16_val(ra.w, sa.w), f16_val(rb.w, sb.w), f16_val(rt.w, st.w), alpha0, alpha);
 }
 
 // interpolate.py:207 _interp_leastsq_core_tinterp (== :52 for a plain field):

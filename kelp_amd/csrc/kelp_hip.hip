@@ -70,6 +70,7 @@ static void free_mesh(kelp_ctx* c)
     c->node_old_of_new = c->node_new_of_old = c->elem_old_of_new = c->elem_new_of_old = nullptr;
     c->vertical_coord = 0;
     c->sig_eta = nullptr;
+    c->sig_given[0] = c->sig_given[1] = false;
 }
 
 void kelp_free_scalar(ScalarField& f)
@@ -86,6 +87,7 @@ static void free_slot(FieldSlot& s)
     if (s.eta) (void)hipFree(s.eta);
     if (s.az) (void)hipFree(s.az);
     if (s.pair) (void)hipFree(s.pair);
+    if (s.scale) (void)hipFree(s.scale);
     for (auto& f : s.scalars) kelp_free_scalar(f);
     for (void* p : s.stage)
         if (p) (void)hipFree(p);
@@ -234,6 +236,104 @@ __global__ void pack2d_bf16_kernel(int64_t N, const int32_t* __restrict__ old_of
     rec[k] = r;
 }
 
+// float32 planes into float16 records with a power-of-two scale per level
+// (KELP_DT_F16S): the largest finite |value| of every u and v layer row and
+// every W interface row, the scale 2^(q - 15) of a row maximum m = f 2^q
+// (f in [0.5, 1), so |value| / scale < 2^15, inside the half range; 1 for
+// an empty or all-zero row, u and v of a layer sharing the larger), then
+// value * (1 / scale) rounded to the nearest even half.
+__global__ void f16s_clear_kernel(int64_t n, uint32_t* __restrict__ mx)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) mx[i] = 0u;
+}
+
+// row r of blockIdx.y: u rows [0, n_u), v rows [n_u, 2 n_u), W rows after
+__global__ __launch_bounds__(256) void f16s_max_kernel(int64_t N, int64_t n_u, const float* __restrict__ u,
+                                                       const float* __restrict__ v, const float* __restrict__ w,
+                                                       uint32_t* __restrict__ mx)
+{
+    __shared__ float red[256];
+    const int64_t r = blockIdx.y;
+    const float* row = r < n_u ? u + r * N : (r < 2 * n_u ? v + (r - n_u) * N : w + (r - 2 * n_u) * N);
+    float m = 0.0f;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < N; k += (int64_t)gridDim.x * blockDim.x) {
+        const float a = fabsf(row[k]);
+        if (a <= 3.402823466e38f && a > m) m = a;  // finite values only
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h && red[threadIdx.x + h] > red[threadIdx.x]) red[threadIdx.x] = red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMax(&mx[r], __float_as_uint(red[0]));
+}
+
+__device__ __forceinline__ float f16s_scale_of(uint32_t mbits)
+{
+    const float m = __uint_as_float(mbits);
+    if (!(m > 0.0f)) return 1.0f;
+    int q;
+    (void)frexpf(m, &q);
+    int e = q - 15;
+    if (e < -126) e = -126;
+    if (e > 127) e = 127;
+    return ldexpf(1.0f, e);
+}
+
+__global__ void f16s_scale_kernel(int64_t n_layers, int is3d, const uint32_t* __restrict__ mx,
+                                  float4* __restrict__ scale)
+{
+    const int64_t L = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (L >= n_layers) return;
+    const uint32_t mu = mx[L], mv = mx[n_layers + L];
+    const float suv = f16s_scale_of(mu > mv ? mu : mv);
+    float4 sc;
+    sc.x = suv;
+    sc.y = suv;
+    sc.z = is3d ? f16s_scale_of(mx[2 * n_layers + L]) : 0.0f;
+    sc.w = is3d ? f16s_scale_of(mx[2 * n_layers + L + 1]) : 0.0f;
+    scale[L] = sc;
+}
+
+__device__ __forceinline__ uint16_t to_f16s(float x, float inv_scale)
+{
+    return __builtin_bit_cast(uint16_t, (_Float16)(x * inv_scale));
+}
+
+__global__ void pack3d_f16s_kernel(int64_t N, int64_t n_layers, const int32_t* __restrict__ old_of_new,
+                                   const float* __restrict__ u, const float* __restrict__ v,
+                                   const float* __restrict__ w, const float4* __restrict__ scale,
+                                   F16sx4* __restrict__ rec)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N * n_layers) return;
+    int64_t L = i / N, k = i - L * N;
+    int64_t o = old_of_new[k];
+    const float4 sc = scale[L];
+    F16sx4 r;
+    r.x = to_f16s(u[L * N + o], 1.0f / sc.x);
+    r.y = to_f16s(v[L * N + o], 1.0f / sc.y);
+    r.z = to_f16s(w[L * N + o], 1.0f / sc.z);
+    r.w = to_f16s(w[(L + 1) * N + o], 1.0f / sc.w);
+    rec[i] = r;
+}
+
+__global__ void pack2d_f16s_kernel(int64_t N, const int32_t* __restrict__ old_of_new, const float* __restrict__ u,
+                                   const float* __restrict__ v, const float4* __restrict__ scale,
+                                   F16sx2* __restrict__ rec)
+{
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    int64_t o = old_of_new[k];
+    const float4 sc = scale[0];
+    F16sx2 r;
+    r.x = to_f16s(u[o], 1.0f / sc.x);
+    r.y = to_f16s(v[o], 1.0f / sc.y);
+    rec[k] = r;
+}
+
 // element-centre u/v planes (mesh element order) into (u, v) records per
 // (layer, element) in internal element order
 template <typename T>
@@ -338,6 +438,27 @@ static void launch_step_t(kelp_ctx* c, int order, const FieldSlot& a, const Fiel
         launch_step_v<3, IS3D, T, ELEM>(c->stream, order, c->m, f, fe, cfg, ps);
     else
         launch_step_v<4, IS3D, T, ELEM>(c->stream, order, c->m, f, fe, cfg, ps);
+}
+
+// float16 records: the Snap also carries each slot's row scales
+template <bool IS3D>
+static void launch_step_f16s(kelp_ctx* c, int order, const FieldSlot& a, const FieldSlot* b, const FieldSlot* t,
+                             const StepCfg& cfg, const PState& ps)
+{
+    using R = typename RecOf<F16s, IS3D>::type;
+    Snap<F16s, IS3D> f;
+    f.a = static_cast<const R*>(a.rec);
+    f.b = b ? static_cast<const R*>(b->rec) : f.a;
+    f.t = t ? static_cast<const R*>(t->rec) : f.a;
+    f.sa = a.scale;
+    f.sb = b ? b->scale : a.scale;
+    f.st = t ? t->scale : a.scale;
+    f.row_len = (uint32_t)c->m.n_nodes;
+    const Snap<F16s, false> fe{};
+    if (c->m.max_v == 3)
+        launch_step_v<3, IS3D, F16s, false>(c->stream, order, c->m, f, fe, cfg, ps);
+    else
+        launch_step_v<4, IS3D, F16s, false>(c->stream, order, c->m, f, fe, cfg, ps);
 }
 
 template <typename T, bool ELEM>
@@ -480,8 +601,10 @@ int kelp_ctx_destroy(kelp_ctx* c)
     if (c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);
     free_mesh(c);
     for (auto& s : c->slots) free_slot(s);
-    for (DevBuf* b : {&c->sort_tmp, &c->sort_keys, &c->sort_keys2, &c->sort_idx, &c->sort_perm, &c->sort_stage})
+    for (DevBuf* b : {&c->sort_tmp, &c->sort_keys, &c->sort_keys2, &c->sort_idx, &c->sort_perm, &c->sort_stage,
+                      &c->batch_clouds, &c->batch_steps, &c->batch_counter})
         kelp_release(*b);
+    if (c->batch_graph) (void)hipGraphExecDestroy(c->batch_graph);
     for (int k = 0; k < kMaxEvents; ++k)
         if (c->events[k]) (void)hipEventDestroy(c->events[k]);
     if (c->compute_mark) (void)hipEventDestroy(c->compute_mark);
@@ -639,7 +762,7 @@ int kelp_mesh_upload(kelp_ctx* c, const kelp_mesh_desc* d)
         (d->host_elem_coastal && (rc = dev_alloc(c, &d_coastal, E))) ||
         (has_centres && (rc = dev_alloc(c, &d_ecent, E))) || (rc = dev_alloc(c, &d_eframe, 4 * E)) ||
         (rc = dev_alloc(c, &d_eflags, E)) ||
-        (d->vertical_coord == 1 && (rc = dev_alloc(c, &d_sig_eta, 5 * N))) ||
+        (d->vertical_coord == 1 && (rc = dev_alloc(c, &d_sig_eta, 7 * N))) ||
         (d->host_elem_neighbors && (rc = dev_alloc(c, &d_enbr, E)))) {
         free_mesh(c);
         return rc;
@@ -820,6 +943,24 @@ static hipError_t launch_pack(kelp_ctx* c, hipStream_t stream, FieldSlot& s, int
                                                                      (const float*)tv, (Bf16x2*)s.rec);
         return hipGetLastError();
     }
+    if (dtype == KELP_DT_F16S) {  // node layout only (checked by the callers)
+        const int64_t n_rows = 2 * n_planes + (is3d ? n_planes + 1 : 0);
+        float4* scale = s.scale;
+        uint32_t* mx = reinterpret_cast<uint32_t*>(s.scale + n_planes);
+        f16s_clear_kernel<<<grid_for(n_rows, 256), 256, 0, stream>>>(n_rows, mx);
+        const unsigned gx = (unsigned)std::min<int64_t>((N + 255) / 256, 64);
+        f16s_max_kernel<<<dim3(gx, (unsigned)n_rows), 256, 0, stream>>>(N, n_planes, (const float*)tu,
+                                                                         (const float*)tv, (const float*)tw, mx);
+        f16s_scale_kernel<<<grid_for(n_planes, 64), 64, 0, stream>>>(n_planes, is3d ? 1 : 0, mx, scale);
+        if (is3d)
+            pack3d_f16s_kernel<<<grid_for(N * n_planes, 256), 256, 0, stream>>>(
+                N, n_planes, c->node_old_of_new, (const float*)tu, (const float*)tv, (const float*)tw, scale,
+                (F16sx4*)s.rec);
+        else
+            pack2d_f16s_kernel<<<grid_for(N, 256), 256, 0, stream>>>(N, c->node_old_of_new, (const float*)tu,
+                                                                     (const float*)tv, scale, (F16sx2*)s.rec);
+        return hipGetLastError();
+    }
     if (is3d) {
         const unsigned g = grid_for(N * n_planes, 256);
         if (f32)
@@ -857,10 +998,12 @@ static int field_upload(kelp_ctx* c, const char* fn, int slot, int dtype, int la
     if (!c) return fail(KELP_EINVAL, "%s: NULL context", fn);
     if (!c->has_mesh) return fail(KELP_ESTATE, "%s: upload the mesh first", fn);
     if (slot < 0 || slot >= kMaxSlots) return fail(KELP_EINVAL, "%s: slot %d not in [0, %d)", fn, slot, kMaxSlots);
-    if (dtype != KELP_DT_F64 && dtype != KELP_DT_F32 && dtype != KELP_DT_BF16)
+    if (dtype != KELP_DT_F64 && dtype != KELP_DT_F32 && dtype != KELP_DT_BF16 && dtype != KELP_DT_F16S)
         return fail(KELP_EINVAL, "%s: bad dtype %d", fn, dtype);
-    if (dtype == KELP_DT_BF16 && (layout != 0 || c->vertical_coord == 1))
-        return fail(KELP_EINVAL, "%s: bfloat16 storage is for node-layout z-level snapshots", fn);
+    if ((dtype == KELP_DT_BF16 || dtype == KELP_DT_F16S) && (layout != 0 || c->vertical_coord == 1))
+        return fail(KELP_EINVAL, "%s: bfloat16 / float16 storage is for node-layout z-level snapshots", fn);
+    if (dtype == KELP_DT_F16S && (uint64_t)n_planes * (uint64_t)c->m.n_nodes >= (uint64_t(1) << 32))
+        return fail(KELP_EINVAL, "%s: float16 storage holds at most 2^32 records per snapshot", fn);
     if (n_planes < 1 || !hu || !hv) return fail(KELP_EINVAL, "%s: u/v missing", fn);
     const bool is3d = hw != nullptr;
     if (is3d && n_w_planes != n_planes + 1) return fail(KELP_EINVAL, "%s: W needs n_planes + 1 interface planes", fn);
@@ -881,6 +1024,8 @@ static int field_upload(kelp_ctx* c, const char* fn, int slot, int dtype, int la
     void *tu = nullptr, *tv = nullptr, *tw = nullptr;
     hipError_t e = hipSuccess;
     if (is3d || layout == 0) e = hipMalloc(&s.rec, (size_t)n_planes * N * vs * (is3d ? 4 : 2));
+    if (e == hipSuccess && dtype == KELP_DT_F16S)  // row scales, then the row maxima
+        e = hipMalloc(&s.scale, (size_t)n_planes * sizeof(float4) + (size_t)(3 * n_planes + 1) * sizeof(uint32_t));
     if (e == hipSuccess && layout == 1) e = hipMalloc(&s.erec, (size_t)n_planes * E * es * 2);
     if (e == hipSuccess) e = hipMalloc(&tu, uv_bytes);
     if (e == hipSuccess) e = hipMalloc(&tv, uv_bytes);
@@ -946,6 +1091,33 @@ int kelp_field_upload_eta(kelp_ctx* c, int slot, const double* host_eta)
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     (void)hipFree(tmp);
     if (e != hipSuccess) return fail(KELP_EHIP, "kelp_field_upload_eta: %s", hipGetErrorString(e));
+    return KELP_OK;
+}
+
+int kelp_sigma_surface_upload(kelp_ctx* c, int which, const double* host_eta)
+{
+    if (!c) return fail(KELP_EINVAL, "kelp_sigma_surface_upload: NULL context");
+    if (which != 0 && which != 1) return fail(KELP_EINVAL, "kelp_sigma_surface_upload: which must be 0 or 1");
+    if (!c->has_mesh || c->vertical_coord != 1 || !c->sig_eta)
+        return fail(KELP_ESTATE, "kelp_sigma_surface_upload: no sigma mesh uploaded");
+    if (!host_eta) {
+        c->sig_given[which] = false;
+        return KELP_OK;
+    }
+    HIP_TRY(hipSetDevice(c->device));
+    const int64_t N = c->m.n_nodes;
+    double* tmp = nullptr;
+    HIP_TRY(hipMalloc(&tmp, N * sizeof(double)));
+    hipError_t e = hipMemcpyAsync(tmp, host_eta, N * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        permute_nodes_kernel<<<grid_for(N, 256), 256, 0, c->stream>>>(N, c->node_old_of_new, tmp,
+                                                                     c->sig_eta + (5 + which) * N);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return fail(KELP_EHIP, "kelp_sigma_surface_upload: %s", hipGetErrorString(e));
+    c->sig_given[which] = true;
     return KELP_OK;
 }
 
@@ -1126,9 +1298,19 @@ static const double* slot_eta(kelp_ctx* c, int k)
 }
 
 // A nested blend of slot surfaces; every source the blend reads must exist.
+// sa == KELP_SURFACE_GIVEN: the surface of kelp_sigma_surface_upload
+// (`which` 0 at the start, 1 at the end of the step).
 static int eta_source(kelp_ctx* c, const char* what, int sa, int sb, int st, double alpha0, double alpha,
-                      EtaSrc& out)
+                      EtaSrc& out, int which = 0)
 {
+    if (sa == KELP_SURFACE_GIVEN) {
+        if (!c->sig_given[which])
+            return fail(KELP_ESTATE, "kelp_step: the sigma %s surface was not uploaded (kelp_sigma_surface_upload)",
+                        what);
+        const double* g = c->sig_eta + (int64_t)(5 + which) * c->m.n_nodes;
+        out = EtaSrc{g, g, g, 0.0, 0.0};
+        return KELP_OK;
+    }
     out = EtaSrc{slot_eta(c, sa), slot_eta(c, sb), slot_eta(c, st), alpha0, alpha};
     const bool need_t = alpha >= 1e-12;
     const bool need_a = alpha <= 1.0 - 1e-12;
@@ -1154,10 +1336,10 @@ static int sigma_prepare(kelp_ctx* c, const kelp_step_args* a, bool is3d, const 
     int rc = eta_source(c, "start", a->sig_start_a, a->sig_start_b, a->sig_start_t, a->sig_start_alpha0,
                         a->sig_start_alpha, start);
     if (rc) return rc;
-    const bool end_is_start = a->sig_end_a < 0;
+    const bool end_is_start = a->sig_end_a < 0 && a->sig_end_a != KELP_SURFACE_GIVEN;
     if (!end_is_start) {
         rc = eta_source(c, "end", a->sig_end_a, a->sig_end_b, a->sig_end_t, a->sig_end_alpha0, a->sig_end_alpha,
-                        end);
+                        end, 1);
         if (rc) return rc;
     } else {
         end = start;
@@ -1181,7 +1363,9 @@ static int sigma_prepare(kelp_ctx* c, const kelp_step_args* a, bool is3d, const 
     return KELP_OK;
 }
 
-int kelp_step(kelp_ctx* c, const kelp_particles* p, const kelp_step_args* a)
+// Validation and the kernel configuration of one step (everything kelp_step
+// does before its launch); also the base of a batched step.
+int kelp_step_prepare(kelp_ctx* c, const kelp_particles* p, const kelp_step_args* a, StepCfg& cfg, PState& ps)
 {
     if (!c || !p || !a) return fail(KELP_EINVAL, "kelp_step: NULL argument");
     if (!c->has_mesh) return fail(KELP_ESTATE, "kelp_step: no mesh uploaded");
@@ -1262,7 +1446,7 @@ int kelp_step(kelp_ctx* c, const kelp_particles* p, const kelp_step_args* a)
             int rc = kelp_wait_slot(c, c->slots[k]);
             if (rc) return rc;
         }
-    StepCfg cfg{};
+    cfg = StepCfg{};
     cfg.alpha0 = a->alpha0;
     for (int k = 0; k < 4; ++k) cfg.alphas[k] = a->alphas[k];
     cfg.dt = a->dt;
@@ -1323,8 +1507,23 @@ int kelp_step(kelp_ctx* c, const kelp_particles* p, const kelp_step_args* a)
         int rc = sigma_prepare(c, a, is3d, sb, st, cfg);
         if (rc) return rc;
     }
-    PState ps{p->lon, p->lat, p->z, p->status, p->nn, p->fr, p->u, p->v, p->w, p->pid, p->wet_streak,
-              reinterpret_cast<unsigned long long*>(p->alive_count), p->n, p->elem};
+    ps = PState{p->lon, p->lat, p->z, p->status, p->nn, p->fr, p->u, p->v, p->w, p->pid, p->wet_streak,
+                reinterpret_cast<unsigned long long*>(p->alive_count), p->n, p->elem};
+    return KELP_OK;
+}
+
+int kelp_step(kelp_ctx* c, const kelp_particles* p, const kelp_step_args* a)
+{
+    StepCfg cfg{};
+    PState ps{};
+    int rc = kelp_step_prepare(c, p, a, cfg, ps);
+    if (rc) return rc;
+    if (p->n == 0) return KELP_OK;
+    const bool sigma = c->vertical_coord == 1;
+    const FieldSlot& sa = c->slots[a->slot_a];
+    const FieldSlot* sb = opt_slot(c, a->slot_b);
+    const FieldSlot* st = opt_slot(c, a->slot_t);
+    const bool is3d = sa.is3d;
     if (a->order == 5) {
         // integrators.py:6485-6513: rk45 is 2-D, element-layout least squares
         if (is3d || sa.layout != 1 || sigma || cfg.walk)
@@ -1350,6 +1549,16 @@ int kelp_step(kelp_ctx* c, const kelp_particles* p, const kelp_step_args* a)
             if (c->m.max_v == 3) launch_rk45_v<3, double>(c->stream, c->m, fd, cfg, ps);
             else launch_rk45_v<4, double>(c->stream, c->m, fd, cfg, ps);
         }
+        HIP_TRY(hipGetLastError());
+        return KELP_OK;
+    }
+    if (sa.dtype == KELP_DT_F16S) {
+        // node-layout z-level only (refused at upload otherwise)
+        if (sigma) return fail(KELP_EINVAL, "kelp_step: float16 snapshots on a sigma mesh");
+        if (is3d)
+            launch_step_f16s<true>(c, a->order, sa, sb, st, cfg, ps);
+        else
+            launch_step_f16s<false>(c, a->order, sa, sb, st, cfg, ps);
         HIP_TRY(hipGetLastError());
         return KELP_OK;
     }

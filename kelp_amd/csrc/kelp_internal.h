@@ -10,6 +10,7 @@
 
 #include "../../include/kelp_hip.h"
 #include "kelp_device.cuh"
+#include "kelp_step.cuh"
 
 int kelp_fail(int code, const char* fmt, ...);
 
@@ -52,6 +53,8 @@ struct FieldSlot {
     // sampling, built on first use; pair_vars = {-1, -1} when stale
     double* pair = nullptr;
     int pair_vars[2] = {-1, -1};
+    // KELP_DT_F16S: per-row scales (float4 per layer), then the row maxima
+    float4* scale = nullptr;
     int64_t az_k = 0;       // 0: no Az for the slot's current snapshot
     ScalarField scalars[kMaxScalars];
     // asynchronous refill (kelp_field_upload_async): device staging of the
@@ -84,12 +87,27 @@ struct kelp_ctx {
     int32_t* elem_old_of_new = nullptr;
     int32_t* elem_new_of_old = nullptr;
     int vertical_coord = 0;        // 0 z-level, 1 sigma
-    double* sig_eta = nullptr;     // sigma meshes: 5 x N surfaces of one step (mesh-owned)
+    double* sig_eta = nullptr;     // sigma meshes: 7 x N (mesh-owned): the stage surfaces 0..3 and the
+                                   // end surface of one step, then the given start and end surfaces
+    bool sig_given[2] = {false, false};
     FieldSlot slots[kMaxSlots];
     DevBuf sort_tmp, sort_keys, sort_keys2, sort_idx, sort_perm, sort_stage;
+    // batched clouds (kelp_batch_*): the cloud table, the per-cloud step
+    // records the bracket kernel writes, the device step counter, and the
+    // captured graph of a run of batched steps with what it was built for
+    DevBuf batch_clouds, batch_steps, batch_counter;
+    int64_t batch_n_clouds = 0;
+    kelp_batch_axis batch_axis{};
+    hipGraphExec_t batch_graph = nullptr;
+    std::vector<unsigned char> batch_graph_key;
 };
 
 void kelp_free_scalar(ScalarField& f);
+
+// Validation and kernel configuration of one step (kelp_step without its
+// launch); the sigma stage surfaces are built on the context stream.
+extern "C" int kelp_step_prepare(kelp_ctx* c, const kelp_particles* p, const kelp_step_args* a,
+                                 kelp::StepCfg& cfg, kelp::PState& ps);
 
 // Order `c->stream` after a pending refill of slot s (no-op otherwise).
 int kelp_wait_slot(kelp_ctx* c, FieldSlot& s);

@@ -7,8 +7,9 @@
 // cache lines from HBM.  Sorting the state by the hash cell of each particle
 // (row-major cell order, so a wavefront covers a few adjacent cells) makes
 // neighbouring lanes share windows, elements and field lines in L2/MALL.
-// Alive particles sort first, then beached ones, then the rest, so the
-// tail of the grid holds the cheap non-alive lanes.
+// Alive particles sort first, then beached ones, then the rest, then the
+// free entries of a particle pool, so the tail of the grid holds the cheap
+// non-alive lanes.
 //
 // Results do not depend on the order: every particle is computed
 // independently and its random draws are keyed by its global id (pid), not
@@ -34,7 +35,8 @@ __global__ void sort_key_kernel(DMesh m, int64_t n, const double* __restrict__ l
     uint32_t n_cells = (uint32_t)m.nx * (uint32_t)m.ny;
     uint32_t cell = (uint32_t)cj * (uint32_t)m.nx + (uint32_t)ci;
     int32_t st = status[i];
-    uint32_t band = st == kAlive ? 0u : (st == kBeached ? 1u : 2u);
+    // free entries of a particle pool (status 0, see kelp_batch_collect) last
+    uint32_t band = st == kAlive ? 0u : (st == kBeached ? 1u : (st == 0 ? 3u : 2u));
     keys[i] = band * n_cells + cell;
     idx[i] = (int32_t)i;
 }

@@ -458,20 +458,15 @@ __device__ inline void beach_one(const DMesh& m, const StepCfg& c, uint64_t pid,
 #define KELP_STEP_ATTR
 #endif
 
+// One particle's full step (deterministic step, kicks, periodic fold,
+// beaching), the state read and written at slot i.  `prev` is its status at
+// the step start; its random draws are keyed by pid[i] & key_mask (the
+// whole pid in a plain step; the cloud-local part in a batched one).
 template <int ORDER, int MAXV, bool IS3D, typename T, bool ELEM>
-__global__ __launch_bounds__(256) KELP_STEP_ATTR void step_kernel(DMesh m, Snap<T, IS3D> f, Snap<T, false> fe,
-                                                                  StepCfg c, PState s)
+__device__ __forceinline__ void step_particle(const DMesh& m, const Snap<T, IS3D>& f, const Snap<T, false>& fe,
+                                              const StepCfg& c, const PState& s, int64_t i, int32_t prev,
+                                              uint64_t key_mask)
 {
-    int64_t i = xcd_thread_index();
-    if (i >= s.n) return;
-    const int32_t prev = s.status[i];
-    if (s.alive_count) {
-        // particles alive at the step start: one atomic per wavefront
-        unsigned long long alive = __ballot(prev == kAlive);
-        unsigned long long active = __ballot(1);
-        if ((int)(threadIdx.x & 63) == __ffsll((long long)active) - 1 && alive)
-            atomicAdd(s.alive_count, (unsigned long long)__popcll(alive));
-    }
     int32_t st = prev;
     double lon = s.lon[i], lat = s.lat[i];
     double z = IS3D ? s.z[i] : 0.0;
@@ -487,12 +482,13 @@ __global__ __launch_bounds__(256) KELP_STEP_ATTR void step_kernel(DMesh m, Snap<
             st = kOutside;
             fr = (int8_t)code;
         } else if (c.kick && !(c.kh_x == 0.0 && c.kh_y == 0.0)) {
-            kick_one<MAXV, IS3D>(m, c, (uint64_t)s.pid[i], pre_lon, pre_lat, pre_z, pre_nn, lon, lat, z, st, nn, fr);
+            kick_one<MAXV, IS3D>(m, c, (uint64_t)s.pid[i] & key_mask, pre_lon, pre_lat, pre_z, pre_nn, lon, lat, z,
+                                 st, nn, fr);
         }
-        if (IS3D && c.az && st == kAlive) az_one(m, c, (uint64_t)s.pid[i], nn, z);
+        if (IS3D && c.az && st == kAlive) az_one(m, c, (uint64_t)s.pid[i] & key_mask, nn, z);
         if ((c.periodic_lon || c.periodic_lat) && st == kAlive) periodic_one(m, c, lon, lat, nn, elem);
     }
-    if (c.beach) beach_one<MAXV>(m, c, (uint64_t)s.pid[i], prev, lon, lat, st, nn, fr, s.ws[i]);
+    if (c.beach) beach_one<MAXV>(m, c, (uint64_t)s.pid[i] & key_mask, prev, lon, lat, st, nn, fr, s.ws[i]);
     if (s.elem) s.elem[i] = elem;
     s.lon[i] = lon;
     s.lat[i] = lat;
@@ -503,6 +499,28 @@ __global__ __launch_bounds__(256) KELP_STEP_ATTR void step_kernel(DMesh m, Snap<
     s.u[i] = U;
     s.v[i] = V;
     if (IS3D && s.w) s.w[i] = W;
+}
+
+// particles alive at the step start: one atomic per wavefront (the lanes
+// that reach the call)
+__device__ __forceinline__ void count_alive(unsigned long long* count, bool alive_lane)
+{
+    if (!count) return;
+    unsigned long long alive = __ballot(alive_lane);
+    unsigned long long active = __ballot(1);
+    if ((int)(threadIdx.x & 63) == __ffsll((long long)active) - 1 && alive)
+        atomicAdd(count, (unsigned long long)__popcll(alive));
+}
+
+template <int ORDER, int MAXV, bool IS3D, typename T, bool ELEM>
+__global__ __launch_bounds__(256) KELP_STEP_ATTR void step_kernel(DMesh m, Snap<T, IS3D> f, Snap<T, false> fe,
+                                                                  StepCfg c, PState s)
+{
+    int64_t i = xcd_thread_index();
+    if (i >= s.n) return;
+    const int32_t prev = s.status[i];
+    count_alive(s.alive_count, prev == kAlive);
+    step_particle<ORDER, MAXV, IS3D, T, ELEM>(m, f, fe, c, s, i, prev, ~0ull);
 }
 
 // ---------------------------------------------------------------------------

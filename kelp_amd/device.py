@@ -14,7 +14,7 @@ from typing import Dict, Optional
 import numpy as np
 
 from . import _native
-from ._native import KELP_DT_BF16, KELP_DT_F32, KELP_DT_F64, MeshDesc, Particles, StepArgs, check
+from ._native import KELP_DT_BF16, KELP_DT_F16S, KELP_DT_F32, KELP_DT_F64, MeshDesc, Particles, StepArgs, check
 from .mesh import Mesh
 
 
@@ -89,6 +89,8 @@ class DeviceParticles:
 
 
 RESUSPENSION_CODES = {"never": 0, "auto": 1, "stochastic": 2, "auto_or_stochastic": 3}
+#: step argument slot of a sigma surface given by DeviceContext.sigma_surface
+SURFACE_GIVEN = -2
 
 
 def make_step_args(order, slot_a, dt, time_dir, slot_b=-1, slot_t=-1, alpha0=0.0,
@@ -238,6 +240,20 @@ class DeviceContext:
             raise ValueError(f"eta of shape {eta.shape} does not match the mesh's {self.mesh.n_nodes} nodes")
         check(self.lib.kelp_field_upload_eta(self.h, int(slot), eta.ctypes.data), "kelp_field_upload_eta")
 
+    def sigma_surface(self, which: int, eta) -> None:
+        """The surface a sigma step starts (``which`` 0) or ends (1) on,
+        given as an (N,) array (kelp_sigma_surface_upload); ``None``
+        forgets it.  A step selects it with the slot ``SURFACE_GIVEN``."""
+        if eta is None:
+            check(self.lib.kelp_sigma_surface_upload(self.h, int(which), None), "kelp_sigma_surface_upload")
+            return
+        eta = np.ascontiguousarray(eta, np.float64)
+        if eta.shape != (self.mesh.n_nodes,):
+            raise ValueError(f"sigma surface of shape {eta.shape} does not match the mesh's "
+                             f"{self.mesh.n_nodes} nodes")
+        check(self.lib.kelp_sigma_surface_upload(self.h, int(which), eta.ctypes.data),
+              "kelp_sigma_surface_upload")
+
     def upload_az(self, slot: int, az) -> None:
         """Attach the snapshot's vertical diffusivity (K, N) at the
         interfaces (mesh node order) to ``slot`` (kelp_field_upload_az)."""
@@ -291,14 +307,17 @@ class DeviceContext:
         """Upload one snapshot into ``slot``; skipped when ``key`` matches the
         slot's current content.  ``uv_layout``: "node" (u, v per node) or
         "element" (u, v per element centre, interpolated by least squares).
-        ``storage="bf16"``: the values are rounded to bfloat16 on the device
-        (node layout, z-level meshes); else stored in their own dtype."""
+        ``storage="bf16"``: the values are rounded to bfloat16 on the device;
+        ``storage="f16s"``: to float16 with a power-of-two scale per level
+        (both node layout, z-level meshes); else stored in their own dtype."""
         if key is not None and self._slot_keys.get(slot) is key:
             return
         u = np.asarray(u)
         dt = KELP_DT_F32 if u.dtype == np.float32 else KELP_DT_F64
         if storage == "bf16":
             dt = KELP_DT_BF16
+        elif storage == "f16s":
+            dt = KELP_DT_F16S
         elif storage not in (None, "native"):
             raise ValueError(f"unknown storage {storage!r}")
         npd = np.float64 if dt == KELP_DT_F64 else np.float32
@@ -345,7 +364,7 @@ class DeviceContext:
 
         u = host(u)
         f32 = (u.dtype == t.float32) if isinstance(u, t.Tensor) else (u.dtype == np.float32)
-        f32 = f32 or storage == "bf16"
+        f32 = f32 or storage in ("bf16", "f16s")
         npd = np.float32 if f32 else np.float64
         u, v, w = host(u, npd), host(v, npd), host(w, npd)
         e = host(eta, np.float64)
@@ -358,7 +377,8 @@ class DeviceContext:
         if slot in keep:
             check(self.lib.kelp_field_upload_wait(self.h, int(slot)), "kelp_field_upload_wait")
         layout = 0 if uv_layout == "node" else 1
-        code = KELP_DT_BF16 if storage == "bf16" else KELP_DT_F32 if f32 else KELP_DT_F64
+        code = (KELP_DT_BF16 if storage == "bf16" else KELP_DT_F16S if storage == "f16s"
+                else KELP_DT_F32 if f32 else KELP_DT_F64)
         check(self.lib.kelp_field_upload_async(self.h, int(slot), code,
                                                layout, n_planes, ptr(u), ptr(v), ptr(w), nw, ptr(e)),
               "kelp_field_upload_async")
@@ -400,6 +420,7 @@ class DeviceContext:
         pid_t = (t.arange(P, dtype=t.int64, device=dev) if pid is None
                  else t.as_tensor(np.ascontiguousarray(pid, np.int64), device=dev))
         nn_t = i32(nn, -1)
+        el_t = None if elem is None else i32(elem, -1)
         # the tensors above were filled on torch's stream; the id maps below
         # and every later call run on the context's stream
         t.cuda.current_stream().synchronize()
@@ -407,9 +428,7 @@ class DeviceContext:
             check(self.lib.kelp_nodes_from_mesh_ids(self.h, nn_t.numel(), nn_t.data_ptr()),
                   "kelp_nodes_from_mesh_ids")
             self.synchronize()
-        el_t = None
         if elem is not None:
-            el_t = i32(elem, -1)
             check(self.lib.kelp_elems_from_mesh_ids(self.h, el_t.numel(), el_t.data_ptr()),
                   "kelp_elems_from_mesh_ids")
             self.synchronize()

@@ -22,10 +22,17 @@ anchor mode 2 of the kernel.
 u/v at the nodes are interpolated barycentrically in the containing element
 (``uv_interp="barycentric"``), u/v at element centres by least squares over
 the anchor's element ring (``uv_interp="leastsq"``, the reference default);
-W by ``nearest``, ``leastsq`` or ``barycentric``.  Combinations the device
-path does not implement raise ``NotImplementedError`` (edge_leastsq, W idw,
-sigma meshes, EM/M1 diffusion, an Az kick from an Az field); there is no CPU
-fallback.
+W by ``nearest``, ``leastsq`` or ``barycentric`` (sigma meshes also ``idw``).
+Sigma meshes take the reference's ``sigma_eta_start`` / ``sigma_eta_end``
+/ ``Dmin`` / ``z0b_min`` keywords; a PCHIP ``recon`` bundle (Euler, RK2) is
+reconstructed on the device; a state that carries ``elem`` runs the edge
+walk (node-layout u/v, z-level).  The mesh may be this package's
+:class:`~kelp_amd.mesh.Mesh` or the reference's own ``Mesh`` (adapted
+without copies by :func:`~kelp_amd.mesh.from_reference_mesh`), and the
+state, fields and diffusion arguments may be the reference's objects.
+Combinations the device path does not implement raise
+``NotImplementedError`` (edge_leastsq, W idw on z-level meshes, EM/M1
+diffusion, PCHIP with RK4); there is no CPU fallback.
 """
 from __future__ import annotations
 
@@ -36,13 +43,23 @@ from typing import Optional
 import numpy as np
 
 from .diffusion import DiffusionStepArgs
-from .fields import LazyLerpField, VelocityField
-from .mesh import Mesh, exact_nearest_on
+from .fields import VelocityField
+from .mesh import Mesh, build_elem_neighbors, exact_nearest_on, from_reference_mesh
 from .particles import ParticleState
 
 _C_COEFFS = {1: (0.0,), 2: (0.0, 0.5), 4: (0.0, 0.5, 0.5, 1.0)}
 W_METHODS = {"nearest": 0, "leastsq": 1, "barycentric": 2}
-N_SLOTS = 16
+SIGMA_W_METHODS = {"nearest": 0, "leastsq": 1, "barycentric": 2, "idw": 3}
+#: library slots (kMaxSlots); the last two hold PCHIP stage fields
+MAX_SLOTS = 16
+RECON_SLOTS = (MAX_SLOTS - 2, MAX_SLOTS - 1)
+N_SLOTS = RECON_SLOTS[0]
+
+
+def _is_lazy(vel) -> bool:
+    """A two-snapshot bracket (core/io.py:386 LazyLerpField, this package's
+    or the reference's)."""
+    return hasattr(vel, "vel_a") and hasattr(vel, "vel_b")
 
 
 def _validate_time_aware_kwargs(vel_next, dt_to_next):
@@ -54,7 +71,7 @@ def _validate_time_aware_kwargs(vel_next, dt_to_next):
 
 def _is_3d(vel) -> bool:
     """integrators.py:949: W present and u per layer."""
-    base = vel.vel_a if isinstance(vel, LazyLerpField) else vel
+    base = vel.vel_a if _is_lazy(vel) else vel
     return base.w is not None and np.ndim(base.u) == 2
 
 
@@ -92,6 +109,11 @@ class Engine:
             self._mesh = mesh
             self._slots.clear()
 
+    def rebind_mesh(self, mesh: Mesh) -> None:
+        """Upload ``mesh`` again (its search structures grew)."""
+        self._mesh = None
+        self.bind_mesh(mesh)
+
     def slot_for(self, vel: VelocityField, dtype) -> int:
         key = id(vel)
         hit = self._slots.get(key)
@@ -111,6 +133,8 @@ class Engine:
                               uv_layout=vel.uv_layout)
         if getattr(vel, "az", None) is not None and w is not None:
             self.ctx.upload_az(slot, vel.az)
+        if self._mesh is not None and self._mesh.vertical_coord == "sigma" and getattr(vel, "eta", None) is not None:
+            self.ctx.upload_eta(slot, vel.eta)
         self._slots[key] = (slot, vel, dtype)
         return slot
 
@@ -137,28 +161,33 @@ def _field_dtype(*fields):
 def _payload_az(vel):
     """The step payload's Az: a bracket's only when both snapshots carry
     one (core/io.py:475-492), else the snapshot's own."""
-    if isinstance(vel, LazyLerpField):
-        if vel.vel_a.az is None or vel.vel_b.az is None:
+    if _is_lazy(vel):
+        if getattr(vel.vel_a, "az", None) is None or getattr(vel.vel_b, "az", None) is None:
             return None
         return vel.vel_a.az
     return getattr(vel, "az", None)
 
 
-def _check_supported(vel, mesh, uv_interp, w_interp, diffusion, is3d):
-    base = vel.vel_a if isinstance(vel, LazyLerpField) else vel
-    ok = ((uv_interp == "barycentric" and base.uv_layout == "node")
-          or (uv_interp == "leastsq" and base.uv_layout == "element"))
+def _check_supported(vel, mesh, uv_interp, w_interp, diffusion, is3d, walk):
+    base = vel.vel_a if _is_lazy(vel) else vel
+    layout = getattr(base, "uv_layout", "element")
+    ok = ((uv_interp == "barycentric" and layout == "node")
+          or (uv_interp == "leastsq" and layout == "element"))
     if not ok:
         raise NotImplementedError(
-            f"device path: uv_interp={uv_interp!r} with uv_layout={base.uv_layout!r} is not "
+            f"device path: uv_interp={uv_interp!r} with uv_layout={layout!r} is not "
             "implemented (node-layout barycentric, element-layout leastsq)")
+    if walk and (layout != "node" or (is3d and mesh.vertical_coord == "sigma")):
+        raise NotImplementedError("device path: the edge walk (a state carrying elem) with "
+                                  "element-layout u/v or on a sigma mesh")
     if is3d:
-        if mesh.vertical_coord != "zlevel":
-            raise NotImplementedError("device path: sigma meshes are not implemented")
-        if w_interp not in W_METHODS:
+        if mesh.vertical_coord not in ("zlevel", "sigma"):
+            raise NotImplementedError(f"device path: vertical_coord={mesh.vertical_coord!r}")
+        methods = SIGMA_W_METHODS if mesh.vertical_coord == "sigma" else W_METHODS
+        if w_interp not in methods:
             raise NotImplementedError(f"device path: w_interp={w_interp!r} is not implemented "
-                                      "(nearest, leastsq, barycentric)")
-        if base.uv_vertical != "layer":
+                                      f"({', '.join(methods)})")
+        if getattr(base, "uv_vertical", "layer") != "layer":
             raise NotImplementedError(f"device path: uv_vertical={base.uv_vertical!r}")
     if diffusion is not None:
         if diffusion.scheme != "uniform":
@@ -174,14 +203,77 @@ def _check_supported(vel, mesh, uv_interp, w_interp, diffusion, is3d):
             if not is3d and (has_field or diffusion.az_const > 0.0):
                 # integrators.py:893
                 raise ValueError("diffusion.az_enabled requires a 3-D run (Fdrift3D)")
+            if has_field and mesh.vertical_coord == "sigma":
+                # integrators.py:902-906
+                raise ValueError("diffusion.var_az is not supported on sigma meshes (v1)")
 
 
-def _step(order: int, state: ParticleState, vel, mesh: Mesh, dt: float, time_dir: float,
+def _recon_stage_slots(eng, recon, order, dt, dtype):
+    """temporal_reconstruct.py:298 make_stage_vels_recon on the device: the
+    four snapshots of ``recon = (fields, times, q, direction)`` resident,
+    stage c's field (at q + direction c dt) reconstructed into the reserved
+    slots; returns (slot_a, slot_t, alphas) for the step."""
+    if order not in (1, 2):
+        raise NotImplementedError(f"device path: pchip reconstruction with order {order} "
+                                  "(euler and rk2)")
+    rs, rt, rq, rd = recon
+    if len(rs) != 4 or len(rt) != 4:
+        raise ValueError("recon needs four snapshots and four times")
+    src = [eng.slot_for(f, dtype) for f in rs]
+    t4 = [float(t) for t in rt]
+    cs = _C_COEFFS[order]
+    for dst, c in zip(RECON_SLOTS, cs):
+        eng.ctx.reconstruct(dst, src, t4, float(rq) + float(rd) * c * float(dt))
+    if order == 1:
+        return RECON_SLOTS[0], -1, [0.0, 0.0, 0.0, 0.0]
+    return RECON_SLOTS[0], RECON_SLOTS[1], [0.0, 1.0, 0.0, 0.0]
+
+
+def _sigma_args(eng, vel, kwargs, order, dt, dt_to_next, slot_a, slot_b, slot_t, alpha0, alphas, recon):
+    """The surfaces of a sigma step (integrators.py:1381 _sigma_end_eta,
+    :1423 _sigma_given): the given ``sigma_eta_start`` / ``sigma_eta_end``
+    arrays, else the step field's own surface at the start and its blend
+    towards ``vel_next`` (or the PCHIP stage at c = 1) at the end, else the
+    start surface."""
+    from .device import SURFACE_GIVEN
+    ctx = eng.ctx
+    n = eng._mesh.n_nodes
+    given_s = kwargs.get("sigma_eta_start")
+    given_e = kwargs.get("sigma_eta_end")
+    for g in (given_s, given_e):
+        if g is not None and np.shape(g) != (n,):
+            raise ValueError(f"sigma step: supplied surface has shape {np.shape(g)}, the velocity "
+                             f"field's eta ({n},)")
+    if given_s is None or given_e is None:
+        bases = [vel.vel_a, vel.vel_b] if _is_lazy(vel) else [vel]
+        if any(getattr(b, "eta", None) is None for b in bases):
+            raise ValueError("sigma step: the velocity field carries no eta and no surface was given")
+    if given_s is not None:
+        ctx.sigma_surface(0, given_s)
+        start = (SURFACE_GIVEN, -1, -1, 0.0, 0.0)
+    else:
+        start = (slot_a, slot_b, slot_t, alpha0, 0.0)
+    end = None
+    if given_e is not None:
+        ctx.sigma_surface(1, given_e)
+        end = (SURFACE_GIVEN, -1, -1, 0.0, 0.0)
+    elif recon is not None:
+        end = (slot_a, slot_b, slot_t, 0.0, 1.0)
+    elif slot_t >= 0 and dt_to_next is not None and dt_to_next > 0:
+        end = (slot_a, slot_b, slot_t, alpha0, float(dt) / float(dt_to_next))
+    elif given_s is not None:
+        end = start
+    return dict(start=start, end=end, Dmin=float(kwargs.get("Dmin", 0.5)),
+                z0b_min=float(kwargs.get("z0b_min", 0.1)))
+
+
+def _step(order: int, state: ParticleState, vel, mesh, dt: float, time_dir: float,
           kwargs) -> ParticleState:
-    if kwargs.get("recon") is not None:
-        raise NotImplementedError("device path: pchip reconstruction is not implemented")
-    if kwargs.get("elem_search", "local") != "local" or state.elem is not None:
-        raise NotImplementedError("device path: elem_search='walk' is not implemented")
+    mesh = from_reference_mesh(mesh)
+    recon = kwargs.get("recon")
+    walk = getattr(state, "elem", None) is not None
+    if kwargs.get("elem_search", "walk" if walk else "local") not in ("local", "walk"):
+        raise ValueError(f"elem_search must be 'local' or 'walk', got {kwargs.get('elem_search')!r}")
     uv_interp = kwargs.get("uv_interp", "leastsq")
     w_interp = kwargs.get("w_interp", "leastsq")
     vel_next = kwargs.get("vel_next")
@@ -189,21 +281,28 @@ def _step(order: int, state: ParticleState, vel, mesh: Mesh, dt: float, time_dir
     diffusion: Optional[DiffusionStepArgs] = kwargs.get("diffusion")
     _validate_time_aware_kwargs(vel_next, dt_to_next)
     is3d = _is_3d(vel)
+    sig3d = is3d and mesh.vertical_coord == "sigma"
     rk45 = None
     if order == 5:
         # integrators.py:6485-6513: 2-D element-layout least squares only
-        base = vel.vel_a if isinstance(vel, LazyLerpField) else vel
+        base = vel.vel_a if _is_lazy(vel) else vel
         if uv_interp != "leastsq":
             raise NotImplementedError(f"rk45 supports uv_interp='leastsq' only (v1), got {uv_interp!r}")
         if is3d:
             raise NotImplementedError("rk45 is 2-D only (v1)")
         if base.uv_layout != "element":
             raise ValueError("rk45 requires element-layout velocities (v1)")
+        if walk:
+            raise ValueError("rk45 does not support elem_search='walk' (v1)")
+        if recon is not None:
+            raise NotImplementedError("device path: rk45 with pchip reconstruction")
         tol = float(kwargs.get("rk45_tol_m", 1.0))
         h_min = kwargs.get("rk45_h_min")
         rk45 = dict(tol_m=tol, h_min=None if h_min is None else float(h_min),
                     inv_dtn=0.0 if dt_to_next is None else 1.0 / float(dt_to_next))
-    _check_supported(vel, mesh, uv_interp, w_interp, diffusion, is3d)
+    _check_supported(vel, mesh, uv_interp, w_interp, diffusion, is3d, walk)
+    if recon is not None and sig3d:
+        raise NotImplementedError("device path: pchip reconstruction on a sigma mesh")
     if dt <= 0:
         raise ValueError(f"dt must be positive (direction goes in time_dir), got {dt}")
 
@@ -215,23 +314,35 @@ def _step(order: int, state: ParticleState, vel, mesh: Mesh, dt: float, time_dir
                              w=None if state.w is None else np.zeros_like(state.w))
 
     eng = get_engine()
+    if walk and mesh.elem_neighbors is None:
+        build_elem_neighbors(mesh)          # core/mesh.py:2675: built lazily on first use
+        eng.rebind_mesh(mesh)
     eng.bind_mesh(mesh)
-    lazy = isinstance(vel, LazyLerpField)
+    lazy = _is_lazy(vel)
     va = vel.vel_a if lazy else vel
     vb = vel.vel_b if lazy else None
-    dtype = _field_dtype(va, vb, vel_next)
-    slot_a = eng.slot_for(va, dtype)
-    slot_b = eng.slot_for(vb, dtype) if vb is not None else -1
-    slot_t = eng.slot_for(vel_next, dtype) if vel_next is not None else -1
+    dtype = _field_dtype(va, vb, vel_next, *(recon[0] if recon is not None else ()))
     alpha0 = vel.alpha if lazy else 0.0
-    alphas = stage_alphas(order, dt, dt_to_next)
+    if recon is not None:
+        slot_a, slot_t, alphas = _recon_stage_slots(eng, recon, order, dt, dtype)
+        slot_b, alpha0 = -1, 0.0
+    else:
+        slot_a = eng.slot_for(va, dtype)
+        slot_b = eng.slot_for(vb, dtype) if vb is not None else -1
+        slot_t = eng.slot_for(vel_next, dtype) if vel_next is not None else -1
+        alphas = stage_alphas(order, dt, dt_to_next)
+    sigma = None
+    if sig3d:
+        sigma = _sigma_args(eng, vel, kwargs, order, dt, dt_to_next, slot_a, slot_b, slot_t, alpha0,
+                            alphas, recon)
 
     P = state.n_particles
-    pid_off = int(diffusion.pid_offset) if diffusion is not None else 0
+    pid_off = int(diffusion.pid_offset) if diffusion is not None and hasattr(diffusion, "pid_offset") else 0
     ctx = eng.ctx
     p = ctx.particles(state.lon, state.lat, state.z, status=state.status,
                       nn=state.nearest_node, fr=state.failure_reason,
-                      pid=pid_off + np.arange(P, dtype=np.int64))
+                      pid=pid_off + np.arange(P, dtype=np.int64),
+                      elem=state.elem if walk else None)
     kick = None
     step_i, real_id = 0, 0
     if diffusion is not None:
@@ -240,16 +351,21 @@ def _step(order: int, state: ParticleState, vel, mesh: Mesh, dt: float, time_dir
             kick["az_const"] = float(diffusion.az_const)
         if diffusion.az_enabled and is3d and _payload_az(vel) is not None:
             # integrators.py:873 vel.az: the payload's bracket, blended per value
-            kick["az_field"] = (slot_a, slot_b, alpha0)
+            kick["az_field"] = (eng.slot_for(va, dtype), eng.slot_for(vb, dtype) if vb is not None else -1,
+                                vel.alpha if lazy else 0.0)
         step_i, real_id = int(diffusion.step_i), int(diffusion.real_id)
     exact = 1 if exact_nearest_on(mesh) else 2
+    wm = (SIGMA_W_METHODS if sig3d else W_METHODS).get(w_interp, 2)
     ctx.step(p, order, slot_a, dt, time_dir, slot_b=slot_b, slot_t=slot_t, alpha0=alpha0,
-             alphas=alphas, exact_nearest=exact, w_method=W_METHODS.get(w_interp, 2),
-             kick=kick, step_i=step_i, real_id=real_id, rk45=rk45)
+             alphas=alphas, exact_nearest=exact, w_method=wm,
+             kick=kick, step_i=step_i, real_id=real_id, rk45=rk45, sigma=sigma, walk=walk)
     h = p.to_host()
-    return state.replace(lon=h["lon"], lat=h["lat"], z=h["z"] if is3d else state.z,
-                         status=h["status"], nearest_node=h["nn"], failure_reason=h["fr"],
-                         u=h["u"], v=h["v"], w=h["w"] if is3d else None)
+    out = dict(lon=h["lon"], lat=h["lat"], z=h["z"] if is3d else state.z,
+               status=h["status"], nearest_node=h["nn"], failure_reason=h["fr"],
+               u=h["u"], v=h["v"], w=h["w"] if is3d else None)
+    if walk:
+        out["elem"] = h["elem"]
+    return state.replace(**out)
 
 
 def euler_step(state, vel, mesh, dt, time_dir, layer=0, **kwargs):

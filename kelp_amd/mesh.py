@@ -371,3 +371,94 @@ def build_elem_neighbors(mesh: Mesh) -> np.ndarray:
         out[e2, k2] = e1
     mesh.elem_neighbors = out
     return out
+
+
+def _normalized_connectivity(elem_nodes) -> np.ndarray:
+    """``elem_nodes`` as int32 with no repeated vertex in a row (a FESOM-C
+    triangle stored as a quad repeats its first vertex, core/mesh.py:52-56):
+    the array itself when it already qualifies, else a compacted copy."""
+    en = np.asarray(elem_nodes)
+    if en.dtype != np.int32:
+        en = en.astype(np.int32)
+    if en.ndim != 2 or en.shape[1] < 2:
+        return en
+    rep = np.zeros(en.shape[0], dtype=bool)
+    for k in range(1, en.shape[1]):
+        for j in range(k):
+            rep |= (en[:, k] == en[:, j]) & (en[:, k] >= 0)
+    if not rep.any():
+        return en
+    out = en.copy()
+    for e in np.flatnonzero(rep):
+        seen = []
+        for nd in en[e]:
+            nd = int(nd)
+            if nd >= 0 and nd not in seen:
+                seen.append(nd)
+        out[e] = -1
+        out[e, :len(seen)] = seen
+    return out
+
+
+def from_reference_mesh(ref) -> Mesh:
+    """This package's :class:`Mesh` over a reference ``kelp.core.mesh.Mesh``
+    (src/kelp/core/mesh.py:41-210), sharing its arrays (no copies) and its
+    ``metadata`` dict (so ``set_exact_nearest`` on either is seen by both).
+
+    The grid hash is the reference's own (``_gh_offsets``, ``_gh_node_ids``,
+    ``_gh_cell_size`` ... ``_gh_far_rows``, built by its ``build_grid_hash``,
+    core/mesh.py:844) when present, else built here the same way; the
+    coastal flags (``_elem_is_coastal``) and element neighbours
+    (``_elem_neighbors``) are taken when the reference has built them.  The
+    adapter is memoised on the reference object and rebuilt only when one of
+    those arrays or the cell changes, so a device context bound to it keeps
+    its uploaded mesh and snapshots across calls."""
+    if isinstance(ref, Mesh):
+        return ref
+    gh_off = getattr(ref, "_gh_offsets", None)
+    key = (id(ref.nodes_lon), id(ref.elem_nodes), id(gh_off), float(getattr(ref, "_gh_cell_size", 0.0)),
+           id(getattr(ref, "_elem_neighbors", None)), id(getattr(ref, "_elem_is_coastal", None)),
+           id(ref.sigma))
+    memo = getattr(ref, "__dict__", {}).get("_kelp_amd_mesh")
+    if memo is not None and memo[0] == key:
+        return memo[1]
+    md = getattr(ref, "metadata", None)
+    if md is None:
+        md = {}
+    m = Mesh(name=str(getattr(ref, "name", "mesh")),
+             nodes_lon=np.asarray(ref.nodes_lon, dtype=np.float64),
+             nodes_lat=np.asarray(ref.nodes_lat, dtype=np.float64),
+             depth=np.asarray(ref.depth, dtype=np.float64),
+             elem_nodes=_normalized_connectivity(ref.elem_nodes),
+             elem_center_lon=np.asarray(ref.elem_center_lon, dtype=np.float64),
+             elem_center_lat=np.asarray(ref.elem_center_lat, dtype=np.float64),
+             node_to_elems=np.asarray(ref.node_to_elems, dtype=np.int32),
+             node_to_elems_count=np.asarray(ref.node_to_elems_count, dtype=np.int32),
+             sigma=np.asarray(getattr(ref, "sigma", np.array([], dtype=np.float64)), dtype=np.float64),
+             vertical_coord=str(getattr(ref, "vertical_coord", "zlevel")),
+             r_earth=float(getattr(ref, "r_earth", R_EARTH_STANDARD)),
+             metadata=md)
+    if gh_off is not None:
+        m.gh_offsets = np.asarray(gh_off, dtype=np.int32)
+        m.gh_node_ids = np.asarray(ref._gh_node_ids, dtype=np.int32)
+        m.gh_cell_size = float(ref._gh_cell_size)
+        m.gh_lon_min = float(ref._gh_lon_min)
+        m.gh_lat_min = float(ref._gh_lat_min)
+        m.gh_nx = int(ref._gh_nx)
+        m.gh_ny = int(ref._gh_ny)
+        m.gh_wrap_lon = bool(ref._gh_wrap_lon)
+        m.gh_far_rows = int(ref._gh_far_rows)
+        m.gh_max_edge_deg = getattr(ref, "_gh_max_edge_deg", None)
+    else:
+        build_grid_hash(m)
+    coastal = getattr(ref, "_elem_is_coastal", None)
+    if coastal is not None:
+        m.elem_is_coastal = np.asarray(coastal, dtype=bool)
+    nbr = getattr(ref, "_elem_neighbors", None)
+    if nbr is not None:
+        m.elem_neighbors = np.asarray(nbr, dtype=np.int32)
+    try:
+        ref.__dict__["_kelp_amd_mesh"] = (key, m)
+    except (AttributeError, TypeError):
+        pass
+    return m

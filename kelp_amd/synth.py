@@ -258,3 +258,215 @@ def snapshot_bracket(t, snap_dt, n_snaps, direction=1.0):
     if abs(alpha - 1.0) < 1e-12:
         return ib, -1, 0.0, it, dt_to_next
     return ia, ib, alpha, it, dt_to_next
+
+
+# ---------------------------------------------------------------------------
+# The batching workload: a mixed triangle/quad lattice with sigma layers and
+# 256 independent release clouds.
+# ---------------------------------------------------------------------------
+
+C4_SEED = 40404
+DAY = 86400.0
+
+
+def mixed_lattice_mesh(lon0=0.0, lon1=80.0, lat0=20.0, lat1=70.0, step=0.25, jitter=0.05, n_sigma=21,
+                       seed=C4_SEED, name="mixed_lattice") -> Mesh:
+    """Jittered lon-lat lattice (321 x 201 nodes over 0-80E x 20-70N at the
+    default step) whose quads are kept whole where the jitter's convexity
+    test passes and split into two triangles otherwise.  The test: a quad's
+    sharpest corner (the smallest sine of its four corner angles, all
+    counter-clockwise) is at least the median over all quads, so half the
+    quads stay whole.  Sigma coordinates with ``n_sigma`` interfaces (1 at
+    the surface .. 0 at the bed), bathymetry a seeded smooth field in
+    50-4000 m."""
+    nlon = int(round((lon1 - lon0) / step)) + 1
+    nlat = int(round((lat1 - lat0) / step)) + 1
+    LON, LAT = np.meshgrid(lon0 + step * np.arange(nlon), lat0 + step * np.arange(nlat))
+    rng = np.random.default_rng(seed)
+    LON = LON + rng.uniform(-jitter, jitter, LON.shape)
+    LAT = LAT + rng.uniform(-jitter, jitter, LAT.shape)
+    lon, lat = LON.ravel(), LAT.ravel()
+    nid = np.arange(nlat * nlon, dtype=np.int64).reshape(nlat, nlon)
+    q = np.stack([nid[:-1, :-1].ravel(), nid[:-1, 1:].ravel(), nid[1:, 1:].ravel(), nid[1:, :-1].ravel()], 1)
+    x = lon[q] * np.cos(np.radians(lat[q]))
+    y = lat[q]
+    sines = []
+    for k in range(4):
+        p, c, n = (k - 1) % 4, k, (k + 1) % 4
+        ax, ay = x[:, c] - x[:, p], y[:, c] - y[:, p]
+        bx, by = x[:, n] - x[:, c], y[:, n] - y[:, c]
+        sines.append((ax * by - ay * bx) / (np.hypot(ax, ay) * np.hypot(bx, by)))
+    sharp = np.min(np.stack(sines, 1), axis=1)
+    keep = (sharp >= np.median(sharp)) & (sharp > 0.0)
+    quads = q[keep]
+    split = q[~keep]
+    tris = np.concatenate([split[:, [0, 1, 2]], split[:, [0, 2, 3]]])
+    en = np.full((quads.shape[0] + tris.shape[0], 4), -1, dtype=np.int32)
+    en[:quads.shape[0]] = quads
+    en[quads.shape[0]:, :3] = tris
+    rng2 = np.random.default_rng(seed + 1)
+    phase = rng2.uniform(0, 2 * np.pi, 6)
+    xs, ys = np.radians(lon), np.radians(lat)
+    f = (np.sin(3 * xs + phase[0]) * np.cos(4 * ys + phase[1]) + 0.6 * np.sin(7 * xs + phase[2])
+         * np.sin(5 * ys + phase[3]) + 0.4 * np.cos(11 * xs + phase[4] + 9 * ys + phase[5]))
+    f = (f - f.min()) / (f.max() - f.min())
+    depth = 50.0 + 3950.0 * f ** 1.5
+    mesh = make_mesh(name, lon, lat, en, depth=depth)
+    mesh.sigma = 1.0 - np.linspace(0.0, 1.0, n_sigma)
+    mesh.vertical_coord = "sigma"
+    from .mesh import build_boundary_info
+    build_boundary_info(mesh)
+    return mesh
+
+
+def mixed_lattice_fields(mesh, U=0.4, dtype=np.float32, lon0=0.0, lon1=80.0, lat0=20.0, lat1=70.0):
+    """Steady fields of the batching workload: a double gyre over the domain
+    with a depth-decaying sigma profile on the K-1 layers, a small dia-sigma
+    flux W on the K interfaces (0 at the surface and the bed) and a surface
+    elevation; and the 2-D (surface layer) u, v."""
+    K = mesh.sigma.size
+    x = (mesh.nodes_lon - lon0) / (lon1 - lon0)
+    y = (mesh.nodes_lat - lat0) / (lat1 - lat0)
+    u0 = -U * np.sin(np.pi * x) * np.cos(np.pi * y)
+    v0 = U * np.cos(np.pi * x) * np.sin(np.pi * y)
+    smid = 0.5 * (mesh.sigma[:-1] + mesh.sigma[1:])[:, None]
+    prof = 0.3 + 0.7 * smid ** 2
+    u = (prof * u0[None, :]).astype(dtype)
+    v = (prof * v0[None, :]).astype(dtype)
+    s = mesh.sigma[:, None]
+    w = (2e-6 * np.sin(np.pi * s) * np.sin(2 * np.pi * x)[None, :] * np.cos(np.pi * y)[None, :]).astype(dtype)
+    w[0] = 0.0
+    w[-1] = 0.0
+    eta = 0.3 * np.sin(2 * np.pi * x) * np.sin(np.pi * y)
+    return u, v, w, eta, u[0].copy(), v[0].copy()
+
+
+def batch_clouds(mesh, n_clouds=256, seed=C4_SEED + 2, min_size=100, max_size=200_000, min_days=10,
+                 max_days=365, kh=10.0, size_scale=1.0):
+    """The batching workload's release clouds: sizes log-uniform in
+    [min_size, max_size] (times ``size_scale``), durations uniform in
+    [min_days, max_days] daily steps, half forward and half backward, 2-D
+    and 3-D 1:1, the random walk on for a random half; each cloud uniform
+    in a disc of radius 0.5-2 degrees around a random interior point, 3-D
+    depths uniform in 0-40 m.  Returns a list of dicts (lon, lat, z,
+    n_steps, direction, is3d, kh, seed)."""
+    rng = np.random.default_rng(seed)
+    lon_min, lon_max = mesh.nodes_lon.min() + 3.0, mesh.nodes_lon.max() - 3.0
+    lat_min, lat_max = mesh.nodes_lat.min() + 3.0, mesh.nodes_lat.max() - 3.0
+    out = []
+    dirs = np.array([1.0, -1.0] * (n_clouds // 2) + [1.0] * (n_clouds % 2))
+    rng.shuffle(dirs)
+    kinds = np.array([True, False] * (n_clouds // 2) + [True] * (n_clouds % 2))
+    rng.shuffle(kinds)
+    for k in range(n_clouds):
+        size = int(round(np.exp(rng.uniform(np.log(min_size), np.log(max_size))) * size_scale))
+        size = max(size, 1)
+        days = int(rng.integers(min_days, max_days + 1))
+        c_lon, c_lat = rng.uniform(lon_min, lon_max), rng.uniform(lat_min, lat_max)
+        rad = rng.uniform(0.5, 2.0)
+        r = rad * np.sqrt(rng.random(size))
+        th = rng.uniform(0, 2 * np.pi, size)
+        lat = c_lat + r * np.sin(th)
+        lon = c_lon + r * np.cos(th) / np.cos(np.radians(lat))
+        z = rng.uniform(0.0, 40.0, size)
+        out.append(dict(lon=lon, lat=lat, z=z, n_steps=days, direction=float(dirs[k]), is3d=bool(kinds[k]),
+                        kh=kh if rng.random() < 0.5 else 0.0, seed=int(rng.integers(1, 2 ** 62))))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# The precision workload: a NEST HEALPix dual mesh with 69 z-levels.
+# ---------------------------------------------------------------------------
+
+C3_LEVELS = np.concatenate([[0.0], np.geomspace(5.0, 6000.0, 68)])     # 69 interfaces
+
+
+def real_harmonics_field(lon, lat, lmax=8, seed=C2_SEED, chunk=2_000_000):
+    """:func:`harmonic_field` evaluated by the three-term recurrence of the
+    fully normalised associated Legendre functions (the normalisation and
+    Condon-Shortley phase of scipy's sph_harm_y, the same coefficients in
+    the same order), in chunks: fast enough for the 12.6 M pixel centres of
+    a HEALPix grid at nside 1024.  Equal to :func:`harmonic_field` to
+    rounding."""
+    lon = np.asarray(lon, dtype=np.float64)
+    lat = np.asarray(lat, dtype=np.float64)
+    rng = np.random.default_rng(seed)
+    coef = []
+    for l in range(1, lmax + 1):
+        for m in range(0, l + 1):
+            coef.append((l, m, rng.normal() / (l + 1), rng.normal() / (l + 1) if m > 0 else 0.0))
+    f = np.empty(lon.shape)
+    for s in range(0, lon.size, chunk):
+        th = np.radians(90.0 - lat[s:s + chunk])
+        ph = np.radians(lon[s:s + chunk])
+        x, sx = np.cos(th), np.sin(th)
+        acc = np.zeros(th.shape)
+        for m in range(0, lmax + 1):
+            # P_mm, then upwards in l
+            norm = (2 * m + 1) / (4 * np.pi) * np.prod([(2 * k - 1) / (2 * k) for k in range(1, m + 1)])
+            pmm = (-1.0) ** m * np.sqrt(norm) * sx ** m
+            p = {m: pmm}
+            if m + 1 <= lmax:
+                p[m + 1] = np.sqrt(2 * m + 3.0) * x * pmm
+            for l in range(m + 2, lmax + 1):
+                a = np.sqrt((4.0 * l * l - 1.0) / (l * l - m * m))
+                b = np.sqrt(((l - 1.0) ** 2 - m * m) / (4.0 * (l - 1.0) ** 2 - 1.0))
+                p[l] = a * (x * p[l - 1] - b * p[l - 2])
+            cm, sm = np.cos(m * ph), np.sin(m * ph)
+            for l, mm, cre, cim in coef:
+                if mm == m:
+                    acc += cre * p[l] * cm
+                    if m > 0:
+                        acc += cim * p[l] * sm
+        f[s:s + chunk] = acc
+    return (f - f.mean()) / f.std()
+
+
+def healpix_c3_mesh(nside=1024, land_threshold=0.7, seed=C2_SEED, z_levels=None):
+    """NEST HEALPix dual mesh (pixel centres = nodes) over the wet pixels of
+    the seeded l <= 8 harmonic mask of the 3-D workload (land where the
+    field exceeds ``land_threshold``), 69 z-level interfaces from 0 to
+    6000 m, bottom depth from the field as on the Fibonacci mesh."""
+    from .healpix import dual_mesh, pix2ang_nest
+    npix = 12 * nside * nside
+    lon, lat = pix2ang_nest(nside, np.arange(npix, dtype=np.int64), lonlat=True)
+    f = real_harmonics_field(lon, lat, seed=seed)
+    wet = f <= land_threshold
+    depth = 50.0 + 5950.0 * np.clip((land_threshold - f) / 2.0, 0.0, 1.0) ** 0.7
+    mesh = dual_mesh(nside, wet_mask=wet, depth=depth,
+                     z_levels=C3_LEVELS if z_levels is None else z_levels, name=f"healpix_nest_{nside}")
+    from .mesh import build_boundary_info
+    build_boundary_info(mesh)
+    return mesh
+
+
+def jet_rossby_snapshot_torch(mesh, t, torch, device, dtype, amp=0.3, H=800.0):
+    """:func:`jet_rossby_snapshot` (u, v, W) computed with torch on
+    ``device`` in float64, returned as numpy arrays of ``dtype``; for meshes
+    of millions of nodes and tens of layers."""
+    f64 = torch.float64
+    lam = torch.as_tensor(np.radians(mesh.nodes_lon), dtype=f64, device=device)[None, :]
+    phi = torch.as_tensor(np.radians(mesh.nodes_lat), dtype=f64, device=device)[None, :]
+    latd = torch.as_tensor(mesh.nodes_lat, dtype=f64, device=device)[None, :]
+    zl = torch.as_tensor(mesh.sigma, dtype=f64, device=device)
+    zm = (0.5 * (zl[:-1] + zl[1:]))[:, None]
+    g = torch.exp(-zm / H)
+    cphi, sphi = torch.cos(phi), torch.sin(phi)
+    u = amp * (torch.exp(-((latd - 35.0) / 8.0) ** 2) - 0.7 * torch.exp(-((latd + 45.0) / 9.0) ** 2))
+    v = torch.zeros_like(u)
+    for m, A, cdeg in ((3, 0.45 * amp * R_EARTH, -4.0), (5, 0.25 * amp * R_EARTH, -2.5)):
+        ph = m * (lam - np.radians(cdeg) * t / 86400.0)
+        dpsi_dphi = A * torch.cos(ph) * (-2.0 * cphi * sphi * torch.sin(2 * phi) + 2.0 * cphi ** 2 * torch.cos(2 * phi))
+        dpsi_dlam = -A * m * cphi ** 2 * torch.sin(2 * phi) * torch.sin(ph)
+        u = u - dpsi_dphi / R_EARTH
+        v = v + dpsi_dlam / (R_EARTH * torch.clamp(cphi, min=1e-6))
+    B = 0.05 * amp * R_EARTH
+    ph2 = 2.0 * lam - 2.0 * np.pi * t / (5 * 86400.0)
+    chi = B * cphi ** 2 * torch.cos(ph2)
+    u = u + (-2.0 * B * cphi * torch.sin(ph2)) / R_EARTH
+    v = v + (-2.0 * B * cphi * sphi * torch.cos(ph2)) / R_EARTH
+    tdt = torch.float32 if dtype == np.float32 else f64
+    U = (g * u).to(tdt).cpu().numpy()
+    V = (g * v).to(tdt).cpu().numpy()
+    W = (6.0 * chi * H * (1.0 - torch.exp(-zl[:, None] / H)) / R_EARTH ** 2).to(tdt).cpu().numpy()
+    return U, V, W

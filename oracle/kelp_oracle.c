@@ -14,6 +14,9 @@
 #include <string.h>
 
 #include "kelp_oracle.h"
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #ifdef ORACLE_KMATH
 #include "../kelp_amd/csrc/kmath.h"
@@ -2246,6 +2249,12 @@ void or_step(const or_mesh_t *m, const or_fields_t *f, const or_step_args_t *a, 
              double *lon, double *lat, double *z, int32_t *status, int32_t *nn, int8_t *fr,
              double *U, double *V, double *W, const int64_t *pid, int32_t *ws, int32_t *elem)
 {
+    /* particles are independent within a step (the reference's prange,
+     * integrators.py fused kernels): one thread per block of particles in
+     * the threaded build, identical results in any thread count */
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 256)
+#endif
     for (int64_t i = 0; i < P; i++) {
         int32_t prev = status[i];
         int32_t el_dummy = -1;
@@ -2295,6 +2304,9 @@ void or_step2d(const or_mesh_t *m, const or_fields_t *f, int order, const double
     a.dt = dt;
     a.time_dir = time_dir;
     for (int k = 0; k < 4; k++) a.alphas[k] = alphas[k];
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 256)
+#endif
     for (int64_t i = 0; i < P; i++) {
         lon_out[i] = lon[i];
         lat_out[i] = lat[i];
@@ -2590,6 +2602,9 @@ void or_sample(const or_mesh_t *m, const double *fa, const double *fb, double al
     fw.w_t = fa;
     fw.dtype = OR_F64;
     fw.alpha0 = alpha;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 256)
+#endif
     for (int64_t i = 0; i < P; i++) {
         if (status[i] != ST_ALIVE || nn[i] < 0) {
             out[i] = NAN;
@@ -2647,4 +2662,17 @@ void or_pchip4(int64_t n, const double *f0, const double *f1, const double *f2, 
         double h11 = s3 - s2;
         out[i] = h00 * f1[i] + h10 * h1 * m1 + h01 * f2[i] + h11 * h1 * m2;
     }
+}
+
+/* Thread count of the threaded build's particle loops (1 in the others);
+ * returns the count in effect. */
+int or_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
 }

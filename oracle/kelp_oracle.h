@@ -96,4 +96,6 @@ typedef struct {
     int64_t rk45_max_sub;
 } or_step_args_t;
 
+int or_set_threads(int n);
+
 #endif

@@ -2,8 +2,10 @@
 
 Used by tests/, the smoke check and the benchmark's CPU-baseline leg as the
 checker; never by the product path.  ``flavour`` selects the build:
-``"glibc"`` (host libm, as the reference's Numba JIT) or ``"kmath"`` (the
-device kernels' math, for bit-exact comparison with the GPU).
+``"glibc"`` (host libm, as the reference's Numba JIT), ``"kmath"`` (the
+device kernels' math, for bit-exact comparison with the GPU) or
+``"glibc_omp"`` (the glibc build with OpenMP over particles, whose thread
+count :func:`set_threads` sets).
 """
 from __future__ import annotations
 
@@ -146,6 +148,8 @@ def lib(flavour: str = "glibc") -> ctypes.CDLL:
         build()
     L = ctypes.CDLL(path)
     vp = c_void_p
+    L.or_set_threads.argtypes = [c_int]
+    L.or_set_threads.restype = c_int
     L.or_step2d.argtypes = [POINTER(OrMesh), POINTER(OrFields), c_int, vp, c_double, c_double,
                             c_int, c_int64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
     L.or_step2d.restype = None
@@ -376,6 +380,11 @@ class OracleMesh:
                          for x, y, e in zip(lon, lat, start)], np.int32)
 
 
+def set_threads(n: int, flavour: str = "glibc_omp") -> int:
+    """Threads of the threaded build's particle loops; returns the count in effect."""
+    return int(lib(flavour).or_set_threads(int(n)))
+
+
 def math_probe(fn: int, x, y=None, flavour="glibc") -> np.ndarray:
     x = _c(x, np.float64)
     y = x if y is None else _c(y, np.float64)
@@ -441,3 +450,42 @@ def bf16_round(a) -> np.ndarray:
     r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) & 0xFFFF
     r = np.where(nan, (u >> 16) | 0x40, r)
     return (r.astype(np.uint32) << 16).view(np.float32)
+
+
+def f16s_scales(rows) -> np.ndarray:
+    """Power-of-two scale of each row of float32 values (KELP_DT_F16S): the
+    largest finite |value| m = f 2^q (f in [0.5, 1)) gives 2^(q - 15), an
+    empty or all-zero row 1; exponents clamped to the float32 normal range."""
+    out = np.ones(len(rows), dtype=np.float32)
+    for k, r in enumerate(rows):
+        a = np.abs(np.asarray(r, dtype=np.float32))
+        a = a[np.isfinite(a)]
+        m = np.float32(a.max()) if a.size else np.float32(0.0)
+        if m > 0:
+            _, q = np.frexp(m)
+            out[k] = np.ldexp(np.float32(1.0), int(min(max(int(q) - 15, -126), 127)))
+    return out
+
+
+def f16s_round(u, v, w=None):
+    """The values a float16-with-level-scale snapshot holds, as float32
+    arrays: u and v of layer L share the scale of the larger of their row
+    maxima, every W interface row has its own; value * (1 / scale) rounded to
+    the nearest even half, read back as half * scale.  2-D: u, v of shape
+    (N,)."""
+    u = np.asarray(u, dtype=np.float32)
+    v = np.asarray(v, dtype=np.float32)
+    u2, v2 = np.atleast_2d(u), np.atleast_2d(v)
+    su, sv = f16s_scales(list(u2)), f16s_scales(list(v2))
+    suv = np.maximum(su, sv)
+
+    def q(a, s):
+        inv = (np.float32(1.0) / s).astype(np.float32)
+        h = (a * inv[:, None]).astype(np.float16)
+        return (h.astype(np.float32) * s[:, None]).astype(np.float32)
+
+    uq, vq = q(u2, suv).reshape(u.shape), q(v2, suv).reshape(v.shape)
+    if w is None:
+        return uq, vq
+    w = np.asarray(w, dtype=np.float32)
+    return uq, vq, q(w, f16s_scales(list(w)))

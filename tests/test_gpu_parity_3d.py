@@ -344,3 +344,61 @@ def test_bf16_storage_error_against_float32(gpu_ctx, world):
           f"max {d.max():.2f} m; depth p99 {np.percentile(np.abs(a['z'][ok] - b['z'][ok]), 99):.4f} m")
     assert d.max() > 0.0
     assert np.percentile(d, 99) < 5000.0
+
+
+def _f16s_world(world):
+    from oracle.oracle import f16s_round
+    mesh, snaps, lon, lat, z, pid = world
+    return [f16s_round(*(a.astype(np.float32) for a in sn[:3])) for sn in snaps]
+
+
+@pytest.mark.parametrize("order,direction", [(4, 1.0), (2, -1.0)])
+def test_3d_f16s_storage_bitwise(gpu_ctx, world, order, direction):
+    """float16 snapshots with a power-of-two scale per level (KELP_DT_F16S:
+    scales and rounding on the device, every read widened exactly) against
+    the oracle fed the same decoded values as float32: bit-identical through
+    a time blend, the Kh walk and beaching, twelve steps."""
+    mesh, snaps, lon, lat, z, pid = world
+    q = _f16s_world(world)
+    gpu_ctx.upload_mesh(mesh)
+    for s, sn in enumerate(snaps):
+        gpu_ctx.upload_field(s, sn[0].astype(np.float32), sn[1].astype(np.float32), sn[2].astype(np.float32),
+                             storage="f16s")
+    p = gpu_ctx.particles(lon, lat, z, pid=pid)
+    om = OracleMesh(mesh, "kmath")
+    st = dict(lon=lon, lat=lat, z=z, status=np.ones(lon.size, np.int32),
+              nn=np.full(lon.size, -1, np.int32), pid=pid)
+    kick = dict(kh_x=10.0, kh_y=10.0, seed=17)
+    t0 = 0.0 if direction > 0 else 2 * synth.SNAP_DT
+    for k in range(12):
+        f, (ia, ib, it, a0, dtn) = _fields(q, t0 + direction * k * DT, direction, np.float32)
+        c = (0.0, 0.5, 0.5, 1.0) if order == 4 else (0.0, 0.5)
+        al = [ck * DT / dtn for ck in c]
+        st = om.step(st, f, step_args(order, DT, direction, is3d=True, w_method=2, alphas=al, kick=kick,
+                                      beach=BEACH, step_i=k))
+        gpu_ctx.step(p, order, ia, DT, direction, slot_b=ib, slot_t=it, alpha0=a0, alphas=al, w_method=2,
+                     kick=kick, beach=BEACH, step_i=k)
+    gpu_ctx.synchronize()
+    assert (st["status"] == 1).mean() > 0.6
+    _assert_bitwise(p.to_host(), st)
+
+
+def test_2d_f16s_storage_bitwise(gpu_ctx, world):
+    from oracle.oracle import f16s_round
+    mesh, snaps, lon, lat, z, pid = world
+    u2, v2 = f16s_round(snaps[0][0][0].astype(np.float32), snaps[0][1][0].astype(np.float32))
+    gpu_ctx.upload_mesh(mesh)
+    gpu_ctx.upload_field(0, snaps[0][0][0].astype(np.float32), snaps[0][1][0].astype(np.float32), storage="f16s")
+    p = gpu_ctx.particles(lon, lat, z, pid=pid)
+    om = OracleMesh(mesh, "kmath")
+    st = dict(lon=lon, lat=lat, z=z, status=np.ones(lon.size, np.int32),
+              nn=np.full(lon.size, -1, np.int32), pid=pid)
+    for k in range(10):
+        st = om.step(st, dict(u_a=u2, v_a=v2, alpha0=0.0), step_args(2, DT, 1.0, is3d=False, step_i=k))
+        gpu_ctx.step(p, 2, 0, DT, 1.0, step_i=k)
+    gpu_ctx.synchronize()
+    h = p.to_host()
+    for key in ("lon", "lat", "u", "v"):
+        assert np.array_equal(h[key].view(np.int64), st[key].view(np.int64)), key
+    for key in ("status", "nn", "fr"):
+        assert np.array_equal(h[key], st[key]), key
