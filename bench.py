@@ -400,6 +400,145 @@ def run_batch(args):
     ctx.close()
 
 
+def _gc_m(lon1, lat1, lon2, lat2):
+    """Great-circle distance (m) on the 6371 km sphere (haversine)."""
+    p1, p2 = np.radians(lat1), np.radians(lat2)
+    dl = np.radians(lon2 - lon1)
+    h = np.sin((p2 - p1) / 2) ** 2 + np.cos(p1) * np.cos(p2) * np.sin(dl / 2) ** 2
+    return 2 * 6371e3 * np.arcsin(np.sqrt(np.clip(h, 0.0, 1.0)))
+
+
+def run_precision(args):
+    """The precision workload: the HEALPix NEST dual mesh at ``--nside`` with
+    69 z-levels over the wet pixels of the 3-D workload's harmonic mask, u/v/W
+    from the 3-D workload's generators (four six-hourly snapshots, cycled),
+    RK2 with dt 6 h and the Kh = 10 m^2/s random walk, forward for a year.
+    Every storage dtype -- float64, float32, bfloat16, float16 with
+    per-level scales -- runs the same particles; the error of each against
+    the float64-field run (great-circle and depth: median and p99 at 30, 90
+    and 365 days) and each one's plain-path throughput on ``--rate-particles``
+    particles (24 timed steps) are reported."""
+    from kelp_amd import synth
+    from kelp_amd.device import DeviceContext
+    t = time.time()
+    mesh = synth.healpix_c3_mesh(args.nside)
+    log(f"HEALPix nside {args.nside} dual mesh in {time.time() - t:.1f} s: {mesh.n_nodes} nodes, "
+        f"{mesh.n_elements} elements, {mesh.sigma.size} levels")
+    ctx = DeviceContext(0)
+    torch = ctx.torch
+    ctx.upload_mesh(mesh)
+    n_snap, dt, kh = 4, 21600.0, 10.0
+    t = time.time()
+    snaps64 = [synth.jet_rossby_snapshot_torch(mesh, k * synth.SNAP_DT, torch, "cuda:0", np.float64)
+               for k in range(n_snap)]
+    torch.cuda.empty_cache()
+    log(f"fields generated in {time.time() - t:.1f} s")
+    n_err = args.particles or 200_000
+    n_rate = args.rate_particles
+    lon, lat, z = synth.wet_area_seeds(mesh, max(n_err, n_rate), zmax=1000.0, seed=synth.C2_SEED + 3)
+    checkpoints = {30: 120, 90: 360, 365: 1460}
+    n_steps = max(checkpoints.values())
+    kick = dict(kh_x=kh, kh_y=kh, seed=20241231)
+    b_alg = {"f64": 1025.0, "f32": 769.0, "bf16": 641.0, "f16s": 641.0}
+    runs, rates = {}, {}
+
+    def bracket(k):
+        ia, ib, a0, it, dtn = synth.snapshot_bracket(k * dt, synth.SNAP_DT, n_snap)
+        al = [c * dt / dtn for c in (0.0, 0.5)] + [0.0, 0.0]
+        return ia, ib, a0, it, al
+
+    for storage in ("f64", "f32", "bf16", "f16s"):
+        t = time.time()
+        for k, (U, V, W) in enumerate(snaps64):
+            if storage == "f64":
+                ctx.upload_field(k, U, V, W)
+            else:
+                ctx.upload_field(k, U.astype(np.float32), V.astype(np.float32), W.astype(np.float32),
+                                 storage=None if storage == "f32" else storage)
+        log(f"{storage}: snapshots uploaded in {time.time() - t:.1f} s")
+        # the error run
+        p = ctx.particles(lon[:n_err], lat[:n_err], z[:n_err], pid=np.arange(n_err))
+        rec = {}
+        t = time.time()
+        for k in range(n_steps):
+            if args.resort_every and k % args.resort_every == 0:
+                ctx.sort(p)
+            ia, ib, a0, it, al = bracket(k)
+            ctx.step(p, 2, ia, dt, 1.0, slot_b=ib, slot_t=it, alpha0=a0, alphas=al, exact_nearest=1,
+                     w_method=2, kick=kick, step_i=k, reuse_anchor=True)
+            for day, st in checkpoints.items():
+                if k + 1 == st:
+                    h = p.to_host()
+                    o = np.argsort(h["pid"], kind="stable")
+                    rec[day] = {key: h[key][o] for key in ("lon", "lat", "z", "status")}
+            if (k + 1) % 365 == 0:
+                log(f"{storage}: {k + 1} of {n_steps} steps ({time.time() - t:.1f} s)")
+        runs[storage] = rec
+        del p
+        # the throughput run
+        p = ctx.particles(lon[:n_rate], lat[:n_rate], z[:n_rate], pid=np.arange(n_rate))
+        p.alive_count = torch.zeros(1, dtype=torch.int64, device="cuda:0")
+        n_t = 24
+        for k in range(n_t + 2):
+            if k == 2:
+                ctx.synchronize()
+                p.alive_count.zero_()
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+            if args.resort_every and k % args.resort_every == 0:
+                ctx.sort(p)
+            ia, ib, a0, it, al = bracket(k)
+            if k >= 2:
+                ctx.event_record(2 * (k - 2) + 2)
+            ctx.step(p, 2, ia, dt, 1.0, slot_b=ib, slot_t=it, alpha0=a0, alphas=al, exact_nearest=1,
+                     w_method=2, kick=kick, step_i=k, reuse_anchor=True)
+            if k >= 2:
+                ctx.event_record(2 * (k - 2) + 3)
+        ctx.synchronize()
+        wall = time.perf_counter() - t0
+        kern = sum(ctx.event_elapsed_ms(2 * j + 2, 2 * j + 3) for j in range(n_t)) / n_t / 1e3
+        alive = int(p.alive_count.item())
+        ach = b_alg[storage] * alive / n_t / kern / 1e9
+        rates[storage] = {"particle_steps_per_s": alive / wall, "ms_per_step": wall / n_t * 1e3,
+                          "kernel_avg_us": kern * 1e6, "b_alg_per_particle_step": b_alg[storage],
+                          "achieved_GBs": ach, "frac_of_hbm_peak": ach / HBM_PEAK_GBS}
+        log(f"{storage}: {alive / wall / 1e6:.1f} M particle-steps/s on {n_rate} particles, step kernel "
+            f"{kern * 1e6:.0f} us ({ach:.0f} GB/s)")
+        del p
+        torch.cuda.empty_cache()
+    errors = {}
+    ref = runs["f64"]
+    for storage in ("f32", "bf16", "f16s"):
+        errors[storage] = {}
+        for day in checkpoints:
+            a, b = runs[storage][day], ref[day]
+            ok = (a["status"] == 1) & (b["status"] == 1)
+            d = _gc_m(a["lon"][ok], a["lat"][ok], b["lon"][ok], b["lat"][ok])
+            dz = np.abs(a["z"][ok] - b["z"][ok])
+            errors[storage][f"{day}d"] = {
+                "median_m": float(np.median(d)), "p99_m": float(np.percentile(d, 99)),
+                "depth_median_m": float(np.median(dz)), "depth_p99_m": float(np.percentile(dz, 99)),
+                "alive_both": int(ok.sum()),
+                "status_differs": int((a["status"] != b["status"]).sum())}
+    f32 = rates["f32"]
+    line = {
+        "metric": "particle-steps/s", "value": f32["particle_steps_per_s"], "unit": "particle-steps/s",
+        "n_gpus": 1, "steps": 24, "warmup": 2, "ms_per_step": f32["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "field_dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"precision_healpix_nest{args.nside}_69lev", "mesh_nodes": mesh.n_nodes,
+                   "mesh_elements": mesh.n_elements, "z_levels": int(mesh.sigma.size), "snapshots": n_snap,
+                   "integrator": "rk2", "dt_s": dt, "kh_m2_s": kh, "particles_rate": n_rate,
+                   "particles_error": n_err, "steps_error_run": n_steps, "resort_every": args.resort_every},
+        "roofline": {"bound": "hbm", "achieved": f32["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": f32["frac_of_hbm_peak"], "traffic": None, "kernel": "step_kernel",
+                     "kernel_avg_us": f32["kernel_avg_us"], "b_alg_per_particle_step": f32["b_alg_per_particle_step"]},
+        "by_storage": rates,
+        "error_vs_f64": errors,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
 def run_serving(args):
     """The serving workload on the 3-D global mesh's fields (resident, cycled
     six-hourly snapshots): Poisson request arrivals against the wall clock,
@@ -423,7 +562,20 @@ def run_serving(args):
 
     def make(rid, rng, arrival):
         n = int(min(500_000, max(1, round(5000.0 * np.exp(args.size_sigma * rng.standard_normal())))))
-        # a random spherical cap of radius 1-10 degrees, seeds on wet elements inside it
+        # a random spherical cap of radius 1-10 degrees, seeds on wet elements
+        # inside it; a cap without enough ocean is drawn again
+        while True:
+            lon, lat = _cap_seeds(rng, n)
+            if lon is not None:
+                break
+        days = rng.uniform(1.0, 90.0)
+        return Request(rid=rid, lon=lon, lat=lat, z=rng.uniform(0.0, 1000.0, lon.size),
+                       n_steps=int(round(days * 24)), direction=float(rng.choice([1.0, -1.0])),
+                       is3d=bool(rng.random() < 0.5), kh=float(rng.choice([0.0, 10.0])),
+                       seed=int(rng.integers(1, 2 ** 62)), t0=float(rng.uniform(0, n_snap * w.snap_dt)),
+                       arrival=arrival)
+
+    def _cap_seeds(rng, n):
         rad = np.radians(rng.uniform(1.0, 10.0))
         cz = rng.uniform(-1.0, 1.0)
         clon = rng.uniform(-180.0, 180.0)
@@ -431,7 +583,9 @@ def run_serving(args):
                       np.sqrt(1 - cz * cz) * np.sin(np.radians(clon)), cz])
         lon_out, lat_out = [], []
         need = n
-        while need > 0:
+        for _attempt in range(8):
+            if need <= 0:
+                break
             m = max(4 * need, 256)
             zc = rng.uniform(np.cos(rad), 1.0, m)
             ph = rng.uniform(0, 2 * np.pi, m)
@@ -447,18 +601,9 @@ def run_serving(args):
             lon_out.append(lo[inside][:need])
             lat_out.append(la[inside][:need])
             need -= int(min(inside.sum(), need))
-            if m > 50 * n and need == n:
-                break           # a cap on land: resample the centre next time
-        lon = np.concatenate(lon_out) if lon_out else np.zeros(0)
-        lat = np.concatenate(lat_out) if lat_out else np.zeros(0)
-        if lon.size == 0:
-            return make(rid, rng, arrival)
-        days = rng.uniform(1.0, 90.0)
-        return Request(rid=rid, lon=lon, lat=lat, z=rng.uniform(0.0, 1000.0, lon.size),
-                       n_steps=int(round(days * 24)), direction=float(rng.choice([1.0, -1.0])),
-                       is3d=bool(rng.random() < 0.5), kh=float(rng.choice([0.0, 10.0])),
-                       seed=int(rng.integers(1, 2 ** 62)), t0=float(rng.uniform(0, n_snap * w.snap_dt)),
-                       arrival=arrival)
+        if need > 0:
+            return None, None
+        return np.concatenate(lon_out), np.concatenate(lat_out)
 
     t = time.time()
     reqs = poisson_stream(args.rate, args.serve_seconds, 5050, make)
@@ -469,7 +614,7 @@ def run_serving(args):
     srv = Server({"c2": pool}, chunk=4)
     for r in reqs:
         r.pool = "c2"
-    res = srv.run(reqs, realtime=True)
+    res = srv.run(reqs, realtime=True, log=log, progress_s=10.0)
     lat = latency_summary(res)
     pss = srv.alive_steps / srv.wall
     log(f"served {len(res)} requests in {srv.wall:.1f} s: {pss / 1e6:.1f} M particle-steps/s, "
@@ -502,7 +647,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=120)
     ap.add_argument("--warmup", type=int, default=12)
-    ap.add_argument("--workload", default="global_3d", choices=sorted(WORKLOADS) + ["batch", "serving"])
+    ap.add_argument("--workload", default="global_3d", choices=sorted(WORKLOADS) + ["batch", "serving", "precision"])
     ap.add_argument("--particles", type=int, default=0)
     ap.add_argument("--nodes", type=int, default=0)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
@@ -521,6 +666,9 @@ def main():
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first copy a buffer of this size 3x (calibrates profiler byte counters)")
     ap.add_argument("--clouds", type=int, default=256, help="batch: number of release clouds")
+    ap.add_argument("--nside", type=int, default=1024, help="precision: HEALPix nside")
+    ap.add_argument("--rate-particles", type=int, default=6_000_000,
+                    help="precision: particles of the throughput runs")
     ap.add_argument("--rate", type=float, default=50.0, help="serving: request arrivals per second")
     ap.add_argument("--serve-seconds", type=float, default=20.0, help="serving: arrival window (s)")
     ap.add_argument("--size-sigma", type=float, default=1.0, help="serving: lognormal sigma of cloud sizes")
@@ -536,6 +684,8 @@ def main():
         return run_batch(args)
     if args.workload == "serving":
         return run_serving(args)
+    if args.workload == "precision":
+        return run_precision(args)
     wl = WORKLOADS[args.workload](args)
     ctx = DeviceContext(0)
     torch = ctx.torch

@@ -404,7 +404,9 @@ int kelp_event_elapsed(kelp_ctx *ctx, int a, int b, float *ms);
 
 /* Diagnostics: evaluate the device math routines elementwise on device
  * arrays (fn: 0 sin, 1 cos, 2 asin, 3 acos, 4 atan2(x, y), 5 log, 6 exp,
- * 7 Python x % 360, 8 sqrt).  No reference counterpart; used to check the
+ * 7 Python x % 360, 8 sqrt, 9 Python x % y, 10 reflective fold of x into
+ * [0, y], 11 / 12 the first / second normal of the random walk's pair for
+ * key (seed 20240501, realization 0, particle x, step y, component 0)).  No reference counterpart; used to check the
  * device compiler against a host build of the same routines. */
 int kelp_math_probe(kelp_ctx *ctx, int fn, int64_t n, const double *x, const double *y,
                     double *out);

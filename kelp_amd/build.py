@@ -46,7 +46,7 @@ UNITS = [("kelp_hip", "kelp_hip.hip", []), ("kelp_sort", "kelp_sort.hip", []),
     (f"kelp_batch_{dt}_{'sigma' if sg else 'zlevel'}", "kelp_batch.hip",
      [f"-DKB_F32={int(dt == 'f32')}", f"-DKB_BF16={int(dt == 'bf16')}", f"-DKB_F16S={int(dt == 'f16s')}",
       f"-DKB_SIGMA={sg}"])
-    for dt, sg in (("f32", 0), ("f32", 1), ("f64", 0), ("f64", 1), ("bf16", 0), ("f16s", 0))
+    for dt, sg in (("f32", 0), ("f32", 1), ("bf16", 0), ("f16s", 0))
 ]
 
 

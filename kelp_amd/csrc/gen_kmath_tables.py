@@ -1,6 +1,7 @@
 """Print the constants of kmath.h (200-bit mpmath): the three-part pi/2
-split, pi and pi/2 as hi+lo, ln2 split 42+53 bits, the atan(j/8) table and the
-series coefficients.  Run: python kelp_amd/csrc/gen_kmath_tables.py"""
+split, pi and pi/2 as hi+lo, ln2 split 42+53 bits, the atan(j/8) table, the
+series coefficients, and the tables of the double-double log and sin/cos.
+Run: python kelp_amd/csrc/gen_kmath_tables.py"""
 import mpmath as mp
 
 mp.mp.prec = 200
@@ -40,5 +41,38 @@ def main():
     print("exp", [h(1 / mp.factorial(k)) for k in range(2, 16)])
 
 
+
+
+def cr_tables():
+    """The tables of the double-double log and sin/cos of kmath.h:
+    c_i = RN(1 / (1 + i/128)) and -log(c_i) as hi + lo for i = -37..53;
+    sin(j/64) and cos(j/64) as hi + lo for j = 0..51."""
+    cs, th, tl = [], [], []
+    for i in range(-37, 54):
+        c = float(1 / (1 + mp.mpf(i) / 128))
+        t = -mp.log(mp.mpf(c))
+        hi = float(t)
+        cs.append(h(c))
+        th.append(h(hi))
+        tl.append(h(t - hi))
+    print("static const double KM_LOGT_C[91] = {" + ", ".join(cs) + "};")
+    print("static const double KM_LOGT_H[91] = {" + ", ".join(th) + "};")
+    print("static const double KM_LOGT_L[91] = {" + ", ".join(tl) + "};")
+    sh, sl, ch, cl = [], [], [], []
+    for j in range(52):
+        a = mp.mpf(j) / 64
+        s, c = mp.sin(a), mp.cos(a)
+        sh.append(h(s)), sl.append(h(s - float(s))), ch.append(h(c)), cl.append(h(c - float(c)))
+    for nm, arr in (("SH", sh), ("SL", sl), ("CH", ch), ("CL", cl)):
+        print(f"static const double KM_SINT_{nm}[52] = {{" + ", ".join(arr) + "};")
+    third = mp.mpf(1) / 3
+    sixth = mp.mpf(1) / 6
+    print("KM_THIRD", h(third), h(third - float(third)), "KM_SIXTH", h(sixth), h(sixth - float(sixth)))
+    print("log1p tail", [h(mp.mpf((-1) ** (k + 1)) / k) for k in range(4, 13)])
+    print("sin tail", [h(mp.mpf((-1) ** k) / mp.factorial(2 * k + 1)) for k in range(2, 6)])
+    print("cos tail", [h(mp.mpf((-1) ** k) / mp.factorial(2 * k)) for k in range(2, 6)])
+
+
 if __name__ == "__main__":
     main()
+    cr_tables()

@@ -22,8 +22,8 @@
 // A particle's draws are keyed by its cloud-local id, the cloud's seed and
 // the cloud-local step, so its result is the one of its cloud run alone.
 //
-// Compiled once per (field dtype, vertical coordinate): -DKB_F32=0|1
-// -DKB_BF16=0|1 -DKB_F16S=0|1 -DKB_SIGMA=0|1.
+// Compiled once per (field dtype, vertical coordinate): -DKB_F32=1 (with
+// -DKB_SIGMA=0|1), -DKB_BF16=1 or -DKB_F16S=1 (z-level).
 #include <type_traits>
 
 #include "kelp_batch.cuh"

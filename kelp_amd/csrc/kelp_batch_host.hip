@@ -173,6 +173,8 @@ int batch_prepare(kelp_ctx* c, const kelp_particles* p, const kelp_step_args* ba
         }
     }
     if (ax.n_snap_3d > 0 && !p->z) return fail(KELP_EINVAL, "kelp_batch: 3-D clouds need z");
+    if (dtype == KELP_DT_F64) return fail(KELP_EINVAL, "kelp_batch: float64 snapshots are not batched (float32, "
+                                          "bfloat16 or float16 with level scales)");
     if ((dtype == KELP_DT_BF16 || dtype == KELP_DT_F16S) && sigma)
         return fail(KELP_EINVAL, "kelp_batch: bfloat16 / float16 snapshots on a sigma mesh");
     for (int k = 0; k < kMaxSlots; ++k) {
@@ -202,15 +204,9 @@ int batch_enqueue(kelp_ctx* c, const kelp_step_args* base, const StepCfg& cfg, c
             else if (dtype == KELP_DT_F32 && sigma)
                 launch_batch<float, true>(c->stream, base->order, block, c->m, sr, cfg, ps, clouds, steps,
                                           c->batch_n_clouds);
-            else if (dtype == KELP_DT_F32)
+            else
                 launch_batch<float, false>(c->stream, base->order, block, c->m, sr, cfg, ps, clouds, steps,
                                            c->batch_n_clouds);
-            else if (sigma)
-                launch_batch<double, true>(c->stream, base->order, block, c->m, sr, cfg, ps, clouds, steps,
-                                           c->batch_n_clouds);
-            else
-                launch_batch<double, false>(c->stream, base->order, block, c->m, sr, cfg, ps, clouds, steps,
-                                            c->batch_n_clouds);
         }
         HIP_TRY(hipGetLastError());
     }

@@ -412,6 +412,15 @@ __global__ void math_probe_kernel(int fn, int64_t n, const double* __restrict__ 
     case 7: r = km_pymod360(a); break;
     case 9: r = km_pymod_pos(a, y[i]); break;
     case 10: r = km_reflect_depth(a, 0.0, y[i]); break;
+    case 11:
+    case 12: {
+        // the random walk's normal pair (numerics/diffusion.py:234) of key
+        // (seed 20240501, real 0, pid x, step y, comp 0)
+        double g1, g2;
+        normal_pair(20240501ull, 0ull, (uint64_t)a, (uint64_t)y[i], 0ull, g1, g2);
+        r = fn == 11 ? g1 : g2;
+        break;
+    }
     default: r = km_sqrt(a); break;
     }
     out[i] = r;

@@ -5,26 +5,37 @@
  * results bit for bit.  Every routine is built from IEEE-754 binary64
  * add / sub / mul / div / sqrt only (correctly rounded on both targets),
  * with floating-point contraction disabled by the build (-ffp-contract=off):
- * no fused multiply-add, no platform libm, no lookup of vendor math code.
+ * no fused multiply-add (exact products by Dekker's splitting), no platform
+ * libm, no lookup of vendor math code.
  *
  * Methods (textbook, see J.-M. Muller, "Elementary Functions", 3rd ed.):
  *   sin/cos  Cody-Waite reduction by pi/2 split in three parts (33+33+53
- *            bits), remainder carried as a double-double; Taylor series of
- *            sin/cos on |r| <= pi/4 (truncation < 2^-62 relative); cos uses
- *            Dekker's exact square and a compensated 1 - r^2/2.
+ *            bits), remainder carried as a double-double; then a table of
+ *            sin/cos(j/64) as double-doubles and the addition formulas with
+ *            the Taylor series of sin/cos on |d| <= 1/128, the leading
+ *            terms in double-double arithmetic (Dekker's exact product):
+ *            relative error before the final rounding below 2^-80, so the
+ *            result is the correctly rounded sine / cosine except when the
+ *            exact value lies within that distance of a rounding boundary.
  *   atan     |x| < 3/16: Taylor series; 3/16 <= |x| <= 2: table of
  *            atan(j/8) (hi+lo) and atan(x) = atan(c) + atan((x-c)/(1+xc)),
  *            |t| <= 1/16; |x| > 2: pi/2 - atan(1/x).
  *   atan2    C99 special cases, quadrant fix with pi as hi+lo.
  *   asin/acos  through atan2 and sqrt((1-x)(1+x)).
- *   log      x = 2^k m, m in [sqrt(1/2), sqrt(2)), log(1+f) = 2 atanh(s),
- *            s = f/(2+f), written as f - (s f - s^3 P(s^2)).
+ *   log      x = 2^k m, m in [sqrt(1/2), sqrt(2)), c_i = RN(1/(1 + i/128))
+ *            for the nearest i, r = m c_i - 1 exact as a double-double
+ *            (|r| <= 2^-7.5), log x = k ln2 - log(c_i) + log1p(r) with
+ *            -log(c_i) tabulated as a double-double and log1p's terms up
+ *            to r^3 in double-double, the rest in double: relative error
+ *            before the final rounding below 2^-76 (near x = 1, where
+ *            c_i = 1, relative to log1p(r) itself).
  *   exp      x = k ln2 + r (ln2 split 42+53 bits), Taylor series on
  *            |r| <= ln2/2, scaled by 2^k.
  * Coefficients are the round-to-nearest binary64 values of the exact
- * rationals (-1)^k/(2k+1)!, 1/k!, 2/(2k+1); the tables of atan(j/8), pi/2
- * and ln2 come from kelp_amd/csrc/gen_kmath_tables.py (200-bit mpmath).
- * Accuracy against glibc 2.35 is measured in tests/test_kmath.py.
+ * rationals (-1)^k/(2k+1)!, 1/k!, 2/(2k+1), 1/k; the tables (atan(j/8),
+ * log(c_i), sin/cos(j/64)) and the constants pi/2 and ln2 come from
+ * kelp_amd/csrc/gen_kmath_tables.py (200-bit mpmath).
+ * Accuracy against glibc is measured in tests/test_oracle_pins.py.
  *
  * Valid as C99 (the oracle's diagnostic build) and as HIP C++.
  */
@@ -149,20 +160,6 @@ KM_FN int km_reduce_pio2(double x, double *rh, double *rl)
     return (int)fn;
 }
 
-KM_FN double km_sin_kernel(double rh, double rl)
-{
-    double z = rh * rh;
-    double p = -0x1.5555555555555p-3 +
-               z * (0x1.1111111111111p-7 +
-               z * (-0x1.a01a01a01a01ap-13 +
-               z * (0x1.71de3a556c734p-19 +
-               z * (-0x1.ae64567f544e4p-26 +
-               z * (0x1.6124613a86d09p-33 +
-               z * (-0x1.ae7f3e733b81fp-41 +
-               z * 0x1.952c77030ad4ap-49))))));
-    return rh + ((rh * z) * p + rl * (1.0 - 0.5 * z));
-}
-
 /* Dekker's exact product error a*b - fl(a*b), no fma. */
 KM_FN double km_two_prod_err(double a, double b, double p)
 {
@@ -173,22 +170,277 @@ KM_FN double km_two_prod_err(double a, double b, double p)
     return ((ah * bh - p) + ah * bl + al * bh) + al * bl;
 }
 
-KM_FN double km_cos_kernel(double rh, double rl)
+/* ------------------------------------------------------ double-double */
+
+KM_FN void km_two_sum(double a, double b, double *s, double *e)
 {
-    double zh = rh * rh;
-    double zl = km_two_prod_err(rh, rh, zh);
-    double q = 0x1.5555555555555p-5 +
+    double t = a + b;
+    double bb = t - a;
+    *e = (a - (t - bb)) + (b - bb);
+    *s = t;
+}
+
+KM_FN void km_fast_two_sum(double a, double b, double *s, double *e) /* |a| >= |b| or a == 0 */
+{
+    double t = a + b;
+    *e = b - (t - a);
+    *s = t;
+}
+
+/* (ah + al)(bh + bl) as a normalised double-double (relative error < 2^-100) */
+KM_FN void km_dd_mul(double ah, double al, double bh, double bl, double *ph, double *pl)
+{
+    double p = ah * bh;
+    double e = km_two_prod_err(ah, bh, p) + (ah * bl + al * bh);
+    km_fast_two_sum(p, e, ph, pl);
+}
+
+KM_FN void km_dd_add(double ah, double al, double bh, double bl, double *sh, double *sl)
+{
+    double s, e;
+    km_two_sum(ah, bh, &s, &e);
+    e += al + bl;
+    km_fast_two_sum(s, e, sh, sl);
+}
+
+static const double KM_LOGT_C[91] = {
+    0x1.6816816816817p+0, 0x1.642c8590b2164p+0, 0x1.6058160581606p+0,
+    0x1.5c9882b931057p+0, 0x1.58ed2308158edp+0, 0x1.5555555555555p+0,
+    0x1.51d07eae2f815p+0, 0x1.4e5e0a72f0539p+0, 0x1.4afd6a052bf5bp+0,
+    0x1.47ae147ae147bp+0, 0x1.446f86562d9fbp+0, 0x1.4141414141414p+0,
+    0x1.3e22cbce4a902p+0, 0x1.3b13b13b13b14p+0, 0x1.3813813813814p+0,
+    0x1.3521cfb2b78c1p+0, 0x1.323e34a2b10bfp+0, 0x1.2f684bda12f68p+0,
+    0x1.2c9fb4d812ca0p+0, 0x1.29e4129e4129ep+0, 0x1.27350b8812735p+0,
+    0x1.2492492492492p+0, 0x1.21fb78121fb78p+0, 0x1.1f7047dc11f70p+0,
+    0x1.1cf06ada2811dp+0, 0x1.1a7b9611a7b96p+0, 0x1.1811811811812p+0,
+    0x1.15b1e5f75270dp+0, 0x1.135c81135c811p+0, 0x1.1111111111111p+0,
+    0x1.0ecf56be69c90p+0, 0x1.0c9714fbcda3bp+0, 0x1.0a6810a6810a7p+0,
+    0x1.0842108421084p+0, 0x1.0624dd2f1a9fcp+0, 0x1.0410410410410p+0,
+    0x1.0204081020408p+0, 0x1.0000000000000p+0, 0x1.fc07f01fc07f0p-1,
+    0x1.f81f81f81f820p-1, 0x1.f44659e4a4271p-1, 0x1.f07c1f07c1f08p-1,
+    0x1.ecc07b301ecc0p-1, 0x1.e9131abf0b767p-1, 0x1.e573ac901e574p-1,
+    0x1.e1e1e1e1e1e1ep-1, 0x1.de5d6e3f8868ap-1, 0x1.dae6076b981dbp-1,
+    0x1.d77b654b82c34p-1, 0x1.d41d41d41d41dp-1, 0x1.d0cb58f6ec074p-1,
+    0x1.cd85689039b0bp-1, 0x1.ca4b3055ee191p-1, 0x1.c71c71c71c71cp-1,
+    0x1.c3f8f01c3f8f0p-1, 0x1.c0e070381c0e0p-1, 0x1.bdd2b899406f7p-1,
+    0x1.bacf914c1bad0p-1, 0x1.b7d6c3dda338bp-1, 0x1.b4e81b4e81b4fp-1,
+    0x1.b2036406c80d9p-1, 0x1.af286bca1af28p-1, 0x1.ac5701ac5701bp-1,
+    0x1.a98ef606a63bep-1, 0x1.a6d01a6d01a6dp-1, 0x1.a41a41a41a41ap-1,
+    0x1.a16d3f97a4b02p-1, 0x1.9ec8e951033d9p-1, 0x1.9c2d14ee4a102p-1,
+    0x1.999999999999ap-1, 0x1.970e4f80cb872p-1, 0x1.948b0fcd6e9e0p-1,
+    0x1.920fb49d0e229p-1, 0x1.8f9c18f9c18fap-1, 0x1.8d3018d3018d3p-1,
+    0x1.8acb90f6bf3aap-1, 0x1.886e5f0abb04ap-1, 0x1.8618618618618p-1,
+    0x1.83c977ab2beddp-1, 0x1.8181818181818p-1, 0x1.7f405fd017f40p-1,
+    0x1.7d05f417d05f4p-1, 0x1.7ad2208e0ecc3p-1, 0x1.78a4c8178a4c8p-1,
+    0x1.767dce434a9b1p-1, 0x1.745d1745d1746p-1, 0x1.724287f46debcp-1,
+    0x1.702e05c0b8170p-1, 0x1.6e1f76b4337c7p-1, 0x1.6c16c16c16c17p-1,
+    0x1.6a13cd1537290p-1
+};
+static const double KM_LOGT_H[91] = {
+    -0x1.5d5bddf595f31p-2, -0x1.522ae0738a3d7p-2, -0x1.4718dc271c41cp-2,
+    -0x1.3c25277333183p-2, -0x1.314f1e1d35ce3p-2, -0x1.269621134db91p-2,
+    -0x1.1bf99635a6b95p-2, -0x1.1178e8227e47ap-2, -0x1.07138604d5864p-2,
+    -0x1.f991c6cb3b37ap-3, -0x1.e530effe71013p-3, -0x1.d1037f2655e7bp-3,
+    -0x1.bd087383bd8aap-3, -0x1.a93ed3c8ad9e5p-3, -0x1.95a5adcf70182p-3,
+    -0x1.823c16551a3c0p-3, -0x1.6f0128b756ab9p-3, -0x1.5bf406b543db0p-3,
+    -0x1.4913d8333b563p-3, -0x1.365fcb0159014p-3, -0x1.23d712a49c201p-3,
+    -0x1.1178e8227e47ap-3, -0x1.fe89139dbd565p-4, -0x1.da7276384469ep-4,
+    -0x1.b6ac88dad5b1dp-4, -0x1.9335e5d594988p-4, -0x1.700d30aeac0e8p-4,
+    -0x1.4d3115d207eacp-4, -0x1.2aa04a44717a1p-4, -0x1.08598b59e3a06p-4,
+    -0x1.ccb73cdddb2d0p-5, -0x1.894aa149fb34bp-5, -0x1.466aed42de3f9p-5,
+    -0x1.0415d89e74440p-5, -0x1.8492528c8cac5p-6, -0x1.0205658935837p-6,
+    -0x1.010157588de69p-7, 0x0.0p+0, 0x1.fe02a6b106799p-8,
+    0x1.fc0a8b0fc03c4p-7, 0x1.7b91b07d5b126p-6, 0x1.f829b0e7832f8p-6,
+    0x1.39e87b9febd68p-5, 0x1.77458f632dcffp-5, 0x1.b42dd711971b9p-5,
+    0x1.f0a30c01162a8p-5, 0x1.16536eea37ae3p-4, 0x1.341d7961bd1d0p-4,
+    0x1.51b073f06183cp-4, 0x1.6f0d28ae56b4ep-4, 0x1.8c345d6319b23p-4,
+    0x1.a926d3a4ad562p-4, 0x1.c5e548f5bc743p-4, 0x1.e27076e2af2eap-4,
+    0x1.fec9131dbeabcp-4, 0x1.0d77e7cd08e5bp-3, 0x1.1b72ad52f67a2p-3,
+    0x1.29552f81ff521p-3, 0x1.371fc201e8f75p-3, 0x1.44d2b6ccb7d1cp-3,
+    0x1.526e5e3a1b438p-3, 0x1.5ff3070a793d6p-3, 0x1.6d60fe719d21bp-3,
+    0x1.7ab890210d907p-3, 0x1.87fa06520c911p-3, 0x1.9525a9cf456b6p-3,
+    0x1.a23bc1fe2b561p-3, 0x1.af3c94e80bff3p-3, 0x1.bc286742d8cd4p-3,
+    0x1.c8ff7c79a9a20p-3, 0x1.d5c216b4fbb94p-3, 0x1.e27076e2af2e8p-3,
+    0x1.ef0adcbdc5935p-3, 0x1.fb9186d5e3e29p-3, 0x1.0402594b4d041p-2,
+    0x1.0a324e27390e2p-2, 0x1.1058bf9ae4ad4p-2, 0x1.1675cababa60fp-2,
+    0x1.1c898c16999fbp-2, 0x1.22941fbcf7966p-2, 0x1.2895a13de86a4p-2,
+    0x1.2e8e2bae11d31p-2, 0x1.347dd9a987d56p-2, 0x1.3a64c556945eap-2,
+    0x1.404308686a7e4p-2, 0x1.4618bc21c5ec2p-2, 0x1.4be5f957778a1p-2,
+    0x1.51aad872df82ep-2, 0x1.5767717455a6cp-2, 0x1.5d1bdbf5809cap-2,
+    0x1.62c82f2b9c796p-2
+};
+static const double KM_LOGT_L[91] = {
+    -0x1.d5f75b9a23ae4p-59, -0x1.3840b263acb43p-56, -0x1.d8fb4c14c56eep-56,
+    -0x1.152d81af5713ap-56, -0x1.22966f61a3c23p-56, -0x1.e0efadd9db02ap-56,
+    0x1.e9575c2124912p-56, -0x1.b8ce2d07f1cb7p-56, 0x1.24e912b16ec8bp-60,
+    -0x1.ecca0cdf30143p-58, 0x1.f7627ef82f3f0p-57, 0x1.3f3adb7b71cbcp-58,
+    0x1.1165504ad749ep-59, -0x1.bcafa9de97202p-57, -0x1.8a16283fdbd1cp-57,
+    -0x1.6dcd318f4187ep-57, 0x1.37967087859b9p-59, 0x1.1f5b44c0df7f7p-61,
+    0x1.0d5604930f137p-58, -0x1.bea08d2dca256p-57, -0x1.51c7e9efae297p-57,
+    0x1.0e63a5f01c693p-58, 0x1.ac9f4215f9394p-58, -0x1.401fa71733017p-58,
+    0x1.002bf768e52d0p-58, 0x1.478a85704ccb7p-58, -0x1.a36a677b4c8b2p-59,
+    -0x1.da7d0b1e10b2fp-60, -0x1.aea2c72d05c08p-58, 0x1.dd7009902bf32p-58,
+    0x1.e48fb0500efd5p-59, 0x1.2ba0b44cfaee5p-59, 0x1.9badefe942718p-60,
+    -0x1.c05cf1d753621p-59, 0x1.d192d0619fa68p-60, -0x1.27c8e8416e717p-60,
+    -0x1.46662d417cecep-62, 0x0.0p+0, -0x1.e44b7e3711e7fp-67,
+    -0x1.83092c5964281p-62, -0x1.6d80ab38e9430p-62, 0x1.33e3f04f1ef25p-60,
+    -0x1.5bfa937f551b7p-59, 0x1.8d3ca87b92968p-63, 0x1.0a34531f67db5p-59,
+    0x1.85f325c5bbacdp-59, 0x1.2189705cf74cap-58, -0x1.3599f227becbbp-58,
+    -0x1.5b61c65e5741ap-58, -0x1.20db323097324p-59, -0x1.294d2f5668495p-58,
+    -0x1.d7a16eab1e2adp-59, 0x1.2eb0bf7c0b0d9p-59, -0x1.61578001e015ap-60,
+    -0x1.5746b9981b36cp-58, 0x1.9a5dc5e9030adp-57, -0x1.fbe7ee5c69946p-57,
+    0x1.301771c407dc0p-57, 0x1.e6cb62af18a02p-62, 0x1.7d3d950f87e23p-59,
+    -0x1.546ff8a470d3ap-57, -0x1.bc60efafc6f6cp-58, 0x1.d551d97132e87p-57,
+    -0x1.1072534a57e7dp-57, -0x1.9f7fdbfa08d9ap-57, -0x1.26fb3e2b1d1dap-57,
+    0x1.24dc46c1ea664p-57, 0x1.a3398064df33ep-57, 0x1.cfce744870f57p-58,
+    -0x1.4f689f8434011p-57, -0x1.a37794d03657dp-58, -0x1.61578001e015ep-59,
+    0x1.e8637950dc20dp-57, 0x1.355519b0de535p-57, -0x1.08ec217a5022dp-57,
+    0x1.bdcfde8061c03p-56, 0x1.3f415699663ecp-63, 0x1.ce63eab883727p-61,
+    0x1.9f1a39d500e3cp-56, -0x1.dbd7ac258a2bdp-58, 0x1.7ad24c13f040fp-56,
+    -0x1.1e99b72bd7bf2p-57, -0x1.16ea62c048cfbp-56, 0x1.cbcd735d03424p-60,
+    -0x1.f79f6c1059cdbp-57, -0x1.7a42642661c62p-61, -0x1.4b366b609027ap-58,
+    -0x1.d8db0a7cc1543p-56, -0x1.fb2a49af933e8p-57, -0x1.7dc9c7c23801fp-56,
+    -0x1.090a0dd59fe35p-58
+};
+static const double KM_SINT_SH[52] = {
+    0x0.0p+0, 0x1.fffaaaaeeeed5p-7, 0x1.ffeaaaeeee86fp-6,
+    0x1.7fdc01032fba9p-5, 0x1.ffaaaeeed4edbp-5, 0x1.3facb12d1755bp-4,
+    0x1.7f701032550e4p-4, 0x1.bf1b78568391dp-4, 0x1.feaaeee86ee36p-4,
+    0x1.1f0d3d7afceafp-3, 0x1.3eb312c5d66cbp-3, 0x1.5e44fcfa126f3p-3,
+    0x1.7dc102fbaf2b5p-3, 0x1.9d252d0cec312p-3, 0x1.bc6f84edc6199p-3,
+    0x1.db9e15fb5a5d0p-3, 0x1.faaeed4f31577p-3, 0x1.0cd00cef36436p-2,
+    0x1.1c37d64c6b876p-2, 0x1.2b8ddc43eb49fp-2, 0x1.3ad129769d3d8p-2,
+    0x1.4a00c9b0f3d20p-2, 0x1.591bc9fa2f597p-2, 0x1.682138a38d7f7p-2,
+    0x1.7710255764214p-2, 0x1.85e7a12826949p-2, 0x1.94a6be9f546c5p-2,
+    0x1.a34c91cc50ccap-2, 0x1.b1d8305321617p-2, 0x1.c048b17b140a3p-2,
+    0x1.ce9d2e3d4a51fp-2, 0x1.dcd4c15329c9ap-2, 0x1.eaee8744b05f0p-2,
+    0x1.f8e99e76abc97p-2, 0x1.0362939c69955p-1, 0x1.0a4021e9e1001p-1,
+    0x1.110d0c4b69c3bp-1, 0x1.17c8e5f2eedb0p-1, 0x1.1e7343236574cp-1,
+    0x1.250bb93788bbbp-1, 0x1.2b91dea88421ep-1, 0x1.32054b148bc4fp-1,
+    0x1.386597456282bp-1, 0x1.3eb25d36cd53ap-1, 0x1.44eb381cf386bp-1,
+    0x1.4b0fc46aab761p-1, 0x1.511f9fd7b351cp-1, 0x1.571a6966d59b3p-1,
+    0x1.5cffc16bf8f0dp-1, 0x1.62cf49921ac79p-1, 0x1.6888a4e134b2fp-1,
+    0x1.6e2b77c40bde1p-1
+};
+static const double KM_SINT_SL[52] = {
+    0x0.0p+0, -0x1.2ab639a9f0776p-63, -0x1.cd406fb224ae2p-60,
+    -0x1.599bdf46e997ap-59, -0x1.2d16d32684b69p-59, -0x1.921915299468bp-58,
+    0x1.afc2d1800501ap-60, 0x1.e91841dea4cc8p-58, -0x1.afcb2bcc6f03bp-59,
+    -0x1.6ef95099769a5p-57, 0x1.47d666b66cb91p-57, -0x1.6f443063f89b6p-57,
+    0x1.5ab50e23c97c3p-59, 0x1.9c43d80b1137dp-58, 0x1.9c1a56a7b0cabp-57,
+    -0x1.32e20d6cc6fc2p-57, -0x1.15d88508e32b8p-57, -0x1.9fb0a0c93e2b4p-56,
+    0x1.46076fe0dcff4p-56, 0x1.1553899f2d807p-57, 0x1.03d550487839ap-63,
+    0x1.823ba6bb08eadp-56, 0x1.7c74bac3fe0cbp-57, -0x1.d889202444aadp-56,
+    -0x1.6ead7314bb6cep-57, 0x1.8a40e9b5face0p-56, -0x1.69ce13e683f58p-56,
+    -0x1.a310e3b50cecdp-58, -0x1.ae242cb99f519p-56, 0x1.19fe6757e9fa7p-57,
+    -0x1.2fc8a12dae298p-57, 0x1.0d4c6e171fd9ap-56, -0x1.789b43c9b027dp-58,
+    0x1.9d950af2d00a3p-58, -0x1.2d8cd78397b01p-55, -0x1.6f643a13914f6p-55,
+    0x1.d918998809981p-55, 0x1.35e57102e2488p-57, 0x1.22a3fa4f41d5ap-56,
+    0x1.ea3d02457bccep-56, -0x1.fa371db216ab0p-55, 0x1.f6b42095a135bp-55,
+    -0x1.10fada93b07a8p-56, -0x1.be570e1570fc0p-58, -0x1.3ed6c1e6a5505p-55,
+    0x1.0da05738cc59cp-61, -0x1.5c0e861c48831p-55, 0x1.c843b4d0fb197p-58,
+    0x1.96cb370eb578ap-55, -0x1.edd9855b6241ap-55, -0x1.6b7d37644d5e6p-55,
+    -0x1.0e729857fad53p-56
+};
+static const double KM_SINT_CH[52] = {
+    0x1.0000000000000p+0, 0x1.fff000155549fp-1, 0x1.ffc00155527d3p-1,
+    0x1.ff7006bfdf99fp-1, 0x1.ff0015549f4d3p-1, 0x1.fe7034129ef6fp-1,
+    0x1.fdc06bf7e6b9bp-1, 0x1.fcf0c800e99b1p-1, 0x1.fc015527d5bd3p-1,
+    0x1.faf22263c4bd3p-1, 0x1.f9c340a7cc428p-1, 0x1.f874c2e1eecf6p-1,
+    0x1.f706bdf9ece1cp-1, 0x1.f57948cff6797p-1, 0x1.f3cc7c3b3d16ep-1,
+    0x1.f20073086649fp-1, 0x1.f01549f7deea1p-1, 0x1.ee0b1fbc0f11cp-1,
+    0x1.ebe214f76efa8p-1, 0x1.e99a4c3a7cd83p-1, 0x1.e733ea0193d40p-1,
+    0x1.e4af14b2a449cp-1, 0x1.e20bf49acd6c1p-1, 0x1.df4ab3ebd875ep-1,
+    0x1.dc6b7eb995912p-1, 0x1.d96e82f71a9dcp-1, 0x1.d653f073e4040p-1,
+    0x1.d31bf8d8d7c06p-1, 0x1.cfc6cfa52ad9fp-1, 0x1.cc54aa2b2972ep-1,
+    0x1.c8c5bf8ce1a84p-1, 0x1.c51a48b8b175ep-1, 0x1.c1528065b7d50p-1,
+    0x1.bd6ea310294f5p-1, 0x1.b96eeef58840ep-1, 0x1.b553a410c104ep-1,
+    0x1.b11d04162a4c6p-1, 0x1.accb526f69de5p-1, 0x1.a85ed4373e02dp-1,
+    0x1.a3d7d0352bdcfp-1, 0x1.9f368ed912f85p-1, 0x1.9a7b5a36a6514p-1,
+    0x1.95a67e00cb1fdp-1, 0x1.90b84784ddaf7p-1, 0x1.8bb105a5dc900p-1,
+    0x1.869108d77a6c6p-1, 0x1.8158a31916d5dp-1, 0x1.7c0827f09e54fp-1,
+    0x1.769fec655211fp-1, 0x1.712046fa77678p-1, 0x1.6b898fa9efb5dp-1,
+    0x1.65dc1fdeb8cbap-1
+};
+static const double KM_SINT_CL[52] = {
+    0x0.0p+0, 0x1.28a28a03a5ef3p-55, -0x1.3b54492d89b5bp-55,
+    -0x1.8b3b560648d5fp-56, 0x1.328387b99426fp-55, -0x1.cbf4337c96f97p-57,
+    0x1.31902b535f8dbp-55, 0x1.ea3d786d186acp-57, 0x1.b68f35094efb8p-55,
+    -0x1.52ace133a2769p-58, 0x1.c5b6b063b7462p-55, -0x1.c6514e1332b16p-55,
+    -0x1.698c80c36dcb4p-55, 0x1.e3a0d3e03b1d4p-57, -0x1.21a3ad28a3494p-57,
+    0x1.b940416c1984bp-56, 0x1.d3c1e99e5cafdp-55, -0x1.bfd2380bbc3b1p-59,
+    -0x1.02f9f12ba543ep-55, -0x1.2264b1bc53ce8p-55, -0x1.6428b3546ce13p-55,
+    -0x1.68ca02e8a6833p-55, -0x1.660aec7ef636bp-58, -0x1.e2d8a7e6736c4p-55,
+    0x1.4b364776dcd35p-58, 0x1.ff61bd5d2039dp-55, -0x1.76236434bec37p-55,
+    0x1.e60dd3089cbddp-56, 0x1.8b5b5508f2a0dp-55, 0x1.4ee162ba83a98p-57,
+    0x1.ab3d1a1590123p-56, -0x1.1bbb43b9aa880p-57, -0x1.892111312e828p-55,
+    0x1.31bbcc88c109dp-56, 0x1.45a3cc78fade0p-58, 0x1.8ff7947027a15p-58,
+    0x1.1dd561efbc0c2p-56, 0x1.8fb6a8dd6b6ccp-55, 0x1.9be06385ec792p-57,
+    -0x1.68dbaeca19669p-55, -0x1.1d200c5791606p-55, 0x1.722cfcc9fa7a9p-55,
+    -0x1.0befda21f862dp-55, -0x1.0feb10ab93b87p-56, 0x1.863e03e9474c1p-55,
+    0x1.338ffe2bfe9ddp-56, -0x1.de8b90b8228dep-57, -0x1.c73d6d72aee68p-57,
+    -0x1.827d5cf8c68c5p-57, 0x1.425b0a5029c81p-55, 0x1.15ac786ccf4b2p-56,
+    -0x1.97c1b47337c77p-58
+};
+
+#define KM_SIXTH_H 0x1.5555555555555p-3
+#define KM_SIXTH_L 0x1.5555555555555p-57
+#define KM_THIRD_H 0x1.5555555555555p-2
+#define KM_THIRD_L 0x1.5555555555555p-56
+
+/* sin and cos of r = rh + rl, |r| <= pi/4 (+ a rounding): r = j/64 + d,
+ * sin r = S_j cos d + C_j sin d, cos r = C_j cos d - S_j sin d, with the
+ * double-double sin/cos(j/64) of the table and
+ *   sin d = d - d^3/6 + d^5 (1/5! - d^2/7! + d^4/9! - d^6/11!),
+ *   cos d = 1 - d^2/2 + d^4 (1/4! - d^2/6! + d^4/8! - d^6/10!),
+ * |d| <= 1/128: the first two terms of each in double-double, the rest in
+ * double. */
+KM_FN void km_sincos_dd(double rh, double rl, double *s_out, double *c_out)
+{
+    int neg = rh < 0.0;
+    if (neg) {
+        rh = -rh;
+        rl = -rl;
+    }
+    int j = (int)km_rint_small(rh * 64.0);
+    if (j > 51) j = 51;
+    double dh0 = rh - (double)j * 0.015625; /* exact (Sterbenz, or j = 0) */
+    double dh, dl;
+    km_two_sum(dh0, rl, &dh, &dl);
+    double zh = dh * dh;
+    double zl = km_two_prod_err(dh, dh, zh) + 2.0 * dh * dl;
+    /* sin d */
+    double d3h, d3l;
+    km_dd_mul(dh, dl, zh, zl, &d3h, &d3l);
+    double t3h, t3l;
+    km_dd_mul(d3h, d3l, -KM_SIXTH_H, -KM_SIXTH_L, &t3h, &t3l);
+    double ps = ((zh * zh) * dh) * (0x1.1111111111111p-7 +
+                zh * (-0x1.a01a01a01a01ap-13 +
+                zh * (0x1.71de3a556c734p-19 +
+                zh * -0x1.ae64567f544e4p-26)));
+    double sdh, sdl;
+    km_dd_add(t3h, t3l, ps, 0.0, &sdh, &sdl);
+    km_dd_add(dh, dl, sdh, sdl, &sdh, &sdl);
+    /* cos d */
+    double q = (zh * zh) * (0x1.5555555555555p-5 +
                zh * (-0x1.6c16c16c16c17p-10 +
                zh * (0x1.a01a01a01a01ap-16 +
-               zh * (-0x1.27e4fb7789f5cp-22 +
-               zh * (0x1.1eed8eff8d898p-29 +
-               zh * (-0x1.93974a8c07c9dp-37 +
-               zh * (0x1.ae7f3e733b81fp-45 +
-               zh * -0x1.6827863b97d97p-53))))));
-    double hz = 0.5 * zh;
-    double w = 1.0 - hz;
-    double e = (1.0 - w) - hz; /* rounding error of w, exact */
-    return w + (e + ((zh * zh) * q - (0.5 * zl + rh * rl)));
+               zh * -0x1.27e4fb7789f5cp-22)));
+    double cdh, cdl;
+    km_dd_add(-0.5 * zh, -0.5 * zl, q, 0.0, &cdh, &cdl);
+    km_dd_add(1.0, 0.0, cdh, cdl, &cdh, &cdl);
+    /* the addition formulas */
+    double Sh = KM_SINT_SH[j], Sl = KM_SINT_SL[j], Ch = KM_SINT_CH[j], Cl = KM_SINT_CL[j];
+    double ah, al, bh, bl, sh, sl, ch, cl;
+    km_dd_mul(Sh, Sl, cdh, cdl, &ah, &al);
+    km_dd_mul(Ch, Cl, sdh, sdl, &bh, &bl);
+    km_dd_add(ah, al, bh, bl, &sh, &sl);
+    km_dd_mul(Ch, Cl, cdh, cdl, &ah, &al);
+    km_dd_mul(Sh, Sl, sdh, sdl, &bh, &bl);
+    km_dd_add(ah, al, -bh, -bl, &ch, &cl);
+    double sv = sh + sl;
+    *s_out = neg ? -sv : sv;
+    *c_out = ch + cl;
 }
 
 /* Arguments with |x| >= KM_TRIG_MAX lose the exactness of the reduction;
@@ -200,13 +452,15 @@ KM_FN double km_sin(double x)
     if (!(km_fabs(x) < KM_TRIG_MAX)) {
         if (km_isnan(x) || km_fabs(x) == __builtin_inf()) return x - x;
     }
-    double rh, rl;
+    if (x == 0.0) return x; /* keeps the sign of zero */
+    double rh, rl, sv, cv;
     int n = km_reduce_pio2(x, &rh, &rl);
+    km_sincos_dd(rh, rl, &sv, &cv);
     switch (n & 3) {
-    case 0: return km_sin_kernel(rh, rl);
-    case 1: return km_cos_kernel(rh, rl);
-    case 2: return -km_sin_kernel(rh, rl);
-    default: return -km_cos_kernel(rh, rl);
+    case 0: return sv;
+    case 1: return cv;
+    case 2: return -sv;
+    default: return -cv;
     }
 }
 
@@ -215,13 +469,14 @@ KM_FN double km_cos(double x)
     if (!(km_fabs(x) < KM_TRIG_MAX)) {
         if (km_isnan(x) || km_fabs(x) == __builtin_inf()) return x - x;
     }
-    double rh, rl;
+    double rh, rl, sv, cv;
     int n = km_reduce_pio2(x, &rh, &rl);
+    km_sincos_dd(rh, rl, &sv, &cv);
     switch (n & 3) {
-    case 0: return km_cos_kernel(rh, rl);
-    case 1: return -km_sin_kernel(rh, rl);
-    case 2: return -km_cos_kernel(rh, rl);
-    default: return km_sin_kernel(rh, rl);
+    case 0: return cv;
+    case 1: return -sv;
+    case 2: return -cv;
+    default: return sv;
     }
 }
 
@@ -236,15 +491,16 @@ KM_FN void km_sincos(double x, double *s, double *c)
             return;
         }
     }
-    double rh, rl;
+    double rh, rl, sk, ck;
     int n = km_reduce_pio2(x, &rh, &rl);
-    double sk = km_sin_kernel(rh, rl), ck = km_cos_kernel(rh, rl);
+    km_sincos_dd(rh, rl, &sk, &ck);
     switch (n & 3) {
     case 0: *s = sk; *c = ck; break;
     case 1: *s = ck; *c = -sk; break;
     case 2: *s = -sk; *c = -ck; break;
     default: *s = -ck; *c = sk; break;
     }
+    if (x == 0.0) *s = x;
 }
 
 /* ------------------------------------------------------------------ atan */
@@ -383,23 +639,43 @@ KM_FN double km_log(double x)
         m *= 0.5;
         k += 1;
     }
-    double f = m - 1.0; /* exact */
-    double s = f / (2.0 + f);
-    double z = s * s;
-    double p = 0x1.5555555555555p-1 +
-               z * (0x1.999999999999ap-2 +
-               z * (0x1.2492492492492p-2 +
-               z * (0x1.c71c71c71c71cp-3 +
-               z * (0x1.745d1745d1746p-3 +
-               z * (0x1.3b13b13b13b14p-3 +
-               z * (0x1.1111111111111p-3 +
-               z * (0x1.e1e1e1e1e1e1ep-4 +
-               z * (0x1.af286bca1af28p-4 +
-               z * (0x1.8618618618618p-4 +
-               z * 0x1.642c8590b2164p-4)))))))));
-    double lm = f - (s * f - (s * z) * p);
-    double dk = (double)k;
-    return dk * KM_LN2_HI + (lm + dk * KM_LN2_LO);
+    int i = (int)km_rint_small((m - 1.0) * 128.0); /* -37 .. 53, m - 1 exact */
+    double c = KM_LOGT_C[i + 37];
+    /* r = m c - 1 as a double-double: m c exact by Dekker, then p - 1 exact */
+    double p = m * c;
+    double pe = km_two_prod_err(m, c, p);
+    double rh, rl;
+    km_two_sum(p - 1.0, pe, &rh, &rl);
+    /* log1p(r) = r - r^2/2 + r^3/3 + r^4 P(r) */
+    double zh = rh * rh;
+    double zl = km_two_prod_err(rh, rh, zh) + 2.0 * rh * rl;
+    double r3h, r3l, t3h, t3l;
+    km_dd_mul(rh, rl, zh, zl, &r3h, &r3l);
+    km_dd_mul(r3h, r3l, KM_THIRD_H, KM_THIRD_L, &t3h, &t3l);
+    double tail = (zh * zh) * (-0x1.0000000000000p-2 +
+                  rh * (0x1.999999999999ap-3 +
+                  rh * (-0x1.5555555555555p-3 +
+                  rh * (0x1.2492492492492p-3 +
+                  rh * (-0x1.0000000000000p-3 +
+                  rh * (0x1.c71c71c71c71cp-4 +
+                  rh * (-0x1.999999999999ap-4 +
+                  rh * (0x1.745d1745d1746p-4 +
+                  rh * -0x1.5555555555555p-4))))))));
+    double sh, sl;
+    km_dd_add(t3h, t3l, tail, 0.0, &sh, &sl);
+    km_dd_add(-0.5 * zh, -0.5 * zl, sh, sl, &sh, &sl);
+    km_dd_add(rh, rl, sh, sl, &sh, &sl);
+    km_dd_add(KM_LOGT_H[i + 37], KM_LOGT_L[i + 37], sh, sl, &sh, &sl);
+    if (k != 0) {
+        /* k ln2 = k LN2_HI (exact: 42 bits times |k| < 2^11) + k LN2_LO (exact product) */
+        double dk = (double)k;
+        double kl = dk * KM_LN2_LO;
+        double kle = km_two_prod_err(dk, KM_LN2_LO, kl);
+        double kh, khl;
+        km_fast_two_sum(dk * KM_LN2_HI, kl, &kh, &khl);
+        km_dd_add(kh, khl + kle, sh, sl, &sh, &sl);
+    }
+    return sh + sl;
 }
 
 /* ------------------------------------------------------------------- exp */

@@ -111,9 +111,12 @@ class Server:
         self.pools = pools
         self.chunk = int(chunk)
 
-    def run(self, requests: List[Request], realtime: bool = True, log=None) -> List[Result]:
+    def run(self, requests: List[Request], realtime: bool = True, log=None,
+            progress_s: float = 0.0) -> List[Result]:
         """Serve ``requests`` (arriving at their ``arrival`` times when
-        ``realtime``, else all at once) until every one is done."""
+        ``realtime``, else all at once) until every one is done; with
+        ``log`` and ``progress_s`` a status line every ``progress_s``
+        seconds."""
         arrivals = deque(sorted(requests, key=lambda r: r.arrival))
         queue: List[Request] = []
         results: List[Result] = []
@@ -121,8 +124,15 @@ class Server:
         steps_done = 0
         for pl in self.pools.values():
             pl.batch.alive_steps(reset=True)
+        next_report = progress_s
         while arrivals or queue or any(pl.live for pl in self.pools.values()):
             now = time.perf_counter() - t_start
+            if log and progress_s > 0 and now >= next_report:
+                next_report += progress_s
+                live = sum(len(pl.live) for pl in self.pools.values())
+                used = sum(pl.batch.n_used for pl in self.pools.values())
+                log(f"serving t={now:.0f} s: {len(results)} done, {live} running ({used} slots used), "
+                    f"{len(queue)} waiting, {len(arrivals)} still to arrive")
             while arrivals and (not realtime or arrivals[0].arrival <= now):
                 queue.append(arrivals.popleft())
             # admission: arrival order, smaller requests may pass a waiting one
@@ -159,9 +169,6 @@ class Server:
                     st = b.collect(cid)
                     results.append(Result(req.rid, st, req.arrival, t_adm, time.perf_counter() - t_start,
                                           req.n * req.n_steps))
-                    if log:
-                        log(f"request {req.rid} done: {req.n} particles x {req.n_steps} steps, latency "
-                            f"{results[-1].latency:.3f} s")
         self.wall = time.perf_counter() - t_start
         self.alive_steps = sum(pl.batch.alive_steps() for pl in self.pools.values())
         return results

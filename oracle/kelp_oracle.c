@@ -2366,6 +2366,13 @@ void or_math_probe(int fn, int64_t n, const double *x, const double *y, double *
         case 7: out[i] = PYMOD360(x[i]); break;
         case 9: out[i] = PYMOD_POS(x[i], y[i]); break;
         case 10: out[i] = REFLECT_DEPTH(x[i], 0.0, y[i]); break;
+        case 11:
+        case 12: {
+            double g1, g2;
+            or_normal_pair(20240501ull, 0ull, (uint64_t)x[i], (uint64_t)y[i], 0ull, &g1, &g2);
+            out[i] = fn == 11 ? g1 : g2;
+            break;
+        }
         default: out[i] = sqrt(x[i]); break;
         }
     }

@@ -130,3 +130,46 @@ def test_global_3d_full_mesh_subsample_vs_glibc(gpu_ctx, global3d):
             checked += 1
     assert checked == 28
     assert (st["status"] == 1).sum() > 1800
+
+
+def test_healpix_precision_mesh_reduced_nside_vs_glibc(gpu_ctx):
+    """The precision workload at a reduced nside: the NEST HEALPix dual mesh
+    at nside 32 over the harmonic wet mask (quads and the valence-3
+    triangles), 69 z-levels, four six-hourly float32 snapshots of the 3-D
+    generators cycled, RK2 with dt 6 h and the Kh = 10 m^2/s random walk, 120
+    steps (30 days) of 3,000 particles, re-sorted every 8 steps; compared with
+    the glibc oracle every 20 steps."""
+    mesh = synth.healpix_c3_mesh(32)
+    assert mesh.sigma.size == 69 and mesh.max_v == 4
+    snaps = [synth.jet_rossby_snapshot(mesh, k * synth.SNAP_DT) for k in range(4)]
+    lon, lat, z = synth.wet_area_seeds(mesh, 3000, seed=77)
+    pid = np.arange(lon.size, dtype=np.int64)
+    gpu_ctx.upload_mesh(mesh)
+    for s, (U, V, W, _, _) in enumerate(snaps):
+        gpu_ctx.upload_field(s, U, V, W)
+    kick = dict(kh_x=10.0, kh_y=10.0, seed=20241231)
+    p = gpu_ctx.particles(lon, lat, z, pid=pid)
+    om = OracleMesh(mesh, "glibc")
+    st = dict(lon=lon.copy(), lat=lat.copy(), z=z.copy(), status=np.ones(lon.size, np.int32),
+              nn=np.full(lon.size, -1, np.int32), pid=pid.copy())
+    dt = 21600.0
+    for k in range(120):
+        ia, ib, a0, it, dtn = synth.snapshot_bracket(k * dt, synth.SNAP_DT, 4)
+        al = [c * dt / dtn for c in (0.0, 0.5)] + [0.0, 0.0]
+        A, T_ = snaps[ia], snaps[it]
+        f = dict(u_a=A[0], v_a=A[1], w_a=A[2], u_t=T_[0], v_t=T_[1], w_t=T_[2], alpha0=a0)
+        if ib >= 0:
+            B = snaps[ib]
+            f.update(u_b=B[0], v_b=B[1], w_b=B[2])
+        st = om.step(st, f, step_args(2, dt, 1.0, is3d=True, w_method=2, exact=1, alphas=al, kick=kick,
+                                      step_i=k))
+        if k % 8 == 0:
+            gpu_ctx.sort(p)
+        gpu_ctx.step(p, 2, ia, dt, 1.0, slot_b=ib, slot_t=it, alpha0=a0, alphas=al, exact_nearest=1,
+                     w_method=2, kick=kick, step_i=k, reuse_anchor=True)
+        if (k + 1) % 20 == 0:
+            h = p.to_host()
+            order = np.argsort(h["pid"], kind="stable")
+            h = {kk: v[order] for kk, v in h.items()}
+            _compare(h, st, ("lon", "lat", "z", "u", "v", "w"))
+    assert (st["status"] == 1).mean() > 0.8

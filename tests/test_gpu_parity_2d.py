@@ -26,6 +26,36 @@ def _state(lon, lat):
                 status=np.ones(len(lon), np.int32), nn=np.full(len(lon), -1, np.int32))
 
 
+def test_random_walk_draws_vs_host_builds(gpu_ctx):
+    """The random walk's normal pairs (SplitMix64 key chain, Box-Muller with
+    log, sqrt, cos, sin) of 10^7 keys (pids 0..10^7-1 at one step): the
+    device equals the host build of the same math bit for bit; against the
+    oracle built with glibc (the reference's libm under Numba) the pairs
+    differ only where glibc rounds log, sin or cos otherwise than the
+    correctly rounded result -- measured here, and each such draw is off by
+    at most one unit in the last place."""
+    t = gpu_ctx.torch
+    n = 10_000_000
+    pid = np.arange(n, dtype=np.float64)
+    step = np.full(n, 12345.0)
+    xd = t.as_tensor(pid, device="cuda:0")
+    yd = t.as_tensor(step, device="cuda:0")
+    rates = []
+    for fn in (11, 12):
+        od = t.empty_like(xd)
+        assert gpu_ctx.lib.kelp_math_probe(gpu_ctx.h, fn, n, xd.data_ptr(), yd.data_ptr(), od.data_ptr()) == 0
+        gpu_ctx.synchronize()
+        dev = od.cpu().numpy()
+        host = math_probe(fn, pid, step, flavour="kmath")
+        assert np.array_equal(dev.view(np.int64), host.view(np.int64)), fn
+        g = math_probe(fn, pid, step, flavour="glibc")
+        diff = dev != g
+        rates.append(diff.mean())
+        assert np.all(np.abs(dev[diff] - g[diff]) <= np.spacing(np.abs(g[diff])))
+    print(f"normal draws differing from the glibc build: {rates[0]:.3%} (g1), {rates[1]:.3%} (g2) of {n}")
+    assert max(rates) < 0.005
+
+
 def test_device_math_matches_host_build(gpu_ctx):
     """kmath.h compiled for gfx950 equals its host build bit for bit."""
     t = gpu_ctx.torch
@@ -197,3 +227,78 @@ def test_periodic_fold_bitwise(gpu_ctx, gyre):
         assert np.array_equal(h[k].view(np.int64), st[k].view(np.int64)), k
     for k in ("status", "nn", "fr"):
         assert np.array_equal(h[k], st[k]), k
+
+
+def _hole_mesh():
+    """A 0.1-degree lattice of triangles with a 4 x 4 degree hole filled by a
+    fan of long triangles around one centre node: inside the fan the hash
+    cells (about 0.2 degrees) hold no node, so the 3x3 window of a point
+    there is empty and the guard-off anchor widens ring by ring."""
+    from kelp_amd.mesh import make_mesh
+    n = 101
+    xs = np.linspace(0.0, 10.0, n)
+    LON, LAT = np.meshgrid(xs, xs + 40.0)
+    lon, lat = LON.ravel(), LAT.ravel()
+    ij = np.arange(n * n).reshape(n, n)
+    inner = (LON > 3.0 + 1e-9) & (LON < 7.0 - 1e-9) & (LAT > 43.0 + 1e-9) & (LAT < 47.0 - 1e-9)
+    tris = []
+    for j in range(n - 1):
+        for i in range(n - 1):
+            q = [ij[j, i], ij[j, i + 1], ij[j + 1, i + 1], ij[j + 1, i]]
+            if any(inner.ravel()[k] for k in q) or (3.0 - 1e-9 <= xs[i] < 7.0 - 1e-9 and 3.0 - 1e-9 <= xs[j] < 7.0 - 1e-9):
+                continue
+            tris += [[q[0], q[1], q[2]], [q[0], q[2], q[3]]]
+    # the hole's boundary ring, counter-clockwise, fanned to a centre node
+    ring = ([ij[30, i] for i in range(30, 70)] + [ij[j, 70] for j in range(30, 70)]
+            + [ij[70, i] for i in range(70, 30, -1)] + [ij[j, 30] for j in range(70, 30, -1)])
+    c = n * n
+    lon = np.append(lon, 5.0)
+    lat = np.append(lat, 45.0)
+    for k in range(len(ring)):
+        tris.append([ring[k], ring[(k + 1) % len(ring)], c])
+    keep = np.ones(lon.size, bool)
+    keep[:n * n] = ~inner.ravel()
+    remap = np.cumsum(keep) - 1
+    tris = remap[np.array(tris)]
+    return make_mesh("holed_lattice", lon[keep], lat[keep], tris.astype(np.int32))
+
+
+def test_guard_off_ring_widening_on_empty_windows(gpu_ctx):
+    """Anchor mode 0 (guard off, the fused kernels' widened window,
+    core/mesh.py:1365): particles inside the fan, whose 3x3 hash windows
+    are empty, anchor by ring widening; device = oracle (kmath) bit for bit
+    over 10 RK2 steps, and the windows really are empty."""
+    mesh = _hole_mesh()
+    rng = np.random.default_rng(21)
+    P = 3000
+    r = rng.uniform(0.4, 1.6, P)
+    th = rng.uniform(0, 2 * np.pi, P)
+    lon = 5.0 + r * np.cos(th)
+    lat = 45.0 + r * np.sin(th)
+    # the hash windows of the seeds are empty
+    ci = np.clip(((lon - mesh.gh_lon_min) / mesh.gh_cell_size).astype(int), 0, mesh.gh_nx - 1)
+    cj = np.clip(((lat - mesh.gh_lat_min) / mesh.gh_cell_size).astype(int), 0, mesh.gh_ny - 1)
+    cnt = np.diff(mesh.gh_offsets).reshape(mesh.gh_ny, mesh.gh_nx)
+    win = np.zeros(P, int)
+    for dj in (-1, 0, 1):
+        for di in (-1, 0, 1):
+            win += cnt[np.clip(cj + dj, 0, mesh.gh_ny - 1), np.clip(ci + di, 0, mesh.gh_nx - 1)]
+    assert (win == 0).mean() > 0.5
+    x = (mesh.nodes_lon - 5.0) / 5.0
+    y = (mesh.nodes_lat - 45.0) / 5.0
+    u = (-0.3 * y).astype(np.float64)
+    v = (0.3 * x).astype(np.float64)
+    gpu_ctx.upload_mesh(mesh)
+    gpu_ctx.upload_field(0, u, v)
+    p = gpu_ctx.particles(lon, lat)
+    om = OracleMesh(mesh, "kmath")
+    st = dict(lon=lon, lat=lat, status=np.ones(P, np.int32), nn=np.full(P, -1, np.int32))
+    for _ in range(10):
+        gpu_ctx.step(p, 2, 0, 1800.0, 1.0, exact_nearest=0)
+        st = om.step2d(st, 2, u, v, 1800.0, 1.0, exact=0)
+    h = p.to_host()
+    assert (st["status"] == 1).mean() > 0.9
+    for k in ("status", "nn"):
+        np.testing.assert_array_equal(h[k], st[k], err_msg=k)
+    for k in ("lon", "lat", "u", "v"):
+        np.testing.assert_array_equal(h[k].view(np.int64), st[k].view(np.int64), err_msg=k)

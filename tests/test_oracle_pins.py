@@ -115,6 +115,34 @@ def test_device_math_within_two_ulp_of_glibc(fn, lo, hi):
     assert np.max(np.abs(a - b) / ulp) <= 2.0
 
 
+def _box_muller_args(n, seed):
+    """The uniforms of the counter RNG's Box-Muller pair, (hi, lo 32 bits)
+    2^-32 + 2^-33 (numerics/diffusion.py:219), and 2 pi u2."""
+    rng = np.random.default_rng(seed)
+    u1 = rng.integers(0, 2 ** 32, n) * 2.0 ** -32 + 2.0 ** -33
+    u2 = rng.integers(0, 2 ** 32, n) * 2.0 ** -32 + 2.0 ** -33
+    return u1, 2.0 * np.pi * u2
+
+
+@pytest.mark.parametrize("fn", [5, 0, 1])
+def test_draw_math_correctly_rounded(fn):
+    """log, sin and cos of the random walk's Box-Muller arguments (and cos of
+    latitudes) are correctly rounded: equal to the 160-bit mpmath value
+    rounded to double on every sample.  The same samples measure how often
+    glibc (the reference's libm under Numba) rounds otherwise; those are the
+    only places where device draws can differ from the reference's."""
+    import mpmath as mp
+    mp.mp.prec = 160
+    u1, a = _box_muller_args(4000, 11 + fn)
+    x = u1 if fn == 5 else np.concatenate([a, np.radians(np.linspace(-90.0, 90.0, 1000))])
+    f = {5: mp.log, 0: mp.sin, 1: mp.cos}[fn]
+    ref = np.array([float(f(mp.mpf(float(v)))) for v in x])
+    k = math_probe(fn, x, flavour="kmath")
+    g = math_probe(fn, x, flavour="glibc")
+    assert np.array_equal(k, ref)
+    assert np.mean(g != ref) < 0.01
+
+
 def test_atan2_within_two_ulp_of_glibc():
     rng = np.random.default_rng(4)
     y = rng.uniform(-3, 3, 50000)
