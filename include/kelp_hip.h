@@ -111,7 +111,7 @@ typedef struct {
                                       0 guard off, fused kernels' widened window;
                                       2 guard off, batch 3x3 window healed exactly */
     int32_t w_method;              /* 3-D W: 0 linear at the anchor ("nearest"), 1 leastsq,
-                                      2 barycentric; sigma meshes also 3 idw */
+                                      2 barycentric, 3 idw */
     int32_t kick;                  /* split uniform-Kh random walk (integrators.py:673) */
     int32_t beach;                 /* beaching hook */
     double kh_x, kh_y;             /* m^2/s */

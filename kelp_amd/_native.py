@@ -147,6 +147,9 @@ SIGNATURES = {
     "kelp_event_elapsed": (c_int, [c_void_p, c_int, c_int, POINTER(c_float)]),
     "kelp_math_probe": (c_int, [c_void_p, c_int, c_int64, c_void_p, c_void_p, c_void_p]),
     "kelp_batch_set_clouds": (c_int, [c_void_p, c_int64, c_int64, c_void_p, POINTER(BatchAxis)]),
+    "kelp_gh_sample_stats": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_double,
+                                     c_double, c_double, c_int64, c_int64, c_int, c_void_p, c_void_p, c_void_p,
+                                     c_void_p]),
     "kelp_batch_reset": (c_int, [c_void_p, c_int64]),
     "kelp_batch_set_step": (c_int, [c_void_p, c_int64]),
     "kelp_batch_get_step": (c_int, [c_void_p, POINTER(c_int64)]),

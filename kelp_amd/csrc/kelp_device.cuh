@@ -1618,6 +1618,55 @@ __device__ inline double w_lsq(const DMesh& m, const Snap<T, true>& fw, double a
     return (b11 * fw_ + b12 * f1w + b13 * f2w) * inv;
 }
 
+// numerics/vertical.py:1148 _interp_w_idw_single on the stage field: the
+// same unique 1-ring nodes and bottom-aware W per node as w_lsq, weighted by
+// 1 / d^2 in the local metric; a node within 1 m returns its own W, an empty
+// weight sum falls back to linear W at the anchor.
+template <int MAXV, typename T>
+__device__ inline double w_idw(const DMesh& m, const Snap<T, true>& fw, double alpha0, double alpha, double xp,
+                               double yp, double zp, int node, double time_dir)
+{
+    int k0, k1;
+    n2e_range(m, node, k0, k1);
+    const WBracket bz = w_bracket(m, zp);
+    double z0 = km_cos(yp * kDeg2Rad) * kSLM;
+    if (z0 < 1.0) z0 = 1.0;
+    double sw = 0.0, swv = 0.0;
+    int count = 0;
+    // k = k0 - 1 stands for the centre node
+    for (int k = k0 - 1; k < k1; ++k) {
+        int4 q = make_int4(node, -1, -1, -1);
+        int nv = 1;
+        if (k >= k0) {
+            const int e = m.n2e[k];
+            if (e < 0) continue;
+            q = m.elems[e];
+            nv = MAXV;
+        }
+#pragma unroll
+        for (int v = 0; v < MAXV; ++v) {
+            if (v >= nv) break;
+            const int nd = v == 0 ? q.x : (v == 1 ? q.y : (v == 2 ? q.z : q.w));
+            if (nd < 0) continue;
+            if (k >= k0 && (nd == node || seen_before<MAXV>(m, k0, k, v, nd))) continue;
+            if (count >= 64) continue;
+            ++count;
+            const double w_val = w_clamped(m, fw, alpha0, alpha, zp, bz, nd, time_dir);
+            const double2 ll = m.lonlat[nd];
+            const double dlon = km_pymod360(ll.x - xp + 180.0) - 180.0;
+            const double dx = dlon * z0;
+            const double dy = (ll.y - yp) * kSLM;
+            const double d2 = dx * dx + dy * dy;
+            if (d2 < 1.0) return w_val;
+            const double w = 1.0 / d2;
+            sw += w;
+            swv += w * w_val;
+        }
+    }
+    if (sw < 1e-30) return w_clamped(m, fw, alpha0, alpha, zp, bz, node, time_dir);
+    return swv / sw;
+}
+
 // ---------------------------------------------------------------------------
 // Counter RNG (numerics/diffusion.py:181-246, numerics/beaching.py:46)
 // ---------------------------------------------------------------------------

@@ -16,6 +16,7 @@
 
 #include <algorithm>
 
+#include "../../include/kelp_grid_hash.h"
 #include "kelp_internal.h"
 #include "kelp_step.cuh"
 
@@ -424,6 +425,88 @@ __global__ void math_probe_kernel(int fn, int64_t n, const double* __restrict__ 
     default: r = km_sqrt(a); break;
     }
     out[i] = r;
+}
+
+// core/mesh.py:357 _gh_rect_sum over the prefix sum cum ((ny+1) x (nx+1))
+__device__ inline int64_t gh_rect_sum(const int32_t* __restrict__ cum, int nx, int ny, int wrap, int ci, int cj,
+                                      int rx, int ry)
+{
+    auto C = [&](int j, int i) { return (int64_t)cum[(int64_t)j * (nx + 1) + i]; };
+    int j0 = cj - ry;
+    if (j0 < 0) j0 = 0;
+    int j1 = cj + ry + 1;
+    if (j1 > ny) j1 = ny;
+    if (wrap && 2 * rx + 1 >= nx) return C(j1, nx) - C(j0, nx);
+    int i0 = ci - rx, i1 = ci + rx + 1;
+    if (wrap) {
+        if (i0 < 0) return (C(j1, i1) - C(j0, i1)) + (C(j1, nx) - C(j0, nx)) - (C(j1, i0 + nx) - C(j0, i0 + nx));
+        if (i1 > nx) return (C(j1, nx) - C(j0, nx)) - (C(j1, i0) - C(j0, i0)) + (C(j1, i1 - nx) - C(j0, i1 - nx));
+    } else {
+        if (i0 < 0) i0 = 0;
+        if (i1 > nx) i1 = nx;
+    }
+    return (C(j1, i1) - C(j0, i1)) - (C(j1, i0) - C(j0, i0));
+}
+
+// core/mesh.py:386 _gh_sample_stats for the candidate grid held in g (the
+// uploaded mesh with the candidate's cell, origin and dimensions); nodes in
+// internal numbering
+__global__ void gh_sample_stats_kernel(DMesh g, int64_t n, const double* __restrict__ lon,
+                                       const double* __restrict__ lat, const int32_t* __restrict__ nn_mesh,
+                                       const int32_t* __restrict__ new_of_old, const int32_t* __restrict__ cum,
+                                       int64_t* __restrict__ occ_out, double* __restrict__ cost_out,
+                                       int8_t* __restrict__ unprov_out, int8_t* __restrict__ approx_out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int nx = g.nx, ny = g.ny, wrap = g.wrap;
+    const double cell = g.cell;
+    const double lp = lon[i], tp = lat[i];
+    int ci, cj, cin, cjn;
+    gh_cell_of(g, lp, tp, ci, cj);
+    const int64_t occ = gh_rect_sum(cum, nx, ny, wrap, ci, cj, 1, 1);
+    occ_out[i] = occ;
+    const int node = new_of_old[nn_mesh[i]];
+    const double2 ll = g.lonlat[node];
+    gh_cell_of(g, ll.x, ll.y, cin, cjn);
+    int di = cin - ci;
+    if (di < 0) di = -di;
+    if (wrap && nx - di < di) di = nx - di;
+    int dj = cjn - cj;
+    if (dj < 0) dj = -dj;
+    const int8_t approx = (di > 1 || dj > 1) ? 1 : 0;
+    approx_out[i] = approx;
+    const Query q = make_query(lp, tp);
+    double theta, ext;
+    cap_extents(g, q, lp, tp, node, theta, ext);
+    bool provable = occ > 0 && approx == 0;
+    if (provable) provable = anchor_provable_rect(g, lp, tp, theta, ext, 1, 1) == 0;
+    if (provable) {
+        cost_out[i] = (double)occ + 9.0 * 1.0 + 20.0;
+        unprov_out[i] = 0;
+        return;
+    }
+    int ry = (int)ceil(theta / cell);
+    if (ry < 1) ry = 1;
+    if (ry > ny) ry = ny;
+    int rx;
+    if (ext >= 1.0e29) {
+        rx = nx;
+    } else {
+        rx = (int)ceil(ext / cell);
+        if (rx < 1) rx = 1;
+    }
+    int cells, iters;
+    if (rx * 2 + 1 >= nx) {
+        rx = nx;
+        cells = 9 + 2 * ry;
+        iters = ry;
+    } else {
+        cells = 9 + rx * (2 * ry + 1) + 2 * ry;
+        iters = rx > ry ? rx : ry;
+    }
+    cost_out[i] = ((double)gh_rect_sum(cum, nx, ny, wrap, ci, cj, rx, ry) + cells * 1.0) + (iters + 1) * 20.0;
+    unprov_out[i] = 1;
 }
 
 template <typename T, bool IS3D>
@@ -1393,9 +1476,8 @@ int kelp_step_prepare(kelp_ctx* c, const kelp_particles* p, const kelp_step_args
         if (!c->m.enbr) return fail(KELP_ESTATE, "kelp_step: elem_search walk needs the element neighbours");
         if (sigma) return fail(KELP_EINVAL, "kelp_step: elem_search walk on sigma meshes is not implemented");
     }
-    if (a->w_method < 0 || a->w_method > (sigma ? 3 : 2))
-        return fail(KELP_EINVAL, "kelp_step: w_method must be 0 (nearest), 1 (leastsq), 2 (barycentric)%s",
-                    sigma ? " or 3 (idw)" : "");
+    if (a->w_method < 0 || a->w_method > 3)
+        return fail(KELP_EINVAL, "kelp_step: w_method must be 0 (nearest), 1 (leastsq), 2 (barycentric) or 3 (idw)");
     if (p->n < 0) return fail(KELP_EINVAL, "kelp_step: negative particle count");
     if (p->n == 0) return KELP_OK;
     if (!p->lon || !p->lat || !p->status || !p->nn || !p->fr || !p->u || !p->v)
@@ -1673,6 +1755,31 @@ int kelp_math_probe(kelp_ctx* c, int fn, int64_t n, const double* x, const doubl
     if (n == 0) return KELP_OK;
     HIP_TRY(hipSetDevice(c->device));
     math_probe_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(fn, n, x, y, out);
+    HIP_TRY(hipGetLastError());
+    return KELP_OK;
+}
+
+int kelp_gh_sample_stats(kelp_ctx* c, int64_t n, const double* lon, const double* lat, const int32_t* nn,
+                         const int32_t* cum, double cell, double lon_min, double lat_min, int64_t nx, int64_t ny,
+                         int wrap, int64_t* occ, double* cost, int8_t* unprov, int8_t* approx)
+{
+    if (!c) return fail(KELP_EINVAL, "kelp_gh_sample_stats: NULL context");
+    if (!c->has_mesh) return fail(KELP_ESTATE, "kelp_gh_sample_stats: no mesh uploaded");
+    if (n < 0 || (n > 0 && (!lon || !lat || !nn || !cum || !occ || !cost || !unprov || !approx)))
+        return fail(KELP_EINVAL, "kelp_gh_sample_stats: NULL argument");
+    if (!(cell > 0.0) || nx < 1 || ny < 1 || nx * ny > 50000000)
+        return fail(KELP_EINVAL, "kelp_gh_sample_stats: bad candidate grid");
+    if (n == 0) return KELP_OK;
+    HIP_TRY(hipSetDevice(c->device));
+    DMesh g = c->m;
+    g.cell = cell;
+    g.lon_min = lon_min;
+    g.lat_min = lat_min;
+    g.nx = (int)nx;
+    g.ny = (int)ny;
+    g.wrap = wrap ? 1 : 0;
+    gh_sample_stats_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(g, n, lon, lat, nn, c->node_new_of_old, cum,
+                                                                   occ, cost, unprov, approx);
     HIP_TRY(hipGetLastError());
     return KELP_OK;
 }

@@ -301,6 +301,8 @@ __global__ void sample_kernel(DMesh m, SField s, Snap<double, true> fw, int meth
             r = w_linear(m, fw, s.alpha, 0.0, z, node, 1.0);
         else if (method == 1)
             r = w_lsq<MAXV>(m, fw, s.alpha, 0.0, lon, lat, z, node, 1.0);
+        else if (method == 3)
+            r = w_idw<MAXV>(m, fw, s.alpha, 0.0, lon, lat, z, node, 1.0);
         else
             r = w_bary<MAXV>(m, fw, s.alpha, 0.0, lon, lat, z, node, 1.0);
     } else {
@@ -538,8 +540,6 @@ extern "C" int kelp_sample(kelp_ctx* c, const kelp_particles* p, int var, int sl
     if (alpha != 0.0 && (!fb || !fb->planes || fb->n_planes != fa.n_planes))
         return kelp_fail(KELP_EINVAL, "kelp_sample: a blend needs the variable on slot_b with the same shape");
     const bool ifc = fa.n_planes == c->m.n_levels && c->m.n_levels >= 2;
-    if (ifc && method == 3)
-        return kelp_fail(KELP_EINVAL, "kelp_sample: idw on an interface field is not implemented");
     if (fa.n_planes > 1 && !p->z) return kelp_fail(KELP_EINVAL, "kelp_sample: a layered field needs z");
     if (!p->lon || !p->lat || !p->status || !p->nn) return kelp_fail(KELP_EINVAL, "kelp_sample: missing particle array");
     if (p->n <= 0) return KELP_OK;

@@ -88,6 +88,8 @@ __device__ inline bool eval_site(const DMesh& m, const Snap<T, IS3D>& f, const S
             W = w_linear(m, f, c.alpha0, alpha, zp, nn, c.time_dir);
         else if (c.w_method == 1)
             W = w_lsq<MAXV>(m, f, c.alpha0, alpha, lon, lat, zp, nn, c.time_dir);
+        else if (c.w_method == 3)
+            W = w_idw<MAXV>(m, f, c.alpha0, alpha, lon, lat, zp, nn, c.time_dir);
         else
             W = w_bary<MAXV>(m, f, c.alpha0, alpha, lon, lat, zp, nn, c.time_dir);
     }

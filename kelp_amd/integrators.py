@@ -22,7 +22,7 @@ anchor mode 2 of the kernel.
 u/v at the nodes are interpolated barycentrically in the containing element
 (``uv_interp="barycentric"``), u/v at element centres by least squares over
 the anchor's element ring (``uv_interp="leastsq"``, the reference default);
-W by ``nearest``, ``leastsq`` or ``barycentric`` (sigma meshes also ``idw``).
+W by ``nearest``, ``leastsq``, ``barycentric`` or ``idw``.
 Sigma meshes take the reference's ``sigma_eta_start`` / ``sigma_eta_end``
 / ``Dmin`` / ``z0b_min`` keywords; a PCHIP ``recon`` bundle (Euler, RK2) is
 reconstructed on the device; a state that carries ``elem`` runs the edge
@@ -31,8 +31,8 @@ walk (node-layout u/v, z-level).  The mesh may be this package's
 without copies by :func:`~kelp_amd.mesh.from_reference_mesh`), and the
 state, fields and diffusion arguments may be the reference's objects.
 Combinations the device path does not implement raise
-``NotImplementedError`` (edge_leastsq, W idw on z-level meshes, EM/M1
-diffusion, PCHIP with RK4); there is no CPU fallback.
+``NotImplementedError`` (edge_leastsq, EM/M1 diffusion, PCHIP with RK4);
+there is no CPU fallback.
 """
 from __future__ import annotations
 
@@ -48,7 +48,7 @@ from .mesh import Mesh, build_elem_neighbors, exact_nearest_on, from_reference_m
 from .particles import ParticleState
 
 _C_COEFFS = {1: (0.0,), 2: (0.0, 0.5), 4: (0.0, 0.5, 0.5, 1.0)}
-W_METHODS = {"nearest": 0, "leastsq": 1, "barycentric": 2}
+W_METHODS = {"nearest": 0, "leastsq": 1, "barycentric": 2, "idw": 3}
 SIGMA_W_METHODS = {"nearest": 0, "leastsq": 1, "barycentric": 2, "idw": 3}
 #: library slots (kMaxSlots); the last two hold PCHIP stage fields
 MAX_SLOTS = 16

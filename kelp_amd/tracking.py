@@ -422,7 +422,8 @@ def run_single_realization(mesh: Mesh, ds, reader, init_state: ParticleState, mo
                            writer: Optional[TrajectoryRecord] = None,
                            sim_time_start: Optional[float] = None,
                            checkpoint_freq: Optional[int] = None,
-                           checkpoint_path: Optional[str] = None) -> TrajectoryRecord:
+                           checkpoint_path: Optional[str] = None,
+                           checkpoint_sink=None) -> TrajectoryRecord:
     """One realization on the device (tracking.py:1805).  Returns the
     records, ``(particle, record)``: lon, lat, z, status, failure_reason,
     vel_u/vel_v/vel_w (NaN where undefined), time.
@@ -430,8 +431,11 @@ def run_single_realization(mesh: Mesh, ds, reader, init_state: ParticleState, mo
     Checkpoints (tracking.py:2785-2862): with ``checkpoint_path`` and
     ``checkpoint_freq`` (default ``numerics.checkpoint_freq``) > 0 the
     device state is written every ``checkpoint_freq`` global steps and after
-    the last step that ran (:func:`write_checkpoint`); :func:`load_restart`
-    returns the arguments that resume the run bit for bit."""
+    the last step that ran (:func:`write_checkpoint`, the reference's
+    restart variables); :func:`load_restart` returns the arguments that
+    resume the run bit for bit.  ``checkpoint_sink`` (a callable taking the
+    :func:`checkpoint_payload`) replaces the file at ``checkpoint_path``,
+    e.g. to keep the checkpoint inside the trajectory file."""
     from .device import DeviceContext
     sample_vars = list(sample_vars or [])
     if sample_vars and numerics.sample_interp not in SAMPLE_METHODS:
@@ -536,13 +540,20 @@ def run_single_realization(mesh: Mesh, ds, reader, init_state: ParticleState, mo
                                on_exhausted, dt, direction, start_step_i)
 
     ck_freq = int(numerics.checkpoint_freq if checkpoint_freq is None else checkpoint_freq)
-    ck_on = bool(checkpoint_path) and ck_freq > 0
+    ck_on = (bool(checkpoint_path) or checkpoint_sink is not None) and ck_freq > 0
     last_g = None
 
     def checkpoint(g_done, next_idx):
-        write_checkpoint(checkpoint_path, p.to_host(), g_done, next_idx,
-                         sim_time if use_tinterp else None,
-                         release_step=release_step, particle_offset=particle_offset)
+        meta = rec.meta if rec is not None else {}
+        release = ({k: meta["release_" + k] for k in ("lon", "lat", "z")}
+                   if all(("release_" + k) in meta for k in ("lon", "lat", "z")) else None)
+        payload = checkpoint_payload(p.to_host(), g_done, next_idx, sim_time if use_tinterp else None,
+                                     release_step=release_step, particle_offset=particle_offset, release=release)
+        if checkpoint_sink is not None:
+            checkpoint_sink(payload)
+        else:
+            from .writers import write_restart_file
+            write_restart_file(checkpoint_path, **payload)
 
     prefetch = int(getattr(numerics, "prefetch_size", 1) or 0) > 0
     recon_warned = False
@@ -698,61 +709,59 @@ def run_single_realization(mesh: Mesh, ds, reader, init_state: ParticleState, mo
     return rec
 
 
-def write_checkpoint(path: str, h: Dict[str, np.ndarray], step_i: int, next_time_idx,
-                     sim_time=None, release_step=None, particle_offset: int = 0) -> None:
-    """writers.py:567 write_checkpoint, as a ``.npz``: the device state in
-    particle-id order (``h`` as ``DeviceParticles.to_host`` returns it), the
-    last fully executed global step, the record index the next step reads
-    (``restart_file_idx``, the ``start_time_idx`` of a resume) and, for a
-    time-interpolated run, the clock.  Written to a temporary file, synced
-    and renamed, so an interrupted checkpoint never replaces a valid one
-    (the role of ``restart_valid``, writers.py:581)."""
-    import os
+def checkpoint_payload(h: Dict[str, np.ndarray], step_i: int, next_time_idx, sim_time=None, release_step=None,
+                       particle_offset: int = 0, release=None) -> dict:
+    """The keyword arguments of :func:`~kelp_amd.writers.write_restart_file`
+    for the device state ``h`` (as ``DeviceParticles.to_host`` returns it,
+    put in particle-id order): the last fully executed global step, the
+    record index the next step reads (``restart_file_idx``, the
+    ``start_time_idx`` of a resume; -1 when there is none), the wet
+    streaks, release schedule and positions, shard offset and, for a
+    time-interpolated run, the clock."""
     order = np.argsort(h["pid"], kind="stable")
-    arrays = {"restart_" + k: np.asarray(h[src])[order] for k, src in (
-        ("lon", "lon"), ("lat", "lat"), ("z", "z"), ("status", "status"),
-        ("nearest_node", "nn"), ("failure_reason", "fr"), ("wet_streak", "ws"), ("pid", "pid"))}
-    if h.get("elem") is not None:
-        arrays["restart_elem"] = np.asarray(h["elem"])[order]
-    arrays.update(restart_valid=np.int32(1), restart_step_i=np.int64(step_i),
-                  restart_file_idx=np.int64(-1 if next_time_idx is None else next_time_idx),
-                  restart_sim_time=np.float64(np.nan if sim_time is None else sim_time),
-                  restart_particle_offset=np.int64(particle_offset))
-    if release_step is not None:
-        arrays["restart_release_step"] = np.asarray(release_step, np.int64)
-    tmp = path + ".tmp"
-    with open(tmp, "wb") as fh:
-        np.savez(fh, **arrays)
-        fh.flush()
-        os.fsync(fh.fileno())
-    os.replace(tmp, path)
+    take = lambda k: np.asarray(h[k])[order]
+    rel = release or {}
+    st = ParticleState(lon=take("lon"), lat=take("lat"), z=take("z"), status=take("status").astype(np.int32),
+                       nearest_node=take("nn").astype(np.int32), failure_reason=take("fr").astype(np.int8),
+                       elem=take("elem").astype(np.int32) if h.get("elem") is not None else None,
+                       release_lon=rel.get("lon"), release_lat=rel.get("lat"), release_z=rel.get("z"))
+    return dict(state=st, step_i=int(step_i), file_idx=-1 if next_time_idx is None else int(next_time_idx),
+                wet_streak=take("ws").astype(np.int32),
+                release_step=None if release_step is None else np.asarray(release_step),
+                particle_offset=int(particle_offset), sim_time=sim_time)
+
+
+def write_checkpoint(path: str, h: Dict[str, np.ndarray], step_i: int, next_time_idx,
+                     sim_time=None, release_step=None, particle_offset: int = 0, release=None) -> None:
+    """writers.py:1420 write_restart_file for the device state: the
+    reference's restart variables in a classic NetCDF file, written to a
+    temporary sibling, synced and renamed (an interrupted checkpoint never
+    replaces a valid one; ``restart_valid`` is 1 only in a complete one)."""
+    from .writers import write_restart_file
+    write_restart_file(path, **checkpoint_payload(h, step_i, next_time_idx, sim_time, release_step,
+                                                  particle_offset, release))
 
 
 def load_restart(path: str) -> Dict[str, object]:
-    """writers.py:1541 load_restart for :func:`write_checkpoint` files:
-    ``state`` (ParticleState, particle-id order), ``start_step_i`` (the
-    next global step), ``start_time_idx``, ``sim_time_start`` (None for a
-    discrete run), ``start_wet_streak``, ``particle_offset``: the keyword
-    arguments of :func:`run_single_realization` that continue the run."""
-    with np.load(path, allow_pickle=False) as z:
-        if "restart_valid" not in z.files or int(z["restart_valid"]) != 1:
-            raise ValueError(f"No valid restart checkpoint in {path}")
-        d = {k: z[k] for k in z.files}
-    st = ParticleState(lon=d["restart_lon"], lat=d["restart_lat"], z=d["restart_z"],
-                       status=d["restart_status"].astype(np.int32),
-                       nearest_node=d["restart_nearest_node"].astype(np.int32),
-                       failure_reason=d["restart_failure_reason"].astype(np.int8))
-    if "restart_release_step" in d:
-        st = st.replace(release_step=d["restart_release_step"])
-    if "restart_elem" in d:
-        st = st.replace(elem=d["restart_elem"].astype(np.int32))
-    sim = float(d["restart_sim_time"])
-    idx = int(d["restart_file_idx"])
-    return dict(state=st, start_step_i=int(d["restart_step_i"]) + 1,
-                start_time_idx=(idx if idx >= 0 else 0),
-                sim_time_start=None if np.isnan(sim) else sim,
-                start_wet_streak=d["restart_wet_streak"].astype(np.int32),
-                particle_offset=int(d["restart_particle_offset"]))
+    """writers.py:1541 load_restart for a restart file or a trajectory file
+    with a restart block (this build's or the reference's): the reference's
+    keys, plus the keyword arguments of :func:`run_single_realization`
+    that continue the run -- ``start_step_i`` (the next global step),
+    ``start_time_idx``, ``sim_time_start`` (None for a discrete run),
+    ``start_wet_streak`` (zeros when not stored), ``particle_offset``
+    (0 when not stored)."""
+    from .writers import load_restart as _load
+    r = _load(path)
+    st = r["state"]
+    if st.failure_reason is None:
+        st = st.replace(failure_reason=np.zeros(st.n_particles, np.int8))
+    ws = r["wet_streak"]
+    out = dict(r)
+    out.update(state=st, start_step_i=r["step_i"] + 1, start_time_idx=max(r["file_idx"], 0),
+               sim_time_start=r["sim_time"],
+               start_wet_streak=np.zeros(st.n_particles, np.int32) if ws is None else ws,
+               particle_offset=int(r["particle_offset"] or 0))
+    return out
 
 
 def shift_to_midpoints(file_times: np.ndarray) -> np.ndarray:
@@ -806,11 +815,37 @@ def run_tracking(config: AppConfig, *, mesh: Mesh, dataset, reader="fesom2_array
     guard (tracking.py:3370), the sampled variables (``mode.sample_vars``)
     and ``time.num_runs`` realizations, each shifted by
     ``time.run_interval`` records.  Every record set carries the release
-    positions and time (``release_*``, writers.py:1209)."""
+    positions and time (``release_*``, writers.py:1209).  With
+    ``data.output_path`` each realization writes its file, which also holds
+    the last checkpoint when ``numerics.checkpoint_freq`` > 0 (rewritten at
+    every checkpoint, so :func:`load_restart` on the output file resumes an
+    interrupted run)."""
     from .readers import get_reader
     rd = get_reader(reader) if isinstance(reader, str) else reader
     ds = rd.open_dataset(dataset)
     set_exact_nearest(mesh, bool(config.mesh.exact_nearest))
+    # tracking.py:3378-3386: an explicit grid_hash_cell_size is pinned,
+    # otherwise mesh.grid_hash_cell_rule (occupancy by default) decides the
+    # cell, after the exact-nearest flag whose mode it depends on
+    from .cellrule import apply_grid_hash_cell_rule
+    from .device import DeviceContext
+    from .mesh import build_grid_hash
+    ctx = ctx or DeviceContext(0)
+    if getattr(config.mesh, "grid_hash_cell_size", None) is not None:
+        cs = float(config.mesh.grid_hash_cell_size)
+        if not (np.isfinite(cs) and 0.0 < cs < 45.0):
+            raise ValueError(f"grid-hash cell_size must be a finite value in (0, 45) degrees, got {cs!r}")
+        mesh.metadata["grid_hash_cell"] = cs
+        mesh.metadata["grid_hash_cell_rule"] = "explicit"
+        if mesh.gh_cell_size != cs:
+            build_grid_hash(mesh)
+    else:
+        if ctx.mesh is not mesh:
+            ctx.upload_mesh(mesh)
+        before = mesh.gh_cell_size
+        apply_grid_hash_cell_rule(mesh, ctx, getattr(config.mesh, "grid_hash_cell_rule", None) or "occupancy")
+        if mesh.gh_cell_size != before:
+            ctx.upload_mesh(mesh)
     start, n_steps = resolve_time_params(config.time)
     file_times, file_dt, dt, avg = resolve_record_times(config, ds, str(config.mesh.type).lower())
     numerics = config.numerics
@@ -819,6 +854,7 @@ def run_tracking(config: AppConfig, *, mesh: Mesh, dataset, reader="fesom2_array
         numerics = copy.copy(numerics)
         numerics.dt = dt
     n = int(ds.time.size)
+    from .writers import run_attributes, write_restart_file, write_trajectories
     out = []
     for r in range(int(config.time.num_runs)):
         s0 = start + r * int(config.time.run_interval)
@@ -830,24 +866,45 @@ def run_tracking(config: AppConfig, *, mesh: Mesh, dataset, reader="fesom2_array
                                              index_to_sim_time(s0, file_times, n, config.time.on_exhausted,
                                                                dt) if file_times is not None else 0.0),
                         time_averaging=avg)
+        path = None
+        if config.data.output_path:
+            # tracking.py:4015-4023: one file per realization (_r<i> when several)
+            import os
+            path = str(config.data.output_path)
+            if int(config.time.num_runs) > 1:
+                stem, ext = os.path.splitext(path)
+                path = f"{stem}_r{r}{ext}"
+        origin = ds.attrs.get("time_origin") if isinstance(ds.attrs, dict) else None
+        attrs = run_attributes(config, s0, n_steps, avg, mesh, numerics=numerics)
+        last_ck = {}
+
+        def write_out(restart, path=path, rec=rec, s0=s0, attrs=attrs):
+            """The realization's file: the records so far and, with
+            checkpoints on, the last checkpoint (tracking.py:2785-2862 keeps
+            it in the trajectory file); a checkpoint before the first record
+            is a restart-only file."""
+            if not rec.rows:
+                if restart is not None:
+                    write_restart_file(path, **restart)
+                return
+            arrays = rec.as_arrays()
+            if not config.mode.Fdrift3D:
+                arrays.pop("vel_w", None)      # no vel_w in 2-D
+            write_trajectories(path, arrays, f"seconds since {origin}" if origin else "seconds",
+                               "standard", attrs, restart=restart)
+
+        def sink(payload, write_out=write_out, last_ck=last_ck):
+            last_ck["payload"] = payload
+            write_out(payload)
+
+        ck_on = path is not None and int(numerics.checkpoint_freq or 0) > 0
         out.append(run_single_realization(
             mesh, ds, rd, init_state, config.mode, numerics, start_time_idx=s0, n_steps=n_steps,
             n_time_total=n, file_times=file_times, file_dt=file_dt,
             sample_vars=list(config.mode.sample_vars or []),
             on_exhausted=config.time.on_exhausted, realization_id=r,
-            particle_offset=int(config.data.particle_offset or 0), ctx=ctx, writer=rec))
-        if config.data.output_path:
-            # tracking.py:4015-4023: one file per realization (_r<i> when several)
-            import os
-            from .writers import run_attributes, write_trajectories
-            path = str(config.data.output_path)
-            if int(config.time.num_runs) > 1:
-                stem, ext = os.path.splitext(path)
-                path = f"{stem}_r{r}{ext}"
-            arrays = rec.as_arrays()
-            if not config.mode.Fdrift3D:
-                arrays.pop("vel_w", None)      # no vel_w in 2-D
-            origin = ds.attrs.get("time_origin") if isinstance(ds.attrs, dict) else None
-            write_trajectories(path, arrays, f"seconds since {origin}" if origin else "seconds",
-                               "standard", run_attributes(config, s0, n_steps, avg, mesh, numerics=numerics))
+            particle_offset=int(config.data.particle_offset or 0), ctx=ctx, writer=rec,
+            checkpoint_sink=sink if ck_on else None))
+        if path is not None:
+            write_out(last_ck.get("payload"))
     return out

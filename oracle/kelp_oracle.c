@@ -1140,6 +1140,45 @@ static double w_leastsq(const or_mesh_t *m, const or_fields_t *f, double alpha, 
     return (b11 * fw + b12 * f1w + b13 * f2w) * inv;
 }
 
+/* numerics/vertical.py:1148 _interp_w_idw_single on the stage field: the
+ * unique 1-ring nodes as for the least-squares W, W linear in depth at each
+ * (clamped to that node's bottom), weights 1 / d^2 in the local metric
+ * (z0 = max(cos(lat) * 111120, 1), dateline-safe dlon); a node within 1 m
+ * returns its own W; an empty weight sum falls back to linear W at the
+ * anchor.  pinned by the reference's on-node test (test_w_spatial_interp.py
+ * TestIDW), the 1 m coincidence radius and the inverse-square weights of an
+ * equatorial lattice whose ring distances are known in closed form. */
+static int collect_neighbours(const or_mesh_t *m, int node, int *buf);
+
+static double w_idw(const or_mesh_t *m, const or_fields_t *f, double alpha, double xp, double yp, double zp,
+                    int node, double time_dir)
+{
+    const double DEG2R = 0.017453292519943295;
+    const double EPS_DIST = 1.0;
+    int buf[64];
+    int count = collect_neighbours(m, node, buf);
+    double zc = zp < m->depth[node] ? zp : m->depth[node];
+    if (count < 1) return w_linear(m, f, alpha, zc, node, time_dir);
+    double z0 = M_COS(yp * DEG2R) * SLM;
+    if (z0 < 1.0) z0 = 1.0;
+    double sum_w = 0.0, sum_wv = 0.0;
+    for (int i = 0; i < count; i++) {
+        int nd = buf[i];
+        double bottom = m->depth[nd];
+        double w_val = w_linear(m, f, alpha, zp > bottom ? bottom : zp, nd, time_dir);
+        double dlon = PYMOD360(m->nodes_lon[nd] - xp + 180.0) - 180.0;
+        double dx = dlon * z0;
+        double dy = (m->nodes_lat[nd] - yp) * SLM;
+        double d2 = dx * dx + dy * dy;
+        if (d2 < EPS_DIST * EPS_DIST) return w_val;
+        double w = 1.0 / d2;
+        sum_w += w;
+        sum_wv += w * w_val;
+    }
+    if (sum_w < 1e-30) return w_linear(m, f, alpha, zc, node, time_dir);
+    return sum_wv / sum_w;
+}
+
 /* ------------------------------------------------------------------------
  * Counter RNG and kicks (numerics/diffusion.py)
  * ---------------------------------------------------------------------- */
@@ -1265,6 +1304,8 @@ static int eval_site(const or_mesh_t *m, const or_fields_t *f, const or_step_arg
             *W = w_linear(m, f, a->alphas[stage], zp, nn, a->time_dir);
         else if (a->w_method == 1)
             *W = w_leastsq(m, f, a->alphas[stage], lon, lat, zp, nn, a->time_dir);
+        else if (a->w_method == 3)
+            *W = w_idw(m, f, a->alphas[stage], lon, lat, zp, nn, a->time_dir);
         else
             *W = w_bary(m, f, a->alphas[stage], lon, lat, zp, nn, a->time_dir);
     }
@@ -2596,7 +2637,7 @@ static double combine_horizontal(const or_mesh_t *m, const or_scalar_t *s, doubl
  * NaN where not alive or without an anchor; 2-D and mid-layer fields by the
  * 1-ring combination, interface fields by the W interpolators
  * (vertical.py:1330 interp_w_spatial, time_dir +1).  method: 0 nearest,
- * 1 leastsq, 2 barycentric, 3 idw (idw not for interface fields). */
+ * 1 leastsq, 2 barycentric, 3 idw. */
 void or_sample(const or_mesh_t *m, const double *fa, const double *fb, double alpha, int n_planes,
                int method, int64_t P, const double *lon, const double *lat, const double *z,
                const int32_t *status, const int32_t *nn, double *out)
@@ -2623,6 +2664,8 @@ void or_sample(const or_mesh_t *m, const double *fa, const double *fb, double al
                 out[i] = w_linear(m, &fw, 0.0, z[i], node, 1.0);
             else if (method == 1)
                 out[i] = w_leastsq(m, &fw, 0.0, lon[i], lat[i], z[i], node, 1.0);
+            else if (method == 3)
+                out[i] = w_idw(m, &fw, 0.0, lon[i], lat[i], z[i], node, 1.0);
             else
                 out[i] = w_bary(m, &fw, 0.0, lon[i], lat[i], z[i], node, 1.0);
             continue;
@@ -2682,4 +2725,91 @@ int or_set_threads(int n)
     (void)n;
     return 1;
 #endif
+}
+
+/* ------------------------------------------------------------------------
+ * The occupancy cell rule's per-sample statistics (core/mesh.py:386
+ * _gh_sample_stats; the weights _GH_COST_CELL = 1, _GH_COST_CERT = 20 of
+ * :351-352), against a candidate grid given as the fields of a mesh copy
+ * (gh_cell, gh_lon_min, gh_lat_min, gh_nx, gh_ny, gh_wrap) and its node
+ * counts as the 2-D prefix sum cum ((ny+1) x (nx+1)).
+ * pinned by tests/test_cell_rule.py: occupancy and the approximate-anchor
+ * flag against a brute-force count and a brute-force nearest node, the
+ * widening flag against the 3x3 certificate, the cost against its formula
+ * on hand-built cases.
+ * ---------------------------------------------------------------------- */
+
+/* core/mesh.py:357 _gh_rect_sum */
+static int64_t gh_rect_sum(const int32_t *cum, int nx, int ny, int wrap, int ci, int cj, int rx, int ry)
+{
+#define CUM(j, i) ((int64_t)cum[(int64_t)(j) * (nx + 1) + (i)])
+    int j0 = cj - ry;
+    if (j0 < 0) j0 = 0;
+    int j1 = cj + ry + 1;
+    if (j1 > ny) j1 = ny;
+    if (wrap && 2 * rx + 1 >= nx) return CUM(j1, nx) - CUM(j0, nx);
+    int i0 = ci - rx, i1 = ci + rx + 1;
+    if (wrap) {
+        if (i0 < 0)
+            return (CUM(j1, i1) - CUM(j0, i1)) + (CUM(j1, nx) - CUM(j0, nx)) - (CUM(j1, i0 + nx) - CUM(j0, i0 + nx));
+        if (i1 > nx)
+            return (CUM(j1, nx) - CUM(j0, nx)) - (CUM(j1, i0) - CUM(j0, i0)) + (CUM(j1, i1 - nx) - CUM(j0, i1 - nx));
+    } else {
+        if (i0 < 0) i0 = 0;
+        if (i1 > nx) i1 = nx;
+    }
+    return (CUM(j1, i1) - CUM(j0, i1)) - (CUM(j1, i0) - CUM(j0, i0));
+#undef CUM
+}
+
+void or_gh_sample_stats(const or_mesh_t *m, int64_t n, const double *lon, const double *lat, const int32_t *nn,
+                        const int32_t *cum, int64_t *occ_out, double *cost_out, int8_t *unprov_out,
+                        int8_t *approx_out)
+{
+    const int nx = m->gh_nx, ny = m->gh_ny, wrap = m->gh_wrap;
+    const double cell = m->gh_cell;
+    for (int64_t i = 0; i < n; i++) {
+        int ci, cj, cin, cjn;
+        gh_cell_of(lon[i], lat[i], m, &ci, &cj);
+        int64_t occ = gh_rect_sum(cum, nx, ny, wrap, ci, cj, 1, 1);
+        occ_out[i] = occ;
+        int node = nn[i];
+        gh_cell_of(m->nodes_lon[node], m->nodes_lat[node], m, &cin, &cjn);
+        int di = cin - ci;
+        if (di < 0) di = -di;
+        if (wrap && nx - di < di) di = nx - di;
+        int dj = cjn - cj;
+        if (dj < 0) dj = -dj;
+        approx_out[i] = (di > 1 || dj > 1) ? 1 : 0;
+        double theta, ext;
+        cap_extents(lon[i], lat[i], m->nodes_lon[node], m->nodes_lat[node], &theta, &ext);
+        int provable = occ > 0 && approx_out[i] == 0;
+        if (provable) provable = anchor_provable_rect(m, lon[i], lat[i], node, 1, 1) == 0;
+        if (provable) {
+            cost_out[i] = (double)occ + 9.0 * 1.0 + 20.0;
+            unprov_out[i] = 0;
+        } else {
+            int ry = (int)ceil(theta / cell);
+            if (ry < 1) ry = 1;
+            if (ry > ny) ry = ny;
+            int rx;
+            if (ext >= 1.0e29) {
+                rx = nx;
+            } else {
+                rx = (int)ceil(ext / cell);
+                if (rx < 1) rx = 1;
+            }
+            int cells, iters;
+            if (rx * 2 + 1 >= nx) {
+                rx = nx;
+                cells = 9 + 2 * ry;
+                iters = ry;
+            } else {
+                cells = 9 + rx * (2 * ry + 1) + 2 * ry;
+                iters = rx > ry ? rx : ry;
+            }
+            cost_out[i] = ((double)gh_rect_sum(cum, nx, ny, wrap, ci, cj, rx, ry) + cells * 1.0) + (iters + 1) * 20.0;
+            unprov_out[i] = 1;
+        }
+    }
 }

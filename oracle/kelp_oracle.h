@@ -59,7 +59,7 @@ typedef struct {
 typedef struct {
     int32_t order;     /* 1 Euler, 2 RK2, 4 RK4 */
     int32_t is3d;
-    int32_t w_method;  /* 0 nearest (linear at the anchor), 1 leastsq, 2 barycentric */
+    int32_t w_method;  /* 0 nearest (linear at the anchor), 1 leastsq, 2 barycentric, 3 idw */
     int32_t exact;     /* mesh.exact_nearest */
     double dt, time_dir;
     double alphas[4];  /* stage fractions towards the target snapshot */
@@ -97,5 +97,9 @@ typedef struct {
 } or_step_args_t;
 
 int or_set_threads(int n);
+
+void or_gh_sample_stats(const or_mesh_t *m, int64_t n, const double *lon, const double *lat, const int32_t *nn,
+                        const int32_t *cum, int64_t *occ_out, double *cost_out, int8_t *unprov_out,
+                        int8_t *approx_out);
 
 #endif

@@ -148,6 +148,8 @@ def lib(flavour: str = "glibc") -> ctypes.CDLL:
         build()
     L = ctypes.CDLL(path)
     vp = c_void_p
+    L.or_gh_sample_stats.argtypes = [POINTER(OrMesh), c_int64, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.or_gh_sample_stats.restype = None
     L.or_set_threads.argtypes = [c_int]
     L.or_set_threads.restype = c_int
     L.or_step2d.argtypes = [POINTER(OrMesh), POINTER(OrFields), c_int, vp, c_double, c_double,
@@ -368,6 +370,29 @@ class OracleMesh:
                                                       "u", "v", "w", "pid", "ws")],
                        out["elem"].ctypes.data if "elem" in out else None)
         return out
+
+    def nearest_exact(self, lon, lat):
+        return self.nearest(lon, lat, exact=True)
+
+    def gh_sample_stats(self, qlon, qlat, nn, cum, cell, lon_min, lat_min, nx, ny, wrap):
+        """core/mesh.py:386 _gh_sample_stats against a candidate grid:
+        (occ, cost, unprov, approx) per sample."""
+        g = OrMesh.from_buffer_copy(self.s)
+        g.gh_cell, g.gh_lon_min, g.gh_lat_min = float(cell), float(lon_min), float(lat_min)
+        g.gh_nx, g.gh_ny, g.gh_wrap = int(nx), int(ny), int(bool(wrap))
+        qlon = _c(qlon, np.float64)
+        qlat = _c(qlat, np.float64)
+        nn = _c(nn, np.int32)
+        cum = _c(cum, np.int32)
+        n = qlon.size
+        occ = np.empty(n, np.int64)
+        cost = np.empty(n, np.float64)
+        unprov = np.empty(n, np.int8)
+        approx = np.empty(n, np.int8)
+        self.L.or_gh_sample_stats(ctypes.byref(g), n, qlon.ctypes.data, qlat.ctypes.data, nn.ctypes.data,
+                                  cum.ctypes.data, occ.ctypes.data, cost.ctypes.data, unprov.ctypes.data,
+                                  approx.ctypes.data)
+        return occ, cost, unprov, approx
 
     def walk_locate(self, lon, lat, start) -> np.ndarray:
         """core/mesh.py:2631 find_containing_elements_walk (elements only)."""

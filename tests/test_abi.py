@@ -1,5 +1,5 @@
-"""The C-ABI library loads and exports every symbol include/kelp_hip.h declares
-(no compute calls: runs without a GPU)."""
+"""The C-ABI library loads and exports every symbol the headers under
+include/ declare (no compute calls: runs without a GPU)."""
 import ctypes
 import os
 import re
@@ -10,9 +10,13 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def declared_symbols():
-    text = open(os.path.join(ROOT, "include", "kelp_hip.h")).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(kelp_\w+)\s*\(", text)))
+    names = set()
+    inc = os.path.join(ROOT, "include")
+    for fn in sorted(os.listdir(inc)):
+        if fn.endswith(".h"):
+            text = re.sub(r"/\*.*?\*/", "", open(os.path.join(inc, fn)).read(), flags=re.S)
+            names.update(re.findall(r"\b(kelp_\w+)\s*\(", text))
+    return sorted(names)
 
 
 def test_header_symbols_exported():

@@ -77,7 +77,7 @@ def _assert_bitwise(h, st):
         assert np.array_equal(h[k], st[k]), k
 
 
-@pytest.mark.parametrize("order,w_method", [(4, 2), (2, 2), (2, 0), (1, 2), (4, 1), (2, 1)])
+@pytest.mark.parametrize("order,w_method", [(4, 2), (2, 2), (2, 0), (1, 2), (4, 1), (2, 1), (4, 3), (1, 3)])
 def test_3d_tinterp_bitwise(gpu_ctx, world, order, w_method):
     h, st = _run(gpu_ctx, world, order, w_method, 1.0, 16, None, None)
     _assert_bitwise(h, st)

@@ -31,8 +31,6 @@ def _same(a, b):
 @pytest.mark.parametrize("method", ["nearest", "leastsq", "barycentric", "idw"])
 def test_sampling_bitwise(gpu_ctx, world, shape, method):
     mesh, lon, lat, z = world
-    if shape == "ifc" and method == "idw":
-        pytest.skip("idw on interface fields is not implemented (refused with an error)")
     K, N = mesh.sigma.size, mesh.n_nodes
     rng = np.random.default_rng(9)
     planes = {"2d": 1, "mid": K - 1, "ifc": K}[shape]

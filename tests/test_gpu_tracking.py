@@ -195,7 +195,7 @@ def test_checkpoint_resume_is_bitwise(gpu_ctx, tmp_path, split, tinterp):
     full = run_single_realization(case["mesh"], case["ds"], reader, case["init"], case["mode"],
                                   case["numerics"], start_time_idx=case["start_time_idx"],
                                   n_steps=case["n_steps"], **args).as_arrays()
-    ck = str(tmp_path / "restart.npz")
+    ck = str(tmp_path / "restart.nc")
     first = run_single_realization(case["mesh"], case["ds"], reader, case["init"], case["mode"],
                                    case["numerics"], start_time_idx=case["start_time_idx"],
                                    n_steps=split, checkpoint_freq=split, checkpoint_path=ck,
@@ -417,6 +417,29 @@ def test_run_tracking_matches_reference_multiyear_golden(gpu_ctx, tmp_path):
     for k, g in (("lon", "lon"), ("lat", "lat"), ("z", "z"), ("vel_u", "vel_u"), ("vel_v", "vel_v"),
                  ("vel_w", "vel_w"), ("sample__temp", "temp")):
         np.testing.assert_allclose(out[k], gold[g], rtol=1e-9, atol=1e-9, err_msg=k)
+
+
+def test_run_tracking_output_keeps_the_last_checkpoint(gpu_ctx, tmp_path):
+    """With numerics.checkpoint_freq > 0 the output file also holds the
+    checkpoint (tracking.py:2785-2862 through the writer): after the run it
+    is the last step's state, in the reference's restart variables."""
+    import copy
+    from kelp_amd.tracking import load_restart, run_tracking
+    case = rc.multiyear_case()
+    cfg = copy.deepcopy(case["config"])
+    cfg.data.output_path = str(tmp_path / "traj.nc")
+    cfg.numerics.checkpoint_freq = 3
+    (rec,) = run_tracking(cfg, mesh=case["mesh"], dataset=case["ds"], reader="fesom2_arrays",
+                          init_state=case["init"], ctx=gpu_ctx)
+    out = rec.as_arrays()
+    r = load_restart(cfg.data.output_path)
+    n_rec = out["time"].size
+    assert r["step_i"] == n_rec - 1 and r["start_step_i"] == n_rec
+    np.testing.assert_array_equal(r["state"].status, out["status"][:, -1])
+    alive = out["status"][:, -1] == 1
+    for k in ("lon", "lat", "z"):
+        np.testing.assert_array_equal(getattr(r["state"], k)[alive], out[k][alive, -1], err_msg=k)
+    np.testing.assert_array_equal(r["state"].release_lon, out["release_lon"])
 
 
 def test_rk45_run_agrees_with_fine_rk4_run(gpu_ctx):

@@ -215,3 +215,51 @@ def test_sigma_idw_column_height_weight_law(meshes):
     h_on = om3.column_height(np.array([m.nodes_lon[n0]]), np.array([m.nodes_lat[n0]]), anc, eta,
                              Dmin=0.5, method="idw")
     assert h_on[0] == 0.5
+
+
+def test_interface_idw_weight_law_clamp_and_coincidence(meshes):
+    """W by inverse distance on an interface field (numerics/vertical.py:1148
+    _interp_w_idw_single through sampling.py:875): the 1/d^2 weight law of
+    the 1-ring (as for the node scalars), a node within 1 m returns its own
+    W (the W rule's radius, not the 1e-12 m^2 of the scalar rule), and a
+    shallow neighbour contributes W at its own bottom."""
+    reg = meshes["reg"]
+    zl = np.array([0.0, 10.0, 25.0, 50.0, 100.0, 200.0])
+    K = zl.size
+    depth = np.full(reg.n_nodes, 200.0)
+    m = make_mesh("ifc", reg.nodes_lon, reg.nodes_lat, reg.elem_nodes, depth=depth, z_levels=zl)
+    om = OracleMesh(m, "glibc")
+    lon_p, lat_p, n0, nw, ne, nn_ = _row_case(m)
+    lo, la = np.array([lon_p]), np.array([lat_p])
+    z = np.array([37.5])
+
+    def weight(node):
+        w = np.zeros((K, m.n_nodes))
+        w[:, node] = 1.0
+        out, anchor = _sample(om, w, lo, la, "idw", z=z)
+        assert anchor[0] == n0
+        return out[0]
+
+    w0, ww, we, wn = weight(n0), weight(nw), weight(ne), weight(nn_)
+    assert min(w0, ww, we, wn) > 0
+    np.testing.assert_allclose(we / ww, (1.3 / 0.7) ** 2, rtol=1e-12)
+    np.testing.assert_allclose(w0 / we, (0.7 / 0.3) ** 2, rtol=1e-12)
+    z0 = np.cos(np.radians(lat_p)) * SLM
+    dx = 0.3 * STEP * z0
+    np.testing.assert_allclose(wn / w0, dx ** 2 / (dx ** 2 + (STEP * SLM) ** 2), rtol=1e-12)
+    # W = 2 z at every node, the western neighbour's bed at 12 m: it
+    # contributes 2 * 12, the others 2 * 37.5
+    lin = np.repeat((2.0 * zl)[:, None], m.n_nodes, axis=1)
+    shallow = depth.copy()
+    shallow[nw] = 12.0
+    ms = make_mesh("ifc", reg.nodes_lon, reg.nodes_lat, reg.elem_nodes, depth=shallow, z_levels=zl)
+    out, _ = _sample(OracleMesh(ms, "glibc"), lin, lo, la, "idw", z=z)
+    np.testing.assert_allclose(out[0], 2.0 * 37.5 - ww * 2.0 * (37.5 - 12.0), rtol=1e-12)
+    # 1e-6 deg of longitude east of the anchor (about 0.09 m): its own W,
+    # exactly; 1e-4 deg (about 9 m) blends the ring
+    vals = np.random.default_rng(12).normal(size=(K, m.n_nodes))
+    for off, own in ((1e-6, True), (1e-4, False)):
+        out, anchor = _sample(om, vals, np.array([m.nodes_lon[n0] + off]), np.array([m.nodes_lat[n0]]), "idw",
+                              z=np.array([10.0]))
+        assert anchor[0] == n0
+        assert (out[0] == vals[1, n0]) == own

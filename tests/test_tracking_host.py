@@ -105,7 +105,7 @@ def test_checkpoint_file_protocol(tmp_path):
     h = dict(lon=pid * 1.5, lat=pid * -2.0, z=pid + 0.25, status=np.int32(pid % 3 + 1),
              nn=np.int32(pid * 7), fr=np.int8(pid % 2), ws=np.int32(pid * 2), pid=pid,
              u=np.zeros(P), v=np.zeros(P), w=np.zeros(P))
-    path = str(tmp_path / "ck.npz")
+    path = str(tmp_path / "ck.nc")
     T.write_checkpoint(path, h, 11, 4, 5400.0, release_step=np.arange(P), particle_offset=100)
     r = T.load_restart(path)
     np.testing.assert_array_equal(r["state"].lon, np.arange(P) * 1.5)
@@ -114,13 +114,90 @@ def test_checkpoint_file_protocol(tmp_path):
     np.testing.assert_array_equal(r["state"].release_step, np.arange(P))
     assert (r["start_step_i"], r["start_time_idx"], r["sim_time_start"], r["particle_offset"]) == \
         (12, 4, 5400.0, 100)
-    assert not os.path.exists(path + ".tmp")
+    assert (r["step_i"], r["file_idx"]) == (11, 4)
+    assert sorted(os.listdir(tmp_path)) == ["ck.nc"]
     T.write_checkpoint(path, h, 3, 2, None)
     assert T.load_restart(path)["sim_time_start"] is None
-    bad = str(tmp_path / "bad.npz")
-    np.savez(bad, restart_lon=np.zeros(2))
+    bad = str(tmp_path / "bad.nc")
+    from scipy.io import netcdf_file
+    with netcdf_file(bad, "w", version=2) as nc:
+        nc.createDimension("trajectory", 2)
+        nc.createVariable("restart_lon", "f8", ("trajectory",))[:] = np.zeros(2)
     with pytest.raises(ValueError, match="No valid restart"):
         T.load_restart(bad)
+    with netcdf_file(bad, "w", version=2) as nc:
+        nc.createVariable("restart_valid", "i4", ())[...] = 0
+    with pytest.raises(ValueError, match="No valid restart"):
+        T.load_restart(bad)
+
+
+def test_restart_file_has_the_reference_contract(tmp_path):
+    """writers.py:581-664 / :1420: the variable names, types and shapes
+    the reference's load_restart reads, the flag 1, the optional arrays
+    listed in ``restart_optional``; a failed write keeps the earlier file."""
+    from scipy.io import netcdf_file
+    from kelp_amd.particles import ParticleState
+    from kelp_amd.writers import load_restart, write_restart_file
+    P = 4
+    st = ParticleState(lon=np.linspace(0, 3, P), lat=np.linspace(10, 13, P), z=np.full(P, 2.5),
+                       status=np.array([1, 2, 3, 4], np.int32), nearest_node=np.arange(P, dtype=np.int32),
+                       failure_reason=np.array([0, 1, 2, 0], np.int8), elem=np.arange(P, dtype=np.int32) + 9,
+                       release_lon=np.zeros(P), release_lat=np.ones(P), release_z=np.full(P, 5.0))
+    path = str(tmp_path / "restart_0007.nc")
+    write_restart_file(path, st, 7, 3, wet_streak=np.arange(P) + 1, rotation_angles=(10.0, 20.0, 30.0),
+                       release_step=np.array([0, 0, 2, 9]), particle_offset=40, uv_vertical="linear")
+    want = {"restart_lon": ("d", (P,)), "restart_lat": ("d", (P,)), "restart_z": ("d", (P,)),
+            "restart_status": ("i", (P,)), "restart_nearest_node": ("i", (P,)),
+            "restart_failure_reason": ("b", (P,)), "restart_release_step": ("i", (P,)),
+            "restart_elem": ("i", (P,)), "restart_release_lon": ("d", (P,)),
+            "restart_release_lat": ("d", (P,)), "restart_release_z": ("d", (P,)),
+            "restart_wet_streak": ("i", (P,)), "restart_step_i": ("i", ()), "restart_file_idx": ("i", ()),
+            "restart_valid": ("i", ())}
+    with open(path, "rb") as fh:
+        assert fh.read(3) == b"CDF"
+    with netcdf_file(path, "r", mmap=False) as nc:
+        got = {k: (v.typecode(), v.shape) for k, v in nc.variables.items()}
+        assert got == want
+        assert int(nc.variables["restart_valid"].getValue()) == 1
+        opt = nc.restart_optional.decode().split()
+        assert set(opt) == {k for k in want if k not in ("restart_lon", "restart_lat", "restart_z",
+                                                          "restart_status", "restart_nearest_node",
+                                                          "restart_step_i", "restart_file_idx",
+                                                          "restart_valid")}
+        assert nc.restart_uv_vertical == b"linear" and int(nc.restart_particle_offset) == 40
+    r = load_restart(path)
+    assert (r["step_i"], r["file_idx"], r["particle_offset"], r["uv_vertical"]) == (7, 3, 40, "linear")
+    assert r["rotation_angles"] == (10.0, 20.0, 30.0) and r["sim_time"] is None
+    np.testing.assert_array_equal(r["state"].elem, st.elem)
+    np.testing.assert_array_equal(r["state"].release_z, st.release_z)
+    np.testing.assert_array_equal(r["release_step"], [0, 0, 2, 9])
+    # a wrong length is refused and the valid file stays
+    with pytest.raises(ValueError, match="restart_wet_streak"):
+        write_restart_file(path, st, 8, 4, wet_streak=np.zeros(P + 1))
+    assert load_restart(path)["step_i"] == 7
+    assert sorted(os.listdir(tmp_path)) == ["restart_0007.nc"]
+
+
+def test_trajectory_file_keeps_the_checkpoint(tmp_path):
+    """A checkpoint inside a trajectory file (writers.py:567): the records
+    and the restart block in one file, both read back."""
+    from kelp_amd.writers import load_restart, read_trajectories, write_trajectories
+    P, Tn = 3, 2
+    arrays = dict(lon=np.ones((P, Tn)), lat=np.zeros((P, Tn)), z=np.zeros((P, Tn)),
+                  status=np.ones((P, Tn), np.int32), time=np.array([60.0, 120.0]))
+    h = dict(lon=np.arange(P) * 1.0, lat=np.zeros(P), z=np.zeros(P), status=np.ones(P, np.int32),
+             nn=np.arange(P, dtype=np.int32), fr=np.zeros(P, np.int8), ws=np.zeros(P, np.int32),
+             pid=np.arange(P))
+    path = str(tmp_path / "traj.nc")
+    write_trajectories(path, arrays, restart=T.checkpoint_payload(h, 5, 6, 330.0))
+    d = read_trajectories(path)
+    np.testing.assert_array_equal(d["lon"], arrays["lon"])
+    r = T.load_restart(path)
+    assert (r["start_step_i"], r["start_time_idx"], r["sim_time_start"]) == (6, 6, 330.0)
+    np.testing.assert_array_equal(r["state"].lon, h["lon"])
+    write_trajectories(path, arrays)
+    with pytest.raises(ValueError, match="No valid restart"):
+        load_restart(path)
 
 
 def test_record_clock_origin_of_a_resumed_run():
