@@ -312,6 +312,32 @@ int kelp_scalar_upload(kelp_ctx *ctx, int slot, int var, int64_t n_planes, const
 int kelp_sample(kelp_ctx *ctx, const kelp_particles *p, int var, int slot_a, int slot_b, double alpha,
                 int method, double *out);
 
+/* Device rows of one staged record (kelp_record_stage): n_rows entries each
+ * in output particle order; w may be NULL (2-D); samples: n_vars x n_rows
+ * doubles, or NULL without sampled scalars. */
+typedef struct {
+    double *lon, *lat, *z, *u, *v, *w;
+    int32_t *status;
+    int8_t *fr;
+    double *samples;
+} kelp_record_rows;
+
+/* Record staging: the record append of tracking.py:2757-2781 on the
+ * device, for a tracking loop that keeps its records in HBM and copies
+ * them out in batches.  Every particle's record goes to row pid - pid_offset
+ * of `out`, masked as core/release.py:132 mask_dormant_for_record (lon,
+ * lat, z NaN where NOT_RELEASED) and :170 mask_inactive_velocity (u, v, w
+ * NaN unless ALIVE both at the step start -- prev_status, device int32 per
+ * particle slot, from kelp_activate -- and at its end), the velocities
+ * negated when time_dir < 0 (tracking.py:2772-2777); status and failure
+ * reason as they are; the n_vars sampled scalars (device doubles, n_vars x
+ * n in particle-slot order, as kelp_sample_vars writes them) scattered the
+ * same way.  Particles whose pid falls outside [pid_offset, pid_offset +
+ * n_rows) are skipped.  Enqueued on the context stream; no wait. */
+int kelp_record_stage(kelp_ctx *ctx, const kelp_particles *p, const int32_t *prev_status, int64_t pid_offset,
+                      int64_t n_rows, double time_dir, int n_vars, const double *sampled,
+                      const kelp_record_rows *out);
+
 /* Release (core/release.py:112 activate_released): status NOT_RELEASED (4)
  * becomes ALIVE where step >= release_step[pid - pid_offset] (device int64,
  * n_release entries; NULL = no release schedule); a released particle's nn
@@ -406,8 +432,9 @@ int kelp_event_elapsed(kelp_ctx *ctx, int a, int b, float *ms);
  * arrays (fn: 0 sin, 1 cos, 2 asin, 3 acos, 4 atan2(x, y), 5 log, 6 exp,
  * 7 Python x % 360, 8 sqrt, 9 Python x % y, 10 reflective fold of x into
  * [0, y], 11 / 12 the first / second normal of the random walk's pair for
- * key (seed 20240501, realization 0, particle x, step y, component 0)).  No reference counterpart; used to check the
- * device compiler against a host build of the same routines. */
+ * key (seed 20240501, realization 0, particle x, step y, component 0),
+ * 13 x^y).  No reference counterpart; used to check the device compiler
+ * against a host build of the same routines. */
 int kelp_math_probe(kelp_ctx *ctx, int fn, int64_t n, const double *x, const double *y,
                     double *out);
 

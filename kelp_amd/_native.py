@@ -96,6 +96,14 @@ class BatchAxis(ctypes.Structure):
     ]
 
 
+class RecordRows(ctypes.Structure):
+    """kelp_record_rows: device rows of one staged record."""
+    _fields_ = [
+        ("lon", c_void_p), ("lat", c_void_p), ("z", c_void_p), ("u", c_void_p), ("v", c_void_p),
+        ("w", c_void_p), ("status", c_void_p), ("fr", c_void_p), ("samples", c_void_p),
+    ]
+
+
 CLOUD_SHIFT = 40
 CLOUD_VALID, CLOUD_3D, CLOUD_KICK = 1, 2, 4
 MAX_CLOUDS = 65536
@@ -127,6 +135,8 @@ SIGNATURES = {
                             c_void_p]),
     "kelp_activate": (c_int, [c_void_p, POINTER(Particles), c_void_p, c_int64, c_int64, c_int64,
                               c_void_p]),
+    "kelp_record_stage": (c_int, [c_void_p, POINTER(Particles), c_void_p, c_int64, c_int64, c_double, c_int,
+                                  c_void_p, POINTER(RecordRows)]),
     "kelp_field_upload_async": (c_int, [c_void_p, c_int, c_int, c_int, c_int64, c_void_p, c_void_p,
                                         c_void_p, c_int64, c_void_p]),
     "kelp_field_upload_wait": (c_int, [c_void_p, c_int]),

@@ -73,6 +73,32 @@ def cr_tables():
     print("cos tail", [h(mp.mpf((-1) ** k) / mp.factorial(2 * k)) for k in range(2, 6)])
 
 
+def cr_tables2():
+    """The tables of the double-double atan and exp of kmath.h: atan(j/64)
+    as hi + lo for j = 0..64, 2^(j/64) as hi + lo for j = 0..63, ln2/64 in
+    three parts (36 + 36 bits + the rest) and 64/ln2."""
+    ah, al = [], []
+    for j in range(65):
+        a = mp.atan(mp.mpf(j) / 64)
+        ah.append(h(a)), al.append(h(a - float(a)))
+    print("static const double KM_ATANT_H[65] = {" + ", ".join(ah) + "};")
+    print("static const double KM_ATANT_L[65] = {" + ", ".join(al) + "};")
+    eh, el = [], []
+    for j in range(64):
+        e = mp.mpf(2) ** (mp.mpf(j) / 64)
+        eh.append(h(e)), el.append(h(e - float(e)))
+    print("static const double KM_EXPT_H[64] = {" + ", ".join(eh) + "};")
+    print("static const double KM_EXPT_L[64] = {" + ", ".join(el) + "};")
+    l = mp.log(2) / 64
+    l1 = rnd(l, 36)
+    l2 = rnd(l - l1, 36)
+    l3 = l - l1 - l2
+    print("KM_LN2_64_1", h(l1), "KM_LN2_64_2", h(l2), "KM_LN2_64_3", h(l3), "KM_64_OVER_LN2", h(64 / mp.log(2)))
+    print("atan dd tail", [h(mp.mpf((-1) ** k) / (2 * k + 1)) for k in range(2, 7)])
+    print("exp tail", [h(1 / mp.factorial(k)) for k in range(4, 10)])
+
+
 if __name__ == "__main__":
     main()
     cr_tables()
+    cr_tables2()

@@ -413,6 +413,7 @@ __global__ void math_probe_kernel(int fn, int64_t n, const double* __restrict__ 
     case 7: r = km_pymod360(a); break;
     case 9: r = km_pymod_pos(a, y[i]); break;
     case 10: r = km_reflect_depth(a, 0.0, y[i]); break;
+    case 13: r = km_pow(a, y[i]); break;
     case 11:
     case 12: {
         // the random walk's normal pair (numerics/diffusion.py:234) of key
@@ -1015,7 +1016,67 @@ int kelp_activate(kelp_ctx* c, const kelp_particles* p, const int64_t* release_s
     return KELP_OK;
 }
 
-// Pack one snapshot's planes (device, mesh order) into the slot's records on
+// One staged record: particle slot i to row pid - pid_offset, masked as the
+// host record (core/release.py:132, :170; the backward flip after masking,
+// so a masked velocity of a backward run is -NaN there too).
+__global__ void record_stage_kernel(int64_t n, const double* __restrict__ lon, const double* __restrict__ lat,
+                                    const double* __restrict__ z, const double* __restrict__ u,
+                                    const double* __restrict__ v, const double* __restrict__ w,
+                                    const int32_t* __restrict__ status, const int8_t* __restrict__ fr,
+                                    const int64_t* __restrict__ pid, const int32_t* __restrict__ prev,
+                                    int64_t pid_offset, int64_t n_rows, double time_dir, int n_vars,
+                                    const double* __restrict__ sampled, kelp_record_rows out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t r = pid[i] - pid_offset;
+    if (r < 0 || r >= n_rows) return;
+    const int32_t st = status[i];
+    const bool dormant = st == 4;
+    const bool bad = st != 1 || prev[i] != 1;
+    const double nan = __builtin_nan("");
+    out.lon[r] = dormant ? nan : lon[i];
+    out.lat[r] = dormant ? nan : lat[i];
+    out.z[r] = dormant ? nan : (z ? z[i] : 0.0);
+    double uu = (bad || dormant) ? nan : u[i];
+    double vv = (bad || dormant) ? nan : v[i];
+    double ww = (bad || dormant) ? nan : (w ? w[i] : 0.0);
+    if (time_dir < 0.0) {
+        uu = -uu;
+        vv = -vv;
+        ww = -ww;
+    }
+    out.u[r] = uu;
+    out.v[r] = vv;
+    if (out.w) out.w[r] = ww;
+    out.status[r] = st;
+    out.fr[r] = fr[i];
+    for (int k = 0; k < n_vars; ++k) out.samples[k * n_rows + r] = sampled[k * n + i];
+}
+
+int kelp_record_stage(kelp_ctx* c, const kelp_particles* p, const int32_t* prev_status, int64_t pid_offset,
+                      int64_t n_rows, double time_dir, int n_vars, const double* sampled,
+                      const kelp_record_rows* out)
+{
+    if (!c || !p || !out) return fail(KELP_EINVAL, "kelp_record_stage: NULL argument");
+    if (p->n < 0 || n_rows < 0 || n_vars < 0) return fail(KELP_EINVAL, "kelp_record_stage: negative size");
+    if (!p->lon || !p->lat || !p->status || !p->fr || !p->pid || !p->u || !p->v || !prev_status)
+        return fail(KELP_EINVAL, "kelp_record_stage: particles need lon, lat, status, fr, pid, u, v and prev_status");
+    if (!out->lon || !out->lat || !out->z || !out->u || !out->v || !out->status || !out->fr)
+        return fail(KELP_EINVAL, "kelp_record_stage: missing output row");
+    if (out->w && !p->w) return fail(KELP_EINVAL, "kelp_record_stage: a w row needs particles with w");
+    if (n_vars > 0 && (!sampled || !out->samples))
+        return fail(KELP_EINVAL, "kelp_record_stage: sampled scalars need `sampled` and out->samples");
+    if (p->n == 0 || n_rows == 0) return KELP_OK;
+    HIP_TRY(hipSetDevice(c->device));
+    record_stage_kernel<<<grid_for(p->n, 256), 256, 0, c->stream>>>(
+        p->n, p->lon, p->lat, p->z, p->u, p->v, p->w, p->status, p->fr, p->pid, prev_status, pid_offset, n_rows,
+        time_dir, n_vars, sampled, *out);
+    HIP_TRY(hipGetLastError());
+    return KELP_OK;
+}
+
+// Pack one snapshot's planes (device, mesh order) into the slot's records on// Pack one snapshot's planes (device, mesh order) into the slot's records on
 // `stream` (pack3d/pack2d for node records, pack_elem for element u/v).
 static hipError_t launch_pack(kelp_ctx* c, hipStream_t stream, FieldSlot& s, int dtype, int layout, bool is3d,
                               int64_t n_planes, const void* tu, const void* tv, const void* tw)

@@ -666,7 +666,7 @@ __device__ __noinline__ void rk45_det(const DMesh& m, const Snap<T, false>& fe, 
             }
             double fac;
             if (err > 0.0) {
-                fac = 0.9 * km_exp(0.2 * km_log(tol / err));
+                fac = 0.9 * km_pow(tol / err, 0.2);
                 if (fac < 0.2)
                     fac = 0.2;
                 else if (fac > 5.0)

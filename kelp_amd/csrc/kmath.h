@@ -17,11 +17,15 @@
  *            relative error before the final rounding below 2^-80, so the
  *            result is the correctly rounded sine / cosine except when the
  *            exact value lies within that distance of a rounding boundary.
- *   atan     |x| < 3/16: Taylor series; 3/16 <= |x| <= 2: table of
- *            atan(j/8) (hi+lo) and atan(x) = atan(c) + atan((x-c)/(1+xc)),
- *            |t| <= 1/16; |x| > 2: pi/2 - atan(1/x).
- *   atan2    C99 special cases, quadrant fix with pi as hi+lo.
- *   asin/acos  through atan2 and sqrt((1-x)(1+x)).
+ *   atan2    C99 special cases; the ratio of the smaller to the larger
+ *            argument as a double-double, j = rint(64 t), atan(j/64) as a
+ *            double-double from the table and atan u of the reduced
+ *            u = (t - j/64) / (1 + t j/64), |u| <= 1/128, by its series
+ *            with the leading terms in double-double; pi/2 - and pi -
+ *            in double-double: relative error before the final rounding
+ *            below 2^-84.  atan(x) = atan2(x, 1).
+ *   asin/acos  atan2 of x and sqrt(1 - x^2), the root formed as a
+ *            double-double from (1 - x)(1 + x), one rounding at the end.
  *   log      x = 2^k m, m in [sqrt(1/2), sqrt(2)), c_i = RN(1/(1 + i/128))
  *            for the nearest i, r = m c_i - 1 exact as a double-double
  *            (|r| <= 2^-7.5), log x = k ln2 - log(c_i) + log1p(r) with
@@ -29,12 +33,19 @@
  *            to r^3 in double-double, the rest in double: relative error
  *            before the final rounding below 2^-76 (near x = 1, where
  *            c_i = 1, relative to log1p(r) itself).
- *   exp      x = k ln2 + r (ln2 split 42+53 bits), Taylor series on
- *            |r| <= ln2/2, scaled by 2^k.
+ *   exp      x = (64 k + j) ln2/64 + r (ln2/64 split 36+36+53 bits),
+ *            2^(j/64) as a double-double from the table, exp(r) - 1 by its
+ *            series with the terms to r^3 in double-double: relative
+ *            error before the final rounding below 2^-85.
+ *   pow      x > 0: exp of y log x with log x and the product in
+ *            double-double.
+ * Each is therefore the correctly rounded value except when the exact
+ * value lies within its error bound of a rounding boundary.
  * Coefficients are the round-to-nearest binary64 values of the exact
- * rationals (-1)^k/(2k+1)!, 1/k!, 2/(2k+1), 1/k; the tables (atan(j/8),
- * log(c_i), sin/cos(j/64)) and the constants pi/2 and ln2 come from
- * kelp_amd/csrc/gen_kmath_tables.py (200-bit mpmath).
+ * rationals (-1)^k/(2k+1)!, 1/k!, (-1)^k/(2k+1), 1/k; the tables
+ * (atan(j/64), 2^(j/64), log(c_i), sin/cos(j/64)) and the constants pi/2,
+ * ln2 and ln2/64 come from kelp_amd/csrc/gen_kmath_tables.py (200-bit
+ * mpmath).
  * Accuracy against glibc is measured in tests/test_oracle_pins.py.
  *
  * Valid as C99 (the oracle's diagnostic build) and as HIP C++.
@@ -505,80 +516,130 @@ KM_FN void km_sincos(double x, double *s, double *c)
 
 /* ------------------------------------------------------------------ atan */
 
-KM_FN double km_atan_series_small(double x) /* |x| < 3/16 */
+static const double KM_ATANT_H[65] = {
+    0x0.0p+0, 0x1.fff555bbb729bp-7, 0x1.ffd55bba97625p-6,
+    0x1.7fb818430da2ap-5, 0x1.ff55bb72cfdeap-5, 0x1.3f59f0e7c559dp-4,
+    0x1.7ee182602f10fp-4, 0x1.be39ebe6f07c3p-4, 0x1.fd5ba9aac2f6ep-4,
+    0x1.1e1fafb043727p-3, 0x1.3d6eee8c6626cp-3, 0x1.5c9811e3ec26ap-3,
+    0x1.7b97b4bce5b02p-3, 0x1.9a6a8e96c8626p-3, 0x1.b90d7529260a2p-3,
+    0x1.d77d5df205736p-3, 0x1.f5b75f92c80ddp-3, 0x1.09dc597d86362p-2,
+    0x1.18bf5a30bf178p-2, 0x1.278372057ef46p-2, 0x1.362773707ebccp-2,
+    0x1.44aa436c2af0ap-2, 0x1.530ad9951cd4ap-2, 0x1.614840309cfe2p-2,
+    0x1.6f61941e4def1p-2, 0x1.7d5604b63b3f7p-2, 0x1.8b24d394a1b25p-2,
+    0x1.98cd5454d6b18p-2, 0x1.a64eec3cc23fdp-2, 0x1.b3a911da65c6cp-2,
+    0x1.c0db4c94ec9f0p-2, 0x1.cde53432c1351p-2, 0x1.dac670561bb4fp-2,
+    0x1.e77eb7f175a34p-2, 0x1.f40dd0b541418p-2, 0x1.0039c73c1a40cp-1,
+    0x1.0657e94db30d0p-1, 0x1.0c6145b5b43dap-1, 0x1.1255d9bfbd2a9p-1,
+    0x1.1835a88be7c13p-1, 0x1.1e00babdefeb4p-1, 0x1.23b71e2cc9e6ap-1,
+    0x1.2958e59308e31p-1, 0x1.2ee628406cbcap-1, 0x1.345f01cce37bbp-1,
+    0x1.39c391cd4171ap-1, 0x1.3f13fb89e96f4p-1, 0x1.445065b795b56p-1,
+    0x1.4978fa3269ee1p-1, 0x1.4e8de5bb6ec04p-1, 0x1.538f57b89061fp-1,
+    0x1.587d81f732fbbp-1, 0x1.5d58987169b18p-1, 0x1.6220d115d7b8ep-1,
+    0x1.66d663923e087p-1, 0x1.6b798920b3d99p-1, 0x1.700a7c5784634p-1,
+    0x1.748978fba8e0fp-1, 0x1.78f6bbd5d315ep-1, 0x1.7d528289fa093p-1,
+    0x1.819d0b7158a4dp-1, 0x1.85d69576cc2c5p-1, 0x1.89ff5ff57f1f8p-1,
+    0x1.8e17aa99cc05ep-1, 0x1.921fb54442d18p-1
+};
+static const double KM_ATANT_L[65] = {
+    0x0.0p+0, -0x1.220c39d4dff50p-61, -0x1.5ec431444912cp-60,
+    -0x1.86ef8f794f105p-63, -0x1.c934d86d23f1dp-60, 0x1.ac4ce285df847p-58,
+    -0x1.cfb654c0c3d98p-58, 0x1.f7b8f29a05987p-58, -0x1.cd37686760c17p-59,
+    -0x1.b485914dacf8cp-59, 0x1.61a3b0ce9281bp-57, -0x1.054ab2c010f3dp-58,
+    0x1.347b0b4f881cap-58, 0x1.cf601e7b4348ep-59, 0x1.17b10d2e0e5abp-61,
+    0x1.c648d1534597ep-57, 0x1.8ab6e3cf7afbdp-57, 0x1.62e47390cb865p-56,
+    0x1.30ca4748b1bf9p-57, -0x1.077cdd36dfc81p-56, -0x1.963a544b672d8p-57,
+    -0x1.5d5e43c55b3bap-56, -0x1.2566480884082p-57, -0x1.a725715711f00p-56,
+    -0x1.c63aae6f6e918p-56, 0x1.69c885c2b249ap-56, 0x1.b6d0ba3748fa8p-56,
+    0x1.9e6c988fd0a77p-56, -0x1.24dec1b50b7ffp-56, 0x1.ae187b1ca5040p-56,
+    -0x1.cc1ce70934c34p-56, -0x1.a2cfa4418f1adp-56, 0x1.a2b7f222f65e2p-56,
+    0x1.0e53dc1bf3435p-56, -0x1.a3992dc382a23p-57, -0x1.b32c949c9d593p-55,
+    -0x1.d5b495f6349e6p-56, 0x1.974fa13b5404fp-58, -0x1.2bdaee1c0ee35p-58,
+    0x1.c621cec00c301p-55, -0x1.928df287a668fp-58, 0x1.c421c9f38224ep-57,
+    -0x1.09e73b0c6c087p-56, 0x1.c5d5e9ff0cf8dp-55, 0x1.1021137c71102p-55,
+    -0x1.2304331d8bf46p-55, 0x1.ecf8b492644f0p-56, -0x1.f76d0163f79c8p-56,
+    0x1.2419a87f2a458p-56, 0x1.4a33dbeb3796cp-55, -0x1.1bb74abda520cp-55,
+    -0x1.5e5c9d8c5a950p-56, 0x1.0028e4bc5e7cap-57, -0x1.2b785350ee8c1p-57,
+    -0x1.6ea6febe8bbbap-56, -0x1.a80386188c50ep-55, -0x1.8c34d25aadef6p-56,
+    0x1.7b2a6165884a1p-59, 0x1.406a089803740p-55, 0x1.560821e2f3aa9p-55,
+    -0x1.bf76229d3b917p-56, 0x1.6b66e7fc8b8c3p-57, -0x1.55b9a5e177a1bp-55,
+    -0x1.ec182ab042f61p-56, 0x1.1a62633145c07p-55
+};
+
+KM_FN void km_dd_div(double ah, double al, double bh, double bl, double *qh, double *ql)
 {
-    double z = x * x;
-    double p = -0x1.5555555555555p-2 +
-               z * (0x1.999999999999ap-3 +
-               z * (-0x1.2492492492492p-3 +
-               z * (0x1.c71c71c71c71cp-4 +
-               z * (-0x1.745d1745d1746p-4 +
-               z * (0x1.3b13b13b13b14p-4 +
-               z * (-0x1.1111111111111p-4 +
-               z * (0x1.e1e1e1e1e1e1ep-5 +
-               z * (-0x1.af286bca1af28p-5 +
-               z * (0x1.8618618618618p-5 +
-               z * (-0x1.642c8590b2164p-5 +
-               z * (0x1.47ae147ae147bp-5 +
-               z * -0x1.2f684bda12f68p-5)))))))))));
-    return x + (x * z) * p;
+    /* (ah + al) / (bh + bl): q1 = ah / bh, the remainder (a - q1 b) formed
+     * exactly to first order (ah - q1 bh is exact: q1 bh is within a
+     * factor 2 of ah), q2 = remainder / bh */
+    double q1 = ah / bh;
+    double p = q1 * bh;
+    double pe = km_two_prod_err(q1, bh, p);
+    double r = (((ah - p) - pe) + al) - q1 * bl;
+    km_fast_two_sum(q1, r / bh, qh, ql);
 }
 
-KM_FN double km_atan_series_tiny(double t) /* |t| <= 1/16 */
+KM_FN void km_dd_sqrt(double ah, double al, double *sh, double *sl)
 {
-    double z = t * t;
-    double p = -0x1.5555555555555p-2 +
-               z * (0x1.999999999999ap-3 +
-               z * (-0x1.2492492492492p-3 +
-               z * (0x1.c71c71c71c71cp-4 +
-               z * (-0x1.745d1745d1746p-4 +
-               z * (0x1.3b13b13b13b14p-4 +
-               z * (-0x1.1111111111111p-4 +
-               z * 0x1.e1e1e1e1e1e1ep-5))))));
-    return t + (t * z) * p;
+    /* sqrt(ah + al), ah > 0: s = sqrt(ah), then one Newton correction
+     * (a - s^2) / (2 s) with s^2 formed exactly */
+    double s = km_sqrt(ah);
+    double p = s * s;
+    double pe = km_two_prod_err(s, s, p);
+    double r = ((ah - p) - pe) + al;
+    km_fast_two_sum(s, r / (2.0 * s), sh, sl);
 }
 
-/* atan(a) for 0 <= a <= 2 */
-KM_FN double km_atan_0_2(double a)
+/* atan of t = th + tl, 0 <= t <= 1 (+ a rounding), as a double-double:
+ * j = rint(64 t), c = j/64, atan t = atan c + atan u with
+ * u = (t - c) / (1 + t c), |u| <= 1/128 (+), and
+ *   atan u = u + u^3 (-1/3 + u^2 (1/5 - u^2/7 + u^4/9 - u^6/11 + u^8/13)),
+ * u, u^2, u^3 and the -1/3 in double-double, the bracket in double:
+ * relative error before the final rounding below 2^-84. */
+KM_FN void km_atan01_dd(double th, double tl, double *rh, double *rl)
 {
-    if (a < 0.1875) return km_atan_series_small(a);
-    int j = (int)(a * 8.0 + 0.5); /* 2..16 */
-    double c = (double)j * 0.125;
-    double t = (a - c) / (1.0 + a * c);
-    double hi, lo;
-    switch (j) {
-    case 2: hi = 0x1.f5b75f92c80ddp-3; lo = 0x1.8ab6e3cf7afbdp-57; break;
-    case 3: hi = 0x1.6f61941e4def1p-2; lo = -0x1.c63aae6f6e918p-56; break;
-    case 4: hi = 0x1.dac670561bb4fp-2; lo = 0x1.a2b7f222f65e2p-56; break;
-    case 5: hi = 0x1.1e00babdefeb4p-1; lo = -0x1.928df287a668fp-58; break;
-    case 6: hi = 0x1.4978fa3269ee1p-1; lo = 0x1.2419a87f2a458p-56; break;
-    case 7: hi = 0x1.700a7c5784634p-1; lo = -0x1.8c34d25aadef6p-56; break;
-    case 8: hi = 0x1.921fb54442d18p-1; lo = 0x1.1a62633145c07p-55; break;
-    case 9: hi = 0x1.b034f38649c88p-1; lo = -0x1.be88d6936f833p-55; break;
-    case 10: hi = 0x1.cac7c57846f9ep-1; lo = 0x1.0dae13ad18a6bp-55; break;
-    case 11: hi = 0x1.e24dd44c855d1p-1; lo = 0x1.f7ac612ab33d8p-55; break;
-    case 12: hi = 0x1.f730bd281f69bp-1; lo = 0x1.007887af0cbbdp-56; break;
-    case 13: hi = 0x1.04e67277a01d7p+0; lo = 0x1.7115496c13eb6p-57; break;
-    case 14: hi = 0x1.0d38f2c5ba09fp+0; lo = -0x1.bd0dc231bfd70p-54; break;
-    case 15: hi = 0x1.14b1dd5f90ce1p+0; lo = -0x1.212d570a63fa2p-56; break;
-    default: hi = 0x1.1b6e192ebbe44p+0; lo = 0x1.b1b466a88828ep-54; break;
-    }
-    return hi + (lo + km_atan_series_tiny(t));
+    int j = (int)km_rint_small(th * 64.0);
+    if (j > 64) j = 64;
+    double c = (double)j * 0.015625;
+    double nh, nl;
+    km_two_sum(th - c, tl, &nh, &nl); /* th - c exact (Sterbenz, or c = 0) */
+    double ph = th * c;
+    double pl = km_two_prod_err(th, c, ph) + tl * c;
+    double dh, dl;
+    km_two_sum(1.0, ph, &dh, &dl);
+    km_fast_two_sum(dh, dl + pl, &dh, &dl);
+    double uh, ul;
+    km_dd_div(nh, nl, dh, dl, &uh, &ul);
+    double zh = uh * uh;
+    double zl = km_two_prod_err(uh, uh, zh) + 2.0 * uh * ul;
+    double u3h, u3l;
+    km_dd_mul(uh, ul, zh, zl, &u3h, &u3l);
+    double q = zh * (0x1.999999999999ap-3 +
+               zh * (-0x1.2492492492492p-3 +
+               zh * (0x1.c71c71c71c71cp-4 +
+               zh * (-0x1.745d1745d1746p-4 +
+               zh * 0x1.3b13b13b13b14p-4))));
+    double Ph, Pl, ch, cl, ah, al;
+    km_dd_add(-KM_THIRD_H, -KM_THIRD_L, q, 0.0, &Ph, &Pl);
+    km_dd_mul(u3h, u3l, Ph, Pl, &ch, &cl);
+    km_dd_add(uh, ul, ch, cl, &ah, &al);
+    km_dd_add(KM_ATANT_H[j], KM_ATANT_L[j], ah, al, rh, rl);
 }
 
-KM_FN double km_atan(double x)
+/* atan2 of y = yh + yl > 0 and x = xh + xl > 0 as a double-double (both of
+ * moderate range: their products and squares neither overflow nor
+ * underflow): the smaller over the larger, atan01, pi/2 minus it when
+ * y > x. */
+KM_FN void km_atan2_pos_dd(double yh, double yl, double xh, double xl, double *rh, double *rl)
 {
-    if (km_isnan(x)) return x;
-    double a = km_fabs(x);
-    double r;
-    if (a <= 2.0) {
-        r = km_atan_0_2(a);
-    } else if (a < 0x1p60) {
-        r = KM_PIO2_HI - (km_atan_0_2(1.0 / a) - KM_PIO2_LO);
-    } else {
-        r = KM_PIO2_HI;
-    }
-    return km_copysign(r, x);
+    int swap = yh > xh;
+    double th, tl, ah, al;
+    if (swap)
+        km_dd_div(xh, xl, yh, yl, &th, &tl);
+    else
+        km_dd_div(yh, yl, xh, xl, &th, &tl);
+    km_atan01_dd(th, tl, &ah, &al);
+    if (swap) km_dd_add(KM_PIO2_HI, KM_PIO2_LO, -ah, -al, &ah, &al);
+    *rh = ah;
+    *rl = al;
 }
 
 KM_FN double km_atan2(double y, double x)
@@ -598,35 +659,88 @@ KM_FN double km_atan2(double y, double x)
         return __builtin_signbit(x) ? km_copysign(KM_PI_HI, y) : km_copysign(0.0, y);
     }
     if (ay == inf) return km_copysign(KM_PIO2_HI, y);
-    double q = ay / ax;
-    double a = km_atan(q); /* q may be inf (-> pi/2) or 0 */
-    if (__builtin_signbit(x)) a = KM_PI_HI - (a - KM_PI_LO);
-    return km_copysign(a, y);
+    double rh, rl;
+    if (ay < ax * 0x1p-60 || ax < ay * 0x1p-60) {
+        /* one far smaller: atan of a tiny ratio t is t (t^3/3 is below
+         * half an ulp of t), or pi/2 - t */
+        if (ay < ax) {
+            rh = ay / ax;
+            rl = 0.0;
+        } else {
+            km_fast_two_sum(KM_PIO2_HI, KM_PIO2_LO - ax / ay, &rh, &rl);
+        }
+    } else {
+        /* both scaled by powers of two until the larger is in [1, 2) (the
+         * smaller then in [2^-61, 2)): exact */
+        double big = ax > ay ? ax : ay;
+        int e = (int)((km_bits(big) >> 52) & 0x7ff) - 1023;
+        if (e > 1000) {
+            ax *= 0x1p-600;
+            ay *= 0x1p-600;
+            e -= 600;
+        } else if (e < -1000) {
+            ax *= 0x1p600;
+            ay *= 0x1p600;
+            big *= 0x1p600;
+            e = (int)((km_bits(big) >> 52) & 0x7ff) - 1023;
+        }
+        double sc = km_from_bits((uint64_t)(1023 - e) << 52);
+        km_atan2_pos_dd(ay * sc, 0.0, ax * sc, 0.0, &rh, &rl);
+    }
+    if (__builtin_signbit(x)) km_dd_add(KM_PI_HI, KM_PI_LO, -rh, -rl, &rh, &rl);
+    return km_copysign(rh + rl, y);
 }
 
+KM_FN double km_atan(double x)
+{
+    if (km_isnan(x)) return x;
+    return km_atan2(x, 1.0);
+}
+
+/* sqrt(1 - a^2), 0 <= a < 1, as a double-double: (1 - a)(1 + a) formed
+ * exactly to second order, then km_dd_sqrt */
+KM_FN void km_cosasin_dd(double a, double *sh, double *sl)
+{
+    double oh, ol, ph, pl, wh, wl;
+    km_two_sum(1.0, -a, &oh, &ol);
+    km_two_sum(1.0, a, &ph, &pl);
+    km_dd_mul(oh, ol, ph, pl, &wh, &wl);
+    km_dd_sqrt(wh, wl, sh, sl);
+}
+
+/* asin x = atan2(x, sqrt(1 - x^2)) and acos x = atan2(sqrt(1 - x^2), x)
+ * in double-double, rounded once */
 KM_FN double km_asin(double x)
 {
     double a = km_fabs(x);
     if (!(a <= 1.0)) return (x - x) / (x - x); /* NaN for |x| > 1 or NaN */
-    if (a < 0x1p-27) return x;
-    return km_atan2(x, km_sqrt((1.0 - x) * (1.0 + x)));
+    if (a < 0x1p-26) return x; /* x^3/6 is below half an ulp of x */
+    if (a == 1.0) return km_copysign(KM_PIO2_HI, x);
+    double sh, sl, rh, rl;
+    km_cosasin_dd(a, &sh, &sl);
+    km_atan2_pos_dd(a, 0.0, sh, sl, &rh, &rl);
+    return km_copysign(rh + rl, x);
 }
 
 KM_FN double km_acos(double x)
 {
     double a = km_fabs(x);
     if (!(a <= 1.0)) return (x - x) / (x - x);
-    return km_atan2(km_sqrt((1.0 - x) * (1.0 + x)), x);
+    if (x == 1.0) return 0.0;
+    if (x == -1.0) return KM_PI_HI;
+    if (a < 0x1p-60) return KM_PIO2_HI;
+    double sh, sl, rh, rl;
+    km_cosasin_dd(a, &sh, &sl);
+    km_atan2_pos_dd(sh, sl, a, 0.0, &rh, &rl);
+    if (x < 0.0) km_dd_add(KM_PI_HI, KM_PI_LO, -rh, -rl, &rh, &rl);
+    return rh + rl;
 }
 
 /* ------------------------------------------------------------------- log */
 
-KM_FN double km_log(double x)
+/* log x, x finite and > 0, as a double-double */
+KM_FN void km_log_dd(double x, double *out_h, double *out_l)
 {
-    if (km_isnan(x)) return x;
-    if (x < 0.0) return (x - x) / (x - x);
-    if (x == 0.0) return -__builtin_inf();
-    if (x == __builtin_inf()) return x;
     int k = 0;
     if (x < 0x1p-1022) { /* subnormal */
         x *= 0x1p54;
@@ -675,45 +789,159 @@ KM_FN double km_log(double x)
         km_fast_two_sum(dk * KM_LN2_HI, kl, &kh, &khl);
         km_dd_add(kh, khl + kle, sh, sl, &sh, &sl);
     }
-    return sh + sl;
+    *out_h = sh;
+    *out_l = sl;
+}
+
+KM_FN double km_log(double x)
+{
+    if (km_isnan(x)) return x;
+    if (x < 0.0) return (x - x) / (x - x);
+    if (x == 0.0) return -__builtin_inf();
+    if (x == __builtin_inf()) return x;
+    double h, l;
+    km_log_dd(x, &h, &l);
+    return h + l;
 }
 
 /* ------------------------------------------------------------------- exp */
+
+static const double KM_EXPT_H[64] = {
+    0x1.0000000000000p+0, 0x1.02c9a3e778061p+0, 0x1.059b0d3158574p+0,
+    0x1.0874518759bc8p+0, 0x1.0b5586cf9890fp+0, 0x1.0e3ec32d3d1a2p+0,
+    0x1.11301d0125b51p+0, 0x1.1429aaea92de0p+0, 0x1.172b83c7d517bp+0,
+    0x1.1a35beb6fcb75p+0, 0x1.1d4873168b9aap+0, 0x1.2063b88628cd6p+0,
+    0x1.2387a6e756238p+0, 0x1.26b4565e27cddp+0, 0x1.29e9df51fdee1p+0,
+    0x1.2d285a6e4030bp+0, 0x1.306fe0a31b715p+0, 0x1.33c08b26416ffp+0,
+    0x1.371a7373aa9cbp+0, 0x1.3a7db34e59ff7p+0, 0x1.3dea64c123422p+0,
+    0x1.4160a21f72e2ap+0, 0x1.44e086061892dp+0, 0x1.486a2b5c13cd0p+0,
+    0x1.4bfdad5362a27p+0, 0x1.4f9b2769d2ca7p+0, 0x1.5342b569d4f82p+0,
+    0x1.56f4736b527dap+0, 0x1.5ab07dd485429p+0, 0x1.5e76f15ad2148p+0,
+    0x1.6247eb03a5585p+0, 0x1.6623882552225p+0, 0x1.6a09e667f3bcdp+0,
+    0x1.6dfb23c651a2fp+0, 0x1.71f75e8ec5f74p+0, 0x1.75feb564267c9p+0,
+    0x1.7a11473eb0187p+0, 0x1.7e2f336cf4e62p+0, 0x1.82589994cce13p+0,
+    0x1.868d99b4492edp+0, 0x1.8ace5422aa0dbp+0, 0x1.8f1ae99157736p+0,
+    0x1.93737b0cdc5e5p+0, 0x1.97d829fde4e50p+0, 0x1.9c49182a3f090p+0,
+    0x1.a0c667b5de565p+0, 0x1.a5503b23e255dp+0, 0x1.a9e6b5579fdbfp+0,
+    0x1.ae89f995ad3adp+0, 0x1.b33a2b84f15fbp+0, 0x1.b7f76f2fb5e47p+0,
+    0x1.bcc1e904bc1d2p+0, 0x1.c199bdd85529cp+0, 0x1.c67f12e57d14bp+0,
+    0x1.cb720dcef9069p+0, 0x1.d072d4a07897cp+0, 0x1.d5818dcfba487p+0,
+    0x1.da9e603db3285p+0, 0x1.dfc97337b9b5fp+0, 0x1.e502ee78b3ff6p+0,
+    0x1.ea4afa2a490dap+0, 0x1.efa1bee615a27p+0, 0x1.f50765b6e4540p+0,
+    0x1.fa7c1819e90d8p+0
+};
+static const double KM_EXPT_L[64] = {
+    0x0.0p+0, -0x1.19083535b085dp-56, 0x1.d73e2a475b465p-55,
+    0x1.186be4bb284ffp-57, 0x1.8a62e4adc610bp-54, 0x1.03a1727c57b53p-59,
+    -0x1.6c51039449b3ap-54, -0x1.32fbf9af1369ep-54, -0x1.19041b9d78a76p-55,
+    0x1.e5b4c7b4968e4p-55, 0x1.e016e00a2643cp-54, 0x1.dc775814a8495p-55,
+    0x1.9b07eb6c70573p-54, 0x1.2bd339940e9d9p-55, 0x1.612e8afad1255p-55,
+    0x1.0024754db41d5p-54, 0x1.6f46ad23182e4p-55, 0x1.32721843659a6p-54,
+    -0x1.63aeabf42eae2p-54, -0x1.5e436d661f5e3p-56, 0x1.ada0911f09ebcp-55,
+    -0x1.ef3691c309278p-58, 0x1.89b7a04ef80d0p-59, 0x1.3c1a3b69062f0p-56,
+    0x1.d4397afec42e2p-56, -0x1.4b309d25957e3p-54, -0x1.07abe1db13cadp-55,
+    0x1.9bb2c011d93adp-54, 0x1.6324c054647adp-54, 0x1.ba6f93080e65ep-54,
+    -0x1.383c17e40b497p-54, -0x1.bb60987591c34p-54, -0x1.bdd3413b26456p-54,
+    -0x1.bbe3a683c88abp-57, -0x1.16e4786887a99p-55, -0x1.0245957316dd3p-54,
+    -0x1.41577ee04992fp-55, 0x1.05d02ba15797ep-56, -0x1.d4c1dd41532d8p-54,
+    -0x1.fc6f89bd4f6bap-54, 0x1.6e9f156864b27p-54, 0x1.5cc13a2e3976cp-55,
+    -0x1.75fc781b57ebcp-57, -0x1.d185b7c1b85d1p-54, 0x1.c7c46b071f2bep-56,
+    -0x1.359495d1cd533p-54, -0x1.d2f6edb8d41e1p-54, 0x1.0fac90ef7fd31p-54,
+    0x1.7a1cd345dcc81p-54, -0x1.2805e3084d708p-57, -0x1.5584f7e54ac3bp-56,
+    0x1.23dd07a2d9e84p-55, 0x1.11065895048ddp-55, 0x1.2884dff483cadp-54,
+    0x1.503cbd1e949dbp-56, -0x1.cbc3743797a9cp-54, 0x1.2ed02d75b3707p-55,
+    0x1.c2300696db532p-54, -0x1.1a5cd4f184b5cp-54, 0x1.39e8980a9cc8fp-55,
+    -0x1.e9c23179c2893p-54, 0x1.dc7f486a4b6b0p-54, 0x1.9d3e12dd8a18bp-54,
+    0x1.74853f3a5931ep-55
+};
+
+#define KM_LN2_64_1 0x1.62e42fefa0000p-7 /* 36 bits: n * LN2_64_1 exact, |n| < 2^17 */
+#define KM_LN2_64_2 0x1.cf79abc9e0000p-46 /* next 36 bits */
+#define KM_LN2_64_3 0x1.d9cc01f97b57ap-85
+#define KM_64_OVER_LN2 0x1.71547652b82fep+6
+
+/* exp(xh + xl) = 2^k 2^(j/64) exp(r): n = rint(xh 64/ln2) = 64 k + j,
+ * r = x - n ln2/64 as a double-double (ln2/64 in three parts), |r| <=
+ * ln2/128 (+), and exp(r) - 1 = r + r^2/2 + r^3/6 + r^4 (1/4! + ... +
+ * r^5/9!), the first three terms in double-double: the mantissa
+ * 2^(j/64) exp(r) as a double-double with relative error below 2^-85,
+ * returned with k.  |xh| < 746. */
+KM_FN void km_exp_dd(double xh, double xl, double *mh, double *ml, int *k_out)
+{
+    double fn = km_rint_small(xh * KM_64_OVER_LN2);
+    int n = (int)fn;
+    int j = n & 63;
+    *k_out = (n - j) / 64;
+    double r1 = xh - fn * KM_LN2_64_1; /* exact */
+    double rh, rl;
+    km_two_sum(r1, -(fn * KM_LN2_64_2), &rh, &rl); /* the product exact */
+    rl += xl - fn * KM_LN2_64_3;
+    km_fast_two_sum(rh, rl, &rh, &rl);
+    double zh = rh * rh;
+    double zl = km_two_prod_err(rh, rh, zh) + 2.0 * rh * rl;
+    double r3h, r3l, t3h, t3l;
+    km_dd_mul(rh, rl, zh, zl, &r3h, &r3l);
+    km_dd_mul(r3h, r3l, KM_SIXTH_H, KM_SIXTH_L, &t3h, &t3l);
+    double tail = (zh * zh) * (0x1.5555555555555p-5 +
+                  rh * (0x1.1111111111111p-7 +
+                  rh * (0x1.6c16c16c16c17p-10 +
+                  rh * (0x1.a01a01a01a01ap-13 +
+                  rh * (0x1.a01a01a01a01ap-16 +
+                  rh * 0x1.71de3a556c734p-19)))));
+    double eh, el;
+    km_dd_add(t3h, t3l, tail, 0.0, &eh, &el);
+    km_dd_add(0.5 * zh, 0.5 * zl, eh, el, &eh, &el);
+    km_dd_add(rh, rl, eh, el, &eh, &el); /* exp(r) - 1 */
+    double Th = KM_EXPT_H[j], Tl = KM_EXPT_L[j];
+    double ph, pl;
+    km_dd_mul(Th, Tl, eh, el, &ph, &pl);
+    km_dd_add(Th, Tl, ph, pl, mh, ml);
+}
+
+/* 2^k (mh + ml), rounded once where the result is normal */
+KM_FN double km_scale_dd(double mh, double ml, int k)
+{
+    if (k > 1023) {
+        mh *= 0x1p1023;
+        ml *= 0x1p1023;
+        k -= 1023;
+    } else if (k < -1021) {
+        /* subnormal range: scaled in two steps (may round twice) */
+        double m = (mh + ml) * 0x1p-600;
+        return m * km_from_bits((uint64_t)(k + 600 + 1023) << 52);
+    }
+    return (mh + ml) * km_from_bits((uint64_t)(k + 1023) << 52);
+}
 
 KM_FN double km_exp(double x)
 {
     if (km_isnan(x)) return x;
     if (x > 709.782712893384) return __builtin_inf();
     if (x < -745.2) return 0.0;
-    double fk = km_rint_small(x * KM_INV_LN2);
-    double rh = x - fk * KM_LN2_HI; /* exact */
-    double rl = -(fk * KM_LN2_LO);
-    double r = rh + rl;
-    rl = rl - (r - rh);
-    double q = 0x1.0000000000000p-1 +
-               r * (0x1.5555555555555p-3 +
-               r * (0x1.5555555555555p-5 +
-               r * (0x1.1111111111111p-7 +
-               r * (0x1.6c16c16c16c17p-10 +
-               r * (0x1.a01a01a01a01ap-13 +
-               r * (0x1.a01a01a01a01ap-16 +
-               r * (0x1.71de3a556c734p-19 +
-               r * (0x1.27e4fb7789f5cp-22 +
-               r * (0x1.ae64567f544e4p-26 +
-               r * (0x1.1eed8eff8d898p-29 +
-               r * (0x1.6124613a86d09p-33 +
-               r * (0x1.93974a8c07c9dp-37 +
-               r * 0x1.ae7f3e733b81fp-41))))))))))));
-    double e = 1.0 + (r + ((r * r) * q + rl * (1.0 + r)));
-    int k = (int)fk;
-    if (k > 1023) {
-        e *= 0x1p1023;
-        k -= 1023;
-    } else if (k < -1021) {
-        e *= 0x1p-600;
-        k += 600;
-    }
-    return e * km_from_bits((uint64_t)(k + 1023) << 52);
+    double mh, ml;
+    int k;
+    km_exp_dd(x, 0.0, &mh, &ml, &k);
+    return km_scale_dd(mh, ml, k);
+}
+
+/* x^y for x > 0 finite and y finite: exp(y log x) with log x and the
+ * product in double-double (relative error of the result before its
+ * rounding below 2^-70 |y log x| + 2^-84); other arguments through
+ * exp(y log x) in plain double. */
+KM_FN double km_pow(double x, double y)
+{
+    if (!(x > 0.0) || x == __builtin_inf() || km_isnan(y) || km_fabs(y) == __builtin_inf())
+        return km_exp(y * km_log(x));
+    if (y == 0.0 || x == 1.0) return 1.0;
+    double lh, ll, zh, zl;
+    km_log_dd(x, &lh, &ll);
+    km_dd_mul(lh, ll, y, 0.0, &zh, &zl);
+    if (zh > 709.782712893384) return __builtin_inf();
+    if (zh < -745.2) return 0.0;
+    double mh, ml;
+    int k;
+    km_exp_dd(zh, zl, &mh, &ml, &k);
+    return km_scale_dd(mh, ml, k);
 }
 
 #endif /* KELP_KMATH_H */

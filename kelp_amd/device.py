@@ -14,7 +14,8 @@ from typing import Dict, Optional
 import numpy as np
 
 from . import _native
-from ._native import KELP_DT_BF16, KELP_DT_F16S, KELP_DT_F32, KELP_DT_F64, MeshDesc, Particles, StepArgs, check
+from ._native import (KELP_DT_BF16, KELP_DT_F16S, KELP_DT_F32, KELP_DT_F64, MeshDesc, Particles, RecordRows,
+                      StepArgs, check)
 from .mesh import Mesh
 
 
@@ -421,7 +422,9 @@ class DeviceContext:
                  else t.as_tensor(np.ascontiguousarray(pid, np.int64), device=dev))
         nn_t = i32(nn, -1)
         el_t = None if elem is None else i32(elem, -1)
-        # the tensors above were filled on torch's stream; the id maps below
+        arrays = dict(lon=f64(lon), lat=f64(lat), z=f64(z), status=i32(status, 1), u=f64(None), v=f64(None),
+                      w=f64(None), ws=i32(ws, 0))
+        # every tensor above was filled on torch's stream; the id maps below
         # and every later call run on the context's stream
         t.cuda.current_stream().synchronize()
         if nn is not None:
@@ -432,9 +435,7 @@ class DeviceContext:
             check(self.lib.kelp_elems_from_mesh_ids(self.h, el_t.numel(), el_t.data_ptr()),
                   "kelp_elems_from_mesh_ids")
             self.synchronize()
-        return DeviceParticles(lon=f64(lon), lat=f64(lat), z=f64(z), status=i32(status, 1),
-                               nn=nn_t, fr=fr_t, u=f64(None), v=f64(None), w=f64(None),
-                               pid=pid_t, ws=i32(ws, 0), ctx=self, elem=el_t)
+        return DeviceParticles(nn=nn_t, fr=fr_t, pid=pid_t, ctx=self, elem=el_t, **arrays)
 
     # -- operations ---------------------------------------------------------
     def step(self, p: DeviceParticles, order: int, slot_a: int, dt: float, time_dir: float,
@@ -454,6 +455,22 @@ class DeviceContext:
                            periodic_lon, periodic_lat, reuse_anchor, sigma, walk, rk45)
         s = p.as_struct()
         check(self.lib.kelp_step(self.h, ctypes.byref(s), ctypes.byref(a)), "kelp_step")
+
+    def record_stage(self, p: DeviceParticles, prev_status, pid_offset: int, n_rows: int, time_dir: float,
+                     rows: Dict[str, object], sampled=None) -> None:
+        """One masked record of every particle into the device rows ``rows``
+        (lon, lat, z, u, v, w or None, status, fr, samples or None: device
+        tensors of n_rows entries, samples n_vars x n_rows), at row pid -
+        pid_offset (kelp_record_stage); ``sampled``: the n_vars x n device
+        rows of :meth:`sample_vars`.  No wait."""
+        s = p.as_struct()
+        ptr = lambda t: 0 if t is None else t.data_ptr()
+        out = RecordRows(*(ptr(rows.get(k)) for k in ("lon", "lat", "z", "u", "v", "w", "status", "fr",
+                                                        "samples")))
+        n_vars = 0 if sampled is None else int(sampled.shape[0])
+        check(self.lib.kelp_record_stage(self.h, ctypes.byref(s), prev_status.data_ptr(), int(pid_offset),
+                                         int(n_rows), float(time_dir), n_vars, ptr(sampled), ctypes.byref(out)),
+              "kelp_record_stage")
 
     def activate(self, p: DeviceParticles, release_step, pid_offset: int, step: int,
                  prev_status=None) -> None:

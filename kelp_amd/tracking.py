@@ -411,6 +411,63 @@ def mask_record(h: Dict[str, np.ndarray], prev_status: np.ndarray, direction: in
     return out
 
 
+class RecordStager:
+    """The records of a run kept in HBM and copied to the host in batches:
+    each record is masked and put in particle-id order on the device
+    (``kelp_record_stage``: core/release.py:132 mask_dormant_for_record,
+    :170 mask_inactive_velocity, the backward flip of tracking.py:2772),
+    into a block of ``capacity`` records per variable; a full block (and
+    :meth:`flush`) waits for the device once and moves the block with one
+    copy per variable.  The rows reaching ``rec`` are those :func:`mask_record`
+    forms from the host state, bit for bit."""
+
+    def __init__(self, ctx, p, pid_offset: int, rec: "TrajectoryRecord", sample_vars,
+                 budget_bytes: int = 256 << 20, max_records: int = 64):
+        t = ctx.torch
+        dev = p.lon.device
+        self.ctx, self.p, self.rec = ctx, p, rec
+        self.pid_offset = int(pid_offset)
+        self.P = int(p.n)
+        self.names = list(sample_vars or [])
+        nv = len(self.names)
+        per = self.P * (8 * 6 + 4 + 1 + 8 * nv)
+        self.R = int(max(1, min(max_records, budget_bytes // max(per, 1))))
+        f64 = lambda: t.empty((self.R, self.P), dtype=t.float64, device=dev)
+        self.buf = {k: f64() for k in ("lon", "lat", "z", "u", "v")}
+        if p.w is not None:
+            self.buf["w"] = f64()
+        self.buf["status"] = t.empty((self.R, self.P), dtype=t.int32, device=dev)
+        self.buf["fr"] = t.empty((self.R, self.P), dtype=t.int8, device=dev)
+        if nv:
+            self.buf["samples"] = t.empty((self.R, nv, self.P), dtype=t.float64, device=dev)
+        self.sample_rows = t.empty((nv, self.P), dtype=t.float64, device=dev) if nv else None
+        self.times: List[float] = []
+
+    def append(self, p, prev_status, direction: int, sampled, time_val: float) -> None:
+        c = len(self.times)
+        rows = {k: b[c] for k, b in self.buf.items()}
+        self.ctx.record_stage(p, prev_status, self.pid_offset, self.P, float(direction), rows, sampled)
+        self.times.append(float(time_val))
+        if len(self.times) == self.R:
+            self.flush()
+
+    def flush(self) -> None:
+        c = len(self.times)
+        if c == 0:
+            return
+        self.ctx.synchronize()
+        host = {k: b[:c].cpu().numpy() for k, b in self.buf.items()}
+        for j in range(c):
+            row = dict(lon=host["lon"][j], lat=host["lat"][j], z=host["z"][j], status=host["status"][j],
+                       failure_reason=host["fr"][j], vel_u=host["u"][j], vel_v=host["v"][j])
+            if "w" in host:
+                row["vel_w"] = host["w"][j]
+            for k, name in enumerate(self.names):
+                row["sample__" + name] = host["samples"][j, k]
+            self.rec.append(row, self.times[j])
+        self.times = []
+
+
 def run_single_realization(mesh: Mesh, ds, reader, init_state: ParticleState, mode: ModeConfig,
                            numerics: NumericsConfig, *, start_time_idx: int = 0,
                            n_steps: int, n_time_total: Optional[int] = None,
@@ -543,7 +600,10 @@ def run_single_realization(mesh: Mesh, ds, reader, init_state: ParticleState, mo
     ck_on = (bool(checkpoint_path) or checkpoint_sink is not None) and ck_freq > 0
     last_g = None
 
+    stager = RecordStager(ctx, p, particle_offset, rec, sample_vars)
+
     def checkpoint(g_done, next_idx):
+        stager.flush()      # the file written with a checkpoint holds every record so far
         meta = rec.meta if rec is not None else {}
         release = ({k: meta["release_" + k] for k in ("lon", "lat", "z")}
                    if all(("release_" + k) in meta for k in ("lon", "lat", "z")) else None)
@@ -662,7 +722,6 @@ def run_single_realization(mesh: Mesh, ds, reader, init_state: ParticleState, mo
         if use_tinterp:
             sim_time = sim_time_end if sim_time_end is not None else sim_time + dt * direction
         if (g + 1) % max(int(numerics.output_freq), 1) == 0:
-            sampled = {}
             if sample_vars:
                 # the scalars at the record's instant (tracking.py:2692-2743)
                 ra, rb, ral = sa, sb, pay.alpha0
@@ -680,23 +739,10 @@ def run_single_realization(mesh: Mesh, ds, reader, init_state: ParticleState, mo
                         nxt %= n_time_total
                     if 0 <= nxt < n_time_total:
                         ra, rb, ral = slots.slot(nxt), -1, 0.0
-                rows = ctx.sample_vars(p, range(len(sample_vars)), ra, rb, ral,
-                                       numerics.sample_interp)
-                ctx.synchronize()
-                rows = rows.cpu().numpy()
-                for k, name in enumerate(sample_vars):
-                    sampled["sample__" + name] = rows[k]
-            h = p.to_host()
-            prev = prev_status.cpu().numpy()
-            order_idx = np.argsort(h["pid"], kind="stable")
-            h = {k: v[order_idx] for k, v in h.items()}
-            prev = prev[order_idx]
-            m = mask_record(h, prev, direction, is3d)
-            row = dict(lon=m["lon"], lat=m["lat"], z=m["z"], status=m["status"],
-                       failure_reason=m["fr"], vel_u=m["u"], vel_v=m["v"], vel_w=m["w"])
-            for k, v in sampled.items():
-                row[k] = v[order_idx]
-            rec.append(row, t_g0 + (g + 1) * dt * direction)
+                sampled = ctx.sample_vars(p, range(len(sample_vars)), ra, rb, ral, numerics.sample_interp,
+                                          out=stager.sample_rows)
+            stager.append(p, prev_status, direction, sampled if sample_vars else None,
+                          t_g0 + (g + 1) * dt * direction)
         last_g = g
         next_idx = (start_time_idx + (step_i + 1) * direction) if not use_tinterp else None
         if use_tinterp:
@@ -706,6 +752,7 @@ def run_single_realization(mesh: Mesh, ds, reader, init_state: ParticleState, mo
             checkpoint(g, next_idx)
     if ck_on and last_g is not None and (last_g + 1) % ck_freq != 0:
         checkpoint(last_g, next_idx)
+    stager.flush()
     return rec
 
 

@@ -27,7 +27,7 @@
 #define M_ATAN2 km_atan2
 #define M_LOG km_log
 #define M_EXP km_exp
-#define M_POW(x, y) km_exp((y) * km_log(x))
+#define M_POW km_pow
 #define PYMOD360 km_pymod360
 #else
 #define M_SIN sin
@@ -2407,6 +2407,7 @@ void or_math_probe(int fn, int64_t n, const double *x, const double *y, double *
         case 7: out[i] = PYMOD360(x[i]); break;
         case 9: out[i] = PYMOD_POS(x[i], y[i]); break;
         case 10: out[i] = REFLECT_DEPTH(x[i], 0.0, y[i]); break;
+        case 13: out[i] = M_POW(x[i], y[i]); break;
         case 11:
         case 12: {
             double g1, g2;

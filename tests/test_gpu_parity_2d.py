@@ -33,7 +33,8 @@ def test_random_walk_draws_vs_host_builds(gpu_ctx):
     oracle built with glibc (the reference's libm under Numba) the pairs
     differ only where glibc rounds log, sin or cos otherwise than the
     correctly rounded result -- measured here, and each such draw is off by
-    at most one unit in the last place."""
+    at most two units in the last place (one in log, sin or cos, carried
+    through the root and the product)."""
     t = gpu_ctx.torch
     n = 10_000_000
     pid = np.arange(n, dtype=np.float64)
@@ -51,7 +52,7 @@ def test_random_walk_draws_vs_host_builds(gpu_ctx):
         g = math_probe(fn, pid, step, flavour="glibc")
         diff = dev != g
         rates.append(diff.mean())
-        assert np.all(np.abs(dev[diff] - g[diff]) <= np.spacing(np.abs(g[diff])))
+        assert np.all(np.abs(dev[diff] - g[diff]) <= 2.0 * np.spacing(np.abs(g[diff])))
     print(f"normal draws differing from the glibc build: {rates[0]:.3%} (g1), {rates[1]:.3%} (g2) of {n}")
     assert max(rates) < 0.005
 

@@ -86,6 +86,65 @@ def test_activate_kernel_matches_release_rule(gpu_ctx):
         np.testing.assert_array_equal(prev.cpu().numpy(), want[perm])
 
 
+@pytest.mark.parametrize("direction,is3d", [(1, True), (-1, True), (1, False)])
+def test_record_staging_equals_host_masking(gpu_ctx, direction, is3d):
+    """kelp_record_stage against the host record (core/release.py:132 and
+    :170 masks, the backward flip, particle-id order): every variable bit
+    for bit, NaN signs included, with the particles shuffled on the device,
+    statuses of every kind, sampled scalars, and a block smaller than the
+    number of records (three flushes)."""
+    from kelp_amd import synth
+    from kelp_amd.tracking import RecordStager, TrajectoryRecord, mask_record
+    mesh = synth.sphere_mesh(2000)
+    gpu_ctx.upload_mesh(mesh)
+    rng = np.random.default_rng(5)
+    P, off = 3000, 40
+    perm = rng.permutation(P)
+    torch = gpu_ctx.torch
+    rec = TrajectoryRecord()
+    want = []
+    stager = None
+    for r in range(7):
+        status = rng.choice([1, 1, 1, 2, 3, 4], P).astype(np.int32)
+        prev = rng.choice([1, 1, 1, 2, 4], P).astype(np.int32)
+        vals = {k: rng.normal(size=P) for k in ("lon", "lat", "z", "u", "v", "w")}
+        fr = rng.integers(0, 5, P).astype(np.int8)
+        p = gpu_ctx.particles(vals["lon"][perm], vals["lat"][perm], vals["z"][perm] if is3d else None,
+                              status=status[perm], fr=fr[perm], pid=(off + perm).astype(np.int64))
+        p.u.copy_(torch.as_tensor(vals["u"][perm]))
+        p.v.copy_(torch.as_tensor(vals["v"][perm]))
+        if p.w is not None:
+            p.w.copy_(torch.as_tensor(vals["w"][perm]))
+        if stager is None:
+            stager = RecordStager(gpu_ctx, p, off, rec, ["t", "s"], max_records=3)
+            assert stager.R == 3
+        samp = torch.as_tensor(rng.normal(size=(2, P))[:, perm].copy(), device=p.lon.device)
+        prev_dev = torch.as_tensor(prev[perm], device=p.lon.device)
+        torch.cuda.current_stream().synchronize()     # the fills above ran on torch's stream
+        stager.append(p, prev_dev, direction, samp, float(r))
+        h = p.to_host()
+        order = np.argsort(h["pid"], kind="stable")
+        h = {k: v[order] for k, v in h.items()}
+        m = mask_record(h, prev, direction, is3d)
+        row = dict(lon=m["lon"], lat=m["lat"], z=m["z"], status=m["status"], failure_reason=m["fr"],
+                   vel_u=m["u"], vel_v=m["v"], vel_w=m["w"], sample__t=samp.cpu().numpy()[0][order],
+                   sample__s=samp.cpu().numpy()[1][order])
+        want.append(row)
+    stager.flush()
+    got = rec.as_arrays()
+    assert got["time"].tolist() == [float(r) for r in range(7)]
+    for k in want[0]:
+        if k == "vel_w" and "vel_w" not in got:
+            continue
+        w = np.stack([row[k] for row in want], axis=1)
+        g = got[k]
+        assert g.dtype == w.dtype or g.dtype.kind == w.dtype.kind, k
+        if g.dtype.kind == "f":
+            np.testing.assert_array_equal(g.view(np.int64), w.view(np.int64), err_msg=k)
+        else:
+            np.testing.assert_array_equal(g, w, err_msg=k)
+
+
 def test_staggered_release_run(gpu_ctx):
     """A release schedule in the device loop: dormant particles are masked
     and do not move, particles released at step 0 follow the run without a

@@ -143,6 +143,42 @@ def test_draw_math_correctly_rounded(fn):
     assert np.mean(g != ref) < 0.01
 
 
+@pytest.mark.parametrize("fn", [2, 3, 4, 6, 13])
+def test_inverse_trig_exp_pow_correctly_rounded(fn):
+    """asin and atan2 of the random walk's displaced unit vectors
+    (numerics/diffusion.py:539 sphere_displace), acos near 1 (the exact
+    search's cap angles), exp of the beaching probabilities' arguments and
+    pow(x, 0.2) of the rk45 controller: equal to the 160-bit mpmath value
+    rounded to double on every sample; glibc rounds otherwise on a small
+    fraction, where the device can differ from the reference."""
+    import mpmath as mp
+    mp.mp.prec = 160
+    rng = np.random.default_rng(40 + fn)
+    n = 3000
+    y = None
+    if fn == 2:
+        x = np.concatenate([rng.uniform(-1, 1, n), np.sin(np.radians(rng.uniform(-90, 90, n)))])
+        f = lambda a, b: mp.asin(a)
+    elif fn == 3:
+        x = np.concatenate([rng.uniform(-1, 1, n), 1 - 10 ** rng.uniform(-16, -1, n)])
+        f = lambda a, b: mp.acos(a)
+    elif fn == 4:
+        x, y = rng.normal(size=2 * n), rng.normal(size=2 * n)
+        f = lambda a, b: mp.atan2(a, b)
+    elif fn == 6:
+        x = np.concatenate([rng.uniform(-30, 30, n), -rng.uniform(0, 0.1, n)])
+        f = lambda a, b: mp.exp(a)
+    else:
+        x, y = 10 ** rng.uniform(-6, 6, 2 * n), np.full(2 * n, 0.2)
+        f = lambda a, b: mp.power(a, b)
+    yy = x if y is None else y
+    ref = np.array([float(f(mp.mpf(float(a)), mp.mpf(float(b)))) for a, b in zip(x, yy)])
+    k = math_probe(fn, x, y, flavour="kmath")
+    g = math_probe(fn, x, y, flavour="glibc")
+    assert np.array_equal(k, ref)
+    assert np.mean(g != ref) < 0.01
+
+
 def test_atan2_within_two_ulp_of_glibc():
     rng = np.random.default_rng(4)
     y = rng.uniform(-3, 3, 50000)
