@@ -33,8 +33,9 @@ def test_random_walk_draws_vs_host_builds(gpu_ctx):
     oracle built with glibc (the reference's libm under Numba) the pairs
     differ only where glibc rounds log, sin or cos otherwise than the
     correctly rounded result -- measured here, and each such draw is off by
-    at most two units in the last place (one in log, sin or cos, carried
-    through the root and the product)."""
+    at most three units in the last place: one in log (halved by the root),
+    one in sin or cos, and the roundings of the root and the product (on
+    these keys 2 of the 2*10^7 draws are off by three, about 4,000 by two)."""
     t = gpu_ctx.torch
     n = 10_000_000
     pid = np.arange(n, dtype=np.float64)
@@ -52,7 +53,7 @@ def test_random_walk_draws_vs_host_builds(gpu_ctx):
         g = math_probe(fn, pid, step, flavour="glibc")
         diff = dev != g
         rates.append(diff.mean())
-        assert np.all(np.abs(dev[diff] - g[diff]) <= 2.0 * np.spacing(np.abs(g[diff])))
+        assert np.all(np.abs(dev[diff] - g[diff]) <= 3.0 * np.spacing(np.abs(g[diff])))
     print(f"normal draws differing from the glibc build: {rates[0]:.3%} (g1), {rates[1]:.3%} (g2) of {n}")
     assert max(rates) < 0.005
 
