@@ -50,6 +50,11 @@ UNITS = [("kelp_hip", "kelp_hip.hip", []), ("kelp_sort", "kelp_sort.hip", []),
 ]
 
 
+# the slowest units first, so the parallel build does not end on a long tail
+_SLOW_FIRST = ("kelp_batch_", "kelp_step_3d", "kelp_sigma_")
+UNITS.sort(key=lambda u: next((i for i, p in enumerate(_SLOW_FIRST) if u[0].startswith(p)), len(_SLOW_FIRST)))
+
+
 def hipcc() -> str:
     for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", "hipcc"):
         if cand and (os.path.sep not in cand or os.path.exists(cand)):

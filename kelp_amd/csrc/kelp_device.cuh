@@ -57,6 +57,16 @@ constexpr int32_t kOutside = 3;
 constexpr int8_t kFrInterp = 1;
 constexpr int8_t kFrOOB = 2;
 
+// What the step reads about an anchor node, in one 64-byte record per node
+// (internal order) instead of four arrays: the cap test's longitude and the
+// sine and cosine of its latitude (the z of its unit vector and its coslat
+// entry, the same bits), its bottom depth and its element-list range.
+struct alignas(64) NodeRec {
+    double lon, sinlat, coslat, depth;
+    int32_t e0, e1;
+    double lat, pad;
+};
+
 struct DMesh {
     const double2* __restrict__ lonlat;
     const double4* __restrict__ gh_xyz;
@@ -73,21 +83,22 @@ struct DMesh {
     const uint8_t* __restrict__ eflags;   // (E,) bit 0: the interpolation ring is the containment ring
     const int4* __restrict__ enbr;        // (E,) neighbour across edge k of the compacted ring, -1 boundary
                                           //   (core/mesh.py:2824 build_elem_neighbors); null without it
+    const NodeRec* __restrict__ nrec;     // (N,) anchor record per node
     double cell, lon_min, lat_min, r_earth;
     int32_t nx, ny, wrap, far_rows;
     int32_t n_nodes, n_elems, max_v, n_levels;
+    int32_t tile_on;  // 1 only in launches whose kernel stages a GhTile in its dynamic LDS first
 };
 
 __device__ __forceinline__ double sqd(double x) { return x * x; }
 
-// Node nn's element list n2e[k0 .. k1), from the w of its grid-hash entry
-// (internal node id = CSR slot, so gh_xyz[nn] is the node's entry): the
-// line the anchor search has just read, not a separate offsets array.
+// Node nn's element list n2e[k0 .. k1), from its anchor record: the line
+// the anchor search has just read for the cap test, not a separate offsets
+// array.
 __device__ __forceinline__ void n2e_range(const DMesh& m, int nn, int& k0, int& k1)
 {
-    const double w = m.gh_xyz[nn].w;
-    k0 = __double2loint(w);
-    k1 = __double2hiint(w);
+    k0 = m.nrec[nn].e0;
+    k1 = m.nrec[nn].e1;
 }
 
 // core/mesh.py:1193 _gh_fold_dlon
@@ -133,11 +144,165 @@ __device__ inline Query make_query(double lon_p, double lat_p)
     return q;
 }
 
+// ---------------------------------------------------------------------------
+// The workgroup's grid-hash tile in LDS.
+//
+// The particles of one workgroup are neighbours once they are sorted by hash
+// cell, so the 3x3 windows that all their anchor searches scan (every RK
+// site, the kick, the re-anchoring after a fold) fall inside one small box of
+// cells.  A kernel launched with m.tile_on stages that box -- the cells
+// around its alive particles' start cells, two cells of margin -- in its
+// dynamic LDS once per launch: the CSR offsets of the box's cells and the
+// unit vectors of their nodes.  A scan whose row slice lies inside the box
+// reads LDS; any other (a particle that left the box, a box too large to
+// stage, a window that wraps) reads HBM as before.  Same values, same
+// order, same strict `<`: the answer is the same node either way.
+// ---------------------------------------------------------------------------
+
+// Deferred rare paths (template flag DF of the search functions): with DF
+// the code has no call to the exact search's rectangle widening (an
+// out-of-line function whose call sites inside the step make the compiler
+// spill the step's registers).  A lane that needs it marks itself in the
+// tile's per-lane flags and takes a stand-in answer that keeps every later
+// read in bounds; the step kernel drops that lane's results and steps it
+// again with the full search (DF false) from its unchanged state.
+
+constexpr int kTileRows = 8;       // cell rows a tile can hold
+constexpr int kTileCols = 64;      // cells per row
+constexpr int kTileNodes = 1536;   // nodes (24 B each in LDS)
+
+struct GhTile {
+    int32_t ok, ci0, cj0, nci, ncj;
+    int32_t lo_ci, hi_ci, lo_cj, hi_cj;
+    int32_t delta[kTileRows];               // LDS slot - CSR slot, per row
+    int32_t off[kTileRows][kTileCols + 1];  // gh_offsets of the box's cells (+ the row end)
+    double x[kTileNodes], y[kTileNodes], z[kTileNodes];
+    uint8_t defer[256];                     // per lane: the exact search's widening was needed (DF)
+};
+
+extern __shared__ __align__(16) unsigned char kelp_dyn_lds[];
+
+__device__ __forceinline__ GhTile& gh_tile() { return *reinterpret_cast<GhTile*>(kelp_dyn_lds); }
+
+// this lane needs a rare path the kernel was built without
+__device__ __forceinline__ void mark_deferred() { gh_tile().defer[threadIdx.x] = 1; }
+
+__device__ __forceinline__ int wave_min_i32(int v)
+{
+#pragma unroll
+    for (int s = 32; s > 0; s >>= 1) {
+        const int o = __shfl_xor(v, s, 64);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ int wave_max_i32(int v)
+{
+#pragma unroll
+    for (int s = 32; s > 0; s >>= 1) {
+        const int o = __shfl_xor(v, s, 64);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+// Stage the tile of this workgroup.  Every thread of the workgroup calls it
+// (it synchronises the workgroup); `alive` lanes contribute their position's
+// cell to the box.  Launch with sizeof(GhTile) bytes of dynamic LDS.
+__device__ inline void stage_gh_tile(const DMesh& m, bool alive, double lon_p, double lat_p)
+{
+    GhTile& t = gh_tile();
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    int ci = 0x7fffffff, cj = 0x7fffffff, xi = -1, xj = -1;
+    if (alive) {
+        gh_cell_of(m, lon_p, lat_p, ci, cj);
+        xi = ci;
+        xj = cj;
+    }
+    if (tid == 0) {
+        t.lo_ci = 0x7fffffff;
+        t.lo_cj = 0x7fffffff;
+        t.hi_ci = -1;
+        t.hi_cj = -1;
+    }
+    __syncthreads();
+    ci = wave_min_i32(ci);
+    cj = wave_min_i32(cj);
+    xi = wave_max_i32(xi);
+    xj = wave_max_i32(xj);
+    if ((tid & 63) == 0) {
+        atomicMin(&t.lo_ci, ci);
+        atomicMin(&t.lo_cj, cj);
+        atomicMax(&t.hi_ci, xi);
+        atomicMax(&t.hi_cj, xj);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int ok = t.hi_ci >= 0;
+        const int a0 = t.lo_ci - 2 > 0 ? t.lo_ci - 2 : 0;
+        const int a1 = t.hi_ci + 2 < m.nx - 1 ? t.hi_ci + 2 : m.nx - 1;
+        const int b0 = t.lo_cj - 2 > 0 ? t.lo_cj - 2 : 0;
+        const int b1 = t.hi_cj + 2 < m.ny - 1 ? t.hi_cj + 2 : m.ny - 1;
+        ok = ok && a1 - a0 + 1 <= kTileCols && b1 - b0 + 1 <= kTileRows;
+        t.ci0 = a0;
+        t.cj0 = b0;
+        t.nci = ok ? a1 - a0 + 1 : 0;
+        t.ncj = ok ? b1 - b0 + 1 : 0;
+        t.ok = ok;
+    }
+    __syncthreads();
+    const int nci = t.nci, ncj = t.ncj;
+    // offsets of cells ci0 .. ci0 + nci (the last one ends the row's slice);
+    // ci0 + nci <= nx, so the index stays within the nx * ny + 1 offsets
+    for (int e = tid; e < ncj * (nci + 1); e += nthr) {
+        const int r = e / (nci + 1), cc = e - r * (nci + 1);
+        t.off[r][cc] = m.gh_offsets[(int64_t)(t.cj0 + r) * m.nx + t.ci0 + cc];
+    }
+    __syncthreads();
+    if (tid == 0 && t.ok) {
+        int l = 0;
+        for (int r = 0; r < ncj; ++r) {
+            t.delta[r] = l - t.off[r][0];
+            l += t.off[r][nci] - t.off[r][0];
+        }
+        if (l > kTileNodes) t.ok = 0;
+    }
+    __syncthreads();
+    if (t.ok) {
+        for (int r = 0; r < ncj; ++r) {
+            const int k0 = t.off[r][0], k1 = t.off[r][nci], d = t.delta[r];
+            for (int k = k0 + tid; k < k1; k += nthr) {
+                const double4 p = m.gh_xyz[k];
+                t.x[k + d] = p.x;
+                t.y[k + d] = p.y;
+                t.z[k + d] = p.z;
+            }
+        }
+    }
+    __syncthreads();
+}
+
 // core/mesh.py:1570 _gh_scan_row_range: cells c_lo..c_hi of row cj2 are one
 // contiguous CSR slice; strict `<` keeps the first node found on a tie.
 __device__ inline void scan_range(const DMesh& m, int cj2, int c_lo, int c_hi, const Query& q,
                                   int& best, double& best_d2)
 {
+    if (m.tile_on) {
+        const GhTile& t = gh_tile();
+        const int r = cj2 - t.cj0, a = c_lo - t.ci0, b = c_hi - t.ci0;
+        if (t.ok && r >= 0 && r < t.ncj && a >= 0 && b < t.nci) {
+            const int32_t start = t.off[r][a], end = t.off[r][b + 1], d = t.delta[r];
+            for (int32_t k = start; k < end; ++k) {
+                double d2 = sqd(t.x[k + d] - q.x) + sqd(t.y[k + d] - q.y) + sqd(t.z[k + d] - q.z);
+                if (d2 < best_d2) {
+                    best_d2 = d2;
+                    best = k;
+                }
+            }
+            return;
+        }
+    }
     const int64_t row = (int64_t)cj2 * m.nx;
     int32_t start = m.gh_offsets[row + c_lo];
     int32_t end = m.gh_offsets[row + c_hi + 1];
@@ -215,9 +380,10 @@ __device__ inline int nearest_3x3(const DMesh& m, int ci, int cj, const Query& q
 __device__ inline void cap_extents(const DMesh& m, const Query& q, double lon_p, double lat_p, int node,
                                    double& theta_deg, double& ext_lon)
 {
-    const double lon_n = m.lonlat[node].x;
+    const NodeRec& nr = m.nrec[node];
+    const double lon_n = nr.lon;
     const double sp = q.z, cp = q.cos_lat;
-    const double sn = m.gh_xyz[node].z, cn = m.coslat[node];
+    const double sn = nr.sinlat, cn = nr.coslat;
     double cosd = (sp * sn + cp * cn * km_cos((lon_p - lon_n) * kDeg2Rad));
     if (cosd > 1.0) cosd = 1.0;
     if (cosd < -1.0) cosd = -1.0;
@@ -347,6 +513,7 @@ __device__ inline int nearest_widened(const DMesh& m, double lon_p, double lat_p
 __device__ __noinline__ int exact_widen(const DMesh& m, double lon_p, double lat_p, int best,
                                         double best_d2, int code, int ci, int cj, const Query& q);
 
+template <bool DF = false>
 __device__ inline int nearest_exact(const DMesh& m, double lon_p, double lat_p)
 {
     const double cell = m.cell;
@@ -366,7 +533,12 @@ __device__ inline int nearest_exact(const DMesh& m, double lon_p, double lat_p)
         if (code == 0) return best;
         if (theta_deg > far) return best;
     }
-    return exact_widen(m, lon_p, lat_p, best, best_d2, code, ci, cj, q);
+    if constexpr (DF) {
+        mark_deferred();
+        return best >= 0 ? best : 0;
+    } else {
+        return exact_widen(m, lon_p, lat_p, best, best_d2, code, ci, cj, q);
+    }
 }
 
 __device__ __noinline__ int exact_widen(const DMesh& m, double lon_p, double lat_p, int best,
@@ -513,16 +685,17 @@ __device__ __noinline__ int exact_widen(const DMesh& m, double lon_p, double lat
 // (integrators.py:1471 _fused_site_anchor); 2: the batch guard-off anchor of
 // core/mesh.py:1896 find_nearest_nodes (3x3 window, an empty window healed
 // with the true nearest node).
+template <bool DF = false>
 __device__ inline int site_anchor(const DMesh& m, double lon_p, double lat_p, int mode)
 {
-    if (mode == 1) return nearest_exact(m, lon_p, lat_p);
+    if (mode == 1) return nearest_exact<DF>(m, lon_p, lat_p);
     if (mode == 2) {
         int ci, cj;
         gh_cell_of(m, lon_p, lat_p, ci, cj);
         Query q = make_query(lon_p, lat_p);
         double d2;
         int b = nearest_3x3(m, ci, cj, q, d2);
-        return b >= 0 ? b : nearest_exact(m, lon_p, lat_p);
+        return b >= 0 ? b : nearest_exact<DF>(m, lon_p, lat_p);
     }
     return nearest_widened(m, lon_p, lat_p);
 }
@@ -764,10 +937,13 @@ __device__ __forceinline__ bool elem_touches(const DMesh& m, int c, int nn)
     return MAXV == 4 && e.w == nn;
 }
 
-// The rare 2-ring pass of core/mesh.py:2440-2463, kept out of line so the
-// common 1-ring hit stays a short straight path.
+// The 2-ring pass of core/mesh.py:2440-2463.  Out of line in the one-pass
+// kernels (so the common 1-ring hit stays a short straight path), inline in
+// the first phase of the two-phase step (DF), which has no calls at all: on
+// Delaunay meshes a point's nearest node is often not a vertex of its
+// triangle, so this pass is not rare there.
 template <int MAXV>
-__device__ __noinline__ int containing_2ring(const DMesh& m, double lon_p, double lat_p, int nn)
+__device__ __forceinline__ int containing_2ring_body(const DMesh& m, double lon_p, double lat_p, int nn)
 {
     int k0, k1;
     n2e_range(m, nn, k0, k1);
@@ -790,8 +966,14 @@ __device__ __noinline__ int containing_2ring(const DMesh& m, double lon_p, doubl
     return -1;
 }
 
-// core/mesh.py:2393 _find_containing_element_local
 template <int MAXV>
+__device__ __noinline__ int containing_2ring(const DMesh& m, double lon_p, double lat_p, int nn)
+{
+    return containing_2ring_body<MAXV>(m, lon_p, lat_p, nn);
+}
+
+// core/mesh.py:2393 _find_containing_element_local
+template <int MAXV, bool DF = false>
 __device__ inline int find_containing(const DMesh& m, double lon_p, double lat_p, int nn)
 {
     int k0, k1;
@@ -801,7 +983,10 @@ __device__ inline int find_containing(const DMesh& m, double lon_p, double lat_p
         int c = m.n2e[k];
         if (element_contains<MAXV>(m, c, lon_p, lat_p)) return c;
     }
-    return containing_2ring<MAXV>(m, lon_p, lat_p, nn);
+    if constexpr (DF)
+        return containing_2ring_body<MAXV>(m, lon_p, lat_p, nn);
+    else
+        return containing_2ring<MAXV>(m, lon_p, lat_p, nn);
 }
 
 // ---------------------------------------------------------------------------
