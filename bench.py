@@ -122,6 +122,7 @@ class Global3D:
         self.beach = dict(criteria=["on_failure", "stochastic"], t_beach=86400.0,
                           depth_min=0.3, seed=20240501)
         self.sample_ts = not args.no_sample
+        self.cell_rule = args.cell_rule
         self.storage = getattr(args, "storage", "f32")
         self.field_dtype = self.storage
         if self.storage == "bf16":
@@ -150,6 +151,7 @@ class Global3D:
 
     def upload(self, ctx):
         ctx.upload_mesh(self.w.mesh)
+        self.cell_report = apply_cell_rule(ctx, self.w.mesh, self.cell_rule)
         if self.ring:
             # the host side of the ring: every snapshot page-locked, as a
             # reader decoding into pinned buffers would leave it
@@ -287,6 +289,19 @@ class DoubleGyre:
 
         return cpu_legs(lambda: OracleMesh(w.mesh, "glibc_omp"), run, w.lon.size, 10_000, budget_s,
                         "of the same workload")
+
+
+def apply_cell_rule(ctx, mesh, rule):
+    """The grid-hash cell of ``mesh`` by ``rule`` as run_tracking chooses it
+    (occupancy, the reference's default, or legacy: the automatic cell),
+    the mesh re-uploaded when the cell changes; returns the cell used."""
+    from kelp_amd.cellrule import apply_grid_hash_cell_rule
+    before = float(mesh.gh_cell_size)
+    apply_grid_hash_cell_rule(mesh, ctx, rule)
+    if float(mesh.gh_cell_size) != before:
+        ctx.upload_mesh(mesh)
+    log(f"grid-hash cell ({rule}): {mesh.gh_cell_size:.4f} deg (automatic {before:.4f})")
+    return {"rule": rule, "cell_deg": float(mesh.gh_cell_size), "automatic_cell_deg": before}
 
 
 WORKLOADS = {"global_3d": Global3D, "double_gyre": DoubleGyre}
@@ -530,7 +545,7 @@ def run_precision(args):
                    "integrator": "rk2", "dt_s": dt, "kh_m2_s": kh, "particles_rate": n_rate,
                    "particles_error": n_err, "steps_error_run": n_steps, "resort_every": args.resort_every},
         "roofline": {"bound": "hbm", "achieved": f32["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": f32["frac_of_hbm_peak"], "traffic": None, "kernel": "step_fast_kernel+step_rare_kernel",
+                     "frac": f32["frac_of_hbm_peak"], "traffic": None, "kernel": "step_kernel",
                      "kernel_avg_us": f32["kernel_avg_us"], "b_alg_per_particle_step": f32["b_alg_per_particle_step"]},
         "by_storage": rates,
         "error_vs_f64": errors,
@@ -652,6 +667,8 @@ def main():
     ap.add_argument("--nodes", type=int, default=0)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cell-rule", default="occupancy", choices=["occupancy", "legacy"],
+                    help="global_3d: the grid-hash cell rule (run_tracking's default: occupancy)")
     ap.add_argument("--resort-every", type=int, default=8,
                     help="re-sort particles by grid-hash cell every N steps (0: never)")
     ap.add_argument("--ring", type=int, default=0,
@@ -747,15 +764,12 @@ def main():
         "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "field_dtype": getattr(wl, "field_dtype", "f32"),
         "data": "synthetic",
-        "config": dict(wl.config, alive_end=alive_end, resort_every=args.resort_every),
+        "config": dict(wl.config, alive_end=alive_end, resort_every=args.resort_every,
+                       **({"grid_hash": wl.cell_report} if getattr(wl, "cell_report", None) else {})),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_source": traffic_src,
-                     # kelp_step's kernels on the context stream, between
-                     # the events: the common-path pass (step_fast_kernel)
-                     # and the compacted pass over its deferred particles
-                     # (step_rare_kernel)
-                     "kernel": "step_fast_kernel+step_rare_kernel", "kernel_avg_us": avg_launch_s * 1e6,
+                     "kernel": "step_kernel", "kernel_avg_us": avg_launch_s * 1e6,
                      "b_alg_per_particle_step": wl.b_alg,
                      "particle_steps_per_launch": per_launch_units},
     }

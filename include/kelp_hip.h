@@ -389,19 +389,10 @@ int kelp_sample_vars(kelp_ctx *ctx, const kelp_particles *p, int n_vars, const i
                      int slot_a, int slot_b, double alpha, int method, double *out);
 
 /* Advance every particle by one step, in place, on the context stream.
- * Node-layout u/v on z-level meshes: one kernel in two phases -- every
- * particle's step without the exact search's rectangle widening (its
- * workgroup's grid-hash cells staged in LDS), then, in the wavefronts whose
- * ballot found any, the particles that needed the widening stepped again in
- * full from their unchanged state; same results as one phase.  The first
- * such step on a context allocates a 256-byte counter. */
+ * Each workgroup stages the grid-hash cells around its particles in LDS
+ * (unless KELP_GH_TILE=0 was set when the context was created); same
+ * results either way. */
 int kelp_step(kelp_ctx *ctx, const kelp_particles *p, const kelp_step_args *a);
-
-/* Diagnostics of the last kelp_step: *out = the particles its second phase
- * stepped, or -1 when it ran the one-phase kernel (element layout, sigma
- * meshes, rk45).  Waits for the context stream.  No counterpart in the
- * reference. */
-int kelp_step_deferred(kelp_ctx *ctx, int64_t *out);
 
 /* Reorder every particle array in place by the grid-hash cell of each
  * particle (alive first, then beached, then the rest) so that the step's

@@ -146,7 +146,6 @@ SIGNATURES = {
                                  c_double, c_int, c_void_p]),
     "kelp_step": (c_int, [c_void_p, POINTER(Particles), POINTER(StepArgs)]),
     "kelp_sort_particles": (c_int, [c_void_p, POINTER(Particles)]),
-    "kelp_step_deferred": (c_int, [c_void_p, POINTER(ctypes.c_int64)]),
     "kelp_sort_particles_noalloc": (c_int, [c_void_p, POINTER(Particles)]),
     "kelp_nearest": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_int, c_void_p]),
     "kelp_containing": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p]),

@@ -57,16 +57,6 @@ constexpr int32_t kOutside = 3;
 constexpr int8_t kFrInterp = 1;
 constexpr int8_t kFrOOB = 2;
 
-// What the step reads about an anchor node, in one 64-byte record per node
-// (internal order) instead of four arrays: the cap test's longitude and the
-// sine and cosine of its latitude (the z of its unit vector and its coslat
-// entry, the same bits), its bottom depth and its element-list range.
-struct alignas(64) NodeRec {
-    double lon, sinlat, coslat, depth;
-    int32_t e0, e1;
-    double lat, pad;
-};
-
 struct DMesh {
     const double2* __restrict__ lonlat;
     const double4* __restrict__ gh_xyz;
@@ -83,7 +73,6 @@ struct DMesh {
     const uint8_t* __restrict__ eflags;   // (E,) bit 0: the interpolation ring is the containment ring
     const int4* __restrict__ enbr;        // (E,) neighbour across edge k of the compacted ring, -1 boundary
                                           //   (core/mesh.py:2824 build_elem_neighbors); null without it
-    const NodeRec* __restrict__ nrec;     // (N,) anchor record per node
     double cell, lon_min, lat_min, r_earth;
     int32_t nx, ny, wrap, far_rows;
     int32_t n_nodes, n_elems, max_v, n_levels;
@@ -92,13 +81,14 @@ struct DMesh {
 
 __device__ __forceinline__ double sqd(double x) { return x * x; }
 
-// Node nn's element list n2e[k0 .. k1), from its anchor record: the line
-// the anchor search has just read for the cap test, not a separate offsets
-// array.
+// Node nn's element list n2e[k0 .. k1), from the w of its grid-hash entry
+// (internal node id = CSR slot, so gh_xyz[nn] is the node's entry): the
+// line the anchor search has just read, not a separate offsets array.
 __device__ __forceinline__ void n2e_range(const DMesh& m, int nn, int& k0, int& k1)
 {
-    k0 = m.nrec[nn].e0;
-    k1 = m.nrec[nn].e1;
+    const double w = m.gh_xyz[nn].w;
+    k0 = __double2loint(w);
+    k1 = __double2hiint(w);
 }
 
 // core/mesh.py:1193 _gh_fold_dlon
@@ -159,14 +149,6 @@ __device__ inline Query make_query(double lon_p, double lat_p)
 // order, same strict `<`: the answer is the same node either way.
 // ---------------------------------------------------------------------------
 
-// Deferred rare paths (template flag DF of the search functions): with DF
-// the code has no call to the exact search's rectangle widening (an
-// out-of-line function whose call sites inside the step make the compiler
-// spill the step's registers).  A lane that needs it marks itself in the
-// tile's per-lane flags and takes a stand-in answer that keeps every later
-// read in bounds; the step kernel drops that lane's results and steps it
-// again with the full search (DF false) from its unchanged state.
-
 constexpr int kTileRows = 8;       // cell rows a tile can hold
 constexpr int kTileCols = 64;      // cells per row
 constexpr int kTileNodes = 1536;   // nodes (24 B each in LDS)
@@ -177,15 +159,12 @@ struct GhTile {
     int32_t delta[kTileRows];               // LDS slot - CSR slot, per row
     int32_t off[kTileRows][kTileCols + 1];  // gh_offsets of the box's cells (+ the row end)
     double x[kTileNodes], y[kTileNodes], z[kTileNodes];
-    uint8_t defer[256];                     // per lane: the exact search's widening was needed (DF)
 };
 
 extern __shared__ __align__(16) unsigned char kelp_dyn_lds[];
 
 __device__ __forceinline__ GhTile& gh_tile() { return *reinterpret_cast<GhTile*>(kelp_dyn_lds); }
 
-// this lane needs a rare path the kernel was built without
-__device__ __forceinline__ void mark_deferred() { gh_tile().defer[threadIdx.x] = 1; }
 
 __device__ __forceinline__ int wave_min_i32(int v)
 {
@@ -283,6 +262,40 @@ __device__ inline void stage_gh_tile(const DMesh& m, bool alive, double lon_p, d
     __syncthreads();
 }
 
+// The same scan with eight loads in flight, for the exact search's
+// widening: its rectangles can span whole rows, and one load at a time
+// makes a lane that widens far a long serial tail of its wavefront.  Same
+// order and comparisons as scan_range.
+__device__ inline void scan_range_wide(const DMesh& m, int cj2, int c_lo, int c_hi, const Query& q, int& best,
+                                       double& best_d2)
+{
+    const int64_t row = (int64_t)cj2 * m.nx;
+    const int32_t start = m.gh_offsets[row + c_lo];
+    const int32_t end = m.gh_offsets[row + c_hi + 1];
+    int32_t k = start;
+    for (; k + 8 <= end; k += 8) {
+        double4 p[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) p[u] = m.gh_xyz[k + u];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            double d2 = sqd(p[u].x - q.x) + sqd(p[u].y - q.y) + sqd(p[u].z - q.z);
+            if (d2 < best_d2) {
+                best_d2 = d2;
+                best = k + u;
+            }
+        }
+    }
+    for (; k < end; ++k) {
+        double4 p = m.gh_xyz[k];
+        double d2 = sqd(p.x - q.x) + sqd(p.y - q.y) + sqd(p.z - q.z);
+        if (d2 < best_d2) {
+            best_d2 = d2;
+            best = k;
+        }
+    }
+}
+
 // core/mesh.py:1570 _gh_scan_row_range: cells c_lo..c_hi of row cj2 are one
 // contiguous CSR slice; strict `<` keeps the first node found on a tie.
 __device__ inline void scan_range(const DMesh& m, int cj2, int c_lo, int c_hi, const Query& q,
@@ -341,6 +354,31 @@ __device__ inline void scan_row_window(const DMesh& m, int cj2, int ci, int rx, 
     scan_range(m, cj2, c_lo, c_hi, q, best, best_d2);
 }
 
+// scan_row_window over scan_range_wide
+__device__ inline void scan_row_window_wide(const DMesh& m, int cj2, int ci, int rx, const Query& q,
+                                       int& best, double& best_d2)
+{
+    const int nx = m.nx;
+    if (m.wrap && 2 * rx + 1 >= nx) {
+        scan_range_wide(m, cj2, 0, nx - 1, q, best, best_d2);
+        return;
+    }
+    int c_lo = ci - rx, c_hi = ci + rx;
+    if (m.wrap) {
+        if (c_lo < 0) {
+            scan_range_wide(m, cj2, c_lo + nx, nx - 1, q, best, best_d2);
+            c_lo = 0;
+        } else if (c_hi > nx - 1) {
+            scan_range_wide(m, cj2, 0, c_hi - nx, q, best, best_d2);
+            c_hi = nx - 1;
+        }
+    } else {
+        if (c_lo < 0) c_lo = 0;
+        if (c_hi > nx - 1) c_hi = nx - 1;
+    }
+    scan_range_wide(m, cj2, c_lo, c_hi, q, best, best_d2);
+}
+
 // core/mesh.py:1236 _grid_hash_nearest_3x3 (best_d2 returned for the exact
 // search, which re-derives the same value from the table)
 __device__ inline int nearest_3x3(const DMesh& m, int ci, int cj, const Query& q, double& best_d2)
@@ -380,10 +418,9 @@ __device__ inline int nearest_3x3(const DMesh& m, int ci, int cj, const Query& q
 __device__ inline void cap_extents(const DMesh& m, const Query& q, double lon_p, double lat_p, int node,
                                    double& theta_deg, double& ext_lon)
 {
-    const NodeRec& nr = m.nrec[node];
-    const double lon_n = nr.lon;
+    const double lon_n = m.lonlat[node].x;
     const double sp = q.z, cp = q.cos_lat;
-    const double sn = nr.sinlat, cn = nr.coslat;
+    const double sn = m.gh_xyz[node].z, cn = m.coslat[node];
     double cosd = (sp * sn + cp * cn * km_cos((lon_p - lon_n) * kDeg2Rad));
     if (cosd > 1.0) cosd = 1.0;
     if (cosd < -1.0) cosd = -1.0;
@@ -513,7 +550,6 @@ __device__ inline int nearest_widened(const DMesh& m, double lon_p, double lat_p
 __device__ __noinline__ int exact_widen(const DMesh& m, double lon_p, double lat_p, int best,
                                         double best_d2, int code, int ci, int cj, const Query& q);
 
-template <bool DF = false>
 __device__ inline int nearest_exact(const DMesh& m, double lon_p, double lat_p)
 {
     const double cell = m.cell;
@@ -533,12 +569,7 @@ __device__ inline int nearest_exact(const DMesh& m, double lon_p, double lat_p)
         if (code == 0) return best;
         if (theta_deg > far) return best;
     }
-    if constexpr (DF) {
-        mark_deferred();
-        return best >= 0 ? best : 0;
-    } else {
-        return exact_widen(m, lon_p, lat_p, best, best_d2, code, ci, cj, q);
-    }
+    return exact_widen(m, lon_p, lat_p, best, best_d2, code, ci, cj, q);
 }
 
 __device__ __noinline__ int exact_widen(const DMesh& m, double lon_p, double lat_p, int best,
@@ -630,21 +661,21 @@ __device__ __noinline__ int exact_widen(const DMesh& m, double lon_p, double lat
                     int c_lo = pyimod(ci + rx + 1, nx);
                     int c_hi = pyimod(ci - rx - 1, nx);
                     if (c_lo <= c_hi) {
-                        scan_range(m, cj2, c_lo, c_hi, q, best, best_d2);
+                        scan_range_wide(m, cj2, c_lo, c_hi, q, best, best_d2);
                     } else {
-                        scan_range(m, cj2, c_lo, nx - 1, q, best, best_d2);
-                        scan_range(m, cj2, 0, c_hi, q, best, best_d2);
+                        scan_range_wide(m, cj2, c_lo, nx - 1, q, best, best_d2);
+                        scan_range_wide(m, cj2, 0, c_hi, q, best, best_d2);
                     }
                 }
             } else if (!wrap && rx_new - rx > 1) {
                 for (int cj2 = j_lo; cj2 < j_hi + 1; ++cj2) {
                     if (ci - rx - 1 >= 0) {
                         int c_lo = ci - rx_new > 0 ? ci - rx_new : 0;
-                        scan_range(m, cj2, c_lo, ci - rx - 1, q, best, best_d2);
+                        scan_range_wide(m, cj2, c_lo, ci - rx - 1, q, best, best_d2);
                     }
                     if (ci + rx + 1 <= nx - 1) {
                         int c_hi = ci + rx_new < nx - 1 ? ci + rx_new : nx - 1;
-                        scan_range(m, cj2, ci + rx + 1, c_hi, q, best, best_d2);
+                        scan_range_wide(m, cj2, ci + rx + 1, c_hi, q, best, best_d2);
                     }
                 }
             } else {
@@ -654,15 +685,15 @@ __device__ __noinline__ int exact_widen(const DMesh& m, double lon_p, double lat
                         int two = (2 * r + 1 <= nx);
                         int c_right = pyimod(ci + r, nx);
                         for (int cj2 = j_lo; cj2 < j_hi + 1; ++cj2) {
-                            scan_range(m, cj2, c_left, c_left, q, best, best_d2);
-                            if (two) scan_range(m, cj2, c_right, c_right, q, best, best_d2);
+                            scan_range_wide(m, cj2, c_left, c_left, q, best, best_d2);
+                            if (two) scan_range_wide(m, cj2, c_right, c_right, q, best, best_d2);
                         }
                     } else {
                         for (int s = 0; s < 2; ++s) {
                             int c2 = s == 0 ? ci - r : ci + r;
                             if (c2 < 0 || c2 > nx - 1) continue;
                             for (int cj2 = j_lo; cj2 < j_hi + 1; ++cj2)
-                                scan_range(m, cj2, c2, c2, q, best, best_d2);
+                                scan_range_wide(m, cj2, c2, c2, q, best, best_d2);
                         }
                     }
                 }
@@ -672,7 +703,7 @@ __device__ __noinline__ int exact_widen(const DMesh& m, double lon_p, double lat
             for (int s = 0; s < 2; ++s) {
                 int cj2 = s == 0 ? cj - r : cj + r;
                 if (cj2 < 0 || cj2 > ny - 1) continue;
-                scan_row_window(m, cj2, ci, rx_new, q, best, best_d2);
+                scan_row_window_wide(m, cj2, ci, rx_new, q, best, best_d2);
             }
         }
         rx = rx_new;
@@ -685,17 +716,16 @@ __device__ __noinline__ int exact_widen(const DMesh& m, double lon_p, double lat
 // (integrators.py:1471 _fused_site_anchor); 2: the batch guard-off anchor of
 // core/mesh.py:1896 find_nearest_nodes (3x3 window, an empty window healed
 // with the true nearest node).
-template <bool DF = false>
 __device__ inline int site_anchor(const DMesh& m, double lon_p, double lat_p, int mode)
 {
-    if (mode == 1) return nearest_exact<DF>(m, lon_p, lat_p);
+    if (mode == 1) return nearest_exact(m, lon_p, lat_p);
     if (mode == 2) {
         int ci, cj;
         gh_cell_of(m, lon_p, lat_p, ci, cj);
         Query q = make_query(lon_p, lat_p);
         double d2;
         int b = nearest_3x3(m, ci, cj, q, d2);
-        return b >= 0 ? b : nearest_exact<DF>(m, lon_p, lat_p);
+        return b >= 0 ? b : nearest_exact(m, lon_p, lat_p);
     }
     return nearest_widened(m, lon_p, lat_p);
 }
@@ -937,13 +967,10 @@ __device__ __forceinline__ bool elem_touches(const DMesh& m, int c, int nn)
     return MAXV == 4 && e.w == nn;
 }
 
-// The 2-ring pass of core/mesh.py:2440-2463.  Out of line in the one-pass
-// kernels (so the common 1-ring hit stays a short straight path), inline in
-// the first phase of the two-phase step (DF), which has no calls at all: on
-// Delaunay meshes a point's nearest node is often not a vertex of its
-// triangle, so this pass is not rare there.
+// The rare 2-ring pass of core/mesh.py:2440-2463, kept out of line so the
+// common 1-ring hit stays a short straight path.
 template <int MAXV>
-__device__ __forceinline__ int containing_2ring_body(const DMesh& m, double lon_p, double lat_p, int nn)
+__device__ __noinline__ int containing_2ring(const DMesh& m, double lon_p, double lat_p, int nn)
 {
     int k0, k1;
     n2e_range(m, nn, k0, k1);
@@ -966,14 +993,8 @@ __device__ __forceinline__ int containing_2ring_body(const DMesh& m, double lon_
     return -1;
 }
 
-template <int MAXV>
-__device__ __noinline__ int containing_2ring(const DMesh& m, double lon_p, double lat_p, int nn)
-{
-    return containing_2ring_body<MAXV>(m, lon_p, lat_p, nn);
-}
-
 // core/mesh.py:2393 _find_containing_element_local
-template <int MAXV, bool DF = false>
+template <int MAXV>
 __device__ inline int find_containing(const DMesh& m, double lon_p, double lat_p, int nn)
 {
     int k0, k1;
@@ -983,10 +1004,7 @@ __device__ inline int find_containing(const DMesh& m, double lon_p, double lat_p
         int c = m.n2e[k];
         if (element_contains<MAXV>(m, c, lon_p, lat_p)) return c;
     }
-    if constexpr (DF)
-        return containing_2ring_body<MAXV>(m, lon_p, lat_p, nn);
-    else
-        return containing_2ring<MAXV>(m, lon_p, lat_p, nn);
+    return containing_2ring<MAXV>(m, lon_p, lat_p, nn);
 }
 
 // ---------------------------------------------------------------------------

@@ -140,27 +140,6 @@ __global__ void node_xyz_kernel(int64_t n, const double2* __restrict__ lonlat, c
     out[k] = r;
 }
 
-// the anchor record of every node (NodeRec), from the arrays built above
-__global__ void node_rec_kernel(int64_t n, const double2* __restrict__ lonlat, const double4* __restrict__ xyz,
-                                const double* __restrict__ coslat, const double* __restrict__ depth,
-                                NodeRec* __restrict__ out)
-{
-    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const double2 ll = lonlat[k];
-    const double4 p = xyz[k];
-    NodeRec r;
-    r.lon = ll.x;
-    r.sinlat = p.z;
-    r.coslat = coslat[k];
-    r.depth = depth[k];
-    r.e0 = __double2loint(p.w);
-    r.e1 = __double2hiint(p.w);
-    r.lat = ll.y;
-    r.pad = 0.0;
-    out[k] = r;
-}
-
 // containment frames of all elements (elem_geom)
 template <int MAXV>
 __global__ void elem_geom_kernel(DMesh m, double4* __restrict__ eframe, uint8_t* __restrict__ eflags)
@@ -542,27 +521,17 @@ static Snap<T, IS3D> snap_of(const void* a, const void* b, const void* t)
     return f;
 }
 
-// A z-level step: node-layout kernels run in two phases (the common path,
-// then the lanes that needed the exact search's widening) and count the
-// second phase's particles into this step's counter slot; the two slots
-// alternate by step, each step's kernel clearing the next one's.
+// A z-level step, the context's grid-hash tile setting in the mesh view
 template <bool IS3D, typename T, bool ELEM>
 static void launch_step_snaps(kelp_ctx* c, int order, const Snap<T, IS3D>& f, const Snap<T, false>& fe,
                               const StepCfg& cfg, const PState& ps)
 {
     DMesh mt = c->m;
     mt.tile_on = c->gh_tile ? 1 : 0;
-    uint32_t *count = nullptr, *clear = nullptr;
-    if (!ELEM && c->rare_count.p) {
-        c->rare_slot ^= 1;
-        count = static_cast<uint32_t*>(c->rare_count.p) + c->rare_slot;
-        clear = static_cast<uint32_t*>(c->rare_count.p) + (c->rare_slot ^ 1);
-        c->last_two_pass = true;
-    }
     if (c->m.max_v == 3)
-        launch_step_v<3, IS3D, T, ELEM>(c->stream, order, mt, f, fe, cfg, ps, count, clear);
+        launch_step_v<3, IS3D, T, ELEM>(c->stream, order, mt, f, fe, cfg, ps);
     else
-        launch_step_v<4, IS3D, T, ELEM>(c->stream, order, mt, f, fe, cfg, ps, count, clear);
+        launch_step_v<4, IS3D, T, ELEM>(c->stream, order, mt, f, fe, cfg, ps);
 }
 
 template <bool IS3D, typename T, bool ELEM>
@@ -737,7 +706,7 @@ int kelp_ctx_destroy(kelp_ctx* c)
     free_mesh(c);
     for (auto& s : c->slots) free_slot(s);
     for (DevBuf* b : {&c->sort_tmp, &c->sort_keys, &c->sort_keys2, &c->sort_idx, &c->sort_perm, &c->sort_stage,
-                      &c->batch_clouds, &c->batch_steps, &c->batch_counter, &c->rare_count})
+                      &c->batch_clouds, &c->batch_steps, &c->batch_counter})
         kelp_release(*b);
     if (c->batch_graph) (void)hipGraphExecDestroy(c->batch_graph);
     for (int k = 0; k < kMaxEvents; ++k)
@@ -919,13 +888,6 @@ int kelp_mesh_upload(kelp_ctx* c, const kelp_mesh_desc* d)
     if (d_enbr) HIP_TRY(hipMemcpy(d_enbr, enbr.data(), E * sizeof(int4), hipMemcpyHostToDevice));
     node_xyz_kernel<<<grid_for(N, 256), 256, 0, c->stream>>>(N, d_lonlat, d_n2e_off, d_xyz, d_coslat);
     HIP_TRY(hipGetLastError());
-    NodeRec* d_nrec = nullptr;
-    if ((rc = dev_alloc(c, &d_nrec, N))) {
-        free_mesh(c);
-        return rc;
-    }
-    node_rec_kernel<<<grid_for(N, 256), 256, 0, c->stream>>>(N, d_lonlat, d_xyz, d_coslat, d_depth, d_nrec);
-    HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(c->stream));
 
     m.lonlat = d_lonlat;
@@ -942,7 +904,6 @@ int kelp_mesh_upload(kelp_ctx* c, const kelp_mesh_desc* d)
     m.eframe = d_eframe;
     m.eflags = d_eflags;
     m.enbr = d_enbr;
-    m.nrec = d_nrec;
     m.cell = d->gh_cell;
     m.lon_min = d->gh_lon_min;
     m.lat_min = d->gh_lat_min;
@@ -1714,27 +1675,12 @@ int kelp_step_prepare(kelp_ctx* c, const kelp_particles* p, const kelp_step_args
     return KELP_OK;
 }
 
-int kelp_step_deferred(kelp_ctx* c, int64_t* out)
-{
-    if (!c || !out) return fail(KELP_EINVAL, "kelp_step_deferred: NULL argument");
-    *out = -1;
-    if (!c->last_two_pass || !c->rare_count.p) return KELP_OK;
-    HIP_TRY(hipSetDevice(c->device));
-    uint32_t n = 0;
-    HIP_TRY(hipMemcpyAsync(&n, static_cast<uint32_t*>(c->rare_count.p) + c->rare_slot, sizeof(n),
-                           hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    *out = (int64_t)n;
-    return KELP_OK;
-}
-
 int kelp_step(kelp_ctx* c, const kelp_particles* p, const kelp_step_args* a)
 {
     StepCfg cfg{};
     PState ps{};
     int rc = kelp_step_prepare(c, p, a, cfg, ps);
     if (rc) return rc;
-    c->last_two_pass = false;
     if (p->n == 0) return KELP_OK;
     const bool sigma = c->vertical_coord == 1;
     const FieldSlot& sa = c->slots[a->slot_a];
@@ -1768,11 +1714,6 @@ int kelp_step(kelp_ctx* c, const kelp_particles* p, const kelp_step_args* a)
         }
         HIP_TRY(hipGetLastError());
         return KELP_OK;
-    }
-    if (!sigma && sa.layout != 1 && !c->rare_count.p) {
-        // the two-phase step's counter (diagnostics, kelp_step_deferred)
-        if ((rc = kelp_reserve(c->rare_count, 256))) return rc;
-        HIP_TRY(hipMemsetAsync(c->rare_count.p, 0, 256, c->stream));
     }
     if (sa.dtype == KELP_DT_F16S) {
         // node-layout z-level only (refused at upload otherwise)

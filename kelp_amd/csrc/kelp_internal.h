@@ -92,9 +92,6 @@ struct kelp_ctx {
     bool sig_given[2] = {false, false};
     FieldSlot slots[kMaxSlots];
     DevBuf sort_tmp, sort_keys, sort_keys2, sort_idx, sort_perm, sort_stage;
-    DevBuf rare_count;             // node-layout steps: second-phase particles, one slot per step parity
-    int rare_slot = 0;             // the slot of the last step
-    bool last_two_pass = false;    // the last kelp_step ran the two-phase kernel
     bool gh_tile = true;           // plain steps stage grid-hash tiles in LDS (KELP_GH_TILE=0: off)
     // batched clouds (kelp_batch_*): the cloud table, the per-cloud step
     // records the bracket kernel writes, the device step counter, and the

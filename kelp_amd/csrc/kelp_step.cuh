@@ -67,12 +67,9 @@ struct PState {
 // `stream`.  `f`: the node snapshot records (u, v[, w pair]); `fe`: the
 // element-centre u/v records when ELEM (u/v at element centres, least
 // squares), else unused.  m.tile_on: stage grid-hash tiles in LDS.
-// rare_count (or null): incremented by the particles the node-layout
-// kernel's second phase steps again; rare_clear (or null): set to zero.
 template <int MAXV, bool IS3D, typename T, bool ELEM>
 void launch_step_v(hipStream_t stream, int order, const DMesh& m, const Snap<T, IS3D>& f,
-                   const Snap<T, false>& fe, const StepCfg& cfg, const PState& ps, uint32_t* rare_count,
-                   uint32_t* rare_clear);
+                   const Snap<T, false>& fe, const StepCfg& cfg, const PState& ps);
 
 // The adaptive Dormand-Prince 5(4) step (2-D, u/v at element centres):
 // launch rk45_kernel<MAXV, T>.

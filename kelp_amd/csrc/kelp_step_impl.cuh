@@ -28,7 +28,7 @@ __device__ __forceinline__ double z0_of(double lat)
 __device__ __forceinline__ double clamp_z(const DMesh& m, double z, int nn)
 {
     if (z < m.zlev[0]) z = m.zlev[0];
-    double bottom = m.nrec[nn].depth;
+    double bottom = m.depth[nn];
     if (z > bottom) z = bottom;
     return z;
 }
@@ -36,7 +36,7 @@ __device__ __forceinline__ double clamp_z(const DMesh& m, double z, int nn)
 // interpolate.py:1341/:1444 _interp_barycentric_core_tinterp(_3d):
 // containment around the anchor, vertex weights, blended vertex values of
 // the layer plane starting at `off`.
-template <int MAXV, typename T, bool IS3D, bool DF = false>
+template <int MAXV, typename T, bool IS3D>
 __device__ inline bool interp_uv(const DMesh& m, const Snap<T, IS3D>& f, double alpha0, double alpha,
                                  double time_dir, double xp, double yp, int nn, int64_t off, double& U,
                                  double& V, int ehint = -1)
@@ -45,7 +45,7 @@ __device__ inline bool interp_uv(const DMesh& m, const Snap<T, IS3D>& f, double 
     V = 0.0;
     // a containing element validated for this position (the edge walk,
     // interpolate.py:1266) skips the search
-    int e = ehint >= 0 ? ehint : find_containing<MAXV, DF>(m, xp, yp, nn);
+    int e = ehint >= 0 ? ehint : find_containing<MAXV>(m, xp, yp, nn);
     if (e < 0) return false;
     Weights W;
     if (!vertex_weights<MAXV>(m, e, xp, yp, W)) return false;
@@ -68,21 +68,20 @@ __device__ inline bool interp_uv(const DMesh& m, const Snap<T, IS3D>& f, double 
 // that layer (node layout: barycentric in the containing element; element
 // layout: least squares over the anchor's element centres), W by the
 // configured method (integrators.py:1260 _compute_w).
-template <int MAXV, bool IS3D, typename T, bool ELEM, bool DF = false>
+template <int MAXV, bool IS3D, typename T, bool ELEM>
 __device__ inline bool eval_site(const DMesh& m, const Snap<T, IS3D>& f, const Snap<T, false>& fe,
                                  const StepCfg& c, double alpha, double lon, double lat, double zp, int& nn,
                                  double& U, double& V, double& W, int ehint = -1)
 {
     // nn >= 0 on entry: the anchor of (lon, lat) under this mode is known
-    if (nn < 0) nn = site_anchor<DF>(m, lon, lat, c.exact);
+    if (nn < 0) nn = site_anchor(m, lon, lat, c.exact);
     int64_t layer = 0;
-    if (IS3D) layer = find_layer(m, zp, m.nrec[nn].depth);
+    if (IS3D) layer = find_layer(m, zp, m.depth[nn]);
     bool ok;
     if constexpr (ELEM)
         ok = interp_lsq(m, fe, c.alpha0, alpha, c.time_dir, lon, lat, nn, layer * m.n_elems, U, V);
     else
-        ok = interp_uv<MAXV, T, IS3D, DF>(m, f, c.alpha0, alpha, c.time_dir, lon, lat, nn, layer * m.n_nodes, U, V,
-                                          ehint);
+        ok = interp_uv<MAXV>(m, f, c.alpha0, alpha, c.time_dir, lon, lat, nn, layer * m.n_nodes, U, V, ehint);
     W = 0.0;
     if constexpr (IS3D) {
         if (c.w_method == 0)
@@ -114,7 +113,7 @@ __device__ __noinline__ int walk_at(const DMesh& m, double lon, double lat, int 
 // :5415-5417, :5792-5794) every site's containing element comes from the
 // edge walk started at the previous site's (the carried element at site 1)
 // and the end-of-step domain verdict from the walk too; `elem` is carried.
-template <int ORDER, int MAXV, bool IS3D, typename T, bool ELEM, bool DF = false>
+template <int ORDER, int MAXV, bool IS3D, typename T, bool ELEM>
 __device__ inline int det_step(const DMesh& m, const Snap<T, IS3D>& f, const Snap<T, false>& fe, const StepCfg& c,
                                double& lon, double& lat, double& z, int& nn, double& Uo, double& Vo, double& Wo,
                                int& elem)
@@ -128,7 +127,7 @@ __device__ inline int det_step(const DMesh& m, const Snap<T, IS3D>& f, const Sna
     node_i = (c.reuse_anchor && nn >= 0) ? nn : -1;
     const int e1 = c.walk ? walk_at<MAXV>(m, lon0, lat0, elem) : -1;
     int e_last = e1;
-    bool ok = eval_site<MAXV, IS3D, T, ELEM, DF>(m, f, fe, c, c.alphas[0], lon0, lat0, zz0, node_i, U1, V1, W1, e1);
+    bool ok = eval_site<MAXV, IS3D, T, ELEM>(m, f, fe, c, c.alphas[0], lon0, lat0, zz0, node_i, U1, V1, W1, e1);
     Uo = U1;
     Vo = V1;
     Wo = W1;
@@ -146,7 +145,7 @@ __device__ inline int det_step(const DMesh& m, const Snap<T, IS3D>& f, const Sna
             int nn_mid = -1;
             double U2, V2, W2;
             if (c.walk) e_last = walk_at<MAXV>(m, lon_mid, lat_mid, e_last);
-            ok = eval_site<MAXV, IS3D, T, ELEM, DF>(m, f, fe, c, c.alphas[1], lon_mid, lat_mid, z_mid, nn_mid, U2, V2, W2,
+            ok = eval_site<MAXV, IS3D, T, ELEM>(m, f, fe, c, c.alphas[1], lon_mid, lat_mid, z_mid, nn_mid, U2, V2, W2,
                                                 e_last);
             if (ok) {
                 double z0m = z0_of(lat_mid);
@@ -164,7 +163,7 @@ __device__ inline int det_step(const DMesh& m, const Snap<T, IS3D>& f, const Sna
             int nn2 = -1, nn3 = -1, nn4 = -1;
             double U2, V2, W2, U3, V3, W3, U4, V4, W4;
             if (c.walk) e_last = walk_at<MAXV>(m, lon2, lat2, e_last);
-            ok = eval_site<MAXV, IS3D, T, ELEM, DF>(m, f, fe, c, c.alphas[1], lon2, lat2, z2, nn2, U2, V2, W2, e_last);
+            ok = eval_site<MAXV, IS3D, T, ELEM>(m, f, fe, c, c.alphas[1], lon2, lat2, z2, nn2, U2, V2, W2, e_last);
             if (ok) {
                 double dlon2 = dt * U2 / z0_of(lat2);
                 double dlat2 = dt * V2 / kSLM;
@@ -173,7 +172,7 @@ __device__ inline int det_step(const DMesh& m, const Snap<T, IS3D>& f, const Sna
                 double lat3 = clamp_lat(lat0 + 0.5 * dlat2);
                 double z3 = zz0 + 0.5 * dz2;
                 if (c.walk) e_last = walk_at<MAXV>(m, lon3, lat3, e_last);
-                ok = eval_site<MAXV, IS3D, T, ELEM, DF>(m, f, fe, c, c.alphas[2], lon3, lat3, z3, nn3, U3, V3, W3,
+                ok = eval_site<MAXV, IS3D, T, ELEM>(m, f, fe, c, c.alphas[2], lon3, lat3, z3, nn3, U3, V3, W3,
                                                     e_last);
                 if (ok) {
                     double dlon3 = dt * U3 / z0_of(lat3);
@@ -183,7 +182,7 @@ __device__ inline int det_step(const DMesh& m, const Snap<T, IS3D>& f, const Sna
                     double lat4 = clamp_lat(lat0 + dlat3);
                     double z4 = zz0 + dz3;
                     if (c.walk) e_last = walk_at<MAXV>(m, lon4, lat4, e_last);
-                    ok = eval_site<MAXV, IS3D, T, ELEM, DF>(m, f, fe, c, c.alphas[3], lon4, lat4, z4, nn4, U4, V4,
+                    ok = eval_site<MAXV, IS3D, T, ELEM>(m, f, fe, c, c.alphas[3], lon4, lat4, z4, nn4, U4, V4,
                                                         W4, e_last);
                     if (ok) {
                         double dlon4 = dt * U4 / z0_of(lat4);
@@ -205,8 +204,8 @@ __device__ inline int det_step(const DMesh& m, const Snap<T, IS3D>& f, const Sna
         if (c.walk) elem = e1;
         return kFrInterp;
     }
-    int nn_new = site_anchor<DF>(m, lon_new, lat_new, c.exact);
-    int e = c.walk ? walk_at<MAXV>(m, lon_new, lat_new, e_last) : find_containing<MAXV, DF>(m, lon_new, lat_new, nn_new);
+    int nn_new = site_anchor(m, lon_new, lat_new, c.exact);
+    int e = c.walk ? walk_at<MAXV>(m, lon_new, lat_new, e_last) : find_containing<MAXV>(m, lon_new, lat_new, nn_new);
     if (e < 0) {
         nn = node_i;
         if (c.walk) elem = e1;
@@ -225,7 +224,7 @@ __device__ inline int det_step(const DMesh& m, const Snap<T, IS3D>& f, const Sna
 // SIGMA: the particle keeps its sigma fraction in the end-of-step surface,
 // z / H_p(post-step position) * H_p(kicked position), clamped in the new
 // column (integrators.py:784-801).
-template <int MAXV, bool IS3D, bool SIGMA = false, bool DF = false>
+template <int MAXV, bool IS3D, bool SIGMA = false>
 __device__ inline void kick_one(const DMesh& m, const StepCfg& c, uint64_t pid, double pre_lon,
                                       double pre_lat, double pre_z, int pre_nn, double& lon, double& lat,
                                       double& z, int32_t& st, int& nn, int8_t& fr)
@@ -248,8 +247,8 @@ __device__ inline void kick_one(const DMesh& m, const StepCfg& c, uint64_t pid, 
     }
     double lon_k, lat_k;
     sphere_displace(lon, lat, dx, dy, lon_k, lat_k);
-    int nn_k = site_anchor<DF>(m, lon_k, lat_k, c.exact == 1 ? 1 : 2);  // batch find_nearest_nodes
-    int e = find_containing<MAXV, DF>(m, lon_k, lat_k, nn_k);
+    int nn_k = site_anchor(m, lon_k, lat_k, c.exact == 1 ? 1 : 2);  // batch find_nearest_nodes
+    int e = find_containing<MAXV>(m, lon_k, lat_k, nn_k);
     if (e < 0) {
         lon = pre_lon;
         lat = pre_lat;
@@ -286,7 +285,7 @@ __device__ inline void az_column(const DMesh& m, const double* __restrict__ az_a
     const double beta = 1.0 - alpha0;
     auto A = [&](int k) { return cb ? beta * ca[k] + alpha0 * cb[k] : ca[k]; };
     const double top = zl[0];
-    const double dn = m.nrec[node].depth;
+    const double dn = m.depth[node];
     double bot = dn < zl[K - 1] ? dn : zl[K - 1];
     if (bot < top) bot = top;
     double ze = z;
@@ -342,7 +341,7 @@ __device__ inline void az_one(const DMesh& m, const StepCfg& c, uint64_t pid, in
     const double sdt = c.time_dir * c.dt;
     const double abs_dt = sdt >= 0.0 ? sdt : -sdt;
     const double dz = g * abs_dt + km_sqrt(2.0 * az * abs_dt) * xi;
-    z = km_reflect_depth(z + dz, m.zlev[0], m.nrec[node].depth);
+    z = km_reflect_depth(z + dz, m.zlev[0], m.depth[node]);
 }
 
 // integrators.py:842 _apply_split_az_kick on a sigma mesh (constant source,
@@ -369,7 +368,6 @@ __device__ inline void az_sigma_one(const DMesh& m, const StepCfg& c, uint64_t p
 // coordinate outside [x0, x1) back with numpy's mod (Python % semantics),
 // strictly inside the window, and re-anchor a folded particle with the batch
 // search (find_nearest_nodes).
-template <bool DF = false>
 __device__ inline void periodic_one(const DMesh& m, const StepCfg& c, double& lon, double& lat, int& nn,
                                     int& elem)
 {
@@ -387,26 +385,26 @@ __device__ inline void periodic_one(const DMesh& m, const StepCfg& c, double& lo
         moved = true;
     }
     if (moved) {
-        nn = site_anchor<DF>(m, lon, lat, c.exact == 1 ? 1 : 2);
+        nn = site_anchor(m, lon, lat, c.exact == 1 ? 1 : 2);
         elem = -1;   // tracking.py:1143: a folded particle's carried element is dropped
     }
 }
 
 // tracking.py:199 _apply_beaching_hook + numerics/beaching.py:95
 // evaluate_beaching, one particle (prev = status at the step start).
-template <int MAXV, bool DF = false>
+template <int MAXV>
 __device__ inline void beach_one(const DMesh& m, const StepCfg& c, uint64_t pid, int32_t prev, double lon,
                                        double lat, int32_t& st, int& nn, int8_t& fr, int32_t& ws)
 {
     const bool need_topo = c.use_coastal || c.use_stochastic;
     const bool newly_failed = (st == kOutside) && (prev == kAlive);
-    if (newly_failed && nn < 0) nn = site_anchor<DF>(m, lon, lat, c.exact == 1 ? 1 : 2);
+    if (newly_failed && nn < 0) nn = site_anchor(m, lon, lat, c.exact == 1 ? 1 : 2);
     const int n = nn;
     int containing = -1;
     if (need_topo && (st == kAlive || st == kBeached || newly_failed) && n >= 0)
-        containing = find_containing<MAXV, DF>(m, lon, lat, n);
+        containing = find_containing<MAXV>(m, lon, lat, n);
     const bool coastal = need_topo && containing >= 0 && m.coastal != nullptr && m.coastal[containing];
-    double h_raw = n >= 0 ? m.nrec[n].depth : __builtin_inf();
+    double h_raw = n >= 0 ? m.depth[n] : __builtin_inf();
     // + vel.eta[nn] of the step's field: the bracket pair blended by alpha0
     // (core/io.py:386 LazyLerpField), or the plain snapshot
     if (c.use_eta && n >= 0) {
@@ -466,17 +464,11 @@ __device__ inline void beach_one(const DMesh& m, const StepCfg& c, uint64_t pid,
 // beaching), the state read and written at slot i.  `prev` is its status at
 // the step start; its random draws are keyed by pid[i] & key_mask (the
 // whole pid in a plain step; the cloud-local part in a batched one).
-// DF (the first phase of the two-phase step): returns false, having written
-// nothing, when the step needed the exact search's widening (or the edge
-// walk, which DF does not run); the caller steps it again without DF.
-template <int ORDER, int MAXV, bool IS3D, typename T, bool ELEM, bool DF = false>
-__device__ __forceinline__ bool step_particle(const DMesh& m, const Snap<T, IS3D>& f, const Snap<T, false>& fe,
+template <int ORDER, int MAXV, bool IS3D, typename T, bool ELEM>
+__device__ __forceinline__ void step_particle(const DMesh& m, const Snap<T, IS3D>& f, const Snap<T, false>& fe,
                                               const StepCfg& c, const PState& s, int64_t i, int32_t prev,
                                               uint64_t key_mask)
 {
-    if constexpr (DF) {
-        if (c.walk) return false;
-    }
     int32_t st = prev;
     double lon = s.lon[i], lat = s.lat[i];
     double z = IS3D ? s.z[i] : 0.0;
@@ -487,26 +479,18 @@ __device__ __forceinline__ bool step_particle(const DMesh& m, const Snap<T, IS3D
     if (prev == kAlive) {
         const double pre_lon = lon, pre_lat = lat, pre_z = z;
         const int pre_nn = nn;
-        int code = det_step<ORDER, MAXV, IS3D, T, ELEM, DF>(m, f, fe, c, lon, lat, z, nn, U, V, W, elem);
+        int code = det_step<ORDER, MAXV, IS3D, T, ELEM>(m, f, fe, c, lon, lat, z, nn, U, V, W, elem);
         if (code) {
             st = kOutside;
             fr = (int8_t)code;
         } else if (c.kick && !(c.kh_x == 0.0 && c.kh_y == 0.0)) {
-            kick_one<MAXV, IS3D, false, DF>(m, c, (uint64_t)s.pid[i] & key_mask, pre_lon, pre_lat, pre_z, pre_nn, lon,
-                                            lat, z, st, nn, fr);
+            kick_one<MAXV, IS3D>(m, c, (uint64_t)s.pid[i] & key_mask, pre_lon, pre_lat, pre_z, pre_nn, lon, lat, z,
+                                 st, nn, fr);
         }
         if (IS3D && c.az && st == kAlive) az_one(m, c, (uint64_t)s.pid[i] & key_mask, nn, z);
-        if ((c.periodic_lon || c.periodic_lat) && st == kAlive) periodic_one<DF>(m, c, lon, lat, nn, elem);
+        if ((c.periodic_lon || c.periodic_lat) && st == kAlive) periodic_one(m, c, lon, lat, nn, elem);
     }
-    int32_t ws = 0;
-    if (c.beach) {
-        ws = s.ws[i];
-        beach_one<MAXV, DF>(m, c, (uint64_t)s.pid[i] & key_mask, prev, lon, lat, st, nn, fr, ws);
-    }
-    if constexpr (DF) {
-        if (gh_tile().defer[threadIdx.x]) return false;
-    }
-    if (c.beach) s.ws[i] = ws;
+    if (c.beach) beach_one<MAXV>(m, c, (uint64_t)s.pid[i] & key_mask, prev, lon, lat, st, nn, fr, s.ws[i]);
     if (s.elem) s.elem[i] = elem;
     s.lon[i] = lon;
     s.lat[i] = lat;
@@ -517,7 +501,6 @@ __device__ __forceinline__ bool step_particle(const DMesh& m, const Snap<T, IS3D
     s.u[i] = U;
     s.v[i] = V;
     if (IS3D && s.w) s.w[i] = W;
-    return true;
 }
 
 // particles alive at the step start: one atomic per wavefront (the lanes
@@ -531,42 +514,19 @@ __device__ __forceinline__ void count_alive(unsigned long long* count, bool aliv
         atomicAdd(count, (unsigned long long)__popcll(alive));
 }
 
-// The plain step kernel.  Node-layout u/v: two phases in one launch.  The
-// first runs every particle's step without the exact search's rectangle
-// widening (DF: no out-of-line call in that code, so the step is compiled
-// without spilling); a wavefront ballot finds the lanes that needed it, and
-// those alone run their step again in full from the unchanged state (the
-// second phase, skipped by every wavefront whose ballot is empty), counted
-// in *rare_count (one atomic per such wavefront) when it is not null; the
-// counter of the next step (*rare_clear, the other of two slots) is
-// cleared here, so no extra launch clears it.  Element layout: one phase.  Launched with sizeof(GhTile) bytes of
-// dynamic LDS (the tile, and the per-lane flags of the first phase) unless
-// element layout without m.tile_on.
 template <int ORDER, int MAXV, bool IS3D, typename T, bool ELEM>
 __global__ __launch_bounds__(256) KELP_STEP_ATTR void step_kernel(DMesh m, Snap<T, IS3D> f, Snap<T, false> fe,
-                                                                  StepCfg c, PState s,
-                                                                  uint32_t* __restrict__ rare_count,
-                                                                  uint32_t* __restrict__ rare_clear)
+                                                                  StepCfg c, PState s)
 {
-    if (rare_clear && blockIdx.x == 0 && threadIdx.x == 0) *rare_clear = 0u;
     int64_t i = xcd_thread_index();
     const bool in = i < s.n;
     const int32_t prev = in ? s.status[i] : 0;
+    // every thread of the workgroup takes part in staging its grid-hash
+    // tile (workgroup barriers) before any returns
     if (m.tile_on) stage_gh_tile(m, prev == kAlive, in ? s.lon[i] : 0.0, in ? s.lat[i] : 0.0);
     if (!in) return;
     count_alive(s.alive_count, prev == kAlive);
-    if constexpr (!ELEM) {
-        gh_tile().defer[threadIdx.x] = 0;
-        const bool done = step_particle<ORDER, MAXV, IS3D, T, ELEM, true>(m, f, fe, c, s, i, prev, ~0ull);
-        const unsigned long long again = __ballot(!done);
-        if (again) {
-            if (rare_count && (int)(threadIdx.x & 63) == __ffsll((long long)again) - 1)
-                atomicAdd(rare_count, (uint32_t)__popcll(again));
-            if (!done) step_particle<ORDER, MAXV, IS3D, T, ELEM, false>(m, f, fe, c, s, i, prev, ~0ull);
-        }
-    } else {
-        step_particle<ORDER, MAXV, IS3D, T, ELEM>(m, f, fe, c, s, i, prev, ~0ull);
-    }
+    step_particle<ORDER, MAXV, IS3D, T, ELEM>(m, f, fe, c, s, i, prev, ~0ull);
 }
 
 // ---------------------------------------------------------------------------
@@ -800,17 +760,18 @@ __global__ void anchor_probe_kernel(DMesh m, int64_t n, const double* __restrict
 
 template <int MAXV, bool IS3D, typename T, bool ELEM>
 void launch_step_v(hipStream_t stream, int order, const DMesh& m, const Snap<T, IS3D>& f,
-                   const Snap<T, false>& fe, const StepCfg& cfg, const PState& ps, uint32_t* rare_count,
-                   uint32_t* rare_clear)
+                   const Snap<T, false>& fe, const StepCfg& cfg, const PState& ps)
 {
+    // m.tile_on: the kernel stages its workgroup's grid-hash tile in
+    // dynamic LDS
     const unsigned g = (unsigned)((ps.n + 255) / 256);
-    const size_t lds = (!ELEM || m.tile_on) ? sizeof(GhTile) : 0;
+    const size_t lds = m.tile_on ? sizeof(GhTile) : 0;
     if (order == 1)
-        step_kernel<1, MAXV, IS3D, T, ELEM><<<g, 256, lds, stream>>>(m, f, fe, cfg, ps, rare_count, rare_clear);
+        step_kernel<1, MAXV, IS3D, T, ELEM><<<g, 256, lds, stream>>>(m, f, fe, cfg, ps);
     else if (order == 2)
-        step_kernel<2, MAXV, IS3D, T, ELEM><<<g, 256, lds, stream>>>(m, f, fe, cfg, ps, rare_count, rare_clear);
+        step_kernel<2, MAXV, IS3D, T, ELEM><<<g, 256, lds, stream>>>(m, f, fe, cfg, ps);
     else
-        step_kernel<4, MAXV, IS3D, T, ELEM><<<g, 256, lds, stream>>>(m, f, fe, cfg, ps, rare_count, rare_clear);
+        step_kernel<4, MAXV, IS3D, T, ELEM><<<g, 256, lds, stream>>>(m, f, fe, cfg, ps);
 }
 
 }  // namespace kelp

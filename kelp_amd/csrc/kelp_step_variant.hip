@@ -24,11 +24,11 @@ using KvT = double;
 template void launch_step_v<3, (bool)KV_IS3D, KvT, (bool)KV_ELEM>(hipStream_t, int, const DMesh&,
                                                                   const Snap<KvT, (bool)KV_IS3D>&,
                                                                   const Snap<KvT, false>&, const StepCfg&,
-                                                                  const PState&, uint32_t*, uint32_t*);
+                                                                  const PState&);
 template void launch_step_v<4, (bool)KV_IS3D, KvT, (bool)KV_ELEM>(hipStream_t, int, const DMesh&,
                                                                   const Snap<KvT, (bool)KV_IS3D>&,
                                                                   const Snap<KvT, false>&, const StepCfg&,
-                                                                  const PState&, uint32_t*, uint32_t*);
+                                                                  const PState&);
 
 #if !KV_IS3D && KV_ELEM && !(defined(KV_BF16) && KV_BF16) && !(defined(KV_F16S) && KV_F16S)
 // rk45 (2-D, element-centre u/v) lives with the 2-D element-layout steps

@@ -485,13 +485,6 @@ class DeviceContext:
         check(self.lib.kelp_activate(self.h, ctypes.byref(s), rs, int(pid_offset), nr, int(step), ps),
               "kelp_activate")
 
-    def step_deferred(self) -> int:
-        """Particles the last step's second pass stepped (its rare cases),
-        -1 when it ran in one pass (see kelp_step_deferred)."""
-        out = ctypes.c_int64(0)
-        check(self.lib.kelp_step_deferred(self.h, ctypes.byref(out)), "kelp_step_deferred")
-        return int(out.value)
-
     def sort(self, p: DeviceParticles) -> None:
         """Reorder the particle arrays by grid-hash cell (see kelp_sort_particles)."""
         s = p.as_struct()
