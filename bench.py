@@ -753,7 +753,8 @@ def main():
     # T/S sampling, re-sort every 8 steps, all snapshots resident)
     traffic, traffic_src = None, None
     profiled_cfg = (not (args.particles or args.nodes) and args.resort_every == 8
-                    and not args.no_sample and not args.ring and args.storage == "f32")
+                    and not args.no_sample and not args.ring and args.storage == "f32"
+                    and getattr(args, "cell_rule", "occupancy") == "occupancy")
     if getattr(wl, "profile", None) and profiled_cfg:
         traffic = profiled_traffic(wl.profile)
         if traffic is not None:
