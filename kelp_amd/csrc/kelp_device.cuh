@@ -547,7 +547,10 @@ __device__ inline int nearest_widened(const DMesh& m, double lon_p, double lat_p
 // core/mesh.py:1612 _grid_hash_nearest_exact: the 3x3 window, accepted when
 // the certificate proves it; else the rectangle grown by the certificate's
 // deficits.  The common case (3x3 proven) returns after one window scan.
-__device__ __noinline__ int exact_widen(const DMesh& m, double lon_p, double lat_p, int best,
+// The rare paths (this widening, the 2-ring pass, the edge walk) are out of
+// line and marked cold: the compiler then keeps the step's live values in
+// registers on the common path and saves them around those calls only.
+__device__ __noinline__ __attribute__((cold)) int exact_widen(const DMesh& m, double lon_p, double lat_p, int best,
                                         double best_d2, int code, int ci, int cj, const Query& q);
 
 __device__ inline int nearest_exact(const DMesh& m, double lon_p, double lat_p)
@@ -572,7 +575,7 @@ __device__ inline int nearest_exact(const DMesh& m, double lon_p, double lat_p)
     return exact_widen(m, lon_p, lat_p, best, best_d2, code, ci, cj, q);
 }
 
-__device__ __noinline__ int exact_widen(const DMesh& m, double lon_p, double lat_p, int best,
+__device__ __noinline__ __attribute__((cold)) int exact_widen(const DMesh& m, double lon_p, double lat_p, int best,
                                         double best_d2, int code, int ci, int cj, const Query& q)
 {
     const double cell = m.cell;
@@ -970,7 +973,7 @@ __device__ __forceinline__ bool elem_touches(const DMesh& m, int c, int nn)
 // The rare 2-ring pass of core/mesh.py:2440-2463, kept out of line so the
 // common 1-ring hit stays a short straight path.
 template <int MAXV>
-__device__ __noinline__ int containing_2ring(const DMesh& m, double lon_p, double lat_p, int nn)
+__device__ __noinline__ __attribute__((cold)) int containing_2ring(const DMesh& m, double lon_p, double lat_p, int nn)
 {
     int k0, k1;
     n2e_range(m, nn, k0, k1);

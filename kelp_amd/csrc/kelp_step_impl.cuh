@@ -99,7 +99,7 @@ __device__ inline bool eval_site(const DMesh& m, const Snap<T, IS3D>& f, const S
 // The deterministic step of one alive particle, in place.  Euler follows
 // The walk kept out of line: it runs only with elem_search = walk.
 template <int MAXV>
-__device__ __noinline__ int walk_at(const DMesh& m, double lon, double lat, int start)
+__device__ __noinline__ __attribute__((cold)) int walk_at(const DMesh& m, double lon, double lat, int start)
 {
     return walk_locate<MAXV>(m, lon, lat, start);
 }
